@@ -1,0 +1,1136 @@
+// adda_hip.hip -- MI355X (gfx950) backend of ADDA's FFT matrix-vector product (src/matvec.c:128-425) and of the
+// linalg vector kernels (src/linalg.c), exported through the C ABI of include/adda_hip.h.
+//
+// Matvec pipeline (5 HBM passes, half-planes that are identically zero are never stored or read):
+//   P1 k_scatter_fftx  : gather S.x from dipoles onto x-lines (y<boxY, z<boxZ), zero-pad to gridX, forward FFT-x
+//                        -> Xa[c][z<boxZ][y<boxY][x<gridX]
+//   P2 k_colfft<fwd>   : forward FFT-z of the boxZ nonzero rows -> Xb[c][z<gridZ][y<boxY][x]
+//   P3 k_ydmul         : forward FFT-y of the boxY nonzero rows, product with the 6-component symmetric D-matrix,
+//                        inverse FFT-y, keep y<boxY (in place in Xb)
+//   P4 k_colfft<inv>   : inverse FFT-z, keep z<boxZ -> Xa
+//   P5 k_fftx_gather   : inverse FFT-x, result_i = arg_i + S_i.X(pos_i), per-block partial ||result||^2
+// Strided axes (y, z) are processed as tiles of TX adjacent x-columns so every global access is a run of TX
+// contiguous complex numbers; the transposes of the reference (TransposeYZ, src/fft.c:227-312) disappear.
+#include "fft_device.h"
+#include "../../include/adda_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace adda;
+
+// ==================================================================================================================
+// error handling
+static thread_local char g_err[512] = "";
+
+static int set_err(const char *file, int line, const char *fmt, ...)
+{
+	char msg[400];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(msg, sizeof msg, fmt, ap);
+	va_end(ap);
+	snprintf(g_err, sizeof g_err, "%s:%d: %s", file, line, msg);
+	return -1;
+}
+#define ERR(...) set_err(__FILE__, __LINE__, __VA_ARGS__)
+#define HIPCHK(call)                                                                                             \
+	do {                                                                                                         \
+		hipError_t e_ = (call);                                                                                  \
+		if (e_ != hipSuccess) return set_err(__FILE__, __LINE__, "%s: %s", #call, hipGetErrorString(e_));         \
+	} while (0)
+
+extern "C" const char *adda_hip_last_error(void) { return g_err; }
+
+// ==================================================================================================================
+// plans
+
+static constexpr int kMaxIt = 4;     // butterflies per thread per stage (stockham_stage MAXIT)
+static constexpr int kRedBlocks = 2048;
+static constexpr int kMaxDir = 2048;     // directions per adda_hip_field_sums launch
+static constexpr int kDirBlocks = 32;    // dipole chunks per direction
+
+static bool make_plan(int n, FftPlan &p)
+{
+	// radices 5 and 3 first, then the power of two as radix-8 stages with at most two radix-4 stages and no lone
+	// radix-2 stage unless the power is 2 itself (so a stage never has fewer than 4 elements per butterfly where
+	// avoidable)
+	p.n = n;
+	p.nst = 0;
+	int m = n, k2 = 0;
+	while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; if (p.nst >= kMaxStages) return false; }
+	while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; if (p.nst >= kMaxStages) return false; }
+	while (m % 2 == 0) { k2++; m /= 2; }
+	if (m != 1) return false;
+	int n8 = k2 / 3, n4 = 0, n2 = 0;
+	if (k2 % 3 == 2) n4 = 1;
+	else if (k2 % 3 == 1) {
+		if (n8 > 0) { n8--; n4 = 2; }
+		else n2 = 1;
+	}
+	if (p.nst + n8 + n4 + n2 > kMaxStages) return false;
+	for (int i = 0; i < n8; i++) p.rad[p.nst++] = 8;
+	for (int i = 0; i < n4; i++) p.rad[p.nst++] = 4;
+	for (int i = 0; i < n2; i++) p.rad[p.nst++] = 2;
+	return true;
+}
+
+static int min_radix(const FftPlan &p)
+{
+	int r = 8;
+	for (int s = 0; s < p.nst; s++) r = p.rad[s] < r ? p.rad[s] : r;
+	return r;
+}
+
+// butterflies of one stage must fit MAXIT per thread
+static bool plan_fits(const FftPlan &p, int nlines, int nthreads)
+{
+	return (long)nlines * (p.n / min_radix(p)) <= (long)kMaxIt * nthreads;
+}
+
+// twiddle table tw[e] = exp(-2 pi i e/n), evaluated in long double and rounded once
+static void twiddles(int n, std::vector<double> &out)
+{
+	out.resize(2 * (size_t)n);
+	for (int e = 0; e < n; e++) {
+		long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+		out[2 * e] = (double)cosl(a);
+		out[2 * e + 1] = (double)sinl(a);
+	}
+}
+
+// ==================================================================================================================
+// operator state
+
+struct TwPair {
+	cx<double> *d = nullptr;
+	cx<float> *f = nullptr;
+};
+
+struct adda_hip_op {
+	int dev = 0;
+	hipStream_t stream = nullptr;
+	int bX, bY, bZ, gX, gY, gZ;
+	size_t nd;
+	int nmat;
+	int reduced;
+	int DsY, DsZ;
+	double wavenum, gridspace;
+	int flags;
+	// geometry
+	int *cellmap = nullptr;  // [bZ][bY][bX] -> dipole index or -1
+	int *mat = nullptr;      // [nd]
+	unsigned short *pos = nullptr;  // [3*nd]
+	double *wbuf = nullptr;         // [nmat*3] weights / direction scratch
+	double *dirbuf = nullptr;       // [3*kMaxDir] directions
+	// coupling
+	cx<double> *cc64 = nullptr;  // [nmat][3]
+	cx<float> *cc32 = nullptr;
+	// D-matrix, layout [zt<DsZ][yt<DsY][x<gX][6]
+	cx<double> *D64 = nullptr;
+	cx<float> *D32 = nullptr;
+	// work buffers (FP64-sized; FP32 uses the first half)
+	void *Xa = nullptr;  // [3][bZ][bY][gX]
+	void *Xb = nullptr;  // [3][gZ][bY][gX]
+	FftPlan px, py, pz;
+	TwPair twx, twy, twz;
+	// reductions
+	double *partials = nullptr;  // max(4*kRedBlocks, gather blocks)
+	size_t npartials = 0;
+	double *red = nullptr;       // [4] device result
+	double *red_host = nullptr;  // [4] pinned
+	// launch shapes
+	int LL;       // x-lines per block in P1/P5
+	int TXz;      // columns per block in P2/P4
+	int TXy;      // columns per block in P3
+	int NTy;      // threads per block in P3
+};
+
+static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
+
+// ==================================================================================================================
+// kernels: matvec pipeline
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scatter_fftx(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
+	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan plan,
+	const cx<T> *__restrict__ tw, int bX, int nlines, int LL, int her)
+{
+	extern __shared__ char smem_raw[];
+	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);  // line (c*LL+l) at offset (c*LL+l)*N
+	const int N = plan.n;
+	const long line0 = (long)blockIdx.x * LL;
+	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
+		const int l = idx / N, x = idx % N;
+		const long line = line0 + l;
+		cx<T> v0 = mk<T>(0, 0), v1 = v0, v2 = v0;
+		if (line < nlines && x < bX) {
+			const int d = cellmap[line * bX + x];
+			if (d >= 0) {
+				const cx<T> *s = ccs + 3 * mat[d];
+				cx<T> a0 = arg[3 * (size_t)d], a1 = arg[3 * (size_t)d + 1], a2 = arg[3 * (size_t)d + 2];
+				if (her) { a0 = conjc(a0); a1 = conjc(a1); a2 = conjc(a2); }
+				v0 = mul(s[0], a0);
+				v1 = mul(s[1], a1);
+				v2 = mul(s[2], a2);
+			}
+		}
+		buf[(0 * LL + l) * N + x] = v0;
+		buf[(1 * LL + l) * N + x] = v1;
+		buf[(2 * LL + l) * N + x] = v2;
+	}
+	__syncthreads();
+	lds_fft<T, false, false, kMaxIt>(buf, plan, 3 * LL, 1, N, tw);
+	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
+		const int l = idx / N, x = idx % N;
+		const long line = line0 + l;
+		if (line < nlines) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) Xa[((size_t)c * nlines + line) * N + x] = buf[(c * LL + l) * N + x];
+		}
+	}
+}
+
+template <typename T>
+__device__ __forceinline__ double block_sum(double s, double *red_lds)
+{
+	for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	__syncthreads();
+	if (lane == 0) red_lds[wid] = s;
+	__syncthreads();
+	double t = 0;
+	if (threadIdx.x == 0) {
+		for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red_lds[w];
+	}
+	return t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fftx_gather(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
+	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
+	const cx<T> *__restrict__ ccs, FftPlan plan, const cx<T> *__restrict__ tw, int bX, int nlines, int LL, int her,
+	double *__restrict__ partials)
+{
+	extern __shared__ char smem_raw[];
+	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+	__shared__ double red_lds[8];
+	const int N = plan.n;
+	const long line0 = (long)blockIdx.x * LL;
+	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
+		const int l = idx / N, x = idx % N;
+		const long line = line0 + l;
+#pragma unroll
+		for (int c = 0; c < 3; c++)
+			buf[(c * LL + l) * N + x] = line < nlines ? Xa[((size_t)c * nlines + line) * N + x] : mk<T>(0, 0);
+	}
+	__syncthreads();
+	lds_fft<T, true, false, kMaxIt>(buf, plan, 3 * LL, 1, N, tw);
+	double s = 0;
+	for (int idx = threadIdx.x; idx < LL * bX; idx += blockDim.x) {
+		const int l = idx / bX, x = idx % bX;
+		const long line = line0 + l;
+		if (line >= nlines) continue;
+		const int d = cellmap[line * bX + x];
+		if (d < 0) continue;
+		const cx<T> *sc = ccs + 3 * mat[d];
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			cx<T> a = arg[3 * (size_t)d + c];
+			if (her) a = conjc(a);
+			cx<T> r = add(a, mul(sc[c], buf[(c * LL + l) * N + x]));
+			if (her) r = conjc(r);
+			res[3 * (size_t)d + c] = r;
+			s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+		}
+	}
+	if (partials) {
+		double t = block_sum<T>(s, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
+	}
+}
+
+// Batched FFT along a strided axis. Block (blockIdx.x, blockIdx.y) transforms columns x0..x0+TX-1 of batch b:
+// element k of column x is in[off_in + k*rs_in + x] (k < n_in, zero above), out[off_out + k*rs_out + x] for k < n_out.
+// off = (b / B1)*S1 + (b % B1)*S0.
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) k_colfft(const cx<T> *in, cx<T> *out, FftPlan plan,
+	const cx<T> *__restrict__ tw, int ncols, int TX, int n_in, int n_out, long rs_in, long rs_out, int B1,
+	long S1_in, long S0_in, long S1_out, long S0_out)
+{
+	extern __shared__ char smem_raw[];
+	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);  // (k, col) at k*TX + col
+	const int N = plan.n;
+	const int x0 = blockIdx.x * TX;
+	const int b = blockIdx.y;
+	const long off_in = (long)(b / B1) * S1_in + (long)(b % B1) * S0_in;
+	const long off_out = (long)(b / B1) * S1_out + (long)(b % B1) * S0_out;
+	for (int idx = threadIdx.x; idx < N * TX; idx += blockDim.x) {
+		const int k = idx / TX, col = idx % TX, x = x0 + col;
+		buf[idx] = (k < n_in && x < ncols) ? in[off_in + k * rs_in + x] : mk<T>(0, 0);
+	}
+	__syncthreads();
+	lds_fft<T, INV, true, kMaxIt>(buf, plan, TX, TX, 1, tw);
+	for (int idx = threadIdx.x; idx < n_out * TX; idx += blockDim.x) {
+		const int k = idx / TX, col = idx % TX, x = x0 + col;
+		if (x < ncols) out[off_out + k * rs_out + x] = buf[idx];
+	}
+}
+
+// Batched full-line FFT along the unit-stride axis (used to build the D-matrix).
+template <typename T, bool INV>
+__global__ void __launch_bounds__(256) k_rowfft(cx<T> *data, FftPlan plan, const cx<T> *__restrict__ tw, long nlines,
+	int LL)
+{
+	extern __shared__ char smem_raw[];
+	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+	const int N = plan.n;
+	const long line0 = (long)blockIdx.x * LL;
+	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
+		const long line = line0 + idx / N;
+		buf[idx] = line < nlines ? data[line * N + idx % N] : mk<T>(0, 0);
+	}
+	__syncthreads();
+	lds_fft<T, INV, false, kMaxIt>(buf, plan, LL, 1, N, tw);
+	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
+		const long line = line0 + idx / N;
+		if (line < nlines) data[line * N + idx % N] = buf[idx];
+	}
+}
+
+// P3: forward FFT-y of the 3 components of TX columns at one z, D-matrix product, inverse FFT-y.
+// LDS element (y, c, col) at (y*3 + c)*TX + col. The D-matrix product restates src/matvec.c:270-305 with the
+// reduced-FFT mirror indices and sign flips of IndexDmatrix_mv (87-103) and cSymMatrVec (src/cmplx.h:369-376).
+template <typename T>
+__global__ void __launch_bounds__(512) k_ydmul(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan plan,
+	const cx<T> *__restrict__ tw, int gX, int bY, int gZ, int DsY, int DsZ, int TX, int reduced, int transposed)
+{
+	extern __shared__ char smem_raw[];
+	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+	const int N = plan.n;  // gridY
+	const int x0 = blockIdx.x * TX;
+	const int z = blockIdx.y;
+	const size_t cstride = (size_t)gZ * bY * gX;
+	const size_t zoff = (size_t)z * bY * gX;
+	for (int idx = threadIdx.x; idx < N * 3 * TX; idx += blockDim.x) {
+		const int y = idx / (3 * TX), r = idx % (3 * TX), c = r / TX, col = r % TX, x = x0 + col;
+		buf[idx] = (y < bY && x < gX) ? Xb[c * cstride + zoff + (size_t)y * gX + x] : mk<T>(0, 0);
+	}
+	__syncthreads();
+	lds_fft<T, false, true, kMaxIt>(buf, plan, 3 * TX, 3 * TX, 1, tw);
+	for (int idx = threadIdx.x; idx < N * TX; idx += blockDim.x) {
+		const int y = idx / TX, col = idx % TX, x = x0 + col;
+		if (x >= gX) continue;
+		int xi = x, yi = y, zi = z;
+		if (transposed) {
+			xi = xi > 0 ? gX - xi : 0;
+			yi = yi > 0 ? N - yi : 0;
+			zi = zi > 0 ? gZ - zi : 0;
+		}
+		else {
+			if (yi >= DsY) yi = N - yi;
+			if (zi >= DsZ) zi = gZ - zi;
+		}
+		const cx<T> *dm = D + (((size_t)zi * DsY + yi) * gX + xi) * 6;
+		cx<T> f0 = dm[0], f1 = dm[1], f2 = dm[2], f3 = dm[3], f4 = dm[4], f5 = dm[5];
+		if (reduced) {
+			if (y >= DsY) {
+				f1 = scl(f1, (T)-1);
+				if (z >= DsZ) f2 = scl(f2, (T)-1);
+				else f4 = scl(f4, (T)-1);
+			}
+			else if (z >= DsZ) {
+				f2 = scl(f2, (T)-1);
+				f4 = scl(f4, (T)-1);
+			}
+		}
+		cx<T> *p = buf + (size_t)y * 3 * TX + col;
+		const cx<T> a0 = p[0], a1 = p[TX], a2 = p[2 * TX];
+		p[0] = add(add(mul(f0, a0), mul(f1, a1)), mul(f2, a2));
+		p[TX] = add(add(mul(f1, a0), mul(f3, a1)), mul(f4, a2));
+		p[2 * TX] = add(add(mul(f2, a0), mul(f4, a1)), mul(f5, a2));
+	}
+	__syncthreads();
+	lds_fft<T, true, true, kMaxIt>(buf, plan, 3 * TX, 3 * TX, 1, tw);
+	for (int idx = threadIdx.x; idx < bY * 3 * TX; idx += blockDim.x) {
+		const int y = idx / (3 * TX), r = idx % (3 * TX), c = r / TX, col = r % TX, x = x0 + col;
+		if (x < gX) Xb[c * cstride + zoff + (size_t)y * gX + x] = buf[idx];
+	}
+}
+
+// ==================================================================================================================
+// kernels: D-matrix construction (FP64 throughout, src/fft.c:918-1416)
+
+// Point-dipole Green's tensor at lattice offset (i,j,k): InterTerm_poi (src/interaction.c:159-180,239-266,290-300).
+__device__ void green_poi(int i, int j, int k, double wavenum, double gridspace, cx<double> r[6])
+{
+	double qx = i, qy = j, qz = k;
+	const double rn = sqrt(qx * qx + qy * qy + qz * qz);
+	const double invrn = 1.0 / rn;
+	qx *= invrn;
+	qy *= invrn;
+	qz *= invrn;
+	const double rr = rn * gridspace;
+	const double invr3 = 1 / (rr * rr * rr);
+	const double kr = wavenum * rr, kr2 = kr * kr;
+	const double t1 = 3 - kr2, t2 = -3 * kr, t3 = kr2 - 1;
+	double sn, cs;
+	sincos(kr, &sn, &cs);
+	const cx<double> ev = mk<double>(invr3 * cs, invr3 * sn);
+	const double q[6] = {qx * qx, qx * qy, qx * qz, qy * qy, qy * qz, qz * qz};
+#pragma unroll
+	for (int c = 0; c < 6; c++) {
+		cx<double> t;
+		if (c == 0 || c == 3 || c == 5) t = mk<double>(t1 * q[c] + t3, kr + t2 * q[c]);
+		else t = mk<double>(t1 * q[c], t2 * q[c]);
+		r[c] = mul(t, ev);
+	}
+}
+
+// G on the full periodic grid, 6 component planes: offsets |i|<boxX etc., zero offset and the gap set to 0
+// (src/fft.c:1226-1238; mirrored halves of 1275-1290 are G's exact parity under reflection).
+__global__ void k_green(cx<double> *G, int bX, int bY, int bZ, int gX, int gY, int gZ, double wavenum,
+	double gridspace)
+{
+	const size_t n = (size_t)gX * gY * gZ;
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+		const int x = (int)(idx % gX), y = (int)((idx / gX) % gY), z = (int)(idx / ((size_t)gX * gY));
+		const int i = x < bX ? x : (x > gX - bX ? x - gX : INT32_MAX);
+		const int j = y < bY ? y : (y > gY - bY ? y - gY : INT32_MAX);
+		const int k = z < bZ ? z : (z > gZ - bZ ? z - gZ : INT32_MAX);
+		cx<double> r[6];
+		if (i == INT32_MAX || j == INT32_MAX || k == INT32_MAX || (i == 0 && j == 0 && k == 0)) {
+			for (int c = 0; c < 6; c++) r[c] = mk<double>(0, 0);
+		}
+		else green_poi(i, j, k, wavenum, gridspace, r);
+		for (int c = 0; c < 6; c++) G[c * n + idx] = r[c];
+	}
+}
+
+// D[zt][yt][x][c] = -G~[c][zt][yt][x]/Ngrid for zt<DsZ, yt<DsY (src/fft.c:1310-1314)
+__global__ void k_extract_d(const cx<double> *G, cx<double> *D, int gX, int gY, int gZ, int DsY, int DsZ)
+{
+	const size_t n = (size_t)gX * gY * gZ, m = (size_t)DsZ * DsY * gX;
+	const double invN = 1.0 / (double)n;
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m; idx += (size_t)gridDim.x * blockDim.x) {
+		const int x = (int)(idx % gX), y = (int)((idx / gX) % DsY), z = (int)(idx / ((size_t)gX * DsY));
+		const size_t src = ((size_t)z * gY + y) * gX + x;
+		for (int c = 0; c < 6; c++) {
+			cx<double> v = G[c * n + src];
+			D[idx * 6 + c] = mk<double>(-invN * v.x, -invN * v.y);
+		}
+	}
+}
+
+// ==================================================================================================================
+// kernels: vectors
+
+template <typename TD, typename TS>
+__global__ void k_convert(cx<TD> *dst, const cx<TS> *src, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = mk<TD>((TD)src[i].x, (TD)src[i].y);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lincomb(cx<T> *out, const cx<T> *a, cx<double> ca, const cx<T> *b,
+	cx<double> cb, const cx<T> *c, cx<double> cc, int flags, size_t n, double *partials)
+{
+	__shared__ double red_lds[8];
+	double s = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double rx = 0, ry = 0;
+		if (a) {
+			cx<T> v = a[i];
+			double vx = v.x, vy = (flags & ADDA_CONJ_A) ? -(double)v.y : (double)v.y;
+			rx += ca.x * vx - ca.y * vy;
+			ry += ca.x * vy + ca.y * vx;
+		}
+		if (b) {
+			cx<T> v = b[i];
+			double vx = v.x, vy = (flags & ADDA_CONJ_B) ? -(double)v.y : (double)v.y;
+			rx += cb.x * vx - cb.y * vy;
+			ry += cb.x * vy + cb.y * vx;
+		}
+		if (c) {
+			cx<T> v = c[i];
+			double vx = v.x, vy = v.y;
+			rx += cc.x * vx - cc.y * vy;
+			ry += cc.x * vy + cc.y * vx;
+		}
+		cx<T> r = mk<T>((T)rx, (T)ry);
+		out[i] = r;
+		s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+	}
+	if (partials) {
+		double t = block_sum<T>(s, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
+	}
+}
+
+// partials layout: [4][nblocks]
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int mode, size_t n, double *partials,
+	int nblk)
+{
+	__shared__ double red_lds[8];
+	double s0 = 0, s1 = 0, s2 = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const cx<T> u = a[i];
+		const double ux = u.x, uy = u.y;
+		if (mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) {
+			const cx<T> v = b[i];
+			const double vx = v.x, vy = mode == ADDA_DOT_CONJ ? -(double)v.y : (double)v.y;
+			s0 += ux * vx - uy * vy;
+			s1 += ux * vy + uy * vx;
+		}
+		else {  // self: re^2, im^2, re*im (nDotProdSelf_conj_Norm2, src/linalg.c:127-144)
+			s0 += ux * ux;
+			s1 += uy * uy;
+			s2 += ux * uy;
+		}
+	}
+	double t0 = block_sum<T>(s0, red_lds);
+	double t1 = block_sum<T>(s1, red_lds);
+	double t2 = block_sum<T>(s2, red_lds);
+	if (threadIdx.x == 0) {
+		partials[blockIdx.x] = t0;
+		partials[nblk + blockIdx.x] = t1;
+		partials[2 * nblk + blockIdx.x] = t2;
+	}
+}
+
+// Deterministic final reduction of nk rows of nblk partials: fixed strided order per thread, then a fixed tree.
+__global__ void __launch_bounds__(256) k_reduce_final(const double *partials, int nblk, int nk, double *out)
+{
+	__shared__ double red_lds[8];
+	for (int k = 0; k < nk; k++) {
+		double s = 0;
+		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[k * nblk + i];
+		double t = block_sum<double>(s, red_lds);
+		if (threadIdx.x == 0) out[k] = t;
+		__syncthreads();
+	}
+}
+
+template <typename T>
+__global__ void k_mult_mat(cx<T> *out, const cx<T> *in, const int *mat, const cx<T> *ccs, size_t nd)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
+		const size_t d = i / 3;
+		const int c = (int)(i % 3);
+		out[i] = mul(ccs[3 * mat[d] + c], in[i]);
+	}
+}
+
+// Plane wave (GenerateB, src/GenerateB.c:324-330)
+__global__ void k_plane_wave(cx<double> *e, const unsigned short *pos, size_t nd, double cx_, double cy_, double cz_,
+	double d, double k, double px, double py, double pz, double ex, double ey, double ez)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += (size_t)gridDim.x * blockDim.x) {
+		const double rx = (pos[3 * i] - cx_) * d, ry = (pos[3 * i + 1] - cy_) * d, rz = (pos[3 * i + 2] - cz_) * d;
+		double sn, cs;
+		sincos(k * (rx * px + ry * py + rz * pz), &sn, &cs);
+		e[3 * i] = mk<double>(cs * ex, sn * ex);
+		e[3 * i + 1] = mk<double>(cs * ey, sn * ey);
+		e[3 * i + 2] = mk<double>(cs * ez, sn * ez);
+	}
+}
+
+// Field sums: block (chunk, dir); partials[((dir*6 + q)*kDirBlocks) + chunk], q = re/im of 3 components
+template <typename T>
+__global__ void __launch_bounds__(256) k_field_sums(const cx<T> *P, const unsigned short *pos, size_t nd, double kd,
+	const double *dirs, double *partials)
+{
+	__shared__ double red_lds[8];
+	const int dir = blockIdx.y, chunk = blockIdx.x;
+	const double nx = dirs[3 * dir], ny = dirs[3 * dir + 1], nz = dirs[3 * dir + 2];
+	double s[6] = {0, 0, 0, 0, 0, 0};
+	for (size_t j = (size_t)chunk * blockDim.x + threadIdx.x; j < nd; j += (size_t)gridDim.x * blockDim.x) {
+		double sn, cs;
+		sincos(-kd * (nx * pos[3 * j] + ny * pos[3 * j + 1] + nz * pos[3 * j + 2]), &sn, &cs);
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			const cx<T> p = P[3 * j + c];
+			s[2 * c] += (double)p.x * cs - (double)p.y * sn;
+			s[2 * c + 1] += (double)p.x * sn + (double)p.y * cs;
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 6; q++) {
+		double t = block_sum<T>(s[q], red_lds);
+		if (threadIdx.x == 0) partials[((size_t)dir * 6 + q) * gridDim.x + chunk] = t;
+	}
+}
+
+__global__ void k_field_final(const double *partials, int nchunk, int ndir, double *out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ndir * 6) return;
+	double s = 0;
+	for (int c = 0; c < nchunk; c++) s += partials[(size_t)i * nchunk + c];
+	out[i] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_wnorm(const cx<T> *v, const int *mat, const double *w, size_t nd,
+	double *partials)
+{
+	__shared__ double red_lds[8];
+	double s = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
+		const cx<T> a = v[i];
+		s += w[3 * mat[i / 3] + i % 3] * ((double)a.x * a.x + (double)a.y * a.y);
+	}
+	double t = block_sum<T>(s, red_lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// ==================================================================================================================
+// host helpers
+
+static size_t elsize(int prec) { return prec == ADDA_F32 ? 8 : 16; }
+
+template <typename T> static const cx<T> *tw_of(const TwPair &t);
+template <> const cx<double> *tw_of<double>(const TwPair &t) { return t.d; }
+template <> const cx<float> *tw_of<float>(const TwPair &t) { return t.f; }
+
+static int upload_twiddles(int n, TwPair &t)
+{
+	std::vector<double> w;
+	twiddles(n, w);
+	std::vector<float> wf(w.size());
+	for (size_t i = 0; i < w.size(); i++) wf[i] = (float)w[i];
+	HIPCHK(hipMalloc(&t.d, w.size() * sizeof(double)));
+	HIPCHK(hipMalloc(&t.f, wf.size() * sizeof(float)));
+	HIPCHK(hipMemcpy(t.d, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(t.f, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice));
+	return 0;
+}
+
+static int reduce_finish(adda_hip_op *op, int nblk, int nk, double *out)
+{
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, op->stream, op->partials, nblk, nk, op->red);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipMemcpyAsync(op->red_host, op->red, nk * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	for (int k = 0; k < nk; k++) out[k] = op->red_host[k];
+	return 0;
+}
+
+template <typename T>
+static int set_lds_limits()
+{
+	const int big = 96 * 1024;  // largest dynamic LDS tile used (k_ydmul)
+	HIPCHK(hipFuncSetAttribute((const void *)k_ydmul<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_scatter_fftx<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_fftx_gather<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	return 0;
+}
+
+// ==================================================================================================================
+// matvec driver
+
+template <typename T>
+static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her)
+{
+	const cx<T> *ccs;
+	const cx<T> *D;
+	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
+	else { ccs = op->cc32; D = op->D32; }
+	if (!D) return ERR("FP32 matvec requested but the operator was created without the FP32 flag");
+	cx<T> *Xa = (cx<T> *)op->Xa, *Xb = (cx<T> *)op->Xb;
+	const int gX = op->gX, bX = op->bX, bY = op->bY, bZ = op->bZ, gY = op->gY, gZ = op->gZ;
+	const int nlx = bY * bZ;
+	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
+	// P1
+	{
+		const int nblk = (nlx + op->LL - 1) / op->LL;
+		const size_t lds = (size_t)3 * op->LL * gX * sizeof(cx<T>);
+		hipLaunchKernelGGL(k_scatter_fftx<T>, dim3(nblk), dim3(256), lds, op->stream, arg, Xa, op->cellmap, op->mat,
+			ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her);
+	}
+	// P2: forward FFT-z over bZ nonzero rows; batch b = c*bY + y
+	{
+		const int TX = op->TXz;
+		dim3 grid((gX + TX - 1) / TX, 3 * bY);
+		const size_t lds = (size_t)gZ * TX * sizeof(cx<T>);
+		hipLaunchKernelGGL((k_colfft<T, false>), grid, dim3(256), lds, op->stream, Xa, Xb, op->pz, tw_of<T>(op->twz),
+			gX, TX, bZ, gZ, (long)bY * gX, (long)bY * gX, bY, (long)bZ * bY * gX, (long)gX, (long)gZ * bY * gX,
+			(long)gX);
+	}
+	// P3
+	{
+		const int TX = op->TXy;
+		dim3 grid((gX + TX - 1) / TX, gZ);
+		const size_t lds = (size_t)3 * gY * TX * sizeof(cx<T>);
+		hipLaunchKernelGGL(k_ydmul<T>, grid, dim3(op->NTy), lds, op->stream, Xb, D, op->py, tw_of<T>(op->twy), gX, bY,
+			gZ, op->DsY, op->DsZ, TX, op->reduced, transposed);
+	}
+	// P4: inverse FFT-z, keep z<bZ
+	{
+		const int TX = op->TXz;
+		dim3 grid((gX + TX - 1) / TX, 3 * bY);
+		const size_t lds = (size_t)gZ * TX * sizeof(cx<T>);
+		hipLaunchKernelGGL((k_colfft<T, true>), grid, dim3(256), lds, op->stream, Xb, Xa, op->pz, tw_of<T>(op->twz),
+			gX, TX, gZ, bZ, (long)bY * gX, (long)bY * gX, bY, (long)gZ * bY * gX, (long)gX, (long)bZ * bY * gX,
+			(long)gX);
+	}
+	// P5
+	const int nblk5 = (nlx + op->LL - 1) / op->LL;
+	{
+		const size_t lds = (size_t)3 * op->LL * gX * sizeof(cx<T>);
+		hipLaunchKernelGGL(k_fftx_gather<T>, dim3(nblk5), dim3(256), lds, op->stream, (const cx<T> *)Xa, arg, res,
+			op->cellmap, op->mat, ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her,
+			inprod ? op->partials : (double *)nullptr);
+	}
+	HIPCHK(hipGetLastError());
+	if (inprod) {
+		double v;
+		if (reduce_finish(op, nblk5, 1, &v)) return -1;
+		*inprod = v;
+	}
+	return 0;
+}
+
+// ==================================================================================================================
+// C ABI
+
+extern "C" int adda_hip_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+extern "C" int adda_hip_fftfit(int x, int divis)
+{
+	for (;; x++) {
+		if (x % 2 != 0 || x % divis != 0) continue;
+		int y = x;
+		while (y % 2 == 0) y /= 2;
+		while (y % 3 == 0) y /= 3;
+		while (y % 5 == 0) y /= 5;
+		if (y == 1) return x;
+	}
+}
+
+static int op_build(adda_hip_op *op, const unsigned short *position, const int *material)
+{
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+	if (set_lds_limits<double>() || set_lds_limits<float>()) return -1;
+	const int bX = op->bX, bY = op->bY, bZ = op->bZ, gX = op->gX, gY = op->gY, gZ = op->gZ;
+	if (!make_plan(gX, op->px) || !make_plan(gY, op->py) || !make_plan(gZ, op->pz))
+		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
+	if (gX < 2 * bX || gY < 2 * bY || gZ < 2 * bZ || gY % 2 || gZ % 2)
+		return ERR("FFT grid %dx%dx%d does not embed box %dx%dx%d (need even grid >= 2*box)", gX, gY, gZ, bX, bY, bZ);
+	// launch shapes: all LDS tiles <= 96 KB, butterflies per thread <= kMaxIt
+	op->LL = 4;
+	while (op->LL > 1 && ((size_t)3 * op->LL * gX * 16 > 64 * 1024 || !plan_fits(op->px, 3 * op->LL, 256))) op->LL /= 2;
+	op->TXz = 16;
+	while (op->TXz > 1 && ((size_t)gZ * op->TXz * 16 > 64 * 1024 || !plan_fits(op->pz, op->TXz, 256))) op->TXz /= 2;
+	op->NTy = 512;
+	op->TXy = 8;
+	while (op->TXy > 1 && ((size_t)3 * gY * op->TXy * 16 > 96 * 1024 || !plan_fits(op->py, 3 * op->TXy, op->NTy)))
+		op->TXy /= 2;
+	if (!plan_fits(op->px, 3 * op->LL, 256) || !plan_fits(op->pz, op->TXz, 256) ||
+		!plan_fits(op->py, 3 * op->TXy, op->NTy) || (size_t)3 * op->LL * gX * 16 > 96 * 1024 ||
+		(size_t)gZ * op->TXz * 16 > 96 * 1024 || (size_t)3 * gY * op->TXy * 16 > 96 * 1024)
+		return ERR("grid %dx%dx%d too large for the LDS tiles of this build", gX, gY, gZ);
+	if (upload_twiddles(gX, op->twx) || upload_twiddles(gY, op->twy) || upload_twiddles(gZ, op->twz)) return -1;
+	// geometry: cell map and materials
+	const size_t nbox = (size_t)bX * bY * bZ;
+	std::vector<int> cm(nbox, -1);
+	for (size_t i = 0; i < op->nd; i++) {
+		const unsigned short *p = position + 3 * i;
+		if (p[0] >= bX || p[1] >= bY || p[2] >= bZ) return ERR("dipole %zu at (%u,%u,%u) outside the box", i, p[0], p[1], p[2]);
+		const size_t c = ((size_t)p[2] * bY + p[1]) * bX + p[0];
+		if (cm[c] >= 0) return ERR("two dipoles at the same grid cell (%u,%u,%u)", p[0], p[1], p[2]);
+		cm[c] = (int)i;
+		if (material[i] < 0 || material[i] >= op->nmat) return ERR("dipole %zu has material %d outside [0,%d)", i, material[i], op->nmat);
+	}
+	HIPCHK(hipMalloc(&op->cellmap, nbox * sizeof(int)));
+	HIPCHK(hipMemcpy(op->cellmap, cm.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
+	HIPCHK(hipMalloc(&op->mat, (op->nd ? op->nd : 1) * sizeof(int)));
+	HIPCHK(hipMemcpy(op->mat, material, op->nd * sizeof(int), hipMemcpyHostToDevice));
+	HIPCHK(hipMalloc(&op->pos, 3 * op->nd * sizeof(unsigned short)));
+	HIPCHK(hipMemcpy(op->pos, position, 3 * op->nd * sizeof(unsigned short), hipMemcpyHostToDevice));
+	HIPCHK(hipMalloc(&op->wbuf, (size_t)op->nmat * 3 * sizeof(double)));
+	HIPCHK(hipMalloc(&op->dirbuf, (size_t)3 * kMaxDir * sizeof(double)));
+	HIPCHK(hipMalloc(&op->cc64, (size_t)op->nmat * 3 * sizeof(cx<double>)));
+	HIPCHK(hipMalloc(&op->cc32, (size_t)op->nmat * 3 * sizeof(cx<float>)));
+	HIPCHK(hipMemset(op->cc64, 0, (size_t)op->nmat * 3 * sizeof(cx<double>)));
+	HIPCHK(hipMemset(op->cc32, 0, (size_t)op->nmat * 3 * sizeof(cx<float>)));
+	// work buffers
+	HIPCHK(hipMalloc(&op->Xa, (size_t)3 * bZ * bY * gX * sizeof(cx<double>)));
+	HIPCHK(hipMalloc(&op->Xb, (size_t)3 * gZ * bY * gX * sizeof(cx<double>)));
+	{
+		const size_t nblk5 = ((size_t)bY * bZ + op->LL - 1) / op->LL;
+		op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);  // field sums need the most
+		if (nblk5 > op->npartials) op->npartials = nblk5;
+		HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
+	}
+	HIPCHK(hipMalloc(&op->red, 4 * sizeof(double)));
+	HIPCHK(hipHostMalloc((void **)&op->red_host, 4 * sizeof(double), hipHostMallocDefault));
+	// D-matrix: G on the full grid, 3D FFT (x rows, y and z columns), extraction of the stored octant
+	op->DsY = op->reduced ? gY / 2 + 1 : gY;
+	op->DsZ = op->reduced ? gZ / 2 + 1 : gZ;
+	const size_t ng = (size_t)gX * gY * gZ, nD = (size_t)op->DsZ * op->DsY * gX * 6;
+	cx<double> *G = nullptr;
+	HIPCHK(hipMalloc(&G, 6 * ng * sizeof(cx<double>)));
+	hipLaunchKernelGGL(k_green, dim3(4096), dim3(256), 0, op->stream, G, bX, bY, bZ, gX, gY, gZ, op->wavenum,
+		op->gridspace);
+	HIPCHK(hipGetLastError());
+	int LLr = 8;
+	while (LLr > 1 && ((size_t)LLr * gX * 16 > 64 * 1024 || !plan_fits(op->px, LLr, 256))) LLr /= 2;
+	int TXc = 16;
+	while (TXc > 1 && ((size_t)gY * TXc * 16 > 64 * 1024 || (size_t)gZ * TXc * 16 > 64 * 1024 ||
+		!plan_fits(op->py, TXc, 256) || !plan_fits(op->pz, TXc, 256))) TXc /= 2;
+	for (int c = 0; c < 6; c++) {
+		cx<double> *g = G + c * ng;
+		const long nl = (long)gY * gZ;
+		hipLaunchKernelGGL((k_rowfft<double, false>), dim3((unsigned)((nl + LLr - 1) / LLr)), dim3(256),
+			(size_t)LLr * gX * 16, op->stream, g, op->px, op->twx.d, nl, LLr);
+		// y columns: batch z (offset z*gX*gY)
+		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gZ), dim3(256),
+			(size_t)gY * TXc * 16, op->stream, g, g, op->py, op->twy.d, gX, TXc, gY, gY, (long)gX, (long)gX, 1,
+			(long)gX * gY, 0L, (long)gX * gY, 0L);
+		// z columns: batch y (offset y*gX)
+		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gY), dim3(256),
+			(size_t)gZ * TXc * 16, op->stream, g, g, op->pz, op->twz.d, gX, TXc, gZ, gZ, (long)gX * gY, (long)gX * gY,
+			1, (long)gX, 0L, (long)gX, 0L);
+		HIPCHK(hipGetLastError());
+	}
+	HIPCHK(hipMalloc(&op->D64, nD * sizeof(cx<double>)));
+	hipLaunchKernelGGL(k_extract_d, dim3(4096), dim3(256), 0, op->stream, G, op->D64, gX, gY, gZ, op->DsY, op->DsZ);
+	HIPCHK(hipGetLastError());
+	if (op->flags & 1) {
+		HIPCHK(hipMalloc(&op->D32, nD * sizeof(cx<float>)));
+		hipLaunchKernelGGL((k_convert<float, double>), dim3(4096), dim3(256), 0, op->stream, op->D32, op->D64, nD);
+		HIPCHK(hipGetLastError());
+	}
+	HIPCHK(hipStreamSynchronize(op->stream));
+	HIPCHK(hipFree(G));
+	return 0;
+}
+
+extern "C" adda_hip_op *adda_hip_op_create(int device, int boxX, int boxY, int boxZ, int gridX, int gridY, int gridZ,
+	size_t nd, const unsigned short *position, const int *material, int nmat, double wavenum, double gridspace,
+	int reduced_fft, int flags)
+{
+	int ndev = adda_hip_device_count();
+	if (device < 0 || device >= ndev) {
+		ERR("device %d not available (%d visible HIP devices)", device, ndev);
+		return nullptr;
+	}
+	if (nd == 0 || nmat <= 0 || boxX <= 0 || boxY <= 0 || boxZ <= 0) {
+		ERR("empty particle (nd=%zu, nmat=%d)", nd, nmat);
+		return nullptr;
+	}
+	adda_hip_op *op = new adda_hip_op();
+	op->dev = device;
+	op->bX = boxX; op->bY = boxY; op->bZ = boxZ;
+	op->gX = gridX; op->gY = gridY; op->gZ = gridZ;
+	op->nd = nd;
+	op->nmat = nmat;
+	op->reduced = reduced_fft ? 1 : 0;
+	op->wavenum = wavenum;
+	op->gridspace = gridspace;
+	op->flags = flags;
+	if (op_build(op, position, material)) {
+		char keep[512];
+		memcpy(keep, g_err, sizeof keep);
+		adda_hip_op_destroy(op);
+		memcpy(g_err, keep, sizeof keep);
+		return nullptr;
+	}
+	return op;
+}
+
+extern "C" void adda_hip_op_destroy(adda_hip_op *op)
+{
+	if (!op) return;
+	(void)hipSetDevice(op->dev);
+	if (op->stream) (void)hipStreamSynchronize(op->stream);
+	void *ptrs[] = {op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64, op->D32, op->Xa, op->Xb, op->partials, op->red,
+		op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d, op->twz.f};
+	for (void *p : ptrs)
+		if (p) (void)hipFree(p);
+	if (op->red_host) (void)hipHostFree(op->red_host);
+	if (op->stream) (void)hipStreamDestroy(op->stream);
+	delete op;
+}
+
+extern "C" int adda_hip_set_cc_sqrt(adda_hip_op *op, const adda_cplx *cc_sqrt)
+{
+	if (!op || !cc_sqrt) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = (size_t)op->nmat * 3;
+	std::vector<float> f(2 * n);
+	for (size_t i = 0; i < n; i++) { f[2 * i] = (float)cc_sqrt[i].re; f[2 * i + 1] = (float)cc_sqrt[i].im; }
+	HIPCHK(hipMemcpyAsync(op->cc64, cc_sqrt, n * sizeof(cx<double>), hipMemcpyHostToDevice, op->stream));
+	HIPCHK(hipMemcpyAsync(op->cc32, f.data(), n * sizeof(cx<float>), hipMemcpyHostToDevice, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	return 0;
+}
+
+extern "C" size_t adda_hip_dmatrix_size(const adda_hip_op *op) { return op ? (size_t)6 * op->gX * op->DsY * op->DsZ : 0; }
+
+extern "C" int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst)
+{
+	if (!op || !dst) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = adda_hip_dmatrix_size(op);
+	std::vector<adda_cplx> tmp(n);
+	HIPCHK(hipMemcpyAsync(tmp.data(), op->D64, n * sizeof(adda_cplx), hipMemcpyDeviceToHost, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	// device [zt][yt][x][6] -> ADDA [x][zt][yt][6]
+	for (int z = 0; z < op->DsZ; z++)
+		for (int y = 0; y < op->DsY; y++)
+			for (int x = 0; x < op->gX; x++)
+				for (int c = 0; c < 6; c++)
+					dst[6 * (((size_t)x * op->DsZ + z) * op->DsY + y) + c] =
+						tmp[(((size_t)z * op->DsY + y) * op->gX + x) * 6 + c];
+	return 0;
+}
+
+extern "C" size_t adda_hip_nrows(const adda_hip_op *op) { return op ? 3 * op->nd : 0; }
+
+extern "C" void *adda_hip_vec_alloc(adda_hip_op *op, int prec)
+{
+	if (!op) { ERR("null operator"); return nullptr; }
+	if (hipSetDevice(op->dev) != hipSuccess) { ERR("hipSetDevice failed"); return nullptr; }
+	void *p = nullptr;
+	hipError_t e = hipMalloc(&p, 3 * op->nd * elsize(prec));
+	if (e != hipSuccess) { ERR("hipMalloc of a vector failed: %s", hipGetErrorString(e)); return nullptr; }
+	if (hipMemsetAsync(p, 0, 3 * op->nd * elsize(prec), op->stream) != hipSuccess) { ERR("memset failed"); return nullptr; }
+	return p;
+}
+
+extern "C" int adda_hip_vec_free(adda_hip_op *op, void *v)
+{
+	if (!op) return ERR("null operator");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	if (v) HIPCHK(hipFree(v));
+	return 0;
+}
+
+extern "C" int adda_hip_vec_upload(adda_hip_op *op, int prec, void *dst, const adda_cplx *src)
+{
+	if (!op || !dst || !src) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = 3 * op->nd;
+	if (prec == ADDA_F64) {
+		HIPCHK(hipMemcpyAsync(dst, src, n * 16, hipMemcpyHostToDevice, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+	}
+	else {
+		std::vector<float> f(2 * n);
+		for (size_t i = 0; i < n; i++) { f[2 * i] = (float)src[i].re; f[2 * i + 1] = (float)src[i].im; }
+		HIPCHK(hipMemcpyAsync(dst, f.data(), n * 8, hipMemcpyHostToDevice, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_vec_download(adda_hip_op *op, int prec, adda_cplx *dst, const void *src)
+{
+	if (!op || !dst || !src) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = 3 * op->nd;
+	if (prec == ADDA_F64) {
+		HIPCHK(hipMemcpyAsync(dst, src, n * 16, hipMemcpyDeviceToHost, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+	}
+	else {
+		std::vector<float> f(2 * n);
+		HIPCHK(hipMemcpyAsync(f.data(), src, n * 8, hipMemcpyDeviceToHost, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+		for (size_t i = 0; i < n; i++) { dst[i].re = f[2 * i]; dst[i].im = f[2 * i + 1]; }
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_vec_convert(adda_hip_op *op, void *dst, int dprec, const void *src, int sprec)
+{
+	if (!op || !dst || !src) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	if (dprec == sprec) HIPCHK(hipMemcpyAsync(dst, src, n * elsize(dprec), hipMemcpyDeviceToDevice, op->stream));
+	else if (dprec == ADDA_F32)
+		hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, op->stream, (cx<float> *)dst,
+			(const cx<double> *)src, n);
+	else
+		hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, op->stream, (cx<double> *)dst,
+			(const cx<float> *)src, n);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int adda_hip_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, double *inprod, int her)
+{
+	if (!op || !argvec || !resultvec) return ERR("null argument");
+	if (argvec == resultvec) return ERR("argvec and resultvec must not alias (src/matvec.c:128 restrict)");
+	HIPCHK(hipSetDevice(op->dev));
+	if (prec == ADDA_F64) return matvec_t<double>(op, (const cx<double> *)argvec, (cx<double> *)resultvec, inprod, her);
+	return matvec_t<float>(op, (const cx<float> *)argvec, (cx<float> *)resultvec, inprod, her);
+}
+
+extern "C" int adda_hip_matvec_host(adda_hip_op *op, const adda_cplx *argvec, adda_cplx *resultvec, double *inprod,
+	int her)
+{
+	if (!op || !argvec || !resultvec) return ERR("null argument");
+	void *a = adda_hip_vec_alloc(op, ADDA_F64), *r = adda_hip_vec_alloc(op, ADDA_F64);
+	int rc = (a && r) ? 0 : -1;
+	if (!rc) rc = adda_hip_vec_upload(op, ADDA_F64, a, argvec);
+	if (!rc) rc = adda_hip_matvec(op, ADDA_F64, a, r, inprod, her);
+	if (!rc) rc = adda_hip_vec_download(op, ADDA_F64, resultvec, r);
+	if (a) adda_hip_vec_free(op, a);
+	if (r) adda_hip_vec_free(op, r);
+	return rc;
+}
+
+extern "C" int adda_hip_lincomb(adda_hip_op *op, int prec, void *out, const void *a, adda_cplx ca, const void *b,
+	adda_cplx cb, const void *c, adda_cplx cc, int conj_flags, double *norm2)
+{
+	if (!op || !out) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	cx<double> A = mk<double>(ca.re, ca.im), B = mk<double>(cb.re, cb.im), Cc = mk<double>(cc.re, cc.im);
+	double *part = norm2 ? op->partials : nullptr;
+	if (prec == ADDA_F64)
+		hipLaunchKernelGGL(k_lincomb<double>, dim3(nb), dim3(256), 0, op->stream, (cx<double> *)out,
+			(const cx<double> *)a, A, (const cx<double> *)b, B, (const cx<double> *)c, Cc, conj_flags, n, part);
+	else
+		hipLaunchKernelGGL(k_lincomb<float>, dim3(nb), dim3(256), 0, op->stream, (cx<float> *)out,
+			(const cx<float> *)a, A, (const cx<float> *)b, B, (const cx<float> *)c, Cc, conj_flags, n, part);
+	HIPCHK(hipGetLastError());
+	if (norm2) return reduce_finish(op, nb, 1, norm2);
+	return 0;
+}
+
+extern "C" int adda_hip_dot(adda_hip_op *op, int prec, const void *a, const void *b, int mode, adda_cplx *res,
+	double *norm2)
+{
+	if (!op || !a) return ERR("null argument");
+	if ((mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) && !b) return ERR("dot mode %d needs two vectors", mode);
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	if (prec == ADDA_F64)
+		hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(256), 0, op->stream, (const cx<double> *)a,
+			(const cx<double> *)b, mode, n, op->partials, nb);
+	else
+		hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(256), 0, op->stream, (const cx<float> *)a,
+			(const cx<float> *)b, mode, n, op->partials, nb);
+	HIPCHK(hipGetLastError());
+	double v[3];
+	if (reduce_finish(op, nb, 3, v)) return -1;
+	if (mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) {
+		if (res) { res->re = v[0]; res->im = v[1]; }
+	}
+	else {
+		if (res) { res->re = v[0] - v[1]; res->im = 2 * v[2]; }  // src/linalg.c:143
+		if (norm2) *norm2 = v[0] + v[1];
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_mult_mat(adda_hip_op *op, int prec, void *out, const void *in)
+{
+	if (!op || !out || !in) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const int nb = nblocks_red(3 * op->nd);
+	if (prec == ADDA_F64)
+		hipLaunchKernelGGL(k_mult_mat<double>, dim3(nb), dim3(256), 0, op->stream, (cx<double> *)out,
+			(const cx<double> *)in, op->mat, op->cc64, op->nd);
+	else
+		hipLaunchKernelGGL(k_mult_mat<float>, dim3(nb), dim3(256), 0, op->stream, (cx<float> *)out,
+			(const cx<float> *)in, op->mat, op->cc32, op->nd);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int adda_hip_plane_wave(adda_hip_op *op, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3], void *einc)
+{
+	if (!op || !prop || !incpol || !einc) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	hipLaunchKernelGGL(k_plane_wave, dim3(nblocks_red(op->nd)), dim3(256), 0, op->stream, (cx<double> *)einc, op->pos,
+		op->nd, (op->bX - 1) / 2.0, (op->bY - 1) / 2.0, (op->bZ - 1) / 2.0, gridspace, wavenum, prop[0], prop[1],
+		prop[2], incpol[0], incpol[1], incpol[2]);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, const double *dirs,
+	int ndir, adda_cplx *sums)
+{
+	if (!op || !pvec || !dirs || !sums || ndir < 0) return ERR("bad argument");
+	HIPCHK(hipSetDevice(op->dev));
+	std::vector<double> out;
+	for (int d0 = 0; d0 < ndir; d0 += kMaxDir) {
+		const int nd = ndir - d0 < kMaxDir ? ndir - d0 : kMaxDir;
+		if ((size_t)nd * 6 * kDirBlocks > op->npartials) return ERR("partials buffer too small");
+		HIPCHK(hipMemcpyAsync(op->dirbuf, dirs + 3 * (size_t)d0, 3 * nd * sizeof(double), hipMemcpyHostToDevice,
+			op->stream));
+		if (prec == ADDA_F64)
+			hipLaunchKernelGGL(k_field_sums<double>, dim3(kDirBlocks, nd), dim3(256), 0, op->stream,
+				(const cx<double> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials);
+		else
+			hipLaunchKernelGGL(k_field_sums<float>, dim3(kDirBlocks, nd), dim3(256), 0, op->stream,
+				(const cx<float> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials);
+		// reuse the tail of dirbuf is not possible (too small): final sums go to the head of partials' second half
+		double *fin = op->partials + (size_t)nd * 6 * kDirBlocks;
+		if ((size_t)nd * 6 * (kDirBlocks + 1) > op->npartials) return ERR("partials buffer too small");
+		hipLaunchKernelGGL(k_field_final, dim3((nd * 6 + 255) / 256), dim3(256), 0, op->stream, op->partials,
+			kDirBlocks, nd, fin);
+		HIPCHK(hipGetLastError());
+		out.resize((size_t)nd * 6);
+		HIPCHK(hipMemcpyAsync(out.data(), fin, out.size() * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+		for (int i = 0; i < nd * 3; i++) {
+			sums[3 * (size_t)d0 + i].re = out[2 * i];
+			sums[3 * (size_t)d0 + i].im = out[2 * i + 1];
+		}
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v, const double *w, double *out)
+{
+	if (!op || !v || !w || !out) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipMemcpyAsync(op->wbuf, w, (size_t)op->nmat * 3 * sizeof(double), hipMemcpyHostToDevice, op->stream));
+	const int nb = nblocks_red(3 * op->nd);
+	if (prec == ADDA_F64)
+		hipLaunchKernelGGL(k_wnorm<double>, dim3(nb), dim3(256), 0, op->stream, (const cx<double> *)v, op->mat,
+			op->wbuf, op->nd, op->partials);
+	else
+		hipLaunchKernelGGL(k_wnorm<float>, dim3(nb), dim3(256), 0, op->stream, (const cx<float> *)v, op->mat,
+			op->wbuf, op->nd, op->partials);
+	HIPCHK(hipGetLastError());
+	return reduce_finish(op, nb, 1, out);
+}
+
+extern "C" int adda_hip_sync(adda_hip_op *op)
+{
+	if (!op) return ERR("null operator");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	return 0;
+}

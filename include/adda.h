@@ -1,0 +1,103 @@
+/* adda.h -- host-side C API of the MI355X ADDA solver: particle builder, polarizability, plane-wave beam, the
+ * Krylov driver (IterativeSolver with QMR_CS, BiCGStab, CGNR, BiCG_CS, CSYM), FP32-inner/FP64 defect correction, and
+ * the scattering quantities. Everything numeric runs through include/adda_hip.h; this layer holds the scalar
+ * recurrences and control flow, in C, the way ADDA's src/iterative.c holds them around MatVec.
+ */
+#ifndef ADDA_H
+#define ADDA_H
+
+#include <stddef.h>
+
+#include "adda_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* iterative methods, ADDA's enum iter subset (src/const.h; option -iter, src/param.c:1230-1245) */
+enum adda_iter { ADDA_IT_QMR_CS = 0, ADDA_IT_BICGSTAB = 1, ADDA_IT_CGNR = 2, ADDA_IT_BICG_CS = 3, ADDA_IT_CSYM = 4 };
+/* initial field (option -init_field, src/param.c:1146-1161) */
+enum adda_init { ADDA_IF_AUTO = 0, ADDA_IF_ZERO = 1, ADDA_IF_INC = 2 };
+/* polarizability relation (option -pol, subset) */
+enum adda_pol { ADDA_POL_LDR = 0, ADDA_POL_CM = 1, ADDA_POL_RRC = 2 };
+/* precision form */
+enum adda_prec { ADDA_PREC_FP64 = 0, ADDA_PREC_MIXED = 1 };
+
+/* Error text of the last failing call of this library or of the HIP backend. */
+const char *adda_last_error(void);
+
+/* ---- particle (MakeParticle subset, src/make_particle.c:1952-2319) -------------------------------------------- */
+typedef struct {
+	int boxX, boxY, boxZ;
+	size_t nd;                /* nvoid_Ndip */
+	unsigned short *position; /* [3*nd], dipoles ordered z-major, then y, then x (the loop order of MakeParticle) */
+	int *material;            /* [nd] */
+	int nmat;
+} adda_particle;
+
+/* shape: "sphere", "ellipsoid" (aspect_y, aspect_z = semi-axis ratios y/x, z/x), "box" (aspect = side ratios).
+ * grid = boxX (-grid option); boxY/boxZ follow FitBox_yz (src/make_particle.c:1295-1307). */
+int adda_make_particle(const char *shape, int grid, double aspect_y, double aspect_z, adda_particle *p);
+/* "-shape read" geometry file: lines "x y z [mat]" (1-based mat, comments '#'), src/make_particle.c ReadDipFile. */
+int adda_read_particle(const char *fname, adda_particle *p);
+void adda_free_particle(adda_particle *p);
+/* Volume ratio of the shape (volume_ratio, e.g. pi/6 for a sphere, src/make_particle.c:1553), for -size. */
+double adda_volume_ratio(const char *shape, double aspect_y, double aspect_z);
+
+/* ---- physics ------------------------------------------------------------------------------------------------- */
+/* CoupleConstant for scalar m (src/calculator.c:100-133,170-262). */
+adda_cplx adda_couple_constant(adda_cplx m, int polrel, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3]);
+/* Plane wave Einc_i = exp(i k r_i.prop) e_pol on the device (GenerateB, src/GenerateB.c:324-330). */
+int adda_generate_b(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3], void *d_einc /* FP64 device vector */);
+
+/* ---- solver ---------------------------------------------------------------------------------------------------- */
+typedef struct {
+	int method;          /* enum adda_iter */
+	double eps;          /* stopping tolerance on ||r||/||b|| (-eps k => 10^-k) */
+	int maxiter;         /* -maxiter */
+	int init_field;      /* enum adda_init */
+	int recalc_resid;    /* -recalc_resid: recompute the true residual at the end */
+	int precision;       /* enum adda_prec */
+	double inner_eps;    /* mixed: relative tolerance of each FP32 inner solve (default 1e-5 when 0) */
+	int max_outer;       /* mixed: maximum FP64 correction steps (default 30 when 0) */
+	int verbose;         /* print RE_ lines to stdout */
+} adda_solver_opts;
+
+typedef struct {
+	int rc;              /* 0 ok, -1 breakdown/error, -2 stagnation, 1 maxiter reached (warning) */
+	int niter;           /* Krylov iterations (summed over inner solves in mixed mode) */
+	int nmatvec;         /* matvecs in the working precision of the iterations */
+	int nmatvec_fp64;    /* FP64 matvecs of the outer loop (mixed mode) */
+	int nouter;          /* outer correction steps (mixed mode) */
+	double resid;        /* final relative residual of the recurrence (FP64 mode) or of the true residual (mixed) */
+	double true_resid;   /* true ||b-Ax||/||b|| when recalculated (recalc_resid or mixed), else -1 */
+} adda_solve_stats;
+
+/* IterativeSolver (src/iterative.c:1398-1530) on device vectors: solves (I+S.D.S).x = S.Einc, returns x and the
+ * polarization P = S.x (src/iterative.c:1527). d_einc, d_x, d_pvec: FP64 device vectors. resid_hist (may be NULL)
+ * receives the relative residual after each iteration (RE_ lines), hist_len entries at most. */
+int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const void *d_einc, void *d_x, void *d_pvec,
+	double *resid_hist, int hist_len, adda_solve_stats *st);
+
+/* ---- scattering quantities (src/crosssec.c, src/CalculateE.c) ------------------------------------------------ */
+/* CalcFieldFree for ndir directions (src/crosssec.c:506-595): ebuff[3*i..3*i+2] for unit vector dirs[3*i..]. */
+int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double *dirs, int ndir, adda_cplx *ebuff);
+/* ExtCross, plane wave (src/crosssec.c:811-823). */
+int adda_ext_cross(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double prop[3], const double incpol[3], double *cext);
+/* AbsCross, DRAINE (src/crosssec.c:852-884); cc: [nmat][3]. */
+int adda_abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
+	double *cabs);
+/* CalcEplaneYZ for one solution (src/CalculateE.c:423-507): eplane[2i] = E.per, eplane[2i+1] = E.par. */
+int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double prop[3], const double par[3], const double per[3], int ntheta, adda_cplx *eplane);
+/* ComputeMuellerMatrix over the yz plane (src/CalculateE.c:83-116,236-246); out[16*ntheta]. */
+void adda_mueller_yz(const adda_cplx *eplX, const adda_cplx *eplY, int ntheta, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,118 @@
+/* adda_hip.h -- C ABI of the MI355X (gfx950) backend for ADDA's FFT matrix-vector product and Krylov vector kernels.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, no HIP or torch types. Complex numbers are interleaved
+ * (re,im) pairs, layout-compatible with ADDA's `doublecomplex` (C99 double complex, src/types.h:30).
+ *
+ * Every entry returns 0 on success or a negative code; adda_hip_last_error() gives "file:line: message" for the last
+ * failure in the calling thread (the role of LogError/CL_CH_ERR, src/io.c:274-290, src/oclcore.h:58-75).
+ *
+ * Device vectors are `void *` device addresses of 3*nvoid_Ndip complex numbers (AoS per dipole: x,y,z), stored in
+ * FP64 (ADDA_F64, 16 B/element) or FP32 (ADDA_F32, 8 B/element). All calls are ordered on the operator's single HIP
+ * stream (one in-order queue, like src/oclcore.c:353-355); calls that return scalars synchronise that stream.
+ */
+#ifndef ADDA_HIP_H
+#define ADDA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } adda_cplx;
+
+enum { ADDA_F64 = 0, ADDA_F32 = 1 };
+
+/* dot-product modes (src/linalg.c) */
+enum {
+	ADDA_DOT_CONJ = 0,       /* sum a*conj(b)                 nDotProd            src/linalg.c:75-88   */
+	ADDA_DOT_BILIN = 1,      /* sum a*b                       nDotProd_conj       src/linalg.c:92-105  */
+	ADDA_DOT_SELF_BILIN = 2, /* sum a*a (+ ||a||^2 in norm2)  nDotProdSelf_conj(_Norm2) src/linalg.c:109-144 */
+	ADDA_DOT_NORM2 = 3       /* ||a||^2 in norm2              nNorm2              src/linalg.c:59-71   */
+};
+
+/* lincomb conjugation flags */
+enum { ADDA_CONJ_A = 1, ADDA_CONJ_B = 2 };
+
+typedef struct adda_hip_op adda_hip_op;
+
+/* Last error message of the calling thread ("" if none). */
+const char *adda_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when there is none). */
+int adda_hip_device_count(void);
+
+/* fftFit (src/fft.c:528-562) under the 2/3/5 rule that ADDA applies for clAmdFft (CLFFT_AMD, src/fft.c:542-545):
+ * the first x' >= x that is even, divisible by divis and has no prime factor other than 2, 3, 5. */
+int adda_hip_fftfit(int x, int divis);
+
+/* Creates the operator of one particle on `device` and builds its D-matrix on the GPU in FP64.
+ * Replaces InitDmatrix (src/fft.c:918-1416, with InterTerm_poi, src/interaction.c:290-331) together with oclinit
+ * (src/oclcore.c:327-437). position: ushort[3*nd] grid coordinates (src/vars.h:94); material: int[nd] indices into the
+ * cc_sqrt table of nmat rows (src/vars.h:65; nmat may equal nd for per-dipole materials). grid*: padded FFT grid
+ * (ParSetup, src/comm.c:489-494), each >= 2*box and a 2/3/5-smooth even number. flags: bit0 = also keep an FP32
+ * copy of the D-matrix and FP32 work buffers (mixed precision).  Returns NULL on failure. */
+adda_hip_op *adda_hip_op_create(int device, int boxX, int boxY, int boxZ, int gridX, int gridY, int gridZ, size_t nd,
+	const unsigned short *position, const int *material, int nmat, double wavenum, double gridspace, int reduced_fft,
+	int flags);
+
+/* Free_FFT_Dmat (src/fft.c:1420-1493) + oclunload (src/oclcore.c:476). */
+void adda_hip_op_destroy(adda_hip_op *op);
+
+/* Uploads cc_sqrt[nmat][3] (InitCC's device write, src/calculator.c:283-288); may be called between solves. */
+int adda_hip_set_cc_sqrt(adda_hip_op *op, const adda_cplx *cc_sqrt);
+
+/* Copies the D-matrix to the host in ADDA's layout Dmatrix[6*((x*DsizeZ+z)*DsizeY+y)+comp] (IndexDmatrix,
+ * src/fft.c:144-150); dst holds 6*gridX*DsizeY*DsizeZ complex numbers. For checks against a CPU build. */
+int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst);
+size_t adda_hip_dmatrix_size(const adda_hip_op *op);
+
+/* Device vectors: allocation, host <-> device copies (FP64 host data), precision conversion. */
+void *adda_hip_vec_alloc(adda_hip_op *op, int prec);
+int adda_hip_vec_free(adda_hip_op *op, void *v);
+int adda_hip_vec_upload(adda_hip_op *op, int prec, void *dst, const adda_cplx *src);
+int adda_hip_vec_download(adda_hip_op *op, int prec, adda_cplx *dst, const void *src);
+int adda_hip_vec_convert(adda_hip_op *op, void *dst, int dprec, const void *src, int sprec);
+size_t adda_hip_nrows(const adda_hip_op *op);  /* local_nRows = 3*nd */
+
+/* MatVec (src/matvec.c:128-425; OpenCL twin src/oclmatvec.c:35-184) on device vectors:
+ * result = A.arg with A = I + S.D.S (her: conj(A.conj(arg))); when inprod != NULL, *inprod = ||result||^2
+ * (synchronises). argvec is not modified. prec selects FP64 or FP32 arithmetic and storage. */
+int adda_hip_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, double *inprod, int her);
+
+/* The same with host vectors (the shape of src/oclmatvec.c:35-40: copy in, compute, copy out), FP64. */
+int adda_hip_matvec_host(adda_hip_op *op, const adda_cplx *argvec, adda_cplx *resultvec, double *inprod, int her);
+
+/* Fused vector update out = ca*a' + cb*b' + cc*c (a' = conj(a) if ADDA_CONJ_A, b' likewise) over 3*nd elements;
+ * any of a, b, c may be NULL (term dropped); out may alias a, b or c. When norm2 != NULL, *norm2 = ||out||^2
+ * (synchronises). Covers nIncrem*, nLinComb*, nSubtr, nMult*, nCopy, nInit (src/linalg.c:148-605). */
+int adda_hip_lincomb(adda_hip_op *op, int prec, void *out, const void *a, adda_cplx ca, const void *b, adda_cplx cb,
+	const void *c, adda_cplx cc, int conj_flags, double *norm2);
+
+/* Inner products (src/linalg.c:59-144) with FP64 accumulation and a deterministic two-level reduction
+ * (MyInnerProduct's role, src/comm.c:454-477); synchronises. res and/or norm2 may be NULL if not needed. */
+int adda_hip_dot(adda_hip_op *op, int prec, const void *a, const void *b, int mode, adda_cplx *res, double *norm2);
+
+/* nMult_mat (src/linalg.c:549-569): out[3i+c] = cc_sqrt[mat[i]][c] * in[3i+c]. */
+int adda_hip_mult_mat(adda_hip_op *op, int prec, void *out, const void *in);
+
+/* Plane-wave incident field on the dipoles (GenerateB, src/GenerateB.c:324-330): Einc_i = exp(i k r_i.prop) e_pol,
+ * r_i = (position_i - (box-1)/2)*gridspace (DipoleCoord, src/make_particle.c:2266-2275); FP64 device vector. */
+int adda_hip_plane_wave(adda_hip_op *op, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3], void *einc);
+
+/* Scattering sums of CalcFieldFree (src/crosssec.c:548-584) for ndir unit vectors n:
+ * sums[3*i+c] = sum_j P[3j+c] exp(-i kd n_i.position_j), kd = wavenum*gridspace; deterministic reduction. */
+int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, const double *dirs, int ndir,
+	adda_cplx *sums);
+
+/* sum_j sum_c w[mat_j][c] |v[3j+c]|^2 (the AbsCross sum, src/crosssec.c:873-884); w is [nmat][3]. */
+int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v, const double *w, double *out);
+
+/* Waits for all work queued on the operator's stream. */
+int adda_hip_sync(adda_hip_op *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

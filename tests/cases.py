@@ -1,0 +1,211 @@
+"""Test-side case builders: ADDA particle rules restated with numpy, plus ctypes access to the oracle library.
+
+Geometry follows MakeParticle (src/make_particle.c:1952-2060): with jagged=1 the dipole centre of cell (i,j,k) is
+x_r = (i - box/2 + 0.5)/boxX (all axes scaled by boxX); SH_SPHERE keeps x_r^2+y_r^2+z_r^2 <= 0.25 (2136-2137),
+SH_ELLIPSOID keeps x^2 + y^2/aY^2 + z^2/aZ^2 <= 0.25 (invsqY/invsqZ, 1635-1637), SH_BOX keeps |y|<=aY/2, |z|<=aZ/2.
+Box sizes follow FitBox/FitBox_yz (1283-1307). Dipoles are ordered z-major, then y, then x (the loop order).
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TWO_PI = 2 * math.pi
+
+
+class OrCplx(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class OrGrid(ctypes.Structure):
+    _fields_ = [("boxX", ctypes.c_int), ("boxY", ctypes.c_int), ("boxZ", ctypes.c_int),
+                ("gridX", ctypes.c_int), ("gridY", ctypes.c_int), ("gridZ", ctypes.c_int),
+                ("nd", ctypes.c_size_t), ("pos", ctypes.c_void_p), ("mat", ctypes.c_void_p),
+                ("wavenum", ctypes.c_double), ("gridspace", ctypes.c_double), ("reduced_fft", ctypes.c_int)]
+
+
+def fit_box(box):
+    """FitBox with jagged=1 (src/make_particle.c:1284-1291): smallest even number >= box."""
+    return 2 * ((box + 1) // 2)
+
+
+def fit_box_yz(size):
+    """FitBox_yz with jagged=1 (src/make_particle.c:1295-1307)."""
+    return 2 * int(math.floor((size + 1) / 2))
+
+
+def make_shape(shape, box, aspect_y=1.0, aspect_z=1.0):
+    """Return (position uint16[nd,3], (boxX, boxY, boxZ)) for 'sphere', 'ellipsoid', 'box'."""
+    bx = fit_box(box)
+    if shape == "sphere":
+        by = bz = bx
+    else:
+        by = fit_box(fit_box_yz(aspect_y * bx))
+        bz = fit_box(fit_box_yz(aspect_z * bx))
+    k, j, i = np.meshgrid(np.arange(bz), np.arange(by), np.arange(bx), indexing="ij")
+    xr = (i - bx // 2 + 0.5) / bx
+    yr = (j - by // 2 + 0.5) / bx
+    zr = (k - bz // 2 + 0.5) / bx
+    if shape == "sphere":
+        keep = xr * xr + yr * yr + zr * zr <= 0.25
+    elif shape == "ellipsoid":
+        keep = xr * xr + yr * yr / (aspect_y ** 2) + zr * zr / (aspect_z ** 2) <= 0.25
+    elif shape == "box":
+        keep = (np.abs(yr) <= aspect_y / 2) & (np.abs(zr) <= aspect_z / 2)
+    else:
+        raise ValueError(shape)
+    pos = np.stack([i[keep], j[keep], k[keep]], axis=1).astype(np.uint16)
+    return np.ascontiguousarray(pos), (bx, by, bz)
+
+
+_lib = None
+
+
+def oracle():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        lib.or_fftfit.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.or_fftfit.restype = ctypes.c_int
+        lib.or_green_poi.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 2 + [P]
+        lib.or_fft1d.argtypes = [P, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+        lib.or_dmatrix_size.argtypes = [P]
+        lib.or_dmatrix_size.restype = ctypes.c_size_t
+        lib.or_init_dmatrix.argtypes = [P, P]
+        lib.or_matvec_fft.argtypes = [P, P, P, P, P, P, ctypes.c_int]
+        lib.or_matvec_direct.argtypes = [P, P, P, P, P, ctypes.c_int]
+        lib.or_couple_constant.argtypes = [OrCplx, ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P]
+        lib.or_couple_constant.restype = OrCplx
+        lib.or_generate_b.argtypes = [P, P, P, P]
+        lib.or_iterative_solver.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                            P, P, P, ctypes.c_int, P, P]
+        lib.or_iterative_solver.restype = ctypes.c_int
+        lib.or_calc_field.argtypes = [P, P, P, P]
+        lib.or_ext_cross.argtypes = [P, P, P, P]
+        lib.or_ext_cross.restype = ctypes.c_double
+        lib.or_abs_cross.argtypes = [P, P, P]
+        lib.or_abs_cross.restype = ctypes.c_double
+        lib.or_eplane_yz.argtypes = [P, P, P, P, P, ctypes.c_int, P]
+        lib.or_mueller_yz.argtypes = [P, P, ctypes.c_int, P]
+        _lib = lib
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Case:
+    """One DDA problem: geometry, grid, wavelength, materials; matches ADDA's variables for the same command line."""
+
+    def __init__(self, pos, box, m, dpl=None, lam=TWO_PI, mat=None, rule=1, reduced_fft=True, grid=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.uint16)
+        self.box = tuple(int(b) for b in box)
+        self.nd = len(self.pos)
+        self.m = np.atleast_1d(np.asarray(m, dtype=np.complex128))
+        self.mat = np.zeros(self.nd, np.int32) if mat is None else np.ascontiguousarray(mat, dtype=np.int32)
+        if dpl is None:  # dpl_def = 10*|m| (src/make_particle.c:1353)
+            dpl = 10 * float(np.max(np.abs(self.m)))
+        self.dpl = dpl
+        self.lam = lam
+        self.wavenum = TWO_PI / lam
+        self.gridspace = lam / dpl
+        lib = oracle()
+        if grid is None:  # ParSetup with one process (src/comm.c:489-494)
+            grid = tuple(lib.or_fftfit(2 * b, 1, rule) for b in self.box)
+        self.grid = tuple(grid)
+        self.reduced_fft = reduced_fft
+        self._g = OrGrid(*self.box, *self.grid, self.nd, self.pos.ctypes.data, self.mat.ctypes.data,
+                         self.wavenum, self.gridspace, int(reduced_fft))
+        self.gp = ctypes.byref(self._g)
+
+    def cc(self, prop, incpol, polrel=0):
+        """cc[nmat][3] (isotropic => 3 equal components) and cc_sqrt = csqrt(cc) (InitCC, src/calculator.c:266-289)."""
+        lib = oracle()
+        prop = np.ascontiguousarray(prop, np.float64)
+        incpol = np.ascontiguousarray(incpol, np.float64)
+        out = np.empty((len(self.m), 3), np.complex128)
+        for i, mi in enumerate(self.m):
+            r = lib.or_couple_constant(OrCplx(mi.real, mi.imag), polrel, self.wavenum, self.gridspace, ptr(prop),
+                                       ptr(incpol))
+            out[i, :] = complex(r.re, r.im)
+        return out, np.sqrt(out)
+
+    def dmatrix(self):
+        lib = oracle()
+        d = np.empty(lib.or_dmatrix_size(self.gp), np.complex128)
+        lib.or_init_dmatrix(self.gp, ptr(d))
+        return d
+
+    def matvec_fft(self, dmat, cc_sqrt, x, her=False):
+        lib = oracle()
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.empty_like(x)
+        ip = ctypes.c_double()
+        lib.or_matvec_fft(self.gp, ptr(dmat), ptr(np.ascontiguousarray(cc_sqrt)), ptr(x), ptr(y), ctypes.byref(ip),
+                          int(her))
+        return y, ip.value
+
+    def matvec_direct(self, cc_sqrt, x, her=False):
+        lib = oracle()
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.empty_like(x)
+        ip = ctypes.c_double()
+        lib.or_matvec_direct(self.gp, ptr(np.ascontiguousarray(cc_sqrt)), ptr(x), ptr(y), ctypes.byref(ip), int(her))
+        return y, ip.value
+
+    def einc(self, prop, incpol):
+        lib = oracle()
+        e = np.empty(3 * self.nd, np.complex128)
+        lib.or_generate_b(self.gp, ptr(np.ascontiguousarray(prop, np.float64)),
+                          ptr(np.ascontiguousarray(incpol, np.float64)), ptr(e))
+        return e
+
+    def solve(self, dmat, cc_sqrt, einc, method=0, eps=1e-5, maxiter=None, init_field=0, hist=1000):
+        lib = oracle()
+        if maxiter is None:  # 3*Ndip (src/make_particle.c:1939)
+            maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
+        x = np.empty(3 * self.nd, np.complex128)
+        p = np.empty_like(x)
+        h = np.zeros(hist)
+        ni, nm = ctypes.c_int(), ctypes.c_int()
+        rc = lib.or_iterative_solver(self.gp, None if dmat is None else ptr(dmat), ptr(np.ascontiguousarray(cc_sqrt)),
+                                     ptr(np.ascontiguousarray(einc)), method, eps, maxiter, init_field, ptr(x), ptr(p),
+                                     ptr(h), hist, ctypes.byref(ni), ctypes.byref(nm))
+        return dict(rc=rc, x=x, p=p, hist=h[:ni.value + 1].copy(), niter=ni.value, nmatvec=nm.value)
+
+    def cext(self, p, prop, incpol):
+        return oracle().or_ext_cross(self.gp, ptr(p), ptr(np.ascontiguousarray(prop, np.float64)),
+                                     ptr(np.ascontiguousarray(incpol, np.float64)))
+
+    def cabs(self, p, cc):
+        return oracle().or_abs_cross(self.gp, ptr(p), ptr(np.ascontiguousarray(cc)))
+
+    def eplane(self, p, prop, par, per, ntheta):
+        e = np.empty(2 * ntheta, np.complex128)
+        oracle().or_eplane_yz(self.gp, ptr(p), ptr(np.ascontiguousarray(prop, np.float64)),
+                              ptr(np.ascontiguousarray(par, np.float64)), ptr(np.ascontiguousarray(per, np.float64)),
+                              ntheta, ptr(e))
+        return e
+
+    def mueller(self, eX, eY, ntheta):
+        out = np.empty((ntheta, 16))
+        oracle().or_mueller_yz(ptr(eX), ptr(eY), ntheta, ptr(out))
+        return out
+
+
+PROP_Z = np.array([0.0, 0.0, 1.0])
+POL_Y = np.array([0.0, 1.0, 0.0])
+POL_X = np.array([1.0, 0.0, 0.0])
+
+
+def sample_case():
+    """Config 1: sphere 16^3 (2176 dipoles), m=1.5, dpl=15 (ADDA defaults), grid 32^3."""
+    pos, box = make_shape("sphere", 16)
+    return Case(pos, box, 1.5, dpl=15)

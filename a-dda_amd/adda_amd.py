@@ -70,6 +70,12 @@ def hip():
     L.adda_hip_dot.argtypes = [P, I, P, P, I, P, P]
     L.adda_hip_mult_mat.argtypes = [P, I, P, P]
     L.adda_hip_sync.argtypes = [P]
+    L.adda_hip_plane_wave.argtypes = [P, D, D, P, P, P]
+    L.adda_hip_field_sums.argtypes = [P, I, P, D, P, I, P]
+    L.adda_hip_weighted_norm2.argtypes = [P, I, P, P, P]
+    L.adda_hip_profile_matvec.argtypes = [P, I, P, P, I, P, P]
+    L.adda_hip_matvec_bytes.argtypes = [P, I, I]
+    L.adda_hip_matvec_bytes.restype = D
     _hip = L
     return L
 
@@ -169,3 +175,208 @@ class Operator:
 
     def sync(self):
         check(hip().adda_hip_sync(self.h))
+
+    def profile_matvec(self, a, r, prec=F64, nrep=10):
+        """Average per-pass and total matvec time (ms), HIP events on the operator's stream."""
+        ms = np.zeros(5)
+        tot = ctypes.c_double()
+        check(hip().adda_hip_profile_matvec(self.h, prec, a, r, nrep, _p(ms), ctypes.byref(tot)))
+        return ms, tot.value
+
+    def matvec_bytes(self, prec=F64, pass_=5):
+        return hip().adda_hip_matvec_bytes(self.h, prec, pass_)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# C host layer (include/adda.h)
+
+IT_QMR, IT_BICGSTAB, IT_CGNR, IT_BICG, IT_CSYM = 0, 1, 2, 3, 4
+ITER_NAMES = {"qmr": IT_QMR, "bicgstab": IT_BICGSTAB, "cgnr": IT_CGNR, "bicg": IT_BICG, "csym": IT_CSYM}
+IF_AUTO, IF_ZERO, IF_INC = 0, 1, 2
+POL_LDR, POL_CM, POL_RRC = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class Particle(ctypes.Structure):
+    _fields_ = [("boxX", ctypes.c_int), ("boxY", ctypes.c_int), ("boxZ", ctypes.c_int), ("nd", ctypes.c_size_t),
+                ("position", ctypes.POINTER(ctypes.c_ushort)), ("material", ctypes.POINTER(ctypes.c_int)),
+                ("nmat", ctypes.c_int)]
+
+
+class SolverOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("eps", ctypes.c_double), ("maxiter", ctypes.c_int),
+                ("init_field", ctypes.c_int), ("recalc_resid", ctypes.c_int), ("precision", ctypes.c_int),
+                ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("rc", ctypes.c_int), ("niter", ctypes.c_int), ("nmatvec", ctypes.c_int),
+                ("nmatvec_fp64", ctypes.c_int), ("nouter", ctypes.c_int), ("resid", ctypes.c_double),
+                ("true_resid", ctypes.c_double)]
+
+
+_host = None
+
+
+def host():
+    """Load libadda.so (C host layer); raise if missing."""
+    global _host
+    if _host is not None:
+        return _host
+    hip()
+    path = os.path.join(HERE, "libadda.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build first (__graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.adda_last_error.restype = ctypes.c_char_p
+    L.adda_make_particle.argtypes = [ctypes.c_char_p, I, D, D, ctypes.POINTER(Particle)]
+    L.adda_read_particle.argtypes = [ctypes.c_char_p, ctypes.POINTER(Particle)]
+    L.adda_free_particle.argtypes = [ctypes.POINTER(Particle)]
+    L.adda_free_particle.restype = None
+    L.adda_volume_ratio.argtypes = [ctypes.c_char_p, D, D]
+    L.adda_volume_ratio.restype = D
+    L.adda_couple_constant.argtypes = [Cplx, I, D, D, P, P]
+    L.adda_couple_constant.restype = Cplx
+    L.adda_generate_b.argtypes = [P, P, D, D, P, P, P]
+    L.adda_iterative_solver.argtypes = [P, ctypes.POINTER(SolverOpts), P, P, P, P, I, ctypes.POINTER(SolveStats)]
+    L.adda_calc_field.argtypes = [P, P, D, D, P, P, I, P]
+    L.adda_ext_cross.argtypes = [P, P, D, D, P, P, P, P]
+    L.adda_abs_cross.argtypes = [P, P, D, P, P, P]
+    L.adda_eplane_yz.argtypes = [P, P, D, D, P, P, P, P, I, I, P]
+    L.adda_mueller_yz.argtypes = [P, P, I, P]
+    L.adda_mueller_yz.restype = None
+    _host = L
+    return L
+
+
+def hcheck(rc):
+    if rc < 0:
+        raise RuntimeError(host().adda_last_error().decode())
+    return rc
+
+
+def make_particle(shape, grid, aspect_y=1.0, aspect_z=1.0):
+    """C particle builder -> (position uint16[nd,3], material int32[nd], box)."""
+    L = host()
+    p = Particle()
+    hcheck(L.adda_make_particle(shape.encode(), grid, aspect_y, aspect_z, ctypes.byref(p)))
+    try:
+        pos = np.ctypeslib.as_array(p.position, shape=(p.nd * 3,)).reshape(-1, 3).copy()
+        mat = np.ctypeslib.as_array(p.material, shape=(p.nd,)).copy()
+        return pos, mat, (p.boxX, p.boxY, p.boxZ)
+    finally:
+        L.adda_free_particle(ctypes.byref(p))
+
+
+def couple_constant(m, polrel, wavenum, gridspace, prop, incpol):
+    prop = np.ascontiguousarray(prop, np.float64)
+    incpol = np.ascontiguousarray(incpol, np.float64)
+    return host().adda_couple_constant(Cplx.of(m), polrel, wavenum, gridspace, _p(prop), _p(incpol)).value()
+
+
+class Problem:
+    """A particle + operator + the incident-field / solve / scattering pipeline of one orientation (the Calculator
+    flow of src/calculator.c:293-344) on the GPU."""
+
+    def __init__(self, pos, box, m, mat=None, dpl=None, lam=2 * np.pi, reduced_fft=True, fp32=False, device=0,
+                 polrel=POL_LDR, grid=None):
+        self.pos = np.ascontiguousarray(pos, np.uint16)
+        self.box = tuple(int(b) for b in box)
+        self.nd = len(self.pos)
+        self.m = np.atleast_1d(np.asarray(m, np.complex128))
+        self.mat = np.zeros(self.nd, np.int32) if mat is None else np.ascontiguousarray(mat, np.int32)
+        self.nmat = len(self.m)
+        if dpl is None:
+            dpl = 10 * float(np.max(np.abs(self.m)))
+        self.dpl, self.lam = dpl, lam
+        self.wavenum = 2 * np.pi / lam
+        self.gridspace = lam / dpl
+        self.polrel = polrel
+        self.grid = tuple(grid) if grid else tuple(hip().adda_hip_fftfit(2 * b, 1) for b in self.box)
+        self.op = Operator(self.pos, self.box, self.grid, self.mat, self.nmat, self.wavenum, self.gridspace,
+                           reduced_fft, fp32=fp32, device=device)
+        self._p = Particle(self.box[0], self.box[1], self.box[2], self.nd,
+                           self.pos.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort)),
+                           self.mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), self.nmat)
+        self.einc = self.op.alloc()
+        self.x = self.op.alloc()
+        self.pvec = self.op.alloc()
+
+    def close(self):
+        if self.op is not None:
+            for v in (self.einc, self.x, self.pvec):
+                self.op.free(v)
+            self.op.close()
+            self.op = None
+
+    def set_polarization(self, prop, incpol):
+        """InitCC (cc, cc_sqrt for every material) + GenerateB for this incident polarization."""
+        self.prop = np.ascontiguousarray(prop, np.float64)
+        self.incpol = np.ascontiguousarray(incpol, np.float64)
+        cc = np.array([couple_constant(mi, self.polrel, self.wavenum, self.gridspace, self.prop, self.incpol)
+                       for mi in self.m])
+        self.cc = np.repeat(cc[:, None], 3, axis=1)
+        self.cc_sqrt = np.sqrt(self.cc)
+        self.op.set_cc_sqrt(self.cc_sqrt)
+        hcheck(host().adda_generate_b(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, _p(self.prop),
+                                      _p(self.incpol), self.einc))
+
+    def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
+              max_outer=0, recalc_resid=False, hist_len=20000, verbose=False):
+        if maxiter is None:
+            maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
+        o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
+                       inner_eps, max_outer, int(verbose))
+        st = SolveStats()
+        hist = np.zeros(hist_len)
+        rc = host().adda_iterative_solver(self.op.h, ctypes.byref(o), self.einc, self.x, self.pvec, _p(hist),
+                                          hist_len, ctypes.byref(st))
+        hcheck(rc)
+        n = min(st.niter + 1, hist_len)
+        return dict(rc=st.rc, niter=st.niter, nmatvec=st.nmatvec, nmatvec_fp64=st.nmatvec_fp64, nouter=st.nouter,
+                    resid=st.resid, true_resid=st.true_resid, hist=hist[:n].copy())
+
+    def cext(self):
+        out = ctypes.c_double()
+        hcheck(host().adda_ext_cross(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, self.pvec,
+                                     _p(self.prop), _p(self.incpol), ctypes.byref(out)))
+        return out.value
+
+    def cabs(self):
+        out = ctypes.c_double()
+        cc = np.ascontiguousarray(self.cc)
+        hcheck(host().adda_abs_cross(self.op.h, ctypes.byref(self._p), self.wavenum, _p(cc), self.pvec,
+                                     ctypes.byref(out)))
+        return out.value
+
+    def eplane(self, par, per, ntheta, npoints=None):
+        npoints = ntheta if npoints is None else npoints
+        out = np.empty(2 * npoints, np.complex128)
+        par = np.ascontiguousarray(par, np.float64)
+        per = np.ascontiguousarray(per, np.float64)
+        hcheck(host().adda_eplane_yz(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, self.pvec,
+                                     _p(self.prop), _p(par), _p(per), ntheta, npoints, _p(out)))
+        return out
+
+    def polarization(self):
+        return self.op.download(self.pvec)
+
+    def field_x(self):
+        return self.op.download(self.x)
+
+
+def mueller_yz(eX, eY, ntheta):
+    out = np.empty((ntheta, 16))
+    host().adda_mueller_yz(_p(np.ascontiguousarray(eX)), _p(np.ascontiguousarray(eY)), ntheta, _p(out))
+    return out
+
+
+def rotate_zyz(alpha, beta, gamma, v):
+    """InitRotation (src/crosssec.c:82-118): beta_matr . v for zyz Euler angles in degrees."""
+    a, b, g = np.radians([alpha, beta, gamma])
+    ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(g), np.sin(g)
+    m = np.array([[ca * cb * cg - sa * sg, sa * cb * cg + ca * sg, -sb * cg],
+                  [-ca * cb * sg - sa * cg, -sa * cb * sg + ca * cg, sb * sg],
+                  [ca * sb, sa * sb, cb]])
+    return m @ np.asarray(v, np.float64)

@@ -642,7 +642,7 @@ static int set_lds_limits()
 // matvec driver
 
 template <typename T>
-static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her)
+static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her, hipEvent_t *ev = nullptr)
 {
 	const cx<T> *ccs;
 	const cx<T> *D;
@@ -653,6 +653,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 	const int gX = op->gX, bX = op->bX, bY = op->bY, bZ = op->bZ, gY = op->gY, gZ = op->gZ;
 	const int nlx = bY * bZ;
 	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
+	if (ev) HIPCHK(hipEventRecord(ev[0], op->stream));
 	// P1
 	{
 		const int nblk = (nlx + op->LL - 1) / op->LL;
@@ -660,6 +661,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 		hipLaunchKernelGGL(k_scatter_fftx<T>, dim3(nblk), dim3(256), lds, op->stream, arg, Xa, op->cellmap, op->mat,
 			ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her);
 	}
+	if (ev) HIPCHK(hipEventRecord(ev[1], op->stream));
 	// P2: forward FFT-z over bZ nonzero rows; batch b = c*bY + y
 	{
 		const int TX = op->TXz;
@@ -669,6 +671,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 			gX, TX, bZ, gZ, (long)bY * gX, (long)bY * gX, bY, (long)bZ * bY * gX, (long)gX, (long)gZ * bY * gX,
 			(long)gX);
 	}
+	if (ev) HIPCHK(hipEventRecord(ev[2], op->stream));
 	// P3
 	{
 		const int TX = op->TXy;
@@ -677,6 +680,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 		hipLaunchKernelGGL(k_ydmul<T>, grid, dim3(op->NTy), lds, op->stream, Xb, D, op->py, tw_of<T>(op->twy), gX, bY,
 			gZ, op->DsY, op->DsZ, TX, op->reduced, transposed);
 	}
+	if (ev) HIPCHK(hipEventRecord(ev[3], op->stream));
 	// P4: inverse FFT-z, keep z<bZ
 	{
 		const int TX = op->TXz;
@@ -686,6 +690,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 			gX, TX, gZ, bZ, (long)bY * gX, (long)bY * gX, bY, (long)gZ * bY * gX, (long)gX, (long)bZ * bY * gX,
 			(long)gX);
 	}
+	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
 	// P5
 	const int nblk5 = (nlx + op->LL - 1) / op->LL;
 	{
@@ -694,6 +699,7 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 			op->cellmap, op->mat, ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her,
 			inprod ? op->partials : (double *)nullptr);
 	}
+	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
 	if (inprod) {
 		double v;
@@ -1125,6 +1131,51 @@ extern "C" int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v,
 			op->wbuf, op->nd, op->partials);
 	HIPCHK(hipGetLastError());
 	return reduce_finish(op, nb, 1, out);
+}
+
+extern "C" int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, int nrep,
+	double *pass_ms, double *total_ms)
+{
+	if (!op || !argvec || !resultvec || nrep <= 0 || !pass_ms || !total_ms) return ERR("bad argument");
+	HIPCHK(hipSetDevice(op->dev));
+	hipEvent_t ev[6];
+	for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
+	double acc[6] = {0, 0, 0, 0, 0, 0};
+	int rc = 0;
+	for (int r = 0; r < nrep && !rc; r++) {
+		if (prec == ADDA_F64) rc = matvec_t<double>(op, (const cx<double> *)argvec, (cx<double> *)resultvec, nullptr, 0, ev);
+		else rc = matvec_t<float>(op, (const cx<float> *)argvec, (cx<float> *)resultvec, nullptr, 0, ev);
+		if (rc) break;
+		HIPCHK(hipEventSynchronize(ev[5]));
+		for (int i = 0; i < 5; i++) {
+			float ms;
+			HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+			acc[i] += ms;
+		}
+		float tot;
+		HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[5]));
+		acc[5] += tot;
+	}
+	for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+	if (rc) return rc;
+	for (int i = 0; i < 5; i++) pass_ms[i] = acc[i] / nrep;
+	*total_ms = acc[5] / nrep;
+	return 0;
+}
+
+extern "C" double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pass)
+{
+	if (!op) return 0;
+	const double c = prec == ADDA_F32 ? 8 : 16;
+	const double gX = op->gX, bY = op->bY, bZ = op->bZ, gZ = op->gZ, nd = (double)op->nd;
+	const double nbox = (double)op->bX * bY * bZ;
+	const double xa = 3 * gX * bY * bZ * c, xb = 3 * gX * bY * gZ * c, vec = 3 * nd * c;
+	const double dm = 6 * gX * op->DsY * op->DsZ * c;
+	// index data: cell map (4 B per box cell) + material (4 B per dipole), read by P1 and by P5
+	const double idx = 4 * nbox + 4 * nd;
+	const double b[5] = {vec + idx + xa, xa + xb, 2 * xb + dm, xb + xa, xa + 2 * vec + idx};
+	if (pass >= 0 && pass < 5) return b[pass];
+	return b[0] + b[1] + b[2] + b[3] + b[4];
 }
 
 extern "C" int adda_hip_sync(adda_hip_op *op)

@@ -1,0 +1,317 @@
+/* adda_host.c -- host side of the MI355X ADDA solver: particle builder (MakeParticle subset), polarizability
+ * (CoupleConstant), plane wave, scattering quantities (CalcFieldFree, ExtCross, AbsCross, CalcEplaneYZ, Mueller).
+ * Bulk sums over dipoles run on the device (adda_hip_field_sums, adda_hip_weighted_norm2); the O(1) prefactors and
+ * projections stay here in FP64.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../../include/adda.h"
+#include "host_internal.h"
+
+typedef double complex dcx;
+
+#define PI 3.14159265358979323846264338327950288
+#define TWO_PI 6.283185307179586476925286766559
+#define FOUR_PI 12.566370614359172953850573533118
+/* src/const.h:234-236 */
+#define LDR_B1 1.8915316
+#define LDR_B2 -0.1648469
+#define LDR_B3 1.7700004
+
+static char g_err[600];
+
+int host_err(const char *file, int line, const char *fmt, ...)
+{
+	char msg[480];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(msg, sizeof msg, fmt, ap);
+	va_end(ap);
+	snprintf(g_err, sizeof g_err, "%s:%d: %s", file, line, msg);
+	return -1;
+}
+
+int host_err_hip(const char *file, int line)
+{
+	const char *e = adda_hip_last_error();
+	if (e && e[0]) snprintf(g_err, sizeof g_err, "%s:%d: %s", file, line, e);
+	else if (!g_err[0]) snprintf(g_err, sizeof g_err, "%s:%d: backend call failed", file, line);
+	return -1;
+}
+
+const char *adda_last_error(void) { return g_err; }
+
+/* ---- particle ------------------------------------------------------------------------------------------------- */
+/* FitBox / FitBox_yz with jagged = 1, src/make_particle.c:1284-1307 */
+static int fit_box(int box) { return 2 * ((box + 1) / 2); }
+static int fit_box_yz(double size) { return 2 * (int)floor((size + 1) / 2); }
+
+int adda_make_particle(const char *shape, int grid, double aspect_y, double aspect_z, adda_particle *p)
+{
+	if (!shape || !p || grid <= 0) return host_err(__FILE__, __LINE__, "bad particle arguments");
+	memset(p, 0, sizeof *p);
+	int sph = !strcmp(shape, "sphere"), ell = !strcmp(shape, "ellipsoid"), box = !strcmp(shape, "box");
+	if (!sph && !ell && !box) return host_err(__FILE__, __LINE__, "unsupported shape '%s'", shape);
+	int bx = fit_box(grid), by, bz;
+	if (sph) by = bz = bx;
+	else {
+		by = fit_box(fit_box_yz(aspect_y * bx));
+		bz = fit_box(fit_box_yz(aspect_z * bx));
+	}
+	if (bx > 65535 || by > 65535 || bz > 65535) return host_err(__FILE__, __LINE__, "box too large");
+	size_t cap = (size_t)bx * by * bz;
+	p->position = malloc(3 * cap * sizeof(unsigned short));
+	p->material = malloc(cap * sizeof(int));
+	if (!p->position || !p->material) return host_err(__FILE__, __LINE__, "out of host memory");
+	size_t n = 0;
+	/* src/make_particle.c:1996-2007 with jagged=1: centre (i - box/2 + 0.5)/boxX; SH_SPHERE 2136-2137,
+	 * SH_ELLIPSOID (invsqY/invsqZ, 1635-1637), SH_BOX (haspY/haspZ, 1480-1481) */
+	for (int k = 0; k < bz; k++)
+		for (int j = 0; j < by; j++)
+			for (int i = 0; i < bx; i++) {
+				double xr = (i - bx / 2 + 0.5) / bx, yr = (j - by / 2 + 0.5) / bx, zr = (k - bz / 2 + 0.5) / bx;
+				int in;
+				if (sph) in = xr * xr + yr * yr + zr * zr <= 0.25;
+				else if (ell) in = xr * xr + yr * yr / (aspect_y * aspect_y) + zr * zr / (aspect_z * aspect_z) <= 0.25;
+				else in = fabs(yr) <= aspect_y / 2 && fabs(zr) <= aspect_z / 2;
+				if (!in) continue;
+				p->position[3 * n] = (unsigned short)i;
+				p->position[3 * n + 1] = (unsigned short)j;
+				p->position[3 * n + 2] = (unsigned short)k;
+				p->material[n] = 0;
+				n++;
+			}
+	if (n == 0) return host_err(__FILE__, __LINE__, "All dipoles of the scatterer are void");
+	p->boxX = bx;
+	p->boxY = by;
+	p->boxZ = bz;
+	p->nd = n;
+	p->nmat = 1;
+	return 0;
+}
+
+double adda_volume_ratio(const char *shape, double aspect_y, double aspect_z)
+{
+	if (!strcmp(shape, "sphere")) return PI / 6;                       /* src/make_particle.c:1553 */
+	if (!strcmp(shape, "ellipsoid")) return PI / 6 * aspect_y * aspect_z; /* src/make_particle.c:1637 */
+	if (!strcmp(shape, "box")) return aspect_y * aspect_z;               /* src/make_particle.c:1482 */
+	return -1;
+}
+
+int adda_read_particle(const char *fname, adda_particle *p)
+{
+	if (!fname || !p) return host_err(__FILE__, __LINE__, "null argument");
+	memset(p, 0, sizeof *p);
+	FILE *f = fopen(fname, "r");
+	if (!f) return host_err(__FILE__, __LINE__, "cannot open geometry file '%s'", fname);
+	size_t cap = 1024, n = 0;
+	p->position = malloc(3 * cap * sizeof(unsigned short));
+	p->material = malloc(cap * sizeof(int));
+	char line[512];
+	int maxc[3] = {0, 0, 0}, nmat = 1;
+	while (fgets(line, sizeof line, f)) {
+		if (line[0] == '#' || line[0] == '\n' || line[0] == '\r') continue;
+		int x, y, z, m = 1;
+		int got = sscanf(line, "%d %d %d %d", &x, &y, &z, &m);
+		if (got < 3) continue;
+		if (x < 0 || y < 0 || z < 0 || x > 65534 || y > 65534 || z > 65534 || m < 1) {
+			fclose(f);
+			return host_err(__FILE__, __LINE__, "bad geometry line '%s'", line);
+		}
+		if (n == cap) {
+			cap *= 2;
+			p->position = realloc(p->position, 3 * cap * sizeof(unsigned short));
+			p->material = realloc(p->material, cap * sizeof(int));
+			if (!p->position || !p->material) { fclose(f); return host_err(__FILE__, __LINE__, "out of host memory"); }
+		}
+		p->position[3 * n] = (unsigned short)x;
+		p->position[3 * n + 1] = (unsigned short)y;
+		p->position[3 * n + 2] = (unsigned short)z;
+		p->material[n] = m - 1;
+		if (m > nmat) nmat = m;
+		if (x > maxc[0]) maxc[0] = x;
+		if (y > maxc[1]) maxc[1] = y;
+		if (z > maxc[2]) maxc[2] = z;
+		n++;
+	}
+	fclose(f);
+	if (n == 0) return host_err(__FILE__, __LINE__, "All dipoles of the scatterer are void");
+	/* box = even size covering the dipoles (FitBox of the extent, src/make_particle.c:1194-1203) */
+	p->boxX = fit_box(maxc[0] + 1);
+	p->boxY = fit_box(maxc[1] + 1);
+	p->boxZ = fit_box(maxc[2] + 1);
+	p->nd = n;
+	p->nmat = nmat;
+	return 0;
+}
+
+void adda_free_particle(adda_particle *p)
+{
+	if (!p) return;
+	free(p->position);
+	free(p->material);
+	memset(p, 0, sizeof *p);
+}
+
+/* ---- physics ------------------------------------------------------------------------------------------------- */
+/* polCM, polM, polMplusRR, pol3coef: src/calculator.c:100-133; LDR S-term 214-221 */
+adda_cplx adda_couple_constant(adda_cplx mo, int polrel, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3])
+{
+	dcx m = mo.re + I * mo.im, m2 = m * m, res;
+	double dipvol = gridspace * gridspace * gridspace, kd = wavenum * gridspace;
+	dcx cm = (3 / FOUR_PI) * dipvol * (m2 - 1) / (m2 + 2);
+	if (polrel == ADDA_POL_CM) res = cm;
+	else {
+		dcx M = 0;
+		if (polrel == ADDA_POL_LDR) {
+			double S = 0;
+			for (int i = 0; i < 3; i++) S += prop[i] * prop[i] * incpol[i] * incpol[i];
+			M = (LDR_B1 + (LDR_B2 + LDR_B3 * S) * m2) * kd * kd;
+		}
+		M += I * 2 * kd * kd * kd / 3;
+		res = cm / (1 - (cm / dipvol) * M);
+	}
+	adda_cplx r = {creal(res), cimag(res)};
+	return r;
+}
+
+int adda_generate_b(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3], void *d_einc)
+{
+	(void)p;
+	if (adda_hip_plane_wave(op, wavenum, gridspace, prop, incpol, d_einc)) return host_err_hip(__FILE__, __LINE__);
+	return 0;
+}
+
+/* ---- scattering quantities -------------------------------------------------------------------------------------- */
+/* CalcFieldFree, src/crosssec.c:506-595 (DRAINE scattering relation, free space) */
+int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double *dirs, int ndir, adda_cplx *ebuff)
+{
+	const double kd = wavenum * gridspace;
+	adda_cplx *sums = malloc((size_t)ndir * 3 * sizeof(adda_cplx));
+	if (!sums) return host_err(__FILE__, __LINE__, "out of host memory");
+	if (adda_hip_field_sums(op, ADDA_F64, d_pvec, kd, dirs, ndir, sums)) {
+		free(sums);
+		return host_err_hip(__FILE__, __LINE__);
+	}
+	/* r0 = box_origin_unif = -d*(cX,cY,cZ), src/make_particle.c:2305-2311 */
+	const double r0[3] = {-gridspace * (p->boxX - 1) / 2.0, -gridspace * (p->boxY - 1) / 2.0,
+		-gridspace * (p->boxZ - 1) / 2.0};
+	const double kkk = wavenum * wavenum * wavenum;
+	for (int i = 0; i < ndir; i++) {
+		const double *n = dirs + 3 * i;
+		dcx s[3], tb[3];
+		for (int c = 0; c < 3; c++) s[c] = sums[3 * i + c].re + I * sums[3 * i + c].im;
+		dcx dpr = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
+		for (int c = 0; c < 3; c++) tb[c] = s[c] - dpr * n[c];
+		double ph = -wavenum * (r0[0] * n[0] + r0[1] * n[1] + r0[2] * n[2]);
+		dcx a = cos(ph) + I * sin(ph);
+		dcx tmp = -I * a * kkk;
+		for (int c = 0; c < 3; c++) {
+			dcx e = tmp * tb[c];
+			ebuff[3 * i + c].re = creal(e);
+			ebuff[3 * i + c].im = cimag(e);
+		}
+	}
+	free(sums);
+	return 0;
+}
+
+/* ExtCross, plane wave: src/crosssec.c:811-823 */
+int adda_ext_cross(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double prop[3], const double incpol[3], double *cext)
+{
+	adda_cplx e[3];
+	if (adda_calc_field(op, p, wavenum, gridspace, d_pvec, prop, 1, e)) return -1;
+	double sum = e[0].re * incpol[0] + e[1].re * incpol[1] + e[2].re * incpol[2];
+	*cext = sum * FOUR_PI / (wavenum * wavenum);
+	return 0;
+}
+
+/* AbsCross, SQ_DRAINE: src/crosssec.c:873-884,916 */
+int adda_abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
+	double *cabs)
+{
+	const double temp1 = 2 * wavenum * wavenum * wavenum / 3;
+	double *w = malloc((size_t)p->nmat * 3 * sizeof(double));
+	if (!w) return host_err(__FILE__, __LINE__, "out of host memory");
+	for (int i = 0; i < p->nmat * 3; i++) {
+		dcx c = cc[i].re + I * cc[i].im;
+		w[i] = -cimag(1 / c) - temp1;
+	}
+	double sum;
+	int rc = adda_hip_weighted_norm2(op, ADDA_F64, d_pvec, w, &sum);
+	free(w);
+	if (rc) return host_err_hip(__FILE__, __LINE__);
+	*cabs = FOUR_PI * wavenum * sum;
+	return 0;
+}
+
+/* CalcEplaneYZ, CE_NORMAL, no orientation averaging: src/CalculateE.c:423-507 */
+int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double prop[3], const double par[3], const double per[3], int ntheta, int npoints, adda_cplx *eplane)
+{
+	if (ntheta < 2 || npoints < 1) return host_err(__FILE__, __LINE__, "ntheta must be >= 2");
+	const double dth = PI / (double)(ntheta - 1); /* dtheta_rad, src/calculator.c:671-672 */
+	double *dirs = malloc((size_t)npoints * 3 * sizeof(double));
+	adda_cplx *e = malloc((size_t)npoints * 3 * sizeof(adda_cplx));
+	if (!dirs || !e) { free(dirs); free(e); return host_err(__FILE__, __LINE__, "out of host memory"); }
+	for (int i = 0; i < npoints; i++) {
+		double th = i * dth, co = cos(th), si = sin(th);
+		for (int c = 0; c < 3; c++) dirs[3 * i + c] = co * prop[c] + si * par[c];
+	}
+	int rc = adda_calc_field(op, p, wavenum, gridspace, d_pvec, dirs, npoints, e);
+	if (!rc)
+		for (int i = 0; i < npoints; i++) {
+			double th = i * dth, co = cos(th), si = sin(th), epar[3];
+			for (int c = 0; c < 3; c++) epar[c] = -si * prop[c] + co * par[c];
+			dcx eper = 0, epa = 0;
+			for (int c = 0; c < 3; c++) {
+				dcx ec = e[3 * i + c].re + I * e[3 * i + c].im;
+				eper += ec * per[c];
+				epa += ec * epar[c];
+			}
+			eplane[2 * i].re = creal(eper);
+			eplane[2 * i].im = cimag(eper);
+			eplane[2 * i + 1].re = creal(epa);
+			eplane[2 * i + 1].im = cimag(epa);
+		}
+	free(dirs);
+	free(e);
+	return rc;
+}
+
+static inline double ca2(dcx a) { return creal(a) * creal(a) + cimag(a) * cimag(a); }
+
+/* ComputeMuellerMatrix (src/CalculateE.c:83-116) with the yz-plane amplitudes of MuellerMatrix (236-246) */
+void adda_mueller_yz(const adda_cplx *eX, const adda_cplx *eY, int npoints, double *out)
+{
+	for (int i = 0; i < npoints; i++) {
+		dcx s1 = eX[2 * i].re + I * eX[2 * i].im, s2 = eY[2 * i + 1].re + I * eY[2 * i + 1].im;
+		dcx s3 = eX[2 * i + 1].re + I * eX[2 * i + 1].im, s4 = eY[2 * i].re + I * eY[2 * i].im;
+		double *m = out + 16 * i;
+		m[0] = 0.5 * (ca2(s1) + ca2(s2) + ca2(s3) + ca2(s4));
+		m[1] = 0.5 * (ca2(s2) - ca2(s1) + ca2(s4) - ca2(s3));
+		m[2] = creal(s2 * conj(s3) + s1 * conj(s4));
+		m[3] = cimag(s2 * conj(s3) - s1 * conj(s4));
+		m[4] = 0.5 * (ca2(s2) - ca2(s1) + ca2(s3) - ca2(s4));
+		m[5] = 0.5 * (ca2(s2) + ca2(s1) - ca2(s3) - ca2(s4));
+		m[6] = creal(s2 * conj(s3) - s1 * conj(s4));
+		m[7] = cimag(s2 * conj(s3) + s1 * conj(s4));
+		m[8] = creal(s2 * conj(s4) + s1 * conj(s3));
+		m[9] = creal(s2 * conj(s4) - s1 * conj(s3));
+		m[10] = creal(s1 * conj(s2) + s3 * conj(s4));
+		m[11] = cimag(s2 * conj(s1) + s4 * conj(s3));
+		m[12] = cimag(s4 * conj(s2) + s1 * conj(s3));
+		m[13] = cimag(s4 * conj(s2) - s1 * conj(s3));
+		m[14] = cimag(s1 * conj(s2) - s3 * conj(s4));
+		m[15] = creal(s1 * conj(s2) - s3 * conj(s4));
+	}
+}

@@ -1,0 +1,257 @@
+/* adda_main.c -- command-line front end with ADDA's option names (src/param.c:401-650) for the hot path:
+ *
+ *   adda_hip [-shape sphere | ellipsoid <y/x> <z/x> | box <y/x> <z/x> | read <file>] [-grid <n>] [-size <d>]
+ *            [-dpl <n>] [-lambda <l>] [-m <re> <im> [<re> <im> ...]] [-iter qmr|bicgstab|cgnr|bicg|csym]
+ *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
+ *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
+ *            [-prec fp64|mixed] [-eps_inner <k>]
+ *
+ * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
+ * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
+ * mueller and log in ADDA's formats (GFORM/EFORM, src/const.h:142-146; src/CalculateE.c:236-246,758-759).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "../../../include/adda.h"
+
+#define PI 3.14159265358979323846264338327950288
+
+static double now(void)
+{
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+static void die(const char *what)
+{
+	fprintf(stderr, "ERROR: %s: %s\n", what, adda_last_error()[0] ? adda_last_error() : adda_hip_last_error());
+	exit(EXIT_FAILURE);
+}
+
+static void usage(void)
+{
+	fprintf(stderr, "usage: adda_hip [-shape ...] [-grid n] [-size d] [-dpl n] [-m re im] [-iter name] [-eps k] ...\n");
+	exit(EXIT_FAILURE);
+}
+
+/* InitRotation, zyz Euler angles in degrees: src/crosssec.c:82-138 */
+static void rotate(double a_deg, double b_deg, double g_deg, const double in[3], double out[3])
+{
+	double a = a_deg * PI / 180, b = b_deg * PI / 180, g = g_deg * PI / 180;
+	double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+	double m[3][3] = {{ca * cb * cg - sa * sg, sa * cb * cg + ca * sg, -sb * cg},
+		{-ca * cb * sg - sa * cg, -sa * cb * sg + ca * cg, sb * sg},
+		{ca * sb, sa * sb, cb}};
+	for (int i = 0; i < 3; i++) out[i] = m[i][0] * in[0] + m[i][1] * in[1] + m[i][2] * in[2];
+}
+
+int main(int argc, char **argv)
+{
+	const char *shape = "sphere", *geom = NULL, *dir = "run_adda_hip";
+	double asp_y = 1, asp_z = 1, size = -1, dpl = -1, lambda = 2 * PI, eps_k = 5, eps_inner_k = 0;
+	int grid = 16, maxiter = -1, ntheta = -1, gpu = 0, reduced = 1, have_orient = 0;
+	double orient[3] = {0, 0, 0};
+	adda_cplx m[16];
+	int nm = 0;
+	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1};
+	int polrel = ADDA_POL_LDR;
+	for (int i = 1; i < argc; i++) {
+		const char *a = argv[i];
+#define NEED(n) do { if (i + (n) >= argc) usage(); } while (0)
+		if (!strcmp(a, "-shape")) {
+			NEED(1);
+			shape = argv[++i];
+			if (!strcmp(shape, "ellipsoid") || !strcmp(shape, "box")) {
+				if (i + 2 < argc && argv[i + 1][0] != '-') { asp_y = atof(argv[++i]); asp_z = atof(argv[++i]); }
+			}
+			else if (!strcmp(shape, "read")) { NEED(1); geom = argv[++i]; }
+		}
+		else if (!strcmp(a, "-grid")) { NEED(1); grid = atoi(argv[++i]); }
+		else if (!strcmp(a, "-size")) { NEED(1); size = atof(argv[++i]); }
+		else if (!strcmp(a, "-dpl")) { NEED(1); dpl = atof(argv[++i]); }
+		else if (!strcmp(a, "-lambda")) { NEED(1); lambda = atof(argv[++i]); }
+		else if (!strcmp(a, "-m")) {
+			while (i + 2 < argc && argv[i + 1][0] != '-' && nm < 16) {
+				m[nm].re = atof(argv[++i]);
+				m[nm].im = atof(argv[++i]);
+				nm++;
+			}
+		}
+		else if (!strcmp(a, "-iter")) {
+			NEED(1);
+			const char *s = argv[++i];
+			if (!strcmp(s, "qmr")) opt.method = ADDA_IT_QMR_CS;
+			else if (!strcmp(s, "bicgstab")) opt.method = ADDA_IT_BICGSTAB;
+			else if (!strcmp(s, "cgnr")) opt.method = ADDA_IT_CGNR;
+			else if (!strcmp(s, "bicg")) opt.method = ADDA_IT_BICG_CS;
+			else if (!strcmp(s, "csym")) opt.method = ADDA_IT_CSYM;
+			else { fprintf(stderr, "ERROR: unsupported -iter %s\n", s); return EXIT_FAILURE; }
+		}
+		else if (!strcmp(a, "-eps")) { NEED(1); eps_k = atof(argv[++i]); }
+		else if (!strcmp(a, "-eps_inner")) { NEED(1); eps_inner_k = atof(argv[++i]); }
+		else if (!strcmp(a, "-maxiter")) { NEED(1); maxiter = atoi(argv[++i]); }
+		else if (!strcmp(a, "-init_field")) {
+			NEED(1);
+			const char *s = argv[++i];
+			opt.init_field = !strcmp(s, "zero") ? ADDA_IF_ZERO : !strcmp(s, "inc") ? ADDA_IF_INC : ADDA_IF_AUTO;
+		}
+		else if (!strcmp(a, "-recalc_resid")) opt.recalc_resid = 1;
+		else if (!strcmp(a, "-pol")) {
+			NEED(1);
+			const char *s = argv[++i];
+			polrel = !strcmp(s, "cm") ? ADDA_POL_CM : !strcmp(s, "rrc") ? ADDA_POL_RRC : ADDA_POL_LDR;
+		}
+		else if (!strcmp(a, "-ntheta")) { NEED(1); ntheta = atoi(argv[++i]) + 1; } /* intervals, src/param.c:1285-1290 */
+		else if (!strcmp(a, "-orient")) {
+			NEED(3);
+			for (int c = 0; c < 3; c++) orient[c] = atof(argv[++i]);
+			have_orient = 1;
+		}
+		else if (!strcmp(a, "-no_reduced_fft")) reduced = 0;
+		else if (!strcmp(a, "-gpu")) { NEED(1); gpu = atoi(argv[++i]); }
+		else if (!strcmp(a, "-dir")) { NEED(1); dir = argv[++i]; }
+		else if (!strcmp(a, "-prec")) {
+			NEED(1);
+			opt.precision = !strcmp(argv[++i], "mixed") ? ADDA_PREC_MIXED : ADDA_PREC_FP64;
+		}
+		else { fprintf(stderr, "ERROR: unknown option %s\n", a); usage(); }
+	}
+	if (nm == 0) { m[0].re = 1.5; m[0].im = 0; nm = 1; } /* default refractive index, src/param.c:1852 */
+	opt.eps = pow(10, -eps_k);
+	if (eps_inner_k > 0) opt.inner_eps = pow(10, -eps_inner_k);
+	double t0 = now();
+	adda_particle p;
+	if (geom) { if (adda_read_particle(geom, &p)) die("reading the geometry"); }
+	else if (adda_make_particle(shape, grid, asp_y, asp_z, &p)) die("making the particle");
+	if (p.nmat > nm) { fprintf(stderr, "ERROR: geometry has %d materials, -m gives %d\n", p.nmat, nm); return 1; }
+	/* dpl, gridspace: src/make_particle.c:2185-2205 (volume correction when -size is given) */
+	if (size > 0) {
+		double vr = geom ? -1 : adda_volume_ratio(shape, asp_y, asp_z);
+		dpl = vr > 0 ? lambda * cbrt(p.nd / vr) / size : lambda * p.boxX / size;
+	}
+	else if (dpl <= 0) {
+		double mx = 0;
+		for (int i = 0; i < nm; i++) { double a = hypot(m[i].re, m[i].im); if (a > mx) mx = a; }
+		dpl = 10 * mx; /* dpl_def, src/make_particle.c:1353 */
+	}
+	const double gridspace = lambda / dpl, wavenum = 2 * PI / lambda;
+	const double a_eq = cbrt(3 / (4 * PI) * p.nd) * gridspace, inv_G = 1 / (PI * a_eq * a_eq);
+	if (maxiter <= 0) maxiter = 3 * p.boxX * p.boxY * p.boxZ; /* src/make_particle.c:1939 */
+	opt.maxiter = maxiter;
+	if (ntheta <= 0) { /* src/make_particle.c:1941-1946 */
+		size_t nbox = (size_t)p.boxX * p.boxY * p.boxZ;
+		ntheta = nbox < 1000 ? 91 : nbox < 10000 ? 181 : nbox < 100000 ? 361 : 721;
+	}
+	const int gx = adda_hip_fftfit(2 * p.boxX, 1), gy = adda_hip_fftfit(2 * p.boxY, 1), gz = adda_hip_fftfit(2 * p.boxZ, 1);
+	/* incident frame: prop along z, polarizations Y (par) and X (per), src/param.c:2116-2119; rotated by -orient */
+	double prop0[3] = {0, 0, 1}, py0[3] = {0, 1, 0}, px0[3] = {1, 0, 0}, prop[3], incY[3], incX[3];
+	if (have_orient) {
+		rotate(orient[0], orient[1], orient[2], prop0, prop);
+		rotate(orient[0], orient[1], orient[2], py0, incY);
+		rotate(orient[0], orient[1], orient[2], px0, incX);
+	}
+	else { memcpy(prop, prop0, sizeof prop); memcpy(incY, py0, sizeof incY); memcpy(incX, px0, sizeof incX); }
+	mkdir(dir, 0755);
+	char fn[1024];
+	snprintf(fn, sizeof fn, "%s/log", dir);
+	FILE *log = fopen(fn, "w");
+	if (!log) { fprintf(stderr, "ERROR: cannot write %s\n", fn); return 1; }
+	fprintf(log, "Generated by adda_hip (MI355X backend of ADDA's FFT MatVec + Krylov path)\n");
+	fprintf(log, "command: '");
+	for (int i = 0; i < argc; i++) fprintf(log, "%s%s", argv[i], i + 1 < argc ? " " : "");
+	fprintf(log, "'\nlambda: %.10g\nshape: %s\nbox dimensions: %dx%dx%d\n", lambda, geom ? "read" : shape, p.boxX,
+		p.boxY, p.boxZ);
+	fprintf(log, "refractive index: %.10g%+.10gi\nDipoles/lambda: %.10g\nRequired relative residual norm: %g\n",
+		m[0].re, m[0].im, dpl, opt.eps);
+	fprintf(log, "Total number of occupied dipoles: %zu\nVolume-equivalent size parameter: %.10g\n", p.nd,
+		wavenum * a_eq);
+	fprintf(log, "The FFT grid is: %dx%dx%d\nPrecision: %s\n", gx, gy, gz,
+		opt.precision == ADDA_PREC_MIXED ? "FP32 inner iterations, FP64 defect correction" : "FP64");
+	double tD = now();
+	adda_hip_op *op = adda_hip_op_create(gpu, p.boxX, p.boxY, p.boxZ, gx, gy, gz, p.nd, p.position, p.material, p.nmat,
+		wavenum, gridspace, reduced, opt.precision == ADDA_PREC_MIXED);
+	if (!op) die("creating the operator");
+	tD = now() - tD;
+	void *einc = adda_hip_vec_alloc(op, ADDA_F64), *x = adda_hip_vec_alloc(op, ADDA_F64),
+		*pv = adda_hip_vec_alloc(op, ADDA_F64);
+	if (!einc || !x || !pv) die("allocating vectors");
+	/* symmetries of the predefined shapes (src/make_particle.c:1478,1633; param.c:1903,2138) and the 360-degree
+	 * extension of the yz-plane range when incPolY and -incPolY are not equivalent (TestExtendThetaRange,
+	 * src/CalculateE.c:396-418, calculator.c:674) */
+	int symXY = !geom && !have_orient, symR = symXY && (!strcmp(shape, "sphere") || asp_y == 1),
+		symZ = !geom && !have_orient;
+	int sym_incY = ((symXY || fabs(incY[0]) < 1e-10) && (symXY || fabs(incY[1]) < 1e-10)) || symR;
+	sym_incY = sym_incY && (symZ || fabs(incY[2]) < 1e-10);
+	const int npoints = sym_incY ? ntheta : 2 * (ntheta - 1);
+	adda_cplx *eplX = malloc(2 * npoints * sizeof(adda_cplx)), *eplY = malloc(2 * npoints * sizeof(adda_cplx));
+	adda_cplx *cc = malloc((size_t)p.nmat * 3 * sizeof(adda_cplx)), *ccs = malloc((size_t)p.nmat * 3 * sizeof(adda_cplx));
+	int total_mv = 0, total_iter = 0;
+	double t_solve = 0;
+	for (int pol = 0; pol < 2; pol++) { /* Y then X, src/calculator.c:312-330 */
+		const double *inc = pol == 0 ? incY : incX;
+		const char *nm_ = pol == 0 ? "Y" : "X";
+		for (int i = 0; i < p.nmat; i++) { /* InitCC, src/calculator.c:266-289 */
+			adda_cplx c = adda_couple_constant(m[i], polrel, wavenum, gridspace, prop, inc);
+			double r = hypot(c.re, c.im), ph = atan2(c.im, c.re);
+			for (int j = 0; j < 3; j++) {
+				cc[3 * i + j] = c;
+				ccs[3 * i + j].re = sqrt(r) * cos(ph / 2);
+				ccs[3 * i + j].im = sqrt(r) * sin(ph / 2);
+			}
+		}
+		fprintf(log, "\nhere we go, calc %s\n\nCoupleConstant:%.10g%+.10gi\n", nm_, cc[0].re, cc[0].im);
+		if (adda_hip_set_cc_sqrt(op, ccs)) die("uploading cc_sqrt");
+		if (adda_generate_b(op, &p, wavenum, gridspace, prop, inc, einc)) die("GenerateB");
+		double hist[4096];
+		adda_solve_stats st;
+		double ts = now();
+		int rc = adda_iterative_solver(op, &opt, einc, x, pv, hist, 4096, &st);
+		ts = now() - ts;
+		t_solve += ts;
+		if (rc < 0) die("IterativeSolver");
+		if (rc == 1) fprintf(stderr, "WARNING: iterations haven't converged in %d iterations\n", maxiter);
+		for (int i = 0; i <= st.niter && i < 4096; i++) fprintf(log, "RE_%03d = %.10E\n", i, hist[i]);
+		total_mv += st.nmatvec + st.nmatvec_fp64;
+		total_iter += st.niter;
+		double cext, cabs;
+		if (adda_ext_cross(op, &p, wavenum, gridspace, pv, prop, inc, &cext)) die("ExtCross");
+		if (adda_abs_cross(op, &p, wavenum, cc, pv, &cabs)) die("AbsCross");
+		if (adda_eplane_yz(op, &p, wavenum, gridspace, pv, prop, incY, incX, ntheta, npoints, pol == 0 ? eplY : eplX))
+			die("CalcEplaneYZ");
+		snprintf(fn, sizeof fn, "%s/CrossSec-%s", dir, nm_);
+		FILE *f = fopen(fn, "w");
+		if (!f) die("writing CrossSec");
+		fprintf(f, "Cext\t= %.10g\nQext\t= %.10g\nCabs\t= %.10g\nQabs\t= %.10g\n", cext, cext * inv_G, cabs, cabs * inv_G);
+		fclose(f);
+		printf("%s: iterations %d, matvecs %d (+%d FP64), residual %.3E, Cext = %.10g, Cabs = %.10g, %.3f s\n", nm_,
+			st.niter, st.nmatvec, st.nmatvec_fp64, st.resid, cext, cabs, ts);
+	}
+	double *mu = malloc((size_t)npoints * 16 * sizeof(double));
+	adda_mueller_yz(eplX, eplY, npoints, mu);
+	snprintf(fn, sizeof fn, "%s/mueller", dir);
+	FILE *f = fopen(fn, "w");
+	if (!f) die("writing mueller");
+	fprintf(f, "theta s11 s12 s13 s14 s21 s22 s23 s24 s31 s32 s33 s34 s41 s42 s43 s44\n");
+	for (int i = 0; i < npoints; i++) {
+		fprintf(f, "%.2f", i * 180.0 / (ntheta - 1));
+		for (int j = 0; j < 16; j++) fprintf(f, " %.10E", mu[16 * i + j]);
+		fprintf(f, "\n");
+	}
+	fclose(f);
+	fprintf(log, "\nTotal number of iterations: %d\nTotal number of matrix-vector products: %d\n", total_iter, total_mv);
+	fprintf(log, "Total wall time: %.4f\n  init Dmatrix: %.4f\n  solves: %.4f\n", now() - t0, tD, t_solve);
+	fclose(log);
+	adda_hip_vec_free(op, einc);
+	adda_hip_vec_free(op, x);
+	adda_hip_vec_free(op, pv);
+	adda_hip_op_destroy(op);
+	adda_free_particle(&p);
+	free(eplX); free(eplY); free(cc); free(ccs); free(mu);
+	return 0;
+}

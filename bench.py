@@ -1,0 +1,168 @@
+"""Benchmark of the ADDA FFT-matvec + Krylov hot path on one MI355X.
+
+Workload (the grid-256 configuration): homogeneous sphere on a 128^3 box (1,099,136 dipoles, padded FFT grid 256^3,
+3 complex components), refractive index 1.5+0.01i, size parameter ~40 (-size 80 with lambda = 2 pi), LDR
+polarizability, plane wave along z, Y polarization. Synthetic by construction (the particle is generated, nothing is
+read).
+
+One step = one complete time-to-solution run of the FP32-mixed solver (FP32 inner QMR_CS, FP64 defect correction to
+||r||/||b|| <= 1e-10, then Cext), inputs (incident field, D-matrix) already resident in HBM. The plain FP64 QMR_CS
+solve is timed once beside it, as are BiCGStab in both forms, the matvec throughput of both precisions, and the
+per-pass kernel times of the matvec (HIP events on the operator's stream) that give the roofline figures.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--box 128]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "a-dda_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "time-to-solution (s) & matvecs/s, grid 256, FP32-mixed vs FP64, Cext rel err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ["P1 scatter+FFT-x", "P2 FFT-z", "P3 FFT-y.D.FFT-y^-1", "P4 inverse FFT-z", "P5 inverse FFT-x+gather"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--box", type=int, default=128)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bicgstab", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import adda_amd as A
+    if A.device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+
+    lam = 2 * np.pi
+    pos, mat, box = A.make_particle("sphere", args.box)
+    nd = len(pos)
+    size = 80.0 * args.box / 128
+    dpl = lam * (nd / (np.pi / 6)) ** (1 / 3) / size  # -size with volume correction, src/make_particle.c:2192-2196
+    m = 1.5 + 0.01j
+    t0 = time.perf_counter()
+    pr = A.Problem(pos, box, m, dpl=dpl, fp32=True)
+    t_dmat = time.perf_counter() - t0
+    log(f"particle: {nd} dipoles, box {box}, grid {pr.grid}, dpl {dpl:.4f}, x = {2*np.pi/lam*(3*nd/(4*np.pi))**(1/3)*lam/dpl:.3f}; "
+        f"operator + FP64 D-matrix build {t_dmat:.3f} s")
+    prop, pol = np.array([0.0, 0, 1]), np.array([0.0, 1, 0])
+    pr.set_polarization(prop, pol)
+    op = pr.op
+
+    # ---- matvec throughput and per-pass timing (HIP events on the operator's stream) --------------------------------
+    mv = {}
+    for prec, name in ((A.F64, "fp64"), (A.F32, "fp32")):
+        a, r = op.alloc(prec), op.alloc(prec)
+        A.check(A.hip().adda_hip_vec_convert(op.h, a, prec, pr.einc, A.F64))
+        op.profile_matvec(a, r, prec, nrep=3)
+        passes, tot = op.profile_matvec(a, r, prec, nrep=20)
+        op.free(a)
+        op.free(r)
+        bytes_pass = [op.matvec_bytes(prec, i) for i in range(5)]
+        mv[name] = dict(ms=tot, passes=passes, bytes=bytes_pass, total_bytes=op.matvec_bytes(prec, 5))
+        log(f"matvec {name}: {tot*1e3:.1f} us ({1e3/tot:.1f} matvecs/s), "
+            f"{mv[name]['total_bytes']/tot/1e6:.0f} GB/s algorithmic; passes (us): "
+            + ", ".join(f"{p*1e3:.1f}" for p in passes))
+
+    # ---- plain FP64 QMR_CS (reference solve) ------------------------------------------------------------------------
+    def run(method, precision):
+        op.sync()
+        t = time.perf_counter()
+        res = pr.solve(method, eps=1e-10, precision=precision)
+        cext = pr.cext()
+        dt = time.perf_counter() - t
+        return dt, cext, res
+
+    t64, cext64, r64 = run("qmr", A.PREC_FP64)
+    log(f"FP64 QMR_CS: {t64:.3f} s, {r64['niter']} iterations, {r64['nmatvec']} matvecs, Cext {cext64:.10g}")
+
+    # ---- timed steps: FP32-mixed QMR_CS time-to-solution ------------------------------------------------------------
+    for _ in range(args.warmup):
+        run("qmr", A.PREC_MIXED)
+    op.sync()
+    tstart = time.perf_counter()
+    mixed = []
+    for _ in range(args.steps):
+        mixed.append(run("qmr", A.PREC_MIXED))
+    op.sync()
+    elapsed = time.perf_counter() - tstart
+    tmix = elapsed / args.steps
+    _, cextmix, rmix = mixed[-1]
+    err = abs(cextmix - cext64) / abs(cext64)
+    log(f"mixed QMR_CS: {tmix:.3f} s/solve, {rmix['niter']} FP32 iterations, {rmix['nmatvec']} FP32 + "
+        f"{rmix['nmatvec_fp64']} FP64 matvecs, {rmix['nouter']} outer steps, true residual {rmix['true_resid']:.2e}, "
+        f"Cext {cextmix:.10g} (rel err vs FP64 {err:.2e}), speedup {t64/tmix:.2f}x")
+
+    extra = {}
+    if not args.no_bicgstab:
+        tb64, cb64, rb64 = run("bicgstab", A.PREC_FP64)
+        tbm, cbm, rbm = run("bicgstab", A.PREC_MIXED)
+        extra["bicgstab"] = dict(tts_fp64_s=round(tb64, 4), tts_mixed_s=round(tbm, 4), speedup=round(tb64 / tbm, 3),
+                                 iterations_fp64=rb64["niter"], iterations_mixed=rbm["niter"],
+                                 cext_rel_err=abs(cbm - cb64) / abs(cb64), cext_vs_qmr_fp64=abs(cb64 - cext64) / cext64)
+        log(f"BiCGStab: FP64 {tb64:.3f} s ({rb64['niter']} it), mixed {tbm:.3f} s ({rbm['niter']} it), "
+            f"Cext rel err {extra['bicgstab']['cext_rel_err']:.2e}")
+
+    # ---- roofline: dominant kernel of the timed workload (FP32 matvec pass P3) --------------------------------------
+    dom = mv["fp32"]
+    k = int(np.argmax(dom["passes"]))
+    achieved = dom["bytes"][k] / (dom["passes"][k] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": PASS_NAMES[k] + " (fp32)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    matvec_roof = {name: {"GBps": round(v["total_bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
+                          "frac": round(v["total_bytes"] / (v["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "bytes_per_matvec": v["total_bytes"], "us_per_matvec": round(v["ms"] * 1e3, 2),
+                          "pass_us": [round(p * 1e3, 2) for p in v["passes"]]} for name, v in mv.items()}
+
+    # ---- CPU baseline: the oracle's FP64 FFT matvec on one host core, bounded sample of this workload ---------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        from cases import Case
+        oc = Case(pos, box, m, dpl=dpl)
+        dmat = op.dmatrix()  # the oracle's matvec needs only D; building it on the CPU would add minutes
+        x = op.download(pr.einc)
+        t = time.perf_counter()
+        oc.matvec_fft(dmat, pr.cc_sqrt, x)
+        tc = time.perf_counter() - t
+        cpu = {"value": round(1.0 / tc, 5), "unit": "matvecs/s", "cores": 1, "kind": "port",
+               "sample": f"one FP64 FFT matvec of this workload (grid {pr.grid[0]}^3) by the oracle (textbook FFT, "
+                         f"1 thread, {tc:.1f} s); full FP64 QMR solve extrapolates to {tc * r64['nmatvec']:.0f} s"}
+        log(f"CPU baseline: {tc:.2f} s per matvec")
+    pr.close()
+
+    out = {
+        "metric": METRIC, "value": round(tmix, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(tmix * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 inner (fp32 complex) + f64 outer", "data": "synthetic",
+        "config": {"workload": f"sphere {box[0]}^3 box ({nd} dipoles), FFT grid {pr.grid[0]}x{pr.grid[1]}x{pr.grid[2]}, "
+                               f"m=1.5+0.01i, size parameter ~40, LDR, QMR_CS to 1e-10 (FP32-mixed); plain FP64 timed beside",
+                   "dipoles": nd, "grid": list(pr.grid), "eps": 1e-10, "solver": "qmr"},
+        "tts_fp64_s": round(t64, 4), "tts_mixed_s": round(tmix, 4), "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
+        "cext_fp64": cext64, "cext_mixed": cextmix, "cext_rel_err": err,
+        "iterations": {"fp64": r64["niter"], "mixed_fp32": rmix["niter"], "mixed_outer": rmix["nouter"]},
+        "matvecs_per_s": {"fp64": round(1e3 / mv["fp64"]["ms"], 1), "fp32": round(1e3 / mv["fp32"]["ms"], 1)},
+        "matvec_roofline": matvec_roof, "dmatrix_build_s": round(t_dmat, 3),
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

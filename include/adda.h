@@ -91,11 +91,13 @@ int adda_ext_cross(adda_hip_op *op, const adda_particle *p, double wavenum, doub
 /* AbsCross, DRAINE (src/crosssec.c:852-884); cc: [nmat][3]. */
 int adda_abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
 	double *cabs);
-/* CalcEplaneYZ for one solution (src/CalculateE.c:423-507): eplane[2i] = E.per, eplane[2i+1] = E.par. */
+/* CalcEplaneYZ for one solution (src/CalculateE.c:423-507): eplane[2i] = E.per, eplane[2i+1] = E.par at
+ * theta_i = i*180/(ntheta-1), i < npoints (npoints = 2*(ntheta-1) for the 360-degree range of an asymmetric
+ * scatterer, src/calculator.c:671-674). */
 int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
-	const double prop[3], const double par[3], const double per[3], int ntheta, adda_cplx *eplane);
-/* ComputeMuellerMatrix over the yz plane (src/CalculateE.c:83-116,236-246); out[16*ntheta]. */
-void adda_mueller_yz(const adda_cplx *eplX, const adda_cplx *eplY, int ntheta, double *out);
+	const double prop[3], const double par[3], const double per[3], int ntheta, int npoints, adda_cplx *eplane);
+/* ComputeMuellerMatrix over the yz plane (src/CalculateE.c:83-116,236-246); out[16*npoints]. */
+void adda_mueller_yz(const adda_cplx *eplX, const adda_cplx *eplY, int npoints, double *out);
 
 #ifdef __cplusplus
 }

@@ -109,6 +109,17 @@ int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, 
 /* sum_j sum_c w[mat_j][c] |v[3j+c]|^2 (the AbsCross sum, src/crosssec.c:873-884); w is [nmat][3]. */
 int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v, const double *w, double *out);
 
+/* Measurement: runs nrep matvecs and times each of the 5 passes with HIP events on the operator's stream;
+ * pass_ms[5] receives the average duration of P1 (scatter + FFT-x), P2 (FFT-z), P3 (FFT-y, D-matrix product,
+ * inverse FFT-y), P4 (inverse FFT-z), P5 (inverse FFT-x + gather); total_ms the average per matvec.
+ * (The role of ADDA's PRECISE_TIMING breakdown, src/matvec.c:147-422.) */
+int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, int nrep, double *pass_ms,
+	double *total_ms);
+
+/* Algorithmic HBM bytes of one matvec pass under the streaming model (each pass reads and writes its operand once,
+ * zero half-planes skipped, D-matrix read once): pass 0..4 as above, or 5 for the whole matvec. */
+double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pass);
+
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 

@@ -674,10 +674,10 @@ double or_abs_cross(const or_grid *g, const or_cplx *pvec, const or_cplx *cc)
 
 /* CalcEplaneYZ, CE_NORMAL, no orientation averaging: src/CalculateE.c:423-507 */
 void or_eplane_yz(const or_grid *g, const or_cplx *pvec, const double prop[3], const double par[3],
-	const double per[3], int ntheta, or_cplx *eplane)
+	const double per[3], int ntheta, int npoints, or_cplx *eplane)
 {
 	double dth = TWO_PI / 2 / (double)(ntheta - 1); /* dtheta_rad, src/calculator.c:671-672 */
-	for (int i = 0; i < ntheta; i++) {
+	for (int i = 0; i < npoints; i++) {
 		double th = i * dth, co = cos(th), si = sin(th), robs[3], epar[3];
 		for (int c = 0; c < 3; c++) { robs[c] = co * prop[c] + si * par[c]; epar[c] = -si * prop[c] + co * par[c]; }
 		or_cplx e[3];

@@ -87,13 +87,14 @@ double or_ext_cross(const or_grid *g, const or_cplx *pvec, const double prop[3],
 double or_abs_cross(const or_grid *g, const or_cplx *pvec, const or_cplx *cc);
 
 /* CalcEplaneYZ (src/CalculateE.c:423-507), one polarization, CE_NORMAL: eplane[2*i] = E.incPolper,
- * eplane[2*i+1] = E.epar for theta_i = i*180/(ntheta-1). */
+ * eplane[2*i+1] = E.epar for theta_i = i*180/(ntheta-1), i < npoints (npoints = 2*(ntheta-1) when the range is
+ * extended to 360 deg, src/calculator.c:674). */
 void or_eplane_yz(const or_grid *g, const or_cplx *pvec, const double prop[3], const double incpol_par[3],
-	const double incpol_per[3], int ntheta, or_cplx *eplane);
+	const double incpol_per[3], int ntheta, int npoints, or_cplx *eplane);
 
 /* ComputeMuellerMatrix (src/CalculateE.c:83-116) applied as in MuellerMatrix for the yz plane
  * (src/CalculateE.c:236-246): out[16*i..] row-major 4x4 per angle. */
-void or_mueller_yz(const or_cplx *eplX, const or_cplx *eplY, int ntheta, double *out);
+void or_mueller_yz(const or_cplx *eplX, const or_cplx *eplY, int npoints, double *out);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,16 @@ def make_shape(shape, box, aspect_y=1.0, aspect_z=1.0):
     return np.ascontiguousarray(pos), (bx, by, bz)
 
 
+def read_shape(pos):
+    """What ADDA makes of a '-shape read' file holding these dipoles (InitDipFile/ReadDipFile,
+    src/make_particle.c:1150-1260): positions shifted to start at 0, box = FitBox(max-min+1) per axis."""
+    pos = np.asarray(pos, np.int64)
+    mn = pos.min(axis=0)
+    p = (pos - mn).astype(np.uint16)
+    box = tuple(fit_box(int(e)) for e in (p.max(axis=0) + 1))
+    return np.ascontiguousarray(p), box
+
+
 _lib = None
 
 
@@ -91,7 +101,7 @@ def oracle():
         lib.or_ext_cross.restype = ctypes.c_double
         lib.or_abs_cross.argtypes = [P, P, P]
         lib.or_abs_cross.restype = ctypes.c_double
-        lib.or_eplane_yz.argtypes = [P, P, P, P, P, ctypes.c_int, P]
+        lib.or_eplane_yz.argtypes = [P, P, P, P, P, ctypes.c_int, ctypes.c_int, P]
         lib.or_mueller_yz.argtypes = [P, P, ctypes.c_int, P]
         _lib = lib
     return _lib
@@ -187,16 +197,17 @@ class Case:
     def cabs(self, p, cc):
         return oracle().or_abs_cross(self.gp, ptr(p), ptr(np.ascontiguousarray(cc)))
 
-    def eplane(self, p, prop, par, per, ntheta):
-        e = np.empty(2 * ntheta, np.complex128)
+    def eplane(self, p, prop, par, per, ntheta, npoints=None):
+        npoints = ntheta if npoints is None else npoints
+        e = np.empty(2 * npoints, np.complex128)
         oracle().or_eplane_yz(self.gp, ptr(p), ptr(np.ascontiguousarray(prop, np.float64)),
                               ptr(np.ascontiguousarray(par, np.float64)), ptr(np.ascontiguousarray(per, np.float64)),
-                              ntheta, ptr(e))
+                              ntheta, npoints, ptr(e))
         return e
 
-    def mueller(self, eX, eY, ntheta):
-        out = np.empty((ntheta, 16))
-        oracle().or_mueller_yz(ptr(eX), ptr(eY), ntheta, ptr(out))
+    def mueller(self, eX, eY, npoints):
+        out = np.empty((npoints, 16))
+        oracle().or_mueller_yz(ptr(eX), ptr(eY), npoints, ptr(out))
         return out
 
 

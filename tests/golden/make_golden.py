@@ -1,0 +1,110 @@
+"""Regenerates the golden fixtures under tests/golden/ with ADDA's own SPARSE-mode executable.
+
+The executable is built from the reference sources by oracle/Makefile.ref into oracle/_ref/adda_sparse (direct-sum
+MatVec, same Green's tensor, polarizability, Krylov solvers and post-processing as the FFT path). This script runs it
+in this container only; the fixtures it writes are plain text data (geometry, CrossSec-*, mueller, DipPol-*, log
+residual lines) and are committed with the exact command line of every run in cmd.txt.
+
+    python tests/golden/make_golden.py [name ...]
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from cases import make_shape, read_shape  # noqa: E402
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "adda_sparse")
+
+
+def write_geom(path, pos, mat=None):
+    with open(path, "w") as f:
+        if mat is None:
+            for p in pos:
+                f.write("%d %d %d\n" % tuple(p))
+        else:
+            f.write("Nmat=%d\n" % (int(mat.max()) + 1))
+            for p, m in zip(pos, mat):
+                f.write("%d %d %d %d\n" % (p[0], p[1], p[2], m + 1))
+
+
+def geometries():
+    g = {}
+    pos, _ = make_shape("sphere", 16)
+    g["sphere16"] = (pos, None)
+    pos, _ = make_shape("ellipsoid", 12, 0.8, 0.5)
+    g["ell12"] = (pos, None)
+    pos, _ = make_shape("sphere", 8)
+    g["sphere8"] = (pos, None)
+    pos, _ = make_shape("box", 8, 0.75, 0.5)
+    rng = np.random.default_rng(20261203)
+    g["box8mm"] = (pos, rng.integers(0, 3, len(pos)))
+    return g
+
+
+RUNS = {
+    # name: (geometry, args, files to keep)
+    "c1_qmr": ("sphere16", "-dpl 15 -m 1.5 0 -iter qmr -eps 10", ["CrossSec-Y", "CrossSec-X", "mueller"]),
+    "c1_qmr_e5": ("sphere16", "-dpl 15 -m 1.5 0 -iter qmr -eps 5", ["CrossSec-Y", "log"]),
+    "c1_bicgstab": ("sphere16", "-dpl 15 -m 1.5 0 -iter bicgstab -eps 10", ["CrossSec-Y", "CrossSec-X"]),
+    "c1_cgnr": ("sphere16", "-dpl 15 -m 1.5 0 -iter cgnr -eps 10", ["CrossSec-Y", "CrossSec-X"]),
+    "c1_bicg": ("sphere16", "-dpl 15 -m 1.5 0 -iter bicg -eps 10", ["CrossSec-Y", "CrossSec-X"]),
+    "c1_csym": ("sphere16", "-dpl 15 -m 1.5 0 -iter csym -eps 10", ["CrossSec-Y", "CrossSec-X"]),
+    "ell12_orient": ("ell12", "-dpl 12 -m 1.33 0.05 -orient 30 40 50 -iter qmr -eps 10 -ntheta 19 -store_dip_pol",
+                     ["CrossSec-Y", "CrossSec-X", "mueller", "DipPol-Y", "DipPol-X"]),
+    "sphere8_cm": ("sphere8", "-dpl 10 -m 1.1 0.1 -pol cm -iter qmr -eps 10 -ntheta 19 -store_dip_pol",
+                   ["CrossSec-Y", "CrossSec-X", "mueller", "DipPol-Y"]),
+    "box8_multimat": ("box8mm", "-dpl 14 -m 1.2 0.01 1.5 0.1 1.8 0 -iter bicgstab -eps 10 -ntheta 19 -store_dip_pol",
+                      ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
+}
+
+
+def run(name):
+    geo, args, keep = RUNS[name]
+    out = os.path.join(HERE, name)
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        gfile = os.path.join(HERE, "geom", geo + ".geom")
+        cmd = [EXE, "-shape", "read", gfile] + args.split() + ["-dir", os.path.join(tmp, "run")]
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, cwd=tmp)
+        os.makedirs(out, exist_ok=True)
+        for f in keep:
+            src = os.path.join(tmp, "run", f)
+            if f == "log":  # keep the residual history and solver counters only
+                with open(src) as fi, open(os.path.join(out, "log_RE"), "w") as fo:
+                    for line in fi:
+                        if line.startswith("RE_") or line.startswith("CoupleConstant") or "matrix-vector" in line:
+                            fo.write(line)
+            else:
+                shutil.copy(src, os.path.join(out, f))
+        rel = " ".join(os.path.relpath(c, ROOT) if c.startswith(ROOT) else os.path.basename(c) if c.startswith(tmp)
+                       else c for c in cmd)
+        with open(os.path.join(out, "cmd.txt"), "w") as f:
+            f.write("oracle/_ref/adda_sparse -shape read tests/golden/geom/%s.geom %s -dir run\n" % (geo, args))
+        return name
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    if not os.path.exists(EXE):
+        sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    os.makedirs(os.path.join(HERE, "geom"), exist_ok=True)
+    for gname, (pos, mat) in geometries().items():
+        p, _ = read_shape(pos)
+        write_geom(os.path.join(HERE, "geom", gname + ".geom"), p, mat)
+    names = sys.argv[1:] or list(RUNS)
+    with ThreadPoolExecutor(max_workers=6) as ex:
+        for n in ex.map(run, names):
+            print("done", n, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+"""GPU parity of the Krylov solvers, the mixed-precision form and the scattering quantities against ADDA's own
+SPARSE-build outputs (tests/golden/) and the oracle.
+
+Bars: FP64 solve converged to 1e-10 -> Cext/Cabs within the reference's CrossSec tolerance (abs 1e-14 / rel 1e-8),
+Mueller within abs 1e-9 / rel 1e-5, polarizations P within abs 1e-12 / rel 1e-6 (comp2exec classes), internal fields
+x within 1e-8 relative (max-norm) of the oracle's FP64 solve. Mixed precision (FP32 inner, FP64 defect correction):
+Cext within 1e-6 relative of the FP64 value.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_io as G
+from adda_rot import rotate_zyz
+from cases import Case, read_shape, PROP_Z, POL_X, POL_Y
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [("qmr", "c1_qmr", 0), ("bicgstab", "c1_bicgstab", 1), ("cgnr", "c1_cgnr", 2), ("bicg", "c1_bicg", 3),
+           ("csym", "c1_csym", 4)]
+
+
+def _problem(adda, geo, m, dpl, mat=None, polrel=0, fp32=False):
+    pos, gmat = G.geometry(geo)
+    p, box = read_shape(pos)
+    return adda.Problem(p, box, m, mat=gmat if mat is None else mat, dpl=dpl, polrel=polrel, fp32=fp32)
+
+
+@pytest.mark.parametrize("method,gold,mi", METHODS)
+def test_c1_solvers(hipdev, method, gold, mi):
+    pr = _problem(hipdev, "sphere16", 1.5, 15)
+    try:
+        for pol, vec in (("Y", POL_Y), ("X", POL_X)):
+            pr.set_polarization(PROP_Z, vec)
+            r = pr.solve(method, eps=1e-10)
+            assert r["rc"] == 0
+            g = G.crosssec(gold, pol)
+            assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0], (pr.cext(), g["Cext"])
+            assert abs(pr.cabs()) < 1e-12
+    finally:
+        pr.close()
+
+
+def test_c1_history_and_fields_vs_oracle(hipdev):
+    """QMR at eps 1e-5: the residual history follows the sample's RE_ lines; at 1e-10 the field x matches the oracle's
+    FP64 solve to 1e-8 and the Mueller matrix matches the reference's 360-degree yz-plane output."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15)
+    pos, _ = G.geometry("sphere16")
+    p, box = read_shape(pos)
+    oc = Case(p, box, 1.5, dpl=15)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        r = pr.solve("qmr", eps=1e-5)
+        assert r["niter"] == 22
+        assert np.allclose(r["hist"][:19], G.log_re("c1_qmr_e5")[:19], rtol=1e-5)
+        assert abs(pr.cext() - 135.0449047) / 135.0449047 < 1e-8
+        epl = {}
+        for pol, vec in (("Y", POL_Y), ("X", POL_X)):
+            pr.set_polarization(PROP_Z, vec)
+            r = pr.solve("qmr", eps=1e-10)
+            x = pr.field_x()
+            cc, ccs = oc.cc(PROP_Z, vec)
+            ro = oc.solve(oc.dmatrix(), ccs, oc.einc(PROP_Z, vec), method=0, eps=1e-10)
+            assert np.abs(x - ro["x"]).max() / np.abs(ro["x"]).max() < 1e-8
+            epl[pol] = pr.eplane(POL_Y, POL_X, 181, 360)
+        mu = hipdev.mueller_yz(epl["X"], epl["Y"], 360)
+        ok, bad = G.numeric_close(mu, G.mueller("c1_qmr")[:, 1:], 1e-9, 1e-5)
+        assert ok, bad[:5]
+    finally:
+        pr.close()
+
+
+def test_orientation_absorbing_ellipsoid(hipdev):
+    pr = _problem(hipdev, "ell12", 1.33 + 0.05j, 12)
+    prop, pY, pX = (rotate_zyz(30, 40, 50, v) for v in (PROP_Z, POL_Y, POL_X))
+    try:
+        epl = {}
+        for pol, vec in (("Y", pY), ("X", pX)):
+            pr.set_polarization(prop, vec)
+            assert pr.solve("qmr", eps=1e-10)["rc"] == 0
+            g = G.crosssec("ell12_orient", pol)
+            assert G.numeric_close([pr.cext(), pr.cabs()], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0]
+            P, gp = pr.polarization(), G.dippol("ell12_orient", pol)
+            assert G.numeric_close(np.c_[P.real, P.imag], np.c_[gp.real, gp.imag], 1e-12, 1e-6)[0]
+            epl[pol] = pr.eplane(pY, pX, 20, 38)
+        mu = hipdev.mueller_yz(epl["X"], epl["Y"], 38)
+        assert G.numeric_close(mu, G.mueller("ell12_orient")[:, 1:], 1e-9, 1e-5)[0]
+    finally:
+        pr.close()
+
+
+def test_cm_and_multimaterial(hipdev):
+    pr = _problem(hipdev, "sphere8", 1.1 + 0.1j, 10, polrel=hipdev.POL_CM)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        pr.solve("qmr", eps=1e-10)
+        g = G.crosssec("sphere8_cm", "Y")
+        assert G.numeric_close([pr.cext(), pr.cabs()], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0]
+    finally:
+        pr.close()
+    pr = _problem(hipdev, "box8mm", [1.2 + 0.01j, 1.5 + 0.1j, 1.8], 14)
+    try:
+        for pol, vec in (("Y", POL_Y), ("X", POL_X)):
+            pr.set_polarization(PROP_Z, vec)
+            pr.solve("bicgstab", eps=1e-10)
+            g = G.crosssec("box8_multimat", pol)
+            assert G.numeric_close([pr.cext(), pr.cabs()], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0]
+        gp = G.dippol("box8_multimat", "Y")
+        pr.set_polarization(PROP_Z, POL_Y)
+        pr.solve("bicgstab", eps=1e-10)
+        P = pr.polarization()
+        assert G.numeric_close(np.c_[P.real, P.imag], np.c_[gp.real, gp.imag], 1e-12, 1e-6)[0]
+    finally:
+        pr.close()
+
+
+@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr"])
+def test_mixed_precision_c1(hipdev, method):
+    """FP32 inner Krylov + FP64 defect correction reaches the FP64 answer: true residual <= 1e-10 and Cext within 1e-6
+    (the reference value 135.0448603 is itself converged to 1e-10)."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED)
+        assert r["true_resid"] <= 1e-10, r
+        assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6
+        assert r["nmatvec_fp64"] >= 1 and r["nouter"] >= 2
+    finally:
+        pr.close()
+
+
+def test_recalc_resid_and_init_fields(hipdev):
+    pr = _problem(hipdev, "sphere8", 1.1 + 0.1j, 10)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        cext = None
+        for init in (hipdev.IF_AUTO, hipdev.IF_ZERO, hipdev.IF_INC):
+            r = pr.solve("qmr", eps=1e-10, init_field=init, recalc_resid=True)
+            assert r["true_resid"] < 2e-10
+            c = pr.cext()
+            cext = c if cext is None else cext
+            assert abs(c - cext) < 1e-8 * abs(cext)
+    finally:
+        pr.close()
+
+
+def test_maxiter_warning(hipdev):
+    """-maxiter 5 stops with the warning code (src/iterative.c:1506-1508), not an error."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        r = pr.solve("qmr", eps=1e-10, maxiter=5)
+        assert r["rc"] == 1 and r["niter"] == 5
+    finally:
+        pr.close()
+
+
+def test_cli_end_to_end(hipdev, tmp_path):
+    """adda_hip -shape read ... reproduces the reference's CrossSec-Y/X and mueller files (comp2exec tolerances)."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "a-dda_amd", "adda_hip")
+    cmd = [exe, "-shape", "read", G.path("geom", "sphere16.geom"), "-dpl", "15", "-m", "1.5", "0", "-iter", "qmr",
+           "-eps", "10", "-dir", str(tmp_path / "run")]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    for pol in ("Y", "X"):
+        a = G.crosssec("c1_qmr", pol)
+        with open(tmp_path / "run" / ("CrossSec-" + pol)) as f:
+            b = {l.split("=")[0].strip(): float(l.split("=")[1]) for l in f}
+        assert G.numeric_close(b["Cext"], a["Cext"], 1e-14, 1e-8)[0]
+        assert G.numeric_close(b["Qext"], a["Qext"], 1e-14, 1e-8)[0]
+    mu = np.loadtxt(tmp_path / "run" / "mueller", skiprows=1)
+    ref = G.mueller("c1_qmr")
+    assert mu.shape == ref.shape
+    assert G.numeric_close(mu, ref, 1e-9, 1e-5)[0]

@@ -2,16 +2,17 @@
 // linalg vector kernels (src/linalg.c), exported through the C ABI of include/adda_hip.h.
 //
 // Matvec pipeline (5 HBM passes, half-planes that are identically zero are never stored or read):
-//   P1 k_scatter_fftx  : gather S.x from dipoles onto x-lines (y<boxY, z<boxZ), zero-pad to gridX, forward FFT-x
-//                        -> Xa[c][z<boxZ][y<boxY][x<gridX]
-//   P2 k_colfft<fwd>   : forward FFT-z of the boxZ nonzero rows -> Xb[c][z<gridZ][y<boxY][x]
-//   P3 k_ydmul         : forward FFT-y of the boxY nonzero rows, product with the 6-component symmetric D-matrix,
-//                        inverse FFT-y, keep y<boxY (in place in Xb)
-//   P4 k_colfft<inv>   : inverse FFT-z, keep z<boxZ -> Xa
-//   P5 k_fftx_gather   : inverse FFT-x, result_i = arg_i + S_i.X(pos_i), per-block partial ||result||^2
-// Strided axes (y, z) are processed as tiles of TX adjacent x-columns so every global access is a run of TX
-// contiguous complex numbers; the transposes of the reference (TransposeYZ, src/fft.c:227-312) disappear.
-#include "fft_device.h"
+//   P1 k_p1           : gather S.x from dipoles onto x-lines (y<boxY, z<boxZ), zero-pad to gridX, forward FFT-x,
+//                       store transposed -> Xa[z<boxZ][x<gridX][c][y<boxY]
+//   P2 k_zfft<fwd>    : forward FFT-z of the boxZ nonzero rows of the gridX*3*boxY contiguous columns
+//                       -> Xb[z<gridZ][x][c][y<boxY]
+//   P3 k_p3           : forward FFT-y of contiguous boxY runs (zero-padded to gridY), product with the 6-component
+//                       symmetric D-matrix, inverse FFT-y, keep y<boxY (in place in Xb)
+//   P4 k_zfft<inv>    : inverse FFT-z, keep z<boxZ -> Xa
+//   P5 k_p5           : inverse FFT-x, result_i = arg_i + S_i.X(pos_i), per-block partial ||result||^2
+// P2-P4 read and write only contiguous runs; the transposes of the reference (TransposeYZ, src/fft.c:227-312) are
+// folded into the P1 store and the P5 load (runs of LY lines).
+#include "fft_core.h"
 #include "../../include/adda_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -139,18 +140,15 @@ struct adda_hip_op {
 	// work buffers (FP64-sized; FP32 uses the first half)
 	void *Xa = nullptr;  // [3][bZ][bY][gX]
 	void *Xb = nullptr;  // [3][gZ][bY][gX]
-	FftPlan px, py, pz;
+	FftPlan px, py, pz;      // plans of the D-matrix construction kernels
+	FftPlan2 qx, qy, qz;     // plans of the matvec kernels
 	TwPair twx, twy, twz;
+	int LYp[2], TCp[2], LPp[2];  // matvec tile parameters per precision (index ADDA_F64 / ADDA_F32)
 	// reductions
 	double *partials = nullptr;  // max(4*kRedBlocks, gather blocks)
 	size_t npartials = 0;
 	double *red = nullptr;       // [4] device result
 	double *red_host = nullptr;  // [4] pinned
-	// launch shapes
-	int LL;       // x-lines per block in P1/P5
-	int TXz;      // columns per block in P2/P4
-	int TXy;      // columns per block in P3
-	int NTy;      // threads per block in P3
 };
 
 static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
@@ -158,103 +156,20 @@ static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > 
 // ==================================================================================================================
 // kernels: matvec pipeline
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_scatter_fftx(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
-	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan plan,
-	const cx<T> *__restrict__ tw, int bX, int nlines, int LL, int her)
-{
-	extern __shared__ char smem_raw[];
-	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);  // line (c*LL+l) at offset (c*LL+l)*N
-	const int N = plan.n;
-	const long line0 = (long)blockIdx.x * LL;
-	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
-		const int l = idx / N, x = idx % N;
-		const long line = line0 + l;
-		cx<T> v0 = mk<T>(0, 0), v1 = v0, v2 = v0;
-		if (line < nlines && x < bX) {
-			const int d = cellmap[line * bX + x];
-			if (d >= 0) {
-				const cx<T> *s = ccs + 3 * mat[d];
-				cx<T> a0 = arg[3 * (size_t)d], a1 = arg[3 * (size_t)d + 1], a2 = arg[3 * (size_t)d + 2];
-				if (her) { a0 = conjc(a0); a1 = conjc(a1); a2 = conjc(a2); }
-				v0 = mul(s[0], a0);
-				v1 = mul(s[1], a1);
-				v2 = mul(s[2], a2);
-			}
-		}
-		buf[(0 * LL + l) * N + x] = v0;
-		buf[(1 * LL + l) * N + x] = v1;
-		buf[(2 * LL + l) * N + x] = v2;
-	}
-	__syncthreads();
-	lds_fft<T, false, false, kMaxIt>(buf, plan, 3 * LL, 1, N, tw);
-	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
-		const int l = idx / N, x = idx % N;
-		const long line = line0 + l;
-		if (line < nlines) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) Xa[((size_t)c * nlines + line) * N + x] = buf[(c * LL + l) * N + x];
-		}
-	}
-}
-
+// Deterministic block reduction for any block size up to 512 (not necessarily a multiple of the wave size):
+// a fixed-order tree over LDS. Every thread of the block must call it; the total is returned to thread 0.
 template <typename T>
 __device__ __forceinline__ double block_sum(double s, double *red_lds)
 {
-	for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int tid = threadIdx.x, n = blockDim.x;
 	__syncthreads();
-	if (lane == 0) red_lds[wid] = s;
+	red_lds[tid] = s;
 	__syncthreads();
-	double t = 0;
-	if (threadIdx.x == 0) {
-		for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red_lds[w];
+	for (int stride = 256; stride > 0; stride >>= 1) {
+		if (tid < stride && tid + stride < n) red_lds[tid] += red_lds[tid + stride];
+		__syncthreads();
 	}
-	return t;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_fftx_gather(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
-	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
-	const cx<T> *__restrict__ ccs, FftPlan plan, const cx<T> *__restrict__ tw, int bX, int nlines, int LL, int her,
-	double *__restrict__ partials)
-{
-	extern __shared__ char smem_raw[];
-	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-	__shared__ double red_lds[8];
-	const int N = plan.n;
-	const long line0 = (long)blockIdx.x * LL;
-	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
-		const int l = idx / N, x = idx % N;
-		const long line = line0 + l;
-#pragma unroll
-		for (int c = 0; c < 3; c++)
-			buf[(c * LL + l) * N + x] = line < nlines ? Xa[((size_t)c * nlines + line) * N + x] : mk<T>(0, 0);
-	}
-	__syncthreads();
-	lds_fft<T, true, false, kMaxIt>(buf, plan, 3 * LL, 1, N, tw);
-	double s = 0;
-	for (int idx = threadIdx.x; idx < LL * bX; idx += blockDim.x) {
-		const int l = idx / bX, x = idx % bX;
-		const long line = line0 + l;
-		if (line >= nlines) continue;
-		const int d = cellmap[line * bX + x];
-		if (d < 0) continue;
-		const cx<T> *sc = ccs + 3 * mat[d];
-#pragma unroll
-		for (int c = 0; c < 3; c++) {
-			cx<T> a = arg[3 * (size_t)d + c];
-			if (her) a = conjc(a);
-			cx<T> r = add(a, mul(sc[c], buf[(c * LL + l) * N + x]));
-			if (her) r = conjc(r);
-			res[3 * (size_t)d + c] = r;
-			s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-		}
-	}
-	if (partials) {
-		double t = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[blockIdx.x] = t;
-	}
+	return tid == 0 ? red_lds[0] : 0.0;
 }
 
 // Batched FFT along a strided axis. Block (blockIdx.x, blockIdx.y) transforms columns x0..x0+TX-1 of batch b:
@@ -305,29 +220,91 @@ __global__ void __launch_bounds__(256) k_rowfft(cx<T> *data, FftPlan plan, const
 	}
 }
 
-// P3: forward FFT-y of the 3 components of TX columns at one z, D-matrix product, inverse FFT-y.
-// LDS element (y, c, col) at (y*3 + c)*TX + col. The D-matrix product restates src/matvec.c:270-305 with the
-// reduced-FFT mirror indices and sign flips of IndexDmatrix_mv (87-103) and cSymMatrVec (src/cmplx.h:369-376).
+// ==================================================================================================================
+// kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][c][y<boxY], Xb[z<gridZ][x<gridX][c][y<boxY],
+// D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
+
 template <typename T>
-__global__ void __launch_bounds__(512) k_ydmul(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan plan,
-	const cx<T> *__restrict__ tw, int gX, int bY, int gZ, int DsY, int DsZ, int TX, int reduced, int transposed)
+__device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
+{
+	for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = twg[i];
+	__syncthreads();
+}
+
+// P1: block = (z, y0..y0+LY); lines (c, yl) = c*LY + yl. Stage 0 gathers S.x along x (x fastest across threads);
+// the last stage stores Xa[z][x][c][y] with y fastest across threads.
+template <typename T>
+__global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
+	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her)
 {
 	extern __shared__ char smem_raw[];
-	cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-	const int N = plan.n;  // gridY
-	const int x0 = blockIdx.x * TX;
-	const int z = blockIdx.y;
-	const size_t cstride = (size_t)gZ * bY * gX;
-	const size_t zoff = (size_t)z * bY * gX;
-	for (int idx = threadIdx.x; idx < N * 3 * TX; idx += blockDim.x) {
-		const int y = idx / (3 * TX), r = idx % (3 * TX), c = r / TX, col = r % TX, x = x0 + col;
-		buf[idx] = (y < bY && x < gX) ? Xb[c * cstride + zoff + (size_t)y * gX + x] : mk<T>(0, 0);
-	}
+	const int N = p.n, TPL = N / p.rmax, NL = 3 * LY, LS = N + 1;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = blockIdx.x / nyt, y0 = (blockIdx.x % nyt) * LY;
+	const int tid = threadIdx.x;
+	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest
+	const int lineB = tid % NL, tB = tid / NL;    // y fastest
+	const bool actA = lineA < NL && y0 + lineA % LY < bY, actB = tB < TPL && y0 + lineB % LY < bY;
+	auto ld = [&](int line, int x) -> cx<T> {
+		const int c = line / LY, y = y0 + line % LY;
+		if (x >= bX) return mk<T>(0, 0);
+		const int d = cellmap[((size_t)z * bY + y) * bX + x];
+		if (d < 0) return mk<T>(0, 0);
+		cx<T> a = arg[3 * (size_t)d + c];
+		if (her) a = conjc(a);
+		return mul(ccs[3 * mat[d] + c], a);
+	};
+	auto st = [&](int line, int x, cx<T> v) {
+		const int c = line / LY, y = y0 + line % LY;
+		Xa[(((size_t)z * N + x) * 3 + c) * bY + y] = v;
+	};
+	fft_line<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+}
+
+// P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
+template <typename T, bool INV>
+__global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
+	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out)
+{
+	extern __shared__ char smem_raw[];
+	const int N = p.n, TPL = N / p.rmax;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const long q0 = (long)blockIdx.x * TC;
+	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
+	const bool act = t < TPL && q0 + line < Q;
+	auto ld = [&](int l, int k) -> cx<T> { return k < n_in ? in[(size_t)k * Q + q0 + l] : mk<T>(0, 0); };
+	auto st = [&](int l, int k, cx<T> v) {
+		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
+	};
+	fft_line<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+}
+
+// P3: block = LP consecutive x at one z; lines (pair, c) = pair*3 + c, contiguous boxY runs of Xb. Forward FFT-y into
+// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips of
+// IndexDmatrix_mv 87-103, cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
+template <typename T>
+__global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed)
+{
+	extern __shared__ char smem_raw[];
+	const int N = p.n, TPL = N / p.rmax, NL = 3 * LP, LS = N + 1;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nxt = gX / LP;
+	const int z = blockIdx.x / nxt, x0 = (blockIdx.x % nxt) * LP;
+	cx<T> *base = Xb + ((size_t)z * gX + x0) * 3 * bY;  // lines of this block are contiguous
+	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
+	const bool act = line < NL;
+	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? base[(size_t)l * bY + y] : mk<T>(0, 0); };
+	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + y] = v; };
+	fft_line<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
 	__syncthreads();
-	lds_fft<T, false, true, kMaxIt>(buf, plan, 3 * TX, 3 * TX, 1, tw);
-	for (int idx = threadIdx.x; idx < N * TX; idx += blockDim.x) {
-		const int y = idx / TX, col = idx % TX, x = x0 + col;
-		if (x >= gX) continue;
+	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
+		const int pr = idx / N, y = idx % N, x = x0 + pr;
 		int xi = x, yi = y, zi = z;
 		if (transposed) {
 			xi = xi > 0 ? gX - xi : 0;
@@ -338,7 +315,7 @@ __global__ void __launch_bounds__(512) k_ydmul(cx<T> *Xb, const cx<T> *__restric
 			if (yi >= DsY) yi = N - yi;
 			if (zi >= DsZ) zi = gZ - zi;
 		}
-		const cx<T> *dm = D + (((size_t)zi * DsY + yi) * gX + xi) * 6;
+		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
 		cx<T> f0 = dm[0], f1 = dm[1], f2 = dm[2], f3 = dm[3], f4 = dm[4], f5 = dm[5];
 		if (reduced) {
 			if (y >= DsY) {
@@ -351,17 +328,60 @@ __global__ void __launch_bounds__(512) k_ydmul(cx<T> *Xb, const cx<T> *__restric
 				f4 = scl(f4, (T)-1);
 			}
 		}
-		cx<T> *p = buf + (size_t)y * 3 * TX + col;
-		const cx<T> a0 = p[0], a1 = p[TX], a2 = p[2 * TX];
-		p[0] = add(add(mul(f0, a0), mul(f1, a1)), mul(f2, a2));
-		p[TX] = add(add(mul(f1, a0), mul(f3, a1)), mul(f4, a2));
-		p[2 * TX] = add(add(mul(f2, a0), mul(f4, a1)), mul(f5, a2));
+		cx<T> *q = buf + (size_t)(3 * pr) * LS + y;
+		const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
+		q[0] = add(add(mul(f0, a0), mul(f1, a1)), mul(f2, a2));
+		q[LS] = add(add(mul(f1, a0), mul(f3, a1)), mul(f4, a2));
+		q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(f5, a2));
 	}
 	__syncthreads();
-	lds_fft<T, true, true, kMaxIt>(buf, plan, 3 * TX, 3 * TX, 1, tw);
-	for (int idx = threadIdx.x; idx < bY * 3 * TX; idx += blockDim.x) {
-		const int y = idx / (3 * TX), r = idx % (3 * TX), c = r / TX, col = r % TX, x = x0 + col;
-		if (x < gX) Xb[c * cstride + zoff + (size_t)y * gX + x] = buf[idx];
+	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + y]; };
+	auto stg = [&](int l, int y, cx<T> v) {
+		if (y < bY) base[(size_t)l * bY + y] = v;
+	};
+	fft_line<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+}
+
+// P5: block = (z, y0..y0+LY); stage 0 loads Xa[z][x][c][y] y-fastest, the last stage gathers to the dipoles along x:
+// result = arg + S.X (her: conj), partial ||result||^2 per block.
+template <typename T>
+__global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
+	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
+	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
+	double *__restrict__ partials)
+{
+	extern __shared__ char smem_raw[];
+	__shared__ double red_lds[512];
+	const int N = p.n, TPL = N / p.rmax, NL = 3 * LY, LS = N + 1;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = blockIdx.x / nyt, y0 = (blockIdx.x % nyt) * LY;
+	const int tid = threadIdx.x;
+	const int lineA = tid % NL, tA = tid / NL;    // y fastest
+	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest
+	const bool actA = tA < TPL && y0 + lineA % LY < bY, actB = lineB < NL && y0 + lineB % LY < bY;
+	double s = 0;
+	auto ld = [&](int line, int x) -> cx<T> {
+		const int c = line / LY, y = y0 + line % LY;
+		return Xa[(((size_t)z * N + x) * 3 + c) * bY + y];
+	};
+	auto st = [&](int line, int x, cx<T> v) {
+		if (x >= bX) return;
+		const int c = line / LY, y = y0 + line % LY;
+		const int d = cellmap[((size_t)z * bY + y) * bX + x];
+		if (d < 0) return;
+		cx<T> a = arg[3 * (size_t)d + c];
+		if (her) a = conjc(a);
+		cx<T> r = add(a, mul(ccs[3 * mat[d] + c], v));
+		if (her) r = conjc(r);
+		res[3 * (size_t)d + c] = r;
+		s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+	};
+	fft_line<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	if (partials) {
+		double tt = block_sum<T>(s, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = tt;
 	}
 }
 
@@ -414,13 +434,13 @@ __global__ void k_green(cx<double> *G, int bX, int bY, int bZ, int gX, int gY, i
 	}
 }
 
-// D[zt][yt][x][c] = -G~[c][zt][yt][x]/Ngrid for zt<DsZ, yt<DsY (src/fft.c:1310-1314)
+// D[zt][x][yt][c] = -G~[c][zt][yt][x]/Ngrid for zt<DsZ, yt<DsY (src/fft.c:1310-1314)
 __global__ void k_extract_d(const cx<double> *G, cx<double> *D, int gX, int gY, int gZ, int DsY, int DsZ)
 {
 	const size_t n = (size_t)gX * gY * gZ, m = (size_t)DsZ * DsY * gX;
 	const double invN = 1.0 / (double)n;
 	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m; idx += (size_t)gridDim.x * blockDim.x) {
-		const int x = (int)(idx % gX), y = (int)((idx / gX) % DsY), z = (int)(idx / ((size_t)gX * DsY));
+		const int y = (int)(idx % DsY), x = (int)((idx / DsY) % gX), z = (int)(idx / ((size_t)gX * DsY));
 		const size_t src = ((size_t)z * gY + y) * gX + x;
 		for (int c = 0; c < 6; c++) {
 			cx<double> v = G[c * n + src];
@@ -443,7 +463,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_lincomb(cx<T> *out, const cx<T> *a, cx<double> ca, const cx<T> *b,
 	cx<double> cb, const cx<T> *c, cx<double> cc, int flags, size_t n, double *partials)
 {
-	__shared__ double red_lds[8];
+	__shared__ double red_lds[512];
 	double s = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		double rx = 0, ry = 0;
@@ -480,7 +500,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int mode, size_t n, double *partials,
 	int nblk)
 {
-	__shared__ double red_lds[8];
+	__shared__ double red_lds[512];
 	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		const cx<T> u = a[i];
@@ -510,7 +530,7 @@ __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int
 // Deterministic final reduction of nk rows of nblk partials: fixed strided order per thread, then a fixed tree.
 __global__ void __launch_bounds__(256) k_reduce_final(const double *partials, int nblk, int nk, double *out)
 {
-	__shared__ double red_lds[8];
+	__shared__ double red_lds[512];
 	for (int k = 0; k < nk; k++) {
 		double s = 0;
 		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[k * nblk + i];
@@ -549,7 +569,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_field_sums(const cx<T> *P, const unsigned short *pos, size_t nd, double kd,
 	const double *dirs, double *partials)
 {
-	__shared__ double red_lds[8];
+	__shared__ double red_lds[512];
 	const int dir = blockIdx.y, chunk = blockIdx.x;
 	const double nx = dirs[3 * dir], ny = dirs[3 * dir + 1], nz = dirs[3 * dir + 2];
 	double s[6] = {0, 0, 0, 0, 0, 0};
@@ -583,7 +603,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_wnorm(const cx<T> *v, const int *mat, const double *w, size_t nd,
 	double *partials)
 {
-	__shared__ double red_lds[8];
+	__shared__ double red_lds[512];
 	double s = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
 		const cx<T> a = v[i];
@@ -629,9 +649,11 @@ template <typename T>
 static int set_lds_limits()
 {
 	const int big = 96 * 1024;  // largest dynamic LDS tile used (k_ydmul)
-	HIPCHK(hipFuncSetAttribute((const void *)k_ydmul<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_scatter_fftx<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_fftx_gather<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 64));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
@@ -649,56 +671,34 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
 	else { ccs = op->cc32; D = op->D32; }
 	if (!D) return ERR("FP32 matvec requested but the operator was created without the FP32 flag");
+	const int pi = sizeof(T) == 8 ? ADDA_F64 : ADDA_F32;
 	cx<T> *Xa = (cx<T> *)op->Xa, *Xb = (cx<T> *)op->Xb;
 	const int gX = op->gX, bX = op->bX, bY = op->bY, bZ = op->bZ, gY = op->gY, gZ = op->gZ;
-	const int nlx = bY * bZ;
 	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
+	const int LY = op->LYp[pi], TC = op->TCp[pi], LP = op->LPp[pi];
+	const int tplx = gX / op->qx.rmax, tply = gY / op->qy.rmax, tplz = gZ / op->qz.rmax;
+	const long Q = (long)gX * 3 * bY;
+	const int nyt = (bY + LY - 1) / LY;
+	const size_t lds1 = (size_t)(gX + 3 * LY * (gX + 1)) * sizeof(cx<T>);
+	const size_t lds2 = (size_t)(gZ + gZ * TC) * sizeof(cx<T>);
+	const size_t lds3 = (size_t)(gY + 3 * LP * (gY + 1)) * sizeof(cx<T>);
 	if (ev) HIPCHK(hipEventRecord(ev[0], op->stream));
-	// P1
-	{
-		const int nblk = (nlx + op->LL - 1) / op->LL;
-		const size_t lds = (size_t)3 * op->LL * gX * sizeof(cx<T>);
-		hipLaunchKernelGGL(k_scatter_fftx<T>, dim3(nblk), dim3(256), lds, op->stream, arg, Xa, op->cellmap, op->mat,
-			ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her);
-	}
+	hipLaunchKernelGGL(k_p1<T>, dim3(bZ * nyt), dim3(3 * LY * tplx), lds1, op->stream, arg, Xa, op->cellmap, op->mat,
+		ccs, op->qx, tw_of<T>(op->twx), bX, bY, LY, her);
 	if (ev) HIPCHK(hipEventRecord(ev[1], op->stream));
-	// P2: forward FFT-z over bZ nonzero rows; batch b = c*bY + y
-	{
-		const int TX = op->TXz;
-		dim3 grid((gX + TX - 1) / TX, 3 * bY);
-		const size_t lds = (size_t)gZ * TX * sizeof(cx<T>);
-		hipLaunchKernelGGL((k_colfft<T, false>), grid, dim3(256), lds, op->stream, Xa, Xb, op->pz, tw_of<T>(op->twz),
-			gX, TX, bZ, gZ, (long)bY * gX, (long)bY * gX, bY, (long)bZ * bY * gX, (long)gX, (long)gZ * bY * gX,
-			(long)gX);
-	}
+	hipLaunchKernelGGL((k_zfft<T, false>), dim3((unsigned)((Q + TC - 1) / TC)), dim3(TC * tplz), lds2, op->stream,
+		(const cx<T> *)Xa, Xb, op->qz, tw_of<T>(op->twz), Q, TC, bZ, gZ);
 	if (ev) HIPCHK(hipEventRecord(ev[2], op->stream));
-	// P3
-	{
-		const int TX = op->TXy;
-		dim3 grid((gX + TX - 1) / TX, gZ);
-		const size_t lds = (size_t)3 * gY * TX * sizeof(cx<T>);
-		hipLaunchKernelGGL(k_ydmul<T>, grid, dim3(op->NTy), lds, op->stream, Xb, D, op->py, tw_of<T>(op->twy), gX, bY,
-			gZ, op->DsY, op->DsZ, TX, op->reduced, transposed);
-	}
+	hipLaunchKernelGGL(k_p3<T>, dim3(gZ * (gX / LP)), dim3(3 * LP * tply), lds3, op->stream, Xb, D, op->qy,
+		tw_of<T>(op->twy), gX, bY, gZ, op->DsY, op->DsZ, LP, op->reduced, transposed);
 	if (ev) HIPCHK(hipEventRecord(ev[3], op->stream));
-	// P4: inverse FFT-z, keep z<bZ
-	{
-		const int TX = op->TXz;
-		dim3 grid((gX + TX - 1) / TX, 3 * bY);
-		const size_t lds = (size_t)gZ * TX * sizeof(cx<T>);
-		hipLaunchKernelGGL((k_colfft<T, true>), grid, dim3(256), lds, op->stream, Xb, Xa, op->pz, tw_of<T>(op->twz),
-			gX, TX, gZ, bZ, (long)bY * gX, (long)bY * gX, bY, (long)gZ * bY * gX, (long)gX, (long)bZ * bY * gX,
-			(long)gX);
-	}
+	hipLaunchKernelGGL((k_zfft<T, true>), dim3((unsigned)((Q + TC - 1) / TC)), dim3(TC * tplz), lds2, op->stream,
+		(const cx<T> *)Xb, Xa, op->qz, tw_of<T>(op->twz), Q, TC, gZ, bZ);
 	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
-	// P5
-	const int nblk5 = (nlx + op->LL - 1) / op->LL;
-	{
-		const size_t lds = (size_t)3 * op->LL * gX * sizeof(cx<T>);
-		hipLaunchKernelGGL(k_fftx_gather<T>, dim3(nblk5), dim3(256), lds, op->stream, (const cx<T> *)Xa, arg, res,
-			op->cellmap, op->mat, ccs, op->px, tw_of<T>(op->twx), bX, nlx, op->LL, her,
-			inprod ? op->partials : (double *)nullptr);
-	}
+	const int nblk5 = bZ * nyt;
+	hipLaunchKernelGGL(k_p5<T>, dim3(nblk5), dim3(3 * LY * tplx), lds1, op->stream, (const cx<T> *)Xa, arg, res,
+		op->cellmap, op->mat, ccs, op->qx, tw_of<T>(op->twx), bX, bY, LY, her,
+		inprod ? op->partials : (double *)nullptr);
 	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
 	if (inprod) {
@@ -707,6 +707,23 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 		*inprod = v;
 	}
 	return 0;
+}
+
+// Tile parameters of the matvec kernels for element size es: the largest tiles (LY <= 8 lines, TC <= 16 columns,
+// LP <= 4 pairs dividing gridX) whose block fits 512 threads and whose LDS fits lds_cap bytes.
+static void choose_tiles(adda_hip_op *op, int pi, size_t es, size_t lds_cap)
+{
+	const int gX = op->gX, gY = op->gY, gZ = op->gZ;
+	const int tplx = gX / op->qx.rmax, tply = gY / op->qy.rmax, tplz = gZ / op->qz.rmax;
+	int LY = 8;
+	while (LY > 1 && (3 * LY * tplx > 512 || (gX + 3 * LY * (gX + 1)) * es > lds_cap)) LY /= 2;
+	int TC = 16;
+	while (TC > 1 && (TC * tplz > 512 || (size_t)(gZ + gZ * TC) * es > lds_cap)) TC /= 2;
+	int LP = 4;
+	while (LP > 1 && (3 * LP * tply > 512 || (gY + 3 * LP * (gY + 1)) * es > lds_cap || gX % LP)) LP--;
+	op->LYp[pi] = LY;
+	op->TCp[pi] = TC;
+	op->LPp[pi] = LP;
 }
 
 // ==================================================================================================================
@@ -741,19 +758,14 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
 	if (gX < 2 * bX || gY < 2 * bY || gZ < 2 * bZ || gY % 2 || gZ % 2)
 		return ERR("FFT grid %dx%dx%d does not embed box %dx%dx%d (need even grid >= 2*box)", gX, gY, gZ, bX, bY, bZ);
-	// launch shapes: all LDS tiles <= 96 KB, butterflies per thread <= kMaxIt
-	op->LL = 4;
-	while (op->LL > 1 && ((size_t)3 * op->LL * gX * 16 > 64 * 1024 || !plan_fits(op->px, 3 * op->LL, 256))) op->LL /= 2;
-	op->TXz = 16;
-	while (op->TXz > 1 && ((size_t)gZ * op->TXz * 16 > 64 * 1024 || !plan_fits(op->pz, op->TXz, 256))) op->TXz /= 2;
-	op->NTy = 512;
-	op->TXy = 8;
-	while (op->TXy > 1 && ((size_t)3 * gY * op->TXy * 16 > 96 * 1024 || !plan_fits(op->py, 3 * op->TXy, op->NTy)))
-		op->TXy /= 2;
-	if (!plan_fits(op->px, 3 * op->LL, 256) || !plan_fits(op->pz, op->TXz, 256) ||
-		!plan_fits(op->py, 3 * op->TXy, op->NTy) || (size_t)3 * op->LL * gX * 16 > 96 * 1024 ||
-		(size_t)gZ * op->TXz * 16 > 96 * 1024 || (size_t)3 * gY * op->TXy * 16 > 96 * 1024)
-		return ERR("grid %dx%dx%d too large for the LDS tiles of this build", gX, gY, gZ);
+	if (!make_plan2(gX, op->qx) || !make_plan2(gY, op->qy) || !make_plan2(gZ, op->qz))
+		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
+	choose_tiles(op, ADDA_F64, 16, 64 * 1024);
+	choose_tiles(op, ADDA_F32, 8, 64 * 1024);
+	if (3 * op->LYp[0] * (gX / op->qx.rmax) > 512 || op->TCp[0] * (gZ / op->qz.rmax) > 512 ||
+		3 * op->LPp[0] * (gY / op->qy.rmax) > 512 || (size_t)(gX + 3 * (gX + 1)) * 16 > 96 * 1024 ||
+		(size_t)(gY + 3 * (gY + 1)) * 16 > 96 * 1024 || (size_t)(gZ + gZ) * 16 > 96 * 1024)
+		return ERR("grid %dx%dx%d too large for the tiles of this build", gX, gY, gZ);
 	if (upload_twiddles(gX, op->twx) || upload_twiddles(gY, op->twy) || upload_twiddles(gZ, op->twz)) return -1;
 	// geometry: cell map and materials
 	const size_t nbox = (size_t)bX * bY * bZ;
@@ -782,7 +794,7 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	HIPCHK(hipMalloc(&op->Xa, (size_t)3 * bZ * bY * gX * sizeof(cx<double>)));
 	HIPCHK(hipMalloc(&op->Xb, (size_t)3 * gZ * bY * gX * sizeof(cx<double>)));
 	{
-		const size_t nblk5 = ((size_t)bY * bZ + op->LL - 1) / op->LL;
+		const size_t nblk5 = (size_t)bZ * ((bY + 1) / 1);  // gather blocks with LY >= 1
 		op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);  // field sums need the most
 		if (nblk5 > op->npartials) op->npartials = nblk5;
 		HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
@@ -901,13 +913,13 @@ extern "C" int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst)
 	std::vector<adda_cplx> tmp(n);
 	HIPCHK(hipMemcpyAsync(tmp.data(), op->D64, n * sizeof(adda_cplx), hipMemcpyDeviceToHost, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
-	// device [zt][yt][x][6] -> ADDA [x][zt][yt][6]
+	// device [zt][x][yt][6] -> ADDA [x][zt][yt][6]
 	for (int z = 0; z < op->DsZ; z++)
-		for (int y = 0; y < op->DsY; y++)
-			for (int x = 0; x < op->gX; x++)
+		for (int x = 0; x < op->gX; x++)
+			for (int y = 0; y < op->DsY; y++)
 				for (int c = 0; c < 6; c++)
 					dst[6 * (((size_t)x * op->DsZ + z) * op->DsY + y) + c] =
-						tmp[(((size_t)z * op->DsY + y) * op->gX + x) * 6 + c];
+						tmp[(((size_t)z * op->gX + x) * op->DsY + y) * 6 + c];
 	return 0;
 }
 

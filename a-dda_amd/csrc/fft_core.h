@@ -1,0 +1,196 @@
+// fft_core.h -- register/LDS Stockham FFT core with the first stage loading from global memory and the last stage
+// storing to global memory through caller functors (CDNA4, wave64).
+//
+// A line of length N = R_0*R_1*...*R_{s-1} (radices 16, 8, 4, 2, 3, 5) is owned by TPL = N/RMAX threads (RMAX = the
+// largest radix of the plan). Stage s has N/R_s butterflies; thread t of a line owns butterflies j = t + u*TPL.
+// Stage 0 reads its inputs straight from the Load functor (global memory, zero padding, gathers), the middle stages
+// exchange through LDS, and the last stage hands each output (index j + q*N/R) to the Store functor. A 256-point
+// transform therefore costs one LDS exchange (two radix-16 stages).
+//
+// Unnormalized DFT, forward sign -1, backward +1 (the convention of ADDA's FFT layer, src/fft.h:31-32).
+#pragma once
+#include "fft_device.h"
+
+namespace adda {
+
+struct FftPlan2 {
+	int n;                 // line length
+	int nst;               // stages
+	int rad[kMaxStages];   // radices
+	int rmax;              // largest radix (threads per line = n / rmax)
+};
+
+// DFT-16 as 4 x 4 with the W16 twiddles between the two radix-4 layers.
+template <typename T, bool INV> __device__ __forceinline__ void dft16(cx<T> *v)
+{
+	const T c1 = (T)0.92387953251128675612818318939678829;  // cos(pi/8)
+	const T s1 = (T)0.38268343236508977172845998403039887;  // sin(pi/8)
+	const T h = (T)0.70710678118654752440084436210484904;
+	cx<T> a[4][4];
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		cx<T> t[4] = {v[r], v[r + 4], v[r + 8], v[r + 12]};
+		dft4<T, INV>(t);
+#pragma unroll
+		for (int s = 0; s < 4; s++) a[r][s] = t[s];
+	}
+	// twiddle a[r][s] *= W16^(r*s) (forward exp(-2 pi i rs/16))
+	auto tw = [&](cx<T> x, T c, T s) -> cx<T> {
+		return INV ? mk<T>(x.x * c - x.y * s, x.x * s + x.y * c) : mk<T>(x.x * c + x.y * s, x.y * c - x.x * s);
+	};
+	a[1][1] = tw(a[1][1], c1, s1);
+	a[1][2] = tw(a[1][2], h, h);
+	a[1][3] = tw(a[1][3], s1, c1);
+	a[2][1] = tw(a[2][1], h, h);
+	a[2][2] = mulmi<T, INV>(a[2][2]);
+	a[2][3] = tw(a[2][3], -h, h);
+	a[3][1] = tw(a[3][1], s1, c1);
+	a[3][2] = tw(a[3][2], -h, h);
+	a[3][3] = tw(a[3][3], -c1, -s1);
+#pragma unroll
+	for (int s = 0; s < 4; s++) {
+		cx<T> t[4] = {a[0][s], a[1][s], a[2][s], a[3][s]};
+		dft4<T, INV>(t);
+#pragma unroll
+		for (int q = 0; q < 4; q++) v[s + 4 * q] = t[q];
+	}
+}
+
+template <typename T, bool INV, int R> __device__ __forceinline__ void dftR2(cx<T> *v)
+{
+	if constexpr (R == 16) dft16<T, INV>(v);
+	else dftR<T, INV, R>(v);
+}
+
+// Number of butterflies per thread of a stage of radix R (compile-time bound for RMAX <= 16).
+template <int R> constexpr int maxu() { return (16 + R - 1) / R; }
+
+// One stage. FIRST: inputs come from ld(line, k); LAST: outputs go to st(line, k, v); otherwise through LDS
+// (element k of line l at buf[k*KS + l*LS]). Twiddles from the LDS table tws[e] = exp(-2 pi i e/N).
+// Control flow is uniform: every thread of the block executes the same barriers; `active` only predicates the work
+// (a wave must never reach a barrier from two branches). A non-first stage synchronises between reading LDS and
+// writing its outputs (the LAST stage's Store may target the same LDS), and a non-last stage synchronises after
+// writing.
+template <typename T, bool INV, int R, bool FIRST, bool LAST, class Ld, class St>
+__device__ __forceinline__ void stage2(cx<T> *buf, const cx<T> *tws, int N, int Ns, int KS, int LS, int line, int t,
+	int TPL, bool active, Ld &ld, St &st)
+{
+	constexpr int MU = maxu<R>();
+	const int nb = N / R, twstep = N / (Ns * R);
+	cx<T> v[MU][R];
+#pragma unroll
+	for (int u = 0; u < MU; u++) {
+		const int j = t + u * TPL;
+		if (active && (MU == 1 || j < nb)) {
+#pragma unroll
+			for (int q = 0; q < R; q++) v[u][q] = FIRST ? ld(line, j + q * nb) : buf[(j + q * nb) * KS + line * LS];
+			if (!FIRST) {
+				const int k = j % Ns;
+#pragma unroll
+				for (int q = 1; q < R; q++) {
+					const cx<T> w = tws[q * k * twstep];
+					v[u][q] = mul(v[u][q], INV ? conjc(w) : w);
+				}
+			}
+			dftR2<T, INV, R>(v[u]);
+		}
+	}
+	__syncthreads();  // a FIRST stage's Load may read the LDS buffer it is about to overwrite
+	if (LAST) {
+		// Ns*R == N here: output index j + q*Ns
+#pragma unroll
+		for (int u = 0; u < MU; u++) {
+			const int j = t + u * TPL;
+			if (active && (MU == 1 || j < nb)) {
+#pragma unroll
+				for (int q = 0; q < R; q++) st(line, j + q * Ns, v[u][q]);
+			}
+		}
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < MU; u++) {
+			const int j = t + u * TPL;
+			if (active && (MU == 1 || j < nb)) {
+				const int k = j % Ns;
+				const int d = ((j / Ns) * Ns * R + k) * KS + line * LS;
+#pragma unroll
+				for (int q = 0; q < R; q++) buf[d + q * Ns * KS] = v[u][q];
+			}
+		}
+		__syncthreads();
+	}
+}
+
+template <typename T, bool INV, bool FIRST, bool LAST, class Ld, class St>
+__device__ __forceinline__ void stage_dispatch(int R, cx<T> *buf, const cx<T> *tws, int N, int Ns, int KS, int LS,
+	int line, int t, int TPL, bool active, Ld &ld, St &st)
+{
+	switch (R) {
+		case 16: stage2<T, INV, 16, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 8: stage2<T, INV, 8, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 4: stage2<T, INV, 4, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 2: stage2<T, INV, 2, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 3: stage2<T, INV, 3, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		default: stage2<T, INV, 5, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+	}
+}
+
+// Full transform of one line by its TPL threads. All threads of the block call this together (uniform control flow).
+// The first stage, which reads global memory, uses the thread mapping (lineA, tA, activeA); the other stages, the
+// last of which writes global memory, use (lineB, tB, activeB) -- so each global access can be mapped for contiguity
+// (e.g. x-fastest for a dipole gather, y-fastest for a transposed store). The plan is uniform across the block.
+template <typename T, bool INV, class Ld, class St>
+__device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS, int lineA, int tA,
+	bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
+{
+	const int N = p.n, TPL = N / p.rmax;
+	if (p.nst == 1) {
+		stage_dispatch<T, INV, true, true>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
+		return;
+	}
+	stage_dispatch<T, INV, true, false>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
+	int Ns = p.rad[0];
+	for (int s = 1; s < p.nst - 1; s++) {
+		stage_dispatch<T, INV, false, false>(p.rad[s], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
+		Ns *= p.rad[s];
+	}
+	stage_dispatch<T, INV, false, true>(p.rad[p.nst - 1], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
+}
+
+// Host-side plan: radices 5 and 3, then the power of two as radix-16 stages with one radix-8/4/2 remainder
+// (2^(4a+1) uses 8*4 instead of 16*2). Stages run largest radix first.
+inline bool make_plan2(int n, FftPlan2 &p)
+{
+	p.n = n;
+	p.nst = 0;
+	int m = n, k2 = 0, r5 = 0, r3 = 0;
+	while (m % 5 == 0) { r5++; m /= 5; }
+	while (m % 3 == 0) { r3++; m /= 3; }
+	while (m % 2 == 0) { k2++; m /= 2; }
+	if (m != 1) return false;
+	int rad[kMaxStages], ns = 0;
+	int a = k2 / 4, r = k2 % 4;
+	int extra[2] = {0, 0}, ne = 0;
+	if (r == 1) {
+		if (a > 0) { a--; extra[ne++] = 8; extra[ne++] = 4; }
+		else extra[ne++] = 2;
+	}
+	else if (r == 2) extra[ne++] = 4;
+	else if (r == 3) extra[ne++] = 8;
+	if (a + ne + r3 + r5 > kMaxStages) return false;
+	for (int i = 0; i < a; i++) rad[ns++] = 16;
+	for (int i = 0; i < ne; i++) rad[ns++] = extra[i];
+	for (int i = 0; i < r5; i++) rad[ns++] = 5;
+	for (int i = 0; i < r3; i++) rad[ns++] = 3;
+	if (ns == 0) return false;  // n == 1 is not a valid grid
+	p.rmax = 1;
+	for (int i = 0; i < ns; i++) {
+		p.rad[i] = rad[i];
+		if (rad[i] > p.rmax) p.rmax = rad[i];
+	}
+	p.nst = ns;
+	return true;
+}
+
+}  // namespace adda

@@ -76,6 +76,9 @@ def hip():
     L.adda_hip_profile_matvec.argtypes = [P, I, P, P, I, P, P]
     L.adda_hip_matvec_bytes.argtypes = [P, I, I]
     L.adda_hip_matvec_bytes.restype = D
+    L.adda_hip_get_config.argtypes = [P, I, ctypes.c_char_p, S]
+    L.adda_hip_set_config.argtypes = [P, I, ctypes.c_char_p]
+    L.adda_hip_autotune.argtypes = [P, I, I, ctypes.c_char_p, ctypes.c_char_p, S]
     _hip = L
     return L
 
@@ -183,6 +186,20 @@ class Operator:
         check(hip().adda_hip_profile_matvec(self.h, prec, a, r, nrep, _p(ms), ctypes.byref(tot)))
         return ms, tot.value
 
+    def get_config(self, prec=F64):
+        b = ctypes.create_string_buffer(256)
+        check(hip().adda_hip_get_config(self.h, prec, b, 256))
+        return b.value.decode()
+
+    def set_config(self, cfg, prec=F64):
+        check(hip().adda_hip_set_config(self.h, prec, cfg.encode()))
+
+    def autotune(self, prec=F64, reps=3, cache=None):
+        """Pick the fastest radix schedules and tiles per pass (HIP-event timing); returns the report string."""
+        b = ctypes.create_string_buffer(512)
+        check(hip().adda_hip_autotune(self.h, prec, reps, cache.encode() if cache else None, b, 512))
+        return b.value.decode()
+
     def matvec_bytes(self, prec=F64, pass_=5):
         return hip().adda_hip_matvec_bytes(self.h, prec, pass_)
 
@@ -206,7 +223,8 @@ class Particle(ctypes.Structure):
 class SolverOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("eps", ctypes.c_double), ("maxiter", ctypes.c_int),
                 ("init_field", ctypes.c_int), ("recalc_resid", ctypes.c_int), ("precision", ctypes.c_int),
-                ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -323,11 +341,11 @@ class Problem:
                                       _p(self.incpol), self.einc))
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
-              max_outer=0, recalc_resid=False, hist_len=20000, verbose=False):
+              max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0):
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
-                       inner_eps, max_outer, int(verbose))
+                       inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner))
         st = SolveStats()
         hist = np.zeros(hist_len)
         rc = host().adda_iterative_solver(self.op.h, ctypes.byref(o), self.einc, self.x, self.pvec, _p(hist),

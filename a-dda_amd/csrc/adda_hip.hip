@@ -13,6 +13,7 @@
 // P2-P4 read and write only contiguous runs; the transposes of the reference (TransposeYZ, src/fft.c:227-312) are
 // folded into the P1 store and the P5 load (runs of LY lines).
 #include "fft_core.h"
+#include "krylov_dev.h"
 #include "../../include/adda_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -24,6 +25,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <algorithm>
+#include <functional>
 
 using namespace adda;
 
@@ -110,6 +113,14 @@ static void twiddles(int n, std::vector<double> &out)
 // ==================================================================================================================
 // operator state
 
+// Radix schedules and tile sizes of the five matvec passes for one precision.
+struct MvCfg {
+	FftPlan2 qx, qy, qz;  // radix schedules along x (P1/P5), y (P3), z (P2/P4)
+	int LY;               // lines (y values) per block in P1/P5
+	int TC;               // columns per block in P2/P4
+	int LP;               // x-pairs per block in P3
+};
+
 struct TwPair {
 	cx<double> *d = nullptr;
 	cx<float> *f = nullptr;
@@ -141,12 +152,14 @@ struct adda_hip_op {
 	void *Xa = nullptr;  // [3][bZ][bY][gX]
 	void *Xb = nullptr;  // [3][gZ][bY][gX]
 	FftPlan px, py, pz;      // plans of the D-matrix construction kernels
-	FftPlan2 qx, qy, qz;     // plans of the matvec kernels
+	MvCfg cfg[2];            // matvec plans and tiles per precision (index ADDA_F64 / ADDA_F32)
 	TwPair twx, twy, twz;
-	int LYp[2], TCp[2], LPp[2];  // matvec tile parameters per precision (index ADDA_F64 / ADDA_F32)
 	// reductions
 	double *partials = nullptr;  // max(4*kRedBlocks, gather blocks)
 	size_t npartials = 0;
+	double *partials2 = nullptr; // [4*kRedBlocks] second region for device-resident solver dots
+	const double *gate = nullptr; // device solver state while its iterations are enqueued: kernels return at once
+	                              // when its status slot is non-zero
 	double *red = nullptr;       // [4] device result
 	double *red_host = nullptr;  // [4] pinned
 };
@@ -231,44 +244,51 @@ __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ tw
 	__syncthreads();
 }
 
-// P1: block = (z, y0..y0+LY); lines (c, yl) = c*LY + yl. Stage 0 gathers S.x along x (x fastest across threads);
-// the last stage stores Xa[z][x][c][y] with y fastest across threads.
-template <typename T>
+// P1: block = (z, y0..y0+LY, c) with the component fastest in the block index (the three blocks of a tile read the
+// same dipoles back to back). The gather of S.x along x (x fastest) fills the LDS lines; the last FFT stage stores
+// Xa[z][x][c][y] with y fastest across threads (runs of LY).
+template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
 	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her)
+	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, const double *gate)
 {
+	if (gate && gate[DK_STATUS] != 0) return;
 	extern __shared__ char smem_raw[];
-	const int N = p.n, TPL = N / p.rmax, NL = 3 * LY, LS = N + 1;
+	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
-	const int z = blockIdx.x / nyt, y0 = (blockIdx.x % nyt) * LY;
+	const int c = blockIdx.x % 3, tile = blockIdx.x / 3;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int tid = threadIdx.x;
 	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest
-	const int lineB = tid % NL, tB = tid / NL;    // y fastest
-	const bool actA = lineA < NL && y0 + lineA % LY < bY, actB = tB < TPL && y0 + lineB % LY < bY;
-	auto ld = [&](int line, int x) -> cx<T> {
-		const int c = line / LY, y = y0 + line % LY;
-		if (x >= bX) return mk<T>(0, 0);
-		const int d = cellmap[((size_t)z * bY + y) * bX + x];
-		if (d < 0) return mk<T>(0, 0);
-		cx<T> a = arg[3 * (size_t)d + c];
-		if (her) a = conjc(a);
-		return mul(ccs[3 * mat[d] + c], a);
-	};
-	auto st = [&](int line, int x, cx<T> v) {
-		const int c = line / LY, y = y0 + line % LY;
-		Xa[(((size_t)z * N + x) * 3 + c) * bY + y] = v;
-	};
-	fft_line<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	const int lineB = tid % LY, tB = tid / LY;    // y fastest
+	const bool actA = lineA < LY && y0 + lineA < bY, actB = tB < TPL && y0 + lineB < bY;
+	for (int idx = tid; idx < LY * N; idx += blockDim.x) {
+		const int line = idx / N, x = idx % N;
+		cx<T> v = mk<T>(0, 0);
+		if (x < bX && y0 + line < bY) {
+			const int d = cellmap[((size_t)z * bY + y0 + line) * bX + x];
+			if (d >= 0) {
+				cx<T> a = arg[3 * (size_t)d + c];
+				if (her) a = conjc(a);
+				v = mul(ccs[3 * mat[d] + c], a);
+			}
+		}
+		buf[line * LS + lpos(x, 1)] = v;
+	}
+	__syncthreads();
+	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
+	auto st = [&](int line, int x, cx<T> v) { Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line] = v; };
+	fft_line<T, false, RM>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 }
 
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
-template <typename T, bool INV>
+template <typename T, bool INV, int RM>
 __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
-	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out)
+	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, const double *gate)
 {
+	if (gate && gate[DK_STATUS] != 0) return;
 	extern __shared__ char smem_raw[];
 	const int N = p.n, TPL = N / p.rmax;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
@@ -280,28 +300,37 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	auto st = [&](int l, int k, cx<T> v) {
 		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
 	};
-	fft_line<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+	fft_line<T, INV, RM>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
 }
 
-// P3: block = LP consecutive x at one z; lines (pair, c) = pair*3 + c, contiguous boxY runs of Xb. Forward FFT-y into
-// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips of
-// IndexDmatrix_mv 87-103, cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
-template <typename T>
+// P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
+// which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
+// HBM once per matvec. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
+// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
+// cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
+template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed)
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
+	const double *gate)
 {
+	if (gate && gate[DK_STATUS] != 0) return;
 	extern __shared__ char smem_raw[];
-	const int N = p.n, TPL = N / p.rmax, NL = 3 * LP, LS = N + 1;
+	const int N = p.n, TPL = N / p.rmax, NL1 = 3 * LP, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nxt = gX / LP;
 	const int z = blockIdx.x / nxt, x0 = (blockIdx.x % nxt) * LP;
-	cx<T> *base = Xb + ((size_t)z * gX + x0) * 3 * bY;  // lines of this block are contiguous
+	const bool has2 = reduced && z != 0 && 2 * z != gZ;  // mirror plane handled by this block
+	const int zp = gZ - z;
+	const int nls = has2 ? 2 * NL1 : NL1;
+	cx<T> *base0 = Xb + ((size_t)z * gX + x0) * 3 * bY;
+	cx<T> *base1 = Xb + ((size_t)(has2 ? zp : z) * gX + x0) * 3 * bY;
 	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
-	const bool act = line < NL;
-	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? base[(size_t)l * bY + y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + y] = v; };
-	fft_line<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	const bool act = line < nls;
+	auto gptr = [&](int l) -> cx<T> * { return l < NL1 ? base0 + (size_t)l * bY : base1 + (size_t)(l - NL1) * bY; };
+	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? gptr(l)[y] : mk<T>(0, 0); };
+	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
+	fft_line<T, false, RM>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
 	__syncthreads();
 	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
 		const int pr = idx / N, y = idx % N, x = x0 + pr;
@@ -316,69 +345,75 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 			if (zi >= DsZ) zi = gZ - zi;
 		}
 		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
-		cx<T> f0 = dm[0], f1 = dm[1], f2 = dm[2], f3 = dm[3], f4 = dm[4], f5 = dm[5];
-		if (reduced) {
-			if (y >= DsY) {
-				f1 = scl(f1, (T)-1);
-				if (z >= DsZ) f2 = scl(f2, (T)-1);
-				else f4 = scl(f4, (T)-1);
+		const cx<T> d0 = dm[0], d1 = dm[1], d2 = dm[2], d3 = dm[3], d4 = dm[4], d5 = dm[5];
+		for (int sidx = 0; sidx < (has2 ? 2 : 1); sidx++) {
+			const int zz = sidx ? zp : z;
+			cx<T> f1 = d1, f2 = d2, f4 = d4;
+			if (reduced) {
+				if (y >= DsY) {
+					f1 = scl(f1, (T)-1);
+					if (zz >= DsZ) f2 = scl(f2, (T)-1);
+					else f4 = scl(f4, (T)-1);
+				}
+				else if (zz >= DsZ) {
+					f2 = scl(f2, (T)-1);
+					f4 = scl(f4, (T)-1);
+				}
 			}
-			else if (z >= DsZ) {
-				f2 = scl(f2, (T)-1);
-				f4 = scl(f4, (T)-1);
-			}
+			cx<T> *q = buf + (size_t)(sidx * NL1 + 3 * pr) * LS + lpos(y, 1);
+			const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
+			q[0] = add(add(mul(d0, a0), mul(f1, a1)), mul(f2, a2));
+			q[LS] = add(add(mul(f1, a0), mul(d3, a1)), mul(f4, a2));
+			q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(d5, a2));
 		}
-		cx<T> *q = buf + (size_t)(3 * pr) * LS + y;
-		const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
-		q[0] = add(add(mul(f0, a0), mul(f1, a1)), mul(f2, a2));
-		q[LS] = add(add(mul(f1, a0), mul(f3, a1)), mul(f4, a2));
-		q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(f5, a2));
 	}
 	__syncthreads();
-	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + y]; };
+	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
 	auto stg = [&](int l, int y, cx<T> v) {
-		if (y < bY) base[(size_t)l * bY + y] = v;
+		if (y < bY) gptr(l)[y] = v;
 	};
-	fft_line<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+	fft_line<T, true, RM>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
-// P5: block = (z, y0..y0+LY); stage 0 loads Xa[z][x][c][y] y-fastest, the last stage gathers to the dipoles along x:
-// result = arg + S.X (her: conj), partial ||result||^2 per block.
-template <typename T>
+// P5: block = (z, y0..y0+LY, c), component fastest in the block index; stage 0 loads Xa[z][x][c][y] y-fastest (runs
+// of LY), the results go to LDS and are gathered to the dipoles along x: result = arg + S.X (her: conj), partial
+// ||result||^2 per block.
+template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
 	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
-	double *__restrict__ partials)
+	double *__restrict__ partials, const double *gate)
 {
+	if (gate && gate[DK_STATUS] != 0) return;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
-	const int N = p.n, TPL = N / p.rmax, NL = 3 * LY, LS = N + 1;
+	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
-	const int z = blockIdx.x / nyt, y0 = (blockIdx.x % nyt) * LY;
+	const int c = blockIdx.x % 3, tile = blockIdx.x / 3;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int tid = threadIdx.x;
-	const int lineA = tid % NL, tA = tid / NL;    // y fastest
+	const int lineA = tid % LY, tA = tid / LY;    // y fastest
 	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest
-	const bool actA = tA < TPL && y0 + lineA % LY < bY, actB = lineB < NL && y0 + lineB % LY < bY;
+	const bool actA = tA < TPL && y0 + lineA < bY, actB = lineB < LY && y0 + lineB < bY;
 	double s = 0;
-	auto ld = [&](int line, int x) -> cx<T> {
-		const int c = line / LY, y = y0 + line % LY;
-		return Xa[(((size_t)z * N + x) * 3 + c) * bY + y];
-	};
-	auto st = [&](int line, int x, cx<T> v) {
-		if (x >= bX) return;
-		const int c = line / LY, y = y0 + line % LY;
-		const int d = cellmap[((size_t)z * bY + y) * bX + x];
-		if (d < 0) return;
+	auto ld = [&](int line, int x) -> cx<T> { return Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line]; };
+	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
+	fft_line<T, true, RM>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	__syncthreads();
+	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
+		const int line = idx / bX, x = idx % bX;
+		if (y0 + line >= bY) continue;
+		const int d = cellmap[((size_t)z * bY + y0 + line) * bX + x];
+		if (d < 0) continue;
 		cx<T> a = arg[3 * (size_t)d + c];
 		if (her) a = conjc(a);
-		cx<T> r = add(a, mul(ccs[3 * mat[d] + c], v));
+		cx<T> r = add(a, mul(ccs[3 * mat[d] + c], buf[line * LS + lpos(x, 1)]));
 		if (her) r = conjc(r);
 		res[3 * (size_t)d + c] = r;
 		s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-	};
-	fft_line<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	}
 	if (partials) {
 		double tt = block_sum<T>(s, red_lds);
 		if (threadIdx.x == 0) partials[blockIdx.x] = tt;
@@ -498,8 +533,9 @@ __global__ void __launch_bounds__(256) k_lincomb(cx<T> *out, const cx<T> *a, cx<
 // partials layout: [4][nblocks]
 template <typename T>
 __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int mode, size_t n, double *partials,
-	int nblk)
+	int nblk, const double *gate = nullptr)
 {
+	if (gate && gate[DK_STATUS] != 0) return;
 	__shared__ double red_lds[512];
 	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -649,11 +685,16 @@ template <typename T>
 static int set_lds_limits()
 {
 	const int big = 96 * 1024;  // largest dynamic LDS tile used (k_ydmul)
-	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 64));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
@@ -663,42 +704,92 @@ static int set_lds_limits()
 // ==================================================================================================================
 // matvec driver
 
+static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + c.LY * rowstride(gX)) * es; }
+static size_t lds_p24(const MvCfg &c, int gZ, size_t es) { return (size_t)(gZ + gZ * c.TC) * es; }
+static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
+{
+	return (size_t)(gY + 3 * (reduced ? 2 : 1) * c.LP * rowstride(gY)) * es;
+}
+static constexpr size_t kLdsMax = 96 * 1024;
+
+// whether a configuration is launchable for this operator and element size
+static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
+{
+	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
+	const int ml = op->reduced ? 2 : 1;
+	return c.LY >= 1 && c.TC >= 1 && c.LP >= 1 && op->gX % c.LP == 0 && c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
+		3 * ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax && lds_p24(c, op->gZ, es) <= kLdsMax &&
+		lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
+}
+
 template <typename T>
-static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her, hipEvent_t *ev = nullptr)
+static void launch_p1(adda_hip_op *op, const MvCfg &c, const cx<T> *arg, const cx<T> *ccs, int her)
+{
+	const int nyt = (op->bY + c.LY - 1) / c.LY;
+	auto k = c.qx.rmax <= 8 ? k_p1<T, 8> : k_p1<T, 16>;
+	hipLaunchKernelGGL(k, dim3(3 * op->bZ * nyt), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
+		op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX, op->bY, c.LY,
+		her, op->gate);
+}
+
+template <typename T, bool INV>
+static void launch_zfft(adda_hip_op *op, const MvCfg &c)
+{
+	const long Q = (long)op->gX * 3 * op->bY;
+	const cx<T> *in = (const cx<T> *)(INV ? op->Xb : op->Xa);
+	cx<T> *out = (cx<T> *)(INV ? op->Xa : op->Xb);
+	auto k = c.qz.rmax <= 8 ? k_zfft<T, INV, 8> : k_zfft<T, INV, 16>;
+	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC)), dim3(c.TC * (op->gZ / c.qz.rmax)),
+		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
+		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, op->gate);
+}
+
+template <typename T>
+static void launch_p3(adda_hip_op *op, const MvCfg &c, const cx<T> *D, int transposed)
+{
+	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
+	auto k = c.qy.rmax <= 8 ? k_p3<T, 8> : k_p3<T, 16>;
+	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP)), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
+		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
+		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed, op->gate);
+}
+
+template <typename T>
+static int launch_p5(adda_hip_op *op, const MvCfg &c, const cx<T> *arg, cx<T> *res, const cx<T> *ccs, int her,
+	double *partials)
+{
+	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = 3 * op->bZ * nyt;
+	auto k = c.qx.rmax <= 8 ? k_p5<T, 8> : k_p5<T, 16>;
+	hipLaunchKernelGGL(k, dim3(nblk5), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
+		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
+		op->bY, c.LY, her, partials, op->gate);
+	return nblk5;
+}
+
+// nblk_out (optional): receives the number of P5 blocks; with keep_partials the per-block ||result||^2 partials stay
+// in op->partials for a device-side reduction (no synchronisation).
+template <typename T>
+static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her, hipEvent_t *ev = nullptr,
+	bool keep_partials = false, int *nblk_out = nullptr)
 {
 	const cx<T> *ccs;
 	const cx<T> *D;
 	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
 	else { ccs = op->cc32; D = op->D32; }
 	if (!D) return ERR("FP32 matvec requested but the operator was created without the FP32 flag");
-	const int pi = sizeof(T) == 8 ? ADDA_F64 : ADDA_F32;
-	cx<T> *Xa = (cx<T> *)op->Xa, *Xb = (cx<T> *)op->Xb;
-	const int gX = op->gX, bX = op->bX, bY = op->bY, bZ = op->bZ, gY = op->gY, gZ = op->gZ;
+	const MvCfg &c = op->cfg[sizeof(T) == 8 ? ADDA_F64 : ADDA_F32];
 	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
-	const int LY = op->LYp[pi], TC = op->TCp[pi], LP = op->LPp[pi];
-	const int tplx = gX / op->qx.rmax, tply = gY / op->qy.rmax, tplz = gZ / op->qz.rmax;
-	const long Q = (long)gX * 3 * bY;
-	const int nyt = (bY + LY - 1) / LY;
-	const size_t lds1 = (size_t)(gX + 3 * LY * (gX + 1)) * sizeof(cx<T>);
-	const size_t lds2 = (size_t)(gZ + gZ * TC) * sizeof(cx<T>);
-	const size_t lds3 = (size_t)(gY + 3 * LP * (gY + 1)) * sizeof(cx<T>);
 	if (ev) HIPCHK(hipEventRecord(ev[0], op->stream));
-	hipLaunchKernelGGL(k_p1<T>, dim3(bZ * nyt), dim3(3 * LY * tplx), lds1, op->stream, arg, Xa, op->cellmap, op->mat,
-		ccs, op->qx, tw_of<T>(op->twx), bX, bY, LY, her);
+	launch_p1<T>(op, c, arg, ccs, her);
 	if (ev) HIPCHK(hipEventRecord(ev[1], op->stream));
-	hipLaunchKernelGGL((k_zfft<T, false>), dim3((unsigned)((Q + TC - 1) / TC)), dim3(TC * tplz), lds2, op->stream,
-		(const cx<T> *)Xa, Xb, op->qz, tw_of<T>(op->twz), Q, TC, bZ, gZ);
+	launch_zfft<T, false>(op, c);
 	if (ev) HIPCHK(hipEventRecord(ev[2], op->stream));
-	hipLaunchKernelGGL(k_p3<T>, dim3(gZ * (gX / LP)), dim3(3 * LP * tply), lds3, op->stream, Xb, D, op->qy,
-		tw_of<T>(op->twy), gX, bY, gZ, op->DsY, op->DsZ, LP, op->reduced, transposed);
+	launch_p3<T>(op, c, D, transposed);
 	if (ev) HIPCHK(hipEventRecord(ev[3], op->stream));
-	hipLaunchKernelGGL((k_zfft<T, true>), dim3((unsigned)((Q + TC - 1) / TC)), dim3(TC * tplz), lds2, op->stream,
-		(const cx<T> *)Xb, Xa, op->qz, tw_of<T>(op->twz), Q, TC, gZ, bZ);
+	launch_zfft<T, true>(op, c);
 	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
-	const int nblk5 = bZ * nyt;
-	hipLaunchKernelGGL(k_p5<T>, dim3(nblk5), dim3(3 * LY * tplx), lds1, op->stream, (const cx<T> *)Xa, arg, res,
-		op->cellmap, op->mat, ccs, op->qx, tw_of<T>(op->twx), bX, bY, LY, her,
-		inprod ? op->partials : (double *)nullptr);
+	const int nblk5 = launch_p5<T>(op, c, arg, res, ccs, her, (inprod || keep_partials) ? op->partials : nullptr);
+	if (nblk_out) *nblk_out = nblk5;
 	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
 	if (inprod) {
@@ -709,21 +800,425 @@ static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inpro
 	return 0;
 }
 
-// Tile parameters of the matvec kernels for element size es: the largest tiles (LY <= 8 lines, TC <= 16 columns,
-// LP <= 4 pairs dividing gridX) whose block fits 512 threads and whose LDS fits lds_cap bytes.
-static void choose_tiles(adda_hip_op *op, int pi, size_t es, size_t lds_cap)
+// Default tiles for element size es: the largest (LY <= 8, TC <= 16, LP <= 4 dividing gridX) fitting 512 threads and
+// 64 KB of LDS, with the default radix schedules of make_plan2.
+static void default_cfg(adda_hip_op *op, int pi, size_t es)
 {
+	MvCfg &c = op->cfg[pi];
+	make_plan2(op->gX, c.qx);
+	make_plan2(op->gY, c.qy);
+	make_plan2(op->gZ, c.qz);
+	const size_t cap = 64 * 1024;
 	const int gX = op->gX, gY = op->gY, gZ = op->gZ;
-	const int tplx = gX / op->qx.rmax, tply = gY / op->qy.rmax, tplz = gZ / op->qz.rmax;
-	int LY = 8;
-	while (LY > 1 && (3 * LY * tplx > 512 || (gX + 3 * LY * (gX + 1)) * es > lds_cap)) LY /= 2;
-	int TC = 16;
-	while (TC > 1 && (TC * tplz > 512 || (size_t)(gZ + gZ * TC) * es > lds_cap)) TC /= 2;
-	int LP = 4;
-	while (LP > 1 && (3 * LP * tply > 512 || (gY + 3 * LP * (gY + 1)) * es > lds_cap || gX % LP)) LP--;
-	op->LYp[pi] = LY;
-	op->TCp[pi] = TC;
-	op->LPp[pi] = LP;
+	const int tplx = gX / c.qx.rmax, tply = gY / c.qy.rmax, tplz = gZ / c.qz.rmax;
+	c.LY = es == 8 ? 16 : 8;  // runs of 128 bytes in the transposed Xa accesses
+	while (c.LY > 1 && (c.LY * tplx > 512 || lds_p15(c, gX, es) > cap)) c.LY /= 2;
+	c.TC = 16;
+	while (c.TC > 1 && (c.TC * tplz > 512 || lds_p24(c, gZ, es) > cap)) c.TC /= 2;
+	const int ml = op->reduced ? 2 : 1;
+	c.LP = 4;
+	while (c.LP > 1 && (3 * ml * c.LP * tply > 512 || lds_p3(c, gY, op->reduced, es) > cap || gX % c.LP)) c.LP--;
+}
+
+// ==================================================================================================================
+// device-resident Krylov iterations (scalar recurrences in FP64 on the GPU, see krylov_dev.h)
+
+// Fixed-order reduction of nrows rows of nblk partials by one block; row sums land in out[] (thread 0).
+__device__ void dk_reduce(const double *partials, int nblk, int nrows, double *out, double *red_lds)
+{
+	for (int k = 0; k < nrows; k++) {
+		double s = 0;
+		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[(size_t)k * nblk + i];
+		const double t = block_sum<double>(s, red_lds);
+		if (threadIdx.x == 0) out[k] = t;
+	}
+	__syncthreads();
+}
+
+#define DK_ACTIVE(st) if ((st)[DK_STATUS] != 0) return
+
+// out = st[ia]*a + st[ib]*b + st[ic]*c (complex coefficient slots; null vectors skipped); red: 0 none, 1 ||out||^2
+// (row 0), 2 self-bilinear sums re^2, im^2, re*im (rows 0-2).
+template <typename T>
+__global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const cx<T> *b, const cx<T> *c,
+	const double *st, int ia, int ib, int ic, size_t n, int red, double *partials)
+{
+	__shared__ double red_lds[512];
+	DK_ACTIVE(st);
+	const dcx ca = ld_c(st, ia), cb = ld_c(st, ib), cc = ld_c(st, ic);
+	double s0 = 0, s1 = 0, s2 = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double rx = 0, ry = 0;
+		if (a) { const cx<T> v = a[i]; rx += ca.x * v.x - ca.y * v.y; ry += ca.x * v.y + ca.y * v.x; }
+		if (b) { const cx<T> v = b[i]; rx += cb.x * v.x - cb.y * v.y; ry += cb.x * v.y + cb.y * v.x; }
+		if (c) { const cx<T> v = c[i]; rx += cc.x * v.x - cc.y * v.y; ry += cc.x * v.y + cc.y * v.x; }
+		const cx<T> r = mk<T>((T)rx, (T)ry);
+		out[i] = r;
+		const double ux = r.x, uy = r.y;
+		if (red == 1) s0 += ux * ux + uy * uy;
+		else if (red == 2) { s0 += ux * ux; s1 += uy * uy; s2 += ux * uy; }
+	}
+	if (red) {
+		const int nb = gridDim.x;
+		double t = block_sum<T>(s0, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
+		if (red == 2) {
+			t = block_sum<T>(s1, red_lds);
+			if (threadIdx.x == 0) partials[nb + blockIdx.x] = t;
+			t = block_sum<T>(s2, red_lds);
+			if (threadIdx.x == 0) partials[2 * nb + blockIdx.x] = t;
+		}
+	}
+}
+
+// x += st[ix1]*p + st[ix2]*s2v ; r = st[ir1]*r1 + st[ir2]*r2 ; partial ||r||^2 (BiCGStab and CGNR tails)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const cx<T> *s2v, cx<T> *r, const cx<T> *r1,
+	const cx<T> *r2, const double *st, int ix1, int ix2, int ir1, int ir2, size_t n, double *partials)
+{
+	__shared__ double red_lds[512];
+	DK_ACTIVE(st);
+	const dcx a1 = ld_c(st, ix1), a2 = ld_c(st, ix2), b1 = ld_c(st, ir1), b2 = ld_c(st, ir2);
+	double s0 = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		cx<T> xv = x[i];
+		double xr = xv.x, xi = xv.y;
+		{ const cx<T> v = p[i]; xr += a1.x * v.x - a1.y * v.y; xi += a1.x * v.y + a1.y * v.x; }
+		if (s2v) { const cx<T> v = s2v[i]; xr += a2.x * v.x - a2.y * v.y; xi += a2.x * v.y + a2.y * v.x; }
+		x[i] = mk<T>((T)xr, (T)xi);
+		double rr = 0, ri = 0;
+		{ const cx<T> v = r1[i]; rr += b1.x * v.x - b1.y * v.y; ri += b1.x * v.y + b1.y * v.x; }
+		{ const cx<T> v = r2[i]; rr += b2.x * v.x - b2.y * v.y; ri += b2.x * v.y + b2.y * v.x; }
+		const cx<T> rv = mk<T>((T)rr, (T)ri);
+		r[i] = rv;
+		s0 += (double)rv.x * rv.x + (double)rv.y * rv.y;
+	}
+	const double t = block_sum<T>(s0, red_lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// QMR_CS tail (src/iterative.c:1007-1029): pn = q0*pa + q1*pb + q2*v -> pdst; x += tau*pn; vt = vtilda/beta;
+// vtilda = vt; r = |s|^2 r + cr*vt; partial ||r||^2.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dk_qmr_upd(cx<T> *pdst, const cx<T> *pa, const cx<T> *pb, const cx<T> *v,
+	cx<T> *x, cx<T> *vtilda, cx<T> *r, const double *st, size_t n, double *partials)
+{
+	__shared__ double red_lds[512];
+	DK_ACTIVE(st);
+	const dcx q0 = ld_c(st, DK_SC + 30), q1 = ld_c(st, DK_SC + 32), q2 = ld_c(st, DK_SC + 34);
+	const dcx tau = ld_c(st, DK_SC + 36), ib = ld_c(st, DK_SC + 38), cr = ld_c(st, DK_SC + 40);
+	const double cs = st[DK_SC + 42];
+	double s0 = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		double px = 0, py = 0;
+		if (pa) { const cx<T> w = pa[i]; px += q0.x * w.x - q0.y * w.y; py += q0.x * w.y + q0.y * w.x; }
+		if (pb) { const cx<T> w = pb[i]; px += q1.x * w.x - q1.y * w.y; py += q1.x * w.y + q1.y * w.x; }
+		{ const cx<T> w = v[i]; px += q2.x * w.x - q2.y * w.y; py += q2.x * w.y + q2.y * w.x; }
+		const cx<T> pn = mk<T>((T)px, (T)py);
+		pdst[i] = pn;
+		const cx<T> xv = x[i];
+		x[i] = mk<T>((T)((double)xv.x + tau.x * pn.x - tau.y * pn.y), (T)((double)xv.y + tau.x * pn.y + tau.y * pn.x));
+		const cx<T> w = vtilda[i];
+		const cx<T> vt = mk<T>((T)(ib.x * w.x - ib.y * w.y), (T)(ib.x * w.y + ib.y * w.x));
+		vtilda[i] = vt;
+		const cx<T> rv0 = r[i];
+		const cx<T> rv = mk<T>((T)(cs * rv0.x + cr.x * vt.x - cr.y * vt.y), (T)(cs * rv0.y + cr.x * vt.y + cr.y * vt.x));
+		r[i] = rv;
+		s0 += (double)rv.x * rv.x + (double)rv.y * rv.y;
+	}
+	const double t = block_sum<T>(s0, red_lds);
+	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// ProgressReport (src/iterative.c:282-318) + the loop condition of IterativeSolver (1467) + the final tests
+// (1506-1510): consumes ||r_k||^2 partials.
+__device__ void dk_progress(double *st, double *hist, double inprodRp1)
+{
+	st[DK_INPRODRP1] = inprodRp1;
+	if (inprodRp1 <= st[DK_INPRODR]) { st[DK_INPRODR] = inprodRp1; st[DK_COUNTER] = 0; }
+	else st[DK_COUNTER] += 1;
+	const int niter = (int)st[DK_NITER];
+	if (hist && niter < (int)st[DK_HISTLEN]) hist[niter] = sqrt(st[DK_RSCALE] * inprodRp1);
+	st[DK_NITER] = niter + 1;
+	if (!(st[DK_INPRODR] > st[DK_EPSB])) st[DK_STATUS] = 1;
+	else if (niter + 1 > st[DK_MAXITER]) st[DK_STATUS] = 2;
+	else if (st[DK_COUNTER] > st[DK_MC]) st[DK_STATUS] = -2;
+}
+
+__global__ void __launch_bounds__(256) k_dk_progress(double *st, double *hist, const double *partials, int nblk)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 1, v, red_lds);
+	if (threadIdx.x == 0) dk_progress(st, hist, v[0]);
+}
+
+// QMR_CS scalar step 1 (src/iterative.c:962-982): breakdown test, alpha = v.Av, vtilda coefficients, theta, eta,
+// zetatilda.
+__global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *partials, int nblk, int niter)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const double omega_new = st[DK_SC + 13];
+	if (1 / (omega_new * omega_new) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 1; return; }
+	const dcx alpha = dmk(v[0], v[1]);
+	const dcx beta = ld_c(st, DK_SC + 0), s_old = ld_c(st, DK_SC + 6), s_new = ld_c(st, DK_SC + 4);
+	const double c_old = st[DK_SC + 10], c_new = st[DK_SC + 11], omega_old = st[DK_SC + 12];
+	st_c(st, DK_SC + 2, alpha);
+	// vtilda = cv0*vtilda + cv1*v + 1*Av
+	st_c(st, DK_SC + 24, niter == 1 ? dmk(0, 0) : dneg(beta));
+	st_c(st, DK_SC + 26, dneg(alpha));
+	st_c(st, DK_SC + 28, dmk(1, 0));
+	const dcx theta = dscl(dmul(dconj(s_old), beta), omega_old);
+	dcx eta = dscl(beta, c_old * c_new * omega_old);
+	eta = dadd(eta, dscl(dmul(alpha, dconj(s_new)), omega_new));
+	const dcx zt = dsub(dscl(alpha, c_new * omega_new), dscl(dmul(s_new, beta), c_old * omega_old));
+	st_c(st, DK_SC + 14, theta);
+	st_c(st, DK_SC + 16, eta);
+	st_c(st, DK_SC + 18, zt);
+	st[DK_SC + 12] = omega_new;  // omega_old = omega_new
+}
+
+// QMR_CS scalar step 2 (src/iterative.c:983-1029): beta, omega, Givens rotation, p/x/r coefficients.
+__global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *partials, int nblk, int niter)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 3, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx temp1 = dmk(v[0] - v[1], 2 * v[2]);
+	double dtmp1 = v[0] + v[1];
+	const dcx beta = dsqrt(temp1);
+	const double omega_new = sqrt(dtmp1) / dabs(beta);
+	const dcx zt = ld_c(st, DK_SC + 18), theta = ld_c(st, DK_SC + 14), eta = ld_c(st, DK_SC + 16);
+	const double dtmp2 = dabs2(zt);
+	const double zetaabs = sqrt(dtmp2 + dtmp1);
+	dtmp1 = sqrt(dtmp2);
+	const dcx zeta = dtmp1 < 1e-40 ? dmk(zetaabs, 0) : dscl(zt, zetaabs / dtmp1);
+	const double c_new_old = st[DK_SC + 11];
+	const double c_new = dtmp1 / zetaabs;
+	const dcx s_new_old = ld_c(st, DK_SC + 4);
+	const dcx s_new = ddiv(dscl(beta, omega_new), zeta);
+	const dcx temp4 = ddiv(dmk(1, 0), zeta);
+	st[DK_SC + 10] = c_new_old;  // c_old = c_new
+	st[DK_SC + 11] = c_new;
+	st_c(st, DK_SC + 6, s_new_old);  // s_old = s_new
+	st_c(st, DK_SC + 4, s_new);
+	st_c(st, DK_SC + 0, beta);
+	st[DK_SC + 13] = omega_new;
+	// p coefficients
+	st_c(st, DK_SC + 30, niter >= 3 ? dneg(dmul(theta, temp4)) : dmk(0, 0));
+	st_c(st, DK_SC + 32, niter >= 2 ? dneg(dmul(eta, temp4)) : dmk(0, 0));
+	st_c(st, DK_SC + 34, temp4);
+	dcx tautilda = ld_c(st, DK_SC + 8);
+	const dcx tau = dscl(tautilda, c_new);
+	tautilda = dneg(dmul(s_new, tautilda));
+	st_c(st, DK_SC + 8, tautilda);
+	st_c(st, DK_SC + 36, tau);
+	st_c(st, DK_SC + 38, ddiv(dmk(1, 0), beta));
+	st_c(st, DK_SC + 40, dscl(tautilda, c_new / omega_new));
+	st[DK_SC + 42] = dabs2(s_new);
+}
+
+// BiCGStab scalar steps (src/iterative.c:675-718). Slots: 0 ro_new, 2 ro_old, 4 alpha, 6 omega, 8 beta,
+// 20 p-coef on p, 22 p-coef on v, 24 one, 26 -alpha, 28 alpha, 30 omega (x), 32 -omega (r), 34 denumOmega.
+__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx ro_new = dmk(v[0], v[1]);
+	st_c(st, DK_SC + 0, ro_new);
+	if (niter == 1) { st_c(st, DK_SC + 20, dmk(0, 0)); st_c(st, DK_SC + 22, dmk(0, 0)); return; }
+	const dcx temp1 = dmul(ro_new, ld_c(st, DK_SC + 4)), temp2 = dmul(ld_c(st, DK_SC + 2), ld_c(st, DK_SC + 6));
+	if (dabs(temp2) / dabs(temp1) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 2; return; }
+	const dcx beta = ddiv(temp1, temp2);
+	st_c(st, DK_SC + 8, beta);
+	st_c(st, DK_SC + 20, beta);
+	st_c(st, DK_SC + 22, dneg(dmul(beta, ld_c(st, DK_SC + 6))));
+}
+
+__global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *partials, int nblk)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx temp1 = dmk(v[0], v[1]), ro_new = ld_c(st, DK_SC + 0);
+	if (dabs(temp1) / dabs(ro_new) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 3; return; }
+	const dcx alpha = ddiv(ro_new, temp1);
+	st_c(st, DK_SC + 4, alpha);
+	st_c(st, DK_SC + 26, dneg(alpha));
+	st_c(st, DK_SC + 28, alpha);
+}
+
+// ||s||^2 -> half-step convergence test (src/iterative.c:700-707): if converged, omega := 0 so the tail computes
+// x += alpha p and r := s (whose norm is the one reported).
+__global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 1, v, red_lds);
+	if (threadIdx.x != 0) return;
+	st[DK_SC + 36] = v[0] < st[DK_EPSB] ? 1 : 0;  // half-step flag
+}
+
+__global__ void __launch_bounds__(256) k_dk_bs_s4(double *st, const double *pt, int nblk_t)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_ACTIVE(st);
+	dk_reduce(pt, nblk_t, 1, v, red_lds);
+	if (threadIdx.x == 0) st[DK_SC + 34] = v[0];
+}
+
+__global__ void __launch_bounds__(256) k_dk_bs_s5(double *st, const double *partials, int nblk)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	dcx omega = dscl(dmk(v[0], v[1]), 1 / st[DK_SC + 34]);
+	if (st[DK_SC + 36] != 0) omega = dmk(0, 0);
+	else { st_c(st, DK_SC + 6, omega); st_c(st, DK_SC + 2, ld_c(st, DK_SC + 0)); }  // ro_old = ro_new
+	st_c(st, DK_SC + 30, omega);
+	st_c(st, DK_SC + 32, dneg(omega));
+}
+
+// CGNR scalar steps (src/iterative.c:745-766). Slots: 0 ro_new, 1 ro_old, 2 alpha, 4 beta (c), 6 one, 8 alpha (c),
+// 10 -alpha (c), 12 zero.
+__global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, int nblk_t, int niter)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_ACTIVE(st);
+	dk_reduce(pt, nblk_t, 1, v, red_lds);
+	if (threadIdx.x != 0) return;
+	st[DK_SC + 0] = v[0];
+	if (niter > 1) st_c(st, DK_SC + 4, dmk(v[0] / st[DK_SC + 1], 0));
+}
+
+__global__ void __launch_bounds__(256) k_dk_cg_s2(double *st, const double *pt, int nblk_t)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_ACTIVE(st);
+	dk_reduce(pt, nblk_t, 1, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const double alpha = st[DK_SC + 0] / v[0];
+	st[DK_SC + 2] = alpha;
+	st_c(st, DK_SC + 8, dmk(alpha, 0));
+	st_c(st, DK_SC + 10, dmk(-alpha, 0));
+	st[DK_SC + 1] = st[DK_SC + 0];  // ro_old = ro_new
+}
+
+struct dk_state {
+	double *st;    // [DK_NSLOTS]
+	double *hist;  // [histlen]
+	int histlen;
+};
+
+template <typename T>
+static int dot_partials(adda_hip_op *op, const void *a, const void *b, int mode, int *nb_out)
+{
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, op->stream, (const cx<T> *)a, (const cx<T> *)b, mode, n,
+		op->partials2, nb, op->gate);
+	*nb_out = nb;
+	return 0;
+}
+
+template <typename T>
+static void lc_dev(adda_hip_op *op, void *out, const void *a, const void *b, const void *c, const double *st, int ia,
+	int ib, int ic, int red)
+{
+	const size_t n = 3 * op->nd;
+	hipLaunchKernelGGL(k_dk_lc<T>, dim3(nblocks_red(n)), dim3(256), 0, op->stream, (cx<T> *)out, (const cx<T> *)a,
+		(const cx<T> *)b, (const cx<T> *)c, st, ia, ib, ic, n, red, op->partials2);
+}
+
+template <typename T>
+static int qmr_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *v, void *vtilda, void *Av,
+	void *p_old, void *p_new, void *x, void *r)
+{
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	double *st = k->st;
+	int nbd;
+	if (niter == 1 && matvec_ready) lc_dev<T>(op, Av, Av, nullptr, nullptr, st, DK_SC + 44, DK_SC + 44, DK_SC + 44, 0);
+	else if (matvec_t<T>(op, (const cx<T> *)v, (cx<T> *)Av, nullptr, 0)) return -1;
+	dot_partials<T>(op, v, Av, ADDA_DOT_BILIN, &nbd);
+	hipLaunchKernelGGL(k_dk_qmr_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd, niter);
+	lc_dev<T>(op, vtilda, niter == 1 ? nullptr : vtilda, v, Av, st, DK_SC + 24, DK_SC + 26, DK_SC + 28, 2);
+	hipLaunchKernelGGL(k_dk_qmr_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nb, niter);
+	void *pdst = niter == 1 ? p_new : p_old;
+	const void *pa = niter >= 3 ? p_old : nullptr, *pb = niter >= 2 ? p_new : nullptr;
+	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
+		(const cx<T> *)pb, (const cx<T> *)v, (cx<T> *)x, (cx<T> *)vtilda, (cx<T> *)r, st, n, op->partials2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+template <typename T>
+static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *x, void *r, void *p,
+	void *v, void *s, void *rt, void *t)
+{
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	double *st = k->st;
+	int nbd, nbt;
+	dot_partials<T>(op, r, rt, ADDA_DOT_CONJ, &nbd);
+	hipLaunchKernelGGL(k_dk_bs_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd, niter);
+	// p = beta p - beta omega v + r  (niter 1: p = r)
+	lc_dev<T>(op, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
+	if (niter == 1 && matvec_ready) lc_dev<T>(op, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
+	else if (matvec_t<T>(op, (const cx<T> *)p, (cx<T> *)v, nullptr, 0)) return -1;
+	dot_partials<T>(op, v, rt, ADDA_DOT_CONJ, &nbd);
+	hipLaunchKernelGGL(k_dk_bs_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd);
+	lc_dev<T>(op, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
+	hipLaunchKernelGGL(k_dk_bs_s3, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nb, k->hist);
+	if (matvec_t<T>(op, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_bs_s4, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt);
+	dot_partials<T>(op, s, t, ADDA_DOT_CONJ, &nbd);
+	hipLaunchKernelGGL(k_dk_bs_s5, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd);
+	hipLaunchKernelGGL(k_dk_xr<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
+		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
+		DK_SC + 32, n, op->partials2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+template <typename T>
+static int cgnr_iter_t(adda_hip_op *op, dk_state *k, int niter, void *x, void *r, void *p, void *Av)
+{
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	double *st = k->st;
+	int nbt;
+	if (matvec_t<T>(op, (const cx<T> *)r, (cx<T> *)(niter == 1 ? p : Av), nullptr, 1, nullptr, true, &nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_cg_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt, niter);
+	if (niter > 1) lc_dev<T>(op, p, p, Av, nullptr, st, DK_SC + 4, DK_SC + 6, DK_SC + 6, 0);
+	if (matvec_t<T>(op, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0, nullptr, true, &nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_cg_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt);
+	hipLaunchKernelGGL(k_dk_xr<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
+		(const cx<T> *)nullptr, (cx<T> *)r, (const cx<T> *)r, (const cx<T> *)Av, st, DK_SC + 8, DK_SC + 12, DK_SC + 6,
+		DK_SC + 10, n, op->partials2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+	HIPCHK(hipGetLastError());
+	return 0;
 }
 
 // ==================================================================================================================
@@ -758,13 +1253,14 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
 	if (gX < 2 * bX || gY < 2 * bY || gZ < 2 * bZ || gY % 2 || gZ % 2)
 		return ERR("FFT grid %dx%dx%d does not embed box %dx%dx%d (need even grid >= 2*box)", gX, gY, gZ, bX, bY, bZ);
-	if (!make_plan2(gX, op->qx) || !make_plan2(gY, op->qy) || !make_plan2(gZ, op->qz))
-		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
-	choose_tiles(op, ADDA_F64, 16, 64 * 1024);
-	choose_tiles(op, ADDA_F32, 8, 64 * 1024);
-	if (3 * op->LYp[0] * (gX / op->qx.rmax) > 512 || op->TCp[0] * (gZ / op->qz.rmax) > 512 ||
-		3 * op->LPp[0] * (gY / op->qy.rmax) > 512 || (size_t)(gX + 3 * (gX + 1)) * 16 > 96 * 1024 ||
-		(size_t)(gY + 3 * (gY + 1)) * 16 > 96 * 1024 || (size_t)(gZ + gZ) * 16 > 96 * 1024)
+	{
+		FftPlan2 t;
+		if (!make_plan2(gX, t) || !make_plan2(gY, t) || !make_plan2(gZ, t))
+			return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
+	}
+	default_cfg(op, ADDA_F64, 16);
+	default_cfg(op, ADDA_F32, 8);
+	if (!cfg_ok(op, op->cfg[ADDA_F64], 16) || !cfg_ok(op, op->cfg[ADDA_F32], 8))
 		return ERR("grid %dx%dx%d too large for the tiles of this build", gX, gY, gZ);
 	if (upload_twiddles(gX, op->twx) || upload_twiddles(gY, op->twy) || upload_twiddles(gZ, op->twz)) return -1;
 	// geometry: cell map and materials
@@ -794,10 +1290,11 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	HIPCHK(hipMalloc(&op->Xa, (size_t)3 * bZ * bY * gX * sizeof(cx<double>)));
 	HIPCHK(hipMalloc(&op->Xb, (size_t)3 * gZ * bY * gX * sizeof(cx<double>)));
 	{
-		const size_t nblk5 = (size_t)bZ * ((bY + 1) / 1);  // gather blocks with LY >= 1
+		const size_t nblk5 = (size_t)3 * bZ * bY;  // upper bound of gather blocks (LY >= 1)
 		op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);  // field sums need the most
 		if (nblk5 > op->npartials) op->npartials = nblk5;
 		HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
+		HIPCHK(hipMalloc(&op->partials2, (size_t)4 * kRedBlocks * sizeof(double)));
 	}
 	HIPCHK(hipMalloc(&op->red, 4 * sizeof(double)));
 	HIPCHK(hipHostMalloc((void **)&op->red_host, 4 * sizeof(double), hipHostMallocDefault));
@@ -881,7 +1378,7 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (!op) return;
 	(void)hipSetDevice(op->dev);
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
-	void *ptrs[] = {op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64, op->D32, op->Xa, op->Xb, op->partials, op->red,
+	void *ptrs[] = {op->partials2, op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64, op->D32, op->Xa, op->Xb, op->partials, op->red,
 		op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d, op->twz.f};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
@@ -1051,10 +1548,10 @@ extern "C" int adda_hip_dot(adda_hip_op *op, int prec, const void *a, const void
 	const int nb = nblocks_red(n);
 	if (prec == ADDA_F64)
 		hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(256), 0, op->stream, (const cx<double> *)a,
-			(const cx<double> *)b, mode, n, op->partials, nb);
+			(const cx<double> *)b, mode, n, op->partials, nb, (const double *)nullptr);
 	else
 		hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(256), 0, op->stream, (const cx<float> *)a,
-			(const cx<float> *)b, mode, n, op->partials, nb);
+			(const cx<float> *)b, mode, n, op->partials, nb, (const double *)nullptr);
 	HIPCHK(hipGetLastError());
 	double v[3];
 	if (reduce_finish(op, nb, 3, v)) return -1;
@@ -1188,6 +1685,351 @@ extern "C" double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pas
 	const double b[5] = {vec + idx + xa, xa + xb, 2 * xb + dm, xb + xa, xa + 2 * vec + idx};
 	if (pass >= 0 && pass < 5) return b[pass];
 	return b[0] + b[1] + b[2] + b[3] + b[4];
+}
+
+extern "C" void *adda_hip_dk_create(adda_hip_op *op, int hist_len)
+{
+	if (!op || hist_len < 1) { ERR("bad argument"); return nullptr; }
+	if (hipSetDevice(op->dev) != hipSuccess) { ERR("hipSetDevice failed"); return nullptr; }
+	dk_state *k = new dk_state();
+	k->histlen = hist_len;
+	if (hipMalloc(&k->st, DK_NSLOTS * sizeof(double)) != hipSuccess ||
+		hipMalloc(&k->hist, (size_t)hist_len * sizeof(double)) != hipSuccess) {
+		ERR("hipMalloc of the solver state failed");
+		if (k->st) (void)hipFree(k->st);
+		delete k;
+		return nullptr;
+	}
+	std::vector<double> z(DK_NSLOTS, 0.0);
+	z[DK_HISTLEN] = hist_len;
+	if (hipMemcpy(k->st, z.data(), DK_NSLOTS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+		ERR("state init failed");
+		return nullptr;
+	}
+	return k;
+}
+
+extern "C" int adda_hip_dk_free(adda_hip_op *op, void *state)
+{
+	if (!op || !state) return ERR("null argument");
+	dk_state *k = (dk_state *)state;
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	HIPCHK(hipFree(k->st));
+	HIPCHK(hipFree(k->hist));
+	delete k;
+	return 0;
+}
+
+extern "C" int adda_hip_dk_set(adda_hip_op *op, void *state, int slot, const double *vals, int n)
+{
+	if (!op || !state || !vals || slot < 0 || n < 0 || slot + n > DK_NSLOTS) return ERR("bad slot range");
+	dk_state *k = (dk_state *)state;
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipMemcpyAsync(k->st + slot, vals, n * sizeof(double), hipMemcpyHostToDevice, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	return 0;
+}
+
+extern "C" int adda_hip_dk_get(adda_hip_op *op, void *state, int slot, double *vals, int n)
+{
+	if (!op || !state || !vals || slot < 0 || n < 0 || slot + n > DK_NSLOTS) return ERR("bad slot range");
+	dk_state *k = (dk_state *)state;
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipMemcpyAsync(vals, k->st + slot, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	return 0;
+}
+
+extern "C" int adda_hip_dk_history(adda_hip_op *op, void *state, double *hist, int n)
+{
+	if (!op || !state || !hist) return ERR("null argument");
+	dk_state *k = (dk_state *)state;
+	if (n > k->histlen) n = k->histlen;
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipMemcpyAsync(hist, k->hist, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	return 0;
+}
+
+extern "C" int adda_hip_dk_qmr_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *v,
+	void *vtilda, void *Av, void *p_old, void *p_new, void *x, void *r)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) { op->gate = k->st; int rc = qmr_iter_t<double>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r); op->gate = nullptr; return rc; }
+	{ op->gate = k->st; int rc = qmr_iter_t<float>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r); op->gate = nullptr; return rc; }
+}
+
+extern "C" int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,
+	void *r, void *p, void *v, void *s, void *rtilda, void *t)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) { op->gate = k->st; int rc = bicgstab_iter_t<double>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t); op->gate = nullptr; return rc; }
+	{ op->gate = k->st; int rc = bicgstab_iter_t<float>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t); op->gate = nullptr; return rc; }
+}
+
+extern "C" int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, void *x, void *r, void *p,
+	void *Av)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) { op->gate = k->st; int rc = cgnr_iter_t<double>(op, k, niter, x, r, p, Av); op->gate = nullptr; return rc; }
+	{ op->gate = k->st; int rc = cgnr_iter_t<float>(op, k, niter, x, r, p, Av); op->gate = nullptr; return rc; }
+}
+
+// ==================================================================================================================
+// configuration strings and the autotuner
+
+static void plan_str(const FftPlan2 &p, char *b, size_t n)
+{
+	size_t o = 0;
+	b[0] = 0;
+	for (int i = 0; i < p.nst && o + 4 < n; i++) o += snprintf(b + o, n - o, i ? ".%d" : "%d", p.rad[i]);
+}
+
+static bool plan_parse(const char *s, int n, FftPlan2 &p)
+{
+	p.n = n;
+	p.nst = 0;
+	p.rmax = 1;
+	long prod = 1;
+	while (*s && *s != ' ' && p.nst < kMaxStages) {
+		char *e;
+		long r = strtol(s, &e, 10);
+		if (e == s || !(r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16)) return false;
+		p.rad[p.nst++] = (int)r;
+		if (r > p.rmax) p.rmax = (int)r;
+		prod *= r;
+		s = e;
+		if (*s == '.') s++;
+	}
+	return prod == n && p.nst > 0;
+}
+
+static void cfg_str(const MvCfg &c, char *b, size_t n)
+{
+	char x[64], y[64], z[64];
+	plan_str(c.qx, x, sizeof x);
+	plan_str(c.qy, y, sizeof y);
+	plan_str(c.qz, z, sizeof z);
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d", x, y, z, c.LY, c.TC, c.LP);
+}
+
+static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
+{
+	const char *px = strstr(s, "x="), *py = strstr(s, "y="), *pz = strstr(s, "z="), *ly = strstr(s, "LY="),
+		*tc = strstr(s, "TC="), *lp = strstr(s, "LP=");
+	if (!px || !py || !pz || !ly || !tc || !lp) return false;
+	if (!plan_parse(px + 2, op->gX, c.qx) || !plan_parse(py + 2, op->gY, c.qy) || !plan_parse(pz + 2, op->gZ, c.qz))
+		return false;
+	c.LY = atoi(ly + 3);
+	c.TC = atoi(tc + 3);
+	c.LP = atoi(lp + 3);
+	return true;
+}
+
+extern "C" int adda_hip_get_config(adda_hip_op *op, int prec, char *buf, size_t len)
+{
+	if (!op || !buf || len < 8) return ERR("bad argument");
+	cfg_str(op->cfg[prec == ADDA_F32], buf, len);
+	return 0;
+}
+
+extern "C" int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg)
+{
+	if (!op || !cfg) return ERR("null argument");
+	MvCfg c;
+	if (!cfg_parse(op, cfg, c)) return ERR("cannot parse matvec configuration '%s'", cfg);
+	if (!cfg_ok(op, c, prec == ADDA_F32 ? 8 : 16)) return ERR("configuration '%s' does not fit this grid", cfg);
+	op->cfg[prec == ADDA_F32] = c;
+	return 0;
+}
+
+// Candidate radix schedules for length n: the power of two split into radix-16/8/4/2 stages (at most 4 stages,
+// non-increasing and non-decreasing orders), the radix-5 and radix-3 stages after or before them.
+static void enum_plans(int n, std::vector<FftPlan2> &out)
+{
+	int m = n, k = 0, r3 = 0, r5 = 0;
+	while (m % 5 == 0) { r5++; m /= 5; }
+	while (m % 3 == 0) { r3++; m /= 3; }
+	while (m % 2 == 0) { k++; m /= 2; }
+	if (m != 1) return;
+	std::vector<std::vector<int>> pw;
+	std::vector<int> cur;
+	// non-increasing compositions of k into parts 4,3,2,1 (log2 radices), length <= 4
+	std::function<void(int, int)> rec = [&](int rem, int maxp) {
+		if (rem == 0) { pw.push_back(cur); return; }
+		if (cur.size() >= 4) return;
+		for (int q = maxp < rem ? maxp : rem; q >= 1; q--) {
+			cur.push_back(q);
+			rec(rem - q, q);
+			cur.pop_back();
+		}
+	};
+	if (k > 0) rec(k, 4);
+	else pw.push_back({});
+	for (auto &v : pw) {
+		for (int order = 0; order < 2; order++) {
+			std::vector<int> rad;
+			std::vector<int> v2 = v;
+			if (order) {
+				if (v2.size() < 2 || v2.front() == v2.back()) continue;
+				std::reverse(v2.begin(), v2.end());
+			}
+			for (int e : v2) rad.push_back(1 << e);
+			for (int place = 0; place < 2; place++) {
+				if (place && r3 + r5 == 0) continue;
+				std::vector<int> full;
+				if (place) { for (int i = 0; i < r5; i++) full.push_back(5); for (int i = 0; i < r3; i++) full.push_back(3); }
+				full.insert(full.end(), rad.begin(), rad.end());
+				if (!place) { for (int i = 0; i < r5; i++) full.push_back(5); for (int i = 0; i < r3; i++) full.push_back(3); }
+				if (full.empty() || (int)full.size() > kMaxStages) continue;
+				FftPlan2 p;
+				p.n = n;
+				p.nst = (int)full.size();
+				p.rmax = 1;
+				for (int i = 0; i < p.nst; i++) { p.rad[i] = full[i]; if (full[i] > p.rmax) p.rmax = full[i]; }
+				out.push_back(p);
+			}
+		}
+	}
+}
+
+template <typename T>
+static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
+{
+	const int pi = sizeof(T) == 8 ? ADDA_F64 : ADDA_F32;
+	const size_t es = sizeof(cx<T>);
+	const cx<T> *ccs;
+	const cx<T> *D;
+	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
+	else { ccs = op->cc32; D = op->D32; }
+	if (!D) return ERR("autotune of FP32 needs an operator created with the FP32 flag");
+	cx<T> *a = nullptr, *r = nullptr;
+	HIPCHK(hipMalloc(&a, 3 * op->nd * es));
+	HIPCHK(hipMalloc(&r, 3 * op->nd * es));
+	HIPCHK(hipMemsetAsync(a, 0, 3 * op->nd * es, op->stream));
+	hipEvent_t e0, e1;
+	HIPCHK(hipEventCreate(&e0));
+	HIPCHK(hipEventCreate(&e1));
+	MvCfg best = op->cfg[pi];
+	auto timeit = [&](auto &&fn) -> float {
+		for (int w = 0; w < 2; w++) fn();
+		(void)hipEventRecord(e0, op->stream);
+		for (int i = 0; i < reps; i++) fn();
+		(void)hipEventRecord(e1, op->stream);
+		(void)hipEventSynchronize(e1);
+		float ms = 0;
+		(void)hipEventElapsedTime(&ms, e0, e1);
+		return ms / reps;
+	};
+	std::vector<FftPlan2> px, py, pz;
+	enum_plans(op->gX, px);
+	enum_plans(op->gY, py);
+	enum_plans(op->gZ, pz);
+	float tx = 1e30f, ty = 1e30f, tz = 1e30f;
+	int ncand = 0;
+	// x passes (P1 + P5): radix schedule of x and LY
+	for (auto &q : px)
+		for (int LY : {2, 4, 8, 16, 32}) {
+			MvCfg c = best;
+			c.qx = q;
+			c.LY = LY;
+			if (!cfg_ok(op, c, es)) continue;
+			ncand++;
+			float t = timeit([&] { launch_p1<T>(op, c, a, ccs, 0); launch_p5<T>(op, c, a, r, ccs, 0, nullptr); });
+			if (t < tx) { tx = t; best.qx = q; best.LY = LY; }
+		}
+	// z passes (P2 + P4): radix schedule of z and TC
+	for (auto &q : pz)
+		for (int TC : {4, 8, 16, 32, 64}) {
+			MvCfg c = best;
+			c.qz = q;
+			c.TC = TC;
+			if (!cfg_ok(op, c, es)) continue;
+			ncand++;
+			float t = timeit([&] { launch_zfft<T, false>(op, c); launch_zfft<T, true>(op, c); });
+			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
+		}
+	// y pass (P3): radix schedule of y and LP
+	for (auto &q : py)
+		for (int LP : {1, 2, 3, 4, 6, 8}) {
+			MvCfg c = best;
+			c.qy = q;
+			c.LP = LP;
+			if (!cfg_ok(op, c, es)) continue;
+			ncand++;
+			float t = timeit([&] { launch_p3<T>(op, c, D, 0); });
+			if (t < ty) { ty = t; best.qy = q; best.LP = LP; }
+		}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	HIPCHK(hipStreamSynchronize(op->stream));
+	HIPCHK(hipFree(a));
+	HIPCHK(hipFree(r));
+	HIPCHK(hipGetLastError());
+	if (tx > 1e29f || ty > 1e29f || tz > 1e29f) return ERR("autotune found no launchable configuration");
+	op->cfg[pi] = best;
+	if (report) {
+		char cs[256];
+		cfg_str(best, cs, sizeof cs);
+		snprintf(report, rlen, "%s  (%d candidates; x passes %.1f us, z passes %.1f us, y pass %.1f us)", cs, ncand,
+			tx * 1e3, tz * 1e3, ty * 1e3);
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_autotune(adda_hip_op *op, int prec, int reps, const char *cache_path, char *report,
+	size_t report_len)
+{
+	if (!op) return ERR("null operator");
+	HIPCHK(hipSetDevice(op->dev));
+	if (reps < 1) reps = 3;
+	char key[160];
+	snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d reduced %d prec %s", op->gX, op->gY, op->gZ, op->bX, op->bY,
+		op->bZ, op->reduced, prec == ADDA_F32 ? "fp32" : "fp64");
+	if (cache_path) {
+		FILE *f = fopen(cache_path, "r");
+		if (f) {
+			char line[1024];
+			while (fgets(line, sizeof line, f)) {
+				// lines: {"key": "<key>", "config": "<cfg>", ...}
+				char *k = strstr(line, "\"key\": \""), *c = strstr(line, "\"config\": \"");
+				if (!k || !c) continue;
+				k += 8;
+				c += 11;
+				char *ke = strchr(k, '"'), *ce = strchr(c, '"');
+				if (!ke || !ce) continue;
+				*ke = 0;
+				*ce = 0;
+				if (strcmp(k, key) == 0 && adda_hip_set_config(op, prec, c) == 0) {
+					fclose(f);
+					if (report) snprintf(report, report_len, "%s  (from cache)", c);
+					return 0;
+				}
+			}
+			fclose(f);
+		}
+	}
+	char rep[512];
+	int rc = prec == ADDA_F32 ? tune_t<float>(op, reps, rep, sizeof rep) : tune_t<double>(op, reps, rep, sizeof rep);
+	if (rc) return rc;
+	if (report) snprintf(report, report_len, "%s", rep);
+	if (cache_path) {
+		FILE *f = fopen(cache_path, "a");
+		if (f) {
+			char cs[256];
+			cfg_str(op->cfg[prec == ADDA_F32], cs, sizeof cs);
+			fprintf(f, "{\"key\": \"%s\", \"config\": \"%s\", \"report\": \"%s\"}\n", key, cs, rep);
+			fclose(f);
+		}
+	}
+	return 0;
 }
 
 extern "C" int adda_hip_sync(adda_hip_op *op)

@@ -62,20 +62,29 @@ template <typename T, bool INV, int R> __device__ __forceinline__ void dftR2(cx<
 	else dftR<T, INV, R>(v);
 }
 
-// Number of butterflies per thread of a stage of radix R (compile-time bound for RMAX <= 16).
-template <int R> constexpr int maxu() { return (16 + R - 1) / R; }
+// LDS position of element k of a line: lines stored as rows (KS == 1) get one pad element after every 16, so the
+// stride-R writes of a Stockham stage (R up to 16 complex numbers apart) fall into different LDS banks; column tiles
+// (KS = number of columns, consecutive threads on consecutive columns) need no padding.
+__device__ __forceinline__ int lpos(int k, int KS) { return KS == 1 ? k + (k >> 4) : k * KS; }
+// Row stride of a padded line of length n (one extra element so consecutive lines start in different banks).
+__host__ __device__ __forceinline__ int rowstride(int n) { return n + n / 16 + 1; }
+
+// Number of butterflies per thread of a stage of radix R when the largest radix of the plan is at most RM: a line has
+// N/RMAX >= N/RM threads, so ceil(RM/R) butterflies per thread suffice. Kernels are instantiated for RM = 16 and
+// RM = 8; the radix-8 variant holds half as many values per thread (fewer registers, more waves per SIMD).
+template <int R, int RM> constexpr int maxu() { return (RM + R - 1) / R; }
 
 // One stage. FIRST: inputs come from ld(line, k); LAST: outputs go to st(line, k, v); otherwise through LDS
-// (element k of line l at buf[k*KS + l*LS]). Twiddles from the LDS table tws[e] = exp(-2 pi i e/N).
+// (element k of line l at buf[lpos(k, KS) + l*LS]). Twiddles from the LDS table tws[e] = exp(-2 pi i e/N).
 // Control flow is uniform: every thread of the block executes the same barriers; `active` only predicates the work
 // (a wave must never reach a barrier from two branches). A non-first stage synchronises between reading LDS and
 // writing its outputs (the LAST stage's Store may target the same LDS), and a non-last stage synchronises after
 // writing.
-template <typename T, bool INV, int R, bool FIRST, bool LAST, class Ld, class St>
+template <typename T, bool INV, int R, bool FIRST, bool LAST, int RM, class Ld, class St>
 __device__ __forceinline__ void stage2(cx<T> *buf, const cx<T> *tws, int N, int Ns, int KS, int LS, int line, int t,
 	int TPL, bool active, Ld &ld, St &st)
 {
-	constexpr int MU = maxu<R>();
+	constexpr int MU = maxu<R, RM>();
 	const int nb = N / R, twstep = N / (Ns * R);
 	cx<T> v[MU][R];
 #pragma unroll
@@ -83,7 +92,7 @@ __device__ __forceinline__ void stage2(cx<T> *buf, const cx<T> *tws, int N, int 
 		const int j = t + u * TPL;
 		if (active && (MU == 1 || j < nb)) {
 #pragma unroll
-			for (int q = 0; q < R; q++) v[u][q] = FIRST ? ld(line, j + q * nb) : buf[(j + q * nb) * KS + line * LS];
+			for (int q = 0; q < R; q++) v[u][q] = FIRST ? ld(line, j + q * nb) : buf[lpos(j + q * nb, KS) + line * LS];
 			if (!FIRST) {
 				const int k = j % Ns;
 #pragma unroll
@@ -113,26 +122,28 @@ __device__ __forceinline__ void stage2(cx<T> *buf, const cx<T> *tws, int N, int 
 			const int j = t + u * TPL;
 			if (active && (MU == 1 || j < nb)) {
 				const int k = j % Ns;
-				const int d = ((j / Ns) * Ns * R + k) * KS + line * LS;
+				const int d = (j / Ns) * Ns * R + k;
 #pragma unroll
-				for (int q = 0; q < R; q++) buf[d + q * Ns * KS] = v[u][q];
+				for (int q = 0; q < R; q++) buf[lpos(d + q * Ns, KS) + line * LS] = v[u][q];
 			}
 		}
 		__syncthreads();
 	}
 }
 
-template <typename T, bool INV, bool FIRST, bool LAST, class Ld, class St>
+template <typename T, bool INV, bool FIRST, bool LAST, int RM, class Ld, class St>
 __device__ __forceinline__ void stage_dispatch(int R, cx<T> *buf, const cx<T> *tws, int N, int Ns, int KS, int LS,
 	int line, int t, int TPL, bool active, Ld &ld, St &st)
 {
 	switch (R) {
-		case 16: stage2<T, INV, 16, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
-		case 8: stage2<T, INV, 8, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
-		case 4: stage2<T, INV, 4, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
-		case 2: stage2<T, INV, 2, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
-		case 3: stage2<T, INV, 3, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
-		default: stage2<T, INV, 5, FIRST, LAST>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 16:
+			if constexpr (RM >= 16) stage2<T, INV, 16, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st);
+			break;
+		case 8: stage2<T, INV, 8, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 4: stage2<T, INV, 4, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 2: stage2<T, INV, 2, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		case 3: stage2<T, INV, 3, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		default: stage2<T, INV, 5, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
 	}
 }
 
@@ -140,22 +151,22 @@ __device__ __forceinline__ void stage_dispatch(int R, cx<T> *buf, const cx<T> *t
 // The first stage, which reads global memory, uses the thread mapping (lineA, tA, activeA); the other stages, the
 // last of which writes global memory, use (lineB, tB, activeB) -- so each global access can be mapped for contiguity
 // (e.g. x-fastest for a dipole gather, y-fastest for a transposed store). The plan is uniform across the block.
-template <typename T, bool INV, class Ld, class St>
+template <typename T, bool INV, int RM, class Ld, class St>
 __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS, int lineA, int tA,
 	bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 {
 	const int N = p.n, TPL = N / p.rmax;
 	if (p.nst == 1) {
-		stage_dispatch<T, INV, true, true>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
+		stage_dispatch<T, INV, true, true, RM>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
 		return;
 	}
-	stage_dispatch<T, INV, true, false>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
+	stage_dispatch<T, INV, true, false, RM>(p.rad[0], buf, tws, N, 1, KS, LS, lineA, tA, TPL, activeA, ld, st);
 	int Ns = p.rad[0];
 	for (int s = 1; s < p.nst - 1; s++) {
-		stage_dispatch<T, INV, false, false>(p.rad[s], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
+		stage_dispatch<T, INV, false, false, RM>(p.rad[s], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
 		Ns *= p.rad[s];
 	}
-	stage_dispatch<T, INV, false, true>(p.rad[p.nst - 1], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
+	stage_dispatch<T, INV, false, true, RM>(p.rad[p.nst - 1], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
 }
 
 // Host-side plan: radices 5 and 3, then the power of two as radix-16 stages with one radix-8/4/2 remainder

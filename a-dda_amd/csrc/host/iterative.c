@@ -290,6 +290,151 @@ done:
 #undef CKG
 }
 
+/* Device-resident form of krylov() for QMR_CS, BiCGStab and CGNR: CalcInitField and PHASE_INIT as above (host,
+ * synchronising, once), then whole iterations enqueued with adda_hip_dk_*_iter -- whose FP64 scalar recurrences run
+ * on the GPU -- in chunks of DK_CHUNK, polling the device status slot after each chunk. Iterations enqueued after
+ * convergence are no-ops on the device. Results are identical in exact arithmetic to krylov() (same recurrences, same
+ * order of operations per vector element; reductions differ only in summation order). */
+#define DK_CHUNK 16
+static int krylov_dev(kctx *k, int method, double eps, int maxiter, int init_field, int verbose, const void *b, void *x,
+	double *hist, int hist_len, int *niter_out, double *final_rel)
+{
+	const int mc_tab[5] = {50000, 30000, 10, 50000, 10};
+	int rc = 0;
+	void *r = kvec(k), *Av = kvec(k), *p = kvec(k), *v1 = kvec(k), *v2 = kvec(k), *v3 = kvec(k);
+	void *st = adda_hip_dk_create(k->op, hist_len > 1 ? hist_len : 2);
+	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+	double temp;
+#define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
+	CKG(dotp(k, b, NULL, ADDA_DOT_NORM2, NULL, &temp));
+	if (!(temp > 0)) { rc = host_err(__FILE__, __LINE__, "right-hand side is zero"); goto done; }
+	double resid_scale = 1 / temp, epsB = eps * eps * temp, inprodR = 0;
+	int matvec_ready = 0;
+	if (init_field == ADDA_IF_AUTO) { /* src/iterative.c:1347-1368 */
+		CKG(mv(k, b, Av, NULL, 0));
+		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
+		if (temp < inprodR) {
+			CKG(lc(k, x, NULL, 0, NULL, 0, NULL, 0, 0, NULL));
+			CKG(lc(k, r, b, 1, NULL, 0, NULL, 0, 0, NULL));
+			inprodR = temp;
+			matvec_ready = 1;
+		}
+		else CKG(lc(k, x, b, 1, NULL, 0, NULL, 0, 0, NULL));
+	}
+	else if (init_field == ADDA_IF_ZERO) {
+		CKG(lc(k, x, NULL, 0, NULL, 0, NULL, 0, 0, NULL));
+		CKG(lc(k, r, b, 1, NULL, 0, NULL, 0, 0, NULL));
+		inprodR = temp;
+	}
+	else {
+		CKG(lc(k, x, b, 1, NULL, 0, NULL, 0, 0, NULL));
+		CKG(mv(k, x, Av, NULL, 0));
+		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
+	}
+	if (hist && hist_len > 0) hist[0] = sqrt(resid_scale * inprodR);
+	if (verbose) printf("RE_000 = %.10E\n", sqrt(resid_scale * inprodR));
+	{
+		double base[11] = {inprodR, 0, epsB, resid_scale, 1, 0, inprodR > epsB ? 0 : 1, maxiter, mc_tab[method],
+			hist_len > 1 ? hist_len : 2, 0};
+		if (maxiter < 1 && base[6] == 0) base[6] = 2;
+		CKG(adda_hip_dk_set(k->op, st, 0, base, 11));
+	}
+	/* PHASE_INIT, written into the method slots (layout: krylov_dev.h / adda_hip.hip k_dk_* kernels) */
+	if (method == ADDA_IT_QMR_CS) {
+		dcx t;
+		CKG(dotp(k, r, NULL, ADDA_DOT_SELF_BILIN, &t, NULL));
+		dcx beta = csqrt(t);
+		double omega_new = sqrt(inprodR) / cabs(beta);
+		CKG(lc(k, v1, r, 1 / beta, NULL, 0, NULL, 0, 0, NULL));
+		dcx tautilda = omega_new * beta, ib = 1 / beta;
+		double sc[46] = {0};
+		sc[0] = creal(beta); sc[1] = cimag(beta);
+		sc[8] = creal(tautilda); sc[9] = cimag(tautilda);
+		sc[10] = 1; sc[11] = 1; sc[12] = 0; sc[13] = omega_new;
+		sc[44] = creal(ib); sc[45] = cimag(ib);
+		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 46));
+	}
+	else if (method == ADDA_IT_BICGSTAB) {
+		CKG(lc(k, v3, r, 1, NULL, 0, NULL, 0, 0, NULL)); /* r~ = r_0 */
+		double sc[26] = {0};
+		sc[24] = 1;
+		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 26));
+	}
+	else {
+		double sc[14] = {0};
+		sc[6] = 1;
+		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 14));
+	}
+	{
+		void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;
+		int it = 1;
+		double status = 0;
+		while (status == 0) {
+			for (int c = 0; c < DK_CHUNK; c++, it++) {
+				if (method == ADDA_IT_QMR_CS) {
+					CKG(adda_hip_dk_qmr_iter(k->op, k->prec, st, it, matvec_ready, v, vtilda, Av, p_old, p_new, x, r));
+					if (it >= 2) { void *t = p_old; p_old = p_new; p_new = t; }
+					{ void *t = v; v = vtilda; vtilda = t; }
+					k->nmv += !(it == 1 && matvec_ready);
+				}
+				else if (method == ADDA_IT_BICGSTAB) {
+					/* v = vec1, s = vec2, rtilda = vec3, t = Avecbuffer (src/iterative.c:660-662) */
+					CKG(adda_hip_dk_bicgstab_iter(k->op, k->prec, st, it, matvec_ready, x, r, p, v1, v2, v3, Av));
+					k->nmv += 2 - (it == 1 && matvec_ready);
+				}
+				else {
+					CKG(adda_hip_dk_cgnr_iter(k->op, k->prec, st, it, x, r, p, Av));
+					k->nmv += 2;
+				}
+			}
+			CKG(adda_hip_dk_get(k->op, st, ADDA_DK_STATUS, &status, 1));
+		}
+		double fin[11];
+		CKG(adda_hip_dk_get(k->op, st, 0, fin, 11));
+		int niter = (int)fin[4];
+		/* matvecs of the enqueued iterations that the device skipped are not counted */
+		int skipped = it - niter;
+		if (method == ADDA_IT_QMR_CS) k->nmv -= skipped;
+		else k->nmv -= 2 * skipped;
+		inprodR = fin[0];
+		if (hist && hist_len > 1 && niter > 1) {
+			int n = niter < hist_len ? niter : hist_len;
+			double *tmp = malloc(n * sizeof(double));
+			if (!tmp) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
+			CKG(adda_hip_dk_history(k->op, st, tmp, n));
+			for (int i = 1; i < n; i++) hist[i] = tmp[i];
+			free(tmp);
+		}
+		if (verbose)
+			for (int i = 1; i < niter && hist && i < hist_len; i++) printf("RE_%03d = %.10E\n", i, hist[i]);
+		*niter_out = niter - 1;
+		*final_rel = sqrt(resid_scale * inprodR);
+		if (status == 2) rc = 1;
+		else if (status == -1) rc = host_err(__FILE__, __LINE__, "%s breakdown (test %d failed)",
+			method == ADDA_IT_QMR_CS ? "QMR_CS" : method == ADDA_IT_BICGSTAB ? "BiCGStab" : "CGNR", (int)fin[10]);
+		else if (status == -2) rc = host_err(__FILE__, __LINE__,
+			"Residual norm haven't decreased for maximum allowed number of iterations (%d)", mc_tab[method]) - 1;
+	}
+done:
+	{
+		void *vs[] = {r, Av, p, v1, v2, v3};
+		for (int i = 0; i < 6; i++)
+			if (vs[i]) adda_hip_vec_free(k->op, vs[i]);
+		if (st) adda_hip_dk_free(k->op, st);
+	}
+	return rc;
+#undef CKG
+}
+
+/* dispatch: device-resident recurrences where they exist, unless the caller asks for the host form */
+static int krylov_any(kctx *k, int method, int host_scalars, double eps, int maxiter, int init_field, int verbose,
+	const void *b, void *x, double *hist, int hist_len, int *niter_out, double *final_rel)
+{
+	if (!host_scalars && (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICGSTAB || method == ADDA_IT_CGNR))
+		return krylov_dev(k, method, eps, maxiter, init_field, verbose, b, x, hist, hist_len, niter_out, final_rel);
+	return krylov(k, method, eps, maxiter, init_field, verbose, b, x, hist, hist_len, niter_out, final_rel);
+}
+
 int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const void *d_einc, void *d_x, void *d_pvec,
 	double *resid_hist, int hist_len, adda_solve_stats *st)
 {
@@ -303,8 +448,8 @@ int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const vo
 	if (opt->precision == ADDA_PREC_FP64) {
 		kctx k = {op, ADDA_F64, 0};
 		double rel = 0;
-		rc = krylov(&k, opt->method, opt->eps, opt->maxiter, opt->init_field, opt->verbose, d_pvec, d_x, resid_hist,
-			hist_len, &st->niter, &rel);
+		rc = krylov_any(&k, opt->method, opt->host_scalars, opt->eps, opt->maxiter, opt->init_field, opt->verbose,
+			d_pvec, d_x, resid_hist, hist_len, &st->niter, &rel);
 		st->nmatvec = k.nmv;
 		st->resid = rel;
 		if (rc < 0) { st->rc = rc; return rc; }
@@ -352,7 +497,8 @@ int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const vo
 			/* the inner history starts one slot early: its entry 0 (= 1, the normalized start) overwrites the last
 			 * entry of the previous step, which holds the same residual */
 			int hp = hist_pos > 0 ? hist_pos - 1 : 0;
-			int irc = krylov(&k, opt->method, tgt, opt->maxiter, ADDA_IF_ZERO, 0, r32, e32,
+			int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
+			int irc = krylov_any(&k, opt->method, opt->host_scalars, tgt, cap, ADDA_IF_ZERO, 0, r32, e32,
 				resid_hist ? resid_hist + hp : NULL, resid_hist ? hist_len - hp : 0, &ni, &rel);
 			if (resid_hist) hist_pos = hp + 1;
 			if (irc < 0) {

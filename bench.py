@@ -5,10 +5,11 @@ Workload (the grid-256 configuration): homogeneous sphere on a 128^3 box (1,099,
 polarizability, plane wave along z, Y polarization. Synthetic by construction (the particle is generated, nothing is
 read).
 
-One step = one complete time-to-solution run of the FP32-mixed solver (FP32 inner QMR_CS, FP64 defect correction to
-||r||/||b|| <= 1e-10, then Cext), inputs (incident field, D-matrix) already resident in HBM. The plain FP64 QMR_CS
-solve is timed once beside it, as are BiCGStab in both forms, the matvec throughput of both precisions, and the
-per-pass kernel times of the matvec (HIP events on the operator's stream) that give the roofline figures.
+One step = one complete time-to-solution run of the FP32-mixed solver (FP32 inner BiCGStab, FP64 defect correction
+to ||r||/||b|| <= 1e-10, then Cext), inputs (incident field, D-matrix) already resident in HBM. The plain FP64
+BiCGStab solve is timed beside it, as are QMR_CS in both forms, the matvec throughput of both precisions, and the
+per-pass kernel times of the matvec (HIP events on the operator's stream) that give the roofline figures. The matvec
+radix schedules and tiles are picked by the autotuner first (its time is reported, not counted).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--box 128]
 """
@@ -40,7 +41,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--box", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-bicgstab", action="store_true")
+    ap.add_argument("--no-qmr", action="store_true")
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="profiling mode: autotune, 20 matvecs per precision and one 52.8 MB device copy (the "
+                         "calibration kernel for HBM counters), no solves; prints the matvec figures only")
+    ap.add_argument("--qmr-max-inner", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
@@ -65,6 +70,10 @@ def main():
     prop, pol = np.array([0.0, 0, 1]), np.array([0.0, 1, 0])
     pr.set_polarization(prop, pol)
     op = pr.op
+    t0 = time.perf_counter()
+    tune = {name: op.autotune(prec, reps=3) for prec, name in ((A.F64, "fp64"), (A.F32, "fp32"))}
+    t_tune = time.perf_counter() - t0
+    log(f"autotune {t_tune:.2f} s: " + "; ".join(f"{k}: {v}" for k, v in tune.items()))
 
     # ---- matvec throughput and per-pass timing (HIP events on the operator's stream) --------------------------------
     mv = {}
@@ -81,44 +90,56 @@ def main():
             f"{mv[name]['total_bytes']/tot/1e6:.0f} GB/s algorithmic; passes (us): "
             + ", ".join(f"{p*1e3:.1f}" for p in passes))
 
-    # ---- plain FP64 QMR_CS (reference solve) ------------------------------------------------------------------------
-    def run(method, precision):
+    def run(method, precision, **kw):
         op.sync()
         t = time.perf_counter()
-        res = pr.solve(method, eps=1e-10, precision=precision)
+        res = pr.solve(method, eps=1e-10, precision=precision, **kw)
         cext = pr.cext()
         dt = time.perf_counter() - t
         return dt, cext, res
 
-    t64, cext64, r64 = run("qmr", A.PREC_FP64)
-    log(f"FP64 QMR_CS: {t64:.3f} s, {r64['niter']} iterations, {r64['nmatvec']} matvecs, Cext {cext64:.10g}")
+    if args.matvec_only:
+        a, b = op.alloc(A.F64), op.alloc(A.F64)
+        A.check(A.hip().adda_hip_vec_convert(op.h, b, A.F64, a, A.F64))  # plain copy of 3*nd*16 bytes
+        op.sync()
+        print(json.dumps({"mode": "matvec-only", "copy_bytes": 3 * nd * 16, "matvec": {
+            k: {"us": round(v["ms"] * 1e3, 2), "pass_us": [round(x * 1e3, 2) for x in v["passes"]],
+                "bytes": v["total_bytes"], "pass_bytes": v["bytes"]} for k, v in mv.items()}, "config": tune}),
+              flush=True)
+        pr.close()
+        return
 
-    # ---- timed steps: FP32-mixed QMR_CS time-to-solution ------------------------------------------------------------
+    # ---- plain FP64 BiCGStab (reference solve) ----------------------------------------------------------------------
+    t64, cext64, r64 = run("bicgstab", A.PREC_FP64)
+    log(f"FP64 BiCGStab: {t64:.3f} s, {r64['niter']} iterations, {r64['nmatvec']} matvecs, Cext {cext64:.10g}")
+
+    # ---- timed steps: FP32-mixed BiCGStab time-to-solution ---------------------------------------------------------
     for _ in range(args.warmup):
-        run("qmr", A.PREC_MIXED)
+        run("bicgstab", A.PREC_MIXED)
     op.sync()
     tstart = time.perf_counter()
     mixed = []
     for _ in range(args.steps):
-        mixed.append(run("qmr", A.PREC_MIXED))
+        mixed.append(run("bicgstab", A.PREC_MIXED))
     op.sync()
     elapsed = time.perf_counter() - tstart
     tmix = elapsed / args.steps
     _, cextmix, rmix = mixed[-1]
     err = abs(cextmix - cext64) / abs(cext64)
-    log(f"mixed QMR_CS: {tmix:.3f} s/solve, {rmix['niter']} FP32 iterations, {rmix['nmatvec']} FP32 + "
+    log(f"mixed BiCGStab: {tmix:.3f} s/solve, {rmix['niter']} FP32 iterations, {rmix['nmatvec']} FP32 + "
         f"{rmix['nmatvec_fp64']} FP64 matvecs, {rmix['nouter']} outer steps, true residual {rmix['true_resid']:.2e}, "
         f"Cext {cextmix:.10g} (rel err vs FP64 {err:.2e}), speedup {t64/tmix:.2f}x")
 
     extra = {}
-    if not args.no_bicgstab:
-        tb64, cb64, rb64 = run("bicgstab", A.PREC_FP64)
-        tbm, cbm, rbm = run("bicgstab", A.PREC_MIXED)
-        extra["bicgstab"] = dict(tts_fp64_s=round(tb64, 4), tts_mixed_s=round(tbm, 4), speedup=round(tb64 / tbm, 3),
-                                 iterations_fp64=rb64["niter"], iterations_mixed=rbm["niter"],
-                                 cext_rel_err=abs(cbm - cb64) / abs(cb64), cext_vs_qmr_fp64=abs(cb64 - cext64) / cext64)
-        log(f"BiCGStab: FP64 {tb64:.3f} s ({rb64['niter']} it), mixed {tbm:.3f} s ({rbm['niter']} it), "
-            f"Cext rel err {extra['bicgstab']['cext_rel_err']:.2e}")
+    if not args.no_qmr:
+        tq64, cq64, rq64 = run("qmr", A.PREC_FP64)
+        tqm, cqm, rqm = run("qmr", A.PREC_MIXED, max_inner=args.qmr_max_inner)
+        extra["qmr"] = dict(tts_fp64_s=round(tq64, 4), tts_mixed_s=round(tqm, 4), speedup=round(tq64 / tqm, 3),
+                            iterations_fp64=rq64["niter"], iterations_mixed=rqm["niter"], outer_mixed=rqm["nouter"],
+                            true_resid_mixed=rqm["true_resid"], max_inner=args.qmr_max_inner,
+                            cext_rel_err=abs(cqm - cq64) / abs(cq64), cext_vs_bicgstab_fp64=abs(cq64 - cext64) / cext64)
+        log(f"QMR_CS: FP64 {tq64:.3f} s ({rq64['niter']} it), mixed {tqm:.3f} s ({rqm['niter']} it, "
+            f"{rqm['nouter']} outer, true residual {rqm['true_resid']:.2e}), Cext rel err {extra['qmr']['cext_rel_err']:.2e}")
 
     # ---- roofline: dominant kernel of the timed workload (FP32 matvec pass P3) --------------------------------------
     dom = mv["fp32"]
@@ -143,7 +164,8 @@ def main():
         tc = time.perf_counter() - t
         cpu = {"value": round(1.0 / tc, 5), "unit": "matvecs/s", "cores": 1, "kind": "port",
                "sample": f"one FP64 FFT matvec of this workload (grid {pr.grid[0]}^3) by the oracle (textbook FFT, "
-                         f"1 thread, {tc:.1f} s); full FP64 QMR solve extrapolates to {tc * r64['nmatvec']:.0f} s"}
+                         f"1 thread, {tc:.1f} s); the FP64 BiCGStab solve ({r64['nmatvec']} matvecs) extrapolates to "
+                         f"{tc * r64['nmatvec']:.0f} s"}
         log(f"CPU baseline: {tc:.2f} s per matvec")
     pr.close()
 
@@ -152,13 +174,15 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(tmix * 1e3, 2), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64 inner (fp32 complex) + f64 outer", "data": "synthetic",
         "config": {"workload": f"sphere {box[0]}^3 box ({nd} dipoles), FFT grid {pr.grid[0]}x{pr.grid[1]}x{pr.grid[2]}, "
-                               f"m=1.5+0.01i, size parameter ~40, LDR, QMR_CS to 1e-10 (FP32-mixed); plain FP64 timed beside",
-                   "dipoles": nd, "grid": list(pr.grid), "eps": 1e-10, "solver": "qmr"},
+                               f"m=1.5+0.01i, size parameter ~40, LDR, BiCGStab to 1e-10 (FP32-mixed); plain FP64 timed "
+                               f"beside; QMR_CS in both forms under extra",
+                   "dipoles": nd, "grid": list(pr.grid), "eps": 1e-10, "solver": "bicgstab"},
         "tts_fp64_s": round(t64, 4), "tts_mixed_s": round(tmix, 4), "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
         "cext_fp64": cext64, "cext_mixed": cextmix, "cext_rel_err": err,
         "iterations": {"fp64": r64["niter"], "mixed_fp32": rmix["niter"], "mixed_outer": rmix["nouter"]},
         "matvecs_per_s": {"fp64": round(1e3 / mv["fp64"]["ms"], 1), "fp32": round(1e3 / mv["fp32"]["ms"], 1)},
-        "matvec_roofline": matvec_roof, "dmatrix_build_s": round(t_dmat, 3),
+        "matvec_roofline": matvec_roof, "dmatrix_build_s": round(t_dmat, 3), "autotune_s": round(t_tune, 3),
+        "matvec_config": tune,
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)

@@ -63,6 +63,10 @@ typedef struct {
 	double inner_eps;    /* mixed: relative tolerance of each FP32 inner solve (default 1e-5 when 0) */
 	int max_outer;       /* mixed: maximum FP64 correction steps (default 30 when 0) */
 	int verbose;         /* print RE_ lines to stdout */
+	int host_scalars;    /* 1: scalar recurrences on the host with a synchronising reduction per inner product (the
+	                      * reference's control flow, src/iterative.c); 0: device-resident recurrences (QMR_CS,
+	                      * BiCGStab, CGNR; BiCG_CS and CSYM always use the host form) */
+	int max_inner;       /* mixed: iteration cap of one FP32 inner solve before the FP64 correction (0 = no cap) */
 } adda_solver_opts;
 
 typedef struct {

@@ -120,6 +120,40 @@ int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void 
  * zero half-planes skipped, D-matrix read once): pass 0..4 as above, or 5 for the whole matvec. */
 double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pass);
 
+/* ---- device-resident Krylov iterations ------------------------------------------------------------------------
+ * The FP64 scalar recurrences of QMR_CS (src/iterative.c:898-1032), BiCGStab (640-724) and CGNR (730-770) run on
+ * the GPU: one call enqueues one whole iteration (matvecs, fused vector updates with their reductions, one-block
+ * scalar kernels, ProgressReport 282-318) on the operator's stream without synchronising. The state is an array of
+ * ADDA_DK_NSLOTS doubles in device memory plus a residual history; slot 6 holds the status (0 running, 1 converged,
+ * 2 maxiter, -1 breakdown, -2 stagnation), and once it is non-zero every later enqueued kernel returns at once.
+ * Slots: 0 inprodR, 1 inprodRp1, 2 epsB, 3 1/||b||^2, 4 niter, 5 counter, 6 status, 7 maxiter, 8 mc, 9 history
+ * length, 10 breakdown code, 16.. method scalars (set by the caller at PHASE_INIT, see host/iterative.c). niter is
+ * the iteration number the caller is enqueueing (it selects the niter==1 and niter==2 forms); matvec_ready: the
+ * first iteration reuses A.b held in Av (QMR, BiCGStab: in t). Vectors are in precision prec. */
+enum { ADDA_DK_NSLOTS = 96, ADDA_DK_STATUS = 6, ADDA_DK_NITER = 4, ADDA_DK_SC = 16 };
+void *adda_hip_dk_create(adda_hip_op *op, int hist_len);
+int adda_hip_dk_free(adda_hip_op *op, void *state);
+int adda_hip_dk_set(adda_hip_op *op, void *state, int slot, const double *vals, int n);
+int adda_hip_dk_get(adda_hip_op *op, void *state, int slot, double *vals, int n); /* synchronises */
+int adda_hip_dk_history(adda_hip_op *op, void *state, double *hist, int n);      /* synchronises */
+int adda_hip_dk_qmr_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *v, void *vtilda,
+	void *Av, void *p_old, void *p_new, void *x, void *r);
+int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x, void *r,
+	void *p, void *v, void *s, void *rtilda, void *t);
+int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, void *x, void *r, void *p, void *Av);
+
+/* Matvec configuration per precision: radix schedules along x, y, z and tile sizes, as a string
+ * "x=16.16 y=16.16 z=16.16 LY=4 TC=16 LP=2" (x radices: P1/P5; y: P3; z: P2/P4; LY lines per P1/P5 block, TC columns
+ * per P2/P4 block, LP x-pairs per P3 block). set_config validates it against the grid. */
+int adda_hip_get_config(adda_hip_op *op, int prec, char *buf, size_t len);
+int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg);
+
+/* Autotuner: times every candidate radix schedule (radix-16/8/4/2 splits of the power of two, radix 3/5 stages
+ * before or after) and tile size of each pass group (x: P1+P5, z: P2+P4, y: P3) with HIP events, reps timed runs
+ * each, and keeps the fastest. With cache_path, a line {"key": ..., "config": ...} is looked up first (no timing if
+ * found) and appended after tuning. report (may be NULL) receives the chosen configuration and pass times. */
+int adda_hip_autotune(adda_hip_op *op, int prec, int reps, const char *cache_path, char *report, size_t report_len);
+
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 

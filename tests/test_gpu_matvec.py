@@ -104,3 +104,42 @@ def test_vector_kernels(hipdev):
     r, nn = op.dot(da, None, hipdev.DOT_SELF_BILIN)
     assert abs(r - np.sum(a * a)) / abs(r) < 1e-13
     assert abs(nn - np.sum(np.abs(a) ** 2)) / nn < 1e-13
+
+
+@pytest.mark.parametrize("cfg", ["x=8.4 y=8.4 z=8.4 LY=1 TC=4 LP=1", "x=4.8 y=4.4.2 z=2.4.4 LY=8 TC=32 LP=2",
+                                 "x=2.2.2.4 y=16.2 z=8.2.2 LY=2 TC=8 LP=4"])
+def test_matvec_configs_agree(hipdev, cfg):
+    """Every radix schedule / tile configuration the autotuner may pick gives the same product (grid 32^3)."""
+    c = _case("sphere", 16, 1.5)
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    op = _op(hipdev, c, fp32=True)
+    op.set_cc_sqrt(ccs)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+    y0, _ = op.matvec(x)
+    y032, _ = op.matvec(x, prec=hipdev.F32)
+    for prec in (hipdev.F64, hipdev.F32):
+        op.set_config(cfg, prec)
+        assert op.get_config(prec) == cfg
+    y1, _ = op.matvec(x)
+    y132, _ = op.matvec(x, prec=hipdev.F32)
+    assert _rel(y1, y0) < 1e-13
+    assert _rel(y132, y032) < 1e-5
+
+
+def test_autotune_keeps_results(hipdev, tmp_path):
+    c = _case("ellipsoid", 24, 1.5 + 0.01j, 1.0, 0.5)
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    op = _op(hipdev, c, fp32=True)
+    op.set_cc_sqrt(ccs)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+    y0, _ = op.matvec(x)
+    cache = str(tmp_path / "tune.jsonl")
+    rep = op.autotune(hipdev.F64, reps=2, cache=cache)
+    assert "x=" in rep and "candidates" in rep
+    rep2 = op.autotune(hipdev.F64, reps=2, cache=cache)
+    assert "from cache" in rep2
+    op.autotune(hipdev.F32, reps=2, cache=cache)
+    y1, _ = op.matvec(x)
+    assert _rel(y1, y0) < 1e-13

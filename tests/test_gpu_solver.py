@@ -28,13 +28,16 @@ def _problem(adda, geo, m, dpl, mat=None, polrel=0, fp32=False):
     return adda.Problem(p, box, m, mat=gmat if mat is None else mat, dpl=dpl, polrel=polrel, fp32=fp32)
 
 
+@pytest.mark.parametrize("host_scalars", [False, True])
 @pytest.mark.parametrize("method,gold,mi", METHODS)
-def test_c1_solvers(hipdev, method, gold, mi):
+def test_c1_solvers(hipdev, method, gold, mi, host_scalars):
+    """Both forms of the recurrences: on the host (the reference's control flow, one synchronising reduction per
+    inner product) and device-resident (FP64 scalar kernels on the GPU; BiCG_CS/CSYM fall back to the host form)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
     try:
         for pol, vec in (("Y", POL_Y), ("X", POL_X)):
             pr.set_polarization(PROP_Z, vec)
-            r = pr.solve(method, eps=1e-10)
+            r = pr.solve(method, eps=1e-10, host_scalars=host_scalars)
             assert r["rc"] == 0
             g = G.crosssec(gold, pol)
             assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0], (pr.cext(), g["Cext"])
@@ -43,7 +46,8 @@ def test_c1_solvers(hipdev, method, gold, mi):
         pr.close()
 
 
-def test_c1_history_and_fields_vs_oracle(hipdev):
+@pytest.mark.parametrize("host_scalars", [False, True])
+def test_c1_history_and_fields_vs_oracle(hipdev, host_scalars):
     """QMR at eps 1e-5: the residual history follows the sample's RE_ lines; at 1e-10 the field x matches the oracle's
     FP64 solve to 1e-8 and the Mueller matrix matches the reference's 360-degree yz-plane output."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
@@ -52,14 +56,14 @@ def test_c1_history_and_fields_vs_oracle(hipdev):
     oc = Case(p, box, 1.5, dpl=15)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
-        r = pr.solve("qmr", eps=1e-5)
+        r = pr.solve("qmr", eps=1e-5, host_scalars=host_scalars)
         assert r["niter"] == 22
         assert np.allclose(r["hist"][:19], G.log_re("c1_qmr_e5")[:19], rtol=1e-5)
         assert abs(pr.cext() - 135.0449047) / 135.0449047 < 1e-8
         epl = {}
         for pol, vec in (("Y", POL_Y), ("X", POL_X)):
             pr.set_polarization(PROP_Z, vec)
-            r = pr.solve("qmr", eps=1e-10)
+            r = pr.solve("qmr", eps=1e-10, host_scalars=host_scalars)
             x = pr.field_x()
             cc, ccs = oc.cc(PROP_Z, vec)
             ro = oc.solve(oc.dmatrix(), ccs, oc.einc(PROP_Z, vec), method=0, eps=1e-10)
@@ -116,14 +120,15 @@ def test_cm_and_multimaterial(hipdev):
         pr.close()
 
 
+@pytest.mark.parametrize("max_inner", [0, 40])
 @pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr"])
-def test_mixed_precision_c1(hipdev, method):
+def test_mixed_precision_c1(hipdev, method, max_inner):
     """FP32 inner Krylov + FP64 defect correction reaches the FP64 answer: true residual <= 1e-10 and Cext within 1e-6
     (the reference value 135.0448603 is itself converged to 1e-10)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
-        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED)
+        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED, max_inner=max_inner)
         assert r["true_resid"] <= 1e-10, r
         assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6
         assert r["nmatvec_fp64"] >= 1 and r["nouter"] >= 2
@@ -146,12 +151,14 @@ def test_recalc_resid_and_init_fields(hipdev):
         pr.close()
 
 
-def test_maxiter_warning(hipdev):
+@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr"])
+@pytest.mark.parametrize("host_scalars", [False, True])
+def test_maxiter_warning(hipdev, method, host_scalars):
     """-maxiter 5 stops with the warning code (src/iterative.c:1506-1508), not an error."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
-        r = pr.solve("qmr", eps=1e-10, maxiter=5)
+        r = pr.solve(method, eps=1e-10, maxiter=5, host_scalars=host_scalars)
         assert r["rc"] == 1 and r["niter"] == 5
     finally:
         pr.close()

@@ -79,6 +79,38 @@ def hip():
     L.adda_hip_get_config.argtypes = [P, I, ctypes.c_char_p, S]
     L.adda_hip_set_config.argtypes = [P, I, ctypes.c_char_p]
     L.adda_hip_autotune.argtypes = [P, I, I, ctypes.c_char_p, ctypes.c_char_p, S]
+    L.adda_hip_dk_create.argtypes = [P, I]
+    L.adda_hip_dk_create.restype = P
+    L.adda_hip_dk_free.argtypes = [P, P]
+    L.adda_hip_dk_set.argtypes = [P, P, I, P, I]
+    L.adda_hip_dk_get.argtypes = [P, P, I, P, I]
+    L.adda_hip_dk_history.argtypes = [P, P, P, I]
+    L.adda_hip_dk_qmr_iter.argtypes = [P, I, P, I, I, P, P, P, P, P, P, P]
+    L.adda_hip_dk_bicgstab_iter.argtypes = [P, I, P, I, I, P, P, P, P, P, P, P]
+    L.adda_hip_dk_cgnr_iter.argtypes = [P, I, P, I, P, P, P, P]
+    # batched multi-RHS
+    L.adda_hip_reserve_batch.argtypes = [P, I]
+    L.adda_hip_batch_stride.argtypes = [P]
+    L.adda_hip_batch_stride.restype = S
+    L.adda_hip_set_cc_sqrt_batch.argtypes = [P, I, P]
+    L.adda_hip_vec_alloc_batch.argtypes = [P, I, I]
+    L.adda_hip_vec_alloc_batch.restype = P
+    L.adda_hip_vec_convert_batch.argtypes = [P, I, P, I, P, I]
+    L.adda_hip_matvec_batch.argtypes = [P, I, I, P, P, P, P, I]
+    L.adda_hip_lincomb_batch.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I, P]
+    L.adda_hip_dot_batch.argtypes = [P, I, I, P, P, P, I, P, P]
+    L.adda_hip_mult_mat_batch.argtypes = [P, I, I, P, P]
+    L.adda_hip_profile_matvec_batch.argtypes = [P, I, I, P, P, I, P, P]
+    L.adda_hip_matvec_bytes_batch.argtypes = [P, I, I, I]
+    L.adda_hip_matvec_bytes_batch.restype = D
+    L.adda_hip_dk_create_batch.argtypes = [P, I, I]
+    L.adda_hip_dk_create_batch.restype = P
+    L.adda_hip_autotune_batch.argtypes = [P, I, I, I, ctypes.c_char_p, ctypes.c_char_p, S]
+    L.adda_hip_capture_begin.argtypes = [P]
+    L.adda_hip_capture_end.argtypes = [P]
+    L.adda_hip_capture_end.restype = P
+    L.adda_hip_graph_launch.argtypes = [P, P]
+    L.adda_hip_graph_free.argtypes = [P, P]
     _hip = L
     return L
 
@@ -203,6 +235,59 @@ class Operator:
     def matvec_bytes(self, prec=F64, pass_=5):
         return hip().adda_hip_matvec_bytes(self.h, prec, pass_)
 
+    # batched vectors: K right-hand sides at stride `stride` complex elements
+    def reserve(self, K):
+        check(hip().adda_hip_reserve_batch(self.h, K))
+
+    @property
+    def stride(self):
+        return hip().adda_hip_batch_stride(self.h)
+
+    def alloc_batch(self, K, prec=F64):
+        v = hip().adda_hip_vec_alloc_batch(self.h, prec, K)
+        if not v:
+            raise RuntimeError(hip().adda_hip_last_error().decode())
+        return v
+
+    def rhs(self, v, k, prec=F64):
+        """Device address of RHS k of a batched vector."""
+        return v + k * self.stride * (16 if prec == F64 else 8)
+
+    def set_cc_sqrt_batch(self, cc_sqrt):
+        a = np.ascontiguousarray(np.asarray(cc_sqrt, np.complex128).reshape(-1, self.nmat, 3))
+        check(hip().adda_hip_set_cc_sqrt_batch(self.h, a.shape[0], _p(a)))
+
+    def matvec_batch(self, a, r, K, prec=F64, her=False, inprod=False, gate=None):
+        ip = np.zeros(K)
+        check(hip().adda_hip_matvec_batch(self.h, prec, K, gate, a, r, _p(ip) if inprod else None, int(her)))
+        return ip if inprod else None
+
+    def lincomb_batch(self, K, out, a=None, ca=0, b=None, cb=0, c=None, cc=0, flags=0, prec=F64, norm=False, gate=None):
+        co = [np.ascontiguousarray(np.broadcast_to(np.asarray(z, np.complex128), (K,))) for z in (ca, cb, cc)]
+        n2 = np.zeros(K)
+        check(hip().adda_hip_lincomb_batch(self.h, prec, K, gate, out, a, _p(co[0]), b, _p(co[1]), c, _p(co[2]),
+                                           flags, _p(n2) if norm else None))
+        return n2 if norm else None
+
+    def dot_batch(self, K, a, b=None, mode=DOT_CONJ, prec=F64, gate=None):
+        r, n2 = np.zeros(K, np.complex128), np.zeros(K)
+        check(hip().adda_hip_dot_batch(self.h, prec, K, gate, a, b, mode, _p(r), _p(n2)))
+        return r, n2
+
+    def profile_matvec_batch(self, a, r, K, prec=F64, nrep=10):
+        ms = np.zeros(5)
+        tot = ctypes.c_double()
+        check(hip().adda_hip_profile_matvec_batch(self.h, prec, K, a, r, nrep, _p(ms), ctypes.byref(tot)))
+        return ms, tot.value
+
+    def matvec_bytes_batch(self, K, prec=F64, pass_=5):
+        return hip().adda_hip_matvec_bytes_batch(self.h, prec, K, pass_)
+
+    def autotune_batch(self, K, prec=F64, reps=3, cache=None):
+        b = ctypes.create_string_buffer(512)
+        check(hip().adda_hip_autotune_batch(self.h, prec, K, reps, cache.encode() if cache else None, b, 512))
+        return b.value.decode()
+
 
 # ---------------------------------------------------------------------------------------------------------------------
 # C host layer (include/adda.h)
@@ -224,7 +309,7 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("eps", ctypes.c_double), ("maxiter", ctypes.c_int),
                 ("init_field", ctypes.c_int), ("recalc_resid", ctypes.c_int), ("precision", ctypes.c_int),
                 ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int)]
+                ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -258,6 +343,7 @@ def host():
     L.adda_couple_constant.restype = Cplx
     L.adda_generate_b.argtypes = [P, P, D, D, P, P, P]
     L.adda_iterative_solver.argtypes = [P, ctypes.POINTER(SolverOpts), P, P, P, P, I, ctypes.POINTER(SolveStats)]
+    L.adda_iterative_solver_batch.argtypes = [P, ctypes.POINTER(SolverOpts), I, P, P, P, P, I, P]
     L.adda_calc_field.argtypes = [P, P, D, D, P, P, I, P]
     L.adda_ext_cross.argtypes = [P, P, D, D, P, P, P, P]
     L.adda_abs_cross.argtypes = [P, P, D, P, P, P]
@@ -341,11 +427,12 @@ class Problem:
                                       _p(self.incpol), self.einc))
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
-              max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0):
+              max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0,
+              graph=False):
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
-                       inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner))
+                       inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner), int(graph))
         st = SolveStats()
         hist = np.zeros(hist_len)
         rc = host().adda_iterative_solver(self.op.h, ctypes.byref(o), self.einc, self.x, self.pvec, _p(hist),
@@ -382,6 +469,94 @@ class Problem:
 
     def field_x(self):
         return self.op.download(self.x)
+
+
+class BatchProblem:
+    """K right-hand sides of one particle -- incident directions and polarizations, e.g. the orientations x
+    polarizations of an orientation average (src/crosssec.c:82-138) -- solved together on one operator: one D-matrix,
+    per-RHS cc_sqrt tables (LDR depends on prop and polarization), batched device vectors, one batched launch per
+    kernel for all RHS still iterating."""
+
+    def __init__(self, pos, box, m, props, pols, mat=None, dpl=None, lam=2 * np.pi, reduced_fft=True, fp32=False,
+                 device=0, polrel=POL_LDR, grid=None):
+        self.pos = np.ascontiguousarray(pos, np.uint16)
+        self.box = tuple(int(b) for b in box)
+        self.nd = len(self.pos)
+        self.m = np.atleast_1d(np.asarray(m, np.complex128))
+        self.mat = np.zeros(self.nd, np.int32) if mat is None else np.ascontiguousarray(mat, np.int32)
+        self.nmat = len(self.m)
+        if dpl is None:
+            dpl = 10 * float(np.max(np.abs(self.m)))
+        self.dpl, self.lam = dpl, lam
+        self.wavenum = 2 * np.pi / lam
+        self.gridspace = lam / dpl
+        self.polrel = polrel
+        self.props = np.ascontiguousarray(props, np.float64).reshape(-1, 3)
+        self.pols = np.ascontiguousarray(pols, np.float64).reshape(-1, 3)
+        self.K = K = len(self.props)
+        self.grid = tuple(grid) if grid else tuple(hip().adda_hip_fftfit(2 * b, 1) for b in self.box)
+        self.op = Operator(self.pos, self.box, self.grid, self.mat, self.nmat, self.wavenum, self.gridspace,
+                           reduced_fft, fp32=fp32, device=device)
+        self.op.reserve(K)
+        self._p = Particle(self.box[0], self.box[1], self.box[2], self.nd,
+                           self.pos.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort)),
+                           self.mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), self.nmat)
+        self.cc = np.empty((K, self.nmat, 3), np.complex128)
+        for k in range(K):
+            for i, mi in enumerate(self.m):
+                self.cc[k, i, :] = couple_constant(mi, polrel, self.wavenum, self.gridspace, self.props[k], self.pols[k])
+        self.cc_sqrt = np.sqrt(self.cc)
+        self.op.set_cc_sqrt_batch(self.cc_sqrt)
+        self.einc = self.op.alloc_batch(K)
+        self.x = self.op.alloc_batch(K)
+        self.pvec = self.op.alloc_batch(K)
+        for k in range(K):
+            hcheck(host().adda_generate_b(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace,
+                                          _p(self.props[k]), _p(self.pols[k]), self.op.rhs(self.einc, k)))
+
+    def close(self):
+        if self.op is not None:
+            for v in (self.einc, self.x, self.pvec):
+                self.op.free(v)
+            self.op.close()
+            self.op = None
+
+    def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
+              max_outer=0, recalc_resid=False, hist_len=0, graph=True, max_inner=0):
+        if maxiter is None:
+            maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
+        o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
+                       inner_eps, max_outer, 0, 0, int(max_inner), int(graph))
+        st = (SolveStats * self.K)()
+        hist = np.zeros((self.K, max(hist_len, 1)))
+        rc = host().adda_iterative_solver_batch(self.op.h, ctypes.byref(o), self.K, self.einc, self.x, self.pvec,
+                                                _p(hist) if hist_len else None, hist_len, st)
+        hcheck(rc)
+        out = [dict(rc=s.rc, niter=s.niter, nmatvec=s.nmatvec, nmatvec_fp64=s.nmatvec_fp64, nouter=s.nouter,
+                    resid=s.resid, true_resid=s.true_resid) for s in st]
+        if hist_len:
+            for k, d in enumerate(out):
+                d["hist"] = hist[k, :min(d["niter"] + 1, hist_len)].copy()
+        return out
+
+    def cext(self, k):
+        out = ctypes.c_double()
+        hcheck(host().adda_ext_cross(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace,
+                                     self.op.rhs(self.pvec, k), _p(self.props[k]), _p(self.pols[k]), ctypes.byref(out)))
+        return out.value
+
+    def cabs(self, k):
+        out = ctypes.c_double()
+        cc = np.ascontiguousarray(self.cc[k])
+        hcheck(host().adda_abs_cross(self.op.h, ctypes.byref(self._p), self.wavenum, _p(cc), self.op.rhs(self.pvec, k),
+                                     ctypes.byref(out)))
+        return out.value
+
+    def field_x(self, k):
+        return self.op.download(self.op.rhs(self.x, k))
+
+    def polarization(self, k):
+        return self.op.download(self.op.rhs(self.pvec, k))
 
 
 def mueller_yz(eX, eY, ntheta):

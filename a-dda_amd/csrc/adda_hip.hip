@@ -142,27 +142,42 @@ struct adda_hip_op {
 	unsigned short *pos = nullptr;  // [3*nd]
 	double *wbuf = nullptr;         // [nmat*3] weights / direction scratch
 	double *dirbuf = nullptr;       // [3*kMaxDir] directions
-	// coupling
-	cx<double> *cc64 = nullptr;  // [nmat][3]
+	// coupling: one cc_sqrt table per right-hand side of a batched solve
+	cx<double> *cc64 = nullptr;  // [kcap][nmat][3]
 	cx<float> *cc32 = nullptr;
 	// D-matrix, layout [zt<DsZ][yt<DsY][x<gX][6]
 	cx<double> *D64 = nullptr;
 	cx<float> *D32 = nullptr;
-	// work buffers (FP64-sized; FP32 uses the first half)
-	void *Xa = nullptr;  // [3][bZ][bY][gX]
-	void *Xb = nullptr;  // [3][gZ][bY][gX]
+	// work buffers, kcap right-hand sides (FP64-sized; FP32 uses the first half)
+	void *Xa = nullptr;  // [kcap][bZ][gX][3][bY]
+	void *Xb = nullptr;  // [kcap][gZ][gX][3][bY]
+	int kcap = 0;        // right-hand sides the buffers and tables hold
+	size_t vs = 0;       // stride between the right-hand sides of a batched device vector (complex elements)
+	double *coef = nullptr;       // [kcap][6] per-RHS coefficients of batched vector updates
 	FftPlan px, py, pz;      // plans of the D-matrix construction kernels
 	MvCfg cfg[2];            // matvec plans and tiles per precision (index ADDA_F64 / ADDA_F32)
 	TwPair twx, twy, twz;
 	// reductions
-	double *partials = nullptr;  // max(4*kRedBlocks, gather blocks)
+	double *partials = nullptr;  // max(field-sum partials, kcap * gather blocks)
 	size_t npartials = 0;
-	double *partials2 = nullptr; // [4*kRedBlocks] second region for device-resident solver dots
-	const double *gate = nullptr; // device solver state while its iterations are enqueued: kernels return at once
-	                              // when its status slot is non-zero
-	double *red = nullptr;       // [4] device result
-	double *red_host = nullptr;  // [4] pinned
+	double *partials2 = nullptr; // [kcap][4*kRedBlocks] second region for device-resident solver dots
+	double *red = nullptr;       // [4*kcap] device result
+	double *red_host = nullptr;  // [4*kcap] pinned
 };
+
+// Per-RHS layout of a batched launch: K right-hand sides share the operator (D-matrix, geometry, plans); the RHS
+// index is part of the block index. A single solve is K = 1.
+struct Bq {
+	int K;               // right-hand sides in the launch
+	long vs;             // stride of the dipole vectors (arg, result) between RHS, complex elements
+	long xs, xs2;        // strides of the work buffers the pass reads / writes
+	int cs;              // stride of the cc_sqrt tables (complex elements)
+	const double *gate;  // RHS k is skipped when gate[k*DK_NSLOTS + DK_STATUS] != 0 (null: never)
+};
+__device__ __forceinline__ bool skip_rhs(const Bq &b, int k)
+{
+	return b.gate && b.gate[(size_t)k * DK_NSLOTS + DK_STATUS] != 0;
+}
 
 static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
 
@@ -244,21 +259,25 @@ __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ tw
 	__syncthreads();
 }
 
-// P1: block = (z, y0..y0+LY, c) with the component fastest in the block index (the three blocks of a tile read the
-// same dipoles back to back). The gather of S.x along x (x fastest) fills the LDS lines; the last FFT stage stores
-// Xa[z][x][c][y] with y fastest across threads (runs of LY).
+// P1: block = (z, y0..y0+LY, RHS, c) with the component fastest in the block index, then the RHS (the three blocks
+// of a tile read the same dipoles back to back, and the RHS of a tile share its cell-map and material reads). The
+// gather of S.x along x (x fastest) fills the LDS lines; the last FFT stage stores Xa[z][x][c][y] with y fastest
+// across threads (runs of LY).
 template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
 	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, const double *gate)
+	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
-	if (gate && gate[DK_STATUS] != 0) return;
+	const int c = blockIdx.x % 3, rest = blockIdx.x / 3, kq = rest % bq.K, tile = rest / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	arg += kq * bq.vs;
+	Xa += kq * bq.xs2;
+	ccs += kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
-	const int c = blockIdx.x % 3, tile = blockIdx.x / 3;
 	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int tid = threadIdx.x;
 	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest
@@ -286,14 +305,17 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
 template <typename T, bool INV, int RM>
 __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
-	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, const double *gate)
+	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, Bq bq)
 {
-	if (gate && gate[DK_STATUS] != 0) return;
+	const int kq = blockIdx.x % bq.K;
+	if (skip_rhs(bq, kq)) return;
+	in += kq * bq.xs;
+	out += kq * bq.xs2;
 	extern __shared__ char smem_raw[];
 	const int N = p.n, TPL = N / p.rmax;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
-	const long q0 = (long)blockIdx.x * TC;
+	const long q0 = (long)(blockIdx.x / bq.K) * TC;
 	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
 	const bool act = t < TPL && q0 + line < Q;
 	auto ld = [&](int l, int k) -> cx<T> { return k < n_in ? in[(size_t)k * Q + q0 + l] : mk<T>(0, 0); };
@@ -305,21 +327,24 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 
 // P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
 // which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
-// HBM once per matvec. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
+// HBM once per matvec. The RHS index is fastest in the block index: the K blocks that share a D tile run together
+// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
 // LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
 // cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
 template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
-	const double *gate)
+	Bq bq)
 {
-	if (gate && gate[DK_STATUS] != 0) return;
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
 	const int N = p.n, TPL = N / p.rmax, NL1 = 3 * LP, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nxt = gX / LP;
-	const int z = blockIdx.x / nxt, x0 = (blockIdx.x % nxt) * LP;
+	const int z = tile / nxt, x0 = (tile % nxt) * LP;
 	const bool has2 = reduced && z != 0 && 2 * z != gZ;  // mirror plane handled by this block
 	const int zp = gZ - z;
 	const int nls = has2 ? 2 * NL1 : NL1;
@@ -375,23 +400,27 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	fft_line<T, true, RM>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
-// P5: block = (z, y0..y0+LY, c), component fastest in the block index; stage 0 loads Xa[z][x][c][y] y-fastest (runs
-// of LY), the results go to LDS and are gathered to the dipoles along x: result = arg + S.X (her: conj), partial
-// ||result||^2 per block.
+// P5: block = (z, y0..y0+LY, RHS, c), component fastest in the block index, then the RHS; stage 0 loads
+// Xa[z][x][c][y] y-fastest (runs of LY), the results go to LDS and are gathered to the dipoles along x:
+// result = arg + S.X (her: conj), partial ||result||^2 per block at partials[k*(blocks per RHS) + tile*3 + c].
 template <typename T, int RM>
 __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
 	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
-	double *__restrict__ partials, const double *gate)
+	double *__restrict__ partials, Bq bq)
 {
-	if (gate && gate[DK_STATUS] != 0) return;
+	const int c = blockIdx.x % 3, rest = blockIdx.x / 3, kq = rest % bq.K, tile = rest / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xa += kq * bq.xs;
+	arg += kq * bq.vs;
+	res += kq * bq.vs;
+	ccs += kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
 	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
-	const int c = blockIdx.x % 3, tile = blockIdx.x / 3;
 	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int tid = threadIdx.x;
 	const int lineA = tid % LY, tA = tid / LY;    // y fastest
@@ -416,7 +445,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	}
 	if (partials) {
 		double tt = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[blockIdx.x] = tt;
+		if (threadIdx.x == 0) partials[(size_t)kq * (gridDim.x / bq.K) + tile * 3 + c] = tt;
 	}
 }
 
@@ -494,11 +523,27 @@ __global__ void k_convert(cx<TD> *dst, const cx<TS> *src, size_t n)
 		dst[i] = mk<TD>((TD)src[i].x, (TD)src[i].y);
 }
 
+// Batched over blockIdx.y = RHS k (vectors at offset k*vs); coefficients from coefs[6k..6k+5] when coefs != null,
+// else the by-value ones; RHS with a non-zero gate status are skipped; partial norms at partials[k*gridDim.x + b].
 template <typename T>
 __global__ void __launch_bounds__(256) k_lincomb(cx<T> *out, const cx<T> *a, cx<double> ca, const cx<T> *b,
-	cx<double> cb, const cx<T> *c, cx<double> cc, int flags, size_t n, double *partials)
+	cx<double> cb, const cx<T> *c, cx<double> cc, const double *coefs, int flags, size_t n, long vs,
+	double *partials, const double *gate)
 {
 	__shared__ double red_lds[512];
+	const int kq = blockIdx.y;
+	if (gate && gate[(size_t)kq * DK_NSLOTS + DK_STATUS] != 0) return;
+	const size_t off = (size_t)kq * vs;
+	out += off;
+	if (a) a += off;
+	if (b) b += off;
+	if (c) c += off;
+	if (coefs) {
+		const double *q = coefs + 6 * kq;
+		ca = mk<double>(q[0], q[1]);
+		cb = mk<double>(q[2], q[3]);
+		cc = mk<double>(q[4], q[5]);
+	}
 	double s = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		double rx = 0, ry = 0;
@@ -526,16 +571,20 @@ __global__ void __launch_bounds__(256) k_lincomb(cx<T> *out, const cx<T> *a, cx<
 	}
 	if (partials) {
 		double t = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[blockIdx.x] = t;
+		if (threadIdx.x == 0) partials[(size_t)kq * gridDim.x + blockIdx.x] = t;
 	}
 }
 
-// partials layout: [4][nblocks]
+// Batched over blockIdx.y = RHS k (vectors at offset k*vs); partials layout [k][3][nblk].
 template <typename T>
-__global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int mode, size_t n, double *partials,
-	int nblk, const double *gate = nullptr)
+__global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int mode, size_t n, long vs,
+	double *partials, int nblk, const double *gate)
 {
-	if (gate && gate[DK_STATUS] != 0) return;
+	const int kq = blockIdx.y;
+	if (gate && gate[(size_t)kq * DK_NSLOTS + DK_STATUS] != 0) return;
+	a += (size_t)kq * vs;
+	if (b) b += (size_t)kq * vs;
+	partials += (size_t)kq * 3 * nblk;
 	__shared__ double red_lds[512];
 	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -563,11 +612,12 @@ __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int
 	}
 }
 
-// Deterministic final reduction of nk rows of nblk partials: fixed strided order per thread, then a fixed tree.
+// Deterministic final reduction of nk rows of nblk partials: fixed strided order per thread, then a fixed tree;
+// one block per row.
 __global__ void __launch_bounds__(256) k_reduce_final(const double *partials, int nblk, int nk, double *out)
 {
 	__shared__ double red_lds[512];
-	for (int k = 0; k < nk; k++) {
+	for (int k = blockIdx.x; k < nk; k += gridDim.x) {
 		double s = 0;
 		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[k * nblk + i];
 		double t = block_sum<double>(s, red_lds);
@@ -577,8 +627,11 @@ __global__ void __launch_bounds__(256) k_reduce_final(const double *partials, in
 }
 
 template <typename T>
-__global__ void k_mult_mat(cx<T> *out, const cx<T> *in, const int *mat, const cx<T> *ccs, size_t nd)
+__global__ void k_mult_mat(cx<T> *out, const cx<T> *in, const int *mat, const cx<T> *ccs, size_t nd, long vs, int cs)
 {
+	out += (size_t)blockIdx.y * vs;
+	in += (size_t)blockIdx.y * vs;
+	ccs += (size_t)blockIdx.y * cs;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
 		const size_t d = i / 3;
 		const int c = (int)(i % 3);
@@ -671,9 +724,16 @@ static int upload_twiddles(int n, TwPair &t)
 	return 0;
 }
 
+static int reduce_rows(adda_hip_op *op, const double *partials, int nblk, int nk, double *out);
 static int reduce_finish(adda_hip_op *op, int nblk, int nk, double *out)
 {
-	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, op->stream, op->partials, nblk, nk, op->red);
+	return reduce_rows(op, op->partials, nblk, nk, out);
+}
+
+static int reduce_rows(adda_hip_op *op, const double *partials, int nblk, int nk, double *out)
+{
+	if (nk > 4 * op->kcap) return ERR("reduction of %d rows exceeds the result buffer", nk);
+	hipLaunchKernelGGL(k_reduce_final, dim3(nk), dim3(256), 0, op->stream, partials, nblk, nk, op->red);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipMemcpyAsync(op->red_host, op->red, nk * sizeof(double), hipMemcpyDeviceToHost, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
@@ -722,81 +782,104 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 		lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
 }
 
+// Per-RHS strides of the work buffers (complex elements)
+static long xa_stride(const adda_hip_op *op) { return (long)3 * op->bZ * op->bY * op->gX; }
+static long xb_stride(const adda_hip_op *op) { return (long)3 * op->gZ * op->bY * op->gX; }
+static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, const double *gate)
+{
+	Bq b;
+	b.K = K;
+	b.vs = vs;
+	b.xs = xs;
+	b.xs2 = xs2;
+	b.cs = 3 * op->nmat;
+	b.gate = gate;
+	return b;
+}
+
 template <typename T>
-static void launch_p1(adda_hip_op *op, const MvCfg &c, const cx<T> *arg, const cx<T> *ccs, int her)
+static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
+	const cx<T> *ccs, int her)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
 	auto k = c.qx.rmax <= 8 ? k_p1<T, 8> : k_p1<T, 16>;
-	hipLaunchKernelGGL(k, dim3(3 * op->bZ * nyt), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
-		op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX, op->bY, c.LY,
-		her, op->gate);
+	hipLaunchKernelGGL(k, dim3(3 * op->bZ * nyt * K), dim3(c.LY * (op->gX / c.qx.rmax)),
+		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx,
+		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
 }
 
 template <typename T, bool INV>
-static void launch_zfft(adda_hip_op *op, const MvCfg &c)
+static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *gate)
 {
 	const long Q = (long)op->gX * 3 * op->bY;
 	const cx<T> *in = (const cx<T> *)(INV ? op->Xb : op->Xa);
 	cx<T> *out = (cx<T> *)(INV ? op->Xa : op->Xb);
 	auto k = c.qz.rmax <= 8 ? k_zfft<T, INV, 8> : k_zfft<T, INV, 16>;
-	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC)), dim3(c.TC * (op->gZ / c.qz.rmax)),
+	const long xin = INV ? xb_stride(op) : xa_stride(op), xout = INV ? xa_stride(op) : xb_stride(op);
+	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
 		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
-		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, op->gate);
+		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, make_bq(op, K, 0, xin, xout, gate));
 }
 
 template <typename T>
-static void launch_p3(adda_hip_op *op, const MvCfg &c, const cx<T> *D, int transposed)
+static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
 	auto k = c.qy.rmax <= 8 ? k_p3<T, 8> : k_p3<T, 16>;
-	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP)), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
+	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
-		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed, op->gate);
+		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed,
+		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
 }
 
+// returns the number of P5 blocks per RHS (the row length of its partial sums)
 template <typename T>
-static int launch_p5(adda_hip_op *op, const MvCfg &c, const cx<T> *arg, cx<T> *res, const cx<T> *ccs, int her,
-	double *partials)
+static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
+	cx<T> *res, const cx<T> *ccs, int her, double *partials)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = 3 * op->bZ * nyt;
 	auto k = c.qx.rmax <= 8 ? k_p5<T, 8> : k_p5<T, 16>;
-	hipLaunchKernelGGL(k, dim3(nblk5), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
+	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
-		op->bY, c.LY, her, partials, op->gate);
+		op->bY, c.LY, her, partials, make_bq(op, K, vs, xa_stride(op), 0, gate));
 	return nblk5;
 }
 
-// nblk_out (optional): receives the number of P5 blocks; with keep_partials the per-block ||result||^2 partials stay
-// in op->partials for a device-side reduction (no synchronisation).
+template <typename T> static const cx<T> *cc_of(const adda_hip_op *op);
+template <> const cx<double> *cc_of<double>(const adda_hip_op *op) { return op->cc64; }
+template <> const cx<float> *cc_of<float>(const adda_hip_op *op) { return op->cc32; }
+
+// K right-hand sides at stride vs (K <= op->kcap); RHS k is skipped when gate (a device solver state array) has a
+// non-zero status for it. inprod (optional) receives K values ||result_k||^2 (synchronises). nblk_out (optional):
+// the number of P5 blocks per RHS; with keep_partials the per-block ||result||^2 partials stay in op->partials
+// (layout [k][nblk]) for a device-side reduction (no synchronisation).
 template <typename T>
-static int matvec_t(adda_hip_op *op, const cx<T> *arg, cx<T> *res, double *inprod, int her, hipEvent_t *ev = nullptr,
-	bool keep_partials = false, int *nblk_out = nullptr)
+static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const cx<T> *arg, cx<T> *res,
+	double *inprod, int her, hipEvent_t *ev = nullptr, bool keep_partials = false, int *nblk_out = nullptr)
 {
-	const cx<T> *ccs;
+	const cx<T> *ccs = cc_of<T>(op);
 	const cx<T> *D;
-	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
-	else { ccs = op->cc32; D = op->D32; }
+	if constexpr (sizeof(T) == 8) D = op->D64;
+	else D = op->D32;
 	if (!D) return ERR("FP32 matvec requested but the operator was created without the FP32 flag");
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
 	const MvCfg &c = op->cfg[sizeof(T) == 8 ? ADDA_F64 : ADDA_F32];
 	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
 	if (ev) HIPCHK(hipEventRecord(ev[0], op->stream));
-	launch_p1<T>(op, c, arg, ccs, her);
+	launch_p1<T>(op, c, K, vs, gate, arg, ccs, her);
 	if (ev) HIPCHK(hipEventRecord(ev[1], op->stream));
-	launch_zfft<T, false>(op, c);
+	launch_zfft<T, false>(op, c, K, gate);
 	if (ev) HIPCHK(hipEventRecord(ev[2], op->stream));
-	launch_p3<T>(op, c, D, transposed);
+	launch_p3<T>(op, c, K, gate, D, transposed);
 	if (ev) HIPCHK(hipEventRecord(ev[3], op->stream));
-	launch_zfft<T, true>(op, c);
+	launch_zfft<T, true>(op, c, K, gate);
 	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
-	const int nblk5 = launch_p5<T>(op, c, arg, res, ccs, her, (inprod || keep_partials) ? op->partials : nullptr);
+	const int nblk5 = launch_p5<T>(op, c, K, vs, gate, arg, res, ccs, her,
+		(inprod || keep_partials) ? op->partials : nullptr);
 	if (nblk_out) *nblk_out = nblk5;
 	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
-	if (inprod) {
-		double v;
-		if (reduce_finish(op, nblk5, 1, &v)) return -1;
-		*inprod = v;
-	}
+	if (inprod) return reduce_finish(op, nblk5, K, inprod);
 	return 0;
 }
 
@@ -837,14 +920,30 @@ __device__ void dk_reduce(const double *partials, int nblk, int nrows, double *o
 
 #define DK_ACTIVE(st) if ((st)[DK_STATUS] != 0) return
 
+// Batched solves: the state of RHS k is st + k*DK_NSLOTS, its vectors are at offset k*vs, its partial sums at
+// partials + k*ps. Vector kernels take k = blockIdx.y, one-block scalar kernels k = blockIdx.x.
+#define DK_RHS_VEC(st, partials, ps)                                                                              \
+	const int kq = blockIdx.y;                                                                                   \
+	st += (size_t)kq * DK_NSLOTS;                                                                                \
+	partials += (size_t)kq * (ps);                                                                               \
+	const size_t voff = (size_t)kq * vs
+#define DK_RHS_SCALAR(st, partials, ps)                                                                           \
+	st += (size_t)blockIdx.x * DK_NSLOTS;                                                                        \
+	partials += (size_t)blockIdx.x * (ps)
+
 // out = st[ia]*a + st[ib]*b + st[ic]*c (complex coefficient slots; null vectors skipped); red: 0 none, 1 ||out||^2
 // (row 0), 2 self-bilinear sums re^2, im^2, re*im (rows 0-2).
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const cx<T> *b, const cx<T> *c,
-	const double *st, int ia, int ib, int ic, size_t n, int red, double *partials)
+	const double *st, int ia, int ib, int ic, size_t n, long vs, int red, double *partials, int ps)
 {
 	__shared__ double red_lds[512];
+	DK_RHS_VEC(st, partials, ps);
 	DK_ACTIVE(st);
+	out += voff;
+	if (a) a += voff;
+	if (b) b += voff;
+	if (c) c += voff;
 	const dcx ca = ld_c(st, ia), cb = ld_c(st, ib), cc = ld_c(st, ic);
 	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -874,10 +973,17 @@ __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const
 // x += st[ix1]*p + st[ix2]*s2v ; r = st[ir1]*r1 + st[ir2]*r2 ; partial ||r||^2 (BiCGStab and CGNR tails)
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const cx<T> *s2v, cx<T> *r, const cx<T> *r1,
-	const cx<T> *r2, const double *st, int ix1, int ix2, int ir1, int ir2, size_t n, double *partials)
+	const cx<T> *r2, const double *st, int ix1, int ix2, int ir1, int ir2, size_t n, long vs, double *partials, int ps)
 {
 	__shared__ double red_lds[512];
+	DK_RHS_VEC(st, partials, ps);
 	DK_ACTIVE(st);
+	x += voff;
+	p += voff;
+	if (s2v) s2v += voff;
+	r += voff;
+	r1 += voff;
+	r2 += voff;
 	const dcx a1 = ld_c(st, ix1), a2 = ld_c(st, ix2), b1 = ld_c(st, ir1), b2 = ld_c(st, ir2);
 	double s0 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -901,10 +1007,18 @@ __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const c
 // vtilda = vt; r = |s|^2 r + cr*vt; partial ||r||^2.
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_qmr_upd(cx<T> *pdst, const cx<T> *pa, const cx<T> *pb, const cx<T> *v,
-	cx<T> *x, cx<T> *vtilda, cx<T> *r, const double *st, size_t n, double *partials)
+	cx<T> *x, cx<T> *vtilda, cx<T> *r, const double *st, size_t n, long vs, double *partials, int ps)
 {
 	__shared__ double red_lds[512];
+	DK_RHS_VEC(st, partials, ps);
 	DK_ACTIVE(st);
+	pdst += voff;
+	if (pa) pa += voff;
+	if (pb) pb += voff;
+	v += voff;
+	x += voff;
+	vtilda += voff;
+	r += voff;
 	const dcx q0 = ld_c(st, DK_SC + 30), q1 = ld_c(st, DK_SC + 32), q2 = ld_c(st, DK_SC + 34);
 	const dcx tau = ld_c(st, DK_SC + 36), ib = ld_c(st, DK_SC + 38), cr = ld_c(st, DK_SC + 40);
 	const double cs = st[DK_SC + 42];
@@ -945,21 +1059,24 @@ __device__ void dk_progress(double *st, double *hist, double inprodRp1)
 	else if (st[DK_COUNTER] > st[DK_MC]) st[DK_STATUS] = -2;
 }
 
-__global__ void __launch_bounds__(256) k_dk_progress(double *st, double *hist, const double *partials, int nblk)
+__global__ void __launch_bounds__(256) k_dk_progress(double *st, double *hist, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
+	if (hist) hist += (size_t)blockIdx.x * (size_t)st[DK_HISTLEN];
 	dk_reduce(partials, nblk, 1, v, red_lds);
 	if (threadIdx.x == 0) dk_progress(st, hist, v[0]);
 }
 
 // QMR_CS scalar step 1 (src/iterative.c:962-982): breakdown test, alpha = v.Av, vtilda coefficients, theta, eta,
 // zetatilda.
-__global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *partials, int nblk, int niter)
+__global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 2, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -984,10 +1101,11 @@ __global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *par
 }
 
 // QMR_CS scalar step 2 (src/iterative.c:983-1029): beta, omega, Givens rotation, p/x/r coefficients.
-__global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *partials, int nblk, int niter)
+__global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 3, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1027,10 +1145,11 @@ __global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *par
 
 // BiCGStab scalar steps (src/iterative.c:675-718). Slots: 0 ro_new, 2 ro_old, 4 alpha, 6 omega, 8 beta,
 // 20 p-coef on p, 22 p-coef on v, 24 one, 26 -alpha, 28 alpha, 30 omega (x), 32 -omega (r), 34 denumOmega.
-__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter)
+__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 2, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1045,10 +1164,11 @@ __global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *part
 	st_c(st, DK_SC + 22, dneg(dmul(beta, ld_c(st, DK_SC + 6))));
 }
 
-__global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *partials, int nblk)
+__global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 2, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1062,29 +1182,32 @@ __global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *part
 
 // ||s||^2 -> half-step convergence test (src/iterative.c:700-707): if converged, omega := 0 so the tail computes
 // x += alpha p and r := s (whose norm is the one reported).
-__global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist)
+__global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 1, v, red_lds);
 	if (threadIdx.x != 0) return;
 	st[DK_SC + 36] = v[0] < st[DK_EPSB] ? 1 : 0;  // half-step flag
 }
 
-__global__ void __launch_bounds__(256) k_dk_bs_s4(double *st, const double *pt, int nblk_t)
+__global__ void __launch_bounds__(256) k_dk_bs_s4(double *st, const double *pt, int nblk_t, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
+	DK_RHS_SCALAR(st, pt, ps);
 	DK_ACTIVE(st);
 	dk_reduce(pt, nblk_t, 1, v, red_lds);
 	if (threadIdx.x == 0) st[DK_SC + 34] = v[0];
 }
 
-__global__ void __launch_bounds__(256) k_dk_bs_s5(double *st, const double *partials, int nblk)
+__global__ void __launch_bounds__(256) k_dk_bs_s5(double *st, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
 	dk_reduce(partials, nblk, 2, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1097,10 +1220,11 @@ __global__ void __launch_bounds__(256) k_dk_bs_s5(double *st, const double *part
 
 // CGNR scalar steps (src/iterative.c:745-766). Slots: 0 ro_new, 1 ro_old, 2 alpha, 4 beta (c), 6 one, 8 alpha (c),
 // 10 -alpha (c), 12 zero.
-__global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, int nblk_t, int niter)
+__global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, int nblk_t, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
+	DK_RHS_SCALAR(st, pt, ps);
 	DK_ACTIVE(st);
 	dk_reduce(pt, nblk_t, 1, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1108,10 +1232,11 @@ __global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, 
 	if (niter > 1) st_c(st, DK_SC + 4, dmk(v[0] / st[DK_SC + 1], 0));
 }
 
-__global__ void __launch_bounds__(256) k_dk_cg_s2(double *st, const double *pt, int nblk_t)
+__global__ void __launch_bounds__(256) k_dk_cg_s2(double *st, const double *pt, int nblk_t, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
+	DK_RHS_SCALAR(st, pt, ps);
 	DK_ACTIVE(st);
 	dk_reduce(pt, nblk_t, 1, v, red_lds);
 	if (threadIdx.x != 0) return;
@@ -1123,50 +1248,64 @@ __global__ void __launch_bounds__(256) k_dk_cg_s2(double *st, const double *pt, 
 }
 
 struct dk_state {
-	double *st;    // [DK_NSLOTS]
-	double *hist;  // [histlen]
+	double *st;    // [K][DK_NSLOTS]
+	double *hist;  // [K][histlen]
 	int histlen;
+	int K;         // right-hand sides (batched solve when > 1)
 };
 
-template <typename T>
-static int dot_partials(adda_hip_op *op, const void *a, const void *b, int mode, int *nb_out)
+// Launch geometry of the chain of one device-resident iteration: vector kernels (nb, K) with partial sums in
+// op->partials2 at RHS stride ps2 = 3*nb; scalar kernels K blocks.
+struct DkGeo {
+	int K;
+	long vs;      // vector stride between RHS (0 for a single RHS)
+	size_t n;     // elements per RHS vector
+	int nb, ps2;
+};
+static DkGeo dk_geo(const adda_hip_op *op, const dk_state *k)
 {
-	const size_t n = 3 * op->nd;
-	const int nb = nblocks_red(n);
-	hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, op->stream, (const cx<T> *)a, (const cx<T> *)b, mode, n,
-		op->partials2, nb, op->gate);
-	*nb_out = nb;
-	return 0;
+	DkGeo g;
+	g.K = k->K;
+	g.vs = k->K > 1 ? (long)op->vs : 0;
+	g.n = 3 * op->nd;
+	g.nb = nblocks_red(g.n);
+	g.ps2 = 3 * g.nb;
+	return g;
 }
 
 template <typename T>
-static void lc_dev(adda_hip_op *op, void *out, const void *a, const void *b, const void *c, const double *st, int ia,
-	int ib, int ic, int red)
+static void dot_dev(adda_hip_op *op, const DkGeo &g, const double *gate, const void *a, const void *b, int mode)
 {
-	const size_t n = 3 * op->nd;
-	hipLaunchKernelGGL(k_dk_lc<T>, dim3(nblocks_red(n)), dim3(256), 0, op->stream, (cx<T> *)out, (const cx<T> *)a,
-		(const cx<T> *)b, (const cx<T> *)c, st, ia, ib, ic, n, red, op->partials2);
+	hipLaunchKernelGGL(k_dot<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (const cx<T> *)a, (const cx<T> *)b, mode,
+		g.n, g.vs, op->partials2, g.nb, gate);
+}
+
+template <typename T>
+static void lc_dev(adda_hip_op *op, const DkGeo &g, void *out, const void *a, const void *b, const void *c,
+	const double *st, int ia, int ib, int ic, int red)
+{
+	hipLaunchKernelGGL(k_dk_lc<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)out, (const cx<T> *)a,
+		(const cx<T> *)b, (const cx<T> *)c, st, ia, ib, ic, g.n, g.vs, red, op->partials2, g.ps2);
 }
 
 template <typename T>
 static int qmr_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *v, void *vtilda, void *Av,
 	void *p_old, void *p_new, void *x, void *r)
 {
-	const size_t n = 3 * op->nd;
-	const int nb = nblocks_red(n);
+	const DkGeo g = dk_geo(op, k);
 	double *st = k->st;
-	int nbd;
-	if (niter == 1 && matvec_ready) lc_dev<T>(op, Av, Av, nullptr, nullptr, st, DK_SC + 44, DK_SC + 44, DK_SC + 44, 0);
-	else if (matvec_t<T>(op, (const cx<T> *)v, (cx<T> *)Av, nullptr, 0)) return -1;
-	dot_partials<T>(op, v, Av, ADDA_DOT_BILIN, &nbd);
-	hipLaunchKernelGGL(k_dk_qmr_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd, niter);
-	lc_dev<T>(op, vtilda, niter == 1 ? nullptr : vtilda, v, Av, st, DK_SC + 24, DK_SC + 26, DK_SC + 28, 2);
-	hipLaunchKernelGGL(k_dk_qmr_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nb, niter);
+	if (niter == 1 && matvec_ready) lc_dev<T>(op, g, Av, Av, nullptr, nullptr, st, DK_SC + 44, DK_SC + 44, DK_SC + 44, 0);
+	else if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)v, (cx<T> *)Av, nullptr, 0)) return -1;
+	dot_dev<T>(op, g, st, v, Av, ADDA_DOT_BILIN);
+	hipLaunchKernelGGL(k_dk_qmr_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	lc_dev<T>(op, g, vtilda, niter == 1 ? nullptr : vtilda, v, Av, st, DK_SC + 24, DK_SC + 26, DK_SC + 28, 2);
+	hipLaunchKernelGGL(k_dk_qmr_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	void *pdst = niter == 1 ? p_new : p_old;
 	const void *pa = niter >= 3 ? p_old : nullptr, *pb = niter >= 2 ? p_new : nullptr;
-	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
-		(const cx<T> *)pb, (const cx<T> *)v, (cx<T> *)x, (cx<T> *)vtilda, (cx<T> *)r, st, n, op->partials2);
-	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
+		(const cx<T> *)pb, (const cx<T> *)v, (cx<T> *)x, (cx<T> *)vtilda, (cx<T> *)r, st, g.n, g.vs, op->partials2,
+		g.ps2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1175,28 +1314,27 @@ template <typename T>
 static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *x, void *r, void *p,
 	void *v, void *s, void *rt, void *t)
 {
-	const size_t n = 3 * op->nd;
-	const int nb = nblocks_red(n);
+	const DkGeo g = dk_geo(op, k);
 	double *st = k->st;
-	int nbd, nbt;
-	dot_partials<T>(op, r, rt, ADDA_DOT_CONJ, &nbd);
-	hipLaunchKernelGGL(k_dk_bs_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd, niter);
+	int nbt;
+	dot_dev<T>(op, g, st, r, rt, ADDA_DOT_CONJ);
+	hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	// p = beta p - beta omega v + r  (niter 1: p = r)
-	lc_dev<T>(op, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
-	if (niter == 1 && matvec_ready) lc_dev<T>(op, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
-	else if (matvec_t<T>(op, (const cx<T> *)p, (cx<T> *)v, nullptr, 0)) return -1;
-	dot_partials<T>(op, v, rt, ADDA_DOT_CONJ, &nbd);
-	hipLaunchKernelGGL(k_dk_bs_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd);
-	lc_dev<T>(op, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
-	hipLaunchKernelGGL(k_dk_bs_s3, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nb, k->hist);
-	if (matvec_t<T>(op, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_bs_s4, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt);
-	dot_partials<T>(op, s, t, ADDA_DOT_CONJ, &nbd);
-	hipLaunchKernelGGL(k_dk_bs_s5, dim3(1), dim3(256), 0, op->stream, st, op->partials2, nbd);
-	hipLaunchKernelGGL(k_dk_xr<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
+	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
+	if (niter == 1 && matvec_ready) lc_dev<T>(op, g, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
+	else if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)v, nullptr, 0)) return -1;
+	dot_dev<T>(op, g, st, v, rt, ADDA_DOT_CONJ);
+	hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	lc_dev<T>(op, g, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
+	hipLaunchKernelGGL(k_dk_bs_s3, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, k->hist, g.ps2);
+	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_bs_s4, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, nbt);
+	dot_dev<T>(op, g, st, s, t, ADDA_DOT_CONJ);
+	hipLaunchKernelGGL(k_dk_bs_s5, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
-		DK_SC + 32, n, op->partials2);
-	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1204,19 +1342,19 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 template <typename T>
 static int cgnr_iter_t(adda_hip_op *op, dk_state *k, int niter, void *x, void *r, void *p, void *Av)
 {
-	const size_t n = 3 * op->nd;
-	const int nb = nblocks_red(n);
+	const DkGeo g = dk_geo(op, k);
 	double *st = k->st;
 	int nbt;
-	if (matvec_t<T>(op, (const cx<T> *)r, (cx<T> *)(niter == 1 ? p : Av), nullptr, 1, nullptr, true, &nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_cg_s1, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt, niter);
-	if (niter > 1) lc_dev<T>(op, p, p, Av, nullptr, st, DK_SC + 4, DK_SC + 6, DK_SC + 6, 0);
-	if (matvec_t<T>(op, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0, nullptr, true, &nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_cg_s2, dim3(1), dim3(256), 0, op->stream, st, op->partials, nbt);
-	hipLaunchKernelGGL(k_dk_xr<T>, dim3(nb), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
+	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)r, (cx<T> *)(niter == 1 ? p : Av), nullptr, 1, nullptr, true,
+		&nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_cg_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, niter, nbt);
+	if (niter > 1) lc_dev<T>(op, g, p, p, Av, nullptr, st, DK_SC + 4, DK_SC + 6, DK_SC + 6, 0);
+	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0, nullptr, true, &nbt)) return -1;
+	hipLaunchKernelGGL(k_dk_cg_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, nbt);
+	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)nullptr, (cx<T> *)r, (const cx<T> *)r, (const cx<T> *)Av, st, DK_SC + 8, DK_SC + 12, DK_SC + 6,
-		DK_SC + 10, n, op->partials2);
-	hipLaunchKernelGGL(k_dk_progress, dim3(1), dim3(256), 0, op->stream, st, k->hist, op->partials2, nb);
+		DK_SC + 10, g.n, g.vs, op->partials2, g.ps2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1241,6 +1379,51 @@ extern "C" int adda_hip_fftfit(int x, int divis)
 		while (y % 5 == 0) y /= 5;
 		if (y == 1) return x;
 	}
+}
+
+// (Re)allocates the per-RHS work buffers, cc_sqrt tables and reduction space for K right-hand sides (no-op when
+// the operator already holds K); the cc_sqrt tables already set are kept. Synchronises the operator's stream.
+static int reserve(adda_hip_op *op, int K)
+{
+	if (K < 1) return ERR("batch of %d right-hand sides", K);
+	if (K <= op->kcap) return 0;
+	HIPCHK(hipSetDevice(op->dev));
+	if (op->stream) HIPCHK(hipStreamSynchronize(op->stream));
+	const size_t nc = (size_t)op->nmat * 3;
+	cx<double> *c64 = nullptr;
+	cx<float> *c32 = nullptr;
+	HIPCHK(hipMalloc(&c64, K * nc * sizeof(cx<double>)));
+	HIPCHK(hipMalloc(&c32, K * nc * sizeof(cx<float>)));
+	HIPCHK(hipMemset(c64, 0, K * nc * sizeof(cx<double>)));
+	HIPCHK(hipMemset(c32, 0, K * nc * sizeof(cx<float>)));
+	if (op->cc64) {
+		HIPCHK(hipMemcpy(c64, op->cc64, op->kcap * nc * sizeof(cx<double>), hipMemcpyDeviceToDevice));
+		HIPCHK(hipMemcpy(c32, op->cc32, op->kcap * nc * sizeof(cx<float>), hipMemcpyDeviceToDevice));
+		HIPCHK(hipFree(op->cc64));
+		HIPCHK(hipFree(op->cc32));
+	}
+	op->cc64 = c64;
+	op->cc32 = c32;
+	void **bufs[] = {&op->Xa, &op->Xb, (void **)&op->partials, (void **)&op->partials2, (void **)&op->red,
+		(void **)&op->coef};
+	for (void **b : bufs)
+		if (*b) { HIPCHK(hipFree(*b)); *b = nullptr; }
+	if (op->red_host) { HIPCHK(hipHostFree(op->red_host)); op->red_host = nullptr; }
+	op->kcap = 0;
+	HIPCHK(hipMalloc(&op->Xa, (size_t)K * xa_stride(op) * sizeof(cx<double>)));
+	HIPCHK(hipMalloc(&op->Xb, (size_t)K * xb_stride(op) * sizeof(cx<double>)));
+	// partials: field sums, K rows of gather blocks (LY >= 1), K rows of up to 3*kRedBlocks vector-kernel blocks
+	size_t per = (size_t)3 * op->bZ * op->bY;
+	if (per < (size_t)3 * kRedBlocks) per = (size_t)3 * kRedBlocks;
+	op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);
+	if ((size_t)K * per > op->npartials) op->npartials = (size_t)K * per;
+	HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
+	HIPCHK(hipMalloc(&op->partials2, (size_t)K * 4 * kRedBlocks * sizeof(double)));
+	HIPCHK(hipMalloc(&op->red, (size_t)4 * K * sizeof(double)));
+	HIPCHK(hipHostMalloc((void **)&op->red_host, (size_t)4 * K * sizeof(double), hipHostMallocDefault));
+	HIPCHK(hipMalloc(&op->coef, (size_t)6 * K * sizeof(double)));
+	op->kcap = K;
+	return 0;
 }
 
 static int op_build(adda_hip_op *op, const unsigned short *position, const int *material)
@@ -1282,22 +1465,9 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	HIPCHK(hipMemcpy(op->pos, position, 3 * op->nd * sizeof(unsigned short), hipMemcpyHostToDevice));
 	HIPCHK(hipMalloc(&op->wbuf, (size_t)op->nmat * 3 * sizeof(double)));
 	HIPCHK(hipMalloc(&op->dirbuf, (size_t)3 * kMaxDir * sizeof(double)));
-	HIPCHK(hipMalloc(&op->cc64, (size_t)op->nmat * 3 * sizeof(cx<double>)));
-	HIPCHK(hipMalloc(&op->cc32, (size_t)op->nmat * 3 * sizeof(cx<float>)));
-	HIPCHK(hipMemset(op->cc64, 0, (size_t)op->nmat * 3 * sizeof(cx<double>)));
-	HIPCHK(hipMemset(op->cc32, 0, (size_t)op->nmat * 3 * sizeof(cx<float>)));
-	// work buffers
-	HIPCHK(hipMalloc(&op->Xa, (size_t)3 * bZ * bY * gX * sizeof(cx<double>)));
-	HIPCHK(hipMalloc(&op->Xb, (size_t)3 * gZ * bY * gX * sizeof(cx<double>)));
-	{
-		const size_t nblk5 = (size_t)3 * bZ * bY;  // upper bound of gather blocks (LY >= 1)
-		op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);  // field sums need the most
-		if (nblk5 > op->npartials) op->npartials = nblk5;
-		HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
-		HIPCHK(hipMalloc(&op->partials2, (size_t)4 * kRedBlocks * sizeof(double)));
-	}
-	HIPCHK(hipMalloc(&op->red, 4 * sizeof(double)));
-	HIPCHK(hipHostMalloc((void **)&op->red_host, 4 * sizeof(double), hipHostMallocDefault));
+	// work buffers, cc_sqrt table and reduction space of one right-hand side (grown by batched solves)
+	op->vs = (3 * op->nd + 63) / 64 * 64;
+	if (reserve(op, 1)) return -1;
 	// D-matrix: G on the full grid, 3D FFT (x rows, y and z columns), extraction of the stored octant
 	op->DsY = op->reduced ? gY / 2 + 1 : gY;
 	op->DsZ = op->reduced ? gZ / 2 + 1 : gZ;
@@ -1378,8 +1548,9 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (!op) return;
 	(void)hipSetDevice(op->dev);
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
-	void *ptrs[] = {op->partials2, op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64, op->D32, op->Xa, op->Xb, op->partials, op->red,
-		op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d, op->twz.f};
+	void *ptrs[] = {op->partials2, op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
+		op->D32, op->Xa, op->Xb, op->partials, op->red, op->coef, op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d,
+		op->twz.f};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	if (op->red_host) (void)hipHostFree(op->red_host);
@@ -1389,9 +1560,23 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 
 extern "C" int adda_hip_set_cc_sqrt(adda_hip_op *op, const adda_cplx *cc_sqrt)
 {
+	return adda_hip_set_cc_sqrt_batch(op, 1, cc_sqrt);
+}
+
+extern "C" int adda_hip_reserve_batch(adda_hip_op *op, int K)
+{
+	if (!op) return ERR("null operator");
+	return reserve(op, K);
+}
+
+extern "C" size_t adda_hip_batch_stride(const adda_hip_op *op) { return op ? op->vs : 0; }
+
+extern "C" int adda_hip_set_cc_sqrt_batch(adda_hip_op *op, int K, const adda_cplx *cc_sqrt)
+{
 	if (!op || !cc_sqrt) return ERR("null argument");
+	if (reserve(op, K)) return -1;
 	HIPCHK(hipSetDevice(op->dev));
-	const size_t n = (size_t)op->nmat * 3;
+	const size_t n = (size_t)op->nmat * 3 * K;
 	std::vector<float> f(2 * n);
 	for (size_t i = 0; i < n; i++) { f[2 * i] = (float)cc_sqrt[i].re; f[2 * i + 1] = (float)cc_sqrt[i].im; }
 	HIPCHK(hipMemcpyAsync(op->cc64, cc_sqrt, n * sizeof(cx<double>), hipMemcpyHostToDevice, op->stream));
@@ -1422,14 +1607,21 @@ extern "C" int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst)
 
 extern "C" size_t adda_hip_nrows(const adda_hip_op *op) { return op ? 3 * op->nd : 0; }
 
-extern "C" void *adda_hip_vec_alloc(adda_hip_op *op, int prec)
+// elements of a device vector of K right-hand sides (one RHS: 3*nd, unpadded)
+static size_t vec_elems(const adda_hip_op *op, int K) { return K <= 1 ? 3 * op->nd : (size_t)K * op->vs; }
+
+extern "C" void *adda_hip_vec_alloc(adda_hip_op *op, int prec) { return adda_hip_vec_alloc_batch(op, prec, 1); }
+
+extern "C" void *adda_hip_vec_alloc_batch(adda_hip_op *op, int prec, int K)
 {
 	if (!op) { ERR("null operator"); return nullptr; }
+	if (K < 1) { ERR("batch of %d right-hand sides", K); return nullptr; }
 	if (hipSetDevice(op->dev) != hipSuccess) { ERR("hipSetDevice failed"); return nullptr; }
 	void *p = nullptr;
-	hipError_t e = hipMalloc(&p, 3 * op->nd * elsize(prec));
+	const size_t bytes = vec_elems(op, K) * elsize(prec);
+	hipError_t e = hipMalloc(&p, bytes);
 	if (e != hipSuccess) { ERR("hipMalloc of a vector failed: %s", hipGetErrorString(e)); return nullptr; }
-	if (hipMemsetAsync(p, 0, 3 * op->nd * elsize(prec), op->stream) != hipSuccess) { ERR("memset failed"); return nullptr; }
+	if (hipMemsetAsync(p, 0, bytes, op->stream) != hipSuccess) { ERR("memset failed"); return nullptr; }
 	return p;
 }
 
@@ -1480,9 +1672,14 @@ extern "C" int adda_hip_vec_download(adda_hip_op *op, int prec, adda_cplx *dst, 
 
 extern "C" int adda_hip_vec_convert(adda_hip_op *op, void *dst, int dprec, const void *src, int sprec)
 {
-	if (!op || !dst || !src) return ERR("null argument");
+	return adda_hip_vec_convert_batch(op, 1, dst, dprec, src, sprec);
+}
+
+extern "C" int adda_hip_vec_convert_batch(adda_hip_op *op, int K, void *dst, int dprec, const void *src, int sprec)
+{
+	if (!op || !dst || !src || K < 1) return ERR("bad argument");
 	HIPCHK(hipSetDevice(op->dev));
-	const size_t n = 3 * op->nd;
+	const size_t n = vec_elems(op, K);
 	const int nb = nblocks_red(n);
 	if (dprec == sprec) HIPCHK(hipMemcpyAsync(dst, src, n * elsize(dprec), hipMemcpyDeviceToDevice, op->stream));
 	else if (dprec == ADDA_F32)
@@ -1497,11 +1694,28 @@ extern "C" int adda_hip_vec_convert(adda_hip_op *op, void *dst, int dprec, const
 
 extern "C" int adda_hip_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, double *inprod, int her)
 {
+	return adda_hip_matvec_batch(op, prec, 1, nullptr, argvec, resultvec, inprod, her);
+}
+
+static const double *gate_of(const void *state, int K)
+{
+	if (!state) return nullptr;
+	const dk_state *k = (const dk_state *)state;
+	return K <= k->K ? k->st : nullptr;
+}
+
+extern "C" int adda_hip_matvec_batch(adda_hip_op *op, int prec, int K, const void *gate_state, const void *argvec,
+	void *resultvec, double *inprod, int her)
+{
 	if (!op || !argvec || !resultvec) return ERR("null argument");
 	if (argvec == resultvec) return ERR("argvec and resultvec must not alias (src/matvec.c:128 restrict)");
+	if (gate_state && !gate_of(gate_state, K)) return ERR("gate state holds fewer than %d right-hand sides", K);
 	HIPCHK(hipSetDevice(op->dev));
-	if (prec == ADDA_F64) return matvec_t<double>(op, (const cx<double> *)argvec, (cx<double> *)resultvec, inprod, her);
-	return matvec_t<float>(op, (const cx<float> *)argvec, (cx<float> *)resultvec, inprod, her);
+	const long vs = K > 1 ? (long)op->vs : 0;
+	const double *gate = gate_of(gate_state, K);
+	if (prec == ADDA_F64)
+		return matvec_t<double>(op, K, vs, gate, (const cx<double> *)argvec, (cx<double> *)resultvec, inprod, her);
+	return matvec_t<float>(op, K, vs, gate, (const cx<float> *)argvec, (cx<float> *)resultvec, inprod, her);
 }
 
 extern "C" int adda_hip_matvec_host(adda_hip_op *op, const adda_cplx *argvec, adda_cplx *resultvec, double *inprod,
@@ -1518,64 +1732,113 @@ extern "C" int adda_hip_matvec_host(adda_hip_op *op, const adda_cplx *argvec, ad
 	return rc;
 }
 
+template <typename T>
+static int lincomb_t(adda_hip_op *op, int K, const double *gate, void *out, const void *a, const void *b,
+	const void *c, const adda_cplx *ca, const adda_cplx *cb, const adda_cplx *cc, int conj_flags, double *norm2)
+{
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	const long vs = K > 1 ? (long)op->vs : 0;
+	cx<double> A = mk<double>(ca[0].re, ca[0].im), B = mk<double>(cb[0].re, cb[0].im), C = mk<double>(cc[0].re, cc[0].im);
+	const double *coefs = nullptr;
+	if (K > 1) {  // per-RHS coefficients through the device table (stream-ordered after earlier users of it)
+		std::vector<double> h(6 * (size_t)K);
+		for (int k = 0; k < K; k++) {
+			h[6 * k] = ca[k].re; h[6 * k + 1] = ca[k].im;
+			h[6 * k + 2] = cb[k].re; h[6 * k + 3] = cb[k].im;
+			h[6 * k + 4] = cc[k].re; h[6 * k + 5] = cc[k].im;
+		}
+		HIPCHK(hipMemcpyAsync(op->coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, op->stream));
+		coefs = op->coef;
+	}
+	hipLaunchKernelGGL(k_lincomb<T>, dim3(nb, K), dim3(256), 0, op->stream, (cx<T> *)out, (const cx<T> *)a, A,
+		(const cx<T> *)b, B, (const cx<T> *)c, C, coefs, conj_flags, n, vs, norm2 ? op->partials : nullptr, gate);
+	HIPCHK(hipGetLastError());
+	if (norm2) return reduce_rows(op, op->partials, nb, K, norm2);
+	return 0;
+}
+
 extern "C" int adda_hip_lincomb(adda_hip_op *op, int prec, void *out, const void *a, adda_cplx ca, const void *b,
 	adda_cplx cb, const void *c, adda_cplx cc, int conj_flags, double *norm2)
 {
-	if (!op || !out) return ERR("null argument");
+	return adda_hip_lincomb_batch(op, prec, 1, nullptr, out, a, &ca, b, &cb, c, &cc, conj_flags, norm2);
+}
+
+extern "C" int adda_hip_lincomb_batch(adda_hip_op *op, int prec, int K, const void *gate_state, void *out,
+	const void *a, const adda_cplx *ca, const void *b, const adda_cplx *cb, const void *c, const adda_cplx *cc,
+	int conj_flags, double *norm2)
+{
+	if (!op || !out || !ca || !cb || !cc) return ERR("null argument");
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
+	if (gate_state && !gate_of(gate_state, K)) return ERR("gate state holds fewer than %d right-hand sides", K);
 	HIPCHK(hipSetDevice(op->dev));
-	const size_t n = 3 * op->nd;
-	const int nb = nblocks_red(n);
-	cx<double> A = mk<double>(ca.re, ca.im), B = mk<double>(cb.re, cb.im), Cc = mk<double>(cc.re, cc.im);
-	double *part = norm2 ? op->partials : nullptr;
-	if (prec == ADDA_F64)
-		hipLaunchKernelGGL(k_lincomb<double>, dim3(nb), dim3(256), 0, op->stream, (cx<double> *)out,
-			(const cx<double> *)a, A, (const cx<double> *)b, B, (const cx<double> *)c, Cc, conj_flags, n, part);
-	else
-		hipLaunchKernelGGL(k_lincomb<float>, dim3(nb), dim3(256), 0, op->stream, (cx<float> *)out,
-			(const cx<float> *)a, A, (const cx<float> *)b, B, (const cx<float> *)c, Cc, conj_flags, n, part);
-	HIPCHK(hipGetLastError());
-	if (norm2) return reduce_finish(op, nb, 1, norm2);
-	return 0;
+	const double *gate = gate_of(gate_state, K);
+	if (norm2 && gate)  // skipped RHS leave their partials untouched: report zeros for them
+		HIPCHK(hipMemsetAsync(op->partials, 0, (size_t)nblocks_red(3 * op->nd) * K * sizeof(double), op->stream));
+	if (prec == ADDA_F64) return lincomb_t<double>(op, K, gate, out, a, b, c, ca, cb, cc, conj_flags, norm2);
+	return lincomb_t<float>(op, K, gate, out, a, b, c, ca, cb, cc, conj_flags, norm2);
 }
 
 extern "C" int adda_hip_dot(adda_hip_op *op, int prec, const void *a, const void *b, int mode, adda_cplx *res,
 	double *norm2)
 {
+	return adda_hip_dot_batch(op, prec, 1, nullptr, a, b, mode, res, norm2);
+}
+
+extern "C" int adda_hip_dot_batch(adda_hip_op *op, int prec, int K, const void *gate_state, const void *a,
+	const void *b, int mode, adda_cplx *res, double *norm2)
+{
 	if (!op || !a) return ERR("null argument");
 	if ((mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) && !b) return ERR("dot mode %d needs two vectors", mode);
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
+	if (gate_state && !gate_of(gate_state, K)) return ERR("gate state holds fewer than %d right-hand sides", K);
 	HIPCHK(hipSetDevice(op->dev));
 	const size_t n = 3 * op->nd;
 	const int nb = nblocks_red(n);
+	const long vs = K > 1 ? (long)op->vs : 0;
+	const double *gate = gate_of(gate_state, K);
+	if (gate)  // skipped RHS leave their partials untouched: report zeros for them
+		HIPCHK(hipMemsetAsync(op->partials, 0, (size_t)3 * nb * K * sizeof(double), op->stream));
 	if (prec == ADDA_F64)
-		hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(256), 0, op->stream, (const cx<double> *)a,
-			(const cx<double> *)b, mode, n, op->partials, nb, (const double *)nullptr);
+		hipLaunchKernelGGL(k_dot<double>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<double> *)a,
+			(const cx<double> *)b, mode, n, vs, op->partials, nb, gate);
 	else
-		hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(256), 0, op->stream, (const cx<float> *)a,
-			(const cx<float> *)b, mode, n, op->partials, nb, (const double *)nullptr);
+		hipLaunchKernelGGL(k_dot<float>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<float> *)a,
+			(const cx<float> *)b, mode, n, vs, op->partials, nb, gate);
 	HIPCHK(hipGetLastError());
-	double v[3];
-	if (reduce_finish(op, nb, 3, v)) return -1;
-	if (mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) {
-		if (res) { res->re = v[0]; res->im = v[1]; }
-	}
-	else {
-		if (res) { res->re = v[0] - v[1]; res->im = 2 * v[2]; }  // src/linalg.c:143
-		if (norm2) *norm2 = v[0] + v[1];
+	std::vector<double> v((size_t)3 * K);
+	if (reduce_rows(op, op->partials, nb, 3 * K, v.data())) return -1;
+	for (int k = 0; k < K; k++) {
+		const double *w = &v[3 * (size_t)k];
+		if (mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) {
+			if (res) { res[k].re = w[0]; res[k].im = w[1]; }
+		}
+		else {
+			if (res) { res[k].re = w[0] - w[1]; res[k].im = 2 * w[2]; }  // src/linalg.c:143
+			if (norm2) norm2[k] = w[0] + w[1];
+		}
 	}
 	return 0;
 }
 
 extern "C" int adda_hip_mult_mat(adda_hip_op *op, int prec, void *out, const void *in)
 {
+	return adda_hip_mult_mat_batch(op, prec, 1, out, in);
+}
+
+extern "C" int adda_hip_mult_mat_batch(adda_hip_op *op, int prec, int K, void *out, const void *in)
+{
 	if (!op || !out || !in) return ERR("null argument");
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
 	HIPCHK(hipSetDevice(op->dev));
 	const int nb = nblocks_red(3 * op->nd);
+	const long vs = K > 1 ? (long)op->vs : 0;
 	if (prec == ADDA_F64)
-		hipLaunchKernelGGL(k_mult_mat<double>, dim3(nb), dim3(256), 0, op->stream, (cx<double> *)out,
-			(const cx<double> *)in, op->mat, op->cc64, op->nd);
+		hipLaunchKernelGGL(k_mult_mat<double>, dim3(nb, K), dim3(256), 0, op->stream, (cx<double> *)out,
+			(const cx<double> *)in, op->mat, op->cc64, op->nd, vs, 3 * op->nmat);
 	else
-		hipLaunchKernelGGL(k_mult_mat<float>, dim3(nb), dim3(256), 0, op->stream, (cx<float> *)out,
-			(const cx<float> *)in, op->mat, op->cc32, op->nd);
+		hipLaunchKernelGGL(k_mult_mat<float>, dim3(nb, K), dim3(256), 0, op->stream, (cx<float> *)out,
+			(const cx<float> *)in, op->mat, op->cc32, op->nd, vs, 3 * op->nmat);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1645,15 +1908,24 @@ extern "C" int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v,
 extern "C" int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, int nrep,
 	double *pass_ms, double *total_ms)
 {
+	return adda_hip_profile_matvec_batch(op, prec, 1, argvec, resultvec, nrep, pass_ms, total_ms);
+}
+
+extern "C" int adda_hip_profile_matvec_batch(adda_hip_op *op, int prec, int K, const void *argvec, void *resultvec,
+	int nrep, double *pass_ms, double *total_ms)
+{
 	if (!op || !argvec || !resultvec || nrep <= 0 || !pass_ms || !total_ms) return ERR("bad argument");
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
 	HIPCHK(hipSetDevice(op->dev));
+	const long vs = K > 1 ? (long)op->vs : 0;
 	hipEvent_t ev[6];
 	for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
 	double acc[6] = {0, 0, 0, 0, 0, 0};
 	int rc = 0;
 	for (int r = 0; r < nrep && !rc; r++) {
-		if (prec == ADDA_F64) rc = matvec_t<double>(op, (const cx<double> *)argvec, (cx<double> *)resultvec, nullptr, 0, ev);
-		else rc = matvec_t<float>(op, (const cx<float> *)argvec, (cx<float> *)resultvec, nullptr, 0, ev);
+		if (prec == ADDA_F64)
+			rc = matvec_t<double>(op, K, vs, nullptr, (const cx<double> *)argvec, (cx<double> *)resultvec, nullptr, 0, ev);
+		else rc = matvec_t<float>(op, K, vs, nullptr, (const cx<float> *)argvec, (cx<float> *)resultvec, nullptr, 0, ev);
 		if (rc) break;
 		HIPCHK(hipEventSynchronize(ev[5]));
 		for (int i = 0; i < 5; i++) {
@@ -1674,11 +1946,17 @@ extern "C" int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *ar
 
 extern "C" double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pass)
 {
-	if (!op) return 0;
+	return adda_hip_matvec_bytes_batch(op, prec, 1, pass);
+}
+
+// K right-hand sides: the per-RHS streams (vectors, Xa, Xb) K times, the shared D-matrix and index arrays once.
+extern "C" double adda_hip_matvec_bytes_batch(const adda_hip_op *op, int prec, int K, int pass)
+{
+	if (!op || K < 1) return 0;
 	const double c = prec == ADDA_F32 ? 8 : 16;
 	const double gX = op->gX, bY = op->bY, bZ = op->bZ, gZ = op->gZ, nd = (double)op->nd;
 	const double nbox = (double)op->bX * bY * bZ;
-	const double xa = 3 * gX * bY * bZ * c, xb = 3 * gX * bY * gZ * c, vec = 3 * nd * c;
+	const double xa = 3 * gX * bY * bZ * c * K, xb = 3 * gX * bY * gZ * c * K, vec = 3 * nd * c * K;
 	const double dm = 6 * gX * op->DsY * op->DsZ * c;
 	// index data: cell map (4 B per box cell) + material (4 B per dipole), read by P1 and by P5
 	const double idx = 4 * nbox + 4 * nd;
@@ -1687,23 +1965,32 @@ extern "C" double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pas
 	return b[0] + b[1] + b[2] + b[3] + b[4];
 }
 
-extern "C" void *adda_hip_dk_create(adda_hip_op *op, int hist_len)
+extern "C" void *adda_hip_dk_create(adda_hip_op *op, int hist_len) { return adda_hip_dk_create_batch(op, 1, hist_len); }
+
+extern "C" void *adda_hip_dk_create_batch(adda_hip_op *op, int K, int hist_len)
 {
-	if (!op || hist_len < 1) { ERR("bad argument"); return nullptr; }
+	if (!op || hist_len < 1 || K < 1) { ERR("bad argument"); return nullptr; }
+	if (reserve(op, K)) return nullptr;
 	if (hipSetDevice(op->dev) != hipSuccess) { ERR("hipSetDevice failed"); return nullptr; }
 	dk_state *k = new dk_state();
 	k->histlen = hist_len;
-	if (hipMalloc(&k->st, DK_NSLOTS * sizeof(double)) != hipSuccess ||
-		hipMalloc(&k->hist, (size_t)hist_len * sizeof(double)) != hipSuccess) {
+	k->K = K;
+	k->st = nullptr;
+	k->hist = nullptr;
+	if (hipMalloc(&k->st, (size_t)K * DK_NSLOTS * sizeof(double)) != hipSuccess ||
+		hipMalloc(&k->hist, (size_t)K * hist_len * sizeof(double)) != hipSuccess) {
 		ERR("hipMalloc of the solver state failed");
 		if (k->st) (void)hipFree(k->st);
 		delete k;
 		return nullptr;
 	}
-	std::vector<double> z(DK_NSLOTS, 0.0);
-	z[DK_HISTLEN] = hist_len;
-	if (hipMemcpy(k->st, z.data(), DK_NSLOTS * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+	std::vector<double> z((size_t)K * DK_NSLOTS, 0.0);
+	for (int i = 0; i < K; i++) z[(size_t)i * DK_NSLOTS + DK_HISTLEN] = hist_len;
+	if (hipMemcpy(k->st, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
 		ERR("state init failed");
+		(void)hipFree(k->st);
+		(void)hipFree(k->hist);
+		delete k;
 		return nullptr;
 	}
 	return k;
@@ -1723,8 +2010,9 @@ extern "C" int adda_hip_dk_free(adda_hip_op *op, void *state)
 
 extern "C" int adda_hip_dk_set(adda_hip_op *op, void *state, int slot, const double *vals, int n)
 {
-	if (!op || !state || !vals || slot < 0 || n < 0 || slot + n > DK_NSLOTS) return ERR("bad slot range");
+	if (!op || !state || !vals || slot < 0 || n < 0) return ERR("bad slot range");
 	dk_state *k = (dk_state *)state;
+	if ((size_t)slot + n > (size_t)k->K * DK_NSLOTS) return ERR("bad slot range");
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipMemcpyAsync(k->st + slot, vals, n * sizeof(double), hipMemcpyHostToDevice, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
@@ -1733,8 +2021,9 @@ extern "C" int adda_hip_dk_set(adda_hip_op *op, void *state, int slot, const dou
 
 extern "C" int adda_hip_dk_get(adda_hip_op *op, void *state, int slot, double *vals, int n)
 {
-	if (!op || !state || !vals || slot < 0 || n < 0 || slot + n > DK_NSLOTS) return ERR("bad slot range");
+	if (!op || !state || !vals || slot < 0 || n < 0) return ERR("bad slot range");
 	dk_state *k = (dk_state *)state;
+	if ((size_t)slot + n > (size_t)k->K * DK_NSLOTS) return ERR("bad slot range");
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipMemcpyAsync(vals, k->st + slot, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
@@ -1745,7 +2034,7 @@ extern "C" int adda_hip_dk_history(adda_hip_op *op, void *state, double *hist, i
 {
 	if (!op || !state || !hist) return ERR("null argument");
 	dk_state *k = (dk_state *)state;
-	if (n > k->histlen) n = k->histlen;
+	if (n > k->histlen * k->K) n = k->histlen * k->K;
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipMemcpyAsync(hist, k->hist, n * sizeof(double), hipMemcpyDeviceToHost, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
@@ -1758,8 +2047,8 @@ extern "C" int adda_hip_dk_qmr_iter(adda_hip_op *op, int prec, void *state, int 
 	if (!op || !state) return ERR("null argument");
 	HIPCHK(hipSetDevice(op->dev));
 	dk_state *k = (dk_state *)state;
-	if (prec == ADDA_F64) { op->gate = k->st; int rc = qmr_iter_t<double>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r); op->gate = nullptr; return rc; }
-	{ op->gate = k->st; int rc = qmr_iter_t<float>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r); op->gate = nullptr; return rc; }
+	if (prec == ADDA_F64) return qmr_iter_t<double>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r);
+	return qmr_iter_t<float>(op, k, niter, matvec_ready, v, vtilda, Av, p_old, p_new, x, r);
 }
 
 extern "C" int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,
@@ -1768,8 +2057,8 @@ extern "C" int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state,
 	if (!op || !state) return ERR("null argument");
 	HIPCHK(hipSetDevice(op->dev));
 	dk_state *k = (dk_state *)state;
-	if (prec == ADDA_F64) { op->gate = k->st; int rc = bicgstab_iter_t<double>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t); op->gate = nullptr; return rc; }
-	{ op->gate = k->st; int rc = bicgstab_iter_t<float>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t); op->gate = nullptr; return rc; }
+	if (prec == ADDA_F64) return bicgstab_iter_t<double>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
+	return bicgstab_iter_t<float>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
 }
 
 extern "C" int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, void *x, void *r, void *p,
@@ -1778,8 +2067,51 @@ extern "C" int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int
 	if (!op || !state) return ERR("null argument");
 	HIPCHK(hipSetDevice(op->dev));
 	dk_state *k = (dk_state *)state;
-	if (prec == ADDA_F64) { op->gate = k->st; int rc = cgnr_iter_t<double>(op, k, niter, x, r, p, Av); op->gate = nullptr; return rc; }
-	{ op->gate = k->st; int rc = cgnr_iter_t<float>(op, k, niter, x, r, p, Av); op->gate = nullptr; return rc; }
+	if (prec == ADDA_F64) return cgnr_iter_t<double>(op, k, niter, x, r, p, Av);
+	return cgnr_iter_t<float>(op, k, niter, x, r, p, Av);
+}
+
+// ---- HIP graphs of enqueued work (e.g. a chunk of device-resident iterations) ------------------------------------
+extern "C" int adda_hip_capture_begin(adda_hip_op *op)
+{
+	if (!op) return ERR("null operator");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+	return 0;
+}
+
+extern "C" void *adda_hip_capture_end(adda_hip_op *op)
+{
+	if (!op) { ERR("null operator"); return nullptr; }
+	hipGraph_t g = nullptr;
+	hipError_t e = hipStreamEndCapture(op->stream, &g);
+	if (e != hipSuccess || !g) {
+		ERR("hipStreamEndCapture: %s", hipGetErrorString(e));
+		if (g) (void)hipGraphDestroy(g);
+		return nullptr;
+	}
+	hipGraphExec_t x = nullptr;
+	e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(g);
+	if (e != hipSuccess) { ERR("hipGraphInstantiate: %s", hipGetErrorString(e)); return nullptr; }
+	return (void *)x;
+}
+
+extern "C" int adda_hip_graph_launch(adda_hip_op *op, void *graph)
+{
+	if (!op || !graph) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipGraphLaunch((hipGraphExec_t)graph, op->stream));
+	return 0;
+}
+
+extern "C" int adda_hip_graph_free(adda_hip_op *op, void *graph)
+{
+	if (!op || !graph) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	HIPCHK(hipGraphExecDestroy((hipGraphExec_t)graph));
+	return 0;
 }
 
 // ==================================================================================================================
@@ -1900,20 +2232,24 @@ static void enum_plans(int n, std::vector<FftPlan2> &out)
 	}
 }
 
+// Tuning of the batched launch shape: K right-hand sides per pass (K = 1 for single solves).
 template <typename T>
-static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
+static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 {
 	const int pi = sizeof(T) == 8 ? ADDA_F64 : ADDA_F32;
 	const size_t es = sizeof(cx<T>);
-	const cx<T> *ccs;
+	const cx<T> *ccs = cc_of<T>(op);
 	const cx<T> *D;
-	if constexpr (sizeof(T) == 8) { ccs = op->cc64; D = op->D64; }
-	else { ccs = op->cc32; D = op->D32; }
+	if constexpr (sizeof(T) == 8) D = op->D64;
+	else D = op->D32;
 	if (!D) return ERR("autotune of FP32 needs an operator created with the FP32 flag");
+	if (reserve(op, K)) return -1;
+	const long vs = K > 1 ? (long)op->vs : 0;
+	const size_t nv = vec_elems(op, K);
 	cx<T> *a = nullptr, *r = nullptr;
-	HIPCHK(hipMalloc(&a, 3 * op->nd * es));
-	HIPCHK(hipMalloc(&r, 3 * op->nd * es));
-	HIPCHK(hipMemsetAsync(a, 0, 3 * op->nd * es, op->stream));
+	HIPCHK(hipMalloc(&a, nv * es));
+	HIPCHK(hipMalloc(&r, nv * es));
+	HIPCHK(hipMemsetAsync(a, 0, nv * es, op->stream));
 	hipEvent_t e0, e1;
 	HIPCHK(hipEventCreate(&e0));
 	HIPCHK(hipEventCreate(&e1));
@@ -1942,7 +2278,10 @@ static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
 			c.LY = LY;
 			if (!cfg_ok(op, c, es)) continue;
 			ncand++;
-			float t = timeit([&] { launch_p1<T>(op, c, a, ccs, 0); launch_p5<T>(op, c, a, r, ccs, 0, nullptr); });
+			float t = timeit([&] {
+				launch_p1<T>(op, c, K, vs, nullptr, a, ccs, 0);
+				launch_p5<T>(op, c, K, vs, nullptr, a, r, ccs, 0, nullptr);
+			});
 			if (t < tx) { tx = t; best.qx = q; best.LY = LY; }
 		}
 	// z passes (P2 + P4): radix schedule of z and TC
@@ -1953,7 +2292,7 @@ static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
 			c.TC = TC;
 			if (!cfg_ok(op, c, es)) continue;
 			ncand++;
-			float t = timeit([&] { launch_zfft<T, false>(op, c); launch_zfft<T, true>(op, c); });
+			float t = timeit([&] { launch_zfft<T, false>(op, c, K, nullptr); launch_zfft<T, true>(op, c, K, nullptr); });
 			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
 		}
 	// y pass (P3): radix schedule of y and LP
@@ -1964,7 +2303,7 @@ static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
 			c.LP = LP;
 			if (!cfg_ok(op, c, es)) continue;
 			ncand++;
-			float t = timeit([&] { launch_p3<T>(op, c, D, 0); });
+			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
 			if (t < ty) { ty = t; best.qy = q; best.LP = LP; }
 		}
 	(void)hipEventDestroy(e0);
@@ -1987,12 +2326,20 @@ static int tune_t(adda_hip_op *op, int reps, char *report, size_t rlen)
 extern "C" int adda_hip_autotune(adda_hip_op *op, int prec, int reps, const char *cache_path, char *report,
 	size_t report_len)
 {
+	return adda_hip_autotune_batch(op, prec, 1, reps, cache_path, report, report_len);
+}
+
+extern "C" int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int reps, const char *cache_path,
+	char *report, size_t report_len)
+{
 	if (!op) return ERR("null operator");
+	if (K < 1) return ERR("batch of %d right-hand sides", K);
 	HIPCHK(hipSetDevice(op->dev));
 	if (reps < 1) reps = 3;
-	char key[160];
-	snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d reduced %d prec %s", op->gX, op->gY, op->gZ, op->bX, op->bY,
-		op->bZ, op->reduced, prec == ADDA_F32 ? "fp32" : "fp64");
+	char key[200];
+	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d reduced %d prec %s", op->gX, op->gY, op->gZ, op->bX,
+		op->bY, op->bZ, op->reduced, prec == ADDA_F32 ? "fp32" : "fp64");
+	if (K > 1) snprintf(key + kl, sizeof key - kl, " batch %d", K);
 	if (cache_path) {
 		FILE *f = fopen(cache_path, "r");
 		if (f) {
@@ -2017,7 +2364,7 @@ extern "C" int adda_hip_autotune(adda_hip_op *op, int prec, int reps, const char
 		}
 	}
 	char rep[512];
-	int rc = prec == ADDA_F32 ? tune_t<float>(op, reps, rep, sizeof rep) : tune_t<double>(op, reps, rep, sizeof rep);
+	int rc = prec == ADDA_F32 ? tune_t<float>(op, K, reps, rep, sizeof rep) : tune_t<double>(op, K, reps, rep, sizeof rep);
 	if (rc) return rc;
 	if (report) snprintf(report, report_len, "%s", rep);
 	if (cache_path) {

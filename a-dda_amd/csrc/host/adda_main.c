@@ -4,7 +4,7 @@
  *            [-dpl <n>] [-lambda <l>] [-m <re> <im> [<re> <im> ...]] [-iter qmr|bicgstab|cgnr|bicg|csym]
  *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
- *            [-prec fp64|mixed] [-eps_inner <k>] [-max_inner <n>] [-host_scalars]
+ *            [-prec fp64|mixed] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
  *
  * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
  * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
@@ -59,7 +59,7 @@ int main(int argc, char **argv)
 	double orient[3] = {0, 0, 0};
 	adda_cplx m[16];
 	int nm = 0;
-	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1, 0, 0};
+	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1, 0, 0, 0};
 	int polrel = ADDA_POL_LDR;
 	for (int i = 1; i < argc; i++) {
 		const char *a = argv[i];
@@ -118,6 +118,7 @@ int main(int argc, char **argv)
 		else if (!strcmp(a, "-dir")) { NEED(1); dir = argv[++i]; }
 		else if (!strcmp(a, "-host_scalars")) opt.host_scalars = 1;
 		else if (!strcmp(a, "-max_inner")) { NEED(1); opt.max_inner = atoi(argv[++i]); }
+		else if (!strcmp(a, "-graph")) opt.graph = 1;
 		else if (!strcmp(a, "-prec")) {
 			NEED(1);
 			opt.precision = !strcmp(argv[++i], "mixed") ? ADDA_PREC_MIXED : ADDA_PREC_FP64;

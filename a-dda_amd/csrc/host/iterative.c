@@ -290,258 +290,430 @@ done:
 #undef CKG
 }
 
-/* Device-resident form of krylov() for QMR_CS, BiCGStab and CGNR: CalcInitField and PHASE_INIT as above (host,
- * synchronising, once), then whole iterations enqueued with adda_hip_dk_*_iter -- whose FP64 scalar recurrences run
- * on the GPU -- in chunks of DK_CHUNK, polling the device status slot after each chunk. Iterations enqueued after
- * convergence are no-ops on the device. Results are identical in exact arithmetic to krylov() (same recurrences, same
- * order of operations per vector element; reductions differ only in summation order). */
+/* ---- device-resident recurrences, K right-hand sides at once ---------------------------------------------------
+ * CalcInitField and PHASE_INIT as in krylov() (host, synchronising, once), then whole iterations enqueued with
+ * adda_hip_dk_*_iter -- whose FP64 scalar recurrences run on the GPU -- in chunks of DK_CHUNK, polling the device
+ * status slots after each chunk. Iterations enqueued after convergence are no-ops on the device. With K > 1 every
+ * launch advances all K solves (vectors batched at adda_hip_batch_stride); each RHS has its own scalars, tolerance,
+ * status and history, so the K results equal K separate solves. Results are identical in exact arithmetic to
+ * krylov() (same recurrences, same order of operations per vector element; reductions differ only in summation
+ * order). With use_graph, the chunks after the first are recorded once as a HIP graph and replayed. */
 #define DK_CHUNK 16
-static int krylov_dev(kctx *k, int method, double eps, int maxiter, int init_field, int verbose, const void *b, void *x,
-	double *hist, int hist_len, int *niter_out, double *final_rel)
+#define DK_EXCLUDED 3 /* status of a right-hand side that takes no part in a batched solve */
+
+typedef struct {
+	int rc;      /* 0, 1 (maxiter), -1 breakdown/error, -2 stagnation */
+	int niter;   /* iterations */
+	int nmv;     /* matvecs */
+	double rel;  /* final relative residual of the recurrence */
+} kres;
+
+static const char *method_name(int m)
 {
-	const int mc_tab[5] = {50000, 30000, 10, 50000, 10};
-	int rc = 0;
-	void *r = kvec(k), *Av = kvec(k), *p = kvec(k), *v1 = kvec(k), *v2 = kvec(k), *v3 = kvec(k);
-	void *st = adda_hip_dk_create(k->op, hist_len > 1 ? hist_len : 2);
-	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
-	double temp;
-#define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
-	CKG(dotp(k, b, NULL, ADDA_DOT_NORM2, NULL, &temp));
-	if (!(temp > 0)) { rc = host_err(__FILE__, __LINE__, "right-hand side is zero"); goto done; }
-	double resid_scale = 1 / temp, epsB = eps * eps * temp, inprodR = 0;
-	int matvec_ready = 0;
-	if (init_field == ADDA_IF_AUTO) { /* src/iterative.c:1347-1368 */
-		CKG(mv(k, b, Av, NULL, 0));
-		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
-		if (temp < inprodR) {
-			CKG(lc(k, x, NULL, 0, NULL, 0, NULL, 0, 0, NULL));
-			CKG(lc(k, r, b, 1, NULL, 0, NULL, 0, 0, NULL));
-			inprodR = temp;
-			matvec_ready = 1;
+	static const char *n[5] = {"QMR_CS", "BiCGStab", "CGNR", "BiCG_CS", "CSYM"};
+	return m >= 0 && m < 5 ? n[m] : "?";
+}
+
+static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int *it, int matvec_ready, void **v,
+	void **vtilda, void *Av, void **p_old, void **p_new, void *x, void *r, void *p, void *v1, void *v2, void *v3)
+{
+	for (int c = 0; c < DK_CHUNK; c++, (*it)++) {
+		if (method == ADDA_IT_QMR_CS) {
+			if (adda_hip_dk_qmr_iter(op, prec, st, *it, matvec_ready, *v, *vtilda, Av, *p_old, *p_new, x, r)) return -1;
+			if (*it >= 2) { void *t = *p_old; *p_old = *p_new; *p_new = t; }
+			{ void *t = *v; *v = *vtilda; *vtilda = t; }
 		}
-		else CKG(lc(k, x, b, 1, NULL, 0, NULL, 0, 0, NULL));
+		else if (method == ADDA_IT_BICGSTAB) {
+			/* v = vec1, s = vec2, rtilda = vec3, t = Avecbuffer (src/iterative.c:660-662) */
+			if (adda_hip_dk_bicgstab_iter(op, prec, st, *it, matvec_ready, x, r, p, v1, v2, v3, Av)) return -1;
+		}
+		else if (adda_hip_dk_cgnr_iter(op, prec, st, *it, x, r, p, Av)) return -1;
+	}
+	return 0;
+}
+
+/* active: [K] or NULL (all); eps: [K]; b, x: batched vectors in precision prec; hist: [K][hist_len] or NULL,
+ * hist[k*hist_len + i] = relative residual after iteration i (i = 0: the initial one). res: [K]. Returns the most
+ * severe per-RHS code (an error message is set for the first failing RHS). */
+static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const double *eps, int maxiter,
+	int init_field, int verbose, const int *active, const void *b, void *x, double *hist, int hist_len, int use_graph,
+	kres *res)
+{
+	const int mc_tab[5] = {50000, 30000, 10, 50000, 10}; /* params[].mc, src/iterative.c:118-132 */
+	const int NS = ADDA_DK_NSLOTS, hl = hist_len > 1 ? hist_len : 2;
+	int rc = 0, matvec_ready = 0, init_mv = 0;
+	void *graph = NULL;
+	void *r = adda_hip_vec_alloc_batch(op, prec, K), *Av = adda_hip_vec_alloc_batch(op, prec, K);
+	void *p = adda_hip_vec_alloc_batch(op, prec, K), *v1 = adda_hip_vec_alloc_batch(op, prec, K);
+	void *v2 = adda_hip_vec_alloc_batch(op, prec, K), *v3 = adda_hip_vec_alloc_batch(op, prec, K);
+	void *st = adda_hip_dk_create_batch(op, K, hl);
+	double *S = calloc((size_t)K * NS, sizeof(double)), *temp = calloc(K, sizeof(double));
+	double *inprodR = calloc(K, sizeof(double));
+	int *ready = calloc(K, sizeof(int));
+	adda_cplx *ca = calloc(K, sizeof(adda_cplx)), *cb = calloc(K, sizeof(adda_cplx)), *cc = calloc(K, sizeof(adda_cplx));
+	adda_cplx *t = calloc(K, sizeof(adda_cplx));
+	double *hbuf = NULL;
+#define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
+	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+	if (!S || !temp || !inprodR || !ready || !ca || !cb || !cc || !t) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
+	for (int k = 0; k < K; k++) {
+		res[k].rc = res[k].niter = res[k].nmv = 0;
+		res[k].rel = 0;
+		S[(size_t)k * NS + ADDA_DK_STATUS] = (active && !active[k]) ? DK_EXCLUDED : 0;
+		S[(size_t)k * NS + 9] = hl;
+	}
+	/* the statuses gate every batched call below: excluded RHS are never touched */
+	CKG(adda_hip_dk_set(op, st, 0, S, K * NS));
+	CKG(adda_hip_dot_batch(op, prec, K, st, b, NULL, ADDA_DOT_NORM2, NULL, temp));
+	for (int k = 0; k < K; k++)
+		if (!(active && !active[k]) && !(temp[k] > 0)) { rc = host_err(__FILE__, __LINE__, "right-hand side %d is zero", k); goto done; }
+	/* CalcInitField, src/iterative.c:1342-1394 */
+#define SETC(arr, v) for (int k_ = 0; k_ < K; k_++) { arr[k_].re = creal(v); arr[k_].im = cimag(v); }
+	if (init_field == ADDA_IF_AUTO) {
+		CKG(adda_hip_matvec_batch(op, prec, K, st, b, Av, NULL, 0));
+		init_mv = 1;
+		SETC(ca, 1); SETC(cb, -1); SETC(cc, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, r, b, ca, Av, cb, NULL, cc, 0, inprodR));
+		int any = 0, all = 1;
+		for (int k = 0; k < K; k++) {
+			ready[k] = temp[k] < inprodR[k];
+			if (ready[k]) inprodR[k] = temp[k];
+			if (!(active && !active[k])) { any |= ready[k]; all &= ready[k]; }
+			ca[k].re = ready[k] ? 0 : 1;  /* x = b, or 0 */
+			cb[k].re = ready[k] ? 0 : -1; /* r = b - A.b, or b */
+		}
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, x, b, ca, NULL, cc, NULL, cc, 0, NULL));
+		if (any) {
+			SETC(ca, 1);
+			CKG(adda_hip_lincomb_batch(op, prec, K, st, r, b, ca, Av, cb, NULL, cc, 0, NULL));
+		}
+		matvec_ready = all;
 	}
 	else if (init_field == ADDA_IF_ZERO) {
-		CKG(lc(k, x, NULL, 0, NULL, 0, NULL, 0, 0, NULL));
-		CKG(lc(k, r, b, 1, NULL, 0, NULL, 0, 0, NULL));
-		inprodR = temp;
+		SETC(ca, 0); SETC(cb, 1);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, x, NULL, ca, NULL, ca, NULL, ca, 0, NULL));
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, r, b, cb, NULL, ca, NULL, ca, 0, NULL));
+		for (int k = 0; k < K; k++) inprodR[k] = temp[k];
 	}
 	else {
-		CKG(lc(k, x, b, 1, NULL, 0, NULL, 0, 0, NULL));
-		CKG(mv(k, x, Av, NULL, 0));
-		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
+		SETC(ca, 1); SETC(cb, -1); SETC(cc, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, x, b, ca, NULL, cc, NULL, cc, 0, NULL));
+		CKG(adda_hip_matvec_batch(op, prec, K, st, x, Av, NULL, 0));
+		init_mv = 1;
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, r, b, ca, Av, cb, NULL, cc, 0, inprodR));
 	}
-	if (hist && hist_len > 0) hist[0] = sqrt(resid_scale * inprodR);
-	if (verbose) printf("RE_000 = %.10E\n", sqrt(resid_scale * inprodR));
-	{
-		double base[11] = {inprodR, 0, epsB, resid_scale, 1, 0, inprodR > epsB ? 0 : 1, maxiter, mc_tab[method],
-			hist_len > 1 ? hist_len : 2, 0};
-		if (maxiter < 1 && base[6] == 0) base[6] = 2;
-		CKG(adda_hip_dk_set(k->op, st, 0, base, 11));
+	for (int k = 0; k < K; k++) {
+		double *s = S + (size_t)k * NS;
+		const int excl = active && !active[k];
+		const double epsB = eps[k] * eps[k] * temp[k];
+		s[0] = inprodR[k];
+		s[2] = epsB;
+		s[3] = excl ? 0 : 1 / temp[k];
+		s[4] = 1;
+		s[6] = excl ? DK_EXCLUDED : (inprodR[k] > epsB ? 0 : 1);
+		if (maxiter < 1 && s[6] == 0) s[6] = 2;
+		s[7] = maxiter;
+		s[8] = mc_tab[method];
+		s[9] = hl;
+		if (hist && hist_len > 0) hist[(size_t)k * hist_len] = excl ? 0 : sqrt(inprodR[k] / temp[k]);
 	}
-	/* PHASE_INIT, written into the method slots (layout: krylov_dev.h / adda_hip.hip k_dk_* kernels) */
+	if (verbose && K == 1) printf("RE_000 = %.10E\n", sqrt(inprodR[0] / temp[0]));
+	/* PHASE_INIT, written into the method slots (layout: the k_dk_* kernels of adda_hip.hip) */
 	if (method == ADDA_IT_QMR_CS) {
-		dcx t;
-		CKG(dotp(k, r, NULL, ADDA_DOT_SELF_BILIN, &t, NULL));
-		dcx beta = csqrt(t);
-		double omega_new = sqrt(inprodR) / cabs(beta);
-		CKG(lc(k, v1, r, 1 / beta, NULL, 0, NULL, 0, 0, NULL));
-		dcx tautilda = omega_new * beta, ib = 1 / beta;
-		double sc[46] = {0};
-		sc[0] = creal(beta); sc[1] = cimag(beta);
-		sc[8] = creal(tautilda); sc[9] = cimag(tautilda);
-		sc[10] = 1; sc[11] = 1; sc[12] = 0; sc[13] = omega_new;
-		sc[44] = creal(ib); sc[45] = cimag(ib);
-		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 46));
+		CKG(adda_hip_dot_batch(op, prec, K, st, r, NULL, ADDA_DOT_SELF_BILIN, t, NULL));
+		for (int k = 0; k < K; k++) {
+			double *sc = S + (size_t)k * NS + ADDA_DK_SC;
+			if (active && !active[k]) { ca[k].re = ca[k].im = 0; continue; }
+			const dcx beta = csqrt(CA(t[k]));
+			const double omega_new = sqrt(inprodR[k]) / cabs(beta);
+			const dcx tautilda = omega_new * beta, ib = 1 / beta;
+			sc[0] = creal(beta); sc[1] = cimag(beta);
+			sc[8] = creal(tautilda); sc[9] = cimag(tautilda);
+			sc[10] = 1; sc[11] = 1; sc[12] = 0; sc[13] = omega_new;
+			sc[44] = creal(ib); sc[45] = cimag(ib);
+			ca[k] = AC(ib);
+		}
+		SETC(cb, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, v1, r, ca, NULL, cb, NULL, cb, 0, NULL));
 	}
 	else if (method == ADDA_IT_BICGSTAB) {
-		CKG(lc(k, v3, r, 1, NULL, 0, NULL, 0, 0, NULL)); /* r~ = r_0 */
-		double sc[26] = {0};
-		sc[24] = 1;
-		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 26));
+		SETC(ca, 1); SETC(cb, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, v3, r, ca, NULL, cb, NULL, cb, 0, NULL)); /* r~ = r_0 */
+		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 24] = 1;
 	}
-	else {
-		double sc[14] = {0};
-		sc[6] = 1;
-		CKG(adda_hip_dk_set(k->op, st, ADDA_DK_SC, sc, 14));
-	}
+	else
+		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 6] = 1;
+	CKG(adda_hip_dk_set(op, st, 0, S, K * NS));
+#undef SETC
 	{
 		void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;
-		int it = 1;
-		double status = 0;
-		while (status == 0) {
-			for (int c = 0; c < DK_CHUNK; c++, it++) {
-				if (method == ADDA_IT_QMR_CS) {
-					CKG(adda_hip_dk_qmr_iter(k->op, k->prec, st, it, matvec_ready, v, vtilda, Av, p_old, p_new, x, r));
-					if (it >= 2) { void *t = p_old; p_old = p_new; p_new = t; }
-					{ void *t = v; v = vtilda; vtilda = t; }
-					k->nmv += !(it == 1 && matvec_ready);
+		int it = 1, running = 0;
+		for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+		while (running) {
+			if (use_graph && it > 3) {
+				if (!graph) {
+					/* record the chunk (nothing runs while capturing); the pointer rotation of a chunk is the
+					 * identity (DK_CHUNK is even), so the recorded chunk is valid for every later one */
+					int it0 = it;
+					CKG(adda_hip_capture_begin(op));
+					int erc = enqueue_chunk(op, prec, method, st, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x,
+						r, p, v1, v2, v3);
+					graph = adda_hip_capture_end(op);
+					if (erc || !graph) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+					it = it0;
 				}
-				else if (method == ADDA_IT_BICGSTAB) {
-					/* v = vec1, s = vec2, rtilda = vec3, t = Avecbuffer (src/iterative.c:660-662) */
-					CKG(adda_hip_dk_bicgstab_iter(k->op, k->prec, st, it, matvec_ready, x, r, p, v1, v2, v3, Av));
-					k->nmv += 2 - (it == 1 && matvec_ready);
-				}
-				else {
-					CKG(adda_hip_dk_cgnr_iter(k->op, k->prec, st, it, x, r, p, Av));
-					k->nmv += 2;
-				}
+				CKG(adda_hip_graph_launch(op, graph));
+				it += DK_CHUNK;
 			}
-			CKG(adda_hip_dk_get(k->op, st, ADDA_DK_STATUS, &status, 1));
+			else CKG(enqueue_chunk(op, prec, method, st, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x, r, p,
+				v1, v2, v3));
+			CKG(adda_hip_dk_get(op, st, 0, S, K * NS));
+			running = 0;
+			for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
 		}
-		double fin[11];
-		CKG(adda_hip_dk_get(k->op, st, 0, fin, 11));
-		int niter = (int)fin[4];
-		/* matvecs of the enqueued iterations that the device skipped are not counted */
-		int skipped = it - niter;
-		if (method == ADDA_IT_QMR_CS) k->nmv -= skipped;
-		else k->nmv -= 2 * skipped;
-		inprodR = fin[0];
-		if (hist && hist_len > 1 && niter > 1) {
-			int n = niter < hist_len ? niter : hist_len;
-			double *tmp = malloc(n * sizeof(double));
-			if (!tmp) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
-			CKG(adda_hip_dk_history(k->op, st, tmp, n));
-			for (int i = 1; i < n; i++) hist[i] = tmp[i];
-			free(tmp);
+		if (hist && hist_len > 1) {
+			hbuf = malloc((size_t)K * hl * sizeof(double));
+			if (!hbuf) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
+			CKG(adda_hip_dk_history(op, st, hbuf, K * hl));
 		}
-		if (verbose)
-			for (int i = 1; i < niter && hist && i < hist_len; i++) printf("RE_%03d = %.10E\n", i, hist[i]);
-		*niter_out = niter - 1;
-		*final_rel = sqrt(resid_scale * inprodR);
-		if (status == 2) rc = 1;
-		else if (status == -1) rc = host_err(__FILE__, __LINE__, "%s breakdown (test %d failed)",
-			method == ADDA_IT_QMR_CS ? "QMR_CS" : method == ADDA_IT_BICGSTAB ? "BiCGStab" : "CGNR", (int)fin[10]);
-		else if (status == -2) rc = host_err(__FILE__, __LINE__,
-			"Residual norm haven't decreased for maximum allowed number of iterations (%d)", mc_tab[method]) - 1;
+		for (int k = 0; k < K; k++) {
+			const double *s = S + (size_t)k * NS;
+			const int status = (int)s[ADDA_DK_STATUS];
+			if (status == DK_EXCLUDED) continue;
+			const int niter = (int)s[ADDA_DK_NITER]; /* next iteration number */
+			const int done_it = niter - 1;
+			res[k].niter = done_it;
+			res[k].rel = sqrt(s[3] * s[0]);
+			/* matvecs of the iterations the device ran */
+			if (method == ADDA_IT_QMR_CS) res[k].nmv = done_it - (done_it > 0 && matvec_ready);
+			else if (method == ADDA_IT_BICGSTAB) res[k].nmv = 2 * done_it - (done_it > 0 && matvec_ready);
+			else res[k].nmv = 2 * done_it;
+			res[k].nmv += init_mv;
+			if (hbuf)
+				for (int i = 1; i < niter && i < hist_len; i++) hist[(size_t)k * hist_len + i] = hbuf[(size_t)k * hl + i];
+			if (verbose && K == 1 && hist)
+				for (int i = 1; i < niter && i < hist_len; i++) printf("RE_%03d = %.10E\n", i, hist[i]);
+			if (status == 2) res[k].rc = 1;
+			else if (status == -1) {
+				res[k].rc = -1;
+				if (rc > -1) rc = host_err(__FILE__, __LINE__, "%s breakdown (test %d failed)%s", method_name(method),
+					(int)s[10], K > 1 ? " in a batched solve" : "");
+			}
+			else if (status == -2) {
+				res[k].rc = -2;
+				if (rc > -1) rc = host_err(__FILE__, __LINE__,
+					"Residual norm haven't decreased for maximum allowed number of iterations (%d)", mc_tab[method]) - 1;
+			}
+			if (res[k].rc == 1 && rc == 0) rc = 1;
+		}
 	}
 done:
 	{
 		void *vs[] = {r, Av, p, v1, v2, v3};
 		for (int i = 0; i < 6; i++)
-			if (vs[i]) adda_hip_vec_free(k->op, vs[i]);
-		if (st) adda_hip_dk_free(k->op, st);
+			if (vs[i]) adda_hip_vec_free(op, vs[i]);
+		if (graph) adda_hip_graph_free(op, graph);
+		if (st) adda_hip_dk_free(op, st);
+		free(S); free(temp); free(inprodR); free(ready); free(ca); free(cb); free(cc); free(t); free(hbuf);
 	}
 	return rc;
 #undef CKG
 }
 
-/* dispatch: device-resident recurrences where they exist, unless the caller asks for the host form */
-static int krylov_any(kctx *k, int method, int host_scalars, double eps, int maxiter, int init_field, int verbose,
-	const void *b, void *x, double *hist, int hist_len, int *niter_out, double *final_rel)
+static int device_method(const adda_solver_opts *opt)
 {
-	if (!host_scalars && (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICGSTAB || method == ADDA_IT_CGNR))
-		return krylov_dev(k, method, eps, maxiter, init_field, verbose, b, x, hist, hist_len, niter_out, final_rel);
-	return krylov(k, method, eps, maxiter, init_field, verbose, b, x, hist, hist_len, niter_out, final_rel);
+	return !opt->host_scalars &&
+		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR);
+}
+
+/* one Krylov solve of K right-hand sides: device-resident recurrences, or (K = 1) the host form */
+static int krylov_run(adda_hip_op *op, const adda_solver_opts *opt, int prec, int K, const double *eps, int maxiter,
+	int init_field, int verbose, const int *active, const void *b, void *x, double *hist, int hist_len, kres *res)
+{
+	if (device_method(opt))
+		return krylov_dev_batch(op, prec, K, opt->method, eps, maxiter, init_field, verbose, active, b, x, hist,
+			hist_len, opt->graph, res);
+	kctx k = {op, prec, 0};
+	int ni = 0;
+	double rel = 0;
+	int rc = krylov(&k, opt->method, eps[0], maxiter, init_field, verbose, b, x, hist, hist_len, &ni, &rel);
+	res[0].rc = rc;
+	res[0].niter = ni;
+	res[0].nmv = k.nmv;
+	res[0].rel = rel;
+	return rc;
+}
+
+int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, int K, const void *d_einc, void *d_x,
+	void *d_pvec, double *resid_hist, int hist_len, adda_solve_stats *st)
+{
+	if (!op || !opt || !d_einc || !d_x || !d_pvec || !st) return host_err(__FILE__, __LINE__, "null argument");
+	if (K < 1) return host_err(__FILE__, __LINE__, "batch of %d right-hand sides", K);
+	for (int k = 0; k < K; k++) {
+		memset(&st[k], 0, sizeof st[k]);
+		st[k].true_resid = -1;
+	}
+	if (opt->method < 0 || opt->method > 4) return host_err(__FILE__, __LINE__, "unknown iterative method %d", opt->method);
+	if (K > 1 && !device_method(opt))
+		return host_err(__FILE__, __LINE__, "batched solves need the device-resident recurrences (QMR_CS, BiCGStab, CGNR)");
+	int rc = 0;
+	const int NS = ADDA_DK_NSLOTS;
+	double *eps = malloc(K * sizeof(double)), *b2 = malloc(K * sizeof(double)), *r2 = malloc(K * sizeof(double));
+	double *r2new = malloc(K * sizeof(double)), *ih = NULL, *og_h = NULL;
+	kres *kr = malloc(K * sizeof(kres));
+	adda_cplx *ca = malloc(K * sizeof(adda_cplx)), *cb = malloc(K * sizeof(adda_cplx)), *cz = calloc(K, sizeof(adda_cplx));
+	int *active = malloc(K * sizeof(int)), *hist_pos = calloc(K, sizeof(int));
+	void *r64 = NULL, *Ax = NULL, *r32 = NULL, *e32 = NULL, *e64 = NULL, *og = NULL;
+#define CKB(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto out; } } while (0)
+	if (!eps || !b2 || !r2 || !r2new || !kr || !ca || !cb || !cz || !active || !hist_pos) {
+		rc = host_err(__FILE__, __LINE__, "out of host memory");
+		goto out;
+	}
+	for (int k = 0; k < K; k++) eps[k] = opt->eps;
+	/* b = S.Einc into pvec (nMult_mat, src/iterative.c:1421-1422) */
+	CKB(adda_hip_mult_mat_batch(op, ADDA_F64, K, d_pvec, d_einc));
+	if (opt->precision == ADDA_PREC_FP64) {
+		rc = krylov_run(op, opt, ADDA_F64, K, eps, opt->maxiter, opt->init_field, opt->verbose, NULL, d_pvec, d_x,
+			resid_hist, hist_len, kr);
+		for (int k = 0; k < K; k++) {
+			st[k].rc = kr[k].rc;
+			st[k].niter = kr[k].niter;
+			st[k].nmatvec = kr[k].nmv;
+			st[k].resid = kr[k].rel;
+		}
+		if (rc < 0) goto out;
+		if (opt->recalc_resid) { /* ResidualNorm2, src/iterative.c:1512-1520 */
+			Ax = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+			r64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+			if (!Ax || !r64) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
+			for (int k = 0; k < K; k++) { ca[k].re = 1; ca[k].im = 0; cb[k].re = -1; cb[k].im = 0; }
+			CKB(adda_hip_matvec_batch(op, ADDA_F64, K, NULL, d_x, Ax, NULL, 0));
+			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, NULL, r64, d_pvec, ca, Ax, cb, NULL, cz, 0, r2));
+			CKB(adda_hip_dot_batch(op, ADDA_F64, K, NULL, d_pvec, NULL, ADDA_DOT_NORM2, NULL, b2));
+			for (int k = 0; k < K; k++) {
+				st[k].true_resid = sqrt(r2[k] / b2[k]);
+				st[k].nmatvec_fp64 = 1;
+			}
+		}
+	}
+	else {
+		/* FP64 defect correction around FP32 Krylov solves, per RHS:
+		 *   x = 0; r = b; repeat { e ~ A^-1 r (FP32, relative tol inner_eps); x += e; r = b - A.x (FP64) }
+		 * until ||r||/||b|| <= eps. Inner solves start from e = 0 (init_field zero) on the normalized residual. All
+		 * RHS still correcting run their inner solves together; the others are gated off (status non-zero in og). */
+		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-5;
+		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
+		const int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
+		const int ihl = (hist_len < cap ? hist_len : cap) + 2; /* inner entries beyond hist_len are not kept */
+		r64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+		Ax = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+		e64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+		r32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
+		e32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
+		og = adda_hip_dk_create_batch(op, K, 2);
+		og_h = calloc((size_t)K * NS, sizeof(double));
+		if (resid_hist && hist_len > 0) ih = malloc((size_t)K * ihl * sizeof(double));
+		if (!r64 || !Ax || !e64 || !r32 || !e32 || !og) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
+		if (!og_h || (resid_hist && hist_len > 0 && !ih)) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto out; }
+		for (int k = 0; k < K; k++) { ca[k].re = 1; ca[k].im = 0; cb[k].re = -1; cb[k].im = 0; }
+		CKB(adda_hip_dot_batch(op, ADDA_F64, K, NULL, d_pvec, NULL, ADDA_DOT_NORM2, NULL, b2));
+		for (int k = 0; k < K; k++)
+			if (!(b2[k] > 0)) { rc = host_err(__FILE__, __LINE__, "right-hand side %d is zero", k); goto out; }
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, NULL, d_x, NULL, cz, NULL, cz, NULL, cz, 0, NULL));
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, NULL, r64, d_pvec, ca, NULL, cz, NULL, cz, 0, r2));
+		int nact = 0;
+		for (int k = 0; k < K; k++) {
+			if (resid_hist && hist_len > 0) resid_hist[(size_t)k * hist_len] = sqrt(r2[k] / b2[k]);
+			hist_pos[k] = 1;
+			active[k] = sqrt(r2[k] / b2[k]) > opt->eps;
+			nact += active[k];
+		}
+		int nouter = 0;
+		while (nact > 0 && nouter < max_outer) {
+			for (int k = 0; k < K; k++) og_h[(size_t)k * NS + ADDA_DK_STATUS] = active[k] ? 0 : 1;
+			CKB(adda_hip_dk_set(op, og, 0, og_h, K * NS));
+			/* normalized residual in FP32; the relative target of each step: reach eps overall but no deeper than
+			 * FP32 can resolve */
+			for (int k = 0; k < K; k++) {
+				const double rn = sqrt(r2[k]);
+				ca[k].re = active[k] ? 1 / rn : 0;
+				eps[k] = opt->eps / sqrt(r2[k] / b2[k]);
+				if (eps[k] < inner) eps[k] = inner;
+			}
+			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, Ax, r64, ca, NULL, cz, NULL, cz, 0, NULL));
+			CKB(adda_hip_vec_convert_batch(op, K, r32, ADDA_F32, Ax, ADDA_F64));
+			int irc = krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, 0, active, r32, e32, ih, ih ? ihl : 0, kr);
+			(void)irc; /* per-RHS codes in kr: an FP32 breakdown or stagnation ends that RHS's step, the FP64 residual
+			            * below decides what happens next */
+			for (int k = 0; k < K; k++) {
+				if (!active[k]) continue;
+				if (kr[k].rc < 0 && kr[k].niter == 0) { /* the inner solver could not even start */
+					st[k].rc = kr[k].rc;
+					active[k] = 0;
+					continue;
+				}
+				if (ih) { /* inner residuals relative to this step's rhs -> relative to b */
+					const double scale = sqrt(r2[k] / b2[k]);
+					for (int i = 1; i <= kr[k].niter && hist_pos[k] < hist_len; i++)
+						resid_hist[(size_t)k * hist_len + hist_pos[k]++] = ih[(size_t)k * ihl + i] * scale;
+				}
+				st[k].niter += kr[k].niter;
+				st[k].nmatvec += kr[k].nmv;
+				ca[k].re = 1;
+				cb[k].re = sqrt(r2[k]);
+			}
+			for (int k = 0; k < K; k++) og_h[(size_t)k * NS + ADDA_DK_STATUS] = active[k] ? 0 : 1;
+			CKB(adda_hip_dk_set(op, og, 0, og_h, K * NS));
+			/* x += rn * e (FP64 accumulation) */
+			CKB(adda_hip_vec_convert_batch(op, K, e64, ADDA_F64, e32, ADDA_F32));
+			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, d_x, d_x, ca, e64, cb, NULL, cz, 0, NULL));
+			/* true residual in FP64 */
+			CKB(adda_hip_matvec_batch(op, ADDA_F64, K, og, d_x, Ax, NULL, 0));
+			for (int k = 0; k < K; k++) { ca[k].re = 1; cb[k].re = -1; }
+			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, r64, d_pvec, ca, Ax, cb, NULL, cz, 0, r2new));
+			nouter++;
+			nact = 0;
+			for (int k = 0; k < K; k++) {
+				if (!active[k]) continue;
+				st[k].nmatvec_fp64++;
+				st[k].nouter++;
+				if (opt->verbose && K == 1)
+					printf("outer %d: inner iterations %d, true residual %.6E\n", st[k].nouter, kr[k].niter,
+						sqrt(r2new[k] / b2[k]));
+				const int progress = r2new[k] < r2[k];
+				r2[k] = r2new[k];
+				/* no progress in FP64: give up with the best x so far */
+				active[k] = progress && sqrt(r2[k] / b2[k]) > opt->eps;
+				nact += active[k];
+			}
+		}
+		for (int k = 0; k < K; k++) {
+			st[k].resid = st[k].true_resid = sqrt(r2[k] / b2[k]);
+			if (st[k].resid > opt->eps && st[k].rc == 0) st[k].rc = 1;
+		}
+		rc = 0;
+		for (int k = 0; k < K; k++)
+			if (st[k].rc < rc || (rc == 0 && st[k].rc > 0)) rc = st[k].rc;
+		if (rc < 0) goto out;
+	}
+	/* P = S.x (src/iterative.c:1527) */
+	CKB(adda_hip_mult_mat_batch(op, ADDA_F64, K, d_pvec, d_x));
+	CKB(adda_hip_sync(op));
+out:
+	{
+		void *vs[] = {r64, Ax, r32, e32, e64};
+		for (int i = 0; i < 5; i++)
+			if (vs[i]) adda_hip_vec_free(op, vs[i]);
+		if (og) adda_hip_dk_free(op, og);
+	}
+	free(eps); free(b2); free(r2); free(r2new); free(ih); free(og_h); free(kr); free(ca); free(cb); free(cz);
+	free(active); free(hist_pos);
+	return rc;
+#undef CKB
 }
 
 int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const void *d_einc, void *d_x, void *d_pvec,
 	double *resid_hist, int hist_len, adda_solve_stats *st)
 {
-	if (!op || !opt || !d_einc || !d_x || !d_pvec || !st) return host_err(__FILE__, __LINE__, "null argument");
-	memset(st, 0, sizeof *st);
-	st->true_resid = -1;
-	if (opt->method < 0 || opt->method > 4) return host_err(__FILE__, __LINE__, "unknown iterative method %d", opt->method);
-	int rc = 0;
-	/* b = S.Einc into pvec (nMult_mat, src/iterative.c:1421-1422) */
-	CK(adda_hip_mult_mat(op, ADDA_F64, d_pvec, d_einc));
-	if (opt->precision == ADDA_PREC_FP64) {
-		kctx k = {op, ADDA_F64, 0};
-		double rel = 0;
-		rc = krylov_any(&k, opt->method, opt->host_scalars, opt->eps, opt->maxiter, opt->init_field, opt->verbose,
-			d_pvec, d_x, resid_hist, hist_len, &st->niter, &rel);
-		st->nmatvec = k.nmv;
-		st->resid = rel;
-		if (rc < 0) { st->rc = rc; return rc; }
-		st->rc = rc;
-		if (opt->recalc_resid) { /* ResidualNorm2, src/iterative.c:1512-1520 */
-			void *Ax = adda_hip_vec_alloc(op, ADDA_F64), *r = adda_hip_vec_alloc(op, ADDA_F64);
-			double b2, r2;
-			if (!Ax || !r) return host_err_hip(__FILE__, __LINE__);
-			CK(adda_hip_matvec(op, ADDA_F64, d_x, Ax, NULL, 0));
-			CK(adda_hip_lincomb(op, ADDA_F64, r, d_pvec, AC(1), Ax, AC(-1), NULL, AC(0), 0, &r2));
-			CK(adda_hip_dot(op, ADDA_F64, d_pvec, NULL, ADDA_DOT_NORM2, NULL, &b2));
-			st->true_resid = sqrt(r2 / b2);
-			st->nmatvec_fp64 = 1;
-			adda_hip_vec_free(op, Ax);
-			adda_hip_vec_free(op, r);
-		}
-	}
-	else {
-		/* FP64 defect correction around FP32 Krylov solves:
-		 *   x = 0; r = b; repeat { e ~ A^-1 r (FP32, relative tol inner_eps); x += e; r = b - A.x (FP64) }
-		 * until ||r||/||b|| <= eps. Inner solves start from e = 0 (init_field zero) on the normalized residual. */
-		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-5;
-		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
-		void *r64 = adda_hip_vec_alloc(op, ADDA_F64), *Ax = adda_hip_vec_alloc(op, ADDA_F64);
-		void *r32 = adda_hip_vec_alloc(op, ADDA_F32), *e32 = adda_hip_vec_alloc(op, ADDA_F32);
-		void *e64 = adda_hip_vec_alloc(op, ADDA_F64);
-		if (!r64 || !Ax || !r32 || !e32 || !e64) return host_err_hip(__FILE__, __LINE__);
-		double b2, r2;
-		CK(adda_hip_dot(op, ADDA_F64, d_pvec, NULL, ADDA_DOT_NORM2, NULL, &b2));
-		CK(adda_hip_lincomb(op, ADDA_F64, d_x, NULL, AC(0), NULL, AC(0), NULL, AC(0), 0, NULL));
-		CK(adda_hip_lincomb(op, ADDA_F64, r64, d_pvec, AC(1), NULL, AC(0), NULL, AC(0), 0, &r2));
-		int hist_pos = 0;
-		if (resid_hist && hist_len > 0) resid_hist[hist_pos++] = sqrt(r2 / b2);
-		kctx k = {op, ADDA_F32, 0};
-		while (sqrt(r2 / b2) > opt->eps && st->nouter < max_outer) {
-			const double rn = sqrt(r2);
-			/* normalized residual in FP32 */
-			CK(adda_hip_lincomb(op, ADDA_F64, Ax, r64, AC(1 / rn), NULL, AC(0), NULL, AC(0), 0, NULL));
-			CK(adda_hip_vec_convert(op, r32, ADDA_F32, Ax, ADDA_F64));
-			/* relative target of this step: reach eps overall but no deeper than FP32 can resolve */
-			double tgt = opt->eps / sqrt(r2 / b2);
-			if (tgt < inner) tgt = inner;
-			int ni = 0;
-			double rel;
-			/* the inner history starts one slot early: its entry 0 (= 1, the normalized start) overwrites the last
-			 * entry of the previous step, which holds the same residual */
-			int hp = hist_pos > 0 ? hist_pos - 1 : 0;
-			int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
-			int irc = krylov_any(&k, opt->method, opt->host_scalars, tgt, cap, ADDA_IF_ZERO, 0, r32, e32,
-				resid_hist ? resid_hist + hp : NULL, resid_hist ? hist_len - hp : 0, &ni, &rel);
-			if (resid_hist) hist_pos = hp + 1;
-			if (irc < 0) {
-				/* an FP32 breakdown or stagnation ends this step; the FP64 residual below decides what happens next */
-				if (ni == 0) { st->rc = irc; goto mixed_done; }
-			}
-			if (resid_hist) { /* inner residuals relative to this step's rhs -> relative to b */
-				const double scale = sqrt(r2 / b2);
-				int upto = hist_pos + ni < hist_len ? hist_pos + ni : hist_len;
-				for (int h = hist_pos; h < upto; h++) resid_hist[h] *= scale;
-				hist_pos = upto;
-			}
-			st->niter += ni;
-			/* x += rn * e (FP64 accumulation) */
-			CK(adda_hip_vec_convert(op, e64, ADDA_F64, e32, ADDA_F32));
-			CK(adda_hip_lincomb(op, ADDA_F64, d_x, d_x, AC(1), e64, AC(rn), NULL, AC(0), 0, NULL));
-			/* true residual in FP64 */
-			CK(adda_hip_matvec(op, ADDA_F64, d_x, Ax, NULL, 0));
-			st->nmatvec_fp64++;
-			double r2new;
-			CK(adda_hip_lincomb(op, ADDA_F64, r64, d_pvec, AC(1), Ax, AC(-1), NULL, AC(0), 0, &r2new));
-			st->nouter++;
-			if (opt->verbose) printf("outer %d: inner iterations %d, true residual %.6E\n", st->nouter, ni, sqrt(r2new / b2));
-			if (!(r2new < r2)) { /* no progress in FP64: give up with the best x so far */
-				r2 = r2new;
-				break;
-			}
-			r2 = r2new;
-		}
-		st->nmatvec = k.nmv;
-		st->resid = st->true_resid = sqrt(r2 / b2);
-		if (st->resid > opt->eps && st->rc == 0) st->rc = 1;
-	mixed_done:
-		adda_hip_vec_free(op, r64);
-		adda_hip_vec_free(op, Ax);
-		adda_hip_vec_free(op, r32);
-		adda_hip_vec_free(op, e32);
-		adda_hip_vec_free(op, e64);
-		if (st->rc < 0) return st->rc;
-		rc = st->rc;
-	}
-	/* P = S.x (src/iterative.c:1527) */
-	CK(adda_hip_mult_mat(op, ADDA_F64, d_pvec, d_x));
-	CK(adda_hip_sync(op));
-	return rc;
+	return adda_iterative_solver_batch(op, opt, 1, d_einc, d_x, d_pvec, resid_hist, hist_len, st);
 }

@@ -46,6 +46,8 @@ def main():
                     help="profiling mode: autotune, 20 matvecs per precision and one 52.8 MB device copy (the "
                          "calibration kernel for HBM counters), no solves; prints the matvec figures only")
     ap.add_argument("--qmr-max-inner", type=int, default=2000)
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache (JSON lines): configurations found there are reused without timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
@@ -71,7 +73,7 @@ def main():
     pr.set_polarization(prop, pol)
     op = pr.op
     t0 = time.perf_counter()
-    tune = {name: op.autotune(prec, reps=3) for prec, name in ((A.F64, "fp64"), (A.F32, "fp32"))}
+    tune = {name: op.autotune(prec, reps=3, cache=args.tune_cache) for prec, name in ((A.F64, "fp64"), (A.F32, "fp32"))}
     t_tune = time.perf_counter() - t0
     log(f"autotune {t_tune:.2f} s: " + "; ".join(f"{k}: {v}" for k, v in tune.items()))
 

@@ -154,6 +154,44 @@ int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg);
  * found) and appended after tuning. report (may be NULL) receives the chosen configuration and pass times. */
 int adda_hip_autotune(adda_hip_op *op, int prec, int reps, const char *cache_path, char *report, size_t report_len);
 
+/* ---- batched multi-RHS solves ----------------------------------------------------------------------------------
+ * K right-hand sides (polarizations x orientations of one particle) share the operator: the D-matrix, the geometry
+ * and the plans. ADDA solves them one after another (the orientation loop of src/crosssec.c:82-138 and the two
+ * polarizations of src/CalculateE.c:836-870, each an IterativeSolver call); here every launch of a matvec or Krylov
+ * kernel covers all K. A batched device vector holds the K vectors at stride adda_hip_batch_stride(op) complex
+ * elements (>= 3*nd). Each RHS has its own cc_sqrt table (the LDR polarizability depends on the incident direction
+ * and polarization, src/calculator.c:170-289). gate_state (a state from adda_hip_dk_create_batch, or NULL): RHS k is
+ * left untouched when its status slot is non-zero (converged or excluded). Single-RHS calls are K = 1. */
+int adda_hip_reserve_batch(adda_hip_op *op, int K);    /* grows work buffers and tables to K RHS (synchronises) */
+size_t adda_hip_batch_stride(const adda_hip_op *op);
+int adda_hip_set_cc_sqrt_batch(adda_hip_op *op, int K, const adda_cplx *cc_sqrt); /* cc_sqrt[K][nmat][3] */
+void *adda_hip_vec_alloc_batch(adda_hip_op *op, int prec, int K);
+int adda_hip_vec_convert_batch(adda_hip_op *op, int K, void *dst, int dprec, const void *src, int sprec);
+int adda_hip_matvec_batch(adda_hip_op *op, int prec, int K, const void *gate_state, const void *argvec,
+	void *resultvec, double *inprod /* [K] or NULL */, int her);
+int adda_hip_lincomb_batch(adda_hip_op *op, int prec, int K, const void *gate_state, void *out, const void *a,
+	const adda_cplx *ca, const void *b, const adda_cplx *cb, const void *c, const adda_cplx *cc, int conj_flags,
+	double *norm2); /* coefficients and norm2: [K] */
+int adda_hip_dot_batch(adda_hip_op *op, int prec, int K, const void *gate_state, const void *a, const void *b,
+	int mode, adda_cplx *res, double *norm2); /* res, norm2: [K] */
+int adda_hip_mult_mat_batch(adda_hip_op *op, int prec, int K, void *out, const void *in);
+int adda_hip_profile_matvec_batch(adda_hip_op *op, int prec, int K, const void *argvec, void *resultvec, int nrep,
+	double *pass_ms, double *total_ms);
+double adda_hip_matvec_bytes_batch(const adda_hip_op *op, int prec, int K, int pass);
+/* Solver state of K RHS: slots of RHS k at k*ADDA_DK_NSLOTS (adda_hip_dk_set/get address them so), histories at
+ * k*hist_len. The dk_*_iter calls then advance all K recurrences, with batched vectors. */
+void *adda_hip_dk_create_batch(adda_hip_op *op, int K, int hist_len);
+int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int reps, const char *cache_path, char *report,
+	size_t report_len);
+
+/* HIP graphs: capture_begin starts recording the operator's stream (nothing runs), capture_end instantiates what
+ * was enqueued since (NULL on failure); graph_launch replays it on the stream. Used for chunks of dk_*_iter calls,
+ * which contain no allocation, copy, memset or synchronisation. */
+int adda_hip_capture_begin(adda_hip_op *op);
+void *adda_hip_capture_end(adda_hip_op *op);
+int adda_hip_graph_launch(adda_hip_op *op, void *graph);
+int adda_hip_graph_free(adda_hip_op *op, void *graph);
+
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 

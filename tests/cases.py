@@ -216,6 +216,15 @@ POL_Y = np.array([0.0, 1.0, 0.0])
 POL_X = np.array([1.0, 0.0, 0.0])
 
 
+def orientations(n, seed=20261203):
+    """n random particle orientations as zyz Euler angles in degrees (ADDA's -orient alpha beta gamma,
+    src/crosssec.c:82-110), uniform over the rotation group: alpha, gamma ~ U[0,360), beta = arccos(U[-1,1]);
+    rounded to 6 decimals, the precision they are passed to the reference with."""
+    u = np.random.default_rng(seed).random((n, 3))
+    a = np.c_[360 * u[:, 0], np.degrees(np.arccos(2 * u[:, 1] - 1)), 360 * u[:, 2]]
+    return np.round(a, 6)
+
+
 def sample_case():
     """Config 1: sphere 16^3 (2176 dipoles), m=1.5, dpl=15 (ADDA defaults), grid 32^3."""
     pos, box = make_shape("sphere", 16)

@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from cases import make_shape, read_shape  # noqa: E402
+from cases import make_shape, orientations, read_shape  # noqa: E402
 
 EXE = os.path.join(ROOT, "oracle", "_ref", "adda_sparse")
 
@@ -64,6 +64,10 @@ RUNS = {
     "box8_multimat": ("box8mm", "-dpl 14 -m 1.2 0.01 1.5 0.1 1.8 0 -iter bicgstab -eps 10 -ntheta 19 -store_dip_pol",
                       ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
 }
+# orientations of the batched-RHS workload (ellipsoid 1:0.8:0.5, m = 1.33+0.001i, default dpl 10|m|), first four
+for _i, (_a, _b, _g) in enumerate(orientations(4)):
+    RUNS["ell12_c3o%d" % _i] = ("ell12", "-dpl 13.3 -m 1.33 0.001 -orient %.6f %.6f %.6f -iter qmr -eps 10" % (_a, _b, _g),
+                               ["CrossSec-Y", "CrossSec-X"])
 
 
 def run(name):

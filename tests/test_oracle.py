@@ -219,6 +219,23 @@ def test_orientation_absorbing_ellipsoid():
     assert G.numeric_close(mu, G.mueller("ell12_orient")[:, 1:], 1e-9, 1e-5)[0]
 
 
+@pytest.mark.parametrize("io", range(4))
+def test_batched_workload_orientations(io):
+    """The batched workload's physics (ellipsoid 1:0.8:0.5, m = 1.33+0.001i, dpl 13.3) at four of its seeded
+    orientations: oracle cross sections vs the reference's for each orientation and polarization."""
+    import adda_rot
+    from cases import orientations
+    a, b, g_ = orientations(4)[io]
+    pos, _ = G.geometry("ell12")
+    p, box = read_shape(pos)
+    c = Case(p, box, 1.33 + 0.001j, dpl=13.3, rule=0)
+    prop, pY, pX = (adda_rot.rotate_zyz(a, b, g_, v) for v in (PROP_Z, POL_Y, POL_X))
+    r = _solve_both(c, 0, 1e-10, prop, pY, pX)
+    for pol in ("Y", "X"):
+        g = G.crosssec("ell12_c3o%d" % io, pol)
+        assert G.numeric_close([r[pol]["cext"], r[pol]["cabs"]], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0]
+
+
 def test_cm_polarizability_sphere8():
     pos, _ = G.geometry("sphere8")
     p, box = read_shape(pos)

@@ -111,6 +111,9 @@ def hip():
     L.adda_hip_capture_end.restype = P
     L.adda_hip_graph_launch.argtypes = [P, P]
     L.adda_hip_graph_free.argtypes = [P, P]
+    L.adda_hip_weighted_conj.argtypes = [P, P, P, P]
+    L.adda_hip_adjoint_grad.argtypes = [P, P, P, P, P, D, D, P, P]
+    L.adda_hip_vec_download_part.argtypes = [P, I, P, P, S, S]
     _hip = L
     return L
 
@@ -344,6 +347,9 @@ def host():
     L.adda_generate_b.argtypes = [P, P, D, D, P, P, P]
     L.adda_iterative_solver.argtypes = [P, ctypes.POINTER(SolverOpts), P, P, P, P, I, ctypes.POINTER(SolveStats)]
     L.adda_iterative_solver_batch.argtypes = [P, ctypes.POINTER(SolverOpts), I, P, P, P, P, I, P]
+    L.adda_couple_constant_deps.argtypes = [Cplx, I, D, D, P, P]
+    L.adda_couple_constant_deps.restype = Cplx
+    L.adda_adjoint_gradient.argtypes = [P, P, ctypes.POINTER(SolverOpts), P, I, D, D, P, P, P, P, P, P, P]
     L.adda_calc_field.argtypes = [P, P, D, D, P, P, I, P]
     L.adda_ext_cross.argtypes = [P, P, D, D, P, P, P, P]
     L.adda_abs_cross.argtypes = [P, P, D, P, P, P]
@@ -414,6 +420,13 @@ class Problem:
             self.op.close()
             self.op = None
 
+    def set_materials(self, m):
+        """New refractive indices for the same material table size (takes effect at the next set_polarization)."""
+        m = np.atleast_1d(np.asarray(m, np.complex128))
+        if len(m) != self.nmat:
+            raise ValueError(f"{len(m)} refractive indices for {self.nmat} materials")
+        self.m = m
+
     def set_polarization(self, prop, incpol):
         """InitCC (cc, cc_sqrt for every material) + GenerateB for this incident polarization."""
         self.prop = np.ascontiguousarray(prop, np.float64)
@@ -441,6 +454,24 @@ class Problem:
         n = min(st.niter + 1, hist_len)
         return dict(rc=st.rc, niter=st.niter, nmatvec=st.nmatvec, nmatvec_fp64=st.nmatvec_fp64, nouter=st.nouter,
                     resid=st.resid, true_resid=st.true_resid, hist=hist[:n].copy())
+
+    def adjoint_gradient(self, method="qmr", eps=1e-10, maxiter=None, precision=PREC_FP64, inner_eps=0.0,
+                         max_outer=0, graph=True, max_inner=0):
+        """d(Cext, Cabs)/d(eps_j) for every dipole j of the current solution (call solve() first): two adjoint
+        solves batched together, then per-dipole assembly on the GPU. Returns gext, gabs [nd][2] (d/dRe eps_j,
+        d/dIm eps_j) and the stats of the two adjoint solves."""
+        if maxiter is None:
+            maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
+        o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, IF_AUTO, 0, precision, inner_eps, max_outer, 0, 0,
+                       int(max_inner), int(graph))
+        ge, ga = np.empty(2 * self.nd), np.empty(2 * self.nd)
+        st = (SolveStats * 2)()
+        m = np.ascontiguousarray(self.m)
+        hcheck(host().adda_adjoint_gradient(self.op.h, ctypes.byref(self._p), ctypes.byref(o), _p(m), self.polrel,
+                                            self.wavenum, self.gridspace, _p(self.prop), _p(self.incpol), self.einc,
+                                            self.pvec, _p(ge), _p(ga), st))
+        stats = [dict(rc=s.rc, niter=s.niter, nmatvec=s.nmatvec, nouter=s.nouter, true_resid=s.true_resid) for s in st]
+        return ge.reshape(-1, 2), ga.reshape(-1, 2), stats
 
     def cext(self):
         out = ctypes.c_double()

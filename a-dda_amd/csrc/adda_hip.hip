@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	if (skip_rhs(bq, kq)) return;
 	arg += kq * bq.vs;
 	Xa += kq * bq.xs2;
-	ccs += kq * bq.cs;
+	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	Xa += kq * bq.xs;
 	arg += kq * bq.vs;
 	res += kq * bq.vs;
-	ccs += kq * bq.cs;
+	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
 	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
@@ -700,6 +700,45 @@ __global__ void __launch_bounds__(256) k_wnorm(const cx<T> *v, const int *mat, c
 	}
 	double t = block_sum<T>(s, red_lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// ==================================================================================================================
+// kernels: adjoint gradient
+
+// out[3j+c] = conj(w[mat_j][c] * in[3j+c]); w == null: conj(in)
+__global__ void k_wconj(cx<double> *out, const cx<double> *in, const int *mat, const double *w, size_t nd)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
+		const double s = w ? w[3 * mat[i / 3] + i % 3] : 1.0;
+		const cx<double> v = in[i];
+		out[i] = mk<double>(s * v.x, -s * v.y);
+	}
+}
+
+// Gradient assembly per dipole j (material m = mat_j), q[m][c] = d(1/cc)/d eps:
+//   h = -sum_c q lam_c P_c,  z = -sum_c q mu_c P_c,  loc = sum_c q |P_c|^2
+//   gext[2j] = ce Im h, gext[2j+1] = ce Re h;  gabs[2j] = ca (2 Re z - Im loc), gabs[2j+1] = ca (-2 Im z - Re loc)
+__global__ void k_adjoint_grad(const cx<double> *P, const cx<double> *lam, const cx<double> *mu, const int *mat,
+	const cx<double> *q, double ce, double ca, size_t nd, double *gext, double *gabs)
+{
+	for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < nd; j += (size_t)gridDim.x * blockDim.x) {
+		const int m = mat[j];
+		double hx = 0, hy = 0, zx = 0, zy = 0, lx = 0, ly = 0;
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			const cx<double> qq = q[3 * m + c], p = P[3 * j + c];
+			const cx<double> qp = mul(qq, p);
+			const cx<double> tl = mul(qp, lam[3 * j + c]), tm = mul(qp, mu[3 * j + c]);
+			hx -= tl.x; hy -= tl.y;
+			zx -= tm.x; zy -= tm.y;
+			const double a2 = p.x * p.x + p.y * p.y;
+			lx += qq.x * a2; ly += qq.y * a2;
+		}
+		gext[2 * j] = ce * hy;
+		gext[2 * j + 1] = ce * hx;
+		gabs[2 * j] = ca * (2 * zx - ly);
+		gabs[2 * j + 1] = ca * (-2 * zy - lx);
+	}
 }
 
 // ==================================================================================================================
@@ -2377,6 +2416,63 @@ extern "C" int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int rep
 		}
 	}
 	return 0;
+}
+
+extern "C" int adda_hip_vec_download_part(adda_hip_op *op, int prec, adda_cplx *dst, const void *src, size_t first,
+	size_t n)
+{
+	if (!op || !dst || !src) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t es = elsize(prec);
+	if (prec == ADDA_F64) {
+		HIPCHK(hipMemcpyAsync(dst, (const char *)src + first * es, n * es, hipMemcpyDeviceToHost, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+	}
+	else {
+		std::vector<float> f(2 * n);
+		HIPCHK(hipMemcpyAsync(f.data(), (const char *)src + first * es, n * es, hipMemcpyDeviceToHost, op->stream));
+		HIPCHK(hipStreamSynchronize(op->stream));
+		for (size_t i = 0; i < n; i++) { dst[i].re = f[2 * i]; dst[i].im = f[2 * i + 1]; }
+	}
+	return 0;
+}
+
+extern "C" int adda_hip_weighted_conj(adda_hip_op *op, void *out, const void *in, const double *w)
+{
+	if (!op || !out || !in) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	if (w) HIPCHK(hipMemcpyAsync(op->wbuf, w, (size_t)op->nmat * 3 * sizeof(double), hipMemcpyHostToDevice, op->stream));
+	hipLaunchKernelGGL(k_wconj, dim3(nblocks_red(3 * op->nd)), dim3(256), 0, op->stream, (cx<double> *)out,
+		(const cx<double> *)in, op->mat, w ? (const double *)op->wbuf : nullptr, op->nd);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+extern "C" int adda_hip_adjoint_grad(adda_hip_op *op, const void *pvec, const void *lam, const void *mu,
+	const adda_cplx *q, double ce, double ca, double *grad_ext, double *grad_abs)
+{
+	if (!op || !pvec || !lam || !mu || !q || !grad_ext || !grad_abs) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t nq = (size_t)op->nmat * 3, nd = op->nd;
+	cx<double> *dq = nullptr;
+	double *g = nullptr;
+	HIPCHK(hipMalloc(&dq, nq * sizeof(cx<double>)));
+	hipError_t e = hipMalloc(&g, 4 * nd * sizeof(double));
+	if (e != hipSuccess) { (void)hipFree(dq); return ERR("hipMalloc: %s", hipGetErrorString(e)); }
+	int rc = 0;
+	if (hipMemcpyAsync(dq, q, nq * sizeof(cx<double>), hipMemcpyHostToDevice, op->stream) != hipSuccess) rc = ERR("copy of q failed");
+	if (!rc) {
+		hipLaunchKernelGGL(k_adjoint_grad, dim3(nblocks_red(nd)), dim3(256), 0, op->stream, (const cx<double> *)pvec,
+			(const cx<double> *)lam, (const cx<double> *)mu, op->mat, dq, ce, ca, nd, g, g + 2 * nd);
+		if (hipGetLastError() != hipSuccess) rc = ERR("k_adjoint_grad launch failed");
+	}
+	if (!rc && (hipMemcpyAsync(grad_ext, g, 2 * nd * sizeof(double), hipMemcpyDeviceToHost, op->stream) != hipSuccess ||
+		hipMemcpyAsync(grad_abs, g + 2 * nd, 2 * nd * sizeof(double), hipMemcpyDeviceToHost, op->stream) != hipSuccess ||
+		hipStreamSynchronize(op->stream) != hipSuccess)) rc = ERR("copy of the gradient failed");
+	(void)hipStreamSynchronize(op->stream);
+	(void)hipFree(dq);
+	(void)hipFree(g);
+	return rc;
 }
 
 extern "C" int adda_hip_sync(adda_hip_op *op)

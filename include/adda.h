@@ -115,6 +115,21 @@ int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, doub
 /* ComputeMuellerMatrix over the yz plane (src/CalculateE.c:83-116,236-246); out[16*npoints]. */
 void adda_mueller_yz(const adda_cplx *eplX, const adda_cplx *eplY, int npoints, double *out);
 
+/* ---- adjoint gradient (no counterpart in the reference: derivatives of its Cext and Cabs) ---------------------- */
+/* d(cc)/d(eps) of adda_couple_constant at eps = m^2. */
+adda_cplx adda_couple_constant_deps(adda_cplx m, int polrel, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3]);
+/* Gradient of Cext and Cabs with respect to every dipole's permittivity eps_j = m_j^2 (the permittivity of the
+ * dipole's material; with one material per dipole, nmat = nd, a per-voxel gradient) for the converged forward
+ * solution d_pvec (P) of incident field d_einc with polarization (prop, incpol): two adjoint solves with the same
+ * operator -- A is complex symmetric, so A^T = A -- for the right-hand sides conj(Einc) and conj(W.P), solved as one
+ * batch with opt (method, eps, precision), then per-dipole assembly on the GPU. m: refractive index per material.
+ * grad_ext, grad_abs: [2*nd] = (dC/dRe eps_j, dC/dIm eps_j); st: [2] stats of the two adjoint solves.
+ * Resets the operator's cc_sqrt tables 0 and 1 to this polarization's. */
+int adda_adjoint_gradient(adda_hip_op *op, const adda_particle *p, const adda_solver_opts *opt, const adda_cplx *m,
+	int polrel, double wavenum, double gridspace, const double prop[3], const double incpol[3], const void *d_einc,
+	const void *d_pvec, double *grad_ext, double *grad_abs, adda_solve_stats *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -192,6 +192,20 @@ void *adda_hip_capture_end(adda_hip_op *op);
 int adda_hip_graph_launch(adda_hip_op *op, void *graph);
 int adda_hip_graph_free(adda_hip_op *op, void *graph);
 
+/* ---- adjoint gradient ------------------------------------------------------------------------------------------
+ * No counterpart in the reference (ADDA computes no derivatives); these assemble d(Cext, Cabs)/d(eps_j) from the
+ * solutions of the forward and adjoint systems, see adda_adjoint_gradient in adda.h. */
+/* out[3j+c] = conj(w[mat_j][c] * in[3j+c]) (FP64; w = [nmat][3] host table, NULL: plain conjugate) -- the adjoint
+ * right-hand sides conj(E_inc) and conj(W.P). */
+int adda_hip_weighted_conj(adda_hip_op *op, void *out, const void *in, const double *w);
+/* Per dipole j with material m: h = -sum_c q[m][c] lam_c P_c, z = -sum_c q[m][c] mu_c P_c,
+ * loc = sum_c q[m][c] |P_c|^2; grad_ext[2j] = ce Im h, grad_ext[2j+1] = ce Re h, grad_abs[2j] = ca (2 Re z - Im loc),
+ * grad_abs[2j+1] = ca (-2 Im z - Re loc) (host arrays of 2*nd). q: [nmat][3] = d(1/cc)/d eps. FP64 vectors. */
+int adda_hip_adjoint_grad(adda_hip_op *op, const void *pvec, const void *lam, const void *mu, const adda_cplx *q,
+	double ce, double ca, double *grad_ext, double *grad_abs);
+/* Copies n complex elements starting at element `first` of a device vector to the host (FP64 host data). */
+int adda_hip_vec_download_part(adda_hip_op *op, int prec, adda_cplx *dst, const void *src, size_t first, size_t n);
+
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 

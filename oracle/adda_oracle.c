@@ -716,3 +716,54 @@ void or_mueller_yz(const or_cplx *eplX, const or_cplx *eplY, int ntheta, double 
 		m[15] = creal(s1 * conj(s2) - s3 * conj(s4));
 	}
 }
+
+/*====================================================================================================================*/
+/* Adjoint gradient of Cext and Cabs with respect to each dipole's permittivity eps = m^2. Not a reference function
+ * (ADDA has no gradient); it is the derivative of the reference's own quantities: CoupleConstant
+ * (src/calculator.c:100-133), ExtCross (src/crosssec.c:811-823, equal to 4 pi k Im sum conj(Einc).P for a plane wave)
+ * and AbsCross/DRAINE (src/crosssec.c:852-884). Pinned by central finite differences of the oracle's own forward
+ * solves and by finite differences of the reference's outputs (tests/golden/adj_*). */
+
+/* d(cc)/d(eps) of or_couple_constant: cc = a/(1 - a M/V), a = (3V/4pi)(eps-1)/(eps+2), M = M(eps) */
+or_cplx or_couple_constant_deps(or_cplx mo, int polrel, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3])
+{
+	dcx m = C(mo), e = m * m;
+	double V = gridspace * gridspace * gridspace, kd = wavenum * gridspace;
+	dcx a = (3 / FOUR_PI) * V * (e - 1) / (e + 2), da = (3 / FOUR_PI) * V * 3 / ((e + 2) * (e + 2));
+	if (polrel == 1) return O(da);
+	dcx M = I * 2 * kd * kd * kd / 3, dM = 0;
+	if (polrel != 2) {
+		double S = 0;
+		for (int i = 0; i < 3; i++) S += prop[i] * prop[i] * incpol[i] * incpol[i];
+		M += (LDR_B1 + (LDR_B2 + LDR_B3 * S) * e) * kd * kd;
+		dM = (LDR_B2 + LDR_B3 * S) * kd * kd;
+	}
+	dcx den = 1 - a * M / V;
+	return O((da + a * a * dM / V) / (den * den));
+}
+
+/* Assembly: P solves M P = Einc with M = diag(1/cc) - G; lam = M^-1 conj(Einc), mu = M^-1 conj(W P) (M is
+ * symmetric); q = d(1/cc)/d eps = -cc'/cc^2 per material and component; wabs = -Im(1/cc) - 2k^3/3.
+ *   Cext = ce Im(sum conj(Einc).P):   dCext = ce Im(h de),  h = -sum_c q lam_c P_c
+ *   Cabs = ca sum wabs |P|^2:         dCabs = ca [-Im(q de)|P|^2 + 2 Re(z de)],  z = -sum_c q mu_c P_c
+ * out[2j] = dC/dRe(eps_j), out[2j+1] = dC/dIm(eps_j). */
+void or_adjoint_grad(const or_grid *g, const or_cplx *pvec, const or_cplx *lam, const or_cplx *mu, const or_cplx *q,
+	const double *wabs, double ce, double ca, double *gext, double *gabs)
+{
+	(void)wabs;
+	for (size_t j = 0; j < g->nd; j++) {
+		const int mt = g->mat[j];
+		dcx h = 0, z = 0, loc = 0;
+		for (int c = 0; c < 3; c++) {
+			const dcx qq = C(q[3 * mt + c]), p = C(pvec[3 * j + c]);
+			h -= qq * C(lam[3 * j + c]) * p;
+			z -= qq * C(mu[3 * j + c]) * p;
+			loc += qq * (creal(p) * creal(p) + cimag(p) * cimag(p));
+		}
+		gext[2 * j] = ce * cimag(h);
+		gext[2 * j + 1] = ce * creal(h);
+		gabs[2 * j] = ca * (-cimag(loc) + 2 * creal(z));
+		gabs[2 * j + 1] = ca * (-creal(loc) - 2 * cimag(z));
+	}
+}

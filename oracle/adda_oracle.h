@@ -96,6 +96,18 @@ void or_eplane_yz(const or_grid *g, const or_cplx *pvec, const double prop[3], c
  * (src/CalculateE.c:236-246): out[16*i..] row-major 4x4 per angle. */
 void or_mueller_yz(const or_cplx *eplX, const or_cplx *eplY, int npoints, double *out);
 
+/* d(cc)/d(eps) of or_couple_constant at eps = m^2 (derivative of its closed form). Pinned: central differences of
+ * or_couple_constant in eps (tests/test_oracle.py). */
+or_cplx or_couple_constant_deps(or_cplx m, int polrel, double wavenum, double gridspace, const double prop[3],
+	const double incpol[3]);
+
+/* Adjoint gradient assembly of Cext = ce Im(sum conj(Einc).P) and Cabs = ca sum wabs|P|^2 with respect to each
+ * dipole's permittivity: pvec = P, lam = M^-1 conj(Einc), mu = M^-1 conj(W.P) (M = diag(1/cc) - G, symmetric),
+ * q[nmat][3] = d(1/cc)/d eps; gext/gabs[2j] = dC/dRe(eps_j), [2j+1] = dC/dIm(eps_j). Pinned: central differences of
+ * the oracle's forward solves and of the reference's outputs (tests/golden/adj_*). */
+void or_adjoint_grad(const or_grid *g, const or_cplx *pvec, const or_cplx *lam, const or_cplx *mu, const or_cplx *q,
+	const double *wabs, double ce, double ca, double *gext, double *gabs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,9 @@ def oracle():
         lib.or_matvec_direct.argtypes = [P, P, P, P, P, ctypes.c_int]
         lib.or_couple_constant.argtypes = [OrCplx, ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P]
         lib.or_couple_constant.restype = OrCplx
+        lib.or_couple_constant_deps.argtypes = [OrCplx, ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P]
+        lib.or_couple_constant_deps.restype = OrCplx
+        lib.or_adjoint_grad.argtypes = [P, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
         lib.or_generate_b.argtypes = [P, P, P, P]
         lib.or_iterative_solver.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                             P, P, P, ctypes.c_int, P, P]
@@ -146,6 +149,74 @@ class Case:
                                        ptr(incpol))
             out[i, :] = complex(r.re, r.im)
         return out, np.sqrt(out)
+
+    def cc_deps(self, prop, incpol, polrel=0):
+        """d(cc)/d(eps) per material [nmat][3] at eps = m^2."""
+        lib = oracle()
+        prop = np.ascontiguousarray(prop, np.float64)
+        incpol = np.ascontiguousarray(incpol, np.float64)
+        out = np.empty((len(self.m), 3), np.complex128)
+        for i, mi in enumerate(self.m):
+            r = lib.or_couple_constant_deps(OrCplx(mi.real, mi.imag), polrel, self.wavenum, self.gridspace, ptr(prop),
+                                            ptr(incpol))
+            out[i, :] = complex(r.re, r.im)
+        return out
+
+    def with_m(self, m):
+        """The same geometry and grid with another material table."""
+        return Case(self.pos, self.box, m, dpl=self.dpl, lam=self.lam, mat=self.mat, reduced_fft=self.reduced_fft,
+                    grid=self.grid)
+
+    def adjoint_grad(self, dmat, prop, incpol, polrel=0, method=0, eps=1e-12):
+        """Oracle adjoint gradient of (Cext, Cabs) w.r.t. each dipole's permittivity: forward solve, adjoint solves
+        A.y = S.conj(Einc) and A.y' = S.conj(W.P), assembly (or_adjoint_grad). Returns gext, gabs [nd][2]
+        (d/dRe eps, d/dIm eps), Cext, Cabs."""
+        cc, ccs = self.cc(prop, incpol, polrel)
+        e = self.einc(prop, incpol)
+        P = self.solve(dmat, ccs, e, method=method, eps=eps)["p"]
+        k3 = 2 * self.wavenum ** 3 / 3
+        wabs = -np.imag(1 / cc) - k3
+        lam = self.solve(dmat, ccs, np.conj(e), method=method, eps=eps)["p"]
+        W = wabs[self.mat].reshape(-1)
+        mu = self.solve(dmat, ccs, np.conj(W * P), method=method, eps=eps)["p"]
+        q = np.ascontiguousarray(-self.cc_deps(prop, incpol, polrel) / cc ** 2)
+        ge, ga = np.empty(2 * self.nd), np.empty(2 * self.nd)
+        ce = ca = 4 * np.pi * self.wavenum
+        oracle().or_adjoint_grad(self.gp, ptr(P), ptr(lam), ptr(mu), ptr(q), ptr(np.ascontiguousarray(wabs)), ce, ca,
+                                 ptr(ge), ptr(ga))
+        return ge.reshape(-1, 2), ga.reshape(-1, 2), self.cext(P, prop, incpol), self.cabs(P, cc)
+
+    def fd_grad(self, dmat, prop, incpol, j, h=1e-6, polrel=0, method=0, eps=1e-13):
+        """Central finite differences of (Cext, Cabs) in Re and Im of material j's permittivity (every dipole of
+        material j), from forward solves only. C(eps +- h) - C(eps) is evaluated as an exact difference: with
+        M' = M + dM (dM non-zero only on material j), P' - P solves M'(P' - P) = -dM.P -- a forward solve of the
+        perturbed system with a right-hand side local to material j -- so no cancellation of O(1) sums occurs.
+        Returns [[dCext/dRe, dCext/dIm], [dCabs/dRe, dCabs/dIm]]."""
+        cc, ccs = self.cc(prop, incpol, polrel)
+        e = self.einc(prop, incpol)
+        P = self.solve(dmat, ccs, e, method=method, eps=eps)["p"]
+        k3 = 2 * self.wavenum ** 3 / 3
+        w = (-np.imag(1 / cc) - k3)[self.mat].reshape(-1)
+        sel = np.repeat(self.mat == j, 3)
+        out = np.zeros((2, 2))
+        for col, dz in ((0, h), (1, 1j * h)):
+            dC = []
+            for sgn in (1, -1):
+                m2 = self.m.copy()
+                m2[j] = np.sqrt(self.m[j] ** 2 + sgn * dz)
+                c2 = self.with_m(m2)
+                cc2, ccs2 = c2.cc(prop, incpol, polrel)
+                dMdiag = (1 / cc2 - 1 / cc)[self.mat].reshape(-1)  # non-zero on material j only
+                rhs = np.where(sel, -dMdiag * P, 0)
+                dP = c2.solve(dmat, ccs2, rhs, method=method, eps=eps)["p"]
+                w2 = (-np.imag(1 / cc2) - k3)[self.mat].reshape(-1)
+                dext = 4 * np.pi * self.wavenum * np.imag(np.sum(np.conj(e) * dP))
+                dabs = 4 * np.pi * self.wavenum * (np.sum(w * (2 * np.real(np.conj(P) * dP) + np.abs(dP) ** 2))
+                                                    + np.sum((w2 - w) * np.abs(P + dP) ** 2))
+                dC.append((dext, dabs))
+            out[0, col] = (dC[0][0] - dC[1][0]) / (2 * h)
+            out[1, col] = (dC[0][1] - dC[1][1]) / (2 * h)
+        return out
 
     def dmatrix(self):
         lib = oracle()
@@ -214,6 +285,27 @@ class Case:
 PROP_Z = np.array([0.0, 0.0, 1.0])
 POL_Y = np.array([0.0, 1.0, 0.0])
 POL_X = np.array([1.0, 0.0, 0.0])
+
+
+def adj8_materials(nd):
+    """The at-most-15-material variant of the per-voxel gradient workload (box 8^3): 14 refractive indices drawn
+    like the per-voxel ones (Re ~ U(1.2,1.8), Im ~ U(0,0.05), seed 20261204, rounded to 4 decimals), voxels assigned at
+    random; four finite-difference voxels (seed 20261205), each moved alone to material 15 = its own m +- ADJ_DELTA
+    (real or imaginary part) in the reference runs of tests/golden/adj8/."""
+    rng = np.random.default_rng(20261204)
+    m = np.round(rng.uniform(1.2, 1.8, 14), 4) + 1j * np.round(rng.uniform(0, 0.05, 14), 4)
+    mat = rng.integers(0, 14, nd)
+    vox = np.random.default_rng(20261205).choice(nd, 4, replace=False)
+    return m, mat, vox
+
+
+ADJ_DELTA = 1e-3
+
+
+def per_voxel_m(nd, seed=20261204):
+    """Per-voxel refractive indices of the gradient workload: Re ~ U(1.2,1.8), Im ~ U(0,0.05)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(1.2, 1.8, nd) + 1j * rng.uniform(0, 0.05, nd)
 
 
 def orientations(n, seed=20261203):

@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
-from cases import make_shape, orientations, read_shape  # noqa: E402
+from cases import ADJ_DELTA, adj8_materials, make_shape, orientations, read_shape  # noqa: E402
 
 EXE = os.path.join(ROOT, "oracle", "_ref", "adda_sparse")
 
@@ -46,7 +46,11 @@ def geometries():
     pos, _ = make_shape("box", 8, 0.75, 0.5)
     rng = np.random.default_rng(20261203)
     g["box8mm"] = (pos, rng.integers(0, 3, len(pos)))
+    pos, _ = make_shape("box", 8)
+    g["adj8"] = (pos, adj8_materials(len(pos))[1])
     return g
+
+
 
 
 RUNS = {
@@ -70,7 +74,52 @@ for _i, (_a, _b, _g) in enumerate(orientations(4)):
                                ["CrossSec-Y", "CrossSec-X"])
 
 
+def adj8_runs():
+    """name -> (m list, voxel or None, (dRe, dIm))"""
+    pos, _ = make_shape("box", 8)
+    m, mat, vox = adj8_materials(len(pos))
+    runs = {"adj8_base": (m, None, (0, 0))}
+    for j in vox:
+        for tag, d in (("rp", (ADJ_DELTA, 0)), ("rm", (-ADJ_DELTA, 0)), ("ip", (0, ADJ_DELTA)), ("im", (0, -ADJ_DELTA))):
+            runs["adj8_v%d_%s" % (j, tag)] = (m, int(j), d)
+    return runs
+
+
+def run_adj(name):
+    m, j, (dr, di) = adj8_runs()[name]
+    pos, _ = make_shape("box", 8)
+    p, _ = read_shape(pos)
+    _, mat, _ = adj8_materials(len(pos))
+    mats = list(m)
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        gfile = os.path.join(HERE, "geom", "adj8.geom")
+        if j is not None:
+            mat = mat.copy()
+            mats.append(m[mat[j]] + dr + 1j * di)
+            mat[j] = 14
+            gfile = os.path.join(tmp, "adj8_v%d.geom" % j)
+            write_geom(gfile, p, mat)
+        margs = " ".join("%.10g %.10g" % (z.real, z.imag) for z in mats)
+        args = "-dpl 18 -m %s -iter qmr -eps 12" % margs
+        cmd = [EXE, "-shape", "read", gfile] + args.split() + ["-dir", os.path.join(tmp, "run")]
+        subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, cwd=tmp)
+        out = os.path.join(HERE, "adj8", name)
+        os.makedirs(out, exist_ok=True)
+        for f in ("CrossSec-Y", "CrossSec-X"):
+            shutil.copy(os.path.join(tmp, "run", f), os.path.join(out, f))
+        with open(os.path.join(out, "cmd.txt"), "w") as f:
+            geo = "tests/golden/geom/adj8.geom" if j is None else \
+                "adj8_v%d.geom (tests/golden/geom/adj8.geom with voxel %d moved to material 15)" % (j, j)
+            f.write("oracle/_ref/adda_sparse -shape read %s %s -dir run\n" % (geo, args))
+        return name
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def run(name):
+    if name.startswith("adj8"):
+        return run_adj(name)
     geo, args, keep = RUNS[name]
     out = os.path.join(HERE, name)
     tmp = tempfile.mkdtemp(prefix="golden_")
@@ -104,7 +153,9 @@ def main():
     for gname, (pos, mat) in geometries().items():
         p, _ = read_shape(pos)
         write_geom(os.path.join(HERE, "geom", gname + ".geom"), p, mat)
-    names = sys.argv[1:] or list(RUNS)
+    names = sys.argv[1:] or list(RUNS) + list(adj8_runs())
+    if names == ["adj8"]:
+        names = list(adj8_runs())
     with ThreadPoolExecutor(max_workers=6) as ex:
         for n in ex.map(run, names):
             print("done", n, flush=True)

@@ -63,3 +63,32 @@ def numeric_close(a, b, abs_tol, rel_tol):
     scale = np.maximum(np.abs(a), np.abs(b))
     bad = (d > abs_tol) & (d > rel_tol * np.where(scale > 0, scale, 1))
     return not bad.any(), np.argwhere(bad)
+
+
+def adj8_fd(pol):
+    """Central differences of the reference's Cext and Cabs in Re(m) and Im(m) of each finite-difference voxel of the
+    adj8 fixtures: {voxel: [[dCext/dRe m, dCext/dIm m], [dCabs/dRe m, dCabs/dIm m]]}."""
+    from cases import ADJ_DELTA, adj8_materials
+    _, _, vox = adj8_materials(512)
+    out = {}
+    for j in vox:
+        c = {t: crosssec("adj8/adj8_v%d_%s" % (j, t), pol) for t in ("rp", "rm", "ip", "im")}
+        g = np.zeros((2, 2))
+        for row, q in enumerate(("Cext", "Cabs")):
+            g[row, 0] = (c["rp"][q] - c["rm"][q]) / (2 * ADJ_DELTA)
+            g[row, 1] = (c["ip"][q] - c["im"][q]) / (2 * ADJ_DELTA)
+        out[int(j)] = g
+    return out
+
+
+def adj8_fd_resolution(c):
+    """Resolution of a central difference of a quantity of size c printed to 10 significant digits (one unit in the
+    last place of each of the two values, over 2*delta)."""
+    from cases import ADJ_DELTA
+    ulp = 10.0 ** (np.floor(np.log10(abs(c))) - 9)
+    return ulp / ADJ_DELTA
+
+
+def deps_to_dm(g, m):
+    """(dC/dRe eps, dC/dIm eps) -> (dC/dRe m, dC/dIm m) with d eps = 2 m dm."""
+    return 2 * (g[0] * m.real + g[1] * m.imag), 2 * (g[1] * m.real - g[0] * m.imag)

@@ -258,3 +258,73 @@ def test_multimaterial_box_bicgstab():
         assert G.numeric_close([r[pol]["cext"], r[pol]["cabs"]], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0]
     gp = G.dippol("box8_multimat", "Y")
     assert G.numeric_close(np.c_[r["Y"]["p"].real, r["Y"]["p"].imag], np.c_[gp.real, gp.imag], 1e-12, 1e-6)[0]
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# adjoint gradient of Cext and Cabs with respect to each dipole's permittivity
+
+@pytest.mark.parametrize("polrel", [0, 1, 2])
+def test_couple_constant_deps(polrel):
+    """d(cc)/d(eps) against central differences of CoupleConstant in the real and imaginary directions of eps."""
+    lib = oracle()
+    from cases import OrCplx
+    prop, pol = np.array([0.0, 0.6, 0.8]), np.array([1.0, 0.0, 0.0])
+    for m in (1.5, 1.33 + 0.001j, 1.8 + 0.05j, 1.2 + 0.3j):
+        eps = m * m
+        d = lib.or_couple_constant_deps(OrCplx(m.real, m.imag), polrel, 1.0, 0.35, ptr(prop), ptr(pol))
+        d = complex(d.re, d.im)
+        h = 1e-6
+        def cc(e):
+            z = np.sqrt(e)
+            r = lib.or_couple_constant(OrCplx(z.real, z.imag), polrel, 1.0, 0.35, ptr(prop), ptr(pol))
+            return complex(r.re, r.im)
+        fr = (cc(eps + h) - cc(eps - h)) / (2 * h)
+        fi = (cc(eps + 1j * h) - cc(eps - 1j * h)) / (2j * h)
+        assert abs(fr - d) < 1e-7 * abs(d) and abs(fi - d) < 1e-7 * abs(d), (m, d, fr, fi)
+
+
+def _adj_case(box=5):
+    from cases import per_voxel_m
+    pos, _ = make_shape("box", box)
+    p, b = read_shape(pos)
+    nd = len(p)
+    return Case(p, b, per_voxel_m(nd), dpl=18, mat=np.arange(nd, dtype=np.int32), rule=1)
+
+
+@pytest.mark.parametrize("pol", ["Y", "X"])
+def test_oracle_adjoint_vs_central_differences(pol):
+    """Per-voxel permittivities (box 5^3): adjoint gradient vs central differences (step 1e-6) of forward solves."""
+    c = _adj_case()
+    d = c.dmatrix()
+    vec = POL_Y if pol == "Y" else POL_X
+    ge, ga, _, _ = c.adjoint_grad(d, PROP_Z, vec)
+    scale = np.abs(np.c_[ge, ga]).max()
+    for j in np.random.default_rng(20261205).choice(c.nd, 6, replace=False):
+        fd = c.fd_grad(d, PROP_Z, vec, int(j))
+        err = np.abs(fd - np.array([ge[j], ga[j]])).max()
+        assert err < 1e-7 * scale, (j, fd, ge[j], ga[j])
+
+
+@pytest.mark.parametrize("pol", ["Y", "X"])
+def test_oracle_adjoint_vs_reference_differences(pol):
+    """15-material variant (box 8^3): the oracle's adjoint gradient, converted to d/dRe(m) and d/dIm(m) of the voxel
+    (d eps = 2 m dm), vs central differences of the reference's own Cext and Cabs (voxel moved alone to material 15,
+    m +- 1e-3). Bar: 1e-4 relative plus the resolution of those differences, whose inputs the reference prints to 10
+    significant digits."""
+    from cases import adj8_materials
+    pos, _ = G.geometry("adj8")
+    p, b = read_shape(pos)
+    m, mat, vox = adj8_materials(len(p))
+    c = Case(p, b, m[mat], dpl=18, mat=np.arange(len(p), dtype=np.int32), rule=0)
+    vec = POL_Y if pol == "Y" else POL_X
+    ge, ga, cext, cabs = c.adjoint_grad(c.dmatrix(), PROP_Z, vec)
+    base = G.crosssec("adj8/adj8_base", pol)
+    assert abs(cext - base["Cext"]) < 1e-8 * base["Cext"] and abs(cabs - base["Cabs"]) < 1e-8 * base["Cabs"]
+    fd = G.adj8_fd(pol)
+    for j in vox:
+        for row, g in ((0, ge[j]), (1, ga[j])):
+            dre, dim = G.deps_to_dm(g, m[mat[j]])
+            ref = fd[int(j)][row]
+            tol = G.adj8_fd_resolution(base["Cext" if row == 0 else "Cabs"])
+            assert abs(dre - ref[0]) < 1e-4 * abs(ref[0]) + tol and abs(dim - ref[1]) < 1e-4 * abs(ref[1]) + tol, \
+                (j, row, dre, dim, ref, tol)

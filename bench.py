@@ -46,6 +46,7 @@ def main():
                     help="profiling mode: autotune, 20 matvecs per precision and one 52.8 MB device copy (the "
                          "calibration kernel for HBM counters), no solves; prints the matvec figures only")
     ap.add_argument("--qmr-max-inner", type=int, default=2000)
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched-RHS and adjoint-gradient workloads")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache (JSON lines): configurations found there are reused without timing")
     args = ap.parse_args()
@@ -143,12 +144,17 @@ def main():
         log(f"QMR_CS: FP64 {tq64:.3f} s ({rq64['niter']} it), mixed {tqm:.3f} s ({rqm['niter']} it, "
             f"{rqm['nouter']} outer, true residual {rqm['true_resid']:.2e}), Cext rel err {extra['qmr']['cext_rel_err']:.2e}")
 
+    if not args.no_extra:
+        extra["batched_rhs"] = bench_batched(A, args)
+        extra["adjoint_gradient"] = bench_adjoint(A)
+
     # ---- roofline: dominant kernel of the timed workload (FP32 matvec pass P3) --------------------------------------
     dom = mv["fp32"]
     k = int(np.argmax(dom["passes"]))
     achieved = dom["bytes"][k] / (dom["passes"][k] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": PASS_NAMES[k] + " (fp32)", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(k, tune["fp32"]), "algorithmic_bytes": dom["bytes"][k]}
     matvec_roof = {name: {"GBps": round(v["total_bytes"] / (v["ms"] * 1e-3) / 1e9, 1),
                           "frac": round(v["total_bytes"] / (v["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                           "bytes_per_matvec": v["total_bytes"], "us_per_matvec": round(v["ms"] * 1e3, 2),
@@ -188,6 +194,130 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(pass_index, cfg):
+    """HBM bytes per launch of the FP32 pass kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (calibrated against a copy of known size), when it was taken with the same kernel instantiation."""
+    names = ["k_p1", "k_zfft<float, false", "k_p3", "k_zfft<float, true", "k_p5"]
+    try:
+        radix = cfg.split()[{0: 0, 1: 2, 2: 1, 3: 2, 4: 0}[pass_index]].split("=")[1]
+        rm = 8 if max(int(r) for r in radix.split(".")) <= 8 else 16
+        want = names[pass_index] + (f"<float, {rm}>" if pass_index in (0, 2, 4) else f", {rm}>")
+        with open(os.path.join(ROOT, "profiles", "matvec_grid256_hbm_traffic.txt")) as f:
+            for line in f:
+                if line.startswith("P") and want in line:
+                    return float(line.split()[-3])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
+def bench_batched(A, args):
+    """Batched multi-RHS workload: ellipsoid on a 64^3 grid (semi-axes 1:0.8:0.5, box 64x52x32), m = 1.33+0.001i,
+    default dpl 10|m|, 32 seeded orientations x 2 polarizations = 64 right-hand sides solved together to 1e-10 (FP64
+    and FP32-mixed, HIP-graph replay), against the same 64 solved one after another (ADDA's orientation loop)."""
+    from adda_rot import rotate_zyz
+    from cases import orientations, PROP_Z, POL_X, POL_Y
+    pos, _, box = A.make_particle("ellipsoid", 64, 0.8, 0.5)
+    props, pols = [], []
+    for a, b, g in orientations(32):
+        prop, pY, pX = (rotate_zyz(a, b, g, v) for v in (PROP_Z, POL_Y, POL_X))
+        props += [prop, prop]
+        pols += [pY, pX]
+    K = len(props)
+    m = 1.33 + 0.001j
+    bp = A.BatchProblem(pos, box, m, np.array(props), np.array(pols), fp32=True)
+    out = {"workload": f"ellipsoid box {box[0]}x{box[1]}x{box[2]} ({len(pos)} dipoles), grid "
+                       f"{bp.grid[0]}x{bp.grid[1]}x{bp.grid[2]}, m=1.33+0.001i, 32 orientations x 2 polarizations = "
+                       f"{K} RHS, eps 1e-10", "rhs": K}
+    try:
+        op = bp.op
+        out["autotune"] = {n: op.autotune_batch(K, prec, reps=3, cache=args.tune_cache)
+                           for prec, n in ((A.F64, "fp64"), (A.F32, "fp32"))}
+        for prec, n in ((A.F64, "fp64"), (A.F32, "fp32")):
+            a, r = op.alloc_batch(K, prec), op.alloc_batch(K, prec)
+            op.profile_matvec_batch(a, r, K, prec, nrep=2)
+            passes, tot = op.profile_matvec_batch(a, r, K, prec, nrep=10)
+            op.free(a)
+            op.free(r)
+            by = op.matvec_bytes_batch(K, prec)
+            out[f"batched_matvec_{n}"] = {"us": round(tot * 1e3, 1), "rhs_matvecs_per_s": round(K * 1e3 / tot, 1),
+                                          "GBps": round(by / (tot * 1e-3) / 1e9, 1),
+                                          "pass_us": [round(x * 1e3, 1) for x in passes]}
+        res = {}
+        for method in ("bicgstab", "qmr"):
+            for name, kw in (("fp64", {}), ("mixed", {"precision": A.PREC_MIXED}),
+                             ("fp64_nograph", {"graph": False})):
+                bp.solve(method, eps=1e-10, **kw)  # warm-up (graph capture, allocations)
+                op.sync()
+                t = time.perf_counter()
+                st = bp.solve(method, eps=1e-10, **kw)
+                cx = [bp.cext(k) for k in range(K)]
+                res[(method, name)] = (time.perf_counter() - t, cx, st)
+            c64 = np.array(res[(method, "fp64")][1])
+            cm = np.array(res[(method, "mixed")][1])
+            out[method] = {"tts_batched_fp64_s": round(res[(method, "fp64")][0], 4),
+                           "tts_batched_fp64_no_graph_s": round(res[(method, "fp64_nograph")][0], 4),
+                           "tts_batched_mixed_s": round(res[(method, "mixed")][0], 4),
+                           "iterations_fp64_max": max(s["niter"] for s in res[(method, "fp64")][2]),
+                           "iterations_mixed_max": max(s["niter"] for s in res[(method, "mixed")][2]),
+                           "max_true_resid_mixed": max(s["true_resid"] for s in res[(method, "mixed")][2]),
+                           "cext_max_rel_err_mixed_vs_fp64": float(np.max(np.abs(cm - c64) / np.abs(c64)))}
+        # the same RHS one after another on one operator (ADDA's loop over orientations and polarizations)
+        pr = A.Problem(pos, box, m, fp32=True)
+        try:
+            pr.op.autotune(A.F64, reps=3, cache=args.tune_cache)
+            t = time.perf_counter()
+            cs = []
+            for k in range(K):
+                pr.set_polarization(props[k], pols[k])
+                pr.solve("qmr", eps=1e-10)
+                cs.append(pr.cext())
+            t_seq = time.perf_counter() - t
+        finally:
+            pr.close()
+        out["qmr"]["tts_sequential_fp64_s"] = round(t_seq, 4)
+        out["qmr"]["speedup_batched_fp64_vs_sequential"] = round(t_seq / out["qmr"]["tts_batched_fp64_s"], 3)
+        out["qmr"]["speedup_batched_mixed_vs_sequential_fp64"] = round(t_seq / out["qmr"]["tts_batched_mixed_s"], 3)
+        out["qmr"]["cext_max_rel_err_batched_vs_sequential"] = float(np.max(np.abs(np.array(cs) - np.array(
+            res[("qmr", "fp64")][1])) / np.abs(np.array(cs))))
+        log(f"batched RHS ({K}): " + json.dumps({k: out[k] for k in ("bicgstab", "qmr")}))
+    finally:
+        bp.close()
+    return out
+
+
+def bench_adjoint(A):
+    """Adjoint-gradient workload: 48^3 cube, per-voxel m (Re ~ U(1.2,1.8), Im ~ U(0,0.05), seed 20261204), QMR_CS
+    to 1e-10: forward solve + the two batched adjoint solves + assembly, FP64 and FP32-mixed."""
+    from cases import per_voxel_m, PROP_Z, POL_Y
+    pos, _, box = A.make_particle("box", 48)
+    nd = len(pos)
+    pr = A.Problem(pos, box, per_voxel_m(nd), mat=np.arange(nd, dtype=np.int32), fp32=True)
+    out = {"workload": f"cube {box[0]}^3 ({nd} voxels, one material each), grid {pr.grid[0]}^3, QMR_CS to 1e-10"}
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        g = {}
+        for name, prec in (("fp64", A.PREC_FP64), ("mixed", A.PREC_MIXED)):
+            pr.solve("qmr", eps=1e-10, precision=prec)
+            pr.adjoint_gradient("qmr", eps=1e-10, precision=prec)  # warm-up
+            pr.op.sync()
+            t0 = time.perf_counter()
+            r = pr.solve("qmr", eps=1e-10, precision=prec)
+            t1 = time.perf_counter()
+            ge, ga, st = pr.adjoint_gradient("qmr", eps=1e-10, precision=prec)
+            t2 = time.perf_counter()
+            g[name] = (ge, ga)
+            out[name] = {"forward_s": round(t1 - t0, 4), "adjoint_s": round(t2 - t1, 4),
+                         "forward_iterations": r["niter"], "adjoint_iterations": [s["niter"] for s in st]}
+        out["grad_max_rel_diff_mixed_vs_fp64"] = {
+            q: float(np.abs(g["mixed"][i] - g["fp64"][i]).max() / np.abs(g["fp64"][i]).max())
+            for i, q in enumerate(("cext", "cabs"))}
+        log("adjoint: " + json.dumps(out))
+    finally:
+        pr.close()
+    return out
 
 
 if __name__ == "__main__":

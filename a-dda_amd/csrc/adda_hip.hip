@@ -263,7 +263,7 @@ __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ tw
 // of a tile read the same dipoles back to back, and the RHS of a tile share its cell-map and material reads). The
 // gather of S.x along x (x fastest) fills the LDS lines; the last FFT stage stores Xa[z][x][c][y] with y fastest
 // across threads (runs of LY).
-template <typename T, int RM>
+template <typename T, class PL>
 __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
 	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	Xa += kq * bq.xs2;
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
-	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
@@ -299,11 +299,11 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	__syncthreads();
 	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
 	auto st = [&](int line, int x, cx<T> v) { Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line] = v; };
-	fft_line<T, false, RM>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 }
 
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
-template <typename T, bool INV, int RM>
+template <typename T, bool INV, class PL>
 __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
 	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, Bq bq)
 {
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	in += kq * bq.xs;
 	out += kq * bq.xs2;
 	extern __shared__ char smem_raw[];
-	const int N = p.n, TPL = N / p.rmax;
+	const int N = PL::n(p), TPL = PL::tpl(p);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const long q0 = (long)(blockIdx.x / bq.K) * TC;
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	auto st = [&](int l, int k, cx<T> v) {
 		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
 	};
-	fft_line<T, INV, RM>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+	PL::template fft<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
 }
 
 // P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 // and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
 // LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
 // cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
-template <typename T, int RM>
+template <typename T, class PL>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
 	Bq bq)
@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	if (skip_rhs(bq, kq)) return;
 	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
-	const int N = p.n, TPL = N / p.rmax, NL1 = 3 * LP, LS = rowstride(N);
+	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP, LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nxt = gX / LP;
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	auto gptr = [&](int l) -> cx<T> * { return l < NL1 ? base0 + (size_t)l * bY : base1 + (size_t)(l - NL1) * bY; };
 	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? gptr(l)[y] : mk<T>(0, 0); };
 	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	fft_line<T, false, RM>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	PL::template fft<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
 	__syncthreads();
 	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
 		const int pr = idx / N, y = idx % N, x = x0 + pr;
@@ -397,13 +397,13 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	auto stg = [&](int l, int y, cx<T> v) {
 		if (y < bY) gptr(l)[y] = v;
 	};
-	fft_line<T, true, RM>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+	PL::template fft<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
 // P5: block = (z, y0..y0+LY, RHS, c), component fastest in the block index, then the RHS; stage 0 loads
 // Xa[z][x][c][y] y-fastest (runs of LY), the results go to LDS and are gathered to the dipoles along x:
 // result = arg + S.X (her: conj), partial ||result||^2 per block at partials[k*(blocks per RHS) + tile*3 + c].
-template <typename T, int RM>
+template <typename T, class PL>
 __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
 	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
-	const int N = p.n, TPL = N / p.rmax, LS = rowstride(N);
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	double s = 0;
 	auto ld = [&](int line, int x) -> cx<T> { return Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line]; };
 	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
-	fft_line<T, true, RM>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 	__syncthreads();
 	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
 		const int line = idx / bX, x = idx % bX;
@@ -780,24 +780,72 @@ static int reduce_rows(adda_hip_op *op, const double *partials, int nblk, int nk
 	return 0;
 }
 
+// Compile-time plans built into this library (any other schedule runs on the run-time plan kernels).
+template <class... Ps> struct PlanList {};
+using FixPlans = PlanList<Fix<16, 16>, Fix<8, 8, 4>, Fix<4, 8, 8>, Fix<16, 4, 4>>;
+
+// Calls f(PL{}) for the first compiled plan equal to p; false if none.
+template <class F, class... Ps> static bool with_fix(const FftPlan2 &p, F &&f, PlanList<Ps...>)
+{
+	return (... || (Ps::match(p) ? (f(Ps{}), true) : false));
+}
+
+template <typename T, class PL>
+static int set_lds_limits_pl()
+{
+	const int big = 96 * 1024;  // largest dynamic LDS tile used
+	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	return 0;
+}
+
+template <typename T, class... Ps>
+static int set_lds_limits_all(PlanList<Ps...>)
+{
+	int rc = 0;
+	((rc = rc ? rc : set_lds_limits_pl<T, Ps>()), ...);
+	return rc;
+}
+
 template <typename T>
 static int set_lds_limits()
 {
-	const int big = 96 * 1024;  // largest dynamic LDS tile used (k_ydmul)
-	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	const int big = 96 * 1024;
+	if (set_lds_limits_pl<T, Dyn<16>>() || set_lds_limits_pl<T, Dyn<8>>() || set_lds_limits_all<T>(FixPlans{}))
+		return -1;
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	return 0;
+}
+
+// Kernel of a pass for plan q: the compile-time instantiation when q is one of FixPlans, else the run-time one.
+template <typename T> static auto pick_p1(const FftPlan2 &q)
+{
+	auto f = q.rmax <= 8 ? &k_p1<T, Dyn<8>> : &k_p1<T, Dyn<16>>;
+	with_fix(q, [&](auto pl) { f = &k_p1<T, decltype(pl)>; }, FixPlans{});
+	return f;
+}
+template <typename T> static auto pick_p3(const FftPlan2 &q)
+{
+	auto f = q.rmax <= 8 ? &k_p3<T, Dyn<8>> : &k_p3<T, Dyn<16>>;
+	with_fix(q, [&](auto pl) { f = &k_p3<T, decltype(pl)>; }, FixPlans{});
+	return f;
+}
+template <typename T> static auto pick_p5(const FftPlan2 &q)
+{
+	auto f = q.rmax <= 8 ? &k_p5<T, Dyn<8>> : &k_p5<T, Dyn<16>>;
+	with_fix(q, [&](auto pl) { f = &k_p5<T, decltype(pl)>; }, FixPlans{});
+	return f;
+}
+template <typename T, bool INV> static auto pick_zfft(const FftPlan2 &q)
+{
+	auto f = q.rmax <= 8 ? &k_zfft<T, INV, Dyn<8>> : &k_zfft<T, INV, Dyn<16>>;
+	with_fix(q, [&](auto pl) { f = &k_zfft<T, INV, decltype(pl)>; }, FixPlans{});
+	return f;
 }
 
 // ==================================================================================================================
@@ -841,7 +889,7 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 	const cx<T> *ccs, int her)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
-	auto k = c.qx.rmax <= 8 ? k_p1<T, 8> : k_p1<T, 16>;
+	auto k = pick_p1<T>(c.qx);
 	hipLaunchKernelGGL(k, dim3(3 * op->bZ * nyt * K), dim3(c.LY * (op->gX / c.qx.rmax)),
 		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx,
 		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
@@ -853,7 +901,7 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 	const long Q = (long)op->gX * 3 * op->bY;
 	const cx<T> *in = (const cx<T> *)(INV ? op->Xb : op->Xa);
 	cx<T> *out = (cx<T> *)(INV ? op->Xa : op->Xb);
-	auto k = c.qz.rmax <= 8 ? k_zfft<T, INV, 8> : k_zfft<T, INV, 16>;
+	auto k = pick_zfft<T, INV>(c.qz);
 	const long xin = INV ? xb_stride(op) : xa_stride(op), xout = INV ? xa_stride(op) : xb_stride(op);
 	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
 		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
@@ -864,7 +912,7 @@ template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
-	auto k = c.qy.rmax <= 8 ? k_p3<T, 8> : k_p3<T, 16>;
+	auto k = pick_p3<T>(c.qy);
 	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
 		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed,
@@ -877,7 +925,7 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	cx<T> *res, const cx<T> *ccs, int her, double *partials)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = 3 * op->bZ * nyt;
-	auto k = c.qx.rmax <= 8 ? k_p5<T, 8> : k_p5<T, 16>;
+	auto k = pick_p5<T>(c.qx);
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, c.LY, her, partials, make_bq(op, K, vs, xa_stride(op), 0, gate));

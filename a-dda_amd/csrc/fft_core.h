@@ -169,6 +169,114 @@ __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const Fft
 	stage_dispatch<T, INV, false, true, RM>(p.rad[p.nst - 1], buf, tws, N, Ns, KS, LS, lineB, tB, TPL, activeB, ld, st);
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Compile-time plans. The generic path above takes the length and the radix schedule at run time, so every index of
+// a stage (j % Ns, j / Ns, twiddle indices, padded LDS positions, line offsets) costs integer divisions and
+// multiplies; in the FFT-y pass that integer work outnumbers the butterflies. Fix<R0, R1, ...> fixes N and the
+// schedule at compile time: the same Stockham stages with all of that folded into shifts and immediate offsets.
+// Kernels are templated on a plan policy: Dyn<RM> (run-time plan, RM = bound on the radix) or Fix<...>.
+
+template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, class Ld, class St>
+__device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int LS, int line, int t, bool active,
+	Ld &ld, St &st)
+{
+	constexpr int NB = N / R;                    // butterflies per line
+	constexpr int MU = (RMAX + R - 1) / R;        // butterflies per thread (TPL = N / RMAX threads per line)
+	constexpr int TW = N / (NS * R);              // twiddle stride of this stage
+	cx<T> v[MU][R];
+#pragma unroll
+	for (int u = 0; u < MU; u++) {
+		const int j = t + u * TPL;
+		if (active && (NB % TPL == 0 || j < NB)) {
+#pragma unroll
+			for (int q = 0; q < R; q++) v[u][q] = FIRST ? ld(line, j + q * NB) : buf[lpos(j + q * NB, KS) + line * LS];
+			if (!FIRST) {
+				const int k = j % NS;
+#pragma unroll
+				for (int q = 1; q < R; q++) {
+					const cx<T> w = tws[q * k * TW];
+					v[u][q] = mul(v[u][q], INV ? conjc(w) : w);
+				}
+			}
+			dftR2<T, INV, R>(v[u]);
+		}
+	}
+	__syncthreads();  // a FIRST stage's Load may read the LDS buffer it is about to overwrite
+	if (LAST) {
+#pragma unroll
+		for (int u = 0; u < MU; u++) {
+			const int j = t + u * TPL;
+			if (active && (NB % TPL == 0 || j < NB)) {
+#pragma unroll
+				for (int q = 0; q < R; q++) st(line, j + q * NS, v[u][q]);
+			}
+		}
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < MU; u++) {
+			const int j = t + u * TPL;
+			if (active && (NB % TPL == 0 || j < NB)) {
+				const int d = (j / NS) * NS * R + j % NS;
+#pragma unroll
+				for (int q = 0; q < R; q++) buf[lpos(d + q * NS, KS) + line * LS] = v[u][q];
+			}
+		}
+		__syncthreads();
+	}
+}
+
+template <int A, int... B> constexpr int cmax()
+{
+	if constexpr (sizeof...(B) == 0) return A;
+	else return A > cmax<B...>() ? A : cmax<B...>();
+}
+
+template <int RM> struct Dyn {
+	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
+	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
+	template <typename T, bool INV, class Ld, class St>
+	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS,
+		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
+	{
+		fft_line<T, INV, RM>(buf, tws, p, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+	}
+};
+
+template <int... Rs> struct Fix {
+	static constexpr int NST = sizeof...(Rs);
+	static constexpr int RAD[NST] = {Rs...};
+	static constexpr int N = (Rs * ...);
+	static constexpr int RMAX = cmax<Rs...>();
+	static constexpr int TPL = N / RMAX;
+	static bool match(const FftPlan2 &p)
+	{
+		if (p.n != N || p.nst != NST) return false;
+		for (int i = 0; i < NST; i++)
+			if (p.rad[i] != RAD[i]) return false;
+		return true;
+	}
+	__device__ static __forceinline__ constexpr int n(const FftPlan2 &) { return N; }
+	__device__ static __forceinline__ constexpr int tpl(const FftPlan2 &) { return TPL; }
+	template <typename T, bool INV, int S, int NS, class Ld, class St>
+	__device__ static __forceinline__ void run(cx<T> *buf, const cx<T> *tws, int KS, int LS, int lineA, int tA,
+		bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
+	{
+		constexpr int R = RAD[S];
+		constexpr bool FIRST = S == 0, LAST = S == NST - 1;
+		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX>(buf, tws, KS, LS, FIRST ? lineA : lineB, FIRST ? tA : tB,
+			FIRST ? activeA : activeB, ld, st);
+		if constexpr (!LAST)
+			run<T, INV, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+	}
+	template <typename T, bool INV, class Ld, class St>
+	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &, int KS, int LS,
+		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
+	{
+		run<T, INV, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+	}
+};
+
 // Host-side plan: radices 5 and 3, then the power of two as radix-16 stages with one radix-8/4/2 remainder
 // (2^(4a+1) uses 8*4 instead of 16*2). Stages run largest radix first.
 inline bool make_plan2(int n, FftPlan2 &p)

@@ -3,17 +3,18 @@
 //
 // Matvec pipeline (5 HBM passes, half-planes that are identically zero are never stored or read):
 //   P1 k_p1           : gather S.x from dipoles onto x-lines (y<boxY, z<boxZ), zero-pad to gridX, forward FFT-x,
-//                       store transposed -> Xa[z<boxZ][x<gridX][c][y<boxY]
+//                       store transposed -> Xa[z<boxZ][x<gridX][y<boxY][c]
 //   P2 k_zfft<fwd>    : forward FFT-z of the boxZ nonzero rows of the gridX*3*boxY contiguous columns
-//                       -> Xb[z<gridZ][x][c][y<boxY]
+//                       -> Xb[z<gridZ][x][y<boxY][c]
 //   P3 k_p3           : forward FFT-y of contiguous boxY runs (zero-padded to gridY), product with the 6-component
 //                       symmetric D-matrix, inverse FFT-y, keep y<boxY (in place in Xb)
 //   P4 k_zfft<inv>    : inverse FFT-z, keep z<boxZ -> Xa
 //   P5 k_p5           : inverse FFT-x, result_i = arg_i + S_i.X(pos_i), per-block partial ||result||^2
 // P2-P4 read and write only contiguous runs; the transposes of the reference (TransposeYZ, src/fft.c:227-312) are
-// folded into the P1 store and the P5 load (runs of LY lines).
+// folded into the P1 store and the P5 load (runs of 3*LY elements: y and the component).
 #include "fft_core.h"
 #include "krylov_dev.h"
+#include "mv_kernels.h"
 #include "../../include/adda_hip.h"
 
 #include <hip/hip_runtime.h>
@@ -116,7 +117,8 @@ static void twiddles(int n, std::vector<double> &out)
 // Radix schedules and tile sizes of the five matvec passes for one precision.
 struct MvCfg {
 	FftPlan2 qx, qy, qz;  // radix schedules along x (P1/P5), y (P3), z (P2/P4)
-	int LY;               // lines (y values) per block in P1/P5
+	MvKernels kx, ky, kz; // the kernels that run those schedules (resolve())
+	int LY;               // y rows per block in P1/P5 (3*LY lines: all components)
 	int TC;               // columns per block in P2/P4
 	int LP;               // x-pairs per block in P3
 };
@@ -165,40 +167,10 @@ struct adda_hip_op {
 	double *red_host = nullptr;  // [4*kcap] pinned
 };
 
-// Per-RHS layout of a batched launch: K right-hand sides share the operator (D-matrix, geometry, plans); the RHS
-// index is part of the block index. A single solve is K = 1.
-struct Bq {
-	int K;               // right-hand sides in the launch
-	long vs;             // stride of the dipole vectors (arg, result) between RHS, complex elements
-	long xs, xs2;        // strides of the work buffers the pass reads / writes
-	int cs;              // stride of the cc_sqrt tables (complex elements)
-	const double *gate;  // RHS k is skipped when gate[k*DK_NSLOTS + DK_STATUS] != 0 (null: never)
-};
-__device__ __forceinline__ bool skip_rhs(const Bq &b, int k)
-{
-	return b.gate && b.gate[(size_t)k * DK_NSLOTS + DK_STATUS] != 0;
-}
-
 static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
 
 // ==================================================================================================================
 // kernels: matvec pipeline
-
-// Deterministic block reduction for any block size up to 512 (not necessarily a multiple of the wave size):
-// a fixed-order tree over LDS. Every thread of the block must call it; the total is returned to thread 0.
-template <typename T>
-__device__ __forceinline__ double block_sum(double s, double *red_lds)
-{
-	const int tid = threadIdx.x, n = blockDim.x;
-	__syncthreads();
-	red_lds[tid] = s;
-	__syncthreads();
-	for (int stride = 256; stride > 0; stride >>= 1) {
-		if (tid < stride && tid + stride < n) red_lds[tid] += red_lds[tid + stride];
-		__syncthreads();
-	}
-	return tid == 0 ? red_lds[0] : 0.0;
-}
 
 // Batched FFT along a strided axis. Block (blockIdx.x, blockIdx.y) transforms columns x0..x0+TX-1 of batch b:
 // element k of column x is in[off_in + k*rs_in + x] (k < n_in, zero above), out[off_out + k*rs_out + x] for k < n_out.
@@ -245,207 +217,6 @@ __global__ void __launch_bounds__(256) k_rowfft(cx<T> *data, FftPlan plan, const
 	for (int idx = threadIdx.x; idx < LL * N; idx += blockDim.x) {
 		const long line = line0 + idx / N;
 		if (line < nlines) data[line * N + idx % N] = buf[idx];
-	}
-}
-
-// ==================================================================================================================
-// kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][c][y<boxY], Xb[z<gridZ][x<gridX][c][y<boxY],
-// D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
-
-template <typename T>
-__device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
-{
-	for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = twg[i];
-	__syncthreads();
-}
-
-// P1: block = (z, y0..y0+LY, RHS, c) with the component fastest in the block index, then the RHS (the three blocks
-// of a tile read the same dipoles back to back, and the RHS of a tile share its cell-map and material reads). The
-// gather of S.x along x (x fastest) fills the LDS lines; the last FFT stage stores Xa[z][x][c][y] with y fastest
-// across threads (runs of LY).
-template <typename T, class PL>
-__global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
-	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
-{
-	const int c = blockIdx.x % 3, rest = blockIdx.x / 3, kq = rest % bq.K, tile = rest / bq.K;
-	if (skip_rhs(bq, kq)) return;
-	arg += kq * bq.vs;
-	Xa += kq * bq.xs2;
-	ccs += (size_t)kq * bq.cs;
-	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N);
-	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
-	const int nyt = (bY + LY - 1) / LY;
-	const int z = tile / nyt, y0 = (tile % nyt) * LY;
-	const int tid = threadIdx.x;
-	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest
-	const int lineB = tid % LY, tB = tid / LY;    // y fastest
-	const bool actA = lineA < LY && y0 + lineA < bY, actB = tB < TPL && y0 + lineB < bY;
-	for (int idx = tid; idx < LY * N; idx += blockDim.x) {
-		const int line = idx / N, x = idx % N;
-		cx<T> v = mk<T>(0, 0);
-		if (x < bX && y0 + line < bY) {
-			const int d = cellmap[((size_t)z * bY + y0 + line) * bX + x];
-			if (d >= 0) {
-				cx<T> a = arg[3 * (size_t)d + c];
-				if (her) a = conjc(a);
-				v = mul(ccs[3 * mat[d] + c], a);
-			}
-		}
-		buf[line * LS + lpos(x, 1)] = v;
-	}
-	__syncthreads();
-	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
-	auto st = [&](int line, int x, cx<T> v) { Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line] = v; };
-	PL::template fft<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
-}
-
-// P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
-template <typename T, bool INV, class PL>
-__global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
-	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, Bq bq)
-{
-	const int kq = blockIdx.x % bq.K;
-	if (skip_rhs(bq, kq)) return;
-	in += kq * bq.xs;
-	out += kq * bq.xs2;
-	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p);
-	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
-	const long q0 = (long)(blockIdx.x / bq.K) * TC;
-	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
-	const bool act = t < TPL && q0 + line < Q;
-	auto ld = [&](int l, int k) -> cx<T> { return k < n_in ? in[(size_t)k * Q + q0 + l] : mk<T>(0, 0); };
-	auto st = [&](int l, int k, cx<T> v) {
-		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
-	};
-	PL::template fft<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
-}
-
-// P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
-// which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
-// HBM once per matvec. The RHS index is fastest in the block index: the K blocks that share a D tile run together
-// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
-// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
-// cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
-template <typename T, class PL>
-__global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
-	Bq bq)
-{
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
-	if (skip_rhs(bq, kq)) return;
-	Xb += kq * bq.xs;
-	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP, LS = rowstride(N);
-	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
-	const int nxt = gX / LP;
-	const int z = tile / nxt, x0 = (tile % nxt) * LP;
-	const bool has2 = reduced && z != 0 && 2 * z != gZ;  // mirror plane handled by this block
-	const int zp = gZ - z;
-	const int nls = has2 ? 2 * NL1 : NL1;
-	cx<T> *base0 = Xb + ((size_t)z * gX + x0) * 3 * bY;
-	cx<T> *base1 = Xb + ((size_t)(has2 ? zp : z) * gX + x0) * 3 * bY;
-	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
-	const bool act = line < nls;
-	auto gptr = [&](int l) -> cx<T> * { return l < NL1 ? base0 + (size_t)l * bY : base1 + (size_t)(l - NL1) * bY; };
-	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? gptr(l)[y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	PL::template fft<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
-	__syncthreads();
-	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
-		const int pr = idx / N, y = idx % N, x = x0 + pr;
-		int xi = x, yi = y, zi = z;
-		if (transposed) {
-			xi = xi > 0 ? gX - xi : 0;
-			yi = yi > 0 ? N - yi : 0;
-			zi = zi > 0 ? gZ - zi : 0;
-		}
-		else {
-			if (yi >= DsY) yi = N - yi;
-			if (zi >= DsZ) zi = gZ - zi;
-		}
-		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
-		const cx<T> d0 = dm[0], d1 = dm[1], d2 = dm[2], d3 = dm[3], d4 = dm[4], d5 = dm[5];
-		for (int sidx = 0; sidx < (has2 ? 2 : 1); sidx++) {
-			const int zz = sidx ? zp : z;
-			cx<T> f1 = d1, f2 = d2, f4 = d4;
-			if (reduced) {
-				if (y >= DsY) {
-					f1 = scl(f1, (T)-1);
-					if (zz >= DsZ) f2 = scl(f2, (T)-1);
-					else f4 = scl(f4, (T)-1);
-				}
-				else if (zz >= DsZ) {
-					f2 = scl(f2, (T)-1);
-					f4 = scl(f4, (T)-1);
-				}
-			}
-			cx<T> *q = buf + (size_t)(sidx * NL1 + 3 * pr) * LS + lpos(y, 1);
-			const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
-			q[0] = add(add(mul(d0, a0), mul(f1, a1)), mul(f2, a2));
-			q[LS] = add(add(mul(f1, a0), mul(d3, a1)), mul(f4, a2));
-			q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(d5, a2));
-		}
-	}
-	__syncthreads();
-	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
-	auto stg = [&](int l, int y, cx<T> v) {
-		if (y < bY) gptr(l)[y] = v;
-	};
-	PL::template fft<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
-}
-
-// P5: block = (z, y0..y0+LY, RHS, c), component fastest in the block index, then the RHS; stage 0 loads
-// Xa[z][x][c][y] y-fastest (runs of LY), the results go to LDS and are gathered to the dipoles along x:
-// result = arg + S.X (her: conj), partial ||result||^2 per block at partials[k*(blocks per RHS) + tile*3 + c].
-template <typename T, class PL>
-__global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
-	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
-	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
-	double *__restrict__ partials, Bq bq)
-{
-	const int c = blockIdx.x % 3, rest = blockIdx.x / 3, kq = rest % bq.K, tile = rest / bq.K;
-	if (skip_rhs(bq, kq)) return;
-	Xa += kq * bq.xs;
-	arg += kq * bq.vs;
-	res += kq * bq.vs;
-	ccs += (size_t)kq * bq.cs;
-	extern __shared__ char smem_raw[];
-	__shared__ double red_lds[512];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N);
-	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
-	const int nyt = (bY + LY - 1) / LY;
-	const int z = tile / nyt, y0 = (tile % nyt) * LY;
-	const int tid = threadIdx.x;
-	const int lineA = tid % LY, tA = tid / LY;    // y fastest
-	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest
-	const bool actA = tA < TPL && y0 + lineA < bY, actB = lineB < LY && y0 + lineB < bY;
-	double s = 0;
-	auto ld = [&](int line, int x) -> cx<T> { return Xa[(((size_t)z * N + x) * 3 + c) * bY + y0 + line]; };
-	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
-	PL::template fft<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
-	__syncthreads();
-	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
-		const int line = idx / bX, x = idx % bX;
-		if (y0 + line >= bY) continue;
-		const int d = cellmap[((size_t)z * bY + y0 + line) * bX + x];
-		if (d < 0) continue;
-		cx<T> a = arg[3 * (size_t)d + c];
-		if (her) a = conjc(a);
-		cx<T> r = add(a, mul(ccs[3 * mat[d] + c], buf[line * LS + lpos(x, 1)]));
-		if (her) r = conjc(r);
-		res[3 * (size_t)d + c] = r;
-		s += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-	}
-	if (partials) {
-		double tt = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[(size_t)kq * (gridDim.x / bq.K) + tile * 3 + c] = tt;
 	}
 }
 
@@ -780,78 +551,36 @@ static int reduce_rows(adda_hip_op *op, const double *partials, int nblk, int nk
 	return 0;
 }
 
-// Compile-time plans built into this library (any other schedule runs on the run-time plan kernels).
-template <class... Ps> struct PlanList {};
-using FixPlans = PlanList<Fix<16, 16>, Fix<8, 8, 4>, Fix<4, 8, 8>, Fix<16, 4, 4>>;
-
-// Calls f(PL{}) for the first compiled plan equal to p; false if none.
-template <class F, class... Ps> static bool with_fix(const FftPlan2 &p, F &&f, PlanList<Ps...>)
+// Kernels of the five passes for plan q: the compile-time instantiation when q is one of the compiled plans (the
+// mv_*.hip groups), else the run-time-plan one.
+static MvKernels mv_kernels(const FftPlan2 &q)
 {
-	return (... || (Ps::match(p) ? (f(Ps{}), true) : false));
+	MvKernels m;
+	if (!mv_find_pow2_large(q, m) && !mv_find_pow2_small(q, m) && !mv_find_mixed(q, m)) mv_find_dyn(q, m);
+	return m;
 }
 
-template <typename T, class PL>
-static int set_lds_limits_pl()
-{
-	const int big = 96 * 1024;  // largest dynamic LDS tile used
-	HIPCHK(hipFuncSetAttribute((const void *)k_p1<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_p5<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big - 4096));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	return 0;
-}
-
-template <typename T, class... Ps>
-static int set_lds_limits_all(PlanList<Ps...>)
-{
-	int rc = 0;
-	((rc = rc ? rc : set_lds_limits_pl<T, Ps>()), ...);
-	return rc;
-}
-
-template <typename T>
 static int set_lds_limits()
 {
-	const int big = 96 * 1024;
-	if (set_lds_limits_pl<T, Dyn<16>>() || set_lds_limits_pl<T, Dyn<8>>() || set_lds_limits_all<T>(FixPlans{}))
-		return -1;
-	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
-	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	hipError_t e;
+	if ((e = mv_lds_dyn()) || (e = mv_lds_pow2_large()) || (e = mv_lds_pow2_small()) || (e = mv_lds_mixed()))
+		return ERR("hipFuncSetAttribute (matvec kernels): %s", hipGetErrorString(e));
+	const int big = kMvLdsMax;
+	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_colfft<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
+	HIPCHK(hipFuncSetAttribute((const void *)k_rowfft<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big));
 	return 0;
 }
 
-// Kernel of a pass for plan q: the compile-time instantiation when q is one of FixPlans, else the run-time one.
-template <typename T> static auto pick_p1(const FftPlan2 &q)
-{
-	auto f = q.rmax <= 8 ? &k_p1<T, Dyn<8>> : &k_p1<T, Dyn<16>>;
-	with_fix(q, [&](auto pl) { f = &k_p1<T, decltype(pl)>; }, FixPlans{});
-	return f;
-}
-template <typename T> static auto pick_p3(const FftPlan2 &q)
-{
-	auto f = q.rmax <= 8 ? &k_p3<T, Dyn<8>> : &k_p3<T, Dyn<16>>;
-	with_fix(q, [&](auto pl) { f = &k_p3<T, decltype(pl)>; }, FixPlans{});
-	return f;
-}
-template <typename T> static auto pick_p5(const FftPlan2 &q)
-{
-	auto f = q.rmax <= 8 ? &k_p5<T, Dyn<8>> : &k_p5<T, Dyn<16>>;
-	with_fix(q, [&](auto pl) { f = &k_p5<T, decltype(pl)>; }, FixPlans{});
-	return f;
-}
-template <typename T, bool INV> static auto pick_zfft(const FftPlan2 &q)
-{
-	auto f = q.rmax <= 8 ? &k_zfft<T, INV, Dyn<8>> : &k_zfft<T, INV, Dyn<16>>;
-	with_fix(q, [&](auto pl) { f = &k_zfft<T, INV, decltype(pl)>; }, FixPlans{});
-	return f;
-}
+template <typename T> using P1Fn = decltype(&k_p1<T, Dyn<8>>);
+template <typename T> using ZFn = decltype(&k_zfft<T, false, Dyn<8>>);
+template <typename T> using P3Fn = decltype(&k_p3<T, Dyn<8>>);
+template <typename T> using P5Fn = decltype(&k_p5<T, Dyn<8>>);
 
 // ==================================================================================================================
 // matvec driver
 
-static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + c.LY * rowstride(gX)) * es; }
+static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + 3 * c.LY * rowstride(gX)) * es; }
 static size_t lds_p24(const MvCfg &c, int gZ, size_t es) { return (size_t)(gZ + gZ * c.TC) * es; }
 static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
@@ -864,7 +593,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 {
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
 	const int ml = op->reduced ? 2 : 1;
-	return c.LY >= 1 && c.TC >= 1 && c.LP >= 1 && op->gX % c.LP == 0 && c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
+	return c.LY >= 1 && c.TC >= 1 && c.LP >= 1 && op->gX % c.LP == 0 && 3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
 		3 * ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax && lds_p24(c, op->gZ, es) <= kLdsMax &&
 		lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
 }
@@ -889,8 +618,8 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 	const cx<T> *ccs, int her)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
-	auto k = pick_p1<T>(c.qx);
-	hipLaunchKernelGGL(k, dim3(3 * op->bZ * nyt * K), dim3(c.LY * (op->gX / c.qx.rmax)),
+	auto k = (P1Fn<T>)c.kx.k[MV_P1][sizeof(T) == 4];
+	hipLaunchKernelGGL(k, dim3(op->bZ * nyt * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
 		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx,
 		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
 }
@@ -901,7 +630,7 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 	const long Q = (long)op->gX * 3 * op->bY;
 	const cx<T> *in = (const cx<T> *)(INV ? op->Xb : op->Xa);
 	cx<T> *out = (cx<T> *)(INV ? op->Xa : op->Xb);
-	auto k = pick_zfft<T, INV>(c.qz);
+	auto k = (ZFn<T>)c.kz.k[INV ? MV_ZINV : MV_ZFWD][sizeof(T) == 4];
 	const long xin = INV ? xb_stride(op) : xa_stride(op), xout = INV ? xa_stride(op) : xb_stride(op);
 	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
 		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
@@ -912,7 +641,7 @@ template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
-	auto k = pick_p3<T>(c.qy);
+	auto k = (P3Fn<T>)c.ky.k[MV_P3][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
 		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed,
@@ -924,9 +653,9 @@ template <typename T>
 static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
 	cx<T> *res, const cx<T> *ccs, int her, double *partials)
 {
-	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = 3 * op->bZ * nyt;
-	auto k = pick_p5<T>(c.qx);
-	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
+	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = op->bZ * nyt;
+	auto k = (P5Fn<T>)c.kx.k[MV_P5][sizeof(T) == 4];
+	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, c.LY, her, partials, make_bq(op, K, vs, xa_stride(op), 0, gate));
 	return nblk5;
@@ -970,6 +699,13 @@ static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const c
 	return 0;
 }
 
+static void resolve(MvCfg &c)
+{
+	c.kx = mv_kernels(c.qx);
+	c.ky = mv_kernels(c.qy);
+	c.kz = mv_kernels(c.qz);
+}
+
 // Default tiles for element size es: the largest (LY <= 8, TC <= 16, LP <= 4 dividing gridX) fitting 512 threads and
 // 64 KB of LDS, with the default radix schedules of make_plan2.
 static void default_cfg(adda_hip_op *op, int pi, size_t es)
@@ -981,13 +717,14 @@ static void default_cfg(adda_hip_op *op, int pi, size_t es)
 	const size_t cap = 64 * 1024;
 	const int gX = op->gX, gY = op->gY, gZ = op->gZ;
 	const int tplx = gX / c.qx.rmax, tply = gY / c.qy.rmax, tplz = gZ / c.qz.rmax;
-	c.LY = es == 8 ? 16 : 8;  // runs of 128 bytes in the transposed Xa accesses
-	while (c.LY > 1 && (c.LY * tplx > 512 || lds_p15(c, gX, es) > cap)) c.LY /= 2;
+	c.LY = es == 8 ? 8 : 4;  // runs of 3*LY elements (192 bytes) in the transposed Xa accesses
+	while (c.LY > 1 && (3 * c.LY * tplx > 512 || lds_p15(c, gX, es) > cap)) c.LY /= 2;
 	c.TC = 16;
 	while (c.TC > 1 && (c.TC * tplz > 512 || lds_p24(c, gZ, es) > cap)) c.TC /= 2;
 	const int ml = op->reduced ? 2 : 1;
 	c.LP = 4;
 	while (c.LP > 1 && (3 * ml * c.LP * tply > 512 || lds_p3(c, gY, op->reduced, es) > cap || gX % c.LP)) c.LP--;
+	resolve(c);
 }
 
 // ==================================================================================================================
@@ -1517,7 +1254,7 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 {
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
-	if (set_lds_limits<double>() || set_lds_limits<float>()) return -1;
+	if (set_lds_limits()) return -1;
 	const int bX = op->bX, bY = op->bY, bZ = op->bZ, gX = op->gX, gY = op->gY, gZ = op->gZ;
 	if (!make_plan(gX, op->px) || !make_plan(gY, op->py) || !make_plan(gZ, op->pz))
 		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
@@ -2265,6 +2002,7 @@ extern "C" int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg)
 	MvCfg c;
 	if (!cfg_parse(op, cfg, c)) return ERR("cannot parse matvec configuration '%s'", cfg);
 	if (!cfg_ok(op, c, prec == ADDA_F32 ? 8 : 16)) return ERR("configuration '%s' does not fit this grid", cfg);
+	resolve(c);
 	op->cfg[prec == ADDA_F32] = c;
 	return 0;
 }
@@ -2364,6 +2102,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			c.qx = q;
 			c.LY = LY;
 			if (!cfg_ok(op, c, es)) continue;
+			resolve(c);
 			ncand++;
 			float t = timeit([&] {
 				launch_p1<T>(op, c, K, vs, nullptr, a, ccs, 0);
@@ -2378,6 +2117,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			c.qz = q;
 			c.TC = TC;
 			if (!cfg_ok(op, c, es)) continue;
+			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_zfft<T, false>(op, c, K, nullptr); launch_zfft<T, true>(op, c, K, nullptr); });
 			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
@@ -2389,6 +2129,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			c.qy = q;
 			c.LP = LP;
 			if (!cfg_ok(op, c, es)) continue;
+			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
 			if (t < ty) { ty = t; best.qy = q; best.LP = LP; }
@@ -2400,6 +2141,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 	HIPCHK(hipFree(r));
 	HIPCHK(hipGetLastError());
 	if (tx > 1e29f || ty > 1e29f || tz > 1e29f) return ERR("autotune found no launchable configuration");
+	resolve(best);
 	op->cfg[pi] = best;
 	if (report) {
 		char cs[256];

@@ -563,7 +563,8 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	kres *kr = malloc(K * sizeof(kres));
 	adda_cplx *ca = malloc(K * sizeof(adda_cplx)), *cb = malloc(K * sizeof(adda_cplx)), *cz = calloc(K, sizeof(adda_cplx));
 	int *active = malloc(K * sizeof(int)), *hist_pos = calloc(K, sizeof(int));
-	void *r64 = NULL, *Ax = NULL, *r32 = NULL, *e32 = NULL, *e64 = NULL, *og = NULL;
+	void *r64 = NULL, *Ax = NULL, *r32 = NULL, *e32 = NULL, *e64 = NULL, *xt = NULL, *og = NULL;
+	int *fin = NULL, *acc = NULL;
 #define CKB(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto out; } } while (0)
 	if (!eps || !b2 || !r2 || !r2new || !kr || !ca || !cb || !cz || !active || !hist_pos) {
 		rc = host_err(__FILE__, __LINE__, "out of host memory");
@@ -598,9 +599,13 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	}
 	else {
 		/* FP64 defect correction around FP32 Krylov solves, per RHS:
-		 *   x = 0; r = b; repeat { e ~ A^-1 r (FP32, relative tol inner_eps); x += e; r = b - A.x (FP64) }
-		 * until ||r||/||b|| <= eps. Inner solves start from e = 0 (init_field zero) on the normalized residual. All
-		 * RHS still correcting run their inner solves together; the others are gated off (status non-zero in og). */
+		 *   x = 0; r = b; repeat { e ~ A^-1 (r/|r|) (FP32, relative tol inner_eps); x' = x + |r| e;
+		 *                          r' = b - A.x' (FP64); accept x', r' if |r'| < |r| }
+		 * until ||r||/||b|| <= eps. Inner solves start from e = 0 on the normalized residual. All RHS still
+		 * correcting run their inner solves together; the others are gated off (status non-zero in og). A correction
+		 * that does not reduce the FP64 residual is rejected, and that RHS (like one still above eps after max_outer
+		 * steps) finishes with an FP64 Krylov solve of the same correction equation -- the FP64 residual decides
+		 * every outcome, so no RHS ends on an unchecked FP32 result. */
 		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-5;
 		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
 		const int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
@@ -608,13 +613,21 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		r64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
 		Ax = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
 		e64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
+		xt = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
 		r32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
 		e32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
 		og = adda_hip_dk_create_batch(op, K, 2);
 		og_h = calloc((size_t)K * NS, sizeof(double));
+		fin = calloc(K, sizeof(int));
+		acc = calloc(K, sizeof(int));
 		if (resid_hist && hist_len > 0) ih = malloc((size_t)K * ihl * sizeof(double));
-		if (!r64 || !Ax || !e64 || !r32 || !e32 || !og) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
-		if (!og_h || (resid_hist && hist_len > 0 && !ih)) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto out; }
+		if (!r64 || !Ax || !e64 || !xt || !r32 || !e32 || !og) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
+		if (!og_h || !fin || !acc || (resid_hist && hist_len > 0 && !ih)) {
+			rc = host_err(__FILE__, __LINE__, "out of host memory");
+			goto out;
+		}
+#define GATE(cond) do { for (int k_ = 0; k_ < K; k_++) og_h[(size_t)k_ * NS + ADDA_DK_STATUS] = (cond) ? 0 : 1; \
+		CKB(adda_hip_dk_set(op, og, 0, og_h, K * NS)); } while (0)
 		for (int k = 0; k < K; k++) { ca[k].re = 1; ca[k].im = 0; cb[k].re = -1; cb[k].im = 0; }
 		CKB(adda_hip_dot_batch(op, ADDA_F64, K, NULL, d_pvec, NULL, ADDA_DOT_NORM2, NULL, b2));
 		for (int k = 0; k < K; k++)
@@ -628,10 +641,23 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 			active[k] = sqrt(r2[k] / b2[k]) > opt->eps;
 			nact += active[k];
 		}
+		/* x' = x + |r| e, r' = b - A.x' for the RHS where `sel`; r2new receives |r'|^2; e64 ends up holding r' */
+#define CANDIDATE(sel) do { \
+		GATE(sel); \
+		for (int k_ = 0; k_ < K; k_++) { ca[k_].re = 1; cb[k_].re = sqrt(r2[k_]); } \
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, xt, d_x, ca, e64, cb, NULL, cz, 0, NULL)); \
+		CKB(adda_hip_matvec_batch(op, ADDA_F64, K, og, xt, Ax, NULL, 0)); \
+		for (int k_ = 0; k_ < K; k_++) cb[k_].re = -1; \
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, e64, d_pvec, ca, Ax, cb, NULL, cz, 0, r2new)); } while (0)
+		/* x = x', r = r' for the RHS where acc */
+#define ACCEPT() do { \
+		GATE(acc[k_]); \
+		for (int k_ = 0; k_ < K; k_++) ca[k_].re = 1; \
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, d_x, xt, ca, NULL, cz, NULL, cz, 0, NULL)); \
+		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, r64, e64, ca, NULL, cz, NULL, cz, 0, NULL)); } while (0)
 		int nouter = 0;
 		while (nact > 0 && nouter < max_outer) {
-			for (int k = 0; k < K; k++) og_h[(size_t)k * NS + ADDA_DK_STATUS] = active[k] ? 0 : 1;
-			CKB(adda_hip_dk_set(op, og, 0, og_h, K * NS));
+			GATE(active[k_]);
 			/* normalized residual in FP32; the relative target of each step: reach eps overall but no deeper than
 			 * FP32 can resolve */
 			for (int k = 0; k < K; k++) {
@@ -642,14 +668,14 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 			}
 			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, Ax, r64, ca, NULL, cz, NULL, cz, 0, NULL));
 			CKB(adda_hip_vec_convert_batch(op, K, r32, ADDA_F32, Ax, ADDA_F64));
-			int irc = krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, 0, active, r32, e32, ih, ih ? ihl : 0, kr);
-			(void)irc; /* per-RHS codes in kr: an FP32 breakdown or stagnation ends that RHS's step, the FP64 residual
-			            * below decides what happens next */
+			/* per-RHS codes in kr: an FP32 breakdown or stagnation ends that RHS's step; the FP64 residual below
+			 * decides what happens next */
+			krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, 0, active, r32, e32, ih, ih ? ihl : 0, kr);
 			for (int k = 0; k < K; k++) {
 				if (!active[k]) continue;
 				if (kr[k].rc < 0 && kr[k].niter == 0) { /* the inner solver could not even start */
-					st[k].rc = kr[k].rc;
 					active[k] = 0;
+					fin[k] = 1;
 					continue;
 				}
 				if (ih) { /* inner residuals relative to this step's rhs -> relative to b */
@@ -659,34 +685,67 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 				}
 				st[k].niter += kr[k].niter;
 				st[k].nmatvec += kr[k].nmv;
-				ca[k].re = 1;
-				cb[k].re = sqrt(r2[k]);
 			}
-			for (int k = 0; k < K; k++) og_h[(size_t)k * NS + ADDA_DK_STATUS] = active[k] ? 0 : 1;
-			CKB(adda_hip_dk_set(op, og, 0, og_h, K * NS));
-			/* x += rn * e (FP64 accumulation) */
 			CKB(adda_hip_vec_convert_batch(op, K, e64, ADDA_F64, e32, ADDA_F32));
-			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, d_x, d_x, ca, e64, cb, NULL, cz, 0, NULL));
-			/* true residual in FP64 */
-			CKB(adda_hip_matvec_batch(op, ADDA_F64, K, og, d_x, Ax, NULL, 0));
-			for (int k = 0; k < K; k++) { ca[k].re = 1; cb[k].re = -1; }
-			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, r64, d_pvec, ca, Ax, cb, NULL, cz, 0, r2new));
+			CANDIDATE(active[k_]);
 			nouter++;
 			nact = 0;
 			for (int k = 0; k < K; k++) {
+				acc[k] = 0;
 				if (!active[k]) continue;
 				st[k].nmatvec_fp64++;
 				st[k].nouter++;
 				if (opt->verbose && K == 1)
 					printf("outer %d: inner iterations %d, true residual %.6E\n", st[k].nouter, kr[k].niter,
 						sqrt(r2new[k] / b2[k]));
-				const int progress = r2new[k] < r2[k];
-				r2[k] = r2new[k];
-				/* no progress in FP64: give up with the best x so far */
-				active[k] = progress && sqrt(r2[k] / b2[k]) > opt->eps;
+				if (r2new[k] < r2[k]) {
+					acc[k] = 1;
+					r2[k] = r2new[k];
+					active[k] = sqrt(r2[k] / b2[k]) > opt->eps;
+				}
+				else { /* the FP32 correction made things worse: reject it, finish in FP64 */
+					active[k] = 0;
+					fin[k] = 1;
+				}
 				nact += active[k];
 			}
+			ACCEPT();
 		}
+		int nfin = 0;
+		for (int k = 0; k < K; k++) {
+			if (active[k]) fin[k] = 1; /* still above eps after max_outer steps */
+			nfin += fin[k];
+		}
+		if (nfin) { /* FP64 Krylov solve of A.e = r/|r| to the remaining relative target, then the same acceptance */
+			GATE(fin[k_]);
+			for (int k = 0; k < K; k++) {
+				ca[k].re = fin[k] ? 1 / sqrt(r2[k]) : 0;
+				eps[k] = opt->eps / sqrt(r2[k] / b2[k]);
+			}
+			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, Ax, r64, ca, NULL, cz, NULL, cz, 0, NULL));
+			krylov_run(op, opt, ADDA_F64, K, eps, opt->maxiter, ADDA_IF_ZERO, 0, fin, Ax, e64, NULL, 0, kr);
+			for (int k = 0; k < K; k++)
+				if (fin[k]) {
+					st[k].niter += kr[k].niter;
+					st[k].nmatvec_fp64 += kr[k].nmv;
+				}
+			CANDIDATE(fin[k_]);
+			for (int k = 0; k < K; k++) {
+				acc[k] = 0;
+				if (!fin[k]) continue;
+				st[k].nmatvec_fp64++;
+				st[k].nouter++;
+				if (r2new[k] < r2[k]) {
+					acc[k] = 1;
+					r2[k] = r2new[k];
+				}
+				else if (kr[k].rc < 0) st[k].rc = kr[k].rc;
+			}
+			ACCEPT();
+		}
+#undef GATE
+#undef CANDIDATE
+#undef ACCEPT
 		for (int k = 0; k < K; k++) {
 			st[k].resid = st[k].true_resid = sqrt(r2[k] / b2[k]);
 			if (st[k].resid > opt->eps && st[k].rc == 0) st[k].rc = 1;
@@ -701,13 +760,13 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	CKB(adda_hip_sync(op));
 out:
 	{
-		void *vs[] = {r64, Ax, r32, e32, e64};
-		for (int i = 0; i < 5; i++)
+		void *vs[] = {r64, Ax, r32, e32, e64, xt};
+		for (int i = 0; i < 6; i++)
 			if (vs[i]) adda_hip_vec_free(op, vs[i]);
 		if (og) adda_hip_dk_free(op, og);
 	}
 	free(eps); free(b2); free(r2); free(r2new); free(ih); free(og_h); free(kr); free(ca); free(cb); free(cz);
-	free(active); free(hist_pos);
+	free(active); free(hist_pos); free(fin); free(acc);
 	return rc;
 #undef CKB
 }

@@ -1,0 +1,21 @@
+// mv_dyn.hip -- matvec pass kernels with run-time FFT plans (any 2/3/5-smooth length; RM = bound on the radix).
+#include "mv_kernels.h"
+
+namespace adda {
+
+bool mv_find_dyn(const FftPlan2 &p, MvKernels &m)
+{
+	if (p.rmax <= 8) mv_fill<Dyn<8>>(m);
+	else mv_fill<Dyn<16>>(m);
+	return true;
+}
+
+hipError_t mv_lds_dyn()
+{
+	hipError_t e;
+	if ((e = mv_lds_t<double, Dyn<8>>()) || (e = mv_lds_t<double, Dyn<16>>()) || (e = mv_lds_t<float, Dyn<8>>()))
+		return e;
+	return mv_lds_t<float, Dyn<16>>();
+}
+
+}  // namespace adda

@@ -1,0 +1,330 @@
+// mv_kernels.h -- the five matvec pass kernels (P1 scatter + FFT-x, P2/P4 FFT-z, P3 FFT-y.D.FFT-y^-1, P5 inverse
+// FFT-x + gather), templated on precision and FFT plan policy (fft_core.h: Dyn<RM> run-time plan, Fix<...>
+// compile-time plan). Their instantiations live in several translation units (mv_*.hip) that compile in parallel;
+// each exposes the kernels of its plans through MvKernels.
+#pragma once
+#include "fft_core.h"
+#include "krylov_dev.h"
+
+namespace adda {
+
+// Per-RHS layout of a batched launch: K right-hand sides share the operator (D-matrix, geometry, plans); the RHS
+// index is part of the block index. A single solve is K = 1.
+struct Bq {
+	int K;               // right-hand sides in the launch
+	long vs;             // stride of the dipole vectors (arg, result) between RHS, complex elements
+	long xs, xs2;        // strides of the work buffers the pass reads / writes
+	int cs;              // stride of the cc_sqrt tables (complex elements)
+	const double *gate;  // RHS k is skipped when gate[k*DK_NSLOTS + DK_STATUS] != 0 (null: never)
+};
+static __device__ __forceinline__ bool skip_rhs(const Bq &b, int k)
+{
+	return b.gate && b.gate[(size_t)k * DK_NSLOTS + DK_STATUS] != 0;
+}
+
+// Deterministic block reduction for any block size up to 512 (not necessarily a multiple of the wave size):
+// a fixed-order tree over LDS. Every thread of the block must call it; the total is returned to thread 0.
+template <typename T>
+__device__ __forceinline__ double block_sum(double s, double *red_lds)
+{
+	const int tid = threadIdx.x, n = blockDim.x;
+	__syncthreads();
+	red_lds[tid] = s;
+	__syncthreads();
+	for (int stride = 256; stride > 0; stride >>= 1) {
+		if (tid < stride && tid + stride < n) red_lds[tid] += red_lds[tid + stride];
+		__syncthreads();
+	}
+	return tid == 0 ? red_lds[0] : 0.0;
+}
+
+// ==================================================================================================================
+// kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][y<boxY][c], Xb[z<gridZ][x<gridX][y<boxY][c],
+// D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
+
+template <typename T>
+__device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
+{
+	for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = twg[i];
+	__syncthreads();
+}
+
+// P1: block = (z, y0..y0+LY, RHS), all three components of LY y-rows; the RHS is fastest in the block index (the RHS
+// of a tile share its cell-map and material reads). Each thread gathers the three components of one dipole --
+// contiguous in the AoS vector -- into the LDS lines yl*3 + c (x along the line); the last FFT stage stores
+// Xa[z][x][y][c] with (y, c) fastest across threads, runs of 3*LY elements.
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
+	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
+{
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	arg += kq * bq.vs;
+	Xa += kq * bq.xs2;
+	ccs += (size_t)kq * bq.cs;
+	extern __shared__ char smem_raw[];
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N), NL = 3 * LY;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
+	const int ny = bY - y0 < LY ? bY - y0 : LY;  // rows of this tile
+	const int tid = threadIdx.x;
+	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest (reads the LDS lines)
+	const int lineB = tid % NL, tB = tid / NL;    // (y, c) fastest (writes Xa)
+	const bool actA = lineA < 3 * ny, actB = tB < TPL && lineB < 3 * ny;
+	for (int idx = tid; idx < LY * N; idx += blockDim.x) {
+		const int yl = idx / N, x = idx % N;
+		cx<T> v0 = mk<T>(0, 0), v1 = v0, v2 = v0;
+		if (x < bX && yl < ny) {
+			const int d = cellmap[((size_t)z * bY + y0 + yl) * bX + x];
+			if (d >= 0) {
+				const cx<T> *a = arg + 3 * (size_t)d, *cs = ccs + 3 * mat[d];
+				cx<T> a0 = a[0], a1 = a[1], a2 = a[2];
+				if (her) {
+					a0 = conjc(a0);
+					a1 = conjc(a1);
+					a2 = conjc(a2);
+				}
+				v0 = mul(cs[0], a0);
+				v1 = mul(cs[1], a1);
+				v2 = mul(cs[2], a2);
+			}
+		}
+		cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
+		q[0] = v0;
+		q[LS] = v1;
+		q[2 * LS] = v2;
+	}
+	__syncthreads();
+	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
+	auto st = [&](int line, int x, cx<T> v) { Xa[((size_t)z * N + x) * 3 * bY + 3 * y0 + line] = v; };
+	PL::template fft<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+}
+
+// P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
+template <typename T, bool INV, class PL>
+__global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
+	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, Bq bq)
+{
+	const int kq = blockIdx.x % bq.K;
+	if (skip_rhs(bq, kq)) return;
+	in += kq * bq.xs;
+	out += kq * bq.xs2;
+	extern __shared__ char smem_raw[];
+	const int N = PL::n(p), TPL = PL::tpl(p);
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const long q0 = (long)(blockIdx.x / bq.K) * TC;
+	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
+	const bool act = t < TPL && q0 + line < Q;
+	auto ld = [&](int l, int k) -> cx<T> { return k < n_in ? in[(size_t)k * Q + q0 + l] : mk<T>(0, 0); };
+	auto st = [&](int l, int k, cx<T> v) {
+		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
+	};
+	PL::template fft<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+}
+
+// P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
+// which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
+// HBM once per matvec. The RHS index is fastest in the block index: the K blocks that share a D tile run together
+// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
+// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
+// cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
+	Bq bq)
+{
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xb += kq * bq.xs;
+	extern __shared__ char smem_raw[];
+	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP, LS = rowstride(N);
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nxt = gX / LP;
+	const int z = tile / nxt, x0 = (tile % nxt) * LP;
+	const bool has2 = reduced && z != 0 && 2 * z != gZ;  // mirror plane handled by this block
+	const int zp = gZ - z;
+	const int nls = has2 ? 2 * NL1 : NL1;
+	cx<T> *base0 = Xb + ((size_t)z * gX + x0) * 3 * bY;
+	cx<T> *base1 = Xb + ((size_t)(has2 ? zp : z) * gX + x0) * 3 * bY;
+	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
+	const bool act = line < nls;
+	// line l = (plane, pair, c): element y of it at Xb[z][x0 + pair][y][c]
+	auto gptr = [&](int l) -> cx<T> * {
+		const int ls = l < NL1 ? l : l - NL1;
+		return (l < NL1 ? base0 : base1) + (size_t)(ls / 3) * 3 * bY + ls % 3;
+	};
+	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? gptr(l)[3 * y] : mk<T>(0, 0); };
+	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
+	PL::template fft<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	__syncthreads();
+	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
+		const int pr = idx / N, y = idx % N, x = x0 + pr;
+		int xi = x, yi = y, zi = z;
+		if (transposed) {
+			xi = xi > 0 ? gX - xi : 0;
+			yi = yi > 0 ? N - yi : 0;
+			zi = zi > 0 ? gZ - zi : 0;
+		}
+		else {
+			if (yi >= DsY) yi = N - yi;
+			if (zi >= DsZ) zi = gZ - zi;
+		}
+		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
+		const cx<T> d0 = dm[0], d1 = dm[1], d2 = dm[2], d3 = dm[3], d4 = dm[4], d5 = dm[5];
+		for (int sidx = 0; sidx < (has2 ? 2 : 1); sidx++) {
+			const int zz = sidx ? zp : z;
+			cx<T> f1 = d1, f2 = d2, f4 = d4;
+			if (reduced) {
+				if (y >= DsY) {
+					f1 = scl(f1, (T)-1);
+					if (zz >= DsZ) f2 = scl(f2, (T)-1);
+					else f4 = scl(f4, (T)-1);
+				}
+				else if (zz >= DsZ) {
+					f2 = scl(f2, (T)-1);
+					f4 = scl(f4, (T)-1);
+				}
+			}
+			cx<T> *q = buf + (size_t)(sidx * NL1 + 3 * pr) * LS + lpos(y, 1);
+			const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
+			q[0] = add(add(mul(d0, a0), mul(f1, a1)), mul(f2, a2));
+			q[LS] = add(add(mul(f1, a0), mul(d3, a1)), mul(f4, a2));
+			q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(d5, a2));
+		}
+	}
+	__syncthreads();
+	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
+	auto stg = [&](int l, int y, cx<T> v) {
+		if (y < bY) gptr(l)[3 * y] = v;
+	};
+	PL::template fft<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+}
+
+// P5: block = (z, y0..y0+LY, RHS), all three components; stage 0 loads Xa[z][x][y][c] with (y, c) fastest (runs of
+// 3*LY elements), the inverse FFT lands in the LDS lines yl*3 + c, and each thread then gathers one dipole:
+// result = arg + S.X (her: conj), three contiguous components; partial ||result||^2 per block at
+// partials[k*(blocks per RHS) + tile].
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
+	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
+	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
+	double *__restrict__ partials, Bq bq)
+{
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xa += kq * bq.xs;
+	arg += kq * bq.vs;
+	res += kq * bq.vs;
+	ccs += (size_t)kq * bq.cs;
+	extern __shared__ char smem_raw[];
+	__shared__ double red_lds[512];
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N), NL = 3 * LY;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	load_tw(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
+	const int ny = bY - y0 < LY ? bY - y0 : LY;
+	const int tid = threadIdx.x;
+	const int lineA = tid % NL, tA = tid / NL;    // (y, c) fastest (reads Xa)
+	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest (writes the LDS lines)
+	const bool actA = tA < TPL && lineA < 3 * ny, actB = lineB < 3 * ny;
+	double s = 0;
+	auto ld = [&](int line, int x) -> cx<T> { return Xa[((size_t)z * N + x) * 3 * bY + 3 * y0 + line]; };
+	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
+	PL::template fft<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	__syncthreads();
+	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
+		const int yl = idx / bX, x = idx % bX;
+		if (yl >= ny) continue;
+		const int d = cellmap[((size_t)z * bY + y0 + yl) * bX + x];
+		if (d < 0) continue;
+		const cx<T> *a = arg + 3 * (size_t)d, *cs = ccs + 3 * mat[d];
+		cx<T> *r = res + 3 * (size_t)d;
+		const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			cx<T> av = a[c];
+			if (her) av = conjc(av);
+			cx<T> rv = add(av, mul(cs[c], q[c * LS]));
+			if (her) rv = conjc(rv);
+			r[c] = rv;
+			s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+		}
+	}
+	if (partials) {
+		double tt = block_sum<T>(s, red_lds);
+		if (threadIdx.x == 0) partials[(size_t)kq * (gridDim.x / bq.K) + tile] = tt;
+	}
+}
+
+// The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
+enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_KINDS = 5 };
+struct MvKernels {
+	const void *k[MV_KINDS][2];
+};
+
+template <typename T, class PL> void mv_fill_t(MvKernels &m)
+{
+	const int f = sizeof(T) == 4;
+	m.k[MV_P1][f] = (const void *)&k_p1<T, PL>;
+	m.k[MV_ZFWD][f] = (const void *)&k_zfft<T, false, PL>;
+	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
+	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
+	m.k[MV_P5][f] = (const void *)&k_p5<T, PL>;
+}
+template <class PL> void mv_fill(MvKernels &m)
+{
+	mv_fill_t<double, PL>(m);
+	mv_fill_t<float, PL>(m);
+}
+
+// Largest dynamic LDS a pass uses (set as the kernels' limit; P5 also has a static reduction buffer).
+constexpr int kMvLdsMax = 96 * 1024;
+
+template <typename T, class PL> hipError_t mv_lds_t()
+{
+	hipError_t e;
+	if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
+		return e;
+	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
+		return e;
+	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		kMvLdsMax - 4096))) return e;
+	if ((e = hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		kMvLdsMax))) return e;
+	return hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		kMvLdsMax);
+}
+
+template <class... Ps> struct PlanList {};
+
+// Fills m with the kernels of the first plan of the list equal to p; false if none.
+template <class... Ps> bool mv_find(const FftPlan2 &p, MvKernels &m, PlanList<Ps...>)
+{
+	return (... || (Ps::match(p) ? (mv_fill<Ps>(m), true) : false));
+}
+
+template <class... Ps> hipError_t mv_lds_all(PlanList<Ps...>)
+{
+	hipError_t e = hipSuccess;
+	((e = e ? e : mv_lds_t<double, Ps>()), ...);
+	((e = e ? e : mv_lds_t<float, Ps>()), ...);
+	return e;
+}
+
+// Instantiation groups (one translation unit each).
+bool mv_find_dyn(const FftPlan2 &p, MvKernels &m);  // always succeeds (run-time plan)
+bool mv_find_pow2_large(const FftPlan2 &p, MvKernels &m);
+bool mv_find_pow2_small(const FftPlan2 &p, MvKernels &m);
+bool mv_find_mixed(const FftPlan2 &p, MvKernels &m);
+hipError_t mv_lds_dyn();
+hipError_t mv_lds_pow2_large();
+hipError_t mv_lds_pow2_small();
+hipError_t mv_lds_mixed();
+
+}  // namespace adda

@@ -138,6 +138,8 @@ def main():
         tq64, cq64, rq64 = run("qmr", A.PREC_FP64)
         tqm, cqm, rqm = run("qmr", A.PREC_MIXED, max_inner=args.qmr_max_inner)
         extra["qmr"] = dict(tts_fp64_s=round(tq64, 4), tts_mixed_s=round(tqm, 4), speedup=round(tq64 / tqm, 3),
+                            converged_fp64=rq64["rc"] == 0, converged_mixed=rqm["rc"] == 0,
+                            fp64_matvecs_mixed=rqm["nmatvec_fp64"],
                             iterations_fp64=rq64["niter"], iterations_mixed=rqm["niter"], outer_mixed=rqm["nouter"],
                             true_resid_mixed=rqm["true_resid"], max_inner=args.qmr_max_inner,
                             cext_rel_err=abs(cqm - cq64) / abs(cq64), cext_vs_bicgstab_fp64=abs(cq64 - cext64) / cext64)

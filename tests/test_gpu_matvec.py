@@ -145,20 +145,30 @@ def test_autotune_keeps_results(hipdev, tmp_path):
     assert _rel(y1, y0) < 1e-13
 
 
-COMPILED_256 = ["16.16", "8.8.4", "4.8.8", "16.4.4"]
+COMPILED = {  # length -> compile-time schedules (mv_*.hip) + one run-time schedule of the same length
+    256: ["16.16", "8.8.4", "4.8.8", "16.4.4", "2.16.8"],
+    512: ["16.8.4", "8.8.8", "4.8.16"],
+    128: ["16.8", "8.16", "8.4.4", "4.4.8", "2.8.8"],
+    64: ["16.4", "8.8", "4.16", "2.4.8"],
+    32: ["8.4", "4.8", "2.16"],
+    96: ["8.4.3", "16.2.3", "3.8.4", "4.8.3"],
+    108: ["4.3.3.3", "3.3.3.4", "2.2.3.3.3"],
+    640: ["16.8.5", "8.16.5", "5.16.8", "4.4.8.5"],
+}
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
-def test_matvec_compiled_plans_256(hipdev, axis):
-    """Each compile-time FFT plan of length 256 along each axis (P1/P5 for x, P3 for y, P2/P4 for z), and a run-time
-    plan of the same length, against the oracle: FP64 <= 1e-12, FP32 <= 2e-5 (box 128 along that axis, 3 across)."""
+@pytest.mark.parametrize("n", sorted(COMPILED))
+def test_matvec_compiled_plans(hipdev, axis, n):
+    """Each compile-time FFT plan of length n along each axis (P1/P5 for x, P3 for y, P2/P4 for z), and a run-time
+    plan of the same length, against the oracle: FP64 <= 1e-12, FP32 <= 2e-5 (box n/2 along that axis, 3 across)."""
     from cases import read_shape
     dims = [3, 3, 3]
-    dims[axis] = 128
+    dims[axis] = n // 2
     g = np.indices(dims).reshape(3, -1).T
     pos, box = read_shape(g[(g[:, 0] + 2 * g[:, 1] + 3 * g[:, 2]) % 5 != 0])  # a ragged fill of the box
     c = Case(pos, box, 1.3 + 0.02j, dpl=12)
-    assert c.grid[axis] == 256
+    assert c.grid[axis] == n
     cc, ccs = c.cc(PROP_Z, POL_Y)
     op = _op(hipdev, c, fp32=True)
     op.set_cc_sqrt(ccs)
@@ -166,7 +176,7 @@ def test_matvec_compiled_plans_256(hipdev, axis):
     y_or, _ = c.matvec_fft(c.dmatrix(), ccs, x)
     for prec in (hipdev.F64, hipdev.F32):
         base = op.get_config(prec).split()[:3] + ["LY=2", "TC=4", "LP=1"]  # tiles that fit every schedule
-        for sched in COMPILED_256 + ["2.16.8"]:
+        for sched in COMPILED[n]:
             cfg = list(base)
             cfg[axis] = "xyz"[axis] + "=" + sched
             op.set_config(" ".join(cfg), prec)

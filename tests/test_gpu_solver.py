@@ -136,6 +136,21 @@ def test_mixed_precision_c1(hipdev, method, max_inner):
         pr.close()
 
 
+@pytest.mark.parametrize("method", ["qmr", "bicgstab"])
+def test_mixed_precision_fp64_finish(hipdev, method):
+    """An RHS still above eps after the FP32 correction steps (here: one step of at most 5 FP32 iterations) finishes
+    with an FP64 Krylov solve of its correction equation; the FP64 residual decides, so the result is converged."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED, max_outer=1, max_inner=5)
+        assert r["rc"] == 0 and r["true_resid"] <= 1e-10, r
+        assert r["nouter"] == 2 and r["nmatvec_fp64"] > 10, r
+        assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6
+    finally:
+        pr.close()
+
+
 def test_recalc_resid_and_init_fields(hipdev):
     pr = _problem(hipdev, "sphere8", 1.1 + 0.1j, 10)
     try:

@@ -121,6 +121,7 @@ struct MvCfg {
 	int LY;               // y rows per block in P1/P5 (3*LY lines: all components)
 	int TC;               // columns per block in P2/P4
 	int LP;               // x-pairs per block in P3
+	int p3r = 0;          // P3 with the D-matrix product in registers (k_p3r; compile-time power-of-two y plans)
 };
 
 struct TwPair {
@@ -584,6 +585,8 @@ static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + 
 static size_t lds_p24(const MvCfg &c, int gZ, size_t es) { return (size_t)(gZ + gZ * c.TC) * es; }
 static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
+	// twiddles + lines (+ the block's D rows for the register variant, which always reserves both planes' lines)
+	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
 	return (size_t)(gY + 3 * (reduced ? 2 : 1) * c.LP * rowstride(gY)) * es;
 }
 static constexpr size_t kLdsMax = 96 * 1024;
@@ -591,6 +594,10 @@ static constexpr size_t kLdsMax = 96 * 1024;
 // whether a configuration is launchable for this operator and element size
 static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 {
+	if (c.p3r) {  // the register variant: compiled for this y plan and precision, 2*LP*TPL threads (<= 256)
+		const MvKernels k = mv_kernels(c.qy);
+		if (!k.k[MV_P3R][es == 8] || 2 * c.LP * (op->gY / c.qy.rmax) > 256) return false;
+	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
 	const int ml = op->reduced ? 2 : 1;
 	return c.LY >= 1 && c.TC >= 1 && c.LP >= 1 && op->gX % c.LP == 0 && 3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
@@ -641,8 +648,9 @@ template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
-	auto k = (P3Fn<T>)c.ky.k[MV_P3][sizeof(T) == 4];
-	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(3 * ml3 * c.LP * (op->gY / c.qy.rmax)),
+	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : MV_P3][sizeof(T) == 4];
+	const int nthr = (c.p3r ? 2 : 3 * ml3) * c.LP * (op->gY / c.qy.rmax);
+	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
 		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed,
 		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
@@ -1973,7 +1981,7 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d", x, y, z, c.LY, c.TC, c.LP);
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "");
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -1986,6 +1994,8 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.LY = atoi(ly + 3);
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
+	const char *r3 = strstr(s, "R3=");
+	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	return true;
 }
 
@@ -2122,17 +2132,19 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			float t = timeit([&] { launch_zfft<T, false>(op, c, K, nullptr); launch_zfft<T, true>(op, c, K, nullptr); });
 			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
 		}
-	// y pass (P3): radix schedule of y and LP
+	// y pass (P3): radix schedule of y, LP and the register-product variant
 	for (auto &q : py)
-		for (int LP : {1, 2, 3, 4, 6, 8}) {
+		for (int LP : {1, 2, 3, 4, 6, 8})
+		for (int r3 : {0, 1}) {
 			MvCfg c = best;
 			c.qy = q;
 			c.LP = LP;
+			c.p3r = r3;
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
-			if (t < ty) { ty = t; best.qy = q; best.LP = LP; }
+			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; }
 		}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);

@@ -232,6 +232,80 @@ template <int A, int... B> constexpr int cmax()
 	else return A > cmax<B...>() ? A : cmax<B...>();
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Three lines per thread: the x, y and z components of one grid line, carried through the same butterflies so that a
+// 3x3 product per frequency can be applied in registers. e[c][u*R + q] holds element (u, q) of component c in the
+// butterfly layout of the current stage (thread t owns butterflies j_u = t + u*TPL, u < RMAX/R). A stage takes its
+// inputs from global memory (Ld functor ld(c, k)), LDS (lines lb + c) or the registers left by the previous stage,
+// and leaves its outputs in LDS, in registers (element (u, q) = frequency j_u + q*NS), or in global memory
+// (St functor st(c, k, v)). Requires every radix to divide RMAX (power-of-two plans).
+enum { IO_GLOBAL = 0, IO_LDS = 1, IO_REG = 2 };
+
+template <typename T, bool INV, int N, int TPL, int RMAX, int NS, int R, int IN, int OUT, class Ld, class St>
+__device__ __forceinline__ void stage3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx<T> *tws, int LS, int lb, int t,
+	bool active, Ld &ld, St &st)
+{
+	static_assert(RMAX % R == 0, "three-line stages need every radix to divide the largest one");
+	constexpr int NB = N / R, MU = RMAX / R, TW = N / (NS * R);
+	if (active) {
+#pragma unroll
+		for (int u = 0; u < MU; u++) {
+			const int j = t + u * TPL;
+			if constexpr (IN == IO_GLOBAL) {
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int q = 0; q < R; q++) e[c][u * R + q] = ld(c, j + q * NB);
+			}
+			else if constexpr (IN == IO_LDS) {
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int q = 0; q < R; q++) e[c][u * R + q] = buf[(lb + c) * LS + lpos(j + q * NB, 1)];
+			}
+			if (NS > 1) {
+				const int k = j % NS;
+#pragma unroll
+				for (int q = 1; q < R; q++) {
+					const cx<T> w = tws[q * k * TW];
+					const cx<T> wq = INV ? conjc(w) : w;
+#pragma unroll
+					for (int c = 0; c < 3; c++) e[c][u * R + q] = mul(e[c][u * R + q], wq);
+				}
+			}
+#pragma unroll
+			for (int c = 0; c < 3; c++) dftR2<T, INV, R>(&e[c][u * R]);
+		}
+	}
+	if constexpr (OUT == IO_LDS) {
+		__syncthreads();  // every LDS read of this stage (by any thread) precedes the writes
+		if (active) {
+#pragma unroll
+			for (int u = 0; u < MU; u++) {
+				const int j = t + u * TPL;
+				const int d = (j / NS) * NS * R + j % NS;
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int q = 0; q < R; q++) buf[(lb + c) * LS + lpos(d + q * NS, 1)] = e[c][u * R + q];
+			}
+		}
+		__syncthreads();
+	}
+	else if constexpr (OUT == IO_GLOBAL) {
+		if (active) {
+#pragma unroll
+			for (int u = 0; u < MU; u++) {
+				const int j = t + u * TPL;
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+#pragma unroll
+					for (int q = 0; q < R; q++) st(c, j + q * NS, e[c][u * R + q]);
+			}
+		}
+	}
+}
+
 template <int RM> struct Dyn {
 	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
@@ -274,6 +348,31 @@ template <int... Rs> struct Fix {
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
 		run<T, INV, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+	}
+
+	// Three-line transforms for the register D-product (stage3): the forward FFT takes the radices in plan order,
+	// loads from global memory and leaves the last stage's outputs in registers (frequency t + u*TPL + q*NS_LAST);
+	// the inverse takes them in reverse order, so its first stage consumes exactly those registers, and stores to
+	// global memory.
+	static constexpr bool POW2 = ((RMAX % Rs == 0) && ...);
+	static constexpr int NS_LAST = N / RAD[NST - 1];
+	template <typename T, int S, int NS, class Ld>
+	__device__ static __forceinline__ void fwd3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx<T> *tws, int LS, int lb, int t,
+		bool active, Ld &ld)
+	{
+		constexpr int R = RAD[S];
+		constexpr int IN = S == 0 ? IO_GLOBAL : IO_LDS, OUT = S == NST - 1 ? IO_REG : IO_LDS;
+		stage3<T, false, N, TPL, RMAX, NS, R, IN, OUT>(e, buf, tws, LS, lb, t, active, ld, ld);
+		if constexpr (S < NST - 1) fwd3<T, S + 1, NS * R>(e, buf, tws, LS, lb, t, active, ld);
+	}
+	template <typename T, int S, int NS, class St>
+	__device__ static __forceinline__ void inv3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx<T> *tws, int LS, int lb, int t,
+		bool active, St &st)
+	{
+		constexpr int R = RAD[NST - 1 - S];
+		constexpr int IN = S == 0 ? IO_REG : IO_LDS, OUT = S == NST - 1 ? IO_GLOBAL : IO_LDS;
+		stage3<T, true, N, TPL, RMAX, NS, R, IN, OUT>(e, buf, tws, LS, lb, t, active, st, st);
+		if constexpr (S < NST - 1) inv3<T, S + 1, NS * R>(e, buf, tws, LS, lb, t, active, st);
 	}
 };
 

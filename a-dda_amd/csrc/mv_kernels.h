@@ -6,6 +6,8 @@
 #include "fft_core.h"
 #include "krylov_dev.h"
 
+#include <type_traits>
+
 namespace adda {
 
 // Per-RHS layout of a batched launch: K right-hand sides share the operator (D-matrix, geometry, plans); the RHS
@@ -205,6 +207,94 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	PL::template fft<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
+// D-matrix product at frequency y of one grid line of plane zz, for a block at plane z (src/matvec.c:270-305:
+// reduced-FFT mirror indices and sign flips; cSymMatrVec src/cmplx.h:369-376), applied to the three components a0..a2
+// in place. Dl: the line's stored D row [yt < DsY][6] (already mirrored in x and z, see k_p3r).
+template <typename T>
+__device__ __forceinline__ void dmul3(const cx<T> *Dl, int y, int zz, int N, int DsY, int DsZ, int reduced,
+	int transposed, cx<T> &a0, cx<T> &a1, cx<T> &a2)
+{
+	int yi = y;
+	if (transposed) yi = yi > 0 ? N - yi : 0;
+	else if (yi >= DsY) yi = N - yi;
+	const cx<T> *dm = Dl + yi * 6;
+	const cx<T> d0 = dm[0], d3 = dm[3], d5 = dm[5];
+	cx<T> f1 = dm[1], f2 = dm[2], f4 = dm[4];
+	if (reduced) {
+		if (y >= DsY) {
+			f1 = scl(f1, (T)-1);
+			if (zz >= DsZ) f2 = scl(f2, (T)-1);
+			else f4 = scl(f4, (T)-1);
+		}
+		else if (zz >= DsZ) {
+			f2 = scl(f2, (T)-1);
+			f4 = scl(f4, (T)-1);
+		}
+	}
+	const cx<T> b0 = a0, b1 = a1, b2 = a2;
+	a0 = add(add(mul(d0, b0), mul(f1, b1)), mul(f2, b2));
+	a1 = add(add(mul(f1, b0), mul(d3, b1)), mul(f4, b2));
+	a2 = add(add(mul(f2, b0), mul(f4, b1)), mul(d5, b2));
+}
+
+// P3 with the D-matrix product in registers (compile-time power-of-two plans): thread t of group g = (plane, pair)
+// carries the three components of its butterflies through the forward FFT-y; the last stage leaves frequencies
+// t + u*TPL + q*NS_LAST in registers, where the product with D is applied, and the inverse FFT-y of the reversed
+// radix order starts from exactly those registers -- no LDS round trip and no separate pass over the spectrum.
+// Same blocks, D-tile sharing and mirror-plane pairing as k_p3; groups g < LP are plane z, the others z' = gZ - z.
+template <typename T, class PL>
+__global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
+	Bq bq)
+{
+	constexpr int N = PL::N, TPL = PL::TPL, RMAX = PL::RMAX, RL = PL::RAD[PL::NST - 1], NSL = PL::NS_LAST;
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xb += kq * bq.xs;
+	extern __shared__ char smem_raw[];
+	const int LS = rowstride(N);
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N, *Dt = buf + 6 * LP * LS;
+	const int nxt = gX / LP;
+	const int z = tile / nxt, x0 = (tile % nxt) * LP;
+	// the block's D rows [pr][yt][6] into LDS with coalesced loads (rows of consecutive x are contiguous in D)
+	{
+		int zi = z;
+		if (transposed) zi = zi > 0 ? gZ - zi : 0;
+		else if (zi >= DsZ) zi = gZ - zi;
+		const int rl = DsY * 6;
+		for (int i = threadIdx.x; i < LP * rl; i += blockDim.x) {
+			const int pr = i / rl, x = x0 + pr;
+			const int xi = transposed ? (x > 0 ? gX - x : 0) : x;
+			Dt[i] = D[((size_t)zi * gX + xi) * rl + i % rl];
+		}
+	}
+	load_tw(tws, twg, N);  // ends with a barrier, which also publishes the D rows
+	const bool has2 = reduced && z != 0 && 2 * z != gZ;
+	const int zp = gZ - z;
+	const int tid = threadIdx.x, g = tid / TPL, t = tid % TPL;
+	const int s = g / LP, pr = g % LP, zz = s ? zp : z;
+	const bool act = g < (has2 ? 2 : 1) * LP;
+	cx<T> *base = Xb + ((size_t)zz * gX + x0 + pr) * 3 * bY;  // element (y, c) at base[3y + c]
+	cx<T> e[3][RMAX];
+	auto ld = [&](int c, int y) -> cx<T> { return y < bY ? base[3 * y + c] : mk<T>(0, 0); };
+	PL::template fwd3<T, 0, 1>(e, buf, tws, LS, 3 * g, t, act, ld);
+	if (act) {
+		const cx<T> *Dl = Dt + pr * DsY * 6;
+#pragma unroll
+		for (int u = 0; u < RMAX / RL; u++)
+#pragma unroll
+			for (int q = 0; q < RL; q++) {
+				const int i = u * RL + q;
+				dmul3<T>(Dl, t + u * TPL + q * NSL, zz, N, DsY, DsZ, reduced, transposed, e[0][i], e[1][i], e[2][i]);
+				__builtin_amdgcn_sched_barrier(0);  // one frequency at a time: bounds the registers held by D
+			}
+	}
+	auto st = [&](int c, int y, cx<T> v) {
+		if (y < bY) base[3 * y + c] = v;
+	};
+	PL::template inv3<T, 0, 1>(e, buf, tws, LS, 3 * g, t, act, st);
+}
+
 // P5: block = (z, y0..y0+LY, RHS), all three components; stage 0 loads Xa[z][x][y][c] with (y, c) fastest (runs of
 // 3*LY elements), the inverse FFT lands in the LDS lines yl*3 + c, and each thread then gathers one dipole:
 // result = arg + S.X (her: conj), three contiguous components; partial ||result||^2 per block at
@@ -262,8 +352,18 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	}
 }
 
+// The register D-product variant exists for compile-time power-of-two plans whose three-component register file
+// stays within 48 complex values per thread (FP32 up to radix 16, FP64 up to radix 8).
+template <class PL> struct is_fix : std::false_type {};
+template <int... Rs> struct is_fix<Fix<Rs...>> : std::true_type {};
+template <typename T, class PL> constexpr bool p3r_ok()
+{
+	if constexpr (is_fix<PL>::value) return PL::POW2 && 3 * PL::RMAX * sizeof(cx<T>) <= 48 * 8;
+	else return false;
+}
+
 // The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
-enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_KINDS = 5 };
+enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_KINDS = 6 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
 };
@@ -276,6 +376,8 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
 	m.k[MV_P5][f] = (const void *)&k_p5<T, PL>;
+	m.k[MV_P3R][f] = nullptr;
+	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
 }
 template <class PL> void mv_fill(MvKernels &m)
 {
@@ -297,6 +399,9 @@ template <typename T, class PL> hipError_t mv_lds_t()
 		kMvLdsMax - 4096))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax))) return e;
+	if constexpr (p3r_ok<T, PL>())
+		if ((e = hipFuncSetAttribute((const void *)k_p3r<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+			kMvLdsMax))) return e;
 	return hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax);
 }

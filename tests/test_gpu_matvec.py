@@ -179,6 +179,13 @@ def test_matvec_compiled_plans(hipdev, axis, n):
         for sched in COMPILED[n]:
             cfg = list(base)
             cfg[axis] = "xyz"[axis] + "=" + sched
-            op.set_config(" ".join(cfg), prec)
-            y, _ = op.matvec(x, prec=prec)
-            assert _rel(y, y_or) < (1e-12 if prec == hipdev.F64 else 2e-5), (prec, sched)
+            variants = [cfg]
+            radices = [int(r) for r in sched.split(".")]
+            if axis == 1 and n & (n - 1) == 0 and sched != COMPILED[n][-1] \
+                    and max(radices) <= (16 if prec == hipdev.F32 else 8):
+                variants.append(cfg + ["R3=1"])  # P3 with the D-matrix product in registers
+            for v in variants:
+                op.set_config(" ".join(v), prec)
+                assert op.get_config(prec) == " ".join(v)
+                y, _ = op.matvec(x, prec=prec)
+                assert _rel(y, y_or) < (1e-12 if prec == hipdev.F64 else 2e-5), (prec, v)

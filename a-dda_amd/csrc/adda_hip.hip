@@ -576,7 +576,7 @@ static int set_lds_limits()
 template <typename T> using P1Fn = decltype(&k_p1<T, Dyn<8>>);
 template <typename T> using ZFn = decltype(&k_zfft<T, false, Dyn<8>>);
 template <typename T> using P3Fn = decltype(&k_p3<T, Dyn<8>>);
-template <typename T> using P5Fn = decltype(&k_p5<T, Dyn<8>>);
+template <typename T> using P5Fn = decltype(&k_p5<T, Dyn<8>, false>);
 
 // ==================================================================================================================
 // matvec driver
@@ -659,13 +659,13 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 // returns the number of P5 blocks per RHS (the row length of its partial sums)
 template <typename T>
 static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
-	cx<T> *res, const cx<T> *ccs, int her, double *partials)
+	cx<T> *res, const cx<T> *ccs, int her, double *partials, const cx<T> *dotw = nullptr, int dmode = 0)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = op->bZ * nyt;
-	auto k = (P5Fn<T>)c.kx.k[MV_P5][sizeof(T) == 4];
+	auto k = (P5Fn<T>)c.kx.k[dmode ? MV_P5D : MV_P5][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
-		op->bY, c.LY, her, partials, make_bq(op, K, vs, xa_stride(op), 0, gate));
+		op->bY, c.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
 	return nblk5;
 }
 
@@ -677,9 +677,12 @@ template <> const cx<float> *cc_of<float>(const adda_hip_op *op) { return op->cc
 // non-zero status for it. inprod (optional) receives K values ||result_k||^2 (synchronises). nblk_out (optional):
 // the number of P5 blocks per RHS; with keep_partials the per-block ||result||^2 partials stay in op->partials
 // (layout [k][nblk]) for a device-side reduction (no synchronisation).
+// dmode (with keep_partials): 1 also accumulates sum result.conj(dotw), 2 sum result.conj(arg), as partial rows 1, 2
+// after the ||result||^2 row (per RHS: 3 rows of nblk).
 template <typename T>
 static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const cx<T> *arg, cx<T> *res,
-	double *inprod, int her, hipEvent_t *ev = nullptr, bool keep_partials = false, int *nblk_out = nullptr)
+	double *inprod, int her, hipEvent_t *ev = nullptr, bool keep_partials = false, int *nblk_out = nullptr,
+	const cx<T> *dotw = nullptr, int dmode = 0)
 {
 	const cx<T> *ccs = cc_of<T>(op);
 	const cx<T> *D;
@@ -699,7 +702,7 @@ static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const c
 	launch_zfft<T, true>(op, c, K, gate);
 	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
 	const int nblk5 = launch_p5<T>(op, c, K, vs, gate, arg, res, ccs, her,
-		(inprod || keep_partials) ? op->partials : nullptr);
+		(inprod || keep_partials) ? op->partials : nullptr, dotw, keep_partials ? dmode : 0);
 	if (nblk_out) *nblk_out = nblk5;
 	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
@@ -802,10 +805,12 @@ __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const
 	}
 }
 
-// x += st[ix1]*p + st[ix2]*s2v ; r = st[ir1]*r1 + st[ir2]*r2 ; partial ||r||^2 (BiCGStab and CGNR tails)
+// x += st[ix1]*p + st[ix2]*s2v ; r = st[ir1]*r1 + st[ir2]*r2 ; partial ||r||^2 (BiCGStab and CGNR tails); with rt,
+// also rows 1, 2 = Re, Im of sum r.conj(rt) (the r.r~ of BiCGStab's next iteration, in the order a dot pass uses)
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const cx<T> *s2v, cx<T> *r, const cx<T> *r1,
-	const cx<T> *r2, const double *st, int ix1, int ix2, int ir1, int ir2, size_t n, long vs, double *partials, int ps)
+	const cx<T> *r2, const double *st, int ix1, int ix2, int ir1, int ir2, size_t n, long vs, double *partials, int ps,
+	const cx<T> *rt)
 {
 	__shared__ double red_lds[512];
 	DK_RHS_VEC(st, partials, ps);
@@ -816,8 +821,9 @@ __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const c
 	r += voff;
 	r1 += voff;
 	r2 += voff;
+	if (rt) rt += voff;
 	const dcx a1 = ld_c(st, ix1), a2 = ld_c(st, ix2), b1 = ld_c(st, ir1), b2 = ld_c(st, ir2);
-	double s0 = 0;
+	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		cx<T> xv = x[i];
 		double xr = xv.x, xi = xv.y;
@@ -830,9 +836,21 @@ __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const c
 		const cx<T> rv = mk<T>((T)rr, (T)ri);
 		r[i] = rv;
 		s0 += (double)rv.x * rv.x + (double)rv.y * rv.y;
+		if (rt) {
+			const cx<T> w = rt[i];
+			const double ux = rv.x, uy = rv.y, wx = w.x, wy = -(double)w.y;
+			s1 += ux * wx - uy * wy;
+			s2 += ux * wy + uy * wx;
+		}
 	}
-	const double t = block_sum<T>(s0, red_lds);
+	double t = block_sum<T>(s0, red_lds);
 	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+	if (rt) {
+		t = block_sum<T>(s1, red_lds);
+		if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = t;
+		t = block_sum<T>(s2, red_lds);
+		if (threadIdx.x == 0) partials[2 * gridDim.x + blockIdx.x] = t;
+	}
 }
 
 // QMR_CS tail (src/iterative.c:1007-1029): pn = q0*pa + q1*pb + q2*v -> pdst; x += tau*pn; vt = vtilda/beta;
@@ -1025,25 +1043,17 @@ __global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *part
 	st[DK_SC + 36] = v[0] < st[DK_EPSB] ? 1 : 0;  // half-step flag
 }
 
-__global__ void __launch_bounds__(256) k_dk_bs_s4(double *st, const double *pt, int nblk_t, int ps)
-{
-	__shared__ double red_lds[512];
-	__shared__ double v[1];
-	DK_RHS_SCALAR(st, pt, ps);
-	DK_ACTIVE(st);
-	dk_reduce(pt, nblk_t, 1, v, red_lds);
-	if (threadIdx.x == 0) st[DK_SC + 34] = v[0];
-}
-
-__global__ void __launch_bounds__(256) k_dk_bs_s5(double *st, const double *partials, int nblk, int ps)
+// s4 + s5 from the fused rows of the t = A.s matvec: row 0 ||t||^2, rows 1, 2 sum t.conj(s) = conj(s.t~).
+__global__ void __launch_bounds__(256) k_dk_bs_s45(double *st, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
 	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
-	dk_reduce(partials, nblk, 2, v, red_lds);
+	dk_reduce(partials, nblk, 3, v, red_lds);
 	if (threadIdx.x != 0) return;
-	dcx omega = dscl(dmk(v[0], v[1]), 1 / st[DK_SC + 34]);
+	st[DK_SC + 34] = v[0];
+	dcx omega = dscl(dmk(v[1], -v[2]), 1 / v[0]);
 	if (st[DK_SC + 36] != 0) omega = dmk(0, 0);
 	else { st_c(st, DK_SC + 6, omega); st_c(st, DK_SC + 2, ld_c(st, DK_SC + 0)); }  // ro_old = ro_new
 	st_c(st, DK_SC + 30, omega);
@@ -1148,24 +1158,37 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 {
 	const DkGeo g = dk_geo(op, k);
 	double *st = k->st;
-	int nbt;
-	dot_dev<T>(op, g, st, r, rt, ADDA_DOT_CONJ);
-	hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	int nb5;
+	// r.r~ : a dot pass in the first iteration, afterwards rows 1-2 left by the previous x/r update
+	if (niter == 1) {
+		dot_dev<T>(op, g, st, r, rt, ADDA_DOT_CONJ);
+		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	}
+	else
+		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2 + g.nb, g.nb, niter,
+			g.ps2);
 	// p = beta p - beta omega v + r  (niter 1: p = r)
 	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
-	if (niter == 1 && matvec_ready) lc_dev<T>(op, g, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
-	else if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)v, nullptr, 0)) return -1;
-	dot_dev<T>(op, g, st, v, rt, ADDA_DOT_CONJ);
-	hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	// v = A.p with v.r~ fused into the matvec's last pass
+	if (niter == 1 && matvec_ready) {
+		lc_dev<T>(op, g, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
+		dot_dev<T>(op, g, st, v, rt, ADDA_DOT_CONJ);
+		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	}
+	else {
+		if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)v, nullptr, 0, nullptr, true, &nb5,
+			(const cx<T> *)rt, 1)) return -1;
+		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials + nb5, nb5, 3 * nb5);
+	}
 	lc_dev<T>(op, g, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
 	hipLaunchKernelGGL(k_dk_bs_s3, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, k->hist, g.ps2);
-	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_bs_s4, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, nbt);
-	dot_dev<T>(op, g, st, s, t, ADDA_DOT_CONJ);
-	hipLaunchKernelGGL(k_dk_bs_s5, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	// t = A.s with ||t||^2 and t.s~ fused
+	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nb5, nullptr, 2))
+		return -1;
+	hipLaunchKernelGGL(k_dk_bs_s45, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nb5, 3 * nb5);
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
-		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2);
+		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)rt);
 	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1185,7 +1208,7 @@ static int cgnr_iter_t(adda_hip_op *op, dk_state *k, int niter, void *x, void *r
 	hipLaunchKernelGGL(k_dk_cg_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, nbt);
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)nullptr, (cx<T> *)r, (const cx<T> *)r, (const cx<T> *)Av, st, DK_SC + 8, DK_SC + 12, DK_SC + 6,
-		DK_SC + 10, g.n, g.vs, op->partials2, g.ps2);
+		DK_SC + 10, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)nullptr);
 	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;

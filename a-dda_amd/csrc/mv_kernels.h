@@ -297,19 +297,21 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 
 // P5: block = (z, y0..y0+LY, RHS), all three components; stage 0 loads Xa[z][x][y][c] with (y, c) fastest (runs of
 // 3*LY elements), the inverse FFT lands in the LDS lines yl*3 + c, and each thread then gathers one dipole:
-// result = arg + S.X (her: conj), three contiguous components; partial ||result||^2 per block at
-// partials[k*(blocks per RHS) + tile].
-template <typename T, class PL>
+// result = arg + S.X (her: conj), three contiguous components. Per block, at partials[(k*rows + row)*nb + tile] (nb
+// blocks per RHS): row 0 ||result||^2; with dmode 1 (w = dotw) or 2 (w = arg) also rows 1, 2 = Re, Im of
+// sum result.conj(w) -- an inner product of the Krylov iteration fused into the pass that writes its operand.
+template <typename T, class PL, bool DOT>
 __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
 	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
-	double *__restrict__ partials, Bq bq)
+	double *__restrict__ partials, const cx<T> *__restrict__ dotw, int dmode, Bq bq)
 {
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
 	Xa += kq * bq.xs;
 	arg += kq * bq.vs;
 	res += kq * bq.vs;
+	if (dotw) dotw += kq * bq.vs;
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
@@ -323,7 +325,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const int lineA = tid % NL, tA = tid / NL;    // (y, c) fastest (reads Xa)
 	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest (writes the LDS lines)
 	const bool actA = tA < TPL && lineA < 3 * ny, actB = lineB < 3 * ny;
-	double s = 0;
+	double s = 0, dr = 0, di = 0;
 	auto ld = [&](int line, int x) -> cx<T> { return Xa[((size_t)z * N + x) * 3 * bY + 3 * y0 + line]; };
 	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
 	PL::template fft<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
@@ -344,11 +346,24 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 			if (her) rv = conjc(rv);
 			r[c] = rv;
 			s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+			if (DOT) {
+				const cx<T> w = dmode == 1 ? dotw[3 * (size_t)d + c] : a[c];
+				dr += (double)rv.x * w.x + (double)rv.y * w.y;
+				di += (double)rv.y * w.x - (double)rv.x * w.y;
+			}
 		}
 	}
 	if (partials) {
+		const int rows = DOT ? 3 : 1;
+		const size_t nb = gridDim.x / bq.K, o = (size_t)kq * rows * nb + tile;
 		double tt = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[(size_t)kq * (gridDim.x / bq.K) + tile] = tt;
+		if (threadIdx.x == 0) partials[o] = tt;
+		if (DOT) {
+			tt = block_sum<T>(dr, red_lds);
+			if (threadIdx.x == 0) partials[o + nb] = tt;
+			tt = block_sum<T>(di, red_lds);
+			if (threadIdx.x == 0) partials[o + 2 * nb] = tt;
+		}
 	}
 }
 
@@ -363,7 +378,7 @@ template <typename T, class PL> constexpr bool p3r_ok()
 }
 
 // The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
-enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_KINDS = 6 };
+enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_KINDS = 7 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
 };
@@ -375,7 +390,8 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_ZFWD][f] = (const void *)&k_zfft<T, false, PL>;
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
-	m.k[MV_P5][f] = (const void *)&k_p5<T, PL>;
+	m.k[MV_P5][f] = (const void *)&k_p5<T, PL, false>;
+	m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
 }
@@ -395,7 +411,9 @@ template <typename T, class PL> hipError_t mv_lds_t()
 		return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
 		return e;
-	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		kMvLdsMax - 4096))) return e;
+	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax - 4096))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax))) return e;

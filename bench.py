@@ -47,9 +47,12 @@ def main():
                          "calibration kernel for HBM counters), no solves; prints the matvec figures only")
     ap.add_argument("--qmr-max-inner", type=int, default=2000)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-RHS and adjoint-gradient workloads")
-    ap.add_argument("--tune-cache", default=None,
-                    help="autotuner cache (JSON lines): configurations found there are reused without timing")
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "a-dda_amd", "tune_cache_mi355x.jsonl"),
+                    help="autotuner cache (JSON lines): configurations found there are reused without timing, new "
+                         "ones are tuned and appended (default: the cache the autotuner wrote on an MI355X for these "
+                         "workloads; pass an empty string to always tune)")
     args = ap.parse_args()
+    args.tune_cache = args.tune_cache or None
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -200,15 +203,15 @@ def main():
 
 def measured_traffic(pass_index, cfg):
     """HBM bytes per launch of the FP32 pass kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (calibrated against a copy of known size), when it was taken with the same kernel instantiation."""
-    names = ["k_p1", "k_zfft<float, false", "k_p3", "k_zfft<float, true", "k_p5"]
+    (calibrated against a copy of known size), when it was taken with the same tuned FP32 configuration."""
     try:
-        radix = cfg.split()[{0: 0, 1: 2, 2: 1, 3: 2, 4: 0}[pass_index]].split("=")[1]
-        rm = 8 if max(int(r) for r in radix.split(".")) <= 8 else 16
-        want = names[pass_index] + (f"<float, {rm}>" if pass_index in (0, 2, 4) else f", {rm}>")
+        with open(os.path.join(ROOT, "profiles", "matvec_grid256_kernel_trace.txt")) as f:
+            prof_cfg = [l for l in f if l.strip().startswith("fp32 x=")][0].split("fp32", 1)[1].split("(")[0].strip()
+        if prof_cfg != cfg.split("(")[0].strip():
+            return None
         with open(os.path.join(ROOT, "profiles", "matvec_grid256_hbm_traffic.txt")) as f:
             for line in f:
-                if line.startswith("P") and want in line:
+                if line.startswith(f"P{pass_index + 1} ") and "<float" in line:
                     return float(line.split()[-3])
     except (OSError, ValueError, IndexError):
         pass

@@ -600,7 +600,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
 	const int ml = op->reduced ? 2 : 1;
-	return c.LY >= 1 && c.TC >= 1 && c.LP >= 1 && op->gX % c.LP == 0 && 3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
+	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && op->gX % c.LP == 0 && 3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
 		3 * ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax && lds_p24(c, op->gZ, es) <= kLdsMax &&
 		lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
 }
@@ -731,7 +731,7 @@ static void default_cfg(adda_hip_op *op, int pi, size_t es)
 	c.LY = es == 8 ? 8 : 4;  // runs of 3*LY elements (192 bytes) in the transposed Xa accesses
 	while (c.LY > 1 && (3 * c.LY * tplx > 512 || lds_p15(c, gX, es) > cap)) c.LY /= 2;
 	c.TC = 16;
-	while (c.TC > 1 && (c.TC * tplz > 512 || lds_p24(c, gZ, es) > cap)) c.TC /= 2;
+	while (c.TC > 2 && (c.TC * tplz > 512 || lds_p24(c, gZ, es) > cap)) c.TC /= 2;
 	const int ml = op->reduced ? 2 : 1;
 	c.LP = 4;
 	while (c.LP > 1 && (3 * ml * c.LP * tply > 512 || lds_p3(c, gY, op->reduced, es) > cap || gX % c.LP)) c.LP--;

@@ -62,6 +62,80 @@ template <typename T, bool INV, int R> __device__ __forceinline__ void dftR2(cx<
 	else dftR<T, INV, R>(v);
 }
 
+// DFTs whose upper half of inputs (v[q], q >= R/2) is zero -- the first stage of a transform of zero-padded data,
+// where every input index j + q*N/R with q >= R/2 lies at or beyond N/2 >= the box size. Only v[0..R/2) is read; the
+// additions with the known zeros are skipped, the outputs are those of the full DFT.
+template <typename T, bool INV> __device__ __forceinline__ void dft4h(cx<T> *v)
+{
+	const cx<T> a = v[0], b = v[1], c = mulmi<T, INV>(b);
+	v[0] = add(a, b);
+	v[2] = sub(a, b);
+	v[1] = add(a, c);
+	v[3] = sub(a, c);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft8h(cx<T> *v)
+{
+	const T h = (T)0.70710678118654752440084436210484904;
+	cx<T> e[4] = {v[0], v[2]}, o[4] = {v[1], v[3]};
+	dft4h<T, INV>(e);
+	dft4h<T, INV>(o);
+	cx<T> o1 = INV ? mk<T>(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)) : mk<T>(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+	cx<T> o2 = mulmi<T, INV>(o[2]);
+	cx<T> o3 = INV ? mk<T>(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)) : mk<T>(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+	v[0] = add(e[0], o[0]);
+	v[4] = sub(e[0], o[0]);
+	v[1] = add(e[1], o1);
+	v[5] = sub(e[1], o1);
+	v[2] = add(e[2], o2);
+	v[6] = sub(e[2], o2);
+	v[3] = add(e[3], o3);
+	v[7] = sub(e[3], o3);
+}
+
+template <typename T, bool INV> __device__ __forceinline__ void dft16h(cx<T> *v)
+{
+	const T c1 = (T)0.92387953251128675612818318939678829;
+	const T s1 = (T)0.38268343236508977172845998403039887;
+	const T h = (T)0.70710678118654752440084436210484904;
+	cx<T> a[4][4];
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		cx<T> t[4] = {v[r], v[r + 4]};
+		dft4h<T, INV>(t);
+#pragma unroll
+		for (int s = 0; s < 4; s++) a[r][s] = t[s];
+	}
+	auto tw = [&](cx<T> x, T c, T s) -> cx<T> {
+		return INV ? mk<T>(x.x * c - x.y * s, x.x * s + x.y * c) : mk<T>(x.x * c + x.y * s, x.y * c - x.x * s);
+	};
+	a[1][1] = tw(a[1][1], c1, s1);
+	a[1][2] = tw(a[1][2], h, h);
+	a[1][3] = tw(a[1][3], s1, c1);
+	a[2][1] = tw(a[2][1], h, h);
+	a[2][2] = mulmi<T, INV>(a[2][2]);
+	a[2][3] = tw(a[2][3], -h, h);
+	a[3][1] = tw(a[3][1], s1, c1);
+	a[3][2] = tw(a[3][2], -h, h);
+	a[3][3] = tw(a[3][3], -c1, -s1);
+#pragma unroll
+	for (int s = 0; s < 4; s++) {
+		cx<T> t[4] = {a[0][s], a[1][s], a[2][s], a[3][s]};
+		dft4<T, INV>(t);
+#pragma unroll
+		for (int q = 0; q < 4; q++) v[s + 4 * q] = t[q];
+	}
+}
+
+template <typename T, bool INV, int R> __device__ __forceinline__ void dftR2h(cx<T> *v)
+{
+	static_assert(R % 2 == 0, "half-zero inputs need an even radix");
+	if constexpr (R == 16) dft16h<T, INV>(v);
+	else if constexpr (R == 8) dft8h<T, INV>(v);
+	else if constexpr (R == 4) dft4h<T, INV>(v);
+	else v[1] = v[0];
+}
+
 // LDS position of element k of a line: lines stored as rows (KS == 1) get one pad element after every 16, so the
 // stride-R writes of a Stockham stage (R up to 16 complex numbers apart) fall into different LDS banks; column tiles
 // (KS = number of columns, consecutive threads on consecutive columns) need no padding.
@@ -176,20 +250,25 @@ __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const Fft
 // schedule at compile time: the same Stockham stages with all of that folded into shifts and immediate offsets.
 // Kernels are templated on a plan policy: Dyn<RM> (run-time plan, RM = bound on the radix) or Fix<...>.
 
-template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, class Ld, class St>
+// HIN (first stage, even R): inputs j + q*NB with q >= R/2 are zero and are not loaded. HOUT (last stage, even R):
+// only outputs j + q*NS with q < R/2 (indices below N/2) are stored, and the arithmetic of the others is dropped.
+template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, bool HIN, bool HOUT,
+	class Ld, class St>
 __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int LS, int line, int t, bool active,
 	Ld &ld, St &st)
 {
 	constexpr int NB = N / R;                    // butterflies per line
 	constexpr int MU = (RMAX + R - 1) / R;        // butterflies per thread (TPL = N / RMAX threads per line)
 	constexpr int TW = N / (NS * R);              // twiddle stride of this stage
+	constexpr bool HI = FIRST && HIN && R % 2 == 0, HO = LAST && HOUT && R % 2 == 0;
 	cx<T> v[MU][R];
 #pragma unroll
 	for (int u = 0; u < MU; u++) {
 		const int j = t + u * TPL;
 		if (active && (NB % TPL == 0 || j < NB)) {
 #pragma unroll
-			for (int q = 0; q < R; q++) v[u][q] = FIRST ? ld(line, j + q * NB) : buf[lpos(j + q * NB, KS) + line * LS];
+			for (int q = 0; q < (HI ? R / 2 : R); q++)
+				v[u][q] = FIRST ? ld(line, j + q * NB) : buf[lpos(j + q * NB, KS) + line * LS];
 			if (!FIRST) {
 				const int k = j % NS;
 #pragma unroll
@@ -198,7 +277,8 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 					v[u][q] = mul(v[u][q], INV ? conjc(w) : w);
 				}
 			}
-			dftR2<T, INV, R>(v[u]);
+			if constexpr (HI) dftR2h<T, INV, R>(v[u]);
+			else dftR2<T, INV, R>(v[u]);
 		}
 	}
 	__syncthreads();  // a FIRST stage's Load may read the LDS buffer it is about to overwrite
@@ -208,7 +288,7 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 			const int j = t + u * TPL;
 			if (active && (NB % TPL == 0 || j < NB)) {
 #pragma unroll
-				for (int q = 0; q < R; q++) st(line, j + q * NS, v[u][q]);
+				for (int q = 0; q < (HO ? R / 2 : R); q++) st(line, j + q * NS, v[u][q]);
 			}
 		}
 	}
@@ -307,9 +387,12 @@ __device__ __forceinline__ void stage3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx
 }
 
 template <int RM> struct Dyn {
+	// the HIN / HOUT options are ignored: every input is loaded and every output stored
+	static constexpr bool HALF_IN = false, HALF_OUT = false;
 	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
-	template <typename T, bool INV, class Ld, class St>
+	// HIN / HOUT (zero upper half of the input, unneeded upper half of the output) only apply to compile-time plans
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, class Ld, class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
@@ -323,6 +406,8 @@ template <int... Rs> struct Fix {
 	static constexpr int N = (Rs * ...);
 	static constexpr int RMAX = cmax<Rs...>();
 	static constexpr int TPL = N / RMAX;
+	// whether HIN skips the inputs at indices >= N/2 and HOUT the outputs there (even first / last radix)
+	static constexpr bool HALF_IN = RAD[0] % 2 == 0, HALF_OUT = RAD[NST - 1] % 2 == 0;
 	static bool match(const FftPlan2 &p)
 	{
 		if (p.n != N || p.nst != NST) return false;
@@ -332,22 +417,24 @@ template <int... Rs> struct Fix {
 	}
 	__device__ static __forceinline__ constexpr int n(const FftPlan2 &) { return N; }
 	__device__ static __forceinline__ constexpr int tpl(const FftPlan2 &) { return TPL; }
-	template <typename T, bool INV, int S, int NS, class Ld, class St>
+	template <typename T, bool INV, bool HIN, bool HOUT, int S, int NS, class Ld, class St>
 	__device__ static __forceinline__ void run(cx<T> *buf, const cx<T> *tws, int KS, int LS, int lineA, int tA,
 		bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
 		constexpr int R = RAD[S];
 		constexpr bool FIRST = S == 0, LAST = S == NST - 1;
-		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX>(buf, tws, KS, LS, FIRST ? lineA : lineB, FIRST ? tA : tB,
-			FIRST ? activeA : activeB, ld, st);
+		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT>(buf, tws, KS, LS, FIRST ? lineA : lineB,
+			FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
 		if constexpr (!LAST)
-			run<T, INV, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+			run<T, INV, HIN, HOUT, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
-	template <typename T, bool INV, class Ld, class St>
+	// HIN: inputs at indices >= N/2 are zero (zero-padded data); HOUT: outputs at indices >= N/2 are not needed.
+	// Either may leave the Load / Store functor uncalled for those indices.
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, class Ld, class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
-		run<T, INV, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+		run<T, INV, HIN, HOUT, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
 
 	// Three-line transforms for the register D-product (stage3): the forward FFT takes the radices in plan order,

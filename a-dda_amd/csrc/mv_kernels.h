@@ -76,8 +76,11 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest (reads the LDS lines)
 	const int lineB = tid % NL, tB = tid / NL;    // (y, c) fastest (writes Xa)
 	const bool actA = lineA < 3 * ny, actB = tB < TPL && lineB < 3 * ny;
-	for (int idx = tid; idx < LY * N; idx += blockDim.x) {
-		const int yl = idx / N, x = idx % N;
+	// the box fills at most half of the line (gridX >= 2*boxX); with a compile-time plan the transform never reads the
+	// upper half, so it is not zero-filled
+	const int NF = PL::HALF_IN ? N / 2 : N;
+	for (int idx = tid; idx < LY * NF; idx += blockDim.x) {
+		const int yl = idx / NF, x = idx % NF;
 		cx<T> v0 = mk<T>(0, 0), v1 = v0, v2 = v0;
 		if (x < bX && yl < ny) {
 			const int d = cellmap[((size_t)z * bY + y0 + yl) * bX + x];
@@ -101,8 +104,10 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	}
 	__syncthreads();
 	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
-	auto st = [&](int line, int x, cx<T> v) { Xa[((size_t)z * N + x) * 3 * bY + 3 * y0 + line] = v; };
-	PL::template fft<T, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineB;
+	const size_t xs = (size_t)3 * bY;
+	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
+	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 }
 
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
@@ -118,22 +123,105 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	const int N = PL::n(p), TPL = PL::tpl(p);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
+	__builtin_assume(TC >= 2);  // column tiles (element k of column l at buf[k*TC + l]); the launcher guarantees TC >= 2
 	const long q0 = (long)(blockIdx.x / bq.K) * TC;
 	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
 	const bool act = t < TPL && q0 + line < Q;
-	auto ld = [&](int l, int k) -> cx<T> { return k < n_in ? in[(size_t)k * Q + q0 + l] : mk<T>(0, 0); };
-	auto st = [&](int l, int k, cx<T> v) {
-		if (k < n_out) out[(size_t)k * Q + q0 + l] = v;
-	};
-	PL::template fft<T, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+	const cx<T> *ip = in + q0 + line;
+	cx<T> *op = out + q0 + line;
+	// The forward transform reads rows k < boxZ <= N/2, the inverse writes rows k < boxZ: the other half of the rows is
+	// zero / not needed. With boxZ == N/2 (the usual grid) no per-row check remains.
+	if ((INV ? PL::HALF_OUT : PL::HALF_IN) && (INV ? 2 * n_out == N : 2 * n_in == N)) {
+		auto ld = [&](int, int k) -> cx<T> { return ip[(size_t)k * Q]; };
+		auto st = [&](int, int k, cx<T> v) { op[(size_t)k * Q] = v; };
+		PL::template fft<T, INV, !INV, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+	}
+	else {
+		auto ld = [&](int, int k) -> cx<T> { return k < n_in ? ip[(size_t)k * Q] : mk<T>(0, 0); };
+		auto st = [&](int, int k, cx<T> v) {
+			if (k < n_out) op[(size_t)k * Q] = v;
+		};
+		PL::template fft<T, INV, !INV, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+	}
+}
+
+// Sign flip of both parts of a complex number by an integer mask (0 or the sign bit): one XOR per part.
+template <typename T> __device__ __forceinline__ cx<T> flip(cx<T> a, unsigned m);
+template <> __device__ __forceinline__ cx<float> flip<float>(cx<float> a, unsigned m)
+{
+	return mk<float>(__uint_as_float(__float_as_uint(a.x) ^ m), __uint_as_float(__float_as_uint(a.y) ^ m));
+}
+template <> __device__ __forceinline__ cx<double> flip<double>(cx<double> a, unsigned m)
+{
+	const unsigned long long M = (unsigned long long)m << 32;
+	return mk<double>(__longlong_as_double(__double_as_longlong(a.x) ^ M),
+		__longlong_as_double(__double_as_longlong(a.y) ^ M));
+}
+
+// acc + s*(d*a) for a compile-time sign s (the compiler folds the sign into the FMA operands)
+template <typename T, int S> __device__ __forceinline__ cx<T> cmac(cx<T> acc, cx<T> d, cx<T> a)
+{
+	return S > 0 ? mk<T>(acc.x + d.x * a.x - d.y * a.y, acc.y + d.x * a.y + d.y * a.x)
+	             : mk<T>(acc.x - d.x * a.x + d.y * a.y, acc.y - d.x * a.y - d.y * a.x);
+}
+
+// Symmetric 3x3 product (cSymMatrVec, src/cmplx.h:369-376) at one frequency, in place in LDS (q[0], q[LS], q[2*LS]).
+// D = (d0 f1 f2; f1 d3 f4; f2 f4 d5) with f1, f4 already flipped for the y mirror; Z = -1 flips the xz and yz elements
+// for the z mirror plane (src/matvec.c:270-305).
+template <typename T, int Z>
+__device__ __forceinline__ void sym3_lds(cx<T> *q, int LS, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3, cx<T> f4, cx<T> d5)
+{
+	const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
+	q[0] = cmac<T, Z>(cmac<T, 1>(mul(d0, a0), f1, a1), d2, a2);
+	q[LS] = cmac<T, Z>(cmac<T, 1>(mul(f1, a0), d3, a1), f4, a2);
+	q[2 * LS] = cmac<T, Z>(cmac<T, Z>(mul(d5, a2), f4, a1), d2, a0);
 }
 
 // P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
 // which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
 // HBM once per matvec. The RHS index is fastest in the block index: the K blocks that share a D tile run together
-// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous boxY runs of Xb. Forward FFT-y into
-// LDS (natural order), the D-matrix product (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips,
-// cSymMatrVec src/cmplx.h:369-376), inverse FFT-y, keep y<boxY.
+// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous
+// boxY runs of Xb (stride 3, components interleaved). Forward FFT-y into LDS (natural order), the D-matrix product
+// (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips, cSymMatrVec src/cmplx.h:369-376), inverse FFT-y,
+// keep y < boxY. The box occupies at most half of every grid line (gridY >= 2*boxY), so with a compile-time plan the
+// forward transform skips the zero upper half and the inverse computes only the lower half; FULL (boxY == gridY/2,
+// a plan with both half options) drops the y < boxY checks, which then hold for every index the transforms touch.
+template <typename T, class PL, bool FULL>
+__device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp,
+	const cx<T> *__restrict__ D, int x0, int z, int gX, int bY, int gZ, int DsY, int LP, int NL1, bool has2,
+	int reduced, int transposed, int line, int t, bool act)
+{
+	const int N = PL::n(p), LS = rowstride(N);
+	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
+	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
+	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	__syncthreads();
+	for (int idx = threadIdx.x; idx < LP * N; idx += blockDim.x) {
+		const int pr = idx / N, y = idx - pr * N;
+		int xi = x0 + pr, yi = y, zi = z;
+		if (transposed) {  // A^T with the full D-matrix: D(-r), src/matvec.c:181 (never with the reduced form)
+			xi = xi > 0 ? gX - xi : 0;
+			yi = yi > 0 ? N - yi : 0;
+			zi = zi > 0 ? gZ - zi : 0;
+		}
+		else if (yi >= DsY) yi = N - yi;
+		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
+		const cx<T> d0 = dm[0], d2 = dm[2], d3 = dm[3], d5 = dm[5];
+		// y mirror (reduced form): the xy and yz elements change sign; the block's own plane z < DsizeZ never mirrors
+		const unsigned m = (reduced && y >= DsY) ? 0x80000000u : 0u;
+		const cx<T> f1 = flip<T>(dm[1], m), f4 = flip<T>(dm[4], m);
+		cx<T> *q = buf + (size_t)(3 * pr) * LS + lpos(y, 1);
+		sym3_lds<T, 1>(q, LS, d0, f1, d2, d3, f4, d5);
+		if (has2) sym3_lds<T, -1>(q + (size_t)NL1 * LS, LS, d0, f1, d2, d3, f4, d5);  // plane gZ - z >= DsizeZ
+	}
+	__syncthreads();
+	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
+	auto stg = [&](int, int y, cx<T> v) {
+		if (FULL || y < bY) lp[3 * y] = v;
+	};
+	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+}
+
 template <typename T, class PL>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
@@ -143,68 +231,25 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	if (skip_rhs(bq, kq)) return;
 	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP, LS = rowstride(N);
+	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	load_tw(tws, twg, N);
 	const int nxt = gX / LP;
 	const int z = tile / nxt, x0 = (tile % nxt) * LP;
-	const bool has2 = reduced && z != 0 && 2 * z != gZ;  // mirror plane handled by this block
-	const int zp = gZ - z;
+	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; the mirror plane is handled alongside
+	const bool has2 = reduced && z != 0 && 2 * z != gZ;
 	const int nls = has2 ? 2 * NL1 : NL1;
-	cx<T> *base0 = Xb + ((size_t)z * gX + x0) * 3 * bY;
-	cx<T> *base1 = Xb + ((size_t)(has2 ? zp : z) * gX + x0) * 3 * bY;
 	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
 	const bool act = line < nls;
-	// line l = (plane, pair, c): element y of it at Xb[z][x0 + pair][y][c]
-	auto gptr = [&](int l) -> cx<T> * {
-		const int ls = l < NL1 ? l : l - NL1;
-		return (l < NL1 ? base0 : base1) + (size_t)(ls / 3) * 3 * bY + ls % 3;
-	};
-	auto ldg = [&](int l, int y) -> cx<T> { return y < bY ? gptr(l)[3 * y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	PL::template fft<T, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
-	__syncthreads();
-	for (int idx = tid; idx < LP * N; idx += blockDim.x) {
-		const int pr = idx / N, y = idx % N, x = x0 + pr;
-		int xi = x, yi = y, zi = z;
-		if (transposed) {
-			xi = xi > 0 ? gX - xi : 0;
-			yi = yi > 0 ? N - yi : 0;
-			zi = zi > 0 ? gZ - zi : 0;
-		}
-		else {
-			if (yi >= DsY) yi = N - yi;
-			if (zi >= DsZ) zi = gZ - zi;
-		}
-		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
-		const cx<T> d0 = dm[0], d1 = dm[1], d2 = dm[2], d3 = dm[3], d4 = dm[4], d5 = dm[5];
-		for (int sidx = 0; sidx < (has2 ? 2 : 1); sidx++) {
-			const int zz = sidx ? zp : z;
-			cx<T> f1 = d1, f2 = d2, f4 = d4;
-			if (reduced) {
-				if (y >= DsY) {
-					f1 = scl(f1, (T)-1);
-					if (zz >= DsZ) f2 = scl(f2, (T)-1);
-					else f4 = scl(f4, (T)-1);
-				}
-				else if (zz >= DsZ) {
-					f2 = scl(f2, (T)-1);
-					f4 = scl(f4, (T)-1);
-				}
-			}
-			cx<T> *q = buf + (size_t)(sidx * NL1 + 3 * pr) * LS + lpos(y, 1);
-			const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
-			q[0] = add(add(mul(d0, a0), mul(f1, a1)), mul(f2, a2));
-			q[LS] = add(add(mul(f1, a0), mul(d3, a1)), mul(f4, a2));
-			q[2 * LS] = add(add(mul(f2, a0), mul(f4, a1)), mul(d5, a2));
-		}
-	}
-	__syncthreads();
-	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
-	auto stg = [&](int l, int y, cx<T> v) {
-		if (y < bY) gptr(l)[3 * y] = v;
-	};
-	PL::template fft<T, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+	// line l = (plane, pair, c): element y of it at lp[3*y], lp = &Xb[plane][x0 + pair][0][c]
+	const int ls = line < NL1 ? line : line - NL1;
+	cx<T> *lp = Xb + ((size_t)(line < NL1 ? z : gZ - z) * gX + x0 + ls / 3) * 3 * bY + ls % 3;
+	if (PL::HALF_IN && PL::HALF_OUT && 2 * bY == N)
+		p3_body<T, PL, true>(buf, tws, p, lp, D, x0, z, gX, bY, gZ, DsY, LP, NL1, has2, reduced, transposed, line, t,
+			act);
+	else
+		p3_body<T, PL, false>(buf, tws, p, lp, D, x0, z, gX, bY, gZ, DsY, LP, NL1, has2, reduced, transposed, line,
+			t, act);
 }
 
 // D-matrix product at frequency y of one grid line of plane zz, for a block at plane z (src/matvec.c:270-305:
@@ -326,9 +371,12 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest (writes the LDS lines)
 	const bool actA = tA < TPL && lineA < 3 * ny, actB = lineB < 3 * ny;
 	double s = 0, dr = 0, di = 0;
-	auto ld = [&](int line, int x) -> cx<T> { return Xa[((size_t)z * N + x) * 3 * bY + 3 * y0 + line]; };
+	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
+	const size_t xs = (size_t)3 * bY;
+	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
 	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
-	PL::template fft<T, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	// only x < boxX <= N/2 is gathered below: the inverse transform computes the lower half of each line
+	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 	__syncthreads();
 	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
 		const int yl = idx / bX, x = idx % bX;

@@ -65,6 +65,15 @@ def numeric_close(a, b, abs_tol, rel_tol):
     return not bad.any(), np.argwhere(bad)
 
 
+def mueller_close(mu, ref, s11):
+    """Mueller matrices (rows of 16 elements, or the file layout with a leading theta column) against the reference's:
+    rel 1e-5 as in comp2exec, and an absolute floor of 1e-9 relative to the largest S11 instead of comp2exec's bare
+    1e-9. Elements that vanish by symmetry (S34/S43 in the forward direction of a sphere) carry the solver's noise,
+    about S11 times the residual tolerance (141.5 x 1e-10 for the 16^3 sphere); comp2exec's fixed 1e-9 only holds
+    between two runs with the same iteration rounding."""
+    return numeric_close(mu, ref, 1e-9 * float(np.abs(s11).max()), 1e-5)
+
+
 def adj8_fd(pol):
     """Central differences of the reference's Cext and Cabs in Re(m) and Im(m) of each finite-difference voxel of the
     adj8 fixtures: {voxel: [[dCext/dRe m, dCext/dIm m], [dCabs/dRe m, dCabs/dIm m]]}."""

@@ -581,13 +581,13 @@ template <typename T> using P5Fn = decltype(&k_p5<T, Dyn<8>, false>);
 // ==================================================================================================================
 // matvec driver
 
-static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + 3 * c.LY * rowstride(gX)) * es; }
+static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + 3 * c.LY * rowstride_odd(gX)) * es; }
 static size_t lds_p24(const MvCfg &c, int gZ, size_t es) { return (size_t)(gZ + gZ * c.TC) * es; }
 static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
 	// twiddles + lines (+ the block's D rows for the register variant, which always reserves both planes' lines)
-	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
-	return (size_t)(gY + 3 * (reduced ? 2 : 1) * c.LP * rowstride(gY)) * es;
+	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride_lm(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
+	return (size_t)(gY + 3 * (reduced ? 2 : 1) * c.LP * rowstride_lm(gY)) * es;
 }
 static constexpr size_t kLdsMax = 96 * 1024;
 

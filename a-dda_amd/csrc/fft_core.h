@@ -140,8 +140,20 @@ template <typename T, bool INV, int R> __device__ __forceinline__ void dftR2h(cx
 // stride-R writes of a Stockham stage (R up to 16 complex numbers apart) fall into different LDS banks; column tiles
 // (KS = number of columns, consecutive threads on consecutive columns) need no padding.
 __device__ __forceinline__ int lpos(int k, int KS) { return KS == 1 ? k + (k >> 4) : k * KS; }
-// Row stride of a padded line of length n (one extra element so consecutive lines start in different banks).
-__host__ __device__ __forceinline__ int rowstride(int n) { return n + n / 16 + 1; }
+// Row strides of padded lines of length n (n + n/16 elements at least), chosen by how the lanes of a wave walk the
+// lines. Measured on gfx950 (ds_read_b64 for FP32 complex, ds_read_b128 for FP64 complex):
+//  - line-major (16 consecutive lanes per line, several lines per wave): lines whose starts differ by 16 (mod 32)
+//    elements are conflict-free; at 17 (mod 32) the accesses take 1.6x (b64) to 1.9x (b128) longer;
+//  - line-fastest (consecutive lanes on consecutive lines at the same position): an odd stride spreads the lines over
+//    all banks, while 16 (mod 32) puts every other line on the same banks.
+// Kernels whose transforms run line-major throughout (FFT-y) use rowstride_lm; kernels that also switch to the
+// line-fastest mapping (FFT-x with its transposed global access) use rowstride_odd.
+__host__ __device__ __forceinline__ int rowstride_lm(int n)
+{
+	const int r = n + n / 16;
+	return r + (48 - r % 32) % 32;
+}
+__host__ __device__ __forceinline__ int rowstride_odd(int n) { return (n + n / 16) | 1; }
 
 // Number of butterflies per thread of a stage of radix R when the largest radix of the plan is at most RM: a line has
 // N/RMAX >= N/RM threads, so ceil(RM/R) butterflies per thread suffice. Kernels are instantiated for RM = 16 and
@@ -252,6 +264,8 @@ __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const Fft
 
 // HIN (first stage, even R): inputs j + q*NB with q >= R/2 are zero and are not loaded. HOUT (last stage, even R):
 // only outputs j + q*NS with q < R/2 (indices below N/2) are stored, and the arithmetic of the others is dropped.
+// Twiddles come from this stage's own table tws[(q-1)*NS + k] = exp(-2 pi i q k / (NS R)) (see Fix::load_tw): the
+// threads of a line read consecutive entries, where the natural table's q*k strides conflict in the LDS banks.
 template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, bool HIN, bool HOUT,
 	class Ld, class St>
 __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int LS, int line, int t, bool active,
@@ -259,7 +273,6 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 {
 	constexpr int NB = N / R;                    // butterflies per line
 	constexpr int MU = (RMAX + R - 1) / R;        // butterflies per thread (TPL = N / RMAX threads per line)
-	constexpr int TW = N / (NS * R);              // twiddle stride of this stage
 	constexpr bool HI = FIRST && HIN && R % 2 == 0, HO = LAST && HOUT && R % 2 == 0;
 	cx<T> v[MU][R];
 #pragma unroll
@@ -273,7 +286,7 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 				const int k = j % NS;
 #pragma unroll
 				for (int q = 1; q < R; q++) {
-					const cx<T> w = tws[q * k * TW];
+					const cx<T> w = tws[(q - 1) * NS + k];
 					v[u][q] = mul(v[u][q], INV ? conjc(w) : w);
 				}
 			}
@@ -389,6 +402,14 @@ __device__ __forceinline__ void stage3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx
 template <int RM> struct Dyn {
 	// the HIN / HOUT options are ignored: every input is loaded and every output stored
 	static constexpr bool HALF_IN = false, HALF_OUT = false;
+	static constexpr int RMAXC = 0;  // largest radix unknown at compile time
+	// the natural twiddle table tws[e] = exp(-2 pi i e/N), e < N; ends with a barrier
+	template <typename T>
+	__device__ static __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
+	{
+		for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = twg[i];
+		__syncthreads();
+	}
 	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
 	// HIN / HOUT (zero upper half of the input, unneeded upper half of the output) only apply to compile-time plans
@@ -408,6 +429,37 @@ template <int... Rs> struct Fix {
 	static constexpr int TPL = N / RMAX;
 	// whether HIN skips the inputs at indices >= N/2 and HOUT the outputs there (even first / last radix)
 	static constexpr bool HALF_IN = RAD[0] % 2 == 0, HALF_OUT = RAD[NST - 1] % 2 == 0;
+	static constexpr int RMAXC = RMAX;
+	// Offset of stage S's twiddles in the staged table: stages 1..NST-1 in order, (R_s - 1)*NS_s entries each.
+	static constexpr int twoff(int S)
+	{
+		int o = 0, ns = RAD[0];
+		for (int s = 1; s < S; s++) {
+			o += (RAD[s] - 1) * ns;
+			ns *= RAD[s];
+		}
+		return o;
+	}
+	// Staged twiddle table from the natural one twg[e] = exp(-2 pi i e/N): stage s (span NS, radix R) holds
+	// w[(q-1)*NS + k] = twg[q*k*N/(NS*R)], q = 1..R-1, k < NS. N - RAD[0] entries in all; ends with a barrier.
+	template <typename T, int S, int NS>
+	__device__ static __forceinline__ void fill_tw(cx<T> *tws, const cx<T> *__restrict__ twg)
+	{
+		if constexpr (S < NST) {
+			constexpr int R = RAD[S], TW = N / (NS * R), OFF = twoff(S);
+			for (int e = threadIdx.x; e < (R - 1) * NS; e += blockDim.x) {
+				const int q = e / NS + 1, k = e % NS;
+				tws[OFF + e] = twg[q * k * TW];
+			}
+			fill_tw<T, S + 1, NS * R>(tws, twg);
+		}
+	}
+	template <typename T>
+	__device__ static __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int)
+	{
+		fill_tw<T, 1, RAD[0]>(tws, twg);
+		__syncthreads();
+	}
 	static bool match(const FftPlan2 &p)
 	{
 		if (p.n != N || p.nst != NST) return false;
@@ -423,13 +475,13 @@ template <int... Rs> struct Fix {
 	{
 		constexpr int R = RAD[S];
 		constexpr bool FIRST = S == 0, LAST = S == NST - 1;
-		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT>(buf, tws, KS, LS, FIRST ? lineA : lineB,
-			FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
+		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT>(buf, tws + (FIRST ? 0 : twoff(S)), KS, LS,
+			FIRST ? lineA : lineB, FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
 		if constexpr (!LAST)
 			run<T, INV, HIN, HOUT, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
 	// HIN: inputs at indices >= N/2 are zero (zero-padded data); HOUT: outputs at indices >= N/2 are not needed.
-	// Either may leave the Load / Store functor uncalled for those indices.
+	// Either may leave the Load / Store functor uncalled for those indices. tws: the staged table of load_tw.
 	template <typename T, bool INV, bool HIN = false, bool HOUT = false, class Ld, class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)

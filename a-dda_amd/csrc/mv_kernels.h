@@ -66,9 +66,9 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	Xa += kq * bq.xs2;
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N), NL = 3 * LY;
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride_odd(N), NL = 3 * LY;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
+	PL::template load_tw<T>(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
 	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int ny = bY - y0 < LY ? bY - y0 : LY;  // rows of this tile
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	extern __shared__ char smem_raw[];
 	const int N = PL::n(p), TPL = PL::tpl(p);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
+	PL::template load_tw<T>(tws, twg, N);
 	__builtin_assume(TC >= 2);  // column tiles (element k of column l at buf[k*TC + l]); the launcher guarantees TC >= 2
 	const long q0 = (long)(blockIdx.x / bq.K) * TC;
 	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
@@ -191,28 +191,59 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	const cx<T> *__restrict__ D, int x0, int z, int gX, int bY, int gZ, int DsY, int LP, int NL1, bool has2,
 	int reduced, int transposed, int line, int t, bool act)
 {
-	const int N = PL::n(p), LS = rowstride(N);
-	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
-	__syncthreads();
-	for (int idx = threadIdx.x; idx < LP * N; idx += blockDim.x) {
+	const int N = PL::n(p), LS = rowstride_lm(N);
+	// D-matrix row of element idx = (pair, y) of the block: the stored row index (mirrored for the reduced form and for
+	// the transposed product, src/matvec.c:181 and 270-305) and the y-mirror sign mask of the xy and yz elements
+	auto drow = [&](int idx, unsigned &m) -> const cx<T> * {
 		const int pr = idx / N, y = idx - pr * N;
 		int xi = x0 + pr, yi = y, zi = z;
-		if (transposed) {  // A^T with the full D-matrix: D(-r), src/matvec.c:181 (never with the reduced form)
+		if (transposed) {  // A^T with the full D-matrix: D(-r) (never with the reduced form)
 			xi = xi > 0 ? gX - xi : 0;
 			yi = yi > 0 ? N - yi : 0;
 			zi = zi > 0 ? gZ - zi : 0;
 		}
 		else if (yi >= DsY) yi = N - yi;
-		const cx<T> *dm = D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
-		const cx<T> d0 = dm[0], d2 = dm[2], d3 = dm[3], d5 = dm[5];
-		// y mirror (reduced form): the xy and yz elements change sign; the block's own plane z < DsizeZ never mirrors
-		const unsigned m = (reduced && y >= DsY) ? 0x80000000u : 0u;
-		const cx<T> f1 = flip<T>(dm[1], m), f4 = flip<T>(dm[4], m);
+		m = (reduced && y >= DsY) ? 0x80000000u : 0u;  // the block's own plane z < DsizeZ never mirrors in z
+		return D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
+	};
+	auto dprod = [&](int idx, const cx<T> *d, unsigned m) {
+		const int pr = idx / N, y = idx - pr * N;
+		const cx<T> f1 = flip<T>(d[1], m), f4 = flip<T>(d[4], m);
 		cx<T> *q = buf + (size_t)(3 * pr) * LS + lpos(y, 1);
-		sym3_lds<T, 1>(q, LS, d0, f1, d2, d3, f4, d5);
-		if (has2) sym3_lds<T, -1>(q + (size_t)NL1 * LS, LS, d0, f1, d2, d3, f4, d5);  // plane gZ - z >= DsizeZ
+		sym3_lds<T, 1>(q, LS, d[0], f1, d[2], d[3], f4, d[5]);
+		if (has2) sym3_lds<T, -1>(q + (size_t)NL1 * LS, LS, d[0], f1, d[2], d[3], f4, d[5]);  // plane gZ - z
+	};
+	// In FP32 the D rows of this thread's first KP elements are loaded before the forward transform, which hides their
+	// latency (KP covers all of them with the reduced D-matrix: blockDim = 6*LP*N/RMAX threads for LP*N elements). In
+	// FP64 the 72 extra registers would cost more occupancy than the prefetch gains.
+	constexpr int KP = sizeof(T) == 4 ? (PL::RMAXC + 5) / 6 : 0;
+	cx<T> dpre[KP > 0 ? KP : 1][6];
+	unsigned mpre[KP > 0 ? KP : 1];
+#pragma unroll
+	for (int k = 0; k < KP; k++) {
+		const int idx = threadIdx.x + k * blockDim.x;
+		if (idx < LP * N) {
+			const cx<T> *dm = drow(idx, mpre[k]);
+#pragma unroll
+			for (int e = 0; e < 6; e++) dpre[k][e] = dm[e];
+		}
+	}
+	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
+	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
+	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	__syncthreads();
+#pragma unroll
+	for (int k = 0; k < KP; k++) {
+		const int idx = threadIdx.x + k * blockDim.x;
+		if (idx < LP * N) dprod(idx, dpre[k], mpre[k]);
+	}
+	for (int idx = threadIdx.x + KP * blockDim.x; idx < LP * N; idx += blockDim.x) {
+		unsigned m;
+		const cx<T> *dm = drow(idx, m);
+		cx<T> d[6];
+#pragma unroll
+		for (int e = 0; e < 6; e++) d[e] = dm[e];
+		dprod(idx, d, m);
 	}
 	__syncthreads();
 	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
@@ -233,7 +264,7 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	extern __shared__ char smem_raw[];
 	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
+	PL::template load_tw<T>(tws, twg, N);
 	const int nxt = gX / LP;
 	const int z = tile / nxt, x0 = (tile % nxt) * LP;
 	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; the mirror plane is handled alongside
@@ -297,7 +328,7 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 	if (skip_rhs(bq, kq)) return;
 	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
-	const int LS = rowstride(N);
+	const int LS = rowstride_lm(N);
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N, *Dt = buf + 6 * LP * LS;
 	const int nxt = gX / LP;
 	const int z = tile / nxt, x0 = (tile % nxt) * LP;
@@ -360,9 +391,9 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
 	__shared__ double red_lds[512];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride(N), NL = 3 * LY;
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride_odd(N), NL = 3 * LY;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	load_tw(tws, twg, N);
+	PL::template load_tw<T>(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
 	const int z = tile / nyt, y0 = (tile % nyt) * LY;
 	const int ny = bY - y0 < LY ? bY - y0 : LY;

@@ -403,12 +403,12 @@ template <int RM> struct Dyn {
 	// the HIN / HOUT options are ignored: every input is loaded and every output stored
 	static constexpr bool HALF_IN = false, HALF_OUT = false;
 	static constexpr int RMAXC = 0;  // largest radix unknown at compile time
-	// the natural twiddle table tws[e] = exp(-2 pi i e/N), e < N; ends with a barrier
+	// the natural twiddle table tws[e] = exp(-2 pi i e/N), e < N. No barrier: the caller's next __syncthreads (every
+	// transform reaches one before its first twiddle) publishes the table, so its loads overlap the caller's.
 	template <typename T>
 	__device__ static __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
 	{
 		for (int i = threadIdx.x; i < N; i += blockDim.x) tws[i] = twg[i];
-		__syncthreads();
 	}
 	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
@@ -441,7 +441,7 @@ template <int... Rs> struct Fix {
 		return o;
 	}
 	// Staged twiddle table from the natural one twg[e] = exp(-2 pi i e/N): stage s (span NS, radix R) holds
-	// w[(q-1)*NS + k] = twg[q*k*N/(NS*R)], q = 1..R-1, k < NS. N - RAD[0] entries in all; ends with a barrier.
+	// w[(q-1)*NS + k] = twg[q*k*N/(NS*R)], q = 1..R-1, k < NS. N - RAD[0] entries in all. No barrier (as Dyn).
 	template <typename T, int S, int NS>
 	__device__ static __forceinline__ void fill_tw(cx<T> *tws, const cx<T> *__restrict__ twg)
 	{
@@ -458,7 +458,6 @@ template <int... Rs> struct Fix {
 	__device__ static __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int)
 	{
 		fill_tw<T, 1, RAD[0]>(tws, twg);
-		__syncthreads();
 	}
 	static bool match(const FftPlan2 &p)
 	{

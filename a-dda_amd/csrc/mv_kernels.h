@@ -78,29 +78,36 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	const bool actA = lineA < 3 * ny, actB = tB < TPL && lineB < 3 * ny;
 	// the box fills at most half of the line (gridX >= 2*boxX); with a compile-time plan the transform never reads the
 	// upper half, so it is not zero-filled
-	const int NF = PL::HALF_IN ? N / 2 : N;
-	for (int idx = tid; idx < LY * NF; idx += blockDim.x) {
-		const int yl = idx / NF, x = idx % NF;
-		cx<T> v0 = mk<T>(0, 0), v1 = v0, v2 = v0;
-		if (x < bX && yl < ny) {
-			const int d = cellmap[((size_t)z * bY + y0 + yl) * bX + x];
-			if (d >= 0) {
-				const cx<T> *a = arg + 3 * (size_t)d, *cs = ccs + 3 * mat[d];
-				cx<T> a0 = a[0], a1 = a[1], a2 = a[2];
-				if (her) {
-					a0 = conjc(a0);
-					a1 = conjc(a1);
-					a2 = conjc(a2);
-				}
-				v0 = mul(cs[0], a0);
-				v1 = mul(cs[1], a1);
-				v2 = mul(cs[2], a2);
+	const int NF = PL::HALF_IN ? N / 2 : N, total = LY * NF;
+	// GU elements per thread and round: the cell-map reads of all of them are issued together, then their argument and
+	// material reads, then the coupling constants -- three dependent loads in flight for GU elements at a time.
+	constexpr int GU = 4;
+	const int *cm = cellmap + ((size_t)z * bY + y0) * bX;
+	for (int b0 = tid; b0 < total; b0 += GU * blockDim.x) {
+		int dd[GU], mm[GU] = {};
+		cx<T> av[GU][3];
+#pragma unroll
+		for (int u = 0; u < GU; u++) {
+			const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
+			dd[u] = (idx < total && x < bX && yl < ny) ? cm[yl * bX + x] : -1;
+		}
+#pragma unroll
+		for (int u = 0; u < GU; u++)
+			if (dd[u] >= 0) {
+				const cx<T> *a = arg + 3 * (size_t)dd[u];
+				mm[u] = 3 * mat[dd[u]];
+#pragma unroll
+				for (int c = 0; c < 3; c++) av[u][c] = her ? conjc(a[c]) : a[c];
+			}
+#pragma unroll
+		for (int u = 0; u < GU; u++) {
+			const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
+			if (idx < total) {
+				cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
+#pragma unroll
+				for (int c = 0; c < 3; c++) q[c * LS] = dd[u] >= 0 ? mul(ccs[mm[u] + c], av[u][c]) : mk<T>(0, 0);
 			}
 		}
-		cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
-		q[0] = v0;
-		q[LS] = v1;
-		q[2 * LS] = v2;
 	}
 	__syncthreads();
 	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
@@ -409,28 +416,49 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	// only x < boxX <= N/2 is gathered below: the inverse transform computes the lower half of each line
 	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 	__syncthreads();
-	for (int idx = tid; idx < LY * bX; idx += blockDim.x) {
-		const int yl = idx / bX, x = idx % bX;
-		if (yl >= ny) continue;
-		const int d = cellmap[((size_t)z * bY + y0 + yl) * bX + x];
-		if (d < 0) continue;
-		const cx<T> *a = arg + 3 * (size_t)d, *cs = ccs + 3 * mat[d];
-		cx<T> *r = res + 3 * (size_t)d;
-		const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
+	// x < boxX <= N/2: with a half-output plan the rows are walked at the constant stride N/2 (no division)
+	const int NF = PL::HALF_OUT ? N / 2 : bX, total = LY * NF;
+	constexpr int GU = 4;  // elements per thread and round, their dependent loads issued together (as in k_p1)
+	const int *cm = cellmap + ((size_t)z * bY + y0) * bX;
+	for (int b0 = tid; b0 < total; b0 += GU * blockDim.x) {
+		int dd[GU], mm[GU] = {};
+		cx<T> av[GU][3], wv[GU][3];
 #pragma unroll
-		for (int c = 0; c < 3; c++) {
-			cx<T> av = a[c];
-			if (her) av = conjc(av);
-			cx<T> rv = add(av, mul(cs[c], q[c * LS]));
-			if (her) rv = conjc(rv);
-			r[c] = rv;
-			s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
-			if (DOT) {
-				const cx<T> w = dmode == 1 ? dotw[3 * (size_t)d + c] : a[c];
-				dr += (double)rv.x * w.x + (double)rv.y * w.y;
-				di += (double)rv.y * w.x - (double)rv.x * w.y;
-			}
+		for (int u = 0; u < GU; u++) {
+			const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
+			dd[u] = (idx < total && x < bX && yl < ny) ? cm[yl * bX + x] : -1;
 		}
+#pragma unroll
+		for (int u = 0; u < GU; u++)
+			if (dd[u] >= 0) {
+				const cx<T> *a = arg + 3 * (size_t)dd[u];
+				mm[u] = 3 * mat[dd[u]];
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					av[u][c] = a[c];
+					if (DOT) wv[u][c] = dmode == 1 ? dotw[3 * (size_t)dd[u] + c] : a[c];
+				}
+			}
+#pragma unroll
+		for (int u = 0; u < GU; u++)
+			if (dd[u] >= 0) {
+				const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
+				const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
+				cx<T> *r = res + 3 * (size_t)dd[u];
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					const cx<T> a = her ? conjc(av[u][c]) : av[u][c];
+					cx<T> rv = add(a, mul(ccs[mm[u] + c], q[c * LS]));
+					if (her) rv = conjc(rv);
+					r[c] = rv;
+					s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+					if (DOT) {
+						const cx<T> w = wv[u][c];
+						dr += (double)rv.x * w.x + (double)rv.y * w.y;
+						di += (double)rv.y * w.x - (double)rv.x * w.y;
+					}
+				}
+			}
 	}
 	if (partials) {
 		const int rows = DOT ? 3 : 1;

@@ -83,6 +83,7 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	// material reads, then the coupling constants -- three dependent loads in flight for GU elements at a time.
 	constexpr int GU = 4;
 	const int *cm = cellmap + ((size_t)z * bY + y0) * bX;
+	const bool onemat = bq.cs == 3;  // one material: no per-dipole material lookup
 	for (int b0 = tid; b0 < total; b0 += GU * blockDim.x) {
 		int dd[GU], mm[GU] = {};
 		cx<T> av[GU][3];
@@ -95,7 +96,7 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 		for (int u = 0; u < GU; u++)
 			if (dd[u] >= 0) {
 				const cx<T> *a = arg + 3 * (size_t)dd[u];
-				mm[u] = 3 * mat[dd[u]];
+				mm[u] = onemat ? 0 : 3 * mat[dd[u]];
 #pragma unroll
 				for (int c = 0; c < 3; c++) av[u][c] = her ? conjc(a[c]) : a[c];
 			}
@@ -409,56 +410,72 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest (writes the LDS lines)
 	const bool actA = tA < TPL && lineA < 3 * ny, actB = lineB < 3 * ny;
 	double s = 0, dr = 0, di = 0;
-	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
-	const size_t xs = (size_t)3 * bY;
-	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
-	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
-	// only x < boxX <= N/2 is gathered below: the inverse transform computes the lower half of each line
-	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
-	__syncthreads();
 	// x < boxX <= N/2: with a half-output plan the rows are walked at the constant stride N/2 (no division)
 	const int NF = PL::HALF_OUT ? N / 2 : bX, total = LY * NF;
-	constexpr int GU = 4;  // elements per thread and round, their dependent loads issued together (as in k_p1)
+	// Elements per thread and round: with a compile-time plan one round covers the block (LY*NF <= 3*LY*TPL*RMAX/6).
+	constexpr int GU = PL::RMAXC > 0 ? (PL::RMAXC + 5) / 6 : 4;
 	const int *cm = cellmap + ((size_t)z * bY + y0) * bX;
-	for (int b0 = tid; b0 < total; b0 += GU * blockDim.x) {
-		int dd[GU], mm[GU] = {};
-		cx<T> av[GU][3], wv[GU][3];
+	const bool onemat = bq.cs == 3;  // one material: the coupling constants are the same for every dipole
+	struct Round {
+		int dd[GU];
+		cx<T> av[GU][3], cs[GU][3], wv[GU][3];
+	};
+	// Everything a round needs besides the transform (cell map, argument, coupling constants, the dot weights) is
+	// independent of it: the first round's loads are issued before the transform, which hides their latency.
+	auto fetch = [&](int b0, Round &g) {
 #pragma unroll
 		for (int u = 0; u < GU; u++) {
 			const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
-			dd[u] = (idx < total && x < bX && yl < ny) ? cm[yl * bX + x] : -1;
+			g.dd[u] = (idx < total && x < bX && yl < ny) ? cm[yl * bX + x] : -1;
 		}
 #pragma unroll
 		for (int u = 0; u < GU; u++)
-			if (dd[u] >= 0) {
-				const cx<T> *a = arg + 3 * (size_t)dd[u];
-				mm[u] = 3 * mat[dd[u]];
+			if (g.dd[u] >= 0) {
+				const cx<T> *a = arg + 3 * (size_t)g.dd[u], *c3 = ccs + (onemat ? 0 : 3 * mat[g.dd[u]]);
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
-					av[u][c] = a[c];
-					if (DOT) wv[u][c] = dmode == 1 ? dotw[3 * (size_t)dd[u] + c] : a[c];
+					g.av[u][c] = a[c];
+					g.cs[u][c] = c3[c];
+					if (DOT) g.wv[u][c] = dmode == 1 ? dotw[3 * (size_t)g.dd[u] + c] : a[c];
 				}
 			}
+	};
+	auto finish = [&](int b0, const Round &g) {
 #pragma unroll
 		for (int u = 0; u < GU; u++)
-			if (dd[u] >= 0) {
+			if (g.dd[u] >= 0) {
 				const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
 				const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
-				cx<T> *r = res + 3 * (size_t)dd[u];
+				cx<T> *r = res + 3 * (size_t)g.dd[u];
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
-					const cx<T> a = her ? conjc(av[u][c]) : av[u][c];
-					cx<T> rv = add(a, mul(ccs[mm[u] + c], q[c * LS]));
+					const cx<T> a = her ? conjc(g.av[u][c]) : g.av[u][c];
+					cx<T> rv = add(a, mul(g.cs[u][c], q[c * LS]));
 					if (her) rv = conjc(rv);
 					r[c] = rv;
 					s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
 					if (DOT) {
-						const cx<T> w = wv[u][c];
+						const cx<T> w = g.wv[u][c];
 						dr += (double)rv.x * w.x + (double)rv.y * w.y;
 						di += (double)rv.y * w.x - (double)rv.x * w.y;
 					}
 				}
 			}
+	};
+	Round g0;
+	fetch(tid, g0);
+	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
+	const size_t xs = (size_t)3 * bY;
+	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
+	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
+	// only x < boxX <= N/2 is gathered: the inverse transform computes the lower half of each line
+	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	__syncthreads();
+	finish(tid, g0);
+	for (int b0 = tid + GU * blockDim.x; b0 < total; b0 += GU * blockDim.x) {  // run-time plans only
+		Round g;
+		fetch(b0, g);
+		finish(b0, g);
 	}
 	if (partials) {
 		const int rows = DOT ? 3 : 1;

@@ -266,8 +266,12 @@ __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const Fft
 // only outputs j + q*NS with q < R/2 (indices below N/2) are stored, and the arithmetic of the others is dropped.
 // Twiddles come from this stage's own table tws[(q-1)*NS + k] = exp(-2 pi i q k / (NS R)) (see Fix::load_tw): the
 // threads of a line read consecutive entries, where the natural table's q*k strides conflict in the LDS banks.
+// GIN / GOUT: the first stage's Load reads global memory (not the LDS buffer) / the last stage's Store writes global
+// memory. Such a stage needs no barrier between its reads and its writes: nothing it writes can clobber an LDS input
+// of another thread. (With GIN the caller must have no pending reads of the buffer; with GOUT the transform ends
+// without a barrier, and the caller synchronises before reusing the buffer.)
 template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, bool HIN, bool HOUT,
-	class Ld, class St>
+	bool GIN, bool GOUT, class Ld, class St>
 __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int LS, int line, int t, bool active,
 	Ld &ld, St &st)
 {
@@ -294,7 +298,8 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 			else dftR2<T, INV, R>(v[u]);
 		}
 	}
-	__syncthreads();  // a FIRST stage's Load may read the LDS buffer it is about to overwrite
+	// every LDS read of this stage (by any thread) precedes its LDS writes
+	if constexpr (!(FIRST && GIN) && !(LAST && GOUT)) __syncthreads();
 	if (LAST) {
 #pragma unroll
 		for (int u = 0; u < MU; u++) {
@@ -412,8 +417,10 @@ template <int RM> struct Dyn {
 	}
 	__device__ static __forceinline__ int n(const FftPlan2 &p) { return p.n; }
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
-	// HIN / HOUT (zero upper half of the input, unneeded upper half of the output) only apply to compile-time plans
-	template <typename T, bool INV, bool HIN = false, bool HOUT = false, class Ld, class St>
+	// HIN / HOUT (zero upper half of the input, unneeded upper half of the output) and GIN / GOUT (global Load /
+	// Store, fewer barriers) only apply to compile-time plans: the run-time path keeps every barrier
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false, class Ld,
+		class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
@@ -468,24 +475,27 @@ template <int... Rs> struct Fix {
 	}
 	__device__ static __forceinline__ constexpr int n(const FftPlan2 &) { return N; }
 	__device__ static __forceinline__ constexpr int tpl(const FftPlan2 &) { return TPL; }
-	template <typename T, bool INV, bool HIN, bool HOUT, int S, int NS, class Ld, class St>
+	template <typename T, bool INV, bool HIN, bool HOUT, bool GIN, bool GOUT, int S, int NS, class Ld, class St>
 	__device__ static __forceinline__ void run(cx<T> *buf, const cx<T> *tws, int KS, int LS, int lineA, int tA,
 		bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
 		constexpr int R = RAD[S];
 		constexpr bool FIRST = S == 0, LAST = S == NST - 1;
-		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT>(buf, tws + (FIRST ? 0 : twoff(S)), KS, LS,
-			FIRST ? lineA : lineB, FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
+		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT, GIN, GOUT>(buf, tws + (FIRST ? 0 : twoff(S)), KS,
+			LS, FIRST ? lineA : lineB, FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
 		if constexpr (!LAST)
-			run<T, INV, HIN, HOUT, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+			run<T, INV, HIN, HOUT, GIN, GOUT, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB,
+				ld, st);
 	}
 	// HIN: inputs at indices >= N/2 are zero (zero-padded data); HOUT: outputs at indices >= N/2 are not needed.
-	// Either may leave the Load / Store functor uncalled for those indices. tws: the staged table of load_tw.
-	template <typename T, bool INV, bool HIN = false, bool HOUT = false, class Ld, class St>
+	// Either may leave the Load / Store functor uncalled for those indices. GIN / GOUT: the Load reads / the Store
+	// writes global memory, not the LDS buffer (see sstage). tws: the staged table of load_tw.
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false, class Ld,
+		class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
-		run<T, INV, HIN, HOUT, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+		run<T, INV, HIN, HOUT, GIN, GOUT, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
 
 	// Three-line transforms for the register D-product (stage3): the forward FFT takes the radices in plan order,

@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineB;
 	const size_t xs = (size_t)3 * bY;
 	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
-	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, false, true, false, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 }
 
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
@@ -142,14 +142,14 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	if ((INV ? PL::HALF_OUT : PL::HALF_IN) && (INV ? 2 * n_out == N : 2 * n_in == N)) {
 		auto ld = [&](int, int k) -> cx<T> { return ip[(size_t)k * Q]; };
 		auto st = [&](int, int k, cx<T> v) { op[(size_t)k * Q] = v; };
-		PL::template fft<T, INV, !INV, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+		PL::template fft<T, INV, !INV, INV, true, true>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
 	}
 	else {
 		auto ld = [&](int, int k) -> cx<T> { return k < n_in ? ip[(size_t)k * Q] : mk<T>(0, 0); };
 		auto st = [&](int, int k, cx<T> v) {
 			if (k < n_out) op[(size_t)k * Q] = v;
 		};
-		PL::template fft<T, INV, !INV, INV>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
+		PL::template fft<T, INV, !INV, INV, true, true>(buf, tws, p, TC, 1, line, t, act, line, t, act, ld, st);
 	}
 }
 
@@ -238,7 +238,7 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	}
 	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
 	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	PL::template fft<T, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	PL::template fft<T, false, true, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
 	__syncthreads();
 #pragma unroll
 	for (int k = 0; k < KP; k++) {
@@ -258,7 +258,7 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	auto stg = [&](int, int y, cx<T> v) {
 		if (FULL || y < bY) lp[3 * y] = v;
 	};
-	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+	PL::template fft<T, true, false, true, false, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
 template <typename T, class PL>
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
 	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
 	// only x < boxX <= N/2 is gathered: the inverse transform computes the lower half of each line
-	PL::template fft<T, true, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, true, false, true, true, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
 	__syncthreads();
 	finish(tid, g0);
 	for (int b0 = tid + GU * blockDim.x; b0 < total; b0 += GU * blockDim.x) {  // run-time plans only

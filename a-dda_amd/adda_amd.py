@@ -78,6 +78,7 @@ def hip():
     L.adda_hip_matvec_bytes.restype = D
     L.adda_hip_get_config.argtypes = [P, I, ctypes.c_char_p, S]
     L.adda_hip_set_config.argtypes = [P, I, ctypes.c_char_p]
+    L.adda_hip_set_matvec_mode.argtypes = [P, I]
     L.adda_hip_autotune.argtypes = [P, I, I, ctypes.c_char_p, ctypes.c_char_p, S]
     L.adda_hip_dk_create.argtypes = [P, I]
     L.adda_hip_dk_create.restype = P
@@ -229,6 +230,10 @@ class Operator:
     def set_config(self, cfg, prec=F64):
         check(hip().adda_hip_set_config(self.h, prec, cfg.encode()))
 
+    def set_matvec_mode(self, mode):
+        """1: FP64 matvecs run their transforms in FP32 (FP64 vectors in and out); 0: in FP64."""
+        check(hip().adda_hip_set_matvec_mode(self.h, mode))
+
     def autotune(self, prec=F64, reps=3, cache=None):
         """Pick the fastest radix schedules and tiles per pass (HIP-event timing); returns the report string."""
         b = ctypes.create_string_buffer(512)
@@ -299,7 +304,7 @@ IT_QMR, IT_BICGSTAB, IT_CGNR, IT_BICG, IT_CSYM = 0, 1, 2, 3, 4
 ITER_NAMES = {"qmr": IT_QMR, "bicgstab": IT_BICGSTAB, "cgnr": IT_CGNR, "bicg": IT_BICG, "csym": IT_CSYM}
 IF_AUTO, IF_ZERO, IF_INC = 0, 1, 2
 POL_LDR, POL_CM, POL_RRC = 0, 1, 2
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_V64 = 0, 1, 2
 
 
 class Particle(ctypes.Structure):

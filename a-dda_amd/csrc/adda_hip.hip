@@ -139,6 +139,7 @@ struct adda_hip_op {
 	int DsY, DsZ;
 	double wavenum, gridspace;
 	int flags;
+	int mv32 = 0;  // FP64 matvecs run their transforms in FP32 (adda_hip_set_matvec_mode)
 	// geometry
 	int *cellmap = nullptr;  // [bZ][bY][bX] -> dipole index or -1
 	int *mat = nullptr;      // [nd]
@@ -577,6 +578,8 @@ template <typename T> using P1Fn = decltype(&k_p1<T, Dyn<8>>);
 template <typename T> using ZFn = decltype(&k_zfft<T, false, Dyn<8>>);
 template <typename T> using P3Fn = decltype(&k_p3<T, Dyn<8>>);
 template <typename T> using P5Fn = decltype(&k_p5<T, Dyn<8>, false>);
+using P1WFn = decltype(&k_p1<float, Dyn<8>, double>);
+using P5WFn = decltype(&k_p5<float, Dyn<8>, false, double>);
 
 // ==================================================================================================================
 // matvec driver
@@ -620,12 +623,14 @@ static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, cons
 	return b;
 }
 
-template <typename T>
-static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
+// TV: the dipole-vector type (double with an FP32 transform: the MV_P1W / MV_P5W kernels)
+template <typename T, typename TV>
+static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<TV> *arg,
 	const cx<T> *ccs, int her)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
-	auto k = (P1Fn<T>)c.kx.k[MV_P1][sizeof(T) == 4];
+	using Fn = std::conditional_t<std::is_same_v<T, TV>, P1Fn<T>, P1WFn>;
+	auto k = (Fn)c.kx.k[std::is_same_v<T, TV> ? MV_P1 : MV_P1W][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(op->bZ * nyt * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
 		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx,
 		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
@@ -657,12 +662,14 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 }
 
 // returns the number of P5 blocks per RHS (the row length of its partial sums)
-template <typename T>
-static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<T> *arg,
-	cx<T> *res, const cx<T> *ccs, int her, double *partials, const cx<T> *dotw = nullptr, int dmode = 0)
+template <typename T, typename TV>
+static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<TV> *arg,
+	cx<TV> *res, const cx<T> *ccs, int her, double *partials, const cx<TV> *dotw = nullptr, int dmode = 0)
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = op->bZ * nyt;
-	auto k = (P5Fn<T>)c.kx.k[dmode ? MV_P5D : MV_P5][sizeof(T) == 4];
+	constexpr bool same = std::is_same_v<T, TV>;
+	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
+	auto k = (Fn)c.kx.k[dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, c.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
@@ -679,10 +686,9 @@ template <> const cx<float> *cc_of<float>(const adda_hip_op *op) { return op->cc
 // (layout [k][nblk]) for a device-side reduction (no synchronisation).
 // dmode (with keep_partials): 1 also accumulates sum result.conj(dotw), 2 sum result.conj(arg), as partial rows 1, 2
 // after the ||result||^2 row (per RHS: 3 rows of nblk).
-template <typename T>
-static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const cx<T> *arg, cx<T> *res,
-	double *inprod, int her, hipEvent_t *ev = nullptr, bool keep_partials = false, int *nblk_out = nullptr,
-	const cx<T> *dotw = nullptr, int dmode = 0)
+template <typename T, typename TV>
+static int matvec_io(adda_hip_op *op, int K, long vs, const double *gate, const cx<TV> *arg, cx<TV> *res,
+	double *inprod, int her, hipEvent_t *ev, bool keep_partials, int *nblk_out, const cx<TV> *dotw, int dmode)
 {
 	const cx<T> *ccs = cc_of<T>(op);
 	const cx<T> *D;
@@ -693,7 +699,7 @@ static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const c
 	const MvCfg &c = op->cfg[sizeof(T) == 8 ? ADDA_F64 : ADDA_F32];
 	const int transposed = (!op->reduced) && her;  // src/matvec.c:181
 	if (ev) HIPCHK(hipEventRecord(ev[0], op->stream));
-	launch_p1<T>(op, c, K, vs, gate, arg, ccs, her);
+	launch_p1<T, TV>(op, c, K, vs, gate, arg, ccs, her);
 	if (ev) HIPCHK(hipEventRecord(ev[1], op->stream));
 	launch_zfft<T, false>(op, c, K, gate);
 	if (ev) HIPCHK(hipEventRecord(ev[2], op->stream));
@@ -701,13 +707,27 @@ static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const c
 	if (ev) HIPCHK(hipEventRecord(ev[3], op->stream));
 	launch_zfft<T, true>(op, c, K, gate);
 	if (ev) HIPCHK(hipEventRecord(ev[4], op->stream));
-	const int nblk5 = launch_p5<T>(op, c, K, vs, gate, arg, res, ccs, her,
+	const int nblk5 = launch_p5<T, TV>(op, c, K, vs, gate, arg, res, ccs, her,
 		(inprod || keep_partials) ? op->partials : nullptr, dotw, keep_partials ? dmode : 0);
 	if (nblk_out) *nblk_out = nblk5;
 	if (ev) HIPCHK(hipEventRecord(ev[5], op->stream));
 	HIPCHK(hipGetLastError());
 	if (inprod) return reduce_finish(op, nblk5, K, inprod);
 	return 0;
+}
+
+// The matvec of vectors of type T: in T, or (FP64 vectors with the operator's FP32-transform mode) FP32 transforms
+// between FP64 vectors.
+template <typename T>
+static int matvec_t(adda_hip_op *op, int K, long vs, const double *gate, const cx<T> *arg, cx<T> *res,
+	double *inprod, int her, hipEvent_t *ev = nullptr, bool keep_partials = false, int *nblk_out = nullptr,
+	const cx<T> *dotw = nullptr, int dmode = 0)
+{
+	if constexpr (sizeof(T) == 8)
+		if (op->mv32)
+			return matvec_io<float, double>(op, K, vs, gate, arg, res, inprod, her, ev, keep_partials, nblk_out, dotw,
+				dmode);
+	return matvec_io<T, T>(op, K, vs, gate, arg, res, inprod, her, ev, keep_partials, nblk_out, dotw, dmode);
 }
 
 static void resolve(MvCfg &c)
@@ -2026,6 +2046,15 @@ extern "C" int adda_hip_get_config(adda_hip_op *op, int prec, char *buf, size_t 
 {
 	if (!op || !buf || len < 8) return ERR("bad argument");
 	cfg_str(op->cfg[prec == ADDA_F32], buf, len);
+	return 0;
+}
+
+extern "C" int adda_hip_set_matvec_mode(adda_hip_op *op, int mode)
+{
+	if (!op) return ERR("null argument");
+	if (mode != 0 && mode != 1) return ERR("unknown matvec mode %d", mode);
+	if (mode == 1 && !op->D32) return ERR("FP32 transforms need the operator's FP32 D-matrix (create it with flags & 1)");
+	op->mv32 = mode;
 	return 0;
 }
 

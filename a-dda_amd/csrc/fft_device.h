@@ -21,6 +21,8 @@ template <typename T> __device__ __forceinline__ cx<T> mul(cx<T> a, cx<T> b)
 	return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T> __device__ __forceinline__ cx<T> conjc(cx<T> a) { return mk<T>(a.x, -a.y); }
+// precision conversion (identity when the types agree)
+template <typename T, typename U> __device__ __forceinline__ cx<T> cvt(cx<U> a) { return mk<T>((T)a.x, (T)a.y); }
 template <typename T> __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
 // multiply by -i (forward) or +i (inverse)
 template <typename T, bool INV> __device__ __forceinline__ cx<T> mulmi(cx<T> a)

@@ -4,7 +4,7 @@
  *            [-dpl <n>] [-lambda <l>] [-m <re> <im> [<re> <im> ...]] [-iter qmr|bicgstab|cgnr|bicg|csym]
  *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
- *            [-prec fp64|mixed] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
+ *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
  *
  * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
  * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
@@ -121,7 +121,9 @@ int main(int argc, char **argv)
 		else if (!strcmp(a, "-graph")) opt.graph = 1;
 		else if (!strcmp(a, "-prec")) {
 			NEED(1);
-			opt.precision = !strcmp(argv[++i], "mixed") ? ADDA_PREC_MIXED : ADDA_PREC_FP64;
+			++i;
+			opt.precision = !strcmp(argv[i], "mixed") ? ADDA_PREC_MIXED : !strcmp(argv[i], "mixed64") ?
+				ADDA_PREC_MIXED_V64 : ADDA_PREC_FP64;
 		}
 		else { fprintf(stderr, "ERROR: unknown option %s\n", a); usage(); }
 	}
@@ -175,10 +177,12 @@ int main(int argc, char **argv)
 	fprintf(log, "Total number of occupied dipoles: %zu\nVolume-equivalent size parameter: %.10g\n", p.nd,
 		wavenum * a_eq);
 	fprintf(log, "The FFT grid is: %dx%dx%d\nPrecision: %s\n", gx, gy, gz,
-		opt.precision == ADDA_PREC_MIXED ? "FP32 inner iterations, FP64 defect correction" : "FP64");
+		opt.precision == ADDA_PREC_MIXED ? "FP32 inner iterations, FP64 defect correction" :
+		opt.precision == ADDA_PREC_MIXED_V64 ? "FP64 inner iterations with FP32 transforms, FP64 defect correction" :
+		"FP64");
 	double tD = now();
 	adda_hip_op *op = adda_hip_op_create(gpu, p.boxX, p.boxY, p.boxZ, gx, gy, gz, p.nd, p.position, p.material, p.nmat,
-		wavenum, gridspace, reduced, opt.precision == ADDA_PREC_MIXED);
+		wavenum, gridspace, reduced, opt.precision != ADDA_PREC_FP64);
 	if (!op) die("creating the operator");
 	tD = now() - tD;
 	void *einc = adda_hip_vec_alloc(op, ADDA_F64), *x = adda_hip_vec_alloc(op, ADDA_F64),

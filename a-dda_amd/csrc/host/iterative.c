@@ -6,12 +6,14 @@
  * plain solver, FP32 for the inner solves of the mixed-precision form, whose outer loop recomputes the true residual
  * r = b - A.x in FP64 (ResidualNorm2, src/iterative.c:322-338) and accumulates x += e in FP64 (defect correction).
  */
+#define _POSIX_C_SOURCE 200809L /* clock_gettime */
 #include <complex.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../../include/adda.h"
 #include "host_internal.h"
@@ -417,7 +419,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 		s[9] = hl;
 		if (hist && hist_len > 0) hist[(size_t)k * hist_len] = excl ? 0 : sqrt(inprodR[k] / temp[k]);
 	}
-	if (verbose && K == 1) printf("RE_000 = %.10E\n", sqrt(inprodR[0] / temp[0]));
+	if ((verbose & 1) && K == 1) printf("RE_000 = %.10E\n", sqrt(inprodR[0] / temp[0]));
 	/* PHASE_INIT, written into the method slots (layout: the k_dk_* kernels of adda_hip.hip) */
 	if (method == ADDA_IT_QMR_CS) {
 		CKG(adda_hip_dot_batch(op, prec, K, st, r, NULL, ADDA_DOT_SELF_BILIN, t, NULL));
@@ -449,6 +451,8 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 		void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;
 		int it = 1, running = 0;
 		for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+		struct timespec tp0, tp1;
+		clock_gettime(CLOCK_MONOTONIC, &tp0);
 		while (running) {
 			if (use_graph && it > 3) {
 				if (!graph) {
@@ -470,6 +474,23 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			CKG(adda_hip_dk_get(op, st, 0, S, K * NS));
 			running = 0;
 			for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+			if (verbose & 2) { /* progress of a long solve: the largest relative residual of the running RHS */
+				clock_gettime(CLOCK_MONOTONIC, &tp1);
+				if (tp1.tv_sec - tp0.tv_sec + 1e-9 * (tp1.tv_nsec - tp0.tv_nsec) >= 10) {
+					double worst = 0;
+					int nrun = 0;
+					for (int k = 0; k < K; k++) {
+						const double *sk = S + (size_t)k * NS;
+						if (sk[ADDA_DK_STATUS] != 0) continue;
+						nrun++;
+						if (sqrt(sk[0] * sk[3]) > worst) worst = sqrt(sk[0] * sk[3]);
+					}
+					printf("%s %s: iteration %d, %d RHS running, residual %.3E\n", method_name(method),
+						prec == ADDA_F32 ? "FP32" : "FP64", it - 1, nrun, worst);
+					fflush(stdout);
+					tp0 = tp1;
+				}
+			}
 		}
 		if (hist && hist_len > 1) {
 			hbuf = malloc((size_t)K * hl * sizeof(double));
@@ -491,7 +512,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			res[k].nmv += init_mv;
 			if (hbuf)
 				for (int i = 1; i < niter && i < hist_len; i++) hist[(size_t)k * hist_len + i] = hbuf[(size_t)k * hl + i];
-			if (verbose && K == 1 && hist)
+			if ((verbose & 1) && K == 1 && hist)
 				for (int i = 1; i < niter && i < hist_len; i++) printf("RE_%03d = %.10E\n", i, hist[i]);
 			if (status == 2) res[k].rc = 1;
 			else if (status == -1) {
@@ -536,7 +557,7 @@ static int krylov_run(adda_hip_op *op, const adda_solver_opts *opt, int prec, in
 	kctx k = {op, prec, 0};
 	int ni = 0;
 	double rel = 0;
-	int rc = krylov(&k, opt->method, eps[0], maxiter, init_field, verbose, b, x, hist, hist_len, &ni, &rel);
+	int rc = krylov(&k, opt->method, eps[0], maxiter, init_field, verbose & 1, b, x, hist, hist_len, &ni, &rel);
 	res[0].rc = rc;
 	res[0].niter = ni;
 	res[0].nmv = k.nmv;
@@ -598,7 +619,8 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		}
 	}
 	else {
-		/* FP64 defect correction around FP32 Krylov solves, per RHS:
+		/* FP64 defect correction around inner Krylov solves (FP32 ones, or with ADDA_PREC_MIXED_V64 FP64 ones whose
+		 * matvecs run their transforms in FP32), per RHS:
 		 *   x = 0; r = b; repeat { e ~ A^-1 (r/|r|) (FP32, relative tol inner_eps); x' = x + |r| e;
 		 *                          r' = b - A.x' (FP64); accept x', r' if |r'| < |r| }
 		 * until ||r||/||b|| <= eps. Inner solves start from e = 0 on the normalized residual. All RHS still
@@ -606,6 +628,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		 * that does not reduce the FP64 residual is rejected, and that RHS (like one still above eps after max_outer
 		 * steps) finishes with an FP64 Krylov solve of the same correction equation -- the FP64 residual decides
 		 * every outcome, so no RHS ends on an unchecked FP32 result. */
+		const int v64 = opt->precision == ADDA_PREC_MIXED_V64;
 		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-5;
 		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
 		const int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
@@ -614,14 +637,16 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		Ax = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
 		e64 = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
 		xt = adda_hip_vec_alloc_batch(op, ADDA_F64, K);
-		r32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
-		e32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
+		if (!v64) {
+			r32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
+			e32 = adda_hip_vec_alloc_batch(op, ADDA_F32, K);
+		}
 		og = adda_hip_dk_create_batch(op, K, 2);
 		og_h = calloc((size_t)K * NS, sizeof(double));
 		fin = calloc(K, sizeof(int));
 		acc = calloc(K, sizeof(int));
 		if (resid_hist && hist_len > 0) ih = malloc((size_t)K * ihl * sizeof(double));
-		if (!r64 || !Ax || !e64 || !xt || !r32 || !e32 || !og) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
+		if (!r64 || !Ax || !e64 || !xt || (!v64 && (!r32 || !e32)) || !og) { rc = host_err_hip(__FILE__, __LINE__); goto out; }
 		if (!og_h || !fin || !acc || (resid_hist && hist_len > 0 && !ih)) {
 			rc = host_err(__FILE__, __LINE__, "out of host memory");
 			goto out;
@@ -667,10 +692,19 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 				if (eps[k] < inner) eps[k] = inner;
 			}
 			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, Ax, r64, ca, NULL, cz, NULL, cz, 0, NULL));
-			CKB(adda_hip_vec_convert_batch(op, K, r32, ADDA_F32, Ax, ADDA_F64));
-			/* per-RHS codes in kr: an FP32 breakdown or stagnation ends that RHS's step; the FP64 residual below
+			/* per-RHS codes in kr: an inner breakdown or stagnation ends that RHS's step; the FP64 residual below
 			 * decides what happens next */
-			krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, 0, active, r32, e32, ih, ih ? ihl : 0, kr);
+			if (v64) { /* FP64 vectors and recurrences, FP32 transforms */
+				CKB(adda_hip_set_matvec_mode(op, 1));
+				krylov_run(op, opt, ADDA_F64, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, Ax, e64, ih,
+					ih ? ihl : 0, kr);
+				CKB(adda_hip_set_matvec_mode(op, 0));
+			}
+			else {
+				CKB(adda_hip_vec_convert_batch(op, K, r32, ADDA_F32, Ax, ADDA_F64));
+				krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, r32, e32, ih,
+					ih ? ihl : 0, kr);
+			}
 			for (int k = 0; k < K; k++) {
 				if (!active[k]) continue;
 				if (kr[k].rc < 0 && kr[k].niter == 0) { /* the inner solver could not even start */
@@ -686,7 +720,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 				st[k].niter += kr[k].niter;
 				st[k].nmatvec += kr[k].nmv;
 			}
-			CKB(adda_hip_vec_convert_batch(op, K, e64, ADDA_F64, e32, ADDA_F32));
+			if (!v64) CKB(adda_hip_vec_convert_batch(op, K, e64, ADDA_F64, e32, ADDA_F32));
 			CANDIDATE(active[k_]);
 			nouter++;
 			nact = 0;
@@ -695,9 +729,11 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 				if (!active[k]) continue;
 				st[k].nmatvec_fp64++;
 				st[k].nouter++;
-				if (opt->verbose && K == 1)
+				if (opt->verbose && K == 1) {
 					printf("outer %d: inner iterations %d, true residual %.6E\n", st[k].nouter, kr[k].niter,
 						sqrt(r2new[k] / b2[k]));
+					fflush(stdout);
+				}
 				if (r2new[k] < r2[k]) {
 					acc[k] = 1;
 					r2[k] = r2new[k];
@@ -723,7 +759,8 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 				eps[k] = opt->eps / sqrt(r2[k] / b2[k]);
 			}
 			CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, Ax, r64, ca, NULL, cz, NULL, cz, 0, NULL));
-			krylov_run(op, opt, ADDA_F64, K, eps, opt->maxiter, ADDA_IF_ZERO, 0, fin, Ax, e64, NULL, 0, kr);
+			krylov_run(op, opt, ADDA_F64, K, eps, opt->maxiter, ADDA_IF_ZERO, opt->verbose & 2, fin, Ax, e64, NULL, 0,
+				kr);
 			for (int k = 0; k < K; k++)
 				if (fin[k]) {
 					st[k].niter += kr[k].niter;
@@ -759,6 +796,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	CKB(adda_hip_mult_mat_batch(op, ADDA_F64, K, d_pvec, d_x));
 	CKB(adda_hip_sync(op));
 out:
+	if (opt->precision == ADDA_PREC_MIXED_V64) adda_hip_set_matvec_mode(op, 0); /* also after a failed inner solve */
 	{
 		void *vs[] = {r64, Ax, r32, e32, e64, xt};
 		for (int i = 0; i < 6; i++)

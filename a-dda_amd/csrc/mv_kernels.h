@@ -54,9 +54,10 @@ __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ tw
 // P1: block = (z, y0..y0+LY, RHS), all three components of LY y-rows; the RHS is fastest in the block index (the RHS
 // of a tile share its cell-map and material reads). Each thread gathers the three components of one dipole --
 // contiguous in the AoS vector -- into the LDS lines yl*3 + c (x along the line); the last FFT stage stores
-// Xa[z][x][y][c] with (y, c) fastest across threads, runs of 3*LY elements.
-template <typename T, class PL>
-__global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T> *__restrict__ Xa,
+// Xa[z][x][y][c] with (y, c) fastest across threads, runs of 3*LY elements. TV: the type of the dipole vector (FP64
+// vectors around an FP32 transform: TV = double, T = float; otherwise TV = T).
+template <typename T, class PL, typename TV = T>
+__global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T> *__restrict__ Xa,
 	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
@@ -95,10 +96,10 @@ __global__ void __launch_bounds__(512) k_p1(const cx<T> *__restrict__ arg, cx<T>
 #pragma unroll
 		for (int u = 0; u < GU; u++)
 			if (dd[u] >= 0) {
-				const cx<T> *a = arg + 3 * (size_t)dd[u];
+				const cx<TV> *a = arg + 3 * (size_t)dd[u];
 				mm[u] = onemat ? 0 : 3 * mat[dd[u]];
 #pragma unroll
-				for (int c = 0; c < 3; c++) av[u][c] = her ? conjc(a[c]) : a[c];
+				for (int c = 0; c < 3; c++) av[u][c] = cvt<T>(her ? conjc(a[c]) : a[c]);
 			}
 #pragma unroll
 		for (int u = 0; u < GU; u++) {
@@ -384,11 +385,13 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 // result = arg + S.X (her: conj), three contiguous components. Per block, at partials[(k*rows + row)*nb + tile] (nb
 // blocks per RHS): row 0 ||result||^2; with dmode 1 (w = dotw) or 2 (w = arg) also rows 1, 2 = Re, Im of
 // sum result.conj(w) -- an inner product of the Krylov iteration fused into the pass that writes its operand.
-template <typename T, class PL, bool DOT>
-__global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<T> *__restrict__ arg,
-	cx<T> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
+// TV: the type of the dipole vectors (arg, res, dotw); the transform and the coupling constants are in T. With
+// TV = double, T = float the correction S.X is rounded to FP32 and added to the FP64 argument in FP64.
+template <typename T, class PL, bool DOT, typename TV = T>
+__global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
+	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
-	double *__restrict__ partials, const cx<T> *__restrict__ dotw, int dmode, Bq bq)
+	double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
 {
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
@@ -418,7 +421,8 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const bool onemat = bq.cs == 3;  // one material: the coupling constants are the same for every dipole
 	struct Round {
 		int dd[GU];
-		cx<T> av[GU][3], cs[GU][3], wv[GU][3];
+		cx<TV> av[GU][3], wv[GU][3];
+		cx<T> cs[GU][3];
 	};
 	// Everything a round needs besides the transform (cell map, argument, coupling constants, the dot weights) is
 	// independent of it: the first round's loads are issued before the transform, which hides their latency.
@@ -431,7 +435,8 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 #pragma unroll
 		for (int u = 0; u < GU; u++)
 			if (g.dd[u] >= 0) {
-				const cx<T> *a = arg + 3 * (size_t)g.dd[u], *c3 = ccs + (onemat ? 0 : 3 * mat[g.dd[u]]);
+				const cx<TV> *a = arg + 3 * (size_t)g.dd[u];
+				const cx<T> *c3 = ccs + (onemat ? 0 : 3 * mat[g.dd[u]]);
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
 					g.av[u][c] = a[c];
@@ -446,16 +451,16 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 			if (g.dd[u] >= 0) {
 				const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
 				const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
-				cx<T> *r = res + 3 * (size_t)g.dd[u];
+				cx<TV> *r = res + 3 * (size_t)g.dd[u];
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
-					const cx<T> a = her ? conjc(g.av[u][c]) : g.av[u][c];
-					cx<T> rv = add(a, mul(g.cs[u][c], q[c * LS]));
+					const cx<TV> a = her ? conjc(g.av[u][c]) : g.av[u][c];
+					cx<TV> rv = add(a, cvt<TV>(mul(g.cs[u][c], q[c * LS])));
 					if (her) rv = conjc(rv);
 					r[c] = rv;
 					s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
 					if (DOT) {
-						const cx<T> w = g.wv[u][c];
+						const cx<TV> w = g.wv[u][c];
 						dr += (double)rv.x * w.x + (double)rv.y * w.y;
 						di += (double)rv.y * w.x - (double)rv.x * w.y;
 					}
@@ -502,7 +507,11 @@ template <typename T, class PL> constexpr bool p3r_ok()
 }
 
 // The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
-enum { MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_KINDS = 7 };
+// MV_P1W / MV_P5W / MV_P5DW: FP64 dipole vectors around the FP32 transform (FP32 slot only).
+enum {
+	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
+	MV_P5DW = 9, MV_KINDS = 10
+};
 struct MvKernels {
 	const void *k[MV_KINDS][2];
 };
@@ -518,6 +527,12 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
+	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = nullptr;
+	if constexpr (sizeof(T) == 4) {
+		m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
+		m.k[MV_P5W][f] = (const void *)&k_p5<T, PL, false, double>;
+		m.k[MV_P5DW][f] = (const void *)&k_p5<T, PL, true, double>;
+	}
 }
 template <class PL> void mv_fill(MvKernels &m)
 {
@@ -544,6 +559,14 @@ template <typename T, class PL> hipError_t mv_lds_t()
 	if constexpr (p3r_ok<T, PL>())
 		if ((e = hipFuncSetAttribute((const void *)k_p3r<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 			kMvLdsMax))) return e;
+	if constexpr (sizeof(T) == 4) {
+		if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+			kMvLdsMax))) return e;
+		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false, double>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax - 4096))) return e;
+		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true, double>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax - 4096))) return e;
+	}
 	return hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax);
 }

@@ -20,8 +20,9 @@ enum adda_iter { ADDA_IT_QMR_CS = 0, ADDA_IT_BICGSTAB = 1, ADDA_IT_CGNR = 2, ADD
 enum adda_init { ADDA_IF_AUTO = 0, ADDA_IF_ZERO = 1, ADDA_IF_INC = 2 };
 /* polarizability relation (option -pol, subset) */
 enum adda_pol { ADDA_POL_LDR = 0, ADDA_POL_CM = 1, ADDA_POL_RRC = 2 };
-/* precision form */
-enum adda_prec { ADDA_PREC_FP64 = 0, ADDA_PREC_MIXED = 1 };
+/* precision form: FP64; FP32 Krylov iterations (FP32 vectors and transforms) inside FP64 defect correction; FP64
+ * Krylov iterations (FP64 vectors and recurrences) around FP32 transforms, inside the same FP64 defect correction */
+enum adda_prec { ADDA_PREC_FP64 = 0, ADDA_PREC_MIXED = 1, ADDA_PREC_MIXED_V64 = 2 };
 
 /* Error text of the last failing call of this library or of the HIP backend. */
 const char *adda_last_error(void);
@@ -62,7 +63,8 @@ typedef struct {
 	int precision;       /* enum adda_prec */
 	double inner_eps;    /* mixed: relative tolerance of each FP32 inner solve (default 1e-5 when 0) */
 	int max_outer;       /* mixed: maximum FP64 correction steps (default 30 when 0) */
-	int verbose;         /* print RE_ lines to stdout */
+	int verbose;         /* bit 0: print RE_ lines to stdout; bit 1: a progress line (iteration, residual) about every
+	                        10 s of a device-resident solve, inner solves of the mixed forms included */
 	int host_scalars;    /* 1: scalar recurrences on the host with a synchronising reduction per inner product (the
 	                      * reference's control flow, src/iterative.c); 0: device-resident recurrences (QMR_CS,
 	                      * BiCGStab, CGNR; BiCG_CS and CSYM always use the host form) */

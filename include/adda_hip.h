@@ -148,6 +148,13 @@ int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, voi
 int adda_hip_get_config(adda_hip_op *op, int prec, char *buf, size_t len);
 int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg);
 
+/* Transform precision of FP64 matvecs (no reference counterpart: ADDA's MatVec computes in the vector precision).
+ * mode 0: FP64 matvecs run in FP64 (the default). mode 1: FP64 matvecs (adda_hip_matvec*, and the matvecs inside the
+ * adda_hip_dk_* iterations with ADDA_F64) read and write FP64 vectors but run the FFTs and the D-matrix product in
+ * FP32 (the operator's FP32 D-matrix, created with flags & 1), adding the FP32 correction S.X to the FP64 argument in
+ * FP64 -- Krylov iterations with FP64 vectors and recurrences around an inexact matvec. FP32 matvecs are unaffected. */
+int adda_hip_set_matvec_mode(adda_hip_op *op, int mode);
+
 /* Autotuner: times every candidate radix schedule (radix-16/8/4/2 splits of the power of two, radix 3/5 stages
  * before or after) and tile size of each pass group (x: P1+P5, z: P2+P4, y: P3) with HIP events, reps timed runs
  * each, and keeps the fastest. With cache_path, a line {"key": ..., "config": ...} is looked up first (no timing if

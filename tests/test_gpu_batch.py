@@ -127,14 +127,16 @@ def test_batched_history_and_oracle_field(hipdev):
         bp.close()
 
 
+@pytest.mark.parametrize("v64", [False, True])
 @pytest.mark.parametrize("method", ["bicgstab", "qmr"])
-def test_batched_mixed_precision(hipdev, method):
-    """FP32 inner solves of all RHS in shared launches + per-RHS FP64 defect correction."""
+def test_batched_mixed_precision(hipdev, method, v64):
+    """Inner solves of all RHS in shared launches (FP32, or FP64 around FP32 transforms) + per-RHS FP64 defect
+    correction."""
     bp, names = _batch(hipdev, 4, fp32=True)
     try:
         ref = bp.solve(method, eps=1e-10)
         c64 = [bp.cext(k) for k in range(bp.K)]
-        res = bp.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED)
+        res = bp.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED_V64 if v64 else hipdev.PREC_MIXED)
         for k in range(bp.K):
             assert res[k]["true_resid"] <= 1e-10, (k, res[k])
             assert res[k]["nouter"] >= 2

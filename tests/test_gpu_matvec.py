@@ -84,6 +84,45 @@ def test_matvec_fp32(hipdev, shape, box, m, ay, az, kw):
     assert _rel(y, y_ref) < 2e-5
 
 
+@pytest.mark.parametrize("her", [False, True])
+@pytest.mark.parametrize("shape,box,m,ay,az,kw", CASES[:3])
+def test_matvec_fp64_vectors_fp32_transform(hipdev, shape, box, m, ay, az, kw, her):
+    """Matvec mode 1: FP64 vectors in and out, FFTs and D-matrix product in FP32. Within the FP32 bar of the oracle's
+    FP64 matvec; the transform part equals the FP32 matvec's to FP32 rounding; mode 0 gives back the FP64 matvec bit
+    for bit; a batch of K = 3 equals the single calls bit for bit."""
+    c = _case(shape, box, m, ay, az, **kw)
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    op = _op(hipdev, c, fp32=True)
+    try:
+        op.set_cc_sqrt(ccs)
+        rng = np.random.default_rng(5)
+        x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+        y64, _ = op.matvec(x, her=her)
+        op.set_matvec_mode(1)
+        y, n2 = op.matvec(x, her=her)
+        ref = c.matvec_fft(c.dmatrix(), ccs, x, her=her)[0]
+        assert _rel(y, ref) < 2e-5
+        assert abs(n2 - np.vdot(y, y).real) / n2 < 1e-12
+        y32, _ = op.matvec(x, her=her, prec=hipdev.F32)
+        assert _rel(y - x, y32 - x.astype(np.complex64)) < 2e-5
+        K = 3
+        op.reserve(K)
+        op.set_cc_sqrt_batch([ccs] * K)  # the same coupling constants for every RHS
+        xs = [x * (k + 1) + 0.5j * k for k in range(K)]
+        a, r = op.alloc_batch(K), op.alloc_batch(K)
+        for k in range(K):
+            op.upload(op.rhs(a, k), xs[k])
+        op.matvec_batch(a, r, K, her=her)
+        for k in range(K):
+            assert np.array_equal(op.download(op.rhs(r, k)), op.matvec(xs[k], her=her)[0]), k
+        op.free(a)
+        op.free(r)
+        op.set_matvec_mode(0)
+        assert np.array_equal(op.matvec(x, her=her)[0], y64)
+    finally:
+        op.close()
+
+
 def test_vector_kernels(hipdev):
     c = _case("sphere", 8, 1.5)
     op = _op(hipdev, c)

@@ -122,15 +122,18 @@ def test_cm_and_multimaterial(hipdev):
         pr.close()
 
 
+@pytest.mark.parametrize("v64", [False, True])
 @pytest.mark.parametrize("max_inner", [0, 40])
 @pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr"])
-def test_mixed_precision_c1(hipdev, method, max_inner):
-    """FP32 inner Krylov + FP64 defect correction reaches the FP64 answer: true residual <= 1e-10 and Cext within 1e-6
-    (the reference value 135.0448603 is itself converged to 1e-10)."""
+def test_mixed_precision_c1(hipdev, method, max_inner, v64):
+    """FP32 inner Krylov (or FP64 inner Krylov around FP32 transforms, v64) + FP64 defect correction reaches the FP64
+    answer: true residual <= 1e-10 and Cext within 1e-6 (the reference value 135.0448603 is itself converged to
+    1e-10)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
-        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED, max_inner=max_inner)
+        prec = hipdev.PREC_MIXED_V64 if v64 else hipdev.PREC_MIXED
+        r = pr.solve(method, eps=1e-10, precision=prec, max_inner=max_inner)
         assert r["true_resid"] <= 1e-10, r
         assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6
         assert r["nmatvec_fp64"] >= 1 and r["nouter"] >= 2
@@ -138,14 +141,20 @@ def test_mixed_precision_c1(hipdev, method, max_inner):
         pr.close()
 
 
+@pytest.mark.parametrize("v64", [False, True])
 @pytest.mark.parametrize("method", ["qmr", "bicgstab"])
-def test_mixed_precision_fp64_finish(hipdev, method):
-    """An RHS still above eps after the FP32 correction steps (here: one step of at most 5 FP32 iterations) finishes
-    with an FP64 Krylov solve of its correction equation; the FP64 residual decides, so the result is converged."""
+def test_mixed_precision_fp64_finish(hipdev, method, v64):
+    """An RHS still above eps after the inner correction steps (here: one step of at most 5 inner iterations) finishes
+    with an FP64 Krylov solve of its correction equation; the FP64 residual decides, so the result is converged. The
+    operator is back in FP64 transforms after a v64 solve."""
     pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
-        r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED, max_outer=1, max_inner=5)
+        prec = hipdev.PREC_MIXED_V64 if v64 else hipdev.PREC_MIXED
+        x = np.arange(3 * pr.nd) * (1 - 0.5j) / pr.nd
+        y0, _ = pr.op.matvec(x)
+        r = pr.solve(method, eps=1e-10, precision=prec, max_outer=1, max_inner=5)
+        assert np.array_equal(pr.op.matvec(x)[0], y0)
         assert r["rc"] == 0 and r["true_resid"] <= 1e-10, r
         assert r["nouter"] == 2 and r["nmatvec_fp64"] > 10, r
         assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6

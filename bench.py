@@ -148,6 +148,19 @@ def main():
                             cext_rel_err=abs(cqm - cq64) / abs(cq64), cext_vs_bicgstab_fp64=abs(cq64 - cext64) / cext64)
         log(f"QMR_CS: FP64 {tq64:.3f} s ({rq64['niter']} it), mixed {tqm:.3f} s ({rqm['niter']} it, "
             f"{rqm['nouter']} outer, true residual {rqm['true_resid']:.2e}), Cext rel err {extra['qmr']['cext_rel_err']:.2e}")
+        # FP64 Krylov vectors and recurrences around FP32 transforms (the matvec's FFTs and D-matrix product in FP32)
+        tqv, cqv, rqv = run("qmr", A.PREC_MIXED_V64)
+        extra["qmr"]["fp64_vectors_fp32_transforms"] = dict(
+            tts_s=round(tqv, 4), speedup=round(tq64 / tqv, 3), converged=rqv["rc"] == 0, iterations=rqv["niter"],
+            outer=rqv["nouter"], true_resid=rqv["true_resid"], cext_rel_err=abs(cqv - cq64) / abs(cq64))
+        log(f"QMR_CS with FP64 vectors around FP32 transforms: {tqv:.3f} s ({rqv['niter']} it, {rqv['nouter']} outer, "
+            f"true residual {rqv['true_resid']:.2e}), speedup {tq64 / tqv:.2f}x")
+        tbv, cbv, rbv = run("bicgstab", A.PREC_MIXED_V64)
+        extra["bicgstab_fp64_vectors_fp32_transforms"] = dict(
+            tts_s=round(tbv, 4), speedup=round(t64 / tbv, 3), converged=rbv["rc"] == 0, iterations=rbv["niter"],
+            outer=rbv["nouter"], true_resid=rbv["true_resid"], cext_rel_err=abs(cbv - cext64) / abs(cext64))
+        log(f"BiCGStab with FP64 vectors around FP32 transforms: {tbv:.3f} s ({rbv['niter']} it), speedup "
+            f"{t64 / tbv:.2f}x")
 
     if not args.no_extra:
         extra["batched_rhs"] = bench_batched(A, args)

@@ -584,7 +584,12 @@ using P5WFn = decltype(&k_p5<float, Dyn<8>, false, double>);
 // ==================================================================================================================
 // matvec driver
 
-static size_t lds_p15(const MvCfg &c, int gX, size_t es) { return (size_t)(gX + 3 * c.LY * rowstride_odd(gX)) * es; }
+// LDS line strides of the selected kernels: P1/P5 use swizzled rows for compile-time plans of lengths divisible by 16
+static bool swz_of(const FftPlan2 &q) { return mv_kernels(q).swz; }
+static size_t lds_p15(const MvCfg &c, int gX, size_t es)
+{
+	return (size_t)(gX + 3 * c.LY * (swz_of(c.qx) ? gX + 1 : rowstride_odd(gX))) * es;
+}
 static size_t lds_p24(const MvCfg &c, int gZ, size_t es) { return (size_t)(gZ + gZ * c.TC) * es; }
 static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {

@@ -11,6 +11,8 @@
 #pragma once
 #include "fft_device.h"
 
+#include <type_traits>
+
 namespace adda {
 
 struct FftPlan2 {
@@ -155,6 +157,25 @@ __host__ __device__ __forceinline__ int rowstride_lm(int n)
 }
 __host__ __device__ __forceinline__ int rowstride_odd(int n) { return (n + n / 16) | 1; }
 
+// LDS layouts of a block's lines for the compile-time plans (element k of line l; KS > 1: column tiles, element k of
+// column l at k*KS + l). LayPad: padded rows (k + k/16), line stride LS from rowstride_lm / rowstride_odd. LaySwz: rows
+// without padding, the 16-element groups of a row permuted by XOR (offset o of group g at o ^ (g mod 16)), which keeps
+// the stride-16 butterfly accesses on distinct banks; FLIP also swaps neighbouring groups of odd lines, so that two
+// lines read by one half-wave fall on opposite bank halves (line-major kernels, LS = N). Without FLIP, LS = N + 1 for
+// kernels that also walk across lines. Measured on gfx950 both are conflict-free for the patterns of the kernel they
+// serve (FP32 and FP64), and they take N instead of N + N/16 elements per line. Require N % 16 == 0.
+struct LayPad {
+	__device__ static __forceinline__ int pos(int l, int k, int KS, int LS) { return KS == 1 ? l * LS + k + (k >> 4) : k * KS + l; }
+};
+template <bool FLIP> struct LaySwz {
+	__device__ static __forceinline__ int pos(int l, int k, int KS, int LS)
+	{
+		if (KS != 1) return k * KS + l;
+		const int g = k >> 4, o = (k & 15) ^ (g & 15);
+		return l * LS + 16 * (FLIP ? (g ^ (l & 1)) : g) + o;
+	}
+};
+
 // Number of butterflies per thread of a stage of radix R when the largest radix of the plan is at most RM: a line has
 // N/RMAX >= N/RM threads, so ceil(RM/R) butterflies per thread suffice. Kernels are instantiated for RM = 16 and
 // RM = 8; the radix-8 variant holds half as many values per thread (fewer registers, more waves per SIMD).
@@ -271,7 +292,7 @@ __device__ __forceinline__ void fft_line(cx<T> *buf, const cx<T> *tws, const Fft
 // of another thread. (With GIN the caller must have no pending reads of the buffer; with GOUT the transform ends
 // without a barrier, and the caller synchronises before reusing the buffer.)
 template <typename T, bool INV, int N, int TPL, int NS, bool FIRST, bool LAST, int R, int RMAX, bool HIN, bool HOUT,
-	bool GIN, bool GOUT, class Ld, class St>
+	bool GIN, bool GOUT, class LAY, class Ld, class St>
 __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int LS, int line, int t, bool active,
 	Ld &ld, St &st)
 {
@@ -285,7 +306,7 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 		if (active && (NB % TPL == 0 || j < NB)) {
 #pragma unroll
 			for (int q = 0; q < (HI ? R / 2 : R); q++)
-				v[u][q] = FIRST ? ld(line, j + q * NB) : buf[lpos(j + q * NB, KS) + line * LS];
+				v[u][q] = FIRST ? ld(line, j + q * NB) : buf[LAY::pos(line, j + q * NB, KS, LS)];
 			if (!FIRST) {
 				const int k = j % NS;
 #pragma unroll
@@ -317,7 +338,7 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 			if (active && (NB % TPL == 0 || j < NB)) {
 				const int d = (j / NS) * NS * R + j % NS;
 #pragma unroll
-				for (int q = 0; q < R; q++) buf[lpos(d + q * NS, KS) + line * LS] = v[u][q];
+				for (int q = 0; q < R; q++) buf[LAY::pos(line, d + q * NS, KS, LS)] = v[u][q];
 			}
 		}
 		__syncthreads();
@@ -407,7 +428,8 @@ __device__ __forceinline__ void stage3(cx<T> (&e)[3][RMAX], cx<T> *buf, const cx
 template <int RM> struct Dyn {
 	// the HIN / HOUT options are ignored: every input is loaded and every output stored
 	static constexpr bool HALF_IN = false, HALF_OUT = false;
-	static constexpr int RMAXC = 0;  // largest radix unknown at compile time
+	static constexpr int RMAXC = 0;    // largest radix unknown at compile time
+	static constexpr bool SWZ = false;  // padded LDS rows (LayPad)
 	// the natural twiddle table tws[e] = exp(-2 pi i e/N), e < N. No barrier: the caller's next __syncthreads (every
 	// transform reaches one before its first twiddle) publishes the table, so its loads overlap the caller's.
 	template <typename T>
@@ -419,11 +441,12 @@ template <int RM> struct Dyn {
 	__device__ static __forceinline__ int tpl(const FftPlan2 &p) { return p.n / p.rmax; }
 	// HIN / HOUT (zero upper half of the input, unneeded upper half of the output) and GIN / GOUT (global Load /
 	// Store, fewer barriers) only apply to compile-time plans: the run-time path keeps every barrier
-	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false, class Ld,
-		class St>
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false,
+		class LAY = LayPad, class Ld, class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
+		static_assert(std::is_same_v<LAY, LayPad>, "run-time plans use the padded layout");
 		fft_line<T, INV, RM>(buf, tws, p, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
 };
@@ -437,6 +460,7 @@ template <int... Rs> struct Fix {
 	// whether HIN skips the inputs at indices >= N/2 and HOUT the outputs there (even first / last radix)
 	static constexpr bool HALF_IN = RAD[0] % 2 == 0, HALF_OUT = RAD[NST - 1] % 2 == 0;
 	static constexpr int RMAXC = RMAX;
+	static constexpr bool SWZ = N % 16 == 0;  // swizzled LDS rows (LaySwz) where the length allows
 	// Offset of stage S's twiddles in the staged table: stages 1..NST-1 in order, (R_s - 1)*NS_s entries each.
 	static constexpr int twoff(int S)
 	{
@@ -475,27 +499,28 @@ template <int... Rs> struct Fix {
 	}
 	__device__ static __forceinline__ constexpr int n(const FftPlan2 &) { return N; }
 	__device__ static __forceinline__ constexpr int tpl(const FftPlan2 &) { return TPL; }
-	template <typename T, bool INV, bool HIN, bool HOUT, bool GIN, bool GOUT, int S, int NS, class Ld, class St>
+	template <typename T, bool INV, bool HIN, bool HOUT, bool GIN, bool GOUT, class LAY, int S, int NS, class Ld,
+		class St>
 	__device__ static __forceinline__ void run(cx<T> *buf, const cx<T> *tws, int KS, int LS, int lineA, int tA,
 		bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
 		constexpr int R = RAD[S];
 		constexpr bool FIRST = S == 0, LAST = S == NST - 1;
-		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT, GIN, GOUT>(buf, tws + (FIRST ? 0 : twoff(S)), KS,
-			LS, FIRST ? lineA : lineB, FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
+		sstage<T, INV, N, TPL, NS, FIRST, LAST, R, RMAX, HIN, HOUT, GIN, GOUT, LAY>(buf, tws + (FIRST ? 0 : twoff(S)),
+			KS, LS, FIRST ? lineA : lineB, FIRST ? tA : tB, FIRST ? activeA : activeB, ld, st);
 		if constexpr (!LAST)
-			run<T, INV, HIN, HOUT, GIN, GOUT, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB,
-				ld, st);
+			run<T, INV, HIN, HOUT, GIN, GOUT, LAY, S + 1, NS * R>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB,
+				activeB, ld, st);
 	}
 	// HIN: inputs at indices >= N/2 are zero (zero-padded data); HOUT: outputs at indices >= N/2 are not needed.
 	// Either may leave the Load / Store functor uncalled for those indices. GIN / GOUT: the Load reads / the Store
 	// writes global memory, not the LDS buffer (see sstage). tws: the staged table of load_tw.
-	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false, class Ld,
-		class St>
+	template <typename T, bool INV, bool HIN = false, bool HOUT = false, bool GIN = false, bool GOUT = false,
+		class LAY = LayPad, class Ld, class St>
 	__device__ static __forceinline__ void fft(cx<T> *buf, const cx<T> *tws, const FftPlan2 &, int KS, int LS,
 		int lineA, int tA, bool activeA, int lineB, int tB, bool activeB, Ld &ld, St &st)
 	{
-		run<T, INV, HIN, HOUT, GIN, GOUT, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
+		run<T, INV, HIN, HOUT, GIN, GOUT, LAY, 0, 1>(buf, tws, KS, LS, lineA, tA, activeA, lineB, tB, activeB, ld, st);
 	}
 
 	// Three-line transforms for the register D-product (stage3): the forward FFT takes the radices in plan order,

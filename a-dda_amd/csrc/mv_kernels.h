@@ -24,13 +24,24 @@ static __device__ __forceinline__ bool skip_rhs(const Bq &b, int k)
 	return b.gate && b.gate[(size_t)k * DK_NSLOTS + DK_STATUS] != 0;
 }
 
-// Deterministic block reduction for any block size up to 512 (not necessarily a multiple of the wave size):
-// a fixed-order tree over LDS. Every thread of the block must call it; the total is returned to thread 0.
+// Deterministic block reduction for any block size up to 512: every thread of the block must call it; the total is
+// returned to thread 0. Blocks of whole waves reduce each wave with a fixed xor butterfly (no barrier), then thread 0
+// adds the wave sums in wave order; other block sizes use a fixed-order tree over LDS. red_lds: blockDim doubles.
 template <typename T>
 __device__ __forceinline__ double block_sum(double s, double *red_lds)
 {
 	const int tid = threadIdx.x, n = blockDim.x;
-	__syncthreads();
+	__syncthreads();  // red_lds may still be read by a previous call (or be the caller's buffer)
+	if (n % 64 == 0) {
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+		if ((tid & 63) == 0) red_lds[tid >> 6] = s;
+		__syncthreads();
+		double t = 0;
+		if (tid == 0)
+			for (int w = 0; w < n / 64; w++) t += red_lds[w];
+		return t;
+	}
 	red_lds[tid] = s;
 	__syncthreads();
 	for (int stride = 256; stride > 0; stride >>= 1) {
@@ -67,7 +78,8 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 	Xa += kq * bq.xs2;
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride_odd(N), NL = 3 * LY;
+	using LAY = std::conditional_t<PL::SWZ, LaySwz<false>, LayPad>;  // lines are also walked line-fastest
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = PL::SWZ ? N + 1 : rowstride_odd(N), NL = 3 * LY;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	PL::template load_tw<T>(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
@@ -105,18 +117,19 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 		for (int u = 0; u < GU; u++) {
 			const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
 			if (idx < total) {
-				cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
 #pragma unroll
-				for (int c = 0; c < 3; c++) q[c * LS] = dd[u] >= 0 ? mul(ccs[mm[u] + c], av[u][c]) : mk<T>(0, 0);
+				for (int c = 0; c < 3; c++)
+					buf[LAY::pos(3 * yl + c, x, 1, LS)] = dd[u] >= 0 ? mul(ccs[mm[u] + c], av[u][c]) : mk<T>(0, 0);
 			}
 		}
 	}
 	__syncthreads();
-	auto ld = [&](int line, int x) -> cx<T> { return buf[line * LS + lpos(x, 1)]; };
+	auto ld = [&](int line, int x) -> cx<T> { return buf[LAY::pos(line, x, 1, LS)]; };
 	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineB;
 	const size_t xs = (size_t)3 * bY;
 	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
-	PL::template fft<T, false, true, false, false, true>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, false, true, false, false, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
+		st);
 }
 
 // P2 / P4: FFT along z of Q = gridX*3*boxY contiguous columns; rows k < n_in read (zero above), rows k < n_out written.
@@ -174,16 +187,17 @@ template <typename T, int S> __device__ __forceinline__ cx<T> cmac(cx<T> acc, cx
 	             : mk<T>(acc.x - d.x * a.x + d.y * a.y, acc.y - d.x * a.y - d.y * a.x);
 }
 
-// Symmetric 3x3 product (cSymMatrVec, src/cmplx.h:369-376) at one frequency, in place in LDS (q[0], q[LS], q[2*LS]).
-// D = (d0 f1 f2; f1 d3 f4; f2 f4 d5) with f1, f4 already flipped for the y mirror; Z = -1 flips the xz and yz elements
-// for the z mirror plane (src/matvec.c:270-305).
+// Symmetric 3x3 product (cSymMatrVec, src/cmplx.h:369-376) at one frequency, in place in LDS (the x, y, z components
+// at q0, q1, q2). D = (d0 f1 f2; f1 d3 f4; f2 f4 d5) with f1, f4 already flipped for the y mirror; Z = -1 flips the xz
+// and yz elements for the z mirror plane (src/matvec.c:270-305).
 template <typename T, int Z>
-__device__ __forceinline__ void sym3_lds(cx<T> *q, int LS, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3, cx<T> f4, cx<T> d5)
+__device__ __forceinline__ void sym3_lds(cx<T> *q0, cx<T> *q1, cx<T> *q2, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3,
+	cx<T> f4, cx<T> d5)
 {
-	const cx<T> a0 = q[0], a1 = q[LS], a2 = q[2 * LS];
-	q[0] = cmac<T, Z>(cmac<T, 1>(mul(d0, a0), f1, a1), d2, a2);
-	q[LS] = cmac<T, Z>(cmac<T, 1>(mul(f1, a0), d3, a1), f4, a2);
-	q[2 * LS] = cmac<T, Z>(cmac<T, Z>(mul(d5, a2), f4, a1), d2, a0);
+	const cx<T> a0 = *q0, a1 = *q1, a2 = *q2;
+	*q0 = cmac<T, Z>(cmac<T, 1>(mul(d0, a0), f1, a1), d2, a2);
+	*q1 = cmac<T, Z>(cmac<T, 1>(mul(f1, a0), d3, a1), f4, a2);
+	*q2 = cmac<T, Z>(cmac<T, Z>(mul(d5, a2), f4, a1), d2, a0);
 }
 
 // P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
@@ -200,6 +214,9 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	const cx<T> *__restrict__ D, int x0, int z, int gX, int bY, int gZ, int DsY, int LP, int NL1, bool has2,
 	int reduced, int transposed, int line, int t, bool act)
 {
+	// padded rows: their offsets stay immediates (a swizzle would cost P3, which is close to VALU-bound, an address
+	// computation per access)
+	using LAY = LayPad;
 	const int N = PL::n(p), LS = rowstride_lm(N);
 	// D-matrix row of element idx = (pair, y) of the block: the stored row index (mirrored for the reduced form and for
 	// the transposed product, src/matvec.c:181 and 270-305) and the y-mirror sign mask of the xy and yz elements
@@ -218,9 +235,10 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	auto dprod = [&](int idx, const cx<T> *d, unsigned m) {
 		const int pr = idx / N, y = idx - pr * N;
 		const cx<T> f1 = flip<T>(d[1], m), f4 = flip<T>(d[4], m);
-		cx<T> *q = buf + (size_t)(3 * pr) * LS + lpos(y, 1);
-		sym3_lds<T, 1>(q, LS, d[0], f1, d[2], d[3], f4, d[5]);
-		if (has2) sym3_lds<T, -1>(q + (size_t)NL1 * LS, LS, d[0], f1, d[2], d[3], f4, d[5]);  // plane gZ - z
+		auto q = [&](int l) { return buf + LAY::pos(l, y, 1, LS); };
+		sym3_lds<T, 1>(q(3 * pr), q(3 * pr + 1), q(3 * pr + 2), d[0], f1, d[2], d[3], f4, d[5]);
+		if (has2)  // plane gZ - z
+			sym3_lds<T, -1>(q(NL1 + 3 * pr), q(NL1 + 3 * pr + 1), q(NL1 + 3 * pr + 2), d[0], f1, d[2], d[3], f4, d[5]);
 	};
 	// In FP32 the D rows of this thread's first KP elements are loaded before the forward transform, which hides their
 	// latency (KP covers all of them with the reduced D-matrix: blockDim = 6*LP*N/RMAX threads for LP*N elements). In
@@ -238,8 +256,8 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 		}
 	}
 	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[l * LS + lpos(y, 1)] = v; };
-	PL::template fft<T, false, true, false, true, false>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+	auto stl = [&](int l, int y, cx<T> v) { buf[LAY::pos(l, y, 1, LS)] = v; };
+	PL::template fft<T, false, true, false, true, false, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
 	__syncthreads();
 #pragma unroll
 	for (int k = 0; k < KP; k++) {
@@ -255,11 +273,11 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 		dprod(idx, d, m);
 	}
 	__syncthreads();
-	auto ldl = [&](int l, int y) -> cx<T> { return buf[l * LS + lpos(y, 1)]; };
+	auto ldl = [&](int l, int y) -> cx<T> { return buf[LAY::pos(l, y, 1, LS)]; };
 	auto stg = [&](int, int y, cx<T> v) {
 		if (FULL || y < bY) lp[3 * y] = v;
 	};
-	PL::template fft<T, true, false, true, false, true>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
+	PL::template fft<T, true, false, true, false, true, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
 template <typename T, class PL>
@@ -401,9 +419,10 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	if (dotw) dotw += kq * bq.vs;
 	ccs += (size_t)kq * bq.cs;
 	extern __shared__ char smem_raw[];
-	__shared__ double red_lds[512];
-	const int N = PL::n(p), TPL = PL::tpl(p), LS = rowstride_odd(N), NL = 3 * LY;
+	using LAY = std::conditional_t<PL::SWZ, LaySwz<false>, LayPad>;  // lines are also walked line-fastest
+	const int N = PL::n(p), TPL = PL::tpl(p), LS = PL::SWZ ? N + 1 : rowstride_odd(N), NL = 3 * LY;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	double *red_lds = reinterpret_cast<double *>(buf);  // the lines' space, reused after the gather (>= 48*LY doubles)
 	PL::template load_tw<T>(tws, twg, N);
 	const int nyt = (bY + LY - 1) / LY;
 	const int z = tile / nyt, y0 = (tile % nyt) * LY;
@@ -450,12 +469,11 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 		for (int u = 0; u < GU; u++)
 			if (g.dd[u] >= 0) {
 				const int idx = b0 + u * blockDim.x, yl = idx / NF, x = idx - yl * NF;
-				const cx<T> *q = buf + 3 * yl * LS + lpos(x, 1);
 				cx<TV> *r = res + 3 * (size_t)g.dd[u];
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
 					const cx<TV> a = her ? conjc(g.av[u][c]) : g.av[u][c];
-					cx<TV> rv = add(a, cvt<TV>(mul(g.cs[u][c], q[c * LS])));
+					cx<TV> rv = add(a, cvt<TV>(mul(g.cs[u][c], buf[LAY::pos(3 * yl + c, x, 1, LS)])));
 					if (her) rv = conjc(rv);
 					r[c] = rv;
 					s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
@@ -472,9 +490,10 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
 	const size_t xs = (size_t)3 * bY;
 	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
-	auto st = [&](int line, int x, cx<T> v) { buf[line * LS + lpos(x, 1)] = v; };
+	auto st = [&](int line, int x, cx<T> v) { buf[LAY::pos(line, x, 1, LS)] = v; };
 	// only x < boxX <= N/2 is gathered: the inverse transform computes the lower half of each line
-	PL::template fft<T, true, false, true, true, false>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld, st);
+	PL::template fft<T, true, false, true, true, false, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
+		st);
 	__syncthreads();
 	finish(tid, g0);
 	for (int b0 = tid + GU * blockDim.x; b0 < total; b0 += GU * blockDim.x) {  // run-time plans only
@@ -514,6 +533,7 @@ enum {
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
+	bool swz = false;  // the plan's P1/P5 kernels use the swizzled LDS rows (line stride N + 1)
 };
 
 template <typename T, class PL> void mv_fill_t(MvKernels &m)
@@ -536,11 +556,12 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 }
 template <class PL> void mv_fill(MvKernels &m)
 {
+	m.swz = PL::SWZ;
 	mv_fill_t<double, PL>(m);
 	mv_fill_t<float, PL>(m);
 }
 
-// Largest dynamic LDS a pass uses (set as the kernels' limit; P5 also has a static reduction buffer).
+// Largest dynamic LDS a pass uses (set as the kernels' limit).
 constexpr int kMvLdsMax = 96 * 1024;
 
 template <typename T, class PL> hipError_t mv_lds_t()
@@ -551,9 +572,9 @@ template <typename T, class PL> hipError_t mv_lds_t()
 	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
 		return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax - 4096))) return e;
+		kMvLdsMax))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax - 4096))) return e;
+		kMvLdsMax))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax))) return e;
 	if constexpr (p3r_ok<T, PL>())
@@ -563,9 +584,9 @@ template <typename T, class PL> hipError_t mv_lds_t()
 		if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
 			kMvLdsMax))) return e;
 		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false, double>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax - 4096))) return e;
+			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
 		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true, double>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax - 4096))) return e;
+			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
 	}
 	return hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax);

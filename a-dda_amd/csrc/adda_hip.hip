@@ -766,13 +766,26 @@ static void default_cfg(adda_hip_op *op, int pi, size_t es)
 // ==================================================================================================================
 // device-resident Krylov iterations (scalar recurrences in FP64 on the GPU, see krylov_dev.h)
 
-// Fixed-order reduction of nrows rows of nblk partials by one block; row sums land in out[] (thread 0).
+// Fixed-order reduction of nrows (<= 3) rows of nblk partials by one block; row sums land in out[] (thread 0). Each
+// thread adds its strided elements in index order; the loads of four strides of every row are issued together.
 __device__ void dk_reduce(const double *partials, int nblk, int nrows, double *out, double *red_lds)
 {
+	const int B = blockDim.x;
+	double s[3] = {0, 0, 0};
+	for (int i = threadIdx.x; i < nblk; i += 4 * B) {
+		double v[3][4];
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+#pragma unroll
+			for (int j = 0; j < 4; j++) v[k][j] = (k < nrows && i + j * B < nblk) ? partials[(size_t)k * nblk + i + j * B] : 0.0;
+#pragma unroll
+		for (int k = 0; k < 3; k++)
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+				if (i + j * B < nblk) s[k] += v[k][j];
+	}
 	for (int k = 0; k < nrows; k++) {
-		double s = 0;
-		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[(size_t)k * nblk + i];
-		const double t = block_sum<double>(s, red_lds);
+		const double t = block_sum<double>(s[k], red_lds);
 		if (threadIdx.x == 0) out[k] = t;
 	}
 	__syncthreads();

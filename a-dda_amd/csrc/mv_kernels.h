@@ -439,12 +439,12 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const int *cm = cellmap + ((size_t)z * bY + y0) * bX;
 	const bool onemat = bq.cs == 3;  // one material: the coupling constants are the same for every dipole
 	struct Round {
-		int dd[GU];
-		cx<TV> av[GU][3], wv[GU][3];
-		cx<T> cs[GU][3];
+		int dd[GU], mm[GU];
+		cx<TV> av[GU][3];
 	};
-	// Everything a round needs besides the transform (cell map, argument, coupling constants, the dot weights) is
-	// independent of it: the first round's loads are issued before the transform, which hides their latency.
+	// Cell map, material and argument of a round are independent of the transform: the first round's loads are issued
+	// before it, which hides their latency. The coupling constants (a table of a few entries) and the dot weights are
+	// read after it, so that they do not hold registers through the transform.
 	auto fetch = [&](int b0, Round &g) {
 #pragma unroll
 		for (int u = 0; u < GU; u++) {
@@ -452,19 +452,27 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 			g.dd[u] = (idx < total && x < bX && yl < ny) ? cm[yl * bX + x] : -1;
 		}
 #pragma unroll
-		for (int u = 0; u < GU; u++)
+		for (int u = 0; u < GU; u++) {
+			g.mm[u] = 0;
 			if (g.dd[u] >= 0) {
 				const cx<TV> *a = arg + 3 * (size_t)g.dd[u];
-				const cx<T> *c3 = ccs + (onemat ? 0 : 3 * mat[g.dd[u]]);
+				if (!onemat) g.mm[u] = 3 * mat[g.dd[u]];
 #pragma unroll
-				for (int c = 0; c < 3; c++) {
-					g.av[u][c] = a[c];
-					g.cs[u][c] = c3[c];
-					if (DOT) g.wv[u][c] = dmode == 1 ? dotw[3 * (size_t)g.dd[u] + c] : a[c];
-				}
+				for (int c = 0; c < 3; c++) g.av[u][c] = a[c];
 			}
+		}
 	};
 	auto finish = [&](int b0, const Round &g) {
+		cx<T> cs[GU][3];
+		cx<TV> wv[GU][3];
+#pragma unroll
+		for (int u = 0; u < GU; u++)
+			if (g.dd[u] >= 0)
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					cs[u][c] = ccs[g.mm[u] + c];
+					if (DOT) wv[u][c] = dmode == 1 ? dotw[3 * (size_t)g.dd[u] + c] : g.av[u][c];
+				}
 #pragma unroll
 		for (int u = 0; u < GU; u++)
 			if (g.dd[u] >= 0) {
@@ -473,12 +481,12 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
 					const cx<TV> a = her ? conjc(g.av[u][c]) : g.av[u][c];
-					cx<TV> rv = add(a, cvt<TV>(mul(g.cs[u][c], buf[LAY::pos(3 * yl + c, x, 1, LS)])));
+					cx<TV> rv = add(a, cvt<TV>(mul(cs[u][c], buf[LAY::pos(3 * yl + c, x, 1, LS)])));
 					if (her) rv = conjc(rv);
 					r[c] = rv;
 					s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
 					if (DOT) {
-						const cx<TV> w = g.wv[u][c];
+						const cx<TV> w = wv[u][c];
 						dr += (double)rv.x * w.x + (double)rv.y * w.y;
 						di += (double)rv.y * w.x - (double)rv.x * w.y;
 					}

@@ -1033,15 +1033,9 @@ __global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *par
 
 // BiCGStab scalar steps (src/iterative.c:675-718). Slots: 0 ro_new, 2 ro_old, 4 alpha, 6 omega, 8 beta,
 // 20 p-coef on p, 22 p-coef on v, 24 one, 26 -alpha, 28 alpha, 30 omega (x), 32 -omega (r), 34 denumOmega.
-__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter, int ps)
+// BiCGStab's beta from rho_new = r.r~ (src/iterative.c:676-694); thread 0 only.
+__device__ void dk_bs_beta(double *st, dcx ro_new, int niter)
 {
-	__shared__ double red_lds[512];
-	__shared__ double v[3];
-	DK_RHS_SCALAR(st, partials, ps);
-	DK_ACTIVE(st);
-	dk_reduce(partials, nblk, 2, v, red_lds);
-	if (threadIdx.x != 0) return;
-	const dcx ro_new = dmk(v[0], v[1]);
 	st_c(st, DK_SC + 0, ro_new);
 	if (niter == 1) { st_c(st, DK_SC + 20, dmk(0, 0)); st_c(st, DK_SC + 22, dmk(0, 0)); return; }
 	const dcx temp1 = dmul(ro_new, ld_c(st, DK_SC + 4)), temp2 = dmul(ld_c(st, DK_SC + 2), ld_c(st, DK_SC + 6));
@@ -1050,6 +1044,30 @@ __global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *part
 	st_c(st, DK_SC + 8, beta);
 	st_c(st, DK_SC + 20, beta);
 	st_c(st, DK_SC + 22, dneg(dmul(beta, ld_c(st, DK_SC + 6))));
+}
+__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x == 0) dk_bs_beta(st, dmk(v[0], v[1]), niter);
+}
+// End of a BiCGStab iteration: the progress test on ||r||^2 (row 0 of the x/r update's partial sums) and, if the
+// solve goes on, the next iteration's beta from r.r~ (rows 1-2) -- one launch and one pass over the partials.
+__global__ void __launch_bounds__(256) k_dk_bs_prog_s1(double *st, double *hist, const double *partials, int nblk,
+	int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	if (hist) hist += (size_t)blockIdx.x * (size_t)st[DK_HISTLEN];
+	dk_reduce(partials, nblk, 3, v, red_lds);
+	if (threadIdx.x != 0) return;
+	dk_progress(st, hist, v[0]);
+	if (st[DK_STATUS] == 0) dk_bs_beta(st, dmk(v[1], v[2]), 2);  // any iteration after the first
 }
 
 __global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *partials, int nblk, int ps)
@@ -1197,14 +1215,12 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 	const DkGeo g = dk_geo(op, k);
 	double *st = k->st;
 	int nb5;
-	// r.r~ : a dot pass in the first iteration, afterwards rows 1-2 left by the previous x/r update
+	// r.r~ and beta: a dot pass in the first iteration; afterwards the previous iteration's closing kernel
+	// (k_dk_bs_prog_s1) computed beta from the rows its x/r update left
 	if (niter == 1) {
 		dot_dev<T>(op, g, st, r, rt, ADDA_DOT_CONJ);
 		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	}
-	else
-		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2 + g.nb, g.nb, niter,
-			g.ps2);
 	// p = beta p - beta omega v + r  (niter 1: p = r)
 	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
 	// v = A.p with v.r~ fused into the matvec's last pass
@@ -1227,7 +1243,7 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
 		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)rt);
-	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_bs_prog_s1, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }

@@ -1290,6 +1290,14 @@ extern "C" int adda_hip_fftfit(int x, int divis)
 	}
 }
 
+extern "C" int adda_hip_fastgrid(int x, int divis)
+{
+	const int n = adda_hip_fftfit(x, divis);
+	int p = 2;
+	while (p < x || p % divis != 0) p *= 2;
+	return 4 * (long)p <= 5 * (long)n ? p : n;  // the power of two when at most 25 % larger
+}
+
 // (Re)allocates the per-RHS work buffers, cc_sqrt tables and reduction space for K right-hand sides (no-op when
 // the operator already holds K); the cc_sqrt tables already set are kept. Synchronises the operator's stream.
 static int reserve(adda_hip_op *op, int K)

@@ -234,7 +234,8 @@ def measured_traffic(pass_index, cfg):
 def bench_batched(A, args):
     """Batched multi-RHS workload: ellipsoid on a 64^3 grid (semi-axes 1:0.8:0.5, box 64x52x32), m = 1.33+0.001i,
     default dpl 10|m|, 32 seeded orientations x 2 polarizations = 64 right-hand sides solved together to 1e-10 (FP64
-    and FP32-mixed, HIP-graph replay), against the same 64 solved one after another (ADDA's orientation loop)."""
+    and FP32-mixed, HIP-graph replay), against the same 64 solved one after another (ADDA's orientation loop). The FFT
+    grid is adda_hip_fastgrid's (128 along y, where fftFit gives 108), for the batched and the sequential solves alike."""
     from adda_rot import rotate_zyz
     from cases import orientations, PROP_Z, POL_X, POL_Y
     pos, _, box = A.make_particle("ellipsoid", 64, 0.8, 0.5)
@@ -245,7 +246,8 @@ def bench_batched(A, args):
         pols += [pY, pX]
     K = len(props)
     m = 1.33 + 0.001j
-    bp = A.BatchProblem(pos, box, m, np.array(props), np.array(pols), fp32=True)
+    grid = A.fast_grid(box)
+    bp = A.BatchProblem(pos, box, m, np.array(props), np.array(pols), fp32=True, grid=grid)
     out = {"workload": f"ellipsoid box {box[0]}x{box[1]}x{box[2]} ({len(pos)} dipoles), grid "
                        f"{bp.grid[0]}x{bp.grid[1]}x{bp.grid[2]}, m=1.33+0.001i, 32 orientations x 2 polarizations = "
                        f"{K} RHS, eps 1e-10", "rhs": K}
@@ -283,7 +285,7 @@ def bench_batched(A, args):
                            "max_true_resid_mixed": max(s["true_resid"] for s in res[(method, "mixed")][2]),
                            "cext_max_rel_err_mixed_vs_fp64": float(np.max(np.abs(cm - c64) / np.abs(c64)))}
         # the same RHS one after another on one operator (ADDA's loop over orientations and polarizations)
-        pr = A.Problem(pos, box, m, fp32=True)
+        pr = A.Problem(pos, box, m, fp32=True, grid=grid)
         try:
             pr.op.autotune(A.F64, reps=3, cache=args.tune_cache)
             t = time.perf_counter()

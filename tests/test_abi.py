@@ -45,6 +45,18 @@ def test_fftfit_matches_oracle_rule():
         assert L.adda_hip_fftfit(x, 1) == oracle().or_fftfit(x, 1, 1)
 
 
+def test_fastgrid_rule():
+    """The speed-oriented grid: a power of two at most 25 % above fftFit's size, else fftFit's size."""
+    import adda_amd
+    L = adda_amd.hip()
+    assert [L.adda_hip_fastgrid(x, 1) for x in (104, 256, 640, 96, 120, 32)] == [128, 256, 640, 96, 128, 32]
+    for x in range(2, 2100):
+        n, g = L.adda_hip_fftfit(x, 1), L.adda_hip_fastgrid(x, 1)
+        assert g >= x and g % 2 == 0
+        assert g == n or (g & (g - 1) == 0 and 4 * g <= 5 * n), (x, n, g)
+    assert adda_amd.fast_grid((64, 52, 32)) == (128, 128, 64)
+
+
 @pytest.mark.parametrize("shape,grid,ay,az,nd", [("sphere", 16, 1, 1, 2176), ("ellipsoid", 64, 0.8, 0.5, None),
                                                   ("box", 10, 0.7, 0.4, None), ("sphere", 128, 1, 1, 1099136)])
 def test_particle_builder_matches_restatement(shape, grid, ay, az, nd):

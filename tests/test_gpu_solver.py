@@ -162,6 +162,24 @@ def test_mixed_precision_fp64_finish(hipdev, method, v64):
         pr.close()
 
 
+def test_grid_choice_does_not_change_results(hipdev):
+    """Any FFT grid of at least twice the box gives the same operator up to rounding (adda_hip_fastgrid relies on it):
+    the sphere16 solve on ADDA's 32^3 grid and on a 48x40x36 grid agree to 1e-9 in Cext."""
+    pos, _ = G.geometry("sphere16")
+    p, box = read_shape(pos)
+    out = []
+    for grid in (None, (48, 40, 36)):
+        pr = hipdev.Problem(p, box, 1.5, dpl=15, grid=grid)
+        try:
+            pr.set_polarization(PROP_Z, POL_Y)
+            assert pr.solve("qmr", eps=1e-12)["rc"] == 0
+            out.append((pr.cext(), pr.cabs(), pr.grid))
+        finally:
+            pr.close()
+    assert out[0][2] == (32, 32, 32) and out[1][2] == (48, 40, 36)
+    assert abs(out[0][0] - out[1][0]) / out[0][0] < 1e-9
+
+
 def test_recalc_resid_and_init_fields(hipdev):
     pr = _problem(hipdev, "sphere8", 1.1 + 0.1j, 10)
     try:

@@ -267,8 +267,10 @@ def bench_batched(A, args):
                                           "pass_us": [round(x * 1e3, 1) for x in passes]}
         res = {}
         for method in ("bicgstab", "qmr"):
-            for name, kw in (("fp64", {}), ("mixed", {"precision": A.PREC_MIXED}),
-                             ("fp64_nograph", {"graph": False})):
+            variants = [("fp64", {}), ("mixed", {"precision": A.PREC_MIXED}), ("fp64_nograph", {"graph": False})]
+            if method == "qmr":  # QMR_CS's FP32 Lanczos vectors are fragile: also FP64 vectors around FP32 transforms
+                variants.append(("mixed_v64", {"precision": A.PREC_MIXED_V64}))
+            for name, kw in variants:
                 bp.solve(method, eps=1e-10, **kw)  # warm-up (graph capture, allocations)
                 op.sync()
                 t = time.perf_counter()
@@ -284,6 +286,12 @@ def bench_batched(A, args):
                            "iterations_mixed_max": max(s["niter"] for s in res[(method, "mixed")][2]),
                            "max_true_resid_mixed": max(s["true_resid"] for s in res[(method, "mixed")][2]),
                            "cext_max_rel_err_mixed_vs_fp64": float(np.max(np.abs(cm - c64) / np.abs(c64)))}
+            if (method, "mixed_v64") in res:
+                tv, cv, sv = res[(method, "mixed_v64")]
+                out[method]["fp64_vectors_fp32_transforms"] = {
+                    "tts_batched_s": round(tv, 4), "iterations_max": max(s["niter"] for s in sv),
+                    "max_true_resid": max(s["true_resid"] for s in sv),
+                    "cext_max_rel_err_vs_fp64": float(np.max(np.abs(np.array(cv) - c64) / np.abs(c64)))}
         # the same RHS one after another on one operator (ADDA's loop over orientations and polarizations)
         pr = A.Problem(pos, box, m, fp32=True, grid=grid)
         try:

@@ -208,12 +208,14 @@ def test_maxiter_warning(hipdev, method, host_scalars):
         pr.close()
 
 
-def test_cli_end_to_end(hipdev, tmp_path):
-    """adda_hip -shape read ... reproduces the reference's CrossSec-Y/X and mueller files (comp2exec tolerances)."""
+@pytest.mark.parametrize("prec", ["fp64", "mixed64"])
+def test_cli_end_to_end(hipdev, tmp_path, prec):
+    """adda_hip -shape read ... reproduces the reference's CrossSec-Y/X and mueller files (comp2exec tolerances), also
+    with the mixed form whose FP64 iterations run FP32 transforms (-prec mixed64)."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "a-dda_amd", "adda_hip")
     cmd = [exe, "-shape", "read", G.path("geom", "sphere16.geom"), "-dpl", "15", "-m", "1.5", "0", "-iter", "qmr",
-           "-eps", "10", "-dir", str(tmp_path / "run")]
+           "-eps", "10", "-prec", prec, "-dir", str(tmp_path / "run")]
     subprocess.run(cmd, check=True, capture_output=True, timeout=120)
     for pol in ("Y", "X"):
         a = G.crosssec("c1_qmr", pol)

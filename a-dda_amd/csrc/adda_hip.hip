@@ -671,13 +671,17 @@ template <typename T, typename TV>
 static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const double *gate, const cx<TV> *arg,
 	cx<TV> *res, const cx<T> *ccs, int her, double *partials, const cx<TV> *dotw = nullptr, int dmode = 0)
 {
-	const int nyt = (op->bY + c.LY - 1) / c.LY, nblk5 = op->bZ * nyt;
+	// The variant with the fused inner product holds more registers: it runs half-height tiles, which keeps about as
+	// many waves per CU (the tile height is free here, it does not change the Xa layout)
+	MvCfg cd = c;
+	if (dmode && cd.LY > 2 && (3 * (cd.LY / 2) * (op->gX / c.qx.rmax)) % 64 == 0) cd.LY /= 2;  // whole waves only
+	const int nyt = (op->bY + cd.LY - 1) / cd.LY, nblk5 = op->bZ * nyt;
 	constexpr bool same = std::is_same_v<T, TV>;
 	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
 	auto k = (Fn)c.kx.k[dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
-	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)), lds_p15(c, op->gX, sizeof(cx<T>)),
+	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
-		op->bY, c.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
+		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
 	return nblk5;
 }
 

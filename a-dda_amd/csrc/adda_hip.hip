@@ -2116,7 +2116,8 @@ extern "C" int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg)
 }
 
 // Candidate radix schedules for length n: the power of two split into radix-16/8/4/2 stages (at most 4 stages,
-// non-increasing and non-decreasing orders), the radix-5 and radix-3 stages after or before them.
+// non-increasing and non-decreasing orders), the radix-5 and radix-3 stages after them, before them, or after the first
+// of them (an even first and last radix lets the transforms skip the zero-padded half, see fft_core.h).
 static void enum_plans(int n, std::vector<FftPlan2> &out)
 {
 	int m = n, k = 0, r3 = 0, r5 = 0;
@@ -2147,12 +2148,18 @@ static void enum_plans(int n, std::vector<FftPlan2> &out)
 				std::reverse(v2.begin(), v2.end());
 			}
 			for (int e : v2) rad.push_back(1 << e);
-			for (int place = 0; place < 2; place++) {
+			for (int place = 0; place < 3; place++) {
 				if (place && r3 + r5 == 0) continue;
-				std::vector<int> full;
-				if (place) { for (int i = 0; i < r5; i++) full.push_back(5); for (int i = 0; i < r3; i++) full.push_back(3); }
-				full.insert(full.end(), rad.begin(), rad.end());
-				if (!place) { for (int i = 0; i < r5; i++) full.push_back(5); for (int i = 0; i < r3; i++) full.push_back(3); }
+				if (place == 2 && rad.size() < 2) continue;
+				std::vector<int> full, odd;
+				for (int i = 0; i < r5; i++) odd.push_back(5);
+				for (int i = 0; i < r3; i++) odd.push_back(3);
+				if (place == 1) full = odd;
+				for (size_t i = 0; i < rad.size(); i++) {
+					full.push_back(rad[i]);
+					if (place == 2 && i == 0) full.insert(full.end(), odd.begin(), odd.end());
+				}
+				if (place == 0) full.insert(full.end(), odd.begin(), odd.end());
 				if (full.empty() || (int)full.size() > kMaxStages) continue;
 				FftPlan2 p;
 				p.n = n;

@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--matvec-only", action="store_true",
                     help="profiling mode: autotune, 20 matvecs per precision and one 52.8 MB device copy (the "
                          "calibration kernel for HBM counters), no solves; prints the matvec figures only")
+    ap.add_argument("--solve-only", choices=["fp64", "mixed"],
+                    help="profiling mode: autotune, then one BiCGStab solve to 1e-10 in the given form; prints its time")
     ap.add_argument("--qmr-max-inner", type=int, default=2000)
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-RHS and adjoint-gradient workloads")
     ap.add_argument("--tune-cache", default=os.path.join(ROOT, "a-dda_amd", "tune_cache_mi355x.jsonl"),
@@ -80,6 +82,18 @@ def main():
     tune = {name: op.autotune(prec, reps=3, cache=args.tune_cache) for prec, name in ((A.F64, "fp64"), (A.F32, "fp32"))}
     t_tune = time.perf_counter() - t0
     log(f"autotune {t_tune:.2f} s: " + "; ".join(f"{k}: {v}" for k, v in tune.items()))
+
+    if args.solve_only:
+        prec = A.PREC_FP64 if args.solve_only == "fp64" else A.PREC_MIXED
+        op.sync()
+        t = time.perf_counter()
+        res = pr.solve("bicgstab", eps=1e-10, precision=prec)
+        dt = time.perf_counter() - t
+        print(json.dumps({"mode": "solve-only", "form": args.solve_only, "s": round(dt, 4), "iterations": res["niter"],
+                          "matvecs": res["nmatvec"], "fp64_matvecs": res["nmatvec_fp64"], "outer": res["nouter"],
+                          "config": tune}), flush=True)
+        pr.close()
+        return
 
     # ---- matvec throughput and per-pass timing (HIP events on the operator's stream) --------------------------------
     mv = {}

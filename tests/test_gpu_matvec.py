@@ -192,7 +192,7 @@ COMPILED = {  # length -> compile-time schedules (mv_*.hip) + one run-time sched
     32: ["8.4", "4.8", "2.16"],
     96: ["8.4.3", "16.2.3", "3.8.4", "4.8.3"],
     108: ["4.3.3.3", "3.3.3.4", "2.2.3.3.3"],
-    640: ["16.8.5", "8.16.5", "5.16.8", "4.4.8.5"],
+    640: ["16.8.5", "8.16.5", "5.16.8", "16.5.8", "8.5.16", "4.4.8.5"],
 }
 
 

@@ -5,10 +5,12 @@
  *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
  *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
+ *            [-store_dip_pol] [-store_int_field]
  *
  * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
  * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
- * mueller and log in ADDA's formats (GFORM/EFORM, src/const.h:142-146; src/CalculateE.c:236-246,758-759).
+ * mueller and log in ADDA's formats (GFORM/EFORM, src/const.h:142-146; src/CalculateE.c:236-246,758-759), and on
+ * request DipPol-Y/X and IntField-Y/X (StoreFields, src/CalculateE.c:617-681,833-877).
  */
 #include <math.h>
 #include <stdio.h>
@@ -41,6 +43,39 @@ static void usage(void)
 }
 
 /* InitRotation, zyz Euler angles in degrees: src/crosssec.c:82-138 */
+/* StoreFields (src/CalculateE.c:617-681) for a complex field on the dipoles: a header of column labels, then per dipole
+ * its coordinates (DipoleCoord: (position - (box-1)/2)*gridspace, src/make_particle.c:1979-1981,2266-2275), |F|^2
+ * and the real and imaginary parts of the three components, all GFORM. scale: per-material factor applied to the field
+ * (chi_inv for the internal field E = chi_inv*P, src/CalculateE.c:833-840), or NULL. */
+static void store_field(const char *dir, const char *prefix, const char *pol, const char *name, const adda_particle *p,
+	double gridspace, const adda_cplx *v, const adda_cplx *scale)
+{
+	char fn[1024];
+	snprintf(fn, sizeof fn, "%s/%s-%s", dir, prefix, pol);
+	FILE *f = fopen(fn, "w");
+	if (!f) die("writing a field file");
+	fprintf(f, "x y z |%s|^2 %sx.r %sx.i %sy.r %sy.i %sz.r %sz.i\n", name, name, name, name, name, name, name);
+	const double c[3] = {(p->boxX - 1) / 2.0, (p->boxY - 1) / 2.0, (p->boxZ - 1) / 2.0};
+	for (size_t i = 0; i < p->nd; i++) {
+		double re[3], im[3], n2 = 0;
+		for (int k = 0; k < 3; k++) {
+			adda_cplx a = v[3 * i + k];
+			if (scale) {
+				const adda_cplx s = scale[p->material[i]];
+				const adda_cplx b = {s.re * a.re - s.im * a.im, s.re * a.im + s.im * a.re};
+				a = b;
+			}
+			re[k] = a.re;
+			im[k] = a.im;
+			n2 += a.re * a.re + a.im * a.im;
+		}
+		fprintf(f, "%.10g %.10g %.10g %.10g %.10g %.10g %.10g %.10g %.10g %.10g\n",
+			(p->position[3 * i] - c[0]) * gridspace, (p->position[3 * i + 1] - c[1]) * gridspace,
+			(p->position[3 * i + 2] - c[2]) * gridspace, n2, re[0], im[0], re[1], im[1], re[2], im[2]);
+	}
+	fclose(f);
+}
+
 static void rotate(double a_deg, double b_deg, double g_deg, const double in[3], double out[3])
 {
 	double a = a_deg * PI / 180, b = b_deg * PI / 180, g = g_deg * PI / 180;
@@ -56,6 +91,7 @@ int main(int argc, char **argv)
 	const char *shape = "sphere", *geom = NULL, *dir = "run_adda_hip";
 	double asp_y = 1, asp_z = 1, size = -1, dpl = -1, lambda = 2 * PI, eps_k = 5, eps_inner_k = 0;
 	int grid = 16, maxiter = -1, ntheta = -1, gpu = 0, reduced = 1, have_orient = 0;
+	int store_dip_pol = 0, store_int_field = 0;
 	double orient[3] = {0, 0, 0};
 	adda_cplx m[16];
 	int nm = 0;
@@ -119,6 +155,8 @@ int main(int argc, char **argv)
 		else if (!strcmp(a, "-host_scalars")) opt.host_scalars = 1;
 		else if (!strcmp(a, "-max_inner")) { NEED(1); opt.max_inner = atoi(argv[++i]); }
 		else if (!strcmp(a, "-graph")) opt.graph = 1;
+		else if (!strcmp(a, "-store_dip_pol")) store_dip_pol = 1;
+		else if (!strcmp(a, "-store_int_field")) store_int_field = 1;
 		else if (!strcmp(a, "-prec")) {
 			NEED(1);
 			++i;
@@ -231,6 +269,23 @@ int main(int argc, char **argv)
 		if (adda_abs_cross(op, &p, wavenum, cc, pv, &cabs)) die("AbsCross");
 		if (adda_eplane_yz(op, &p, wavenum, gridspace, pv, prop, incY, incX, ntheta, npoints, pol == 0 ? eplY : eplX))
 			die("CalcEplaneYZ");
+		if (store_dip_pol || store_int_field) { /* CalculateE, src/CalculateE.c:875-877 */
+			adda_cplx *P = malloc(3 * p.nd * sizeof(adda_cplx)), *chi_inv = malloc(p.nmat * sizeof(adda_cplx));
+			if (!P || !chi_inv || adda_hip_vec_download(op, ADDA_F64, P, pv)) die("downloading P");
+			if (store_int_field) { /* chi_inv = 4 pi/(V (m^2-1)), src/calculator.c:275-278 */
+				const double V = gridspace * gridspace * gridspace;
+				for (int i = 0; i < p.nmat; i++) {
+					const double ar = m[i].re * m[i].re - m[i].im * m[i].im - 1, ai = 2 * m[i].re * m[i].im;
+					const double d = V * (ar * ar + ai * ai);
+					chi_inv[i].re = 4 * PI * ar / d;
+					chi_inv[i].im = -4 * PI * ai / d;
+				}
+				store_field(dir, "IntField", nm_, "E", &p, gridspace, P, chi_inv);
+			}
+			if (store_dip_pol) store_field(dir, "DipPol", nm_, "P", &p, gridspace, P, NULL);
+			free(P);
+			free(chi_inv);
+		}
 		snprintf(fn, sizeof fn, "%s/CrossSec-%s", dir, nm_);
 		FILE *f = fopen(fn, "w");
 		if (!f) die("writing CrossSec");

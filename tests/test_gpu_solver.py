@@ -162,6 +162,37 @@ def test_mixed_precision_fp64_finish(hipdev, method, v64):
         pr.close()
 
 
+def test_cli_field_files(hipdev, tmp_path):
+    """adda_hip -store_dip_pol -store_int_field on the reference's ell12 orientation run: DipPol-Y/X equal the
+    reference's files under comp2exec's DipPol rule (abs 1e-12 / rel 1e-6, coordinates included); IntField holds
+    E = chi_inv P with chi_inv = 4 pi/(V (m^2 - 1)) (src/calculator.c:275-278) and the same layout."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "a-dda_amd", "adda_hip")
+    run = tmp_path / "run"
+    cmd = [exe, "-shape", "read", G.path("geom", "ell12.geom"), "-dpl", "12", "-m", "1.33", "0.05", "-orient", "30",
+           "40", "50", "-iter", "qmr", "-eps", "10", "-ntheta", "19", "-store_dip_pol", "-store_int_field",
+           "-dir", str(run)]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    m = 1.33 + 0.05j
+    d = 2 * np.pi / 12
+    chi_inv = 4 * np.pi / (d ** 3 * (m * m - 1))
+    for pol in ("Y", "X"):
+        with open(run / ("DipPol-" + pol)) as f:
+            assert f.readline().split() == ["x", "y", "z", "|P|^2", "Px.r", "Px.i", "Py.r", "Py.i", "Pz.r", "Pz.i"]
+        ours = np.loadtxt(run / ("DipPol-" + pol), skiprows=1)
+        ref = np.loadtxt(G.path("ell12_orient", "DipPol-" + pol), skiprows=1)
+        assert ours.shape == ref.shape
+        assert G.numeric_close(ours, ref, 1e-12, 1e-6)[0]
+        with open(run / ("IntField-" + pol)) as f:
+            assert f.readline().split()[3:5] == ["|E|^2", "Ex.r"]
+        e = np.loadtxt(run / ("IntField-" + pol), skiprows=1)
+        P = ours[:, 4::2] + 1j * ours[:, 5::2]
+        E = e[:, 4::2] + 1j * e[:, 5::2]
+        assert np.allclose(e[:, :3], ours[:, :3])
+        assert np.abs(E - chi_inv * P).max() <= 1e-9 * np.abs(E).max()
+        assert np.allclose(e[:, 3], np.sum(np.abs(E) ** 2, axis=1), rtol=1e-9)
+
+
 def test_grid_choice_does_not_change_results(hipdev):
     """Any FFT grid of at least twice the box gives the same operator up to rounding (adda_hip_fastgrid relies on it):
     the sphere16 solve on ADDA's 32^3 grid and on a 48x40x36 grid agree to 1e-9 in Cext."""

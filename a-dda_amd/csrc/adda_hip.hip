@@ -120,8 +120,9 @@ struct MvCfg {
 	MvKernels kx, ky, kz; // the kernels that run those schedules (resolve())
 	int LY;               // y rows per block in P1/P5 (3*LY lines: all components)
 	int TC;               // columns per block in P2/P4
-	int LP;               // x-pairs per block in P3
+	int LP;               // column units per block in P3
 	int p3r = 0;          // P3 with the D-matrix product in registers (k_p3r; compile-time power-of-two y plans)
+	int p3x = 1;          // P3 blocks hold column x with its mirror gridX - x (reduced D-matrix; else one column each)
 };
 
 struct TwPair {
@@ -136,7 +137,7 @@ struct adda_hip_op {
 	size_t nd;
 	int nmat;
 	int reduced;
-	int DsY, DsZ;
+	int DsX, DsY, DsZ;  // stored D-matrix extents (gX/2+1, gY/2+1, gZ/2+1 with the reduced form, else the grid)
 	double wavenum, gridspace;
 	int flags;
 	int mv32 = 0;  // FP64 matvecs run their transforms in FP32 (adda_hip_set_matvec_mode)
@@ -149,7 +150,7 @@ struct adda_hip_op {
 	// coupling: one cc_sqrt table per right-hand side of a batched solve
 	cx<double> *cc64 = nullptr;  // [kcap][nmat][3]
 	cx<float> *cc32 = nullptr;
-	// D-matrix, layout [zt<DsZ][yt<DsY][x<gX][6]
+	// D-matrix, layout [zt<DsZ][xt<DsX][yt<DsY][6]
 	cx<double> *D64 = nullptr;
 	cx<float> *D32 = nullptr;
 	// work buffers, kcap right-hand sides (FP64-sized; FP32 uses the first half)
@@ -271,13 +272,15 @@ __global__ void k_green(cx<double> *G, int bX, int bY, int bZ, int gX, int gY, i
 	}
 }
 
-// D[zt][x][yt][c] = -G~[c][zt][yt][x]/Ngrid for zt<DsZ, yt<DsY (src/fft.c:1310-1314)
-__global__ void k_extract_d(const cx<double> *G, cx<double> *D, int gX, int gY, int gZ, int DsY, int DsZ)
+// D[zt][xt][yt][c] = -G~[c][zt][yt][xt]/Ngrid for zt<DsZ, xt<DsX, yt<DsY (src/fft.c:1310-1314). With the reduced
+// form ADDA keeps every x (its x-slabs are distributed over processes); here x is reduced like y and z, since G's
+// parity under x -> -x (even for xx, yy, yz, zz; odd for xy, xz) carries over to its transform.
+__global__ void k_extract_d(const cx<double> *G, cx<double> *D, int gX, int gY, int gZ, int DsX, int DsY, int DsZ)
 {
-	const size_t n = (size_t)gX * gY * gZ, m = (size_t)DsZ * DsY * gX;
+	const size_t n = (size_t)gX * gY * gZ, m = (size_t)DsZ * DsY * DsX;
 	const double invN = 1.0 / (double)n;
 	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m; idx += (size_t)gridDim.x * blockDim.x) {
-		const int y = (int)(idx % DsY), x = (int)((idx / DsY) % gX), z = (int)(idx / ((size_t)gX * DsY));
+		const int y = (int)(idx % DsY), x = (int)((idx / DsY) % DsX), z = (int)(idx / ((size_t)DsX * DsY));
 		const size_t src = ((size_t)z * gY + y) * gX + x;
 		for (int c = 0; c < 6; c++) {
 			cx<double> v = G[c * n + src];
@@ -595,7 +598,7 @@ static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
 	// twiddles + lines (+ the block's D rows for the register variant, which always reserves both planes' lines)
 	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride_lm(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
-	return (size_t)(gY + 3 * (reduced ? 2 : 1) * c.LP * rowstride_lm(gY)) * es;
+	return (size_t)(gY + 3 * p3_groups(reduced, c.p3x) * c.LP * rowstride_lm(gY)) * es;
 }
 static constexpr size_t kLdsMax = 96 * 1024;
 
@@ -607,10 +610,10 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 		if (!k.k[MV_P3R][es == 8] || 2 * c.LP * (op->gY / c.qy.rmax) > 256) return false;
 	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
-	const int ml = op->reduced ? 2 : 1;
-	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && op->gX % c.LP == 0 && 3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 &&
-		3 * ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax && lds_p24(c, op->gZ, es) <= kLdsMax &&
-		lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
+	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
+	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
+		3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 && ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax &&
+		lds_p24(c, op->gZ, es) <= kLdsMax && lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
 }
 
 // Per-RHS strides of the work buffers (complex elements)
@@ -657,12 +660,14 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
-	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, ml3 = op->reduced ? 2 : 1;
+	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ;
 	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : MV_P3][sizeof(T) == 4];
-	const int nthr = (c.p3r ? 2 : 3 * ml3) * c.LP * (op->gY / c.qy.rmax);
-	hipLaunchKernelGGL(k, dim3(nz3 * (op->gX / c.LP) * K), dim3(nthr),
+	// k_p3: 3 lines per (plane, column) group, LP column units per block; k_p3r: LP columns, 2*LP groups
+	const int nthr = (c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x)) * c.LP * (op->gY / c.qy.rmax);
+	const int ntile = c.p3r ? op->gX / c.LP : (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
+	hipLaunchKernelGGL(k, dim3(nz3 * ntile * K), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
-		op->gX, op->bY, op->gZ, op->DsY, op->DsZ, c.LP, op->reduced, transposed,
+		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
 		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
 }
 
@@ -746,8 +751,8 @@ static void resolve(MvCfg &c)
 	c.kz = mv_kernels(c.qz);
 }
 
-// Default tiles for element size es: the largest (LY <= 8, TC <= 16, LP <= 4 dividing gridX) fitting 512 threads and
-// 64 KB of LDS, with the default radix schedules of make_plan2.
+// Default tiles for element size es: the largest (LY <= 8, TC <= 16, LP <= 4) fitting 512 threads and 64 KB of LDS,
+// with the default radix schedules of make_plan2.
 static void default_cfg(adda_hip_op *op, int pi, size_t es)
 {
 	MvCfg &c = op->cfg[pi];
@@ -761,9 +766,16 @@ static void default_cfg(adda_hip_op *op, int pi, size_t es)
 	while (c.LY > 1 && (3 * c.LY * tplx > 512 || lds_p15(c, gX, es) > cap)) c.LY /= 2;
 	c.TC = 16;
 	while (c.TC > 2 && (c.TC * tplz > 512 || lds_p24(c, gZ, es) > cap)) c.TC /= 2;
-	const int ml = op->reduced ? 2 : 1;
-	c.LP = 4;
-	while (c.LP > 1 && (3 * ml * c.LP * tply > 512 || lds_p3(c, gY, op->reduced, es) > cap || gX % c.LP)) c.LP--;
+	// P3: the largest LP with column pairs, else single columns (a long y line in FP64)
+	for (c.p3x = 1; c.p3x >= 0; c.p3x--) {
+		c.LP = 4;
+		auto fits = [&] {
+			return 3 * p3_groups(op->reduced, c.p3x) * c.LP * tply <= 512 && lds_p3(c, gY, op->reduced, es) <= cap;
+		};
+		while (c.LP > 1 && !fits()) c.LP--;
+		if (fits()) break;
+	}
+	if (c.p3x < 0) c.p3x = 0;
 	resolve(c);
 }
 
@@ -1390,9 +1402,10 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	op->vs = (3 * op->nd + 63) / 64 * 64;
 	if (reserve(op, 1)) return -1;
 	// D-matrix: G on the full grid, 3D FFT (x rows, y and z columns), extraction of the stored octant
+	op->DsX = op->reduced ? gX / 2 + 1 : gX;
 	op->DsY = op->reduced ? gY / 2 + 1 : gY;
 	op->DsZ = op->reduced ? gZ / 2 + 1 : gZ;
-	const size_t ng = (size_t)gX * gY * gZ, nD = (size_t)op->DsZ * op->DsY * gX * 6;
+	const size_t ng = (size_t)gX * gY * gZ, nD = (size_t)op->DsZ * op->DsY * op->DsX * 6;
 	cx<double> *G = nullptr;
 	HIPCHK(hipMalloc(&G, 6 * ng * sizeof(cx<double>)));
 	hipLaunchKernelGGL(k_green, dim3(4096), dim3(256), 0, op->stream, G, bX, bY, bZ, gX, gY, gZ, op->wavenum,
@@ -1419,7 +1432,8 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 		HIPCHK(hipGetLastError());
 	}
 	HIPCHK(hipMalloc(&op->D64, nD * sizeof(cx<double>)));
-	hipLaunchKernelGGL(k_extract_d, dim3(4096), dim3(256), 0, op->stream, G, op->D64, gX, gY, gZ, op->DsY, op->DsZ);
+	hipLaunchKernelGGL(k_extract_d, dim3(4096), dim3(256), 0, op->stream, G, op->D64, gX, gY, gZ, op->DsX, op->DsY,
+		op->DsZ);
 	HIPCHK(hipGetLastError());
 	if (op->flags & 1) {
 		HIPCHK(hipMalloc(&op->D32, nD * sizeof(cx<float>)));
@@ -1512,17 +1526,26 @@ extern "C" int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst)
 {
 	if (!op || !dst) return ERR("null argument");
 	HIPCHK(hipSetDevice(op->dev));
-	const size_t n = adda_hip_dmatrix_size(op);
+	const size_t n = (size_t)6 * op->DsX * op->DsY * op->DsZ;
 	std::vector<adda_cplx> tmp(n);
 	HIPCHK(hipMemcpyAsync(tmp.data(), op->D64, n * sizeof(adda_cplx), hipMemcpyDeviceToHost, op->stream));
 	HIPCHK(hipStreamSynchronize(op->stream));
-	// device [zt][x][yt][6] -> ADDA [x][zt][yt][6]
+	// device [zt][xt][yt][6] -> ADDA [x][zt][yt][6]; columns x >= DsX of the reduced form are the stored column
+	// gX - x with the xy and xz elements negated
 	for (int z = 0; z < op->DsZ; z++)
-		for (int x = 0; x < op->gX; x++)
+		for (int x = 0; x < op->gX; x++) {
+			const bool m = x >= op->DsX;
+			const int xt = m ? op->gX - x : x;
 			for (int y = 0; y < op->DsY; y++)
-				for (int c = 0; c < 6; c++)
-					dst[6 * (((size_t)x * op->DsZ + z) * op->DsY + y) + c] =
-						tmp[(((size_t)z * op->gX + x) * op->DsY + y) * 6 + c];
+				for (int c = 0; c < 6; c++) {
+					adda_cplx v = tmp[(((size_t)z * op->DsX + xt) * op->DsY + y) * 6 + c];
+					if (m && (c == 1 || c == 2)) {
+						v.re = -v.re;
+						v.im = -v.im;
+					}
+					dst[6 * (((size_t)x * op->DsZ + z) * op->DsY + y) + c] = v;
+				}
+		}
 	return 0;
 }
 
@@ -1878,7 +1901,7 @@ extern "C" double adda_hip_matvec_bytes_batch(const adda_hip_op *op, int prec, i
 	const double gX = op->gX, bY = op->bY, bZ = op->bZ, gZ = op->gZ, nd = (double)op->nd;
 	const double nbox = (double)op->bX * bY * bZ;
 	const double xa = 3 * gX * bY * bZ * c * K, xb = 3 * gX * bY * gZ * c * K, vec = 3 * nd * c * K;
-	const double dm = 6 * gX * op->DsY * op->DsZ * c;
+	const double dm = 6.0 * op->DsX * op->DsY * op->DsZ * c;
 	// index data: cell map (4 B per box cell) + material (4 B per dipole), read by P1 and by P5
 	const double idx = 4 * nbox + 4 * nd;
 	const double b[5] = {vec + idx + xa, xa + xb, 2 * xb + dm, xb + xa, xa + 2 * vec + idx};
@@ -2070,7 +2093,8 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "");
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0");
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2083,8 +2107,9 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.LY = atoi(ly + 3);
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
-	const char *r3 = strstr(s, "R3=");
+	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
+	c.p3x = xp ? atoi(xp + 3) : 1;
 	return true;
 }
 
@@ -2237,19 +2262,22 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			float t = timeit([&] { launch_zfft<T, false>(op, c, K, nullptr); launch_zfft<T, true>(op, c, K, nullptr); });
 			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
 		}
-	// y pass (P3): radix schedule of y, LP and the register-product variant
+	// y pass (P3): radix schedule of y, LP, column pairs and the register-product variant
 	for (auto &q : py)
 		for (int LP : {1, 2, 3, 4, 6, 8})
+		for (int xp : {1, 0})
 		for (int r3 : {0, 1}) {
+			if (!xp && !op->reduced) continue;  // without the reduced form every block holds single columns
 			MvCfg c = best;
 			c.qy = q;
 			c.LP = LP;
 			c.p3r = r3;
+			c.p3x = xp;
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
-			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; }
+			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; }
 		}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
@@ -2283,8 +2311,8 @@ extern "C" int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int rep
 	HIPCHK(hipSetDevice(op->dev));
 	if (reps < 1) reps = 3;
 	char key[200];
-	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d reduced %d prec %s", op->gX, op->gY, op->gZ, op->bX,
-		op->bY, op->bZ, op->reduced, prec == ADDA_F32 ? "fp32" : "fp64");
+	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d dmatrix %s prec %s", op->gX, op->gY, op->gZ,
+		op->bX, op->bY, op->bZ, op->reduced ? "octant" : "full", prec == ADDA_F32 ? "fp32" : "fp64");
 	if (K > 1) snprintf(key + kl, sizeof key - kl, " batch %d", K);
 	if (cache_path) {
 		FILE *f = fopen(cache_path, "r");

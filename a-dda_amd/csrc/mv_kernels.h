@@ -188,68 +188,88 @@ template <typename T, int S> __device__ __forceinline__ cx<T> cmac(cx<T> acc, cx
 }
 
 // Symmetric 3x3 product (cSymMatrVec, src/cmplx.h:369-376) at one frequency, in place in LDS (the x, y, z components
-// at q0, q1, q2). D = (d0 f1 f2; f1 d3 f4; f2 f4 d5) with f1, f4 already flipped for the y mirror; Z = -1 flips the xz
-// and yz elements for the z mirror plane (src/matvec.c:270-305).
-template <typename T, int Z>
+// at q0, q1, q2). D = (d0 f1 f2; f1 d3 f4; f2 f4 d5) with f1, f4 already flipped for the y mirror. The mirror planes
+// of the stored octant flip the elements that are odd in the mirrored coordinate (src/matvec.c:270-305 for y and z;
+// x alike, G being odd in x for xy and xz): SX = -1 flips xy and xz (column gridX - x), SZ = -1 flips xz and yz
+// (plane gridZ - z). The compile-time signs fold into the FMA operands.
+template <typename T, int SX, int SZ>
 __device__ __forceinline__ void sym3_lds(cx<T> *q0, cx<T> *q1, cx<T> *q2, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3,
 	cx<T> f4, cx<T> d5)
 {
 	const cx<T> a0 = *q0, a1 = *q1, a2 = *q2;
-	*q0 = cmac<T, Z>(cmac<T, 1>(mul(d0, a0), f1, a1), d2, a2);
-	*q1 = cmac<T, Z>(cmac<T, 1>(mul(f1, a0), d3, a1), f4, a2);
-	*q2 = cmac<T, Z>(cmac<T, Z>(mul(d5, a2), f4, a1), d2, a0);
+	*q0 = cmac<T, SX * SZ>(cmac<T, SX>(mul(d0, a0), f1, a1), d2, a2);
+	*q1 = cmac<T, SZ>(cmac<T, SX>(mul(d3, a1), f1, a0), f4, a2);
+	*q2 = cmac<T, SX * SZ>(cmac<T, SZ>(mul(d5, a2), f4, a1), d2, a0);
 }
 
-// P3: block = LP consecutive x at one z, and, with the reduced D-matrix, also at the mirror plane z' = gridZ - z,
-// which uses the same stored D rows (IndexDmatrix_mv, src/matvec.c:87-103) -- so every stored D element is read from
-// HBM once per matvec. The RHS index is fastest in the block index: the K blocks that share a D tile run together
-// and the tile is served from cache after its first read. Lines (s, pair, c) = (s*LP + pair)*3 + c are contiguous
-// boxY runs of Xb (stride 3, components interleaved). Forward FFT-y into LDS (natural order), the D-matrix product
-// (src/matvec.c:270-305: reduced-FFT mirror indices and sign flips, cSymMatrVec src/cmplx.h:369-376), inverse FFT-y,
+// P3. With the reduced D-matrix the stored part is an octant: z < DsizeZ, y < DsizeY (IndexDmatrix_mv,
+// src/matvec.c:87-103) and also x < DsizeX = gridX/2 + 1, since G is even or odd in x like in y and z (ADDA keeps x
+// whole for its x-slab distribution). A block takes LP consecutive column units u at one plane z <= gridZ/2. With
+// column pairs (xpair) unit u is column x = u and its mirror gridX - u, at plane z and at the mirror plane gridZ - z --
+// up to four (plane, column) groups that share every stored D element, so each is read from HBM once per matvec.
+// Without them (a block too large for LDS otherwise) unit u is column u alone, which reads the stored column
+// gridX - u when u >= DsizeX. Group g = s*LP + pr (bit 0 of s: plane gridZ - z, bit 1: column gridX - u) holds lines
+// 3g + c; bit g of `pm` says whether the group exists (not for self-mirrored planes or columns, nor beyond the last
+// unit). The RHS index is fastest in the block index: the K
+// blocks that share a D tile run together and the tile is served from cache after its first read. Lines are
+// contiguous boxY runs of Xb (stride 3, components interleaved). Forward FFT-y into LDS (natural order), the D-matrix
+// product (src/matvec.c:270-305: mirror indices and sign flips, cSymMatrVec src/cmplx.h:369-376), inverse FFT-y,
 // keep y < boxY. The box occupies at most half of every grid line (gridY >= 2*boxY), so with a compile-time plan the
 // forward transform skips the zero upper half and the inverse computes only the lower half; FULL (boxY == gridY/2,
 // a plan with both half options) drops the y < boxY checks, which then hold for every index the transforms touch.
+// Without the reduced D-matrix every group is one column at one plane (s = 0).
 template <typename T, class PL, bool FULL>
 __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp,
-	const cx<T> *__restrict__ D, int x0, int z, int gX, int bY, int gZ, int DsY, int LP, int NL1, bool has2,
-	int reduced, int transposed, int line, int t, bool act)
+	const cx<T> *__restrict__ D, int u0, int z, int gX, int bY, int gZ, int DsX, int DsY, int LP, int NL1,
+	unsigned pm, int reduced, int transposed, int line, int t, bool act)
 {
 	// padded rows: their offsets stay immediates (a swizzle would cost P3, which is close to VALU-bound, an address
 	// computation per access)
 	using LAY = LayPad;
 	const int N = PL::n(p), LS = rowstride_lm(N);
-	// D-matrix row of element idx = (pair, y) of the block: the stored row index (mirrored for the reduced form and for
-	// the transposed product, src/matvec.c:181 and 270-305) and the y-mirror sign mask of the xy and yz elements
+	// D-matrix row of element idx = (pr, y) of the block: the stored row index (mirrored for the reduced form and for
+	// the transposed product, src/matvec.c:181 and 270-305) and the sign masks of the y mirror (xy and yz elements,
+	// low half of m) and of a single column beyond DsizeX (xy and xz, high half); the block's own plane never mirrors
 	auto drow = [&](int idx, unsigned &m) -> const cx<T> * {
 		const int pr = idx / N, y = idx - pr * N;
-		int xi = x0 + pr, yi = y, zi = z;
+		int xi = u0 + pr, yi = y, zi = z;
 		if (transposed) {  // A^T with the full D-matrix: D(-r) (never with the reduced form)
 			xi = xi > 0 ? gX - xi : 0;
 			yi = yi > 0 ? N - yi : 0;
 			zi = zi > 0 ? gZ - zi : 0;
 		}
 		else if (yi >= DsY) yi = N - yi;
-		m = (reduced && y >= DsY) ? 0x80000000u : 0u;  // the block's own plane z < DsizeZ never mirrors in z
-		return D + (((size_t)zi * gX + xi) * DsY + yi) * 6;
+		const bool xm = reduced && xi >= DsX;
+		if (xm) xi = gX - xi;
+		m = ((reduced && y >= DsY) ? 0x8000u : 0u) | (xm ? 0x80000000u : 0u);
+		return D + (((size_t)zi * DsX + xi) * DsY + yi) * 6;
 	};
 	auto dprod = [&](int idx, const cx<T> *d, unsigned m) {
 		const int pr = idx / N, y = idx - pr * N;
-		const cx<T> f1 = flip<T>(d[1], m), f4 = flip<T>(d[4], m);
+		if (!((pm >> pr) & 1)) return;  // beyond the last unit
+		const unsigned my = m << 16, mx = m & 0x80000000u;
+		const cx<T> f1 = flip<T>(d[1], my ^ mx), f4 = flip<T>(d[4], my), d2 = flip<T>(d[2], mx);
 		auto q = [&](int l) { return buf + LAY::pos(l, y, 1, LS); };
-		sym3_lds<T, 1>(q(3 * pr), q(3 * pr + 1), q(3 * pr + 2), d[0], f1, d[2], d[3], f4, d[5]);
-		if (has2)  // plane gZ - z
-			sym3_lds<T, -1>(q(NL1 + 3 * pr), q(NL1 + 3 * pr + 1), q(NL1 + 3 * pr + 2), d[0], f1, d[2], d[3], f4, d[5]);
+		const int l0 = 3 * pr;
+		sym3_lds<T, 1, 1>(q(l0), q(l0 + 1), q(l0 + 2), d[0], f1, d2, d[3], f4, d[5]);
+		if ((pm >> (LP + pr)) & 1)
+			sym3_lds<T, 1, -1>(q(NL1 + l0), q(NL1 + l0 + 1), q(NL1 + l0 + 2), d[0], f1, d2, d[3], f4, d[5]);
+		if ((pm >> (2 * LP + pr)) & 1)
+			sym3_lds<T, -1, 1>(q(2 * NL1 + l0), q(2 * NL1 + l0 + 1), q(2 * NL1 + l0 + 2), d[0], f1, d2, d[3], f4, d[5]);
+		if ((pm >> (3 * LP + pr)) & 1)
+			sym3_lds<T, -1, -1>(q(3 * NL1 + l0), q(3 * NL1 + l0 + 1), q(3 * NL1 + l0 + 2), d[0], f1, d2, d[3], f4,
+				d[5]);
 	};
 	// In FP32 the D rows of this thread's first KP elements are loaded before the forward transform, which hides their
-	// latency (KP covers all of them with the reduced D-matrix: blockDim = 6*LP*N/RMAX threads for LP*N elements). In
-	// FP64 the 72 extra registers would cost more occupancy than the prefetch gains.
-	constexpr int KP = sizeof(T) == 4 ? (PL::RMAXC + 5) / 6 : 0;
+	// latency (KP covers all of them with the reduced D-matrix: blockDim = 12*LP*N/RMAX threads for LP*N elements). In
+	// FP64 the extra registers would cost more occupancy than the prefetch gains.
+	constexpr int KP = sizeof(T) == 4 ? (PL::RMAXC + 11) / 12 : 0;
 	cx<T> dpre[KP > 0 ? KP : 1][6];
 	unsigned mpre[KP > 0 ? KP : 1];
 #pragma unroll
 	for (int k = 0; k < KP; k++) {
 		const int idx = threadIdx.x + k * blockDim.x;
-		if (idx < LP * N) {
+		if (idx < LP * N && ((pm >> (idx / N)) & 1)) {
 			const cx<T> *dm = drow(idx, mpre[k]);
 #pragma unroll
 			for (int e = 0; e < 6; e++) dpre[k][e] = dm[e];
@@ -265,6 +285,7 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 		if (idx < LP * N) dprod(idx, dpre[k], mpre[k]);
 	}
 	for (int idx = threadIdx.x + KP * blockDim.x; idx < LP * N; idx += blockDim.x) {
+		if (!((pm >> (idx / N)) & 1)) continue;
 		unsigned m;
 		const cx<T> *dm = drow(idx, m);
 		cx<T> d[6];
@@ -280,10 +301,18 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	PL::template fft<T, true, false, true, false, true, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
+// Column units of P3: x < DsizeX with column pairs (each with its mirror), else every x; and the (plane, column)
+// groups of three lines a unit can hold (z, gZ - z with the reduced D-matrix, times x, gX - x with column pairs)
+__host__ __device__ __forceinline__ int p3_units(int gX, int DsX, int reduced, int xpair)
+{
+	return reduced && xpair ? DsX : gX;
+}
+__host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { return reduced ? (xpair ? 4 : 2) : 1; }
+
 template <typename T, class PL>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
-	Bq bq)
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
+	int transposed, Bq bq)
 {
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
@@ -292,30 +321,37 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP;
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	PL::template load_tw<T>(tws, twg, N);
-	const int nxt = gX / LP;
-	const int z = tile / nxt, x0 = (tile % nxt) * LP;
-	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; the mirror plane is handled alongside
-	const bool has2 = reduced && z != 0 && 2 * z != gZ;
-	const int nls = has2 ? 2 * NL1 : NL1;
+	const int nu = p3_units(gX, DsX, reduced, xpair), nut = (nu + LP - 1) / LP;
+	const int z = tile / nut, u0 = (tile % nut) * LP;
+	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; mirror planes and columns are handled alongside
+	const bool hz = reduced && z != 0 && 2 * z != gZ;
+	unsigned pm = 0;
+	for (int pr = 0; pr < LP && u0 + pr < nu; pr++) {
+		const int u = u0 + pr;
+		const bool hx = reduced && xpair && u != 0 && 2 * u != gX;
+		pm |= (1u | (hz ? 1u << LP : 0u) | (hx ? 1u << (2 * LP) : 0u) | (hz && hx ? 1u << (3 * LP) : 0u)) << pr;
+	}
 	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
-	const bool act = line < nls;
-	// line l = (plane, pair, c): element y of it at lp[3*y], lp = &Xb[plane][x0 + pair][0][c]
-	const int ls = line < NL1 ? line : line - NL1;
-	cx<T> *lp = Xb + ((size_t)(line < NL1 ? z : gZ - z) * gX + x0 + ls / 3) * 3 * bY + ls % 3;
+	const int g = line / 3, s = g / LP, pr = g - s * LP;
+	const bool act = (pm >> g) & 1;
+	// line (s, pr, c): element y of it at lp[3*y], lp = &Xb[plane][column][0][c] (inactive lines never access it)
+	const int zz = (s & 1) ? gZ - z : z, xx = (s & 2) ? gX - (u0 + pr) : u0 + pr;
+	cx<T> *lp = act ? Xb + ((size_t)zz * gX + xx) * 3 * bY + line % 3 : Xb;
 	if (PL::HALF_IN && PL::HALF_OUT && 2 * bY == N)
-		p3_body<T, PL, true>(buf, tws, p, lp, D, x0, z, gX, bY, gZ, DsY, LP, NL1, has2, reduced, transposed, line, t,
-			act);
-	else
-		p3_body<T, PL, false>(buf, tws, p, lp, D, x0, z, gX, bY, gZ, DsY, LP, NL1, has2, reduced, transposed, line,
+		p3_body<T, PL, true>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed, line,
 			t, act);
+	else
+		p3_body<T, PL, false>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
+			line, t, act);
 }
 
 // D-matrix product at frequency y of one grid line of plane zz, for a block at plane z (src/matvec.c:270-305:
 // reduced-FFT mirror indices and sign flips; cSymMatrVec src/cmplx.h:369-376), applied to the three components a0..a2
-// in place. Dl: the line's stored D row [yt < DsY][6] (already mirrored in x and z, see k_p3r).
+// in place. Dl: the line's stored D row [yt < DsY][6] (already mirrored in x and z, see k_p3r); xm: the line's column
+// is the mirror of its stored one (flips xy and xz).
 template <typename T>
 __device__ __forceinline__ void dmul3(const cx<T> *Dl, int y, int zz, int N, int DsY, int DsZ, int reduced,
-	int transposed, cx<T> &a0, cx<T> &a1, cx<T> &a2)
+	int transposed, bool xm, cx<T> &a0, cx<T> &a1, cx<T> &a2)
 {
 	int yi = y;
 	if (transposed) yi = yi > 0 ? N - yi : 0;
@@ -333,6 +369,10 @@ __device__ __forceinline__ void dmul3(const cx<T> *Dl, int y, int zz, int N, int
 			f2 = scl(f2, (T)-1);
 			f4 = scl(f4, (T)-1);
 		}
+		if (xm) {
+			f1 = scl(f1, (T)-1);
+			f2 = scl(f2, (T)-1);
+		}
 	}
 	const cx<T> b0 = a0, b1 = a1, b2 = a2;
 	a0 = add(add(mul(d0, b0), mul(f1, b1)), mul(f2, b2));
@@ -344,11 +384,12 @@ __device__ __forceinline__ void dmul3(const cx<T> *Dl, int y, int zz, int N, int
 // carries the three components of its butterflies through the forward FFT-y; the last stage leaves frequencies
 // t + u*TPL + q*NS_LAST in registers, where the product with D is applied, and the inverse FFT-y of the reversed
 // radix order starts from exactly those registers -- no LDS round trip and no separate pass over the spectrum.
-// Same blocks, D-tile sharing and mirror-plane pairing as k_p3; groups g < LP are plane z, the others z' = gZ - z.
+// Blocks take LP consecutive columns x at one plane z (and at the mirror plane z' = gZ - z, sharing the D tile);
+// groups g < LP are plane z, the others z'. Columns x >= DsizeX read the stored row of gridX - x.
 template <typename T, class PL>
 __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsY, int DsZ, int LP, int reduced, int transposed,
-	Bq bq)
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int,
+	int transposed, Bq bq)
 {
 	constexpr int N = PL::N, TPL = PL::TPL, RMAX = PL::RMAX, RL = PL::RAD[PL::NST - 1], NSL = PL::NS_LAST;
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
@@ -359,7 +400,7 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N, *Dt = buf + 6 * LP * LS;
 	const int nxt = gX / LP;
 	const int z = tile / nxt, x0 = (tile % nxt) * LP;
-	// the block's D rows [pr][yt][6] into LDS with coalesced loads (rows of consecutive x are contiguous in D)
+	// the block's D rows [pr][yt][6] into LDS with coalesced loads (each row is contiguous in D)
 	{
 		int zi = z;
 		if (transposed) zi = zi > 0 ? gZ - zi : 0;
@@ -367,8 +408,8 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 		const int rl = DsY * 6;
 		for (int i = threadIdx.x; i < LP * rl; i += blockDim.x) {
 			const int pr = i / rl, x = x0 + pr;
-			const int xi = transposed ? (x > 0 ? gX - x : 0) : x;
-			Dt[i] = D[((size_t)zi * gX + xi) * rl + i % rl];
+			const int xi = transposed ? (x > 0 ? gX - x : 0) : (x < DsX ? x : gX - x);
+			Dt[i] = D[((size_t)zi * DsX + xi) * rl + i % rl];
 		}
 	}
 	load_tw(tws, twg, N);  // ends with a barrier, which also publishes the D rows
@@ -383,12 +424,14 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 	PL::template fwd3<T, 0, 1>(e, buf, tws, LS, 3 * g, t, act, ld);
 	if (act) {
 		const cx<T> *Dl = Dt + pr * DsY * 6;
+		const bool xm = reduced && x0 + pr >= DsX;
 #pragma unroll
 		for (int u = 0; u < RMAX / RL; u++)
 #pragma unroll
 			for (int q = 0; q < RL; q++) {
 				const int i = u * RL + q;
-				dmul3<T>(Dl, t + u * TPL + q * NSL, zz, N, DsY, DsZ, reduced, transposed, e[0][i], e[1][i], e[2][i]);
+				dmul3<T>(Dl, t + u * TPL + q * NSL, zz, N, DsY, DsZ, reduced, transposed, xm, e[0][i], e[1][i],
+					e[2][i]);
 				__builtin_amdgcn_sched_barrier(0);  // one frequency at a time: bounds the registers held by D
 			}
 	}

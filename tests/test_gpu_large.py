@@ -53,7 +53,7 @@ def test_large_symmetry_her_and_plans(hipdev, large):
         yc, _ = op.matvec(x.conj())
         assert _rel(yh, yc.conj()) < 1e-12
         tuned = op.get_config(hipdev.F64)
-        op.set_config("x=8.2.8.5 y=2.4.16.5 z=4.2.16.5 LY=2 TC=4 LP=1", hipdev.F64)  # run-time plans
+        op.set_config("x=8.2.8.5 y=2.4.16.5 z=4.2.16.5 LY=2 TC=4 LP=1 XP=0", hipdev.F64)  # run-time plans
         y_dyn, _ = op.matvec(x)
         op.set_config(tuned, hipdev.F64)
         assert _rel(y_dyn, ax) < 1e-12

@@ -196,6 +196,13 @@ COMPILED = {  # length -> compile-time schedules (mv_*.hip) + one run-time sched
 }
 
 
+def _p3_pairs_fit(n, rmax, es):
+    """Whether P3 with column pairs (12 lines of length n at LP=1) fits 512 threads and 96 KB of LDS."""
+    r = n + n // 16
+    ls = r + (48 - r % 32) % 32  # padded line stride, a multiple of 32 plus 16
+    return 12 * (n // rmax) <= 512 and (n + 12 * ls) * es <= 96 * 1024
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("n", sorted(COMPILED))
 def test_matvec_compiled_plans(hipdev, axis, n):
@@ -214,15 +221,18 @@ def test_matvec_compiled_plans(hipdev, axis, n):
     x = np.random.default_rng(9).standard_normal(3 * c.nd) * (1 + 0.3j)
     y_or, _ = c.matvec_fft(c.dmatrix(), ccs, x)
     for prec in (hipdev.F64, hipdev.F32):
-        base = op.get_config(prec).split()[:3] + ["LY=2", "TC=4", "LP=1"]  # tiles that fit every schedule
+        # tiles that fit every schedule; P3 on single columns (XP=0) unless its column pairs fit too
+        base = op.get_config(prec).split()[:3] + ["LY=2", "TC=4", "LP=1", "XP=0"]
         for sched in COMPILED[n]:
             cfg = list(base)
             cfg[axis] = "xyz"[axis] + "=" + sched
             variants = [cfg]
             radices = [int(r) for r in sched.split(".")]
+            if axis == 1 and _p3_pairs_fit(n, max(radices), 8 if prec == hipdev.F32 else 16):
+                variants.append(cfg[:-1])  # column pairs: x and gridX - x share the D rows
             if axis == 1 and n & (n - 1) == 0 and sched != COMPILED[n][-1] \
                     and max(radices) <= (16 if prec == hipdev.F32 else 8):
-                variants.append(cfg + ["R3=1"])  # P3 with the D-matrix product in registers
+                variants.append(cfg[:-1] + ["R3=1"])  # P3 with the D-matrix product in registers
             for v in variants:
                 op.set_config(" ".join(v), prec)
                 assert op.get_config(prec) == " ".join(v)

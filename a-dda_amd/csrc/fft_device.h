@@ -11,7 +11,7 @@
 
 namespace adda {
 
-template <typename T> struct cx { T x, y; };
+template <typename T> struct alignas(2 * sizeof(T)) cx { T x, y; };
 
 template <typename T> __host__ __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
 template <typename T> __device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }

@@ -59,6 +59,9 @@ extern "C" const char *adda_hip_last_error(void) { return g_err; }
 
 static constexpr int kMaxIt = 4;     // butterflies per thread per stage (stockham_stage MAXIT)
 static constexpr int kRedBlocks = 2048;
+// Threads of the one-block-per-RHS scalar kernels of the device-resident solvers (1024 measured slower: their time is
+// launch and latency, not the partial-sum loads)
+static constexpr int kDkThreads = 256;
 static constexpr int kMaxDir = 2048;     // directions per adda_hip_field_sums launch
 static constexpr int kDirBlocks = 32;    // dipole chunks per direction
 
@@ -378,13 +381,11 @@ __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int
 			s2 += ux * uy;
 		}
 	}
-	double t0 = block_sum<T>(s0, red_lds);
-	double t1 = block_sum<T>(s1, red_lds);
-	double t2 = block_sum<T>(s2, red_lds);
+	block_sum3<T>(s0, s1, s2, red_lds);
 	if (threadIdx.x == 0) {
-		partials[blockIdx.x] = t0;
-		partials[nblk + blockIdx.x] = t1;
-		partials[2 * nblk + blockIdx.x] = t2;
+		partials[blockIdx.x] = s0;
+		partials[nblk + blockIdx.x] = s1;
+		partials[2 * nblk + blockIdx.x] = s2;
 	}
 }
 
@@ -800,10 +801,16 @@ __device__ void dk_reduce(const double *partials, int nblk, int nrows, double *o
 			for (int j = 0; j < 4; j++)
 				if (i + j * B < nblk) s[k] += v[k][j];
 	}
-	for (int k = 0; k < nrows; k++) {
-		const double t = block_sum<double>(s[k], red_lds);
-		if (threadIdx.x == 0) out[k] = t;
+	if (nrows == 3) {
+		block_sum3<double>(s[0], s[1], s[2], red_lds);
+		if (threadIdx.x == 0)
+			for (int k = 0; k < 3; k++) out[k] = s[k];
 	}
+	else
+		for (int k = 0; k < nrows; k++) {
+			const double t = block_sum<double>(s[k], red_lds);
+			if (threadIdx.x == 0) out[k] = t;
+		}
 	__syncthreads();
 }
 
@@ -846,16 +853,18 @@ __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const
 		if (red == 1) s0 += ux * ux + uy * uy;
 		else if (red == 2) { s0 += ux * ux; s1 += uy * uy; s2 += ux * uy; }
 	}
-	if (red) {
+	if (red == 2) {
 		const int nb = gridDim.x;
-		double t = block_sum<T>(s0, red_lds);
-		if (threadIdx.x == 0) partials[blockIdx.x] = t;
-		if (red == 2) {
-			t = block_sum<T>(s1, red_lds);
-			if (threadIdx.x == 0) partials[nb + blockIdx.x] = t;
-			t = block_sum<T>(s2, red_lds);
-			if (threadIdx.x == 0) partials[2 * nb + blockIdx.x] = t;
+		block_sum3<T>(s0, s1, s2, red_lds);
+		if (threadIdx.x == 0) {
+			partials[blockIdx.x] = s0;
+			partials[nb + blockIdx.x] = s1;
+			partials[2 * nb + blockIdx.x] = s2;
 		}
+	}
+	else if (red) {
+		const double t = block_sum<T>(s0, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
 	}
 }
 
@@ -897,13 +906,17 @@ __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const c
 			s2 += ux * wy + uy * wx;
 		}
 	}
-	double t = block_sum<T>(s0, red_lds);
-	if (threadIdx.x == 0) partials[blockIdx.x] = t;
 	if (rt) {
-		t = block_sum<T>(s1, red_lds);
-		if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = t;
-		t = block_sum<T>(s2, red_lds);
-		if (threadIdx.x == 0) partials[2 * gridDim.x + blockIdx.x] = t;
+		block_sum3<T>(s0, s1, s2, red_lds);
+		if (threadIdx.x == 0) {
+			partials[blockIdx.x] = s0;
+			partials[gridDim.x + blockIdx.x] = s1;
+			partials[2 * gridDim.x + blockIdx.x] = s2;
+		}
+	}
+	else {
+		const double t = block_sum<T>(s0, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
 	}
 }
 
@@ -963,7 +976,7 @@ __device__ void dk_progress(double *st, double *hist, double inprodRp1)
 	else if (st[DK_COUNTER] > st[DK_MC]) st[DK_STATUS] = -2;
 }
 
-__global__ void __launch_bounds__(256) k_dk_progress(double *st, double *hist, const double *partials, int nblk, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_progress(double *st, double *hist, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
@@ -976,7 +989,7 @@ __global__ void __launch_bounds__(256) k_dk_progress(double *st, double *hist, c
 
 // QMR_CS scalar step 1 (src/iterative.c:962-982): breakdown test, alpha = v.Av, vtilda coefficients, theta, eta,
 // zetatilda.
-__global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *partials, int nblk, int niter, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_qmr_s1(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
@@ -1005,7 +1018,7 @@ __global__ void __launch_bounds__(256) k_dk_qmr_s1(double *st, const double *par
 }
 
 // QMR_CS scalar step 2 (src/iterative.c:983-1029): beta, omega, Givens rotation, p/x/r coefficients.
-__global__ void __launch_bounds__(256) k_dk_qmr_s2(double *st, const double *partials, int nblk, int niter, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_qmr_s2(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
@@ -1061,7 +1074,7 @@ __device__ void dk_bs_beta(double *st, dcx ro_new, int niter)
 	st_c(st, DK_SC + 20, beta);
 	st_c(st, DK_SC + 22, dneg(dmul(beta, ld_c(st, DK_SC + 6))));
 }
-__global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s1(double *st, const double *partials, int nblk, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
@@ -1072,7 +1085,7 @@ __global__ void __launch_bounds__(256) k_dk_bs_s1(double *st, const double *part
 }
 // End of a BiCGStab iteration: the progress test on ||r||^2 (row 0 of the x/r update's partial sums) and, if the
 // solve goes on, the next iteration's beta from r.r~ (rows 1-2) -- one launch and one pass over the partials.
-__global__ void __launch_bounds__(256) k_dk_bs_prog_s1(double *st, double *hist, const double *partials, int nblk,
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_prog_s1(double *st, double *hist, const double *partials, int nblk,
 	int ps)
 {
 	__shared__ double red_lds[512];
@@ -1086,7 +1099,7 @@ __global__ void __launch_bounds__(256) k_dk_bs_prog_s1(double *st, double *hist,
 	if (st[DK_STATUS] == 0) dk_bs_beta(st, dmk(v[1], v[2]), 2);  // any iteration after the first
 }
 
-__global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *partials, int nblk, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s2(double *st, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
@@ -1104,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_dk_bs_s2(double *st, const double *part
 
 // ||s||^2 -> half-step convergence test (src/iterative.c:700-707): if converged, omega := 0 so the tail computes
 // x += alpha p and r := s (whose norm is the one reported).
-__global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
@@ -1116,7 +1129,7 @@ __global__ void __launch_bounds__(256) k_dk_bs_s3(double *st, const double *part
 }
 
 // s4 + s5 from the fused rows of the t = A.s matvec: row 0 ||t||^2, rows 1, 2 sum t.conj(s) = conj(s.t~).
-__global__ void __launch_bounds__(256) k_dk_bs_s45(double *st, const double *partials, int nblk, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s45(double *st, const double *partials, int nblk, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[3];
@@ -1134,7 +1147,7 @@ __global__ void __launch_bounds__(256) k_dk_bs_s45(double *st, const double *par
 
 // CGNR scalar steps (src/iterative.c:745-766). Slots: 0 ro_new, 1 ro_old, 2 alpha, 4 beta (c), 6 one, 8 alpha (c),
 // 10 -alpha (c), 12 zero.
-__global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, int nblk_t, int niter, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_cg_s1(double *st, const double *pt, int nblk_t, int niter, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
@@ -1146,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_dk_cg_s1(double *st, const double *pt, 
 	if (niter > 1) st_c(st, DK_SC + 4, dmk(v[0] / st[DK_SC + 1], 0));
 }
 
-__global__ void __launch_bounds__(256) k_dk_cg_s2(double *st, const double *pt, int nblk_t, int ps)
+__global__ void __launch_bounds__(kDkThreads) k_dk_cg_s2(double *st, const double *pt, int nblk_t, int ps)
 {
 	__shared__ double red_lds[512];
 	__shared__ double v[1];
@@ -1211,15 +1224,15 @@ static int qmr_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready,
 	if (niter == 1 && matvec_ready) lc_dev<T>(op, g, Av, Av, nullptr, nullptr, st, DK_SC + 44, DK_SC + 44, DK_SC + 44, 0);
 	else if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)v, (cx<T> *)Av, nullptr, 0)) return -1;
 	dot_dev<T>(op, g, st, v, Av, ADDA_DOT_BILIN);
-	hipLaunchKernelGGL(k_dk_qmr_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	hipLaunchKernelGGL(k_dk_qmr_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	lc_dev<T>(op, g, vtilda, niter == 1 ? nullptr : vtilda, v, Av, st, DK_SC + 24, DK_SC + 26, DK_SC + 28, 2);
-	hipLaunchKernelGGL(k_dk_qmr_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	hipLaunchKernelGGL(k_dk_qmr_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	void *pdst = niter == 1 ? p_new : p_old;
 	const void *pa = niter >= 3 ? p_old : nullptr, *pb = niter >= 2 ? p_new : nullptr;
 	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
 		(const cx<T> *)pb, (const cx<T> *)v, (cx<T> *)x, (cx<T> *)vtilda, (cx<T> *)r, st, g.n, g.vs, op->partials2,
 		g.ps2);
-	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1235,7 +1248,7 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 	// (k_dk_bs_prog_s1) computed beta from the rows its x/r update left
 	if (niter == 1) {
 		dot_dev<T>(op, g, st, r, rt, ADDA_DOT_CONJ);
-		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+		hipLaunchKernelGGL(k_dk_bs_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
 	}
 	// p = beta p - beta omega v + r  (niter 1: p = r)
 	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, niter == 1 ? nullptr : v, r, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 0);
@@ -1243,23 +1256,23 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 	if (niter == 1 && matvec_ready) {
 		lc_dev<T>(op, g, v, t, nullptr, nullptr, st, DK_SC + 24, DK_SC + 24, DK_SC + 24, 0);
 		dot_dev<T>(op, g, st, v, rt, ADDA_DOT_CONJ);
-		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
 	}
 	else {
 		if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)v, nullptr, 0, nullptr, true, &nb5,
 			(const cx<T> *)rt, 1)) return -1;
-		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials + nb5, nb5, 3 * nb5);
+		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials + nb5, nb5, 3 * nb5);
 	}
 	lc_dev<T>(op, g, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
-	hipLaunchKernelGGL(k_dk_bs_s3, dim3(g.K), dim3(256), 0, op->stream, st, op->partials2, g.nb, k->hist, g.ps2);
+	hipLaunchKernelGGL(k_dk_bs_s3, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, k->hist, g.ps2);
 	// t = A.s with ||t||^2 and t.s~ fused
 	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nb5, nullptr, 2))
 		return -1;
-	hipLaunchKernelGGL(k_dk_bs_s45, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nb5, 3 * nb5);
+	hipLaunchKernelGGL(k_dk_bs_s45, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials, nb5, 3 * nb5);
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
 		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)rt);
-	hipLaunchKernelGGL(k_dk_bs_prog_s1, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_bs_prog_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1272,14 +1285,14 @@ static int cgnr_iter_t(adda_hip_op *op, dk_state *k, int niter, void *x, void *r
 	int nbt;
 	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)r, (cx<T> *)(niter == 1 ? p : Av), nullptr, 1, nullptr, true,
 		&nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_cg_s1, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, niter, nbt);
+	hipLaunchKernelGGL(k_dk_cg_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials, nbt, niter, nbt);
 	if (niter > 1) lc_dev<T>(op, g, p, p, Av, nullptr, st, DK_SC + 4, DK_SC + 6, DK_SC + 6, 0);
 	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0, nullptr, true, &nbt)) return -1;
-	hipLaunchKernelGGL(k_dk_cg_s2, dim3(g.K), dim3(256), 0, op->stream, st, op->partials, nbt, nbt);
+	hipLaunchKernelGGL(k_dk_cg_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials, nbt, nbt);
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)nullptr, (cx<T> *)r, (const cx<T> *)r, (const cx<T> *)Av, st, DK_SC + 8, DK_SC + 12, DK_SC + 6,
 		DK_SC + 10, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)nullptr);
-	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(256), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }

@@ -51,6 +51,40 @@ __device__ __forceinline__ double block_sum(double s, double *red_lds)
 	return tid == 0 ? red_lds[0] : 0.0;
 }
 
+// Three block sums at once, each equal bit for bit to block_sum of that value, with one pair of barriers instead of
+// three (blocks of whole waves; other block sizes take three block_sum calls). The totals land in a, b, c of thread 0.
+template <typename T>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red_lds)
+{
+	const int tid = threadIdx.x, n = blockDim.x;
+	if (n % 64 != 0) {
+		a = block_sum<T>(a, red_lds);
+		b = block_sum<T>(b, red_lds);
+		c = block_sum<T>(c, red_lds);
+		return;
+	}
+	__syncthreads();  // red_lds may still be read by a previous call (or be the caller's buffer)
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		a += __shfl_xor(a, off, 64);
+		b += __shfl_xor(b, off, 64);
+		c += __shfl_xor(c, off, 64);
+	}
+	const int nw = n / 64;
+	if ((tid & 63) == 0) {
+		red_lds[tid >> 6] = a;
+		red_lds[nw + (tid >> 6)] = b;
+		red_lds[2 * nw + (tid >> 6)] = c;
+	}
+	__syncthreads();
+	if (tid == 0) {
+		a = b = c = 0;
+		for (int w = 0; w < nw; w++) a += red_lds[w];
+		for (int w = 0; w < nw; w++) b += red_lds[nw + w];
+		for (int w = 0; w < nw; w++) c += red_lds[2 * nw + w];
+	}
+}
+
 // ==================================================================================================================
 // kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][y<boxY][c], Xb[z<gridZ][x<gridX][y<boxY][c],
 // D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
@@ -555,13 +589,17 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	if (partials) {
 		const int rows = DOT ? 3 : 1;
 		const size_t nb = gridDim.x / bq.K, o = (size_t)kq * rows * nb + tile;
-		double tt = block_sum<T>(s, red_lds);
-		if (threadIdx.x == 0) partials[o] = tt;
 		if (DOT) {
-			tt = block_sum<T>(dr, red_lds);
-			if (threadIdx.x == 0) partials[o + nb] = tt;
-			tt = block_sum<T>(di, red_lds);
-			if (threadIdx.x == 0) partials[o + 2 * nb] = tt;
+			block_sum3<T>(s, dr, di, red_lds);
+			if (threadIdx.x == 0) {
+				partials[o] = s;
+				partials[o + nb] = dr;
+				partials[o + 2 * nb] = di;
+			}
+		}
+		else {
+			const double tt = block_sum<T>(s, red_lds);
+			if (threadIdx.x == 0) partials[o] = tt;
 		}
 	}
 }

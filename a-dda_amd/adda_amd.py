@@ -361,6 +361,9 @@ def host():
     L.adda_ext_cross.argtypes = [P, P, D, D, P, P, P, P]
     L.adda_abs_cross.argtypes = [P, P, D, P, P, P]
     L.adda_eplane_yz.argtypes = [P, P, D, D, P, P, P, P, I, I, P]
+    L.adda_ext_cross_batch.argtypes = [P, P, D, D, P, I, P, P, P]
+    L.adda_abs_cross_batch.argtypes = [P, P, D, P, P, I, P]
+    L.adda_eplane_yz_batch.argtypes = [P, P, D, D, P, I, P, P, P, I, I, P]
     L.adda_mueller_yz.argtypes = [P, P, I, P]
     L.adda_mueller_yz.restype = None
     _host = L
@@ -594,6 +597,42 @@ class BatchProblem:
         hcheck(host().adda_abs_cross(self.op.h, ctypes.byref(self._p), self.wavenum, _p(cc), self.op.rhs(self.pvec, k),
                                      ctypes.byref(out)))
         return out.value
+
+    def cext_all(self):
+        """Cext of every RHS from one batched evaluation (adda_ext_cross_batch)."""
+        out = np.empty(self.K)
+        hcheck(host().adda_ext_cross_batch(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, self.pvec,
+                                           self.K, _p(self.props), _p(self.pols), _p(out)))
+        return out
+
+    def cabs_all(self):
+        """Cabs of every RHS from one batched evaluation (adda_abs_cross_batch)."""
+        out = np.empty(self.K)
+        cc = np.ascontiguousarray(self.cc)
+        hcheck(host().adda_abs_cross_batch(self.op.h, ctypes.byref(self._p), self.wavenum, _p(cc), self.pvec, self.K,
+                                           _p(out)))
+        return out
+
+    def eplane(self, k, par, per, ntheta, npoints=None):
+        """Scattered field of RHS k in the plane spanned by its propagation direction and par (CalcEplaneYZ):
+        [npoints, 2] complex, columns E.per and E.par, theta_i = i*180/(ntheta-1)."""
+        npoints = npoints or ntheta
+        out = np.empty((npoints, 2), np.complex128)
+        hcheck(host().adda_eplane_yz(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace,
+                                     self.op.rhs(self.pvec, k), _p(self.props[k]), _p(np.asarray(par, np.float64)),
+                                     _p(np.asarray(per, np.float64)), ntheta, npoints, _p(out)))
+        return out
+
+    def eplane_all(self, pars, pers, ntheta, npoints=None):
+        """eplane for every RHS from one batched evaluation (adda_eplane_yz_batch): [K, npoints, 2] complex;
+        pars, pers: [K, 3], the scattering-plane vectors of each RHS's beam."""
+        npoints = npoints or ntheta
+        pars = np.ascontiguousarray(pars, np.float64).reshape(self.K, 3)
+        pers = np.ascontiguousarray(pers, np.float64).reshape(self.K, 3)
+        out = np.empty((self.K, npoints, 2), np.complex128)
+        hcheck(host().adda_eplane_yz_batch(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, self.pvec,
+                                           self.K, _p(self.props), _p(pars), _p(pers), ntheta, npoints, _p(out)))
+        return out
 
     def field_x(self, k):
         return self.op.download(self.op.rhs(self.x, k))

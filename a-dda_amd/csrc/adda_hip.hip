@@ -433,10 +433,11 @@ __global__ void k_plane_wave(cx<double> *e, const unsigned short *pos, size_t nd
 // Field sums: block (chunk, dir); partials[((dir*6 + q)*kDirBlocks) + chunk], q = re/im of 3 components
 template <typename T>
 __global__ void __launch_bounds__(256) k_field_sums(const cx<T> *P, const unsigned short *pos, size_t nd, double kd,
-	const double *dirs, double *partials)
+	const double *dirs, double *partials, long vs, int ndir, int j0)
 {
 	__shared__ double red_lds[512];
 	const int dir = blockIdx.y, chunk = blockIdx.x;
+	P += (long)((j0 + dir) / ndir) * vs;  // direction j0 + dir of the flattened (RHS, direction) list
 	const double nx = dirs[3 * dir], ny = dirs[3 * dir + 1], nz = dirs[3 * dir + 2];
 	double s[6] = {0, 0, 0, 0, 0, 0};
 	for (size_t j = (size_t)chunk * blockDim.x + threadIdx.x; j < nd; j += (size_t)gridDim.x * blockDim.x) {
@@ -467,9 +468,12 @@ __global__ void k_field_final(const double *partials, int nchunk, int ndir, doub
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_wnorm(const cx<T> *v, const int *mat, const double *w, size_t nd,
-	double *partials)
+	double *partials, long vs, long ws)
 {
 	__shared__ double red_lds[512];
+	v += blockIdx.y * vs;  // RHS blockIdx.y: its vector and its weight table
+	w += blockIdx.y * ws;
+	partials += (size_t)blockIdx.y * gridDim.x;
 	double s = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nd; i += (size_t)gridDim.x * blockDim.x) {
 		const cx<T> a = v[i];
@@ -1812,26 +1816,26 @@ extern "C" int adda_hip_plane_wave(adda_hip_op *op, double wavenum, double grids
 	return 0;
 }
 
-extern "C" int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, const double *dirs,
+// Scattering sums of K right-hand sides, each over its own ndir directions: the (RHS, direction) list is cut into
+// launches of at most kMaxDir entries (one block row per entry), so each entry's reduction is the single-RHS one.
+static int field_sums(adda_hip_op *op, int prec, const void *pvec, int K, long vs, double kd, const double *dirs,
 	int ndir, adda_cplx *sums)
 {
-	if (!op || !pvec || !dirs || !sums || ndir < 0) return ERR("bad argument");
-	HIPCHK(hipSetDevice(op->dev));
 	std::vector<double> out;
-	for (int d0 = 0; d0 < ndir; d0 += kMaxDir) {
-		const int nd = ndir - d0 < kMaxDir ? ndir - d0 : kMaxDir;
-		if ((size_t)nd * 6 * kDirBlocks > op->npartials) return ERR("partials buffer too small");
+	const long nj = (long)K * ndir;
+	for (long d0 = 0; d0 < nj; d0 += kMaxDir) {
+		const int nd = nj - d0 < kMaxDir ? (int)(nj - d0) : kMaxDir;
+		if ((size_t)nd * 6 * (kDirBlocks + 1) > op->npartials) return ERR("partials buffer too small");
 		HIPCHK(hipMemcpyAsync(op->dirbuf, dirs + 3 * (size_t)d0, 3 * nd * sizeof(double), hipMemcpyHostToDevice,
 			op->stream));
 		if (prec == ADDA_F64)
 			hipLaunchKernelGGL(k_field_sums<double>, dim3(kDirBlocks, nd), dim3(256), 0, op->stream,
-				(const cx<double> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials);
+				(const cx<double> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials, vs, ndir, (int)d0);
 		else
 			hipLaunchKernelGGL(k_field_sums<float>, dim3(kDirBlocks, nd), dim3(256), 0, op->stream,
-				(const cx<float> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials);
-		// reuse the tail of dirbuf is not possible (too small): final sums go to the head of partials' second half
+				(const cx<float> *)pvec, op->pos, op->nd, kd, op->dirbuf, op->partials, vs, ndir, (int)d0);
+		// final sums go to the head of partials' second half (dirbuf is too small to hold them)
 		double *fin = op->partials + (size_t)nd * 6 * kDirBlocks;
-		if ((size_t)nd * 6 * (kDirBlocks + 1) > op->npartials) return ERR("partials buffer too small");
 		hipLaunchKernelGGL(k_field_final, dim3((nd * 6 + 255) / 256), dim3(256), 0, op->stream, op->partials,
 			kDirBlocks, nd, fin);
 		HIPCHK(hipGetLastError());
@@ -1846,20 +1850,65 @@ extern "C" int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, 
 	return 0;
 }
 
+extern "C" int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, const double *dirs,
+	int ndir, adda_cplx *sums)
+{
+	if (!op || !pvec || !dirs || !sums || ndir < 0) return ERR("bad argument");
+	HIPCHK(hipSetDevice(op->dev));
+	return field_sums(op, prec, pvec, 1, 0, kd, dirs, ndir, sums);
+}
+
+extern "C" int adda_hip_field_sums_batch(adda_hip_op *op, int prec, const void *pvec, int K, double kd,
+	const double *dirs, int ndir, adda_cplx *sums)
+{
+	if (!op || !pvec || !dirs || !sums || ndir < 0 || K < 1) return ERR("bad argument");
+	if (K > op->kcap) return ERR("batch of %d right-hand sides exceeds the %d reserved", K, op->kcap);
+	HIPCHK(hipSetDevice(op->dev));
+	return field_sums(op, prec, pvec, K, K > 1 ? (long)op->vs : 0, kd, dirs, ndir, sums);
+}
+
+// Weighted norms of K right-hand sides (weight table k at w + k*3*nmat): one launch, one reduction row per RHS
+static int weighted_norm2(adda_hip_op *op, int prec, const void *v, int K, const double *w, double *out)
+{
+	const size_t nw = (size_t)op->nmat * 3;
+	double *wd = op->wbuf;
+	if (K > 1) HIPCHK(hipMalloc(&wd, K * nw * sizeof(double)));  // per-RHS tables (post-processing, not per iteration)
+	int rc = 0;
+	if (hipMemcpyAsync(wd, w, K * nw * sizeof(double), hipMemcpyHostToDevice, op->stream) != hipSuccess)
+		rc = ERR("weight upload failed");
+	const int nb = nblocks_red(3 * op->nd);
+	const long vs = K > 1 ? (long)op->vs : 0, ws = K > 1 ? (long)nw : 0;
+	if (!rc) {
+		if (prec == ADDA_F64)
+			hipLaunchKernelGGL(k_wnorm<double>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<double> *)v, op->mat,
+				wd, op->nd, op->partials, vs, ws);
+		else
+			hipLaunchKernelGGL(k_wnorm<float>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<float> *)v, op->mat,
+				wd, op->nd, op->partials, vs, ws);
+		if (hipGetLastError() != hipSuccess) rc = ERR("weighted-norm launch failed");
+	}
+	if (!rc) rc = reduce_rows(op, op->partials, nb, K, out);
+	if (K > 1) {
+		if (hipStreamSynchronize(op->stream) != hipSuccess && !rc) rc = ERR("stream synchronisation failed");
+		if (hipFree(wd) != hipSuccess && !rc) rc = ERR("freeing the weight tables failed");
+	}
+	return rc;
+}
+
 extern "C" int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v, const double *w, double *out)
 {
 	if (!op || !v || !w || !out) return ERR("null argument");
 	HIPCHK(hipSetDevice(op->dev));
-	HIPCHK(hipMemcpyAsync(op->wbuf, w, (size_t)op->nmat * 3 * sizeof(double), hipMemcpyHostToDevice, op->stream));
-	const int nb = nblocks_red(3 * op->nd);
-	if (prec == ADDA_F64)
-		hipLaunchKernelGGL(k_wnorm<double>, dim3(nb), dim3(256), 0, op->stream, (const cx<double> *)v, op->mat,
-			op->wbuf, op->nd, op->partials);
-	else
-		hipLaunchKernelGGL(k_wnorm<float>, dim3(nb), dim3(256), 0, op->stream, (const cx<float> *)v, op->mat,
-			op->wbuf, op->nd, op->partials);
-	HIPCHK(hipGetLastError());
-	return reduce_finish(op, nb, 1, out);
+	return weighted_norm2(op, prec, v, 1, w, out);
+}
+
+extern "C" int adda_hip_weighted_norm2_batch(adda_hip_op *op, int prec, const void *v, int K, const double *w,
+	double *out)
+{
+	if (!op || !v || !w || !out || K < 1) return ERR("bad argument");
+	if (K > op->kcap) return ERR("batch of %d right-hand sides exceeds the %d reserved", K, op->kcap);
+	HIPCHK(hipSetDevice(op->dev));
+	return weighted_norm2(op, prec, v, K, w, out);
 }
 
 extern "C" int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, int nrep,

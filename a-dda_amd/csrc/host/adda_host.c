@@ -190,14 +190,18 @@ int adda_generate_b(adda_hip_op *op, const adda_particle *p, double wavenum, dou
 }
 
 /* ---- scattering quantities -------------------------------------------------------------------------------------- */
-/* CalcFieldFree, src/crosssec.c:506-595 (DRAINE scattering relation, free space) */
-int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
-	const double *dirs, int ndir, adda_cplx *ebuff)
+/* CalcFieldFree, src/crosssec.c:506-595 (DRAINE scattering relation, free space), for K solutions with ndir
+ * directions each (K == 1: a single device vector; K > 1: a batched one) */
+static int calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	int K, const double *dirs, int ndir, adda_cplx *ebuff)
 {
 	const double kd = wavenum * gridspace;
-	adda_cplx *sums = malloc((size_t)ndir * 3 * sizeof(adda_cplx));
+	const size_t nj = (size_t)K * ndir;
+	adda_cplx *sums = malloc((nj ? nj : 1) * 3 * sizeof(adda_cplx));
 	if (!sums) return host_err(__FILE__, __LINE__, "out of host memory");
-	if (adda_hip_field_sums(op, ADDA_F64, d_pvec, kd, dirs, ndir, sums)) {
+	const int rc = K == 1 ? adda_hip_field_sums(op, ADDA_F64, d_pvec, kd, dirs, ndir, sums)
+	                      : adda_hip_field_sums_batch(op, ADDA_F64, d_pvec, K, kd, dirs, ndir, sums);
+	if (rc) {
 		free(sums);
 		return host_err_hip(__FILE__, __LINE__);
 	}
@@ -205,7 +209,7 @@ int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, dou
 	const double r0[3] = {-gridspace * (p->boxX - 1) / 2.0, -gridspace * (p->boxY - 1) / 2.0,
 		-gridspace * (p->boxZ - 1) / 2.0};
 	const double kkk = wavenum * wavenum * wavenum;
-	for (int i = 0; i < ndir; i++) {
+	for (size_t i = 0; i < nj; i++) {
 		const double *n = dirs + 3 * i;
 		dcx s[3], tb[3];
 		for (int c = 0; c < 3; c++) s[c] = sums[3 * i + c].re + I * sums[3 * i + c].im;
@@ -224,68 +228,138 @@ int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, dou
 	return 0;
 }
 
-/* ExtCross, plane wave: src/crosssec.c:811-823 */
+int adda_calc_field(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double *dirs, int ndir, adda_cplx *ebuff)
+{
+	return calc_field(op, p, wavenum, gridspace, d_pvec, 1, dirs, ndir, ebuff);
+}
+
+int adda_calc_field_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *dirs, int ndir, adda_cplx *ebuff)
+{
+	if (K < 1) return host_err(__FILE__, __LINE__, "batch of %d right-hand sides", K);
+	return calc_field(op, p, wavenum, gridspace, d_pvec, K, dirs, ndir, ebuff);
+}
+
+/* ExtCross, plane wave: src/crosssec.c:811-823 (K solutions, beam k along props[k] polarized along incpols[k]) */
+static int ext_cross(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	int K, const double *props, const double *incpols, double *cext)
+{
+	adda_cplx *e = malloc((size_t)K * 3 * sizeof(adda_cplx));
+	if (!e) return host_err(__FILE__, __LINE__, "out of host memory");
+	const int rc = calc_field(op, p, wavenum, gridspace, d_pvec, K, props, 1, e);
+	if (!rc)
+		for (int k = 0; k < K; k++) {
+			const adda_cplx *ek = e + 3 * k;
+			const double *pol = incpols + 3 * k;
+			double sum = ek[0].re * pol[0] + ek[1].re * pol[1] + ek[2].re * pol[2];
+			cext[k] = sum * FOUR_PI / (wavenum * wavenum);
+		}
+	free(e);
+	return rc;
+}
+
 int adda_ext_cross(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
 	const double prop[3], const double incpol[3], double *cext)
 {
-	adda_cplx e[3];
-	if (adda_calc_field(op, p, wavenum, gridspace, d_pvec, prop, 1, e)) return -1;
-	double sum = e[0].re * incpol[0] + e[1].re * incpol[1] + e[2].re * incpol[2];
-	*cext = sum * FOUR_PI / (wavenum * wavenum);
-	return 0;
+	return ext_cross(op, p, wavenum, gridspace, d_pvec, 1, prop, incpol, cext);
 }
 
-/* AbsCross, SQ_DRAINE: src/crosssec.c:873-884,916 */
-int adda_abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
-	double *cabs)
+int adda_ext_cross_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *props, const double *incpols, double *cext)
+{
+	if (K < 1) return host_err(__FILE__, __LINE__, "batch of %d right-hand sides", K);
+	return ext_cross(op, p, wavenum, gridspace, d_pvec, K, props, incpols, cext);
+}
+
+/* AbsCross, SQ_DRAINE: src/crosssec.c:873-884,916 (K solutions, coupling constants cc[k][nmat][3]) */
+static int abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
+	int K, double *cabs)
 {
 	const double temp1 = 2 * wavenum * wavenum * wavenum / 3;
-	double *w = malloc((size_t)p->nmat * 3 * sizeof(double));
+	const size_t nw = (size_t)K * p->nmat * 3;
+	double *w = malloc(nw * sizeof(double));
 	if (!w) return host_err(__FILE__, __LINE__, "out of host memory");
-	for (int i = 0; i < p->nmat * 3; i++) {
+	for (size_t i = 0; i < nw; i++) {
 		dcx c = cc[i].re + I * cc[i].im;
 		w[i] = -cimag(1 / c) - temp1;
 	}
-	double sum;
-	int rc = adda_hip_weighted_norm2(op, ADDA_F64, d_pvec, w, &sum);
+	int rc = K == 1 ? adda_hip_weighted_norm2(op, ADDA_F64, d_pvec, w, cabs)
+	                : adda_hip_weighted_norm2_batch(op, ADDA_F64, d_pvec, K, w, cabs);
 	free(w);
 	if (rc) return host_err_hip(__FILE__, __LINE__);
-	*cabs = FOUR_PI * wavenum * sum;
+	for (int k = 0; k < K; k++) cabs[k] *= FOUR_PI * wavenum;
 	return 0;
 }
 
-/* CalcEplaneYZ, CE_NORMAL, no orientation averaging: src/CalculateE.c:423-507 */
-int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
-	const double prop[3], const double par[3], const double per[3], int ntheta, int npoints, adda_cplx *eplane)
+int adda_abs_cross(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc, const void *d_pvec,
+	double *cabs)
 {
-	if (ntheta < 2 || npoints < 1) return host_err(__FILE__, __LINE__, "ntheta must be >= 2");
+	return abs_cross(op, p, wavenum, cc, d_pvec, 1, cabs);
+}
+
+int adda_abs_cross_batch(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc,
+	const void *d_pvec, int K, double *cabs)
+{
+	if (K < 1) return host_err(__FILE__, __LINE__, "batch of %d right-hand sides", K);
+	return abs_cross(op, p, wavenum, cc, d_pvec, K, cabs);
+}
+
+/* CalcEplaneYZ, CE_NORMAL, no orientation averaging: src/CalculateE.c:423-507 (K solutions, beam frame k given by
+ * props[k], pars[k], pers[k]) */
+static int eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	int K, const double *props, const double *pars, const double *pers, int ntheta, int npoints, adda_cplx *eplane)
+{
+	if (ntheta < 2 || npoints < 1 || K < 1) return host_err(__FILE__, __LINE__, "ntheta must be >= 2");
 	const double dth = PI / (double)(ntheta - 1); /* dtheta_rad, src/calculator.c:671-672 */
-	double *dirs = malloc((size_t)npoints * 3 * sizeof(double));
-	adda_cplx *e = malloc((size_t)npoints * 3 * sizeof(adda_cplx));
+	const size_t nj = (size_t)K * npoints;
+	double *dirs = malloc(nj * 3 * sizeof(double));
+	adda_cplx *e = malloc(nj * 3 * sizeof(adda_cplx));
 	if (!dirs || !e) { free(dirs); free(e); return host_err(__FILE__, __LINE__, "out of host memory"); }
-	for (int i = 0; i < npoints; i++) {
-		double th = i * dth, co = cos(th), si = sin(th);
-		for (int c = 0; c < 3; c++) dirs[3 * i + c] = co * prop[c] + si * par[c];
-	}
-	int rc = adda_calc_field(op, p, wavenum, gridspace, d_pvec, dirs, npoints, e);
-	if (!rc)
+	for (int k = 0; k < K; k++) {
+		const double *prop = props + 3 * k, *par = pars + 3 * k;
 		for (int i = 0; i < npoints; i++) {
-			double th = i * dth, co = cos(th), si = sin(th), epar[3];
-			for (int c = 0; c < 3; c++) epar[c] = -si * prop[c] + co * par[c];
-			dcx eper = 0, epa = 0;
-			for (int c = 0; c < 3; c++) {
-				dcx ec = e[3 * i + c].re + I * e[3 * i + c].im;
-				eper += ec * per[c];
-				epa += ec * epar[c];
+			double th = i * dth, co = cos(th), si = sin(th);
+			for (int c = 0; c < 3; c++) dirs[3 * ((size_t)k * npoints + i) + c] = co * prop[c] + si * par[c];
+		}
+	}
+	int rc = calc_field(op, p, wavenum, gridspace, d_pvec, K, dirs, npoints, e);
+	if (!rc)
+		for (int k = 0; k < K; k++) {
+			const double *prop = props + 3 * k, *par = pars + 3 * k, *per = pers + 3 * k;
+			for (int i = 0; i < npoints; i++) {
+				const size_t j = (size_t)k * npoints + i;
+				double th = i * dth, co = cos(th), si = sin(th), epar[3];
+				for (int c = 0; c < 3; c++) epar[c] = -si * prop[c] + co * par[c];
+				dcx eper = 0, epa = 0;
+				for (int c = 0; c < 3; c++) {
+					dcx ec = e[3 * j + c].re + I * e[3 * j + c].im;
+					eper += ec * per[c];
+					epa += ec * epar[c];
+				}
+				eplane[2 * j].re = creal(eper);
+				eplane[2 * j].im = cimag(eper);
+				eplane[2 * j + 1].re = creal(epa);
+				eplane[2 * j + 1].im = cimag(epa);
 			}
-			eplane[2 * i].re = creal(eper);
-			eplane[2 * i].im = cimag(eper);
-			eplane[2 * i + 1].re = creal(epa);
-			eplane[2 * i + 1].im = cimag(epa);
 		}
 	free(dirs);
 	free(e);
 	return rc;
+}
+
+int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const void *d_pvec,
+	const double prop[3], const double par[3], const double per[3], int ntheta, int npoints, adda_cplx *eplane)
+{
+	return eplane_yz(op, p, wavenum, gridspace, d_pvec, 1, prop, par, per, ntheta, npoints, eplane);
+}
+
+int adda_eplane_yz_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *props, const double *pars, const double *pers, int ntheta, int npoints,
+	adda_cplx *eplane)
+{
+	if (K < 1) return host_err(__FILE__, __LINE__, "batch of %d right-hand sides", K);
+	return eplane_yz(op, p, wavenum, gridspace, d_pvec, K, props, pars, pers, ntheta, npoints, eplane);
 }
 
 static inline double ca2(dcx a) { return creal(a) * creal(a) + cimag(a) * cimag(a); }

@@ -259,10 +259,13 @@ def bench_batched(A, args):
     from cases import orientations, PROP_Z, POL_X, POL_Y
     pos, _, box = A.make_particle("ellipsoid", 64, 0.8, 0.5)
     props, pols = [], []
+    pars, pers = [], []  # scattering plane of each orientation's beam frame, for both polarizations
     for a, b, g in orientations(32):
         prop, pY, pX = (rotate_zyz(a, b, g, v) for v in (PROP_Z, POL_Y, POL_X))
         props += [prop, prop]
         pols += [pY, pX]
+        pars += [pY, pY]
+        pers += [pX, pX]
     K = len(props)
     m = 1.33 + 0.001j
     grid = A.fast_grid(box)
@@ -294,7 +297,7 @@ def bench_batched(A, args):
                 op.sync()
                 t = time.perf_counter()
                 st = bp.solve(method, eps=1e-10, **kw)
-                cx = [bp.cext(k) for k in range(K)]
+                cx = list(bp.cext_all())
                 res[(method, name)] = (time.perf_counter() - t, cx, st)
             c64 = np.array(res[(method, "fp64")][1])
             cm = np.array(res[(method, "mixed")][1])
@@ -311,6 +314,21 @@ def bench_batched(A, args):
                     "tts_batched_s": round(tv, 4), "iterations_max": max(s["niter"] for s in sv),
                     "max_true_resid": max(s["true_resid"] for s in sv),
                     "cext_max_rel_err_vs_fp64": float(np.max(np.abs(np.array(cv) - c64) / np.abs(c64)))}
+        # post-processing of the last (FP64 QMR_CS) solution: Cext, Cabs and the 181-angle scattering-plane fields of
+        # every RHS, one RHS at a time (ADDA's orientation loop) and in one batched evaluation
+        bp.solve("qmr", eps=1e-10)
+        op.sync()
+        t = time.perf_counter()
+        one = [(bp.cext(k), bp.cabs(k), bp.eplane(k, pars[k], pers[k], 181)) for k in range(K)]
+        t_one = time.perf_counter() - t
+        t = time.perf_counter()
+        ce, ca, ep = bp.cext_all(), bp.cabs_all(), bp.eplane_all(pars, pers, 181)
+        t_all = time.perf_counter() - t
+        out["postprocessing"] = {
+            "what": "Cext + Cabs + E in the 181-angle scattering plane, for all RHS", "per_rhs_s": round(t_one, 4),
+            "batched_s": round(t_all, 4), "speedup": round(t_one / t_all, 2),
+            "batched_equals_per_rhs": bool(all(ce[k] == one[k][0] and ca[k] == one[k][1] and
+                                               np.array_equal(ep[k], one[k][2]) for k in range(K)))}
         # the same RHS one after another on one operator (ADDA's loop over orientations and polarizations)
         pr = A.Problem(pos, box, m, fp32=True, grid=grid)
         try:
@@ -329,7 +347,7 @@ def bench_batched(A, args):
         out["qmr"]["speedup_batched_mixed_vs_sequential_fp64"] = round(t_seq / out["qmr"]["tts_batched_mixed_s"], 3)
         out["qmr"]["cext_max_rel_err_batched_vs_sequential"] = float(np.max(np.abs(np.array(cs) - np.array(
             res[("qmr", "fp64")][1])) / np.abs(np.array(cs))))
-        log(f"batched RHS ({K}): " + json.dumps({k: out[k] for k in ("bicgstab", "qmr")}))
+        log(f"batched RHS ({K}): " + json.dumps({k: out[k] for k in ("bicgstab", "qmr", "postprocessing")}))
     finally:
         bp.close()
     return out

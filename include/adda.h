@@ -117,6 +117,21 @@ int adda_eplane_yz(adda_hip_op *op, const adda_particle *p, double wavenum, doub
 /* ComputeMuellerMatrix over the yz plane (src/CalculateE.c:83-116,236-246); out[16*npoints]. */
 void adda_mueller_yz(const adda_cplx *eplX, const adda_cplx *eplY, int npoints, double *out);
 
+/* The same quantities for the K right-hand sides of a batched solve (d_pvec a batched device vector), each RHS with
+ * its own beam: props, incpols, pars, pers are [K][3], cc is [K][nmat][3], ebuff [K][ndir][3], eplane
+ * [K][npoints][2]. ADDA computes them once per orientation (calculate_one_orientation, src/calculator.c:293-346);
+ * here each kernel runs once for the whole batch, and every RHS's result equals the single-RHS function's bit for
+ * bit. */
+int adda_calc_field_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *dirs, int ndir, adda_cplx *ebuff);
+int adda_ext_cross_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *props, const double *incpols, double *cext);
+int adda_abs_cross_batch(adda_hip_op *op, const adda_particle *p, double wavenum, const adda_cplx *cc,
+	const void *d_pvec, int K, double *cabs);
+int adda_eplane_yz_batch(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace,
+	const void *d_pvec, int K, const double *props, const double *pars, const double *pers, int ntheta, int npoints,
+	adda_cplx *eplane);
+
 /* ---- adjoint gradient (no counterpart in the reference: derivatives of its Cext and Cabs) ---------------------- */
 /* d(cc)/d(eps) of adda_couple_constant at eps = m^2. */
 adda_cplx adda_couple_constant_deps(adda_cplx m, int polrel, double wavenum, double gridspace, const double prop[3],

@@ -114,6 +114,17 @@ int adda_hip_field_sums(adda_hip_op *op, int prec, const void *pvec, double kd, 
 /* sum_j sum_c w[mat_j][c] |v[3j+c]|^2 (the AbsCross sum, src/crosssec.c:873-884); w is [nmat][3]. */
 int adda_hip_weighted_norm2(adda_hip_op *op, int prec, const void *v, const double *w, double *out);
 
+/* Batched forms of the two sums above for K right-hand sides (a batched device vector, K <= the reserved batch):
+ * adda_hip_field_sums_batch takes ndir directions per RHS, dirs[K][ndir][3] -> sums[K][ndir][3];
+ * adda_hip_weighted_norm2_batch takes one weight table per RHS, w[K][nmat][3] -> out[K].
+ * ADDA evaluates CalcFieldFree (src/crosssec.c:506-595) and AbsCross (src/crosssec.c:873-884) once per orientation
+ * (calculate_one_orientation, src/calculator.c:293-346, called by orient_integrand, src/calculator.c:348-358); these
+ * replace that loop for the post-processing of a batched solve, in one launch per kernel. Each RHS's result equals
+ * the single-RHS call bit for bit. */
+int adda_hip_field_sums_batch(adda_hip_op *op, int prec, const void *pvec, int K, double kd, const double *dirs,
+	int ndir, adda_cplx *sums);
+int adda_hip_weighted_norm2_batch(adda_hip_op *op, int prec, const void *v, int K, const double *w, double *out);
+
 /* Measurement: runs nrep matvecs and times each of the 5 passes with HIP events on the operator's stream;
  * pass_ms[5] receives the average duration of P1 (scatter + FFT-x), P2 (FFT-z), P3 (FFT-y, D-matrix product,
  * inverse FFT-y), P4 (inverse FFT-z), P5 (inverse FFT-x + gather); total_ms the average per matvec.

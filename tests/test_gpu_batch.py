@@ -177,3 +177,36 @@ def test_batch_of_one_and_gating(hipdev):
         op.free(r)
     finally:
         bp.close()
+
+
+def test_batched_postprocessing(hipdev):
+    """Cext, Cabs and the scattering-plane fields of every RHS from one batched evaluation equal the single-RHS
+    functions bit for bit; for the orientation the reference ran (30, 40, 50; m = 1.33+0.05i, dpl 12) the batched
+    values reproduce its CrossSec and mueller outputs."""
+    pos, _ = G.geometry("ell12")
+    p, box = read_shape(pos)
+    props, pols, pars, pers = [], [], [], []
+    for a, b, g in [(30, 40, 50)] + [tuple(o) for o in orientations(2)]:
+        prop, pY, pX = (rotate_zyz(a, b, g, v) for v in (PROP_Z, POL_Y, POL_X))
+        for vec in (pY, pX):
+            props.append(prop)
+            pols.append(vec)
+            pars.append(pY)
+            pers.append(pX)
+    bp = hipdev.BatchProblem(p, box, 1.33 + 0.05j, np.array(props), np.array(pols), dpl=12)
+    try:
+        res = bp.solve("qmr", eps=1e-10)
+        assert all(r["rc"] == 0 for r in res)
+        ce, ca = bp.cext_all(), bp.cabs_all()
+        ep = bp.eplane_all(pars, pers, 20, 38)
+        for k in range(bp.K):
+            assert ce[k] == bp.cext(k) and ca[k] == bp.cabs(k), k
+            assert np.array_equal(ep[k], bp.eplane(k, pars[k], pers[k], 20, 38)), k
+        for j, pol in ((0, "Y"), (1, "X")):
+            g = G.crosssec("ell12_orient", pol)
+            assert G.numeric_close([ce[j], ca[j]], [g["Cext"], g["Cabs"]], 1e-14, 1e-8)[0], (pol, ce[j], ca[j], g)
+        mu = hipdev.mueller_yz(ep[1], ep[0], 38)
+        gm = G.mueller("ell12_orient")
+        assert G.mueller_close(mu, gm[:, 1:], gm[:, 1])[0]
+    finally:
+        bp.close()

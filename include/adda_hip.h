@@ -68,7 +68,9 @@ void adda_hip_op_destroy(adda_hip_op *op);
 int adda_hip_set_cc_sqrt(adda_hip_op *op, const adda_cplx *cc_sqrt);
 
 /* Copies the D-matrix to the host in ADDA's layout Dmatrix[6*((x*DsizeZ+z)*DsizeY+y)+comp] (IndexDmatrix,
- * src/fft.c:144-150); dst holds 6*gridX*DsizeY*DsizeZ complex numbers. For checks against a CPU build. */
+ * src/fft.c:144-150); dst holds 6*gridX*DsizeY*DsizeZ complex numbers. For checks against a CPU build. With the
+ * reduced FFT the device stores only x <= gridX/2 (G is even or odd in x as in y and z); columns x > gridX/2 are
+ * filled from column gridX - x with the xy and xz components negated. */
 int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst);
 size_t adda_hip_dmatrix_size(const adda_hip_op *op);
 

@@ -629,7 +629,10 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		 * steps) finishes with an FP64 Krylov solve of the same correction equation -- the FP64 residual decides
 		 * every outcome, so no RHS ends on an unchecked FP32 result. */
 		const int v64 = opt->precision == ADDA_PREC_MIXED_V64;
-		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-5;
+		/* Default step 1e-4: each inner solve stays two decades above the FP32 operator's accuracy (about 1e-6 true
+		 * residual on the 1.1 M-dipole sphere), where its recurrence residual still tracks the true one. Over four
+		 * test problems this took 2.5 % more iterations than FP64 on average, 1e-5 took 5 %. */
+		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-4;
 		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
 		const int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
 		const int ihl = (hist_len < cap ? hist_len : cap) + 2; /* inner entries beyond hist_len are not kept */

@@ -61,7 +61,7 @@ typedef struct {
 	int init_field;      /* enum adda_init */
 	int recalc_resid;    /* -recalc_resid: recompute the true residual at the end */
 	int precision;       /* enum adda_prec */
-	double inner_eps;    /* mixed: relative tolerance of each FP32 inner solve (default 1e-5 when 0) */
+	double inner_eps;    /* mixed: relative tolerance of each FP32 inner solve (default 1e-4 when 0) */
 	int max_outer;       /* mixed: maximum FP64 correction steps (default 30 when 0) */
 	int verbose;         /* bit 0: print RE_ lines to stdout; bit 1: a progress line (iteration, residual) about every
 	                        10 s of a device-resident solve, inner solves of the mixed forms included */

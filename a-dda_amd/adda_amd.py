@@ -302,8 +302,9 @@ class Operator:
 # ---------------------------------------------------------------------------------------------------------------------
 # C host layer (include/adda.h)
 
-IT_QMR, IT_BICGSTAB, IT_CGNR, IT_BICG, IT_CSYM = 0, 1, 2, 3, 4
-ITER_NAMES = {"qmr": IT_QMR, "bicgstab": IT_BICGSTAB, "cgnr": IT_CGNR, "bicg": IT_BICG, "csym": IT_CSYM}
+IT_QMR, IT_BICGSTAB, IT_CGNR, IT_BICG, IT_CSYM, IT_BCGS2, IT_QMR2 = 0, 1, 2, 3, 4, 5, 6
+ITER_NAMES = {"qmr": IT_QMR, "bicgstab": IT_BICGSTAB, "cgnr": IT_CGNR, "bicg": IT_BICG, "csym": IT_CSYM,
+              "bcgs2": IT_BCGS2, "qmr2": IT_QMR2}
 IF_AUTO, IF_ZERO, IF_INC = 0, 1, 2
 POL_LDR, POL_CM, POL_RRC = 0, 1, 2
 PREC_FP64, PREC_MIXED, PREC_MIXED_V64 = 0, 1, 2

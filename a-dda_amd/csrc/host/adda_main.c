@@ -1,7 +1,7 @@
 /* adda_main.c -- command-line front end with ADDA's option names (src/param.c:401-650) for the hot path:
  *
  *   adda_hip [-shape sphere | ellipsoid <y/x> <z/x> | box <y/x> <z/x> | read <file>] [-grid <n>] [-size <d>]
- *            [-dpl <n>] [-lambda <l>] [-m <re> <im> [<re> <im> ...]] [-iter qmr|bicgstab|cgnr|bicg|csym]
+ *            [-dpl <n>] [-lambda <l>] [-m <re> <im> [<re> <im> ...]] [-iter qmr|qmr2|bicgstab|bcgs2|cgnr|bicg|csym]
  *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
  *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
@@ -127,6 +127,8 @@ int main(int argc, char **argv)
 			else if (!strcmp(s, "cgnr")) opt.method = ADDA_IT_CGNR;
 			else if (!strcmp(s, "bicg")) opt.method = ADDA_IT_BICG_CS;
 			else if (!strcmp(s, "csym")) opt.method = ADDA_IT_CSYM;
+			else if (!strcmp(s, "bcgs2")) opt.method = ADDA_IT_BCGS2;
+			else if (!strcmp(s, "qmr2")) opt.method = ADDA_IT_QMR_CS_2;
 			else { fprintf(stderr, "ERROR: unsupported -iter %s\n", s); return EXIT_FAILURE; }
 		}
 		else if (!strcmp(a, "-eps")) { NEED(1); eps_k = atof(argv[++i]); }

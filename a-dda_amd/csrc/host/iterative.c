@@ -55,14 +55,19 @@ static void *kvec(kctx *k) { return adda_hip_vec_alloc(k->op, k->prec); }
 
 /* One Krylov solve of A.x = b in precision k->prec (IterativeSolver body, src/iterative.c:1398-1530).
  * b: right-hand side (already S.Einc); x: in/out (initial field per init_field); on return *niter_out iterations.
- * resid_scale/epsB follow src/iterative.c:1423-1425. The per-method bodies restate src/iterative.c:640-1032. */
+ * resid_scale/epsB follow src/iterative.c:1423-1425. The per-method bodies restate src/iterative.c:342-1143
+ * (BCGS2, BiCG_CS, BiCGStab, CGNR, CSYM, QMR_CS, QMR_CS_2). */
 static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, int verbose, const void *b, void *x,
 	double *hist, int hist_len, int *niter_out, double *final_rel)
 {
-	const int mc_tab[5] = {50000, 30000, 10, 50000, 10}; /* params[].mc, src/iterative.c:118-132 */
+	const int mc_tab[7] = {50000, 30000, 10, 50000, 10, 15000, 50000}; /* params[].mc, src/iterative.c:118-132 */
 	int mc = mc_tab[method], rc = 0;
 	void *r = kvec(k), *Av = kvec(k), *p = kvec(k), *v1 = kvec(k), *v2 = kvec(k), *v3 = kvec(k);
-	if (!r || !Av || !p || !v1 || !v2 || !v3) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+	void *v4 = method == ADDA_IT_BCGS2 ? kvec(k) : NULL;
+	if (!r || !Av || !p || !v1 || !v2 || !v3 || (method == ADDA_IT_BCGS2 && !v4)) {
+		rc = host_err_hip(__FILE__, __LINE__);
+		goto done;
+	}
 	double temp;
 #define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
 	CKG(dotp(k, b, NULL, ADDA_DOT_NORM2, NULL, &temp));
@@ -104,6 +109,13 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 	dcx beta = 0, tautilda = 0, s_new = 0, s_old = 0, ro_old = 0, omega = 0, alpha = 0, tau = 0;
 	void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;    /* QMR_CS renames, src/iterative.c:916-921 */
 	void *q_new = r, *q_old = v1, *pc_new = p, *pc_old = v2; /* CSYM renames, src/iterative.c:795-799 */
+	/* BCGS2 (l = 2): R[0..2] residuals, U[0..2] directions, p = r~_0 (src/iterative.c:365-377); U[1] is Av so that
+	 * the initial field's A.b serves the first iteration */
+	void *R[3] = {r, v1, v3}, *U[3] = {v2, Av, v4};
+	dcx rho0 = 0;
+	/* QMR_CS_2 state (src/iterative.c:1050-1053): v = v1, d = v2, p_k = p */
+	double q2_ro = 0, q2_c = 0, q2_theta = 0;
+	dcx q2_eps = 0, q2_eta = 0;
 	/* PHASE_INIT */
 	if (method == ADDA_IT_QMR_CS) {
 		dcx t;
@@ -117,6 +129,18 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		s_new = s_old = 0.0;
 	}
 	else if (method == ADDA_IT_BICGSTAB) CKG(lc(k, v3, r, 1, NULL, 0, NULL, 0, 0, NULL)); /* r~ = r_0 */
+	else if (method == ADDA_IT_BCGS2) { /* src/iterative.c:384-387 */
+		CKG(lc(k, p, r, 1, NULL, 0, NULL, 0, 0, NULL));
+		rho0 = -1;
+	}
+	else if (method == ADDA_IT_QMR_CS_2) { /* src/iterative.c:1070-1080 */
+		q2_ro = sqrt(inprodR);
+		CKG(lc(k, v1, r, 1, NULL, 0, NULL, 0, 0, NULL));
+		q2_c = 1;
+		q2_eps = 1;
+		q2_theta = 0;
+		q2_eta = -1;
+	}
 	else if (method == ADDA_IT_CSYM) {
 		tau = sqrt(inprodR);
 		CKG(lc(k, q_new, q_new, 1 / creal(tau), NULL, 0, NULL, 0, ADDA_CONJ_A, NULL));
@@ -227,6 +251,101 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 			CKG(lc(k, r, r, 1, Av, -alpha, NULL, 0, 0, &inprodRp1));
 			ro_old = ro_new;
 		}
+		else if (method == ADDA_IT_BCGS2) { /* src/iterative.c:391-483 */
+			dcx rho1, sigma;
+			int brk = 0;
+			for (int j = 0; j < 2 && !brk; j++) { /* BiCG part */
+				CKG(dotp(k, R[j], p, ADDA_DOT_CONJ, &rho1, NULL));
+				if (niter == 1 && j == 0) CKG(lc(k, U[0], R[0], 1, NULL, 0, NULL, 0, 0, NULL));
+				else {
+					if (cabs(rho0) / (cabs(rho1) * cabs(alpha)) < 1e-10) {
+						rc = host_err(__FILE__, __LINE__, "BCGS2 fails: 1/|beta| is too small");
+						brk = 1;
+						break;
+					}
+					beta = alpha * rho1 / rho0;
+					for (int i = 0; i <= j; i++) CKG(lc(k, U[i], U[i], -beta, R[i], 1, NULL, 0, 0, NULL));
+				}
+				rho0 = rho1;
+				if (!(niter == 1 && j == 0 && matvec_ready)) CKG(mv(k, U[j], U[j + 1], NULL, 0));
+				CKG(dotp(k, U[j + 1], p, ADDA_DOT_CONJ, &sigma, NULL));
+				if (cabs(sigma) / cabs(rho1) < 1e-10) {
+					rc = host_err(__FILE__, __LINE__, "BCGS2 fails: |u_%d.r~|/|r_%d.r~| is too small", j + 1, j);
+					brk = 1;
+					break;
+				}
+				alpha = rho1 / sigma;
+				CKG(lc(k, x, x, 1, U[0], alpha, NULL, 0, 0, NULL));
+				for (int i = 0; i <= j; i++) CKG(lc(k, R[i], R[i], 1, U[i + 1], -alpha, NULL, 0, 0, NULL));
+				CKG(mv(k, R[j], R[j + 1], NULL, 0));
+			}
+			if (brk) break;
+			/* minimal-residual polynomial of degree 2 over Z = R^H R, Z[i][j] = R_j . conj(R_i) */
+			dcx Z[3][3], y0[3], yl[3], zy0[3], zyl[3], vr = 0;
+			for (int i = 0; i < 3; i++)
+				for (int j = 0; j <= i; j++) {
+					CKG(dotp(k, R[j], R[i], ADDA_DOT_CONJ, &Z[i][j], NULL));
+					Z[j][i] = conj(Z[i][j]);
+				}
+			y0[0] = -1; y0[1] = Z[1][0] / Z[1][1]; y0[2] = 0;
+			yl[0] = 0; yl[1] = Z[1][2] / Z[1][1]; yl[2] = -1;
+			double k0 = 0, kl = 0;
+			for (int i = 0; i < 3; i++) {
+				zy0[i] = Z[i][0] * y0[0] + Z[i][1] * y0[1] + Z[i][2] * y0[2];
+				zyl[i] = Z[i][0] * yl[0] + Z[i][1] * yl[1] + Z[i][2] * yl[2];
+			}
+			for (int i = 0; i < 3; i++) {
+				k0 += creal(zy0[i] * conj(y0[i]));
+				kl += creal(zyl[i] * conj(yl[i]));
+				vr += zy0[i] * conj(yl[i]);
+			}
+			k0 = sqrt(k0);
+			kl = sqrt(kl);
+			vr /= k0 * kl;
+			double avr = cabs(vr);
+			dcx hg = vr * (avr > 0.7 ? avr : 0.7) / avr;
+			for (int i = 0; i < 3; i++) y0[i] += (-hg * k0 / kl) * yl[i];
+			omega = y0[2];
+			for (int i = 1; i <= 2; i++) {
+				CKG(lc(k, U[0], U[0], 1, U[i], -y0[i], NULL, 0, 0, NULL));
+				CKG(lc(k, x, x, 1, R[i - 1], y0[i], NULL, 0, 0, NULL));
+				CKG(lc(k, R[0], R[0], 1, R[i], -y0[i], NULL, 0, 0, NULL));
+			}
+			inprodRp1 = 0;
+			for (int i = 0; i < 3; i++)
+				inprodRp1 += creal((Z[i][0] * y0[0] + Z[i][1] * y0[1] + Z[i][2] * y0[2]) * conj(y0[i]));
+			rho0 *= -omega;
+		}
+		else if (method == ADDA_IT_QMR_CS_2) { /* src/iterative.c:1082-1136 */
+			void *vq = v1, *d = v2;
+			dcx delta, bt;
+			double nv2;
+			CKG(lc(k, vq, vq, 1 / q2_ro, NULL, 0, NULL, 0, 0, NULL));
+			CKG(dotp(k, vq, NULL, ADDA_DOT_SELF_BILIN, &delta, NULL));
+			if (cabs(delta) < 1e-10) { rc = host_err(__FILE__, __LINE__, "QMR_CS_2 fails: |vT.v| is too small"); break; }
+			if (niter == 1) CKG(lc(k, p, vq, 1, NULL, 0, NULL, 0, 0, NULL));
+			else CKG(lc(k, p, p, -q2_ro * delta / q2_eps, vq, 1, NULL, 0, 0, NULL));
+			if (niter == 1 && matvec_ready) CKG(lc(k, Av, Av, 1 / q2_ro, NULL, 0, NULL, 0, 0, NULL));
+			else CKG(mv(k, p, Av, NULL, 0));
+			CKG(dotp(k, p, Av, ADDA_DOT_BILIN, &q2_eps, NULL));
+			bt = q2_eps / delta;
+			if (cabs(q2_eps) < 1e-10) { rc = host_err(__FILE__, __LINE__, "QMR_CS_2 fails: |pT.A.p| is too small"); break; }
+			CKG(lc(k, vq, vq, -bt, Av, 1, NULL, 0, 0, &nv2));
+			double ro_new = sqrt(nv2);
+			double th = ro_new / (q2_c * cabs(bt));
+			double c_nw = 1 / sqrt(1 + th * th), sabs2 = th * th / (1 + th * th), cr = c_nw / q2_c;
+			q2_eta = -q2_ro * cr * cr * q2_eta / bt;
+			if (niter == 1) CKG(lc(k, d, p, q2_eta, NULL, 0, NULL, 0, 0, NULL));
+			else {
+				double f = q2_theta * c_nw;
+				CKG(lc(k, d, d, f * f, p, q2_eta, NULL, 0, 0, NULL));
+			}
+			CKG(lc(k, x, x, 1, d, 1, NULL, 0, 0, NULL));
+			CKG(lc(k, r, r, sabs2, vq, -q2_eta, NULL, 0, 0, &inprodRp1));
+			q2_ro = ro_new;
+			q2_theta = th;
+			q2_c = c_nw;
+		}
 		else { /* CSYM, src/iterative.c:822-888 */
 			dcx eta, gamma, theta, invksi, temp1;
 			double dtmp, sum;
@@ -284,8 +403,8 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 	*final_rel = sqrt(resid_scale * inprodR);
 done:
 	{
-		void *vs[] = {r, Av, p, v1, v2, v3};
-		for (int i = 0; i < 6; i++)
+		void *vs[] = {r, Av, p, v1, v2, v3, v4};
+		for (int i = 0; i < 7; i++)
 			if (vs[i]) adda_hip_vec_free(k->op, vs[i]);
 	}
 	return rc;
@@ -312,8 +431,8 @@ typedef struct {
 
 static const char *method_name(int m)
 {
-	static const char *n[5] = {"QMR_CS", "BiCGStab", "CGNR", "BiCG_CS", "CSYM"};
-	return m >= 0 && m < 5 ? n[m] : "?";
+	static const char *n[7] = {"QMR_CS", "BiCGStab", "CGNR", "BiCG_CS", "CSYM", "BCGS2", "QMR_CS_2"};
+	return m >= 0 && m < 7 ? n[m] : "?";
 }
 
 static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int *it, int matvec_ready, void **v,
@@ -574,7 +693,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		memset(&st[k], 0, sizeof st[k]);
 		st[k].true_resid = -1;
 	}
-	if (opt->method < 0 || opt->method > 4) return host_err(__FILE__, __LINE__, "unknown iterative method %d", opt->method);
+	if (opt->method < 0 || opt->method > 6) return host_err(__FILE__, __LINE__, "unknown iterative method %d", opt->method);
 	if (K > 1 && !device_method(opt))
 		return host_err(__FILE__, __LINE__, "batched solves need the device-resident recurrences (QMR_CS, BiCGStab, CGNR)");
 	int rc = 0;

@@ -15,7 +15,10 @@ extern "C" {
 #endif
 
 /* iterative methods, ADDA's enum iter subset (src/const.h; option -iter, src/param.c:1230-1245) */
-enum adda_iter { ADDA_IT_QMR_CS = 0, ADDA_IT_BICGSTAB = 1, ADDA_IT_CGNR = 2, ADDA_IT_BICG_CS = 3, ADDA_IT_CSYM = 4 };
+enum adda_iter {
+	ADDA_IT_QMR_CS = 0, ADDA_IT_BICGSTAB = 1, ADDA_IT_CGNR = 2, ADDA_IT_BICG_CS = 3, ADDA_IT_CSYM = 4,
+	ADDA_IT_BCGS2 = 5, ADDA_IT_QMR_CS_2 = 6
+};
 /* initial field (option -init_field, src/param.c:1146-1161) */
 enum adda_init { ADDA_IF_AUTO = 0, ADDA_IF_ZERO = 1, ADDA_IF_INC = 2 };
 /* polarizability relation (option -pol, subset) */

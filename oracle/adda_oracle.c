@@ -352,8 +352,8 @@ int or_iterative_solver(const or_grid *g, const or_cplx *dmat, const or_cplx *cc
 	dcx *x = (dcx *)x_o;
 	dcx *b = xmalloc(n * sizeof(dcx)), *r = xmalloc(n * sizeof(dcx)), *Av = xmalloc(n * sizeof(dcx));
 	dcx *p = xmalloc(n * sizeof(dcx)), *v1 = xmalloc(n * sizeof(dcx)), *v2 = xmalloc(n * sizeof(dcx));
-	dcx *v3 = xmalloc(n * sizeof(dcx));
-	const int mc_tab[5] = {50000, 30000, 10, 50000, 10}; /* params[].mc, src/iterative.c:118-132 */
+	dcx *v3 = xmalloc(n * sizeof(dcx)), *v4 = xmalloc(n * sizeof(dcx));
+	const int mc_tab[7] = {50000, 30000, 10, 50000, 10, 15000, 50000}; /* params[].mc, src/iterative.c:118-132 */
 	int mc = mc_tab[method];
 	int rc = 0;
 	/* b = S.Einc (nMult_mat), src/iterative.c:1421-1425 */
@@ -395,6 +395,12 @@ int or_iterative_solver(const or_grid *g, const or_cplx *dmat, const or_cplx *cc
 	double beta_d = 0;
 	dcx *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;   /* QMR_CS renames, src/iterative.c:916-921 */
 	dcx *q_new = r, *q_old = v1, *pc_new = p, *pc_old = v2; /* CSYM renames, src/iterative.c:795-799 */
+	/* BCGS2 (l = 2) renames, src/iterative.c:365-377: R[0..2] residuals, U[0..2] search directions, p = r~_0 */
+	dcx *R[3] = {r, v1, v3}, *U[3] = {v2, Av, v4};
+	dcx rho0 = 0;
+	/* QMR_CS_2 state, src/iterative.c:1050-1053 (v = vec1, d = vec2, p = pvec) */
+	double q2_ro = 0, q2_c = 0, q2_theta = 0;
+	dcx q2_eps = 0, q2_eta = 0;
 	/* PHASE_INIT */
 	if (method == 0) {
 		omega_old = 0.0;
@@ -406,6 +412,15 @@ int or_iterative_solver(const or_grid *g, const or_cplx *dmat, const or_cplx *cc
 		s_new = s_old = 0.0;
 	}
 	else if (method == 1) memcpy(v3, r, n * sizeof(dcx)); /* r~ = r_0 */
+	else if (method == 5) { memcpy(p, r, n * sizeof(dcx)); rho0 = -1; } /* src/iterative.c:384-387 */
+	else if (method == 6) { /* src/iterative.c:1070-1080 */
+		q2_ro = sqrt(inprodR);
+		memcpy(v1, r, n * sizeof(dcx));
+		q2_c = 1;
+		q2_eps = 1;
+		q2_theta = 0;
+		q2_eta = -1;
+	}
 	else if (method == 4) {
 		tau = sqrt(inprodR);
 		for (size_t i = 0; i < n; i++) q_new[i] = conj(q_new[i]) * (1 / creal(tau));
@@ -558,6 +573,108 @@ int or_iterative_solver(const or_grid *g, const or_cplx *dmat, const or_cplx *cc
 			inprodRp1 = sum;
 			ro_old = ro_new;
 		}
+		else if (method == 5) { /* BCGS2, src/iterative.c:391-483 */
+			dcx rho1, sigma;
+			for (int j = 0; j < 2; j++) { /* BiCG part */
+				rho1 = dot(R[j], p, n);
+				if (niter == 1 && j == 0) memcpy(U[0], R[0], n * sizeof(dcx));
+				else {
+					if (cabs(rho0) / (cabs(rho1) * cabs(alpha)) < 1e-10) { rc = -1; break; }
+					beta = alpha * rho1 / rho0;
+					for (int i = 0; i <= j; i++)
+						for (size_t e = 0; e < n; e++) U[i][e] = -beta * U[i][e] + R[i][e];
+				}
+				rho0 = rho1;
+				if (!(niter == 1 && j == 0 && matvec_ready)) mv(&S, U[j], U[j + 1], NULL, 0);
+				sigma = dot(U[j + 1], p, n);
+				if (cabs(sigma) / cabs(rho1) < 1e-10) { rc = -1; break; }
+				alpha = rho1 / sigma;
+				for (size_t e = 0; e < n; e++) x[e] += alpha * U[0][e];
+				for (int i = 0; i <= j; i++)
+					for (size_t e = 0; e < n; e++) R[i][e] += -alpha * U[i + 1][e];
+				mv(&S, R[j], R[j + 1], NULL, 0);
+			}
+			if (rc) break;
+			/* minimal-residual polynomial of degree 2 over Z = R^H R, with Z[i][j] = R_j . conj(R_i) */
+			dcx Z[3][3], y0[3], yl[3], zy0[3], zyl[3], vr = 0;
+			for (int i = 0; i < 3; i++)
+				for (int j = 0; j <= i; j++) {
+					Z[i][j] = dot(R[j], R[i], n);
+					Z[j][i] = conj(Z[i][j]);
+				}
+			y0[0] = -1; y0[1] = Z[1][0] / Z[1][1]; y0[2] = 0;
+			yl[0] = 0; yl[1] = Z[1][2] / Z[1][1]; yl[2] = -1;
+			double k0 = 0, kl = 0;
+			for (int i = 0; i < 3; i++) {
+				zy0[i] = Z[i][0] * y0[0] + Z[i][1] * y0[1] + Z[i][2] * y0[2];
+				zyl[i] = Z[i][0] * yl[0] + Z[i][1] * yl[1] + Z[i][2] * yl[2];
+			}
+			for (int i = 0; i < 3; i++) {
+				k0 += creal(zy0[i] * conj(y0[i]));
+				kl += creal(zyl[i] * conj(yl[i]));
+				vr += zy0[i] * conj(yl[i]);
+			}
+			k0 = sqrt(k0);
+			kl = sqrt(kl);
+			vr /= k0 * kl;
+			double avr = cabs(vr);
+			dcx hg = vr * (avr > 0.7 ? avr : 0.7) / avr;
+			dcx t = -hg * k0 / kl;
+			for (int i = 0; i < 3; i++) y0[i] += t * yl[i];
+			omega = y0[2];
+			for (int i = 1; i <= 2; i++)
+				for (size_t e = 0; e < n; e++) {
+					U[0][e] += -y0[i] * U[i][e];
+					x[e] += y0[i] * R[i - 1][e];
+					R[0][e] += -y0[i] * R[i][e];
+				}
+			inprodRp1 = 0;
+			for (int i = 0; i < 3; i++) {
+				dcx z = Z[i][0] * y0[0] + Z[i][1] * y0[1] + Z[i][2] * y0[2];
+				inprodRp1 += creal(z * conj(y0[i]));
+			}
+			rho0 *= -omega;
+		}
+		else if (method == 6) { /* QMR_CS_2, src/iterative.c:1082-1136 (v = v1, d = v2, p_k = p) */
+			dcx *vq = v1, *d = v2, delta, bt, temp1;
+			double inv = 1 / q2_ro, nv2 = 0;
+			for (size_t e = 0; e < n; e++) vq[e] *= inv;
+			delta = dotc(vq, vq, n);
+			if (cabs(delta) < 1e-10) { rc = -1; break; }
+			if (niter == 1) memcpy(p, vq, n * sizeof(dcx));
+			else {
+				temp1 = -q2_ro * delta / q2_eps;
+				for (size_t e = 0; e < n; e++) p[e] = temp1 * p[e] + vq[e];
+			}
+			if (niter == 1 && matvec_ready) for (size_t e = 0; e < n; e++) Av[e] *= inv;
+			else mv(&S, p, Av, NULL, 0);
+			q2_eps = dotc(p, Av, n);
+			bt = q2_eps / delta;
+			if (cabs(q2_eps) < 1e-10) { rc = -1; break; }
+			for (size_t e = 0; e < n; e++) {
+				vq[e] = -bt * vq[e] + Av[e];
+				nv2 += creal(vq[e]) * creal(vq[e]) + cimag(vq[e]) * cimag(vq[e]);
+			}
+			double ro_new = sqrt(nv2);
+			double th = ro_new / (q2_c * cabs(bt));
+			double c_nw = 1 / sqrt(1 + th * th), sabs2 = th * th / (1 + th * th), cr = c_nw / q2_c;
+			q2_eta = -q2_ro * cr * cr * q2_eta / bt;
+			if (niter == 1) for (size_t e = 0; e < n; e++) d[e] = q2_eta * p[e];
+			else {
+				double f = q2_theta * c_nw;
+				for (size_t e = 0; e < n; e++) d[e] = f * f * d[e] + q2_eta * p[e];
+			}
+			double sum = 0;
+			for (size_t e = 0; e < n; e++) {
+				x[e] += d[e];
+				r[e] = sabs2 * r[e] - q2_eta * vq[e];
+				sum += creal(r[e]) * creal(r[e]) + cimag(r[e]) * cimag(r[e]);
+			}
+			inprodRp1 = sum;
+			q2_ro = ro_new;
+			q2_theta = th;
+			q2_c = c_nw;
+		}
 		else { /* CSYM, src/iterative.c:822-888 */
 			dcx eta, gamma, theta, invksi, temp1, temp2;
 			double dtmp;
@@ -621,7 +738,7 @@ int or_iterative_solver(const or_grid *g, const or_cplx *dmat, const or_cplx *cc
 		for (int c = 0; c < 3; c++) pvec_o[3 * i + c] = O(C(cc_sqrt[3 * g->mat[i] + c]) * x[3 * i + c]);
 	if (niter_out) *niter_out = niter - 1;
 	if (nmatvec_out) *nmatvec_out = S.nmv;
-	free(b); free(r); free(Av); free(p); free(v1); free(v2); free(v3);
+	free(b); free(r); free(Av); free(p); free(v1); free(v2); free(v3); free(v4);
 	return rc;
 }
 

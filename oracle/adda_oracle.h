@@ -68,8 +68,9 @@ or_cplx or_couple_constant(or_cplx m, int polrel, double wavenum, double gridspa
 void or_generate_b(const or_grid *g, const double prop[3], const double incpol[3], or_cplx *einc);
 
 /* IterativeSolver (src/iterative.c:1398-1530) with QMR_CS (898-1032), BiCGStab (640-724), CGNR (730-770),
- * BiCG_CS (495-634) and CSYM (774-894) restated; matvec is the FFT one if dmat != NULL, else the direct sum.
- * method: 0=qmr 1=bicgstab 2=cgnr 3=bicg 4=csym. init_field: 0=auto 1=zero 2=inc (src/iterative.c:1342-1394).
+ * BiCG_CS (495-634), CSYM (774-894), BCGS2 (342-488) and QMR_CS_2 (1039-1143) restated; matvec is the FFT one if
+ * dmat != NULL, else the direct sum.
+ * method: 0=qmr 1=bicgstab 2=cgnr 3=bicg 4=csym 5=bcgs2 6=qmr2. init_field: 0=auto 1=zero 2=inc (src/iterative.c:1342-1394).
  * On return: x (solution of the symmetrized system), pvec = S.x (polarization), resid_hist[0..niter] relative
  * residuals (may be NULL), *niter_out iterations, *nmatvec_out matvec count. Returns 0, or -1 on breakdown,
  * -2 on stagnation. */

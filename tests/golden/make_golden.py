@@ -61,6 +61,8 @@ RUNS = {
     "c1_cgnr": ("sphere16", "-dpl 15 -m 1.5 0 -iter cgnr -eps 10", ["CrossSec-Y", "CrossSec-X"]),
     "c1_bicg": ("sphere16", "-dpl 15 -m 1.5 0 -iter bicg -eps 10", ["CrossSec-Y", "CrossSec-X"]),
     "c1_csym": ("sphere16", "-dpl 15 -m 1.5 0 -iter csym -eps 10", ["CrossSec-Y", "CrossSec-X"]),
+    "c1_bcgs2": ("sphere16", "-dpl 15 -m 1.5 0 -iter bcgs2 -eps 10", ["CrossSec-Y", "CrossSec-X", "log"]),
+    "c1_qmr2": ("sphere16", "-dpl 15 -m 1.5 0 -iter qmr2 -eps 10", ["CrossSec-Y", "CrossSec-X", "log"]),
     "ell12_orient": ("ell12", "-dpl 12 -m 1.33 0.05 -orient 30 40 50 -iter qmr -eps 10 -ntheta 19 -store_dip_pol",
                      ["CrossSec-Y", "CrossSec-X", "mueller", "DipPol-Y", "DipPol-X"]),
     "sphere8_cm": ("sphere8", "-dpl 10 -m 1.1 0.1 -pol cm -iter qmr -eps 10 -ntheta 19 -store_dip_pol",

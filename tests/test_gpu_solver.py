@@ -19,7 +19,7 @@ from cases import Case, read_shape, PROP_Z, POL_X, POL_Y
 pytestmark = pytest.mark.gpu
 
 METHODS = [("qmr", "c1_qmr", 0), ("bicgstab", "c1_bicgstab", 1), ("cgnr", "c1_cgnr", 2), ("bicg", "c1_bicg", 3),
-           ("csym", "c1_csym", 4)]
+           ("csym", "c1_csym", 4), ("bcgs2", "c1_bcgs2", 5), ("qmr2", "c1_qmr2", 6)]
 
 
 def _problem(adda, geo, m, dpl, mat=None, polrel=0, fp32=False):
@@ -42,6 +42,31 @@ def test_c1_solvers(hipdev, method, gold, mi, host_scalars):
             g = G.crosssec(gold, pol)
             assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0], (pr.cext(), g["Cext"])
             assert abs(pr.cabs()) < 1e-12
+    finally:
+        pr.close()
+
+
+@pytest.mark.parametrize("method,gold,nmv_per_it", [("bcgs2", "c1_bcgs2", 4), ("qmr2", "c1_qmr2", 1)])
+def test_c1_bcgs2_qmr2_history_vs_reference_log(hipdev, method, gold, nmv_per_it):
+    """BCGS2 and QMR_CS_2 (host form): iteration and matvec counts of the reference's log, the first half of each
+    residual history to the log's print resolution (later iterations drift with the summation order)."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15)
+    ref = G.log_re(gold)
+    try:
+        off = 0
+        for vec in (POL_Y, POL_X):
+            pr.set_polarization(PROP_Z, vec)
+            r = pr.solve(method, eps=1e-10)
+            assert r["rc"] == 0
+            n = r["niter"]
+            g = ref[off:off + n + 1]
+            off += n + 1
+            assert len(g) == n + 1 and ref[off - 1] < 1e-10
+            k = (n + 2) // 2
+            assert np.allclose(r["hist"][:k], g[:k], rtol=1e-7)
+            assert np.allclose(r["hist"], g, rtol=0.15)
+            assert r["nmatvec"] == nmv_per_it * n  # x_0 = 0: the initial field's matvec serves iteration 1
+        assert off == len(ref)
     finally:
         pr.close()
 
@@ -124,7 +149,7 @@ def test_cm_and_multimaterial(hipdev):
 
 @pytest.mark.parametrize("v64", [False, True])
 @pytest.mark.parametrize("max_inner", [0, 40])
-@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr"])
+@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr", "bcgs2", "qmr2"])
 def test_mixed_precision_c1(hipdev, method, max_inner, v64):
     """FP32 inner Krylov (or FP64 inner Krylov around FP32 transforms, v64) + FP64 defect correction reaches the FP64
     answer: true residual <= 1e-10 and Cext within 1e-6 (the reference value 135.0448603 is itself converged to

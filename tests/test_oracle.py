@@ -160,7 +160,7 @@ def _solve_both(case, method, eps, prop, pY, pX, ntheta=None, polrel=0):
         r = case.solve(d, ccs, case.einc(prop, pol), method=method, eps=eps)
         assert r["rc"] == 0
         res[name] = dict(p=r["p"], cext=case.cext(r["p"], prop, pol), cabs=case.cabs(r["p"], cc), niter=r["niter"],
-                         hist=r["hist"])
+                         hist=r["hist"], nmatvec=r["nmatvec"])
         if ntheta:
             res[name]["epl"] = case.eplane(r["p"], prop, pY, pX, ntheta, 2 * (ntheta - 1))
     return res
@@ -185,7 +185,7 @@ def test_sample_golden_eps5():
 
 
 @pytest.mark.parametrize("method,name", [(0, "c1_qmr"), (1, "c1_bicgstab"), (2, "c1_cgnr"), (3, "c1_bicg"),
-                                         (4, "c1_csym")])
+                                         (4, "c1_csym"), (5, "c1_bcgs2"), (6, "c1_qmr2")])
 def test_c1_solvers_vs_sparse_reference(method, name):
     c = _c1()
     r = _solve_both(c, method, 1e-10, PROP_Z, POL_Y, POL_X, ntheta=181 if method == 0 else None)
@@ -198,6 +198,28 @@ def test_c1_solvers_vs_sparse_reference(method, name):
         gm = G.mueller("c1_qmr")
         ok, bad = G.numeric_close(mu, gm[:, 1:], 1e-9, 1e-5)
         assert ok, bad[:5]
+
+
+@pytest.mark.parametrize("method,name,nmv_per_it", [(5, "c1_bcgs2", 4), (6, "c1_qmr2", 1)])
+def test_c1_bcgs2_qmr2_history_vs_sparse_reference(method, name, nmv_per_it):
+    """BCGS2 (src/iterative.c:342-488) and QMR_CS_2 (1039-1143): iteration counts, matvec counts and the residual
+    history of the reference's log (both polarizations, Y first), to the summation-order drift of a
+    direct-sum vs FFT matvec."""
+    c = _c1()
+    ref = G.log_re(name)
+    r = _solve_both(c, method, 1e-10, PROP_Z, POL_Y, POL_X)
+    nY = r["Y"]["niter"]
+    assert len(ref) == nY + 1 + r["X"]["niter"] + 1
+    # the first half of each history to the log's print resolution; later iterations drift with the summation
+    # order (QMR_CS_2 reaches a few percent by 1e-10), so the whole history only loosely
+    for h, g in ((r["Y"]["hist"], ref[:nY + 1]), (r["X"]["hist"], ref[nY + 1:])):
+        k = (len(h) + 1) // 2
+        assert np.allclose(h[:k], g[:k], rtol=1e-7)
+        assert np.allclose(h, g, rtol=0.15)
+    with open(G.path(name, "log_RE")) as f:
+        total = int([ln for ln in f if "matrix-vector" in ln][0].split(":")[1])
+    # TotalMatVec counts CalcInitField's matvec too; with x_0 = 0 that matvec stands in for the first iteration's
+    assert total == sum(r[p]["nmatvec"] for p in ("Y", "X")) == sum(nmv_per_it * r[p]["niter"] for p in ("Y", "X"))
 
 
 def test_orientation_absorbing_ellipsoid():

@@ -126,6 +126,7 @@ struct MvCfg {
 	int LP;               // column units per block in P3
 	int p3r = 0;          // P3 with the D-matrix product in registers (k_p3r; compile-time power-of-two y plans)
 	int p3x = 1;          // P3 blocks hold column x with its mirror gridX - x (reduced D-matrix; else one column each)
+	int p3g = 0;          // P3 reads the staged twiddles of its y plan from global memory (k_p3 TWG; compile-time plans)
 };
 
 struct TwPair {
@@ -165,6 +166,8 @@ struct adda_hip_op {
 	FftPlan px, py, pz;      // plans of the D-matrix construction kernels
 	MvCfg cfg[2];            // matvec plans and tiles per precision (index ADDA_F64 / ADDA_F32)
 	TwPair twx, twy, twz;
+	TwPair twys;                // staged y twiddles of the plan P3 last ran with p3g, per precision (gY entries)
+	FftPlan2 twys_plan[2] = {}; // which plan twys holds ([0] FP64, [1] FP32); n == 0: none yet
 	// reductions
 	double *partials = nullptr;  // max(field-sum partials, kcap * gather blocks)
 	size_t npartials = 0;
@@ -603,7 +606,7 @@ static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
 	// twiddles + lines (+ the block's D rows for the register variant, which always reserves both planes' lines)
 	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride_lm(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
-	return (size_t)(gY + 3 * p3_groups(reduced, c.p3x) * c.LP * rowstride_lm(gY)) * es;
+	return (size_t)((c.p3g ? 0 : gY) + 3 * p3_groups(reduced, c.p3x) * c.LP * rowstride_lm(gY)) * es;
 }
 static constexpr size_t kLdsMax = 96 * 1024;
 
@@ -615,6 +618,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 		if (!k.k[MV_P3R][es == 8] || 2 * c.LP * (op->gY / c.qy.rmax) > 256) return false;
 	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
+	if (c.p3g && (c.p3r || !mv_kernels(c.qy).k[MV_P3G][es == 8])) return false;  // compile-time plans only
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
 	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
 		3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 && ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax &&
@@ -665,13 +669,24 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
-	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ;
-	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : MV_P3][sizeof(T) == 4];
+	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, f = sizeof(T) == 4;
+	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? MV_P3G : MV_P3][f];
+	const cx<T> *tw = tw_of<T>(op->twy);
+	if (c.p3g) {  // the staged table of this plan, written once on the operator's stream
+		const FftPlan2 &h = op->twys_plan[f];
+		if (h.n != c.qy.n || h.nst != c.qy.nst || memcmp(h.rad, c.qy.rad, sizeof h.rad)) {
+			using StFn = void (*)(cx<T> *, const cx<T> *);
+			hipLaunchKernelGGL((StFn)c.ky.k[MV_TWST][f], dim3(1), dim3(256), 0, op->stream,
+				const_cast<cx<T> *>(tw_of<T>(op->twys)), tw);
+			op->twys_plan[f] = c.qy;
+		}
+		tw = tw_of<T>(op->twys);
+	}
 	// k_p3: 3 lines per (plane, column) group, LP column units per block; k_p3r: LP columns, 2*LP groups
 	const int nthr = (c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x)) * c.LP * (op->gY / c.qy.rmax);
 	const int ntile = c.p3r ? op->gX / c.LP : (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
 	hipLaunchKernelGGL(k, dim3(nz3 * ntile * K), dim3(nthr),
-		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw_of<T>(op->twy),
+		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw,
 		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
 		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
 }
@@ -1396,6 +1411,8 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	if (!cfg_ok(op, op->cfg[ADDA_F64], 16) || !cfg_ok(op, op->cfg[ADDA_F32], 8))
 		return ERR("grid %dx%dx%d too large for the tiles of this build", gX, gY, gZ);
 	if (upload_twiddles(gX, op->twx) || upload_twiddles(gY, op->twy) || upload_twiddles(gZ, op->twz)) return -1;
+	if (hipMalloc(&op->twys.d, (size_t)gY * sizeof(cx<double>)) || hipMalloc(&op->twys.f, (size_t)gY * sizeof(cx<float>)))
+		return ERR("out of device memory (twiddle tables)");
 	// geometry: cell map and materials
 	const size_t nbox = (size_t)bX * bY * bZ;
 	std::vector<int> cm(nbox, -1);
@@ -1502,7 +1519,7 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
 	void *ptrs[] = {op->partials2, op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
 		op->D32, op->Xa, op->Xb, op->partials, op->red, op->coef, op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d,
-		op->twz.f};
+		op->twz.f, op->twys.d, op->twys.f};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
 	if (op->red_host) (void)hipHostFree(op->red_host);
@@ -2155,8 +2172,8 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
-		c.p3x ? "" : " XP=0");
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "");
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2169,9 +2186,10 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.LY = atoi(ly + 3);
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
-	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP=");
+	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	c.p3x = xp ? atoi(xp + 3) : 1;
+	c.p3g = tg ? atoi(tg + 3) : 0;
 	return true;
 }
 
@@ -2328,18 +2346,20 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 	for (auto &q : py)
 		for (int LP : {1, 2, 3, 4, 6, 8})
 		for (int xp : {1, 0})
-		for (int r3 : {0, 1}) {
+		for (int r3 : {0, 1})
+		for (int tg : {0, 1}) {
 			if (!xp && !op->reduced) continue;  // without the reduced form every block holds single columns
 			MvCfg c = best;
 			c.qy = q;
 			c.LP = LP;
 			c.p3r = r3;
 			c.p3x = xp;
+			c.p3g = tg;
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
-			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; }
+			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; }
 		}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);

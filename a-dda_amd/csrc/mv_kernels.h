@@ -343,8 +343,11 @@ __host__ __device__ __forceinline__ int p3_units(int gX, int DsX, int reduced, i
 }
 __host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { return reduced ? (xpair ? 4 : 2) : 1; }
 
-template <typename T, class PL>
-__global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
+// TWG (compile-time plans, kernel k_p3g): twg is the plan's staged twiddle table in global memory (k_stage_tw), read
+// through the cache instead of being copied into LDS. That frees N elements of LDS per block, which at 256-point lines
+// lets one more block share a CU.
+template <typename T, class PL, bool TWG>
+__device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
 	int transposed, Bq bq)
 {
@@ -353,8 +356,18 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
 	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP;
-	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
-	PL::template load_tw<T>(tws, twg, N);
+	const cx<T> *tws;
+	cx<T> *buf;
+	if constexpr (TWG) {
+		tws = twg;
+		buf = reinterpret_cast<cx<T> *>(smem_raw);
+	}
+	else {
+		cx<T> *tl = reinterpret_cast<cx<T> *>(smem_raw);
+		PL::template load_tw<T>(tl, twg, N);
+		tws = tl;
+		buf = tl + N;
+	}
 	const int nu = p3_units(gX, DsX, reduced, xpair), nut = (nu + LP - 1) / LP;
 	const int z = tile / nut, u0 = (tile % nut) * LP;
 	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; mirror planes and columns are handled alongside
@@ -377,6 +390,23 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	else
 		p3_body<T, PL, false>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
 			line, t, act);
+}
+
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
+	int transposed, Bq bq)
+{
+	p3_kernel<T, PL, false>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
+}
+// Register budget of five waves per SIMD: the twiddle loads from global memory would otherwise be hoisted into
+// registers, costing the occupancy that the LDS saving buys.
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) k_p3g(cx<T> *Xb,
+	const cx<T> *__restrict__ D, FftPlan2 p, const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY,
+	int DsZ, int LP, int reduced, int xpair, int transposed, Bq bq)
+{
+	p3_kernel<T, PL, true>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
 }
 
 // D-matrix product at frequency y of one grid line of plane zz, for a block at plane z (src/matvec.c:270-305:
@@ -604,6 +634,14 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	}
 }
 
+// The staged twiddle table of a compile-time plan (Fix::load_tw's layout) written to global memory, for k_p3<..., true>.
+// One block.
+template <typename T, class PL>
+__global__ void __launch_bounds__(256) k_stage_tw(cx<T> *out, const cx<T> *__restrict__ twg)
+{
+	PL::template load_tw<T>(out, twg, PL::N);
+}
+
 // The register D-product variant exists for compile-time power-of-two plans whose three-component register file
 // stays within 48 complex values per thread (FP32 up to radix 16, FP64 up to radix 8).
 template <class PL> struct is_fix : std::false_type {};
@@ -616,9 +654,11 @@ template <typename T, class PL> constexpr bool p3r_ok()
 
 // The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
 // MV_P1W / MV_P5W / MV_P5DW: FP64 dipole vectors around the FP32 transform (FP32 slot only).
+// MV_P3G: P3 with its twiddles read from global memory; MV_TWST: the kernel that stages that table (compile-time
+// plans only).
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
-	MV_P5DW = 9, MV_KINDS = 10
+	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_KINDS = 12
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -636,6 +676,11 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
+	m.k[MV_P3G][f] = m.k[MV_TWST][f] = nullptr;
+	if constexpr (is_fix<PL>::value) {
+		m.k[MV_P3G][f] = (const void *)&k_p3g<T, PL>;
+		m.k[MV_TWST][f] = (const void *)&k_stage_tw<T, PL>;
+	}
 	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = nullptr;
 	if constexpr (sizeof(T) == 4) {
 		m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
@@ -668,6 +713,9 @@ template <typename T, class PL> hipError_t mv_lds_t()
 		kMvLdsMax))) return e;
 	if constexpr (p3r_ok<T, PL>())
 		if ((e = hipFuncSetAttribute((const void *)k_p3r<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+			kMvLdsMax))) return e;
+	if constexpr (is_fix<PL>::value)
+		if ((e = hipFuncSetAttribute((const void *)k_p3g<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
 			kMvLdsMax))) return e;
 	if constexpr (sizeof(T) == 4) {
 		if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL, double>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -238,3 +238,10 @@ def test_matvec_compiled_plans(hipdev, axis, n):
                 assert op.get_config(prec) == " ".join(v)
                 y, _ = op.matvec(x, prec=prec)
                 assert _rel(y, y_or) < (1e-12 if prec == hipdev.F64 else 2e-5), (prec, v)
+                if axis == 1 and sched != COMPILED[n][-1] and "R3=1" not in v:
+                    # P3 reading the staged twiddles from global memory: the same table values, so the same bits
+                    vg = v + ["TG=1"]
+                    op.set_config(" ".join(vg), prec)
+                    assert op.get_config(prec) == " ".join(vg)
+                    yg, _ = op.matvec(x, prec=prec)
+                    assert np.array_equal(yg, y), (prec, vg)

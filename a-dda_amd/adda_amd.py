@@ -303,6 +303,7 @@ class Operator:
 # C host layer (include/adda.h)
 
 IT_QMR, IT_BICGSTAB, IT_CGNR, IT_BICG, IT_CSYM, IT_BCGS2, IT_QMR2 = 0, 1, 2, 3, 4, 5, 6
+CHP_NONE, CHP_NORMAL, CHP_REGULAR, CHP_ALWAYS = 0, 1, 2, 3
 ITER_NAMES = {"qmr": IT_QMR, "bicgstab": IT_BICGSTAB, "cgnr": IT_CGNR, "bicg": IT_BICG, "csym": IT_CSYM,
               "bcgs2": IT_BCGS2, "qmr2": IT_QMR2}
 IF_AUTO, IF_ZERO, IF_INC = 0, 1, 2
@@ -320,7 +321,47 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("eps", ctypes.c_double), ("maxiter", ctypes.c_int),
                 ("init_field", ctypes.c_int), ("recalc_resid", ctypes.c_int), ("precision", ctypes.c_int),
                 ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int), ("graph", ctypes.c_int)]
+                ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int), ("graph", ctypes.c_int),
+                ("chp_type", ctypes.c_int), ("chp_time", ctypes.c_double), ("chp_dir", ctypes.c_char_p),
+                ("chp_load", ctypes.c_int), ("run_dir", ctypes.c_char_p)]
+
+
+class ChpHead(ctypes.Structure):
+    _fields_ = [("niter", ctypes.c_int), ("counter", ctypes.c_int), ("inprodR", ctypes.c_double),
+                ("prev_err", ctypes.c_double), ("resid_scale", ctypes.c_double)]
+
+
+def chp_layout(method):
+    """(index in the reference's solver table, own scalars in doubles, own vectors) of an iterative method."""
+    v = [ctypes.c_int() for _ in range(3)]
+    hcheck(host().adda_chp_layout(ITER_NAMES.get(method, method), *[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def chp_read(chp_dir, method, nrows):
+    """Reads <chp_dir>/chp.0 (ADDA's iteration checkpoint): head dict, scalars (doubles), vectors [3+n][nrows]."""
+    _, ns, nv = chp_layout(method)
+    h = ChpHead()
+    sc = np.zeros(max(ns, 1))
+    vecs = np.zeros((3 + nv, nrows), np.complex128)
+    ptrs = (ctypes.c_void_p * (3 + nv))(*[v.ctypes.data for v in vecs])
+    hcheck(host().adda_chp_read(chp_dir.encode(), ITER_NAMES.get(method, method), nrows, ctypes.byref(h), _p(sc),
+                                ctypes.cast(ptrs, ctypes.c_void_p)))
+    head = {f: getattr(h, f) for f, _ in ChpHead._fields_}
+    return head, sc[:ns], vecs
+
+
+def chp_write(chp_dir, method, head, scal, vecs, run_dir=None):
+    """Writes <chp_dir>/chp.0 and chp.log in ADDA's format (the inverse of chp_read)."""
+    _, ns, nv = chp_layout(method)
+    vecs = [np.ascontiguousarray(v, np.complex128) for v in vecs]
+    assert len(vecs) == 3 + nv and len(scal) == ns
+    h = ChpHead(**head)
+    sc = np.ascontiguousarray(np.r_[scal, 0.0], np.float64)
+    ptrs = (ctypes.c_void_p * (3 + nv))(*[v.ctypes.data for v in vecs])
+    hcheck(host().adda_chp_write(chp_dir.encode(), run_dir.encode() if run_dir else None,
+                                 ITER_NAMES.get(method, method), len(vecs[0]), ctypes.byref(h), _p(sc),
+                                 ctypes.cast(ptrs, ctypes.c_void_p)))
 
 
 class SolveStats(ctypes.Structure):
@@ -354,6 +395,9 @@ def host():
     L.adda_couple_constant.restype = Cplx
     L.adda_generate_b.argtypes = [P, P, D, D, P, P, P]
     L.adda_iterative_solver.argtypes = [P, ctypes.POINTER(SolverOpts), P, P, P, P, I, ctypes.POINTER(SolveStats)]
+    L.adda_chp_layout.argtypes = [I, P, P, P]
+    L.adda_chp_write.argtypes = [ctypes.c_char_p, ctypes.c_char_p, I, ctypes.c_size_t, ctypes.POINTER(ChpHead), P, P]
+    L.adda_chp_read.argtypes = [ctypes.c_char_p, I, ctypes.c_size_t, ctypes.POINTER(ChpHead), P, P]
     L.adda_iterative_solver_batch.argtypes = [P, ctypes.POINTER(SolverOpts), I, P, P, P, P, I, P]
     L.adda_couple_constant_deps.argtypes = [Cplx, I, D, D, P, P]
     L.adda_couple_constant_deps.restype = Cplx
@@ -457,11 +501,13 @@ class Problem:
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
               max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0,
-              graph=False):
+              graph=False, chp_type=CHP_NONE, chp_time=0.0, chp_dir=None, chp_load=False):
+        """IterativeSolver; chp_*: ADDA's iteration checkpoints (FP64, host-form recurrences), see include/adda.h."""
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
-                       inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner), int(graph))
+                       inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner), int(graph),
+                       chp_type, chp_time, chp_dir.encode() if chp_dir else None, int(chp_load), None)
         st = SolveStats()
         hist = np.zeros(hist_len)
         rc = host().adda_iterative_solver(self.op.h, ctypes.byref(o), self.einc, self.x, self.pvec, _p(hist),

@@ -5,7 +5,10 @@
  *            [-eps <k>] [-maxiter <n>] [-init_field auto|zero|inc] [-recalc_resid] [-pol ldr|cm|rrc]
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
  *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
- *            [-store_dip_pol] [-store_int_field]
+ *            [-store_dip_pol] [-store_int_field] [-chpoint <time>] [-chp_type normal|regular|always]
+ *            [-chp_dir <d>] [-chp_load]
+ *
+ * Checkpoints (ADDA's chp.0 format, interchangeable with the reference's) cover the first polarization's solve.
  *
  * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
  * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
@@ -34,6 +37,24 @@ static void die(const char *what)
 {
 	fprintf(stderr, "ERROR: %s: %s\n", what, adda_last_error()[0] ? adda_last_error() : adda_hip_last_error());
 	exit(EXIT_FAILURE);
+}
+
+/* seconds from "%d[d,D[%d]][h,H[%d]][m,M[%d]][s,S]" (ScanTime, src/param.c:905-924) */
+static double scan_time(const char *str)
+{
+	int t[4], n;
+	char c[4] = {0, 0, 0, 0};
+	n = sscanf(str, "%d%c%d%c%d%c%d%c", t, c, t + 1, c + 1, t + 2, c + 2, t + 3, c + 3);
+	if (n < 1) { fprintf(stderr, "ERROR: Wrong time format '%s'\n", str); exit(EXIT_FAILURE); }
+	double tim = 0;
+	for (int i = 0; n > 0; n -= 2, i++) {
+		const char u = c[i];
+		const double f = u == 'd' || u == 'D' ? 86400 : u == 'h' || u == 'H' ? 3600 : u == 'm' || u == 'M' ? 60 :
+			u == 's' || u == 'S' || u == 0 ? 1 : -1;
+		if (f < 0) { fprintf(stderr, "ERROR: Illegal time format specifier (%c)\n", u); exit(EXIT_FAILURE); }
+		tim += t[i] * f;
+	}
+	return tim;
 }
 
 static void usage(void)
@@ -95,7 +116,9 @@ int main(int argc, char **argv)
 	double orient[3] = {0, 0, 0};
 	adda_cplx m[16];
 	int nm = 0;
-	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1, 0, 0, 0};
+	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1, 0, 0, 0,
+		ADDA_CHP_NONE, 0, "chpoint", 0, NULL};
+	int chp_time_set = 0;
 	int polrel = ADDA_POL_LDR;
 	for (int i = 1; i < argc; i++) {
 		const char *a = argv[i];
@@ -157,6 +180,21 @@ int main(int argc, char **argv)
 		else if (!strcmp(a, "-host_scalars")) opt.host_scalars = 1;
 		else if (!strcmp(a, "-max_inner")) { NEED(1); opt.max_inner = atoi(argv[++i]); }
 		else if (!strcmp(a, "-graph")) opt.graph = 1;
+		else if (!strcmp(a, "-chpoint")) { /* time "%d[d[%d]][h[%d]][m[%d]][s]", src/param.c:905-924,1009-1017 */
+			NEED(1);
+			opt.chp_time = scan_time(argv[++i]);
+			chp_time_set = opt.chp_time > 0;
+		}
+		else if (!strcmp(a, "-chp_type")) {
+			NEED(1);
+			const char *s = argv[++i];
+			if (!strcmp(s, "normal")) opt.chp_type = ADDA_CHP_NORMAL;
+			else if (!strcmp(s, "regular")) opt.chp_type = ADDA_CHP_REGULAR;
+			else if (!strcmp(s, "always")) opt.chp_type = ADDA_CHP_ALWAYS;
+			else usage();
+		}
+		else if (!strcmp(a, "-chp_dir")) { NEED(1); opt.chp_dir = argv[++i]; }
+		else if (!strcmp(a, "-chp_load")) opt.chp_load = 1;
 		else if (!strcmp(a, "-store_dip_pol")) store_dip_pol = 1;
 		else if (!strcmp(a, "-store_int_field")) store_int_field = 1;
 		else if (!strcmp(a, "-prec")) {
@@ -168,6 +206,15 @@ int main(int argc, char **argv)
 		else { fprintf(stderr, "ERROR: unknown option %s\n", a); usage(); }
 	}
 	if (nm == 0) { m[0].re = 1.5; m[0].im = 0; nm = 1; } /* default refractive index, src/param.c:1852 */
+	/* -chpoint without -chp_type: normal for a positive time, always for zero (src/param.c:1011-1016) */
+	for (int i = 1; i < argc; i++)
+		if (!strcmp(argv[i], "-chpoint") && opt.chp_type == ADDA_CHP_NONE)
+			opt.chp_type = chp_time_set ? ADDA_CHP_NORMAL : ADDA_CHP_ALWAYS;
+	if (opt.chp_type != ADDA_CHP_NONE && opt.chp_type != ADDA_CHP_ALWAYS && !chp_time_set) {
+		fprintf(stderr, "ERROR: You must specify time for this checkpoint type\n"); /* src/param.c:2058-2059 */
+		return EXIT_FAILURE;
+	}
+	opt.run_dir = dir;
 	opt.eps = pow(10, -eps_k);
 	if (eps_inner_k > 0) opt.inner_eps = pow(10, -eps_inner_k);
 	double t0 = now();
@@ -258,11 +305,21 @@ int main(int argc, char **argv)
 		double hist[4096];
 		adda_solve_stats st;
 		double ts = now();
-		int rc = adda_iterative_solver(op, &opt, einc, x, pv, hist, 4096, &st);
+		/* checkpoints cover the solve of the first polarization (the reference allows them only for runs that solve
+		 * one, src/param.c:2173-2174) */
+		adda_solver_opts o = opt;
+		if (pol == 1) { o.chp_type = ADDA_CHP_NONE; o.chp_load = 0; }
+		int rc = adda_iterative_solver(op, &o, einc, x, pv, hist, 4096, &st);
 		ts = now() - ts;
 		t_solve += ts;
 		if (rc < 0) die("IterativeSolver");
 		if (rc == 1) fprintf(stderr, "WARNING: iterations haven't converged in %d iterations\n", maxiter);
+		if (o.chp_type != ADDA_CHP_NONE) fprintf(log, "Checkpoint (iteration) saved\n");
+		if (rc == 2) { /* stopped after saving a checkpoint (CHP_EXIT, src/ADDAmain.c) */
+			printf("Checkpoint (iteration) saved after %d iterations; the run stops here\n", st.niter);
+			fclose(log);
+			return 0;
+		}
 		for (int i = 0; i <= st.niter && i < 4096; i++) fprintf(log, "RE_%03d = %.10E\n", i, hist[i]);
 		total_mv += st.nmatvec + st.nmatvec_fp64;
 		total_iter += st.niter;

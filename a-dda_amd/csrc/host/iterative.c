@@ -53,12 +53,42 @@ static int dotp(kctx *k, const void *a, const void *b, int mode, dcx *res, doubl
 
 static void *kvec(kctx *k) { return adda_hip_vec_alloc(k->op, k->prec); }
 
+/* Moves a checkpoint's vectors (xvec, rvec, pvec and the method's own, dv[0..]) between the device and the file
+ * (adda_chp_write / adda_chp_read, SaveIterChpoint / LoadIterChpoint of src/iterative.c:165-278); FP64 only. */
+static int chp_move(kctx *k, const adda_solver_opts *o, int method, int save, adda_chp_head *h, double *sc,
+	void *const *dv)
+{
+	int nv, rc = 0;
+	const size_t nrows = adda_hip_nrows(k->op);
+	const char *dir = o->chp_dir ? o->chp_dir : "chpoint";
+	if (adda_chp_layout(method, NULL, NULL, &nv)) return -1;
+	adda_cplx *hv[7] = {0};
+	for (int i = 0; i < 3 + nv && rc == 0; i++)
+		if (!(hv[i] = malloc(nrows * sizeof(adda_cplx)))) rc = host_err(__FILE__, __LINE__, "out of host memory");
+		else if (save && adda_hip_vec_download(k->op, ADDA_F64, hv[i], dv[i])) rc = host_err_hip(__FILE__, __LINE__);
+	if (rc == 0) {
+		if (save) rc = adda_chp_write(dir, o->run_dir, method, nrows, h, sc, (const adda_cplx *const *)hv);
+		else if ((rc = adda_chp_read(dir, method, nrows, h, sc, hv)) == 0)
+			for (int i = 0; i < 3 + nv && rc == 0; i++)
+				if (adda_hip_vec_upload(k->op, ADDA_F64, dv[i], hv[i])) rc = host_err_hip(__FILE__, __LINE__);
+	}
+	for (int i = 0; i < 7; i++) free(hv[i]);
+	return rc;
+}
+
+static double wall_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
 /* One Krylov solve of A.x = b in precision k->prec (IterativeSolver body, src/iterative.c:1398-1530).
  * b: right-hand side (already S.Einc); x: in/out (initial field per init_field); on return *niter_out iterations.
  * resid_scale/epsB follow src/iterative.c:1423-1425. The per-method bodies restate src/iterative.c:342-1143
  * (BCGS2, BiCG_CS, BiCGStab, CGNR, CSYM, QMR_CS, QMR_CS_2). */
 static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, int verbose, const void *b, void *x,
-	double *hist, int hist_len, int *niter_out, double *final_rel)
+	double *hist, int hist_len, int *niter_out, double *final_rel, const adda_solver_opts *chp)
 {
 	const int mc_tab[7] = {50000, 30000, 10, 50000, 10, 15000, 50000}; /* params[].mc, src/iterative.c:118-132 */
 	int mc = mc_tab[method], rc = 0;
@@ -74,8 +104,10 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 	if (!(temp > 0)) { rc = host_err(__FILE__, __LINE__, "right-hand side is zero"); goto done; }
 	double resid_scale = 1 / temp, epsB = eps * eps * temp, inprodR = 0, inprodRp1 = 0;
 	int matvec_ready = 0;
-	/* CalcInitField, src/iterative.c:1342-1394 */
-	if (init_field == ADDA_IF_AUTO) {
+	const int load = chp && chp->chp_load;
+	/* CalcInitField, src/iterative.c:1342-1394 (skipped when resuming from a checkpoint, src/iterative.c:1421) */
+	if (load) {}
+	else if (init_field == ADDA_IF_AUTO) {
 		CKG(mv(k, b, Av, NULL, 0));
 		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
 		if (temp < inprodR) {
@@ -101,9 +133,9 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		CKG(lc(k, r, b, 1, Av, -1, NULL, 0, 0, &inprodR));
 	}
 	double prev_err = sqrt(resid_scale * inprodR);
-	if (hist && hist_len > 0) hist[0] = prev_err;
-	if (verbose) printf("RE_000 = %.10E\n", prev_err);
-	int niter = 1, counter = 0;
+	if (hist && hist_len > 0 && !load) hist[0] = prev_err;
+	if (verbose && !load) printf("RE_000 = %.10E\n", prev_err);
+	int niter = 1, counter = 0, chp_exit = 0;
 
 	double c_old = 0, c_new = 0, omega_old = 0, omega_new = 0, ro_old_d = 0, beta_d = 0;
 	dcx beta = 0, tautilda = 0, s_new = 0, s_old = 0, ro_old = 0, omega = 0, alpha = 0, tau = 0;
@@ -116,8 +148,69 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 	/* QMR_CS_2 state (src/iterative.c:1050-1053): v = v1, d = v2, p_k = p */
 	double q2_ro = 0, q2_c = 0, q2_theta = 0;
 	dcx q2_eps = 0, q2_eta = 0;
+	/* The checkpoint state of each method: its scalars (PHASE_VARS order, a complex as two doubles) and the physical
+	 * buffers xvec, rvec, pvec, vec1.. of the reference (src/iterative.c:365-1068) */
+	double sc[20];
+	void *dv[7] = {x, r, p, NULL, NULL, NULL, NULL};
+	if (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICGSTAB) { dv[3] = v1; dv[4] = v2; dv[5] = v3; }
+	else if (method == ADDA_IT_CSYM || method == ADDA_IT_QMR_CS_2) { dv[3] = v1; dv[4] = v2; }
+	else if (method == ADDA_IT_BCGS2) dv[3] = v2;
+	double *sd[10] = {0};  /* real scalars, then complex ones */
+	dcx *sz[4] = {0};
+	int nsd = 0, nsz = 0;
+	switch (method) {
+		case ADDA_IT_QMR_CS:
+			sd[0] = &omega_old; sd[1] = &omega_new; sd[2] = &c_old; sd[3] = &c_new; nsd = 4;
+			sz[0] = &beta; sz[1] = &tautilda; sz[2] = &s_old; sz[3] = &s_new; nsz = 4;
+			break;
+		case ADDA_IT_BICGSTAB: sz[0] = &ro_old; sz[1] = &omega; sz[2] = &alpha; nsz = 3; break;
+		case ADDA_IT_CGNR: sd[0] = &ro_old_d; nsd = 1; break;
+		case ADDA_IT_BICG_CS: sz[0] = &ro_old; nsz = 1; break;
+		case ADDA_IT_CSYM:
+			sd[0] = &beta_d; sd[1] = &c_old; sd[2] = &c_new; nsd = 3;
+			sz[0] = &tau; sz[1] = &s_old; sz[2] = &s_new; nsz = 3;
+			break;
+		case ADDA_IT_BCGS2: sz[0] = &rho0; sz[1] = &alpha; nsz = 2; break;
+		default: sd[0] = &q2_c; sd[1] = &q2_theta; sd[2] = &q2_ro; nsd = 3; sz[0] = &q2_eps; sz[1] = &q2_eta; nsz = 2;
+	}
+	adda_chp_head ch;
+	if (load) { /* LoadIterChpoint, src/iterative.c:226-278 */
+		CKG(chp_move(k, chp, method, 0, &ch, sc, dv));
+		niter = ch.niter;
+		counter = ch.counter;
+		inprodR = ch.inprodR;
+		prev_err = ch.prev_err;
+		resid_scale = ch.resid_scale;
+		epsB = eps * eps / resid_scale;
+		for (int i = 0; i < nsd; i++) *sd[i] = sc[i];
+		for (int i = 0; i < nsz; i++) *sz[i] = sc[nsd + 2 * i] + I * sc[nsd + 2 * i + 1];
+		if (verbose) printf("Checkpoint (iteration) loaded\n");
+		/* the renamed buffers of QMR_CS and CSYM by the parity of the iteration count (src/iterative.c:941-944, 817-820) */
+		if (method == ADDA_IT_QMR_CS) {
+			void *t;
+			if (niter % 2 == 0) { t = v; v = vtilda; vtilda = t; }
+			else { t = p_old; p_old = p_new; p_new = t; }
+		}
+		else if (method == ADDA_IT_CSYM) {
+			void *t;
+			if (niter % 2 == 0) { t = q_old; q_old = q_new; q_new = t; }
+			else { t = pc_old; pc_old = pc_new; pc_new = t; }
+		}
+	}
+	const double t_start = wall_s();
+	double t_last = t_start;
+	/* SaveIterChpoint, src/iterative.c:165-222 */
+#define CHP_SAVE() do { \
+		for (int i_ = 0; i_ < nsd; i_++) sc[i_] = *sd[i_]; \
+		for (int i_ = 0; i_ < nsz; i_++) { sc[nsd + 2 * i_] = creal(*sz[i_]); sc[nsd + 2 * i_ + 1] = cimag(*sz[i_]); } \
+		ch.niter = niter; ch.counter = counter; ch.inprodR = inprodR; ch.prev_err = prev_err; \
+		ch.resid_scale = resid_scale; \
+		CKG(chp_move(k, chp, method, 1, &ch, sc, dv)); \
+		if (verbose) printf("Checkpoint (iteration) saved\n"); \
+	} while (0)
 	/* PHASE_INIT */
-	if (method == ADDA_IT_QMR_CS) {
+	if (load) {}
+	else if (method == ADDA_IT_QMR_CS) {
 		dcx t;
 		CKG(dotp(k, r, NULL, ADDA_DOT_SELF_BILIN, &t, NULL));
 		beta = csqrt(t);
@@ -148,7 +241,7 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		c_old = 0;
 		s_new = s_old = 0;
 	}
-	while (inprodR > epsB && niter <= maxiter && counter <= mc) {
+	while (inprodR > epsB && niter <= maxiter && counter <= mc && !chp_exit) {
 		if (method == ADDA_IT_QMR_CS) { /* src/iterative.c:962-1030 */
 			double dtmp1 = 1 / (omega_new * omega_new), dtmp2, zetaabs;
 			dcx theta, eta, zetatilda, zeta, temp1, temp2, temp4;
@@ -394,13 +487,22 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		prev_err = err;
 		if (hist && niter < hist_len) hist[niter] = err;
 		niter++;
+		/* checkpoint on time (src/iterative.c:307-317): normal and always stop after it, regular goes on */
+		if (chp && chp->chp_type != ADDA_CHP_NONE && chp->chp_time > 0 && wall_s() - t_last > chp->chp_time) {
+			CHP_SAVE();
+			t_last = wall_s();
+			if (chp->chp_type != ADDA_CHP_REGULAR) chp_exit = 1;
+		}
 	}
-	if (rc == 0 && inprodR > epsB) {
+	if (rc == 0 && chp && chp->chp_type == ADDA_CHP_ALWAYS && !chp_exit) CHP_SAVE(); /* src/iterative.c:1496 */
+	if (rc == 0 && chp_exit) rc = 2;
+	else if (rc == 0 && inprodR > epsB) {
 		if (niter > maxiter) rc = 1; /* warning only, src/iterative.c:1506-1508 */
 		else if (counter > mc) rc = host_err(__FILE__, __LINE__, "Residual norm haven't decreased for maximum allowed number of iterations (%d)", mc) - 1;
 	}
 	*niter_out = niter - 1;
 	*final_rel = sqrt(resid_scale * inprodR);
+#undef CHP_SAVE
 done:
 	{
 		void *vs[] = {r, Av, p, v1, v2, v3, v4};
@@ -662,7 +764,7 @@ done:
 
 static int device_method(const adda_solver_opts *opt)
 {
-	return !opt->host_scalars &&
+	return !opt->host_scalars && opt->chp_type == ADDA_CHP_NONE && !opt->chp_load &&
 		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR);
 }
 
@@ -676,7 +778,9 @@ static int krylov_run(adda_hip_op *op, const adda_solver_opts *opt, int prec, in
 	kctx k = {op, prec, 0};
 	int ni = 0;
 	double rel = 0;
-	int rc = krylov(&k, opt->method, eps[0], maxiter, init_field, verbose & 1, b, x, hist, hist_len, &ni, &rel);
+	const int chp = opt->chp_type != ADDA_CHP_NONE || opt->chp_load;
+	int rc = krylov(&k, opt->method, eps[0], maxiter, init_field, verbose & 1, b, x, hist, hist_len, &ni, &rel,
+		chp && prec == ADDA_F64 ? opt : NULL);
 	res[0].rc = rc;
 	res[0].niter = ni;
 	res[0].nmv = k.nmv;
@@ -694,6 +798,8 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		st[k].true_resid = -1;
 	}
 	if (opt->method < 0 || opt->method > 6) return host_err(__FILE__, __LINE__, "unknown iterative method %d", opt->method);
+	if ((opt->chp_type != ADDA_CHP_NONE || opt->chp_load) && (K != 1 || opt->precision != ADDA_PREC_FP64))
+		return host_err(__FILE__, __LINE__, "checkpoints are supported for FP64 solves of one right-hand side");
 	if (K > 1 && !device_method(opt))
 		return host_err(__FILE__, __LINE__, "batched solves need the device-resident recurrences (QMR_CS, BiCGStab, CGNR)");
 	int rc = 0;

@@ -74,10 +74,22 @@ typedef struct {
 	int max_inner;       /* mixed: iteration cap of one FP32 inner solve before the FP64 correction (0 = no cap) */
 	int graph;           /* device-resident recurrences: record the steady-state chunks of iterations once as a HIP
 	                      * graph and replay it (1), or enqueue every iteration (0) */
+	/* iteration checkpoints (-chpoint, -chp_type, -chp_dir, -chp_load; src/param.c:412-417, src/iterative.c:165-318);
+	 * FP64 single-RHS solves, which then run the host form of the recurrences */
+	int chp_type;        /* enum adda_chp: when to save <chp_dir>/chp.0 */
+	double chp_time;     /* -chpoint: seconds after the start of the solve (> 0) at which to save (normal: then stop,
+	                      * regular: every chp_time seconds); <= 0 with ADDA_CHP_ALWAYS: save at the end only */
+	const char *chp_dir; /* checkpoint directory (NULL: "chpoint", the reference's default) */
+	int chp_load;        /* 1: resume from <chp_dir>/chp.0 instead of starting from the initial field */
+	const char *run_dir; /* output directory named in chp.log (NULL: ".") */
 } adda_solver_opts;
 
+/* checkpoint types, ADDA's enum chpoint (src/const.h:313-318) */
+enum adda_chp { ADDA_CHP_NONE = 0, ADDA_CHP_NORMAL = 1, ADDA_CHP_REGULAR = 2, ADDA_CHP_ALWAYS = 3 };
+
 typedef struct {
-	int rc;              /* 0 ok, -1 breakdown/error, -2 stagnation, 1 maxiter reached (warning) */
+	int rc;              /* 0 ok, -1 breakdown/error, -2 stagnation, 1 maxiter reached (warning), 2 stopped after
+	                      * saving a checkpoint (CHP_EXIT of the reference, src/iterative.c:1528) */
 	int niter;           /* Krylov iterations (summed over inner solves in mixed mode) */
 	int nmatvec;         /* matvecs in the working precision of the iterations */
 	int nmatvec_fp64;    /* FP64 matvecs of the outer loop (mixed mode) */
@@ -101,6 +113,25 @@ int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const vo
  * resid_hist: [K][hist_len] or NULL; st: [K]. Returns the most severe per-RHS code. */
 int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, int K, const void *d_einc, void *d_x,
 	void *d_pvec, double *resid_hist, int hist_len, adda_solve_stats *st);
+
+/* ---- iteration checkpoint files (SaveIterChpoint / LoadIterChpoint, src/iterative.c:165-278) ----------------- */
+/* The common scalars of a checkpoint, in the order of the file. */
+typedef struct {
+	int niter;           /* the next iteration */
+	int counter;         /* iterations without residual decrease */
+	double inprodR;      /* ||r||^2 of the last accepted iteration */
+	double prev_err;     /* relative residual of the last iteration */
+	double resid_scale;  /* 1/||b||^2 */
+} adda_chp_head;
+/* For method (enum adda_iter): its index in the reference's solver table (written first), its own scalars counted in
+ * doubles (a doublecomplex counts two), its own vectors beyond xvec, rvec and pvec. */
+int adda_chp_layout(int method, int *ind_m, int *nscal, int *nvec);
+/* Write <dir>/chp.0 (creating dir) and <dir>/chp.log; vecs: xvec, rvec, pvec and the method's own vectors, each
+ * nrows values on the host. */
+int adda_chp_write(const char *dir, const char *run_dir, int method, size_t nrows, const adda_chp_head *h,
+	const double *scal, const adda_cplx *const *vecs);
+/* Read <dir>/chp.0 written for method and nrows (the reference's errors for another method, size or length). */
+int adda_chp_read(const char *dir, int method, size_t nrows, adda_chp_head *h, double *scal, adda_cplx *const *vecs);
 
 /* ---- scattering quantities (src/crosssec.c, src/CalculateE.c) ------------------------------------------------ */
 /* CalcFieldFree for ndir directions (src/crosssec.c:506-595): ebuff[3*i..3*i+2] for unit vector dirs[3*i..]. */

@@ -48,6 +48,8 @@ def geometries():
     g["box8mm"] = (pos, rng.integers(0, 3, len(pos)))
     pos, _ = make_shape("box", 8)
     g["adj8"] = (pos, adj8_materials(len(pos))[1])
+    pos, _ = make_shape("box", 4)
+    g["box4"] = (pos, None)
     return g
 
 
@@ -119,9 +121,48 @@ def run_adj(name):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+# Iteration checkpoints (SaveIterChpoint, src/iterative.c:165-220): for each solver the uninterrupted run and a run
+# stopped by -maxiter 4 whose final state -chpoint 0s saves (type "always"); -sym enf computes one polarization only,
+# which the reference requires for checkpoints (src/param.c:2173).
+CHP_METHODS = ["bcgs2", "bicg", "bicgstab", "cgnr", "csym", "qmr", "qmr2"]
+CHP_ARGS = "-sym enf -dpl 10 -m 1.5 0.1 -eps 10"
+
+
+def run_chp(name):
+    meth = name[len("chp_"):]
+    out = os.path.join(HERE, name)
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    try:
+        gfile = os.path.join(HERE, "geom", "box4.geom")
+        full = [EXE, "-shape", "read", gfile] + CHP_ARGS.split() + ["-iter", meth, "-dir", os.path.join(tmp, "full")]
+        part = [EXE, "-shape", "read", gfile] + CHP_ARGS.split() + ["-iter", meth, "-maxiter", "4", "-chpoint", "0s",
+                                                                   "-chp_dir", os.path.join(tmp, "chp"), "-dir",
+                                                                   os.path.join(tmp, "part")]
+        for c in (full, part):
+            subprocess.run(c, check=True, stdout=subprocess.DEVNULL, cwd=tmp)
+        os.makedirs(out, exist_ok=True)
+        shutil.copy(os.path.join(tmp, "full", "CrossSec-Y"), os.path.join(out, "CrossSec-Y"))
+        with open(os.path.join(tmp, "full", "log")) as fi, open(os.path.join(out, "log_RE"), "w") as fo:
+            for line in fi:
+                if line.startswith("RE_") or "matrix-vector" in line:
+                    fo.write(line)
+        shutil.copy(os.path.join(tmp, "chp", "chp.0"), os.path.join(out, "chp.0"))
+        with open(os.path.join(out, "cmd.txt"), "w") as f:
+            f.write("oracle/_ref/adda_sparse -shape read tests/golden/geom/box4.geom %s -iter %s -dir full\n"
+                    % (CHP_ARGS, meth))
+            f.write("oracle/_ref/adda_sparse -shape read tests/golden/geom/box4.geom %s -iter %s -maxiter 4 -chpoint 0s "
+                    "-chp_dir chp -dir part   (chp.0 = chp/chp.0; chp/chp.log reads 'Info about the run, which produced "
+                    "the checkpoint, can be found in ../part')\n" % (CHP_ARGS, meth))
+        return name
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def run(name):
     if name.startswith("adj8"):
         return run_adj(name)
+    if name.startswith("chp_"):
+        return run_chp(name)
     geo, args, keep = RUNS[name]
     out = os.path.join(HERE, name)
     tmp = tempfile.mkdtemp(prefix="golden_")
@@ -155,9 +196,12 @@ def main():
     for gname, (pos, mat) in geometries().items():
         p, _ = read_shape(pos)
         write_geom(os.path.join(HERE, "geom", gname + ".geom"), p, mat)
-    names = sys.argv[1:] or list(RUNS) + list(adj8_runs())
+    chp = ["chp_" + m for m in CHP_METHODS]
+    names = sys.argv[1:] or list(RUNS) + list(adj8_runs()) + chp
     if names == ["adj8"]:
         names = list(adj8_runs())
+    if names == ["chp"]:
+        names = chp
     with ThreadPoolExecutor(max_workers=6) as ex:
         for n in ex.map(run, names):
             print("done", n, flush=True)

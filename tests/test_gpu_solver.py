@@ -7,6 +7,7 @@ x within 1e-8 relative (max-norm) of the oracle's FP64 solve. Mixed precision (F
 Cext within 1e-6 relative of the FP64 value.
 """
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -287,3 +288,110 @@ def test_cli_end_to_end(hipdev, tmp_path, prec):
     ref = G.mueller("c1_qmr")
     assert mu.shape == ref.shape
     assert G.mueller_close(mu, ref, ref[:, 1])[0]
+
+
+CHP_METHODS = ["bcgs2", "bicg", "bicgstab", "cgnr", "csym", "qmr", "qmr2"]
+
+
+def _chp_problem(hipdev):
+    pr = _problem(hipdev, "box4", 1.5 + 0.1j, 10)
+    pr.set_polarization(PROP_Z, POL_Y)
+    return pr
+
+
+@pytest.mark.parametrize("method", CHP_METHODS)
+def test_chp_resume_reference_checkpoint(hipdev, method, tmp_path):
+    """The reference's checkpoint after 4 iterations (-maxiter 4 -chpoint 0s, SPARSE build, 64-dipole cube) resumed
+    on the GPU (LoadIterChpoint, src/iterative.c:226-278): the solve continues from iteration 5 and ends at the
+    reference's uninterrupted Cext, within the CrossSec tolerance, in the reference's iteration count (+-1)."""
+    shutil.copy(G.path("chp_" + method, "chp.0"), tmp_path / "chp.0")
+    nref = len([ln for ln in open(G.path("chp_" + method, "log_RE")) if ln.startswith("RE_")]) - 1
+    pr = _chp_problem(hipdev)
+    try:
+        r = pr.solve(method, eps=1e-10, chp_load=True, chp_dir=str(tmp_path))
+        assert r["rc"] == 0 and r["resid"] < 1e-10
+        assert abs(r["niter"] - nref) <= 1, (r["niter"], nref)
+        g = G.crosssec("chp_" + method, "Y")
+        assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0], (pr.cext(), g["Cext"])
+    finally:
+        pr.close()
+
+
+@pytest.mark.parametrize("method", CHP_METHODS)
+def test_chp_save_matches_reference_state(hipdev, method, tmp_path):
+    """A checkpoint saved on the GPU after 4 iterations (type always, SaveIterChpoint src/iterative.c:165-222) holds
+    the reference's state at that point: the same counters, and scalars and vectors equal to rounding (the FFT and
+    the direct-sum matvec differ by about 1e-15 per product); resumed on the GPU it reaches the reference's Cext."""
+    pr = _chp_problem(hipdev)
+    try:
+        r = pr.solve(method, eps=1e-10, maxiter=4, chp_type=hipdev.CHP_ALWAYS, chp_dir=str(tmp_path / "ours"))
+        assert r["rc"] == 1 and r["niter"] == 4  # maxiter reached: a warning, state saved
+        nrows = 3 * pr.nd
+        h1, s1, v1 = hipdev.chp_read(str(tmp_path / "ours"), method, nrows)
+        h0, s0, v0 = hipdev.chp_read(os.path.dirname(G.path("chp_" + method, "chp.0")), method, nrows)
+        assert (h1["niter"], h1["counter"]) == (h0["niter"], h0["counter"]) == (5, h0["counter"])
+        for f in ("inprodR", "prev_err", "resid_scale"):
+            assert abs(h1[f] - h0[f]) <= 1e-9 * abs(h0[f]), f
+        assert np.allclose(s1, s0, rtol=1e-8, atol=1e-10 * max(1.0, np.abs(s0).max()))
+        for a, b in zip(v1, v0):
+            assert np.abs(a - b).max() <= 1e-8 * max(np.abs(b).max(), 1e-300)
+        r = pr.solve(method, eps=1e-10, chp_load=True, chp_dir=str(tmp_path / "ours"))
+        assert r["rc"] == 0 and r["resid"] < 1e-10
+        g = G.crosssec("chp_" + method, "Y")
+        assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0], (pr.cext(), g["Cext"])
+    finally:
+        pr.close()
+
+
+def test_chp_normal_type_stops(hipdev, tmp_path):
+    """-chp_type normal with a time that has passed by the first progress check: the solve saves and stops (rc 2,
+    CHP_EXIT of src/iterative.c:1528); the saved state resumes to the converged answer."""
+    pr = _chp_problem(hipdev)
+    try:
+        r = pr.solve("qmr", eps=1e-10, chp_type=hipdev.CHP_NORMAL, chp_time=1e-9, chp_dir=str(tmp_path))
+        assert r["rc"] == 2 and r["niter"] == 1
+        h, _, _ = hipdev.chp_read(str(tmp_path), "qmr", 3 * pr.nd)
+        assert h["niter"] == 2
+        r = pr.solve("qmr", eps=1e-10, chp_load=True, chp_dir=str(tmp_path))
+        assert r["rc"] == 0
+        g = G.crosssec("chp_qmr", "Y")
+        assert G.numeric_close(pr.cext(), g["Cext"], 1e-14, 1e-8)[0]
+    finally:
+        pr.close()
+
+
+def test_chp_rejected_forms(hipdev, tmp_path):
+    """Checkpoints are for FP64 solves of one right-hand side (the mixed forms have no counterpart in the reference)."""
+    pr = _problem(hipdev, "box4", 1.5 + 0.1j, 10, fp32=True)
+    pr.set_polarization(PROP_Z, POL_Y)
+    try:
+        with pytest.raises(RuntimeError, match="checkpoints"):
+            pr.solve("qmr", eps=1e-10, precision=hipdev.PREC_MIXED, chp_type=hipdev.CHP_ALWAYS,
+                     chp_dir=str(tmp_path))
+    finally:
+        pr.close()
+
+
+def test_cli_checkpoint_resume(hipdev, tmp_path):
+    """adda_hip -chp_load -chp_dir <dir> resumes the reference's QMR_CS checkpoint (the Y polarization's solve) and
+    writes the reference's uninterrupted CrossSec-Y; -chpoint 0s saves a checkpoint that the same command resumes."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "a-dda_amd", "adda_hip")
+    shutil.copytree(os.path.dirname(G.path("chp_qmr", "chp.0")), tmp_path / "chp")
+    base = [exe, "-shape", "read", G.path("geom", "box4.geom"), "-dpl", "10", "-m", "1.5", "0.1", "-iter", "qmr",
+            "-eps", "10"]
+    subprocess.run(base + ["-chp_load", "-chp_dir", str(tmp_path / "chp"), "-dir", str(tmp_path / "cont")],
+                   check=True, capture_output=True, timeout=120)
+    g = G.crosssec("chp_qmr", "Y")
+    with open(tmp_path / "cont" / "CrossSec-Y") as f:
+        b = {l.split("=")[0].strip(): float(l.split("=")[1]) for l in f}
+    assert G.numeric_close(b["Cext"], g["Cext"], 1e-14, 1e-8)[0]
+    subprocess.run(base + ["-maxiter", "4", "-chpoint", "0s", "-chp_dir", str(tmp_path / "mine"), "-dir",
+                           str(tmp_path / "part")], check=True, capture_output=True, timeout=120)
+    with open(tmp_path / "mine" / "chp.log") as f:
+        assert f.read().endswith("../" + str(tmp_path / "part"))
+    subprocess.run(base + ["-chp_load", "-chp_dir", str(tmp_path / "mine"), "-dir", str(tmp_path / "cont2")],
+                   check=True, capture_output=True, timeout=120)
+    with open(tmp_path / "cont2" / "CrossSec-Y") as f:
+        b = {l.split("=")[0].strip(): float(l.split("=")[1]) for l in f}
+    assert G.numeric_close(b["Cext"], g["Cext"], 1e-14, 1e-8)[0]

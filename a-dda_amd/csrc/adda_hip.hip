@@ -670,7 +670,7 @@ template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, f = sizeof(T) == 4;
-	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? MV_P3G : MV_P3][f];
+	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? (c.LP == 1 ? MV_P3G1 : MV_P3G) : (c.LP == 1 ? MV_P31 : MV_P3)][f];
 	const cx<T> *tw = tw_of<T>(op->twy);
 	if (c.p3g) {  // the staged table of this plan, written once on the operator's stream
 		const FftPlan2 &h = op->twys_plan[f];

@@ -346,11 +346,14 @@ __host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { retu
 // TWG (compile-time plans, kernel k_p3g): twg is the plan's staged twiddle table in global memory (k_stage_tw), read
 // through the cache instead of being copied into LDS. That frees N elements of LDS per block, which at 256-point lines
 // lets one more block share a CU.
-template <typename T, class PL, bool TWG>
+// LPC > 0: the number of column units per block fixed at compile time (LP = 1 is what the autotuner picks at the
+// sizes measured), which turns the group offsets and the group tests of the D-matrix product into constants.
+template <typename T, class PL, bool TWG, int LPC>
 __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
+	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LPr, int reduced, int xpair,
 	int transposed, Bq bq)
 {
+	const int LP = LPC > 0 ? LPC : LPr;
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
 	Xb += kq * bq.xs;
@@ -392,21 +395,21 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 			line, t, act);
 }
 
-template <typename T, class PL>
+template <typename T, class PL, int LPC = 0>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
 	int transposed, Bq bq)
 {
-	p3_kernel<T, PL, false>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
+	p3_kernel<T, PL, false, LPC>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
 }
 // Register budget of five waves per SIMD: the twiddle loads from global memory would otherwise be hoisted into
 // registers, costing the occupancy that the LDS saving buys.
-template <typename T, class PL>
+template <typename T, class PL, int LPC = 0>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) k_p3g(cx<T> *Xb,
 	const cx<T> *__restrict__ D, FftPlan2 p, const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY,
 	int DsZ, int LP, int reduced, int xpair, int transposed, Bq bq)
 {
-	p3_kernel<T, PL, true>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
+	p3_kernel<T, PL, true, LPC>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
 }
 
 // D-matrix product at frequency y of one grid line of plane zz, for a block at plane z (src/matvec.c:270-305:
@@ -656,9 +659,10 @@ template <typename T, class PL> constexpr bool p3r_ok()
 // MV_P1W / MV_P5W / MV_P5DW: FP64 dipole vectors around the FP32 transform (FP32 slot only).
 // MV_P3G: P3 with its twiddles read from global memory; MV_TWST: the kernel that stages that table (compile-time
 // plans only).
+// MV_P31 / MV_P3G1: the same two P3 kernels compiled for one column unit per block (LP = 1).
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
-	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_KINDS = 12
+	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_KINDS = 14
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -672,13 +676,15 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_ZFWD][f] = (const void *)&k_zfft<T, false, PL>;
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
+	m.k[MV_P31][f] = (const void *)&k_p3<T, PL, 1>;
 	m.k[MV_P5][f] = (const void *)&k_p5<T, PL, false>;
 	m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
-	m.k[MV_P3G][f] = m.k[MV_TWST][f] = nullptr;
+	m.k[MV_P3G][f] = m.k[MV_TWST][f] = m.k[MV_P3G1][f] = nullptr;
 	if constexpr (is_fix<PL>::value) {
 		m.k[MV_P3G][f] = (const void *)&k_p3g<T, PL>;
+		m.k[MV_P3G1][f] = (const void *)&k_p3g<T, PL, 1>;
 		m.k[MV_TWST][f] = (const void *)&k_stage_tw<T, PL>;
 	}
 	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = nullptr;
@@ -705,6 +711,8 @@ template <typename T, class PL> hipError_t mv_lds_t()
 		return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
 		return e;
+	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		kMvLdsMax))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		kMvLdsMax))) return e;
 	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -716,7 +724,8 @@ template <typename T, class PL> hipError_t mv_lds_t()
 			kMvLdsMax))) return e;
 	if constexpr (is_fix<PL>::value)
 		if ((e = hipFuncSetAttribute((const void *)k_p3g<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			kMvLdsMax))) return e;
+			kMvLdsMax)) || (e = hipFuncSetAttribute((const void *)k_p3g<T, PL, 1>,
+			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
 	if constexpr (sizeof(T) == 4) {
 		if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
 			kMvLdsMax))) return e;

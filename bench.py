@@ -46,7 +46,12 @@ def main():
                     help="profiling mode: autotune, 20 matvecs per precision and one 52.8 MB device copy (the "
                          "calibration kernel for HBM counters), no solves; prints the matvec figures only")
     ap.add_argument("--solve-only", choices=["fp64", "mixed"],
-                    help="profiling mode: autotune, then one BiCGStab solve to 1e-10 in the given form; prints its time")
+                    help="profiling mode: autotune, then one solve (--method, to --eps) in the given form; prints its "
+                         "time")
+    ap.add_argument("--method", default="bicgstab", help="solver of --solve-only (ADDA's -iter names)")
+    ap.add_argument("--eps", type=float, default=1e-10, help="tolerance of --solve-only (default 1e-10)")
+    ap.add_argument("--m", type=float, nargs=2, default=[1.5, 0.01], metavar=("RE", "IM"),
+                    help="refractive index of the sphere (default 1.5 0.01; the 640^3 workload uses 1.33 0.001)")
     ap.add_argument("--qmr-max-inner", type=int, default=2000,
                     help="inner-iteration cap of QMR_CS with FP32 vectors (its inner solves stall near a 1e-3 true "
                          "residual while the quasi-residual keeps falling, so an uncapped step runs on without gain)")
@@ -73,7 +78,7 @@ def main():
     nd = len(pos)
     size = 80.0 * args.box / 128
     dpl = lam * (nd / (np.pi / 6)) ** (1 / 3) / size  # -size with volume correction, src/make_particle.c:2192-2196
-    m = 1.5 + 0.01j
+    m = args.m[0] + 1j * args.m[1]
     t0 = time.perf_counter()
     pr = A.Problem(pos, box, m, dpl=dpl, fp32=True)
     t_dmat = time.perf_counter() - t0
@@ -91,11 +96,15 @@ def main():
         prec = A.PREC_FP64 if args.solve_only == "fp64" else A.PREC_MIXED
         op.sync()
         t = time.perf_counter()
-        res = pr.solve("bicgstab", eps=1e-10, precision=prec)
+        # verbose bit 1: a progress line about every 10 s, so that a long solve (the 640^3 sphere) shows it is alive
+        res = pr.solve(args.method, eps=args.eps, precision=prec, verbose=2, recalc_resid=True)
+        cext = pr.cext()
         dt = time.perf_counter() - t
-        print(json.dumps({"mode": "solve-only", "form": args.solve_only, "s": round(dt, 4), "iterations": res["niter"],
-                          "matvecs": res["nmatvec"], "fp64_matvecs": res["nmatvec_fp64"], "outer": res["nouter"],
-                          "config": tune}), flush=True)
+        print(json.dumps({"mode": "solve-only", "form": args.solve_only, "method": args.method, "eps": args.eps,
+                          "m": args.m, "dipoles": nd, "grid": list(pr.grid), "s": round(dt, 4), "rc": res["rc"],
+                          "iterations": res["niter"], "matvecs": res["nmatvec"], "fp64_matvecs": res["nmatvec_fp64"],
+                          "outer": res["nouter"], "true_resid": res["true_resid"], "cext": cext, "config": tune}),
+              flush=True)
         pr.close()
         return
 

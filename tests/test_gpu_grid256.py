@@ -1,5 +1,6 @@
-"""Full-size parity of the bench workload against the oracle: sphere on a 128^3 box (1,099,136 dipoles), FFT grid
-256^3, m = 1.5+0.01i, size parameter 40 (-size 80 with the volume correction, src/make_particle.c:2192-2196).
+"""Full-size parity against the oracle of the bench workload -- sphere on a 128^3 box (1,099,136 dipoles), FFT grid
+256^3, m = 1.5+0.01i, size parameter 40 (-size 80 with the volume correction, src/make_particle.c:2192-2196) -- and
+of the 64-RHS batched workload.
 
 The oracle's FP64 restatement builds its own D-matrix (InitDmatrix, src/fft.c:918-1416) and runs its own FFT matvec
 (src/matvec.c:128-425) on one seeded argument, in about a minute of host time. Against it:
@@ -49,3 +50,49 @@ def test_grid256_dmatrix_and_matvec_vs_oracle(hipdev):
     assert abs(ip64 - ip_or) / ip_or < 1e-12
     assert _rel(y32, y_or) < 2e-5
     assert _rel(yv, y_or) < 2e-5
+
+
+def test_batched_workload_matvec_vs_oracle(hipdev):
+    """The batched workload at full size: ellipsoid box 64x52x32 (semi-axes 1:0.8:0.5, 54,928 dipoles) on the fast
+    grid 128x128x64, m = 1.33+0.001i, default dpl 10|m|, 32 seeded orientations x 2 polarizations = 64 RHS, each with
+    its own coupling constants (InitCC per orientation, src/calculator.c:266-289). One batched FP64 matvec of all 64
+    RHS; three of them against the oracle's FP64 matvec (1e-12) with the oracle's coupling constants, and the batched
+    FP32 matvec of the same RHS within 2e-5."""
+    from adda_rot import rotate_zyz
+    from cases import orientations, POL_X
+    pos, box = make_shape("ellipsoid", 64, 0.8, 0.5)
+    m = 1.33 + 0.001j
+    props, pols = [], []
+    for a, b, g in orientations(32):
+        prop, pY, pX = (rotate_zyz(a, b, g, v) for v in (PROP_Z, POL_Y, POL_X))
+        props += [prop, prop]
+        pols += [pY, pX]
+    grid = hipdev.fast_grid(box)
+    assert len(pos) == 54928 and grid == (128, 128, 64)
+    K = 64
+    bp = hipdev.BatchProblem(pos, box, m, np.array(props), np.array(pols), fp32=True, grid=grid)
+    c = Case(pos, box, m, dpl=bp.dpl, grid=grid)
+    assert abs(c.gridspace - bp.gridspace) <= 1e-15 * c.gridspace
+    rng = np.random.default_rng(64)
+    xs = [rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd) for _ in range(K)]
+    check = (0, 21, 63)
+    op = bp.op
+    try:
+        yb = {}
+        for prec in (hipdev.F64, hipdev.F32):
+            a, r = op.alloc_batch(K, prec), op.alloc_batch(K, prec)
+            for k in range(K):
+                op.upload(op.rhs(a, k, prec), xs[k], prec)
+            op.matvec_batch(a, r, K, prec)
+            yb[prec] = {k: op.download(op.rhs(r, k, prec), prec) for k in check}
+            op.free(a)
+            op.free(r)
+    finally:
+        bp.close()
+    d = c.dmatrix()
+    for k in check:
+        cc, ccs = c.cc(props[k], pols[k])
+        assert np.allclose(bp.cc_sqrt[k], np.reshape(ccs, bp.cc_sqrt[k].shape), rtol=1e-13, atol=0)
+        y_or, _ = c.matvec_fft(d, ccs, xs[k])
+        assert _rel(yb[hipdev.F64][k], y_or) < 1e-12, k
+        assert _rel(yb[hipdev.F32][k], y_or) < 2e-5, k

@@ -127,7 +127,17 @@ struct MvCfg {
 	int p3r = 0;          // P3 with the D-matrix product in registers (k_p3r; compile-time power-of-two y plans)
 	int p3x = 1;          // P3 blocks hold column x with its mirror gridX - x (reduced D-matrix; else one column each)
 	int p3g = 0;          // P3 reads the staged twiddles of its y plan from global memory (k_p3 TWG; compile-time plans)
+	int p3t = 0;          // P3 blocks get one thread per D-matrix element (LP*gridY, whole waves) when that is more
+	                      // than the transforms use; the extra threads take part in the product only
 };
+
+// threads of a P3 block (k_p3: 3 lines per (plane, column) group, LP column units; k_p3r: LP columns, 2*LP groups)
+static int p3_threads(const MvCfg &c, int gY, int reduced)
+{
+	const int t = (c.p3r ? 2 : 3 * p3_groups(reduced, c.p3x)) * c.LP * (gY / c.qy.rmax);
+	const int e = (c.LP * gY + 63) / 64 * 64;
+	return c.p3t && !c.p3r && e > t ? e : t;
+}
 
 struct TwPair {
 	cx<double> *d = nullptr;
@@ -620,6 +630,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
 	if (c.p3g && (c.p3r || !mv_kernels(c.qy).k[MV_P3G][es == 8])) return false;  // compile-time plans only
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
+	if (c.p3t && (c.p3r || p3_threads(c, op->gY, op->reduced) > 512)) return false;
 	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
 		3 * c.LY * tplx <= 512 && c.TC * tplz <= 512 && ml * c.LP * tply <= 512 && lds_p15(c, op->gX, es) <= kLdsMax &&
 		lds_p24(c, op->gZ, es) <= kLdsMax && lds_p3(c, op->gY, op->reduced, es) <= kLdsMax;
@@ -682,8 +693,7 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 		}
 		tw = tw_of<T>(op->twys);
 	}
-	// k_p3: 3 lines per (plane, column) group, LP column units per block; k_p3r: LP columns, 2*LP groups
-	const int nthr = (c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x)) * c.LP * (op->gY / c.qy.rmax);
+	const int nthr = p3_threads(c, op->gY, op->reduced);
 	const int ntile = c.p3r ? op->gX / c.LP : (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
 	hipLaunchKernelGGL(k, dim3(nz3 * ntile * K), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw,
@@ -2172,8 +2182,8 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
-		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "");
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "");
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2186,10 +2196,11 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.LY = atoi(ly + 3);
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
-	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG=");
+	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG="), *pt = strstr(s, "PT=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	c.p3x = xp ? atoi(xp + 3) : 1;
 	c.p3g = tg ? atoi(tg + 3) : 0;
+	c.p3t = pt ? atoi(pt + 3) : 0;
 	return true;
 }
 
@@ -2347,7 +2358,8 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 		for (int LP : {1, 2, 3, 4, 6, 8})
 		for (int xp : {1, 0})
 		for (int r3 : {0, 1})
-		for (int tg : {0, 1}) {
+		for (int tg : {0, 1})
+		for (int pt : {0, 1}) {
 			if (!xp && !op->reduced) continue;  // without the reduced form every block holds single columns
 			MvCfg c = best;
 			c.qy = q;
@@ -2355,11 +2367,17 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			c.p3r = r3;
 			c.p3x = xp;
 			c.p3g = tg;
+			c.p3t = pt;
+			if (pt) {  // without extra threads the launch is the same as PT=0
+				MvCfg c0 = c;
+				c0.p3t = 0;
+				if (p3_threads(c, op->gY, op->reduced) == p3_threads(c0, op->gY, op->reduced)) continue;
+			}
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
 			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
-			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; }
+			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; best.p3t = pt; }
 		}
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);

@@ -245,3 +245,12 @@ def test_matvec_compiled_plans(hipdev, axis, n):
                     assert op.get_config(prec) == " ".join(vg)
                     yg, _ = op.matvec(x, prec=prec)
                     assert np.array_equal(yg, y), (prec, vg)
+                    # one thread per D-matrix element (up to 512 per block): the same products on other threads, so
+                    # the same bits
+                    if n > 512:
+                        continue
+                    vp = v + ["PT=1"]
+                    op.set_config(" ".join(vp), prec)
+                    assert op.get_config(prec) == " ".join(vp)
+                    yp, _ = op.matvec(x, prec=prec)
+                    assert np.array_equal(yp, y), (prec, vp)

@@ -395,3 +395,21 @@ def test_cli_checkpoint_resume(hipdev, tmp_path):
     with open(tmp_path / "cont2" / "CrossSec-Y") as f:
         b = {l.split("=")[0].strip(): float(l.split("=")[1]) for l in f}
     assert G.numeric_close(b["Cext"], g["Cext"], 1e-14, 1e-8)[0]
+
+
+def test_chp_regular_type_continues(hipdev, tmp_path):
+    """-chp_type regular saves on every time check that has passed (src/iterative.c:307-317) and keeps iterating: the
+    solve converges as without checkpoints (same iterations, same Cext), and the last checkpoint on disk holds the
+    state after the last iteration. Checkpointed solves run the host form of the recurrences, so that is the form
+    compared (the device-resident one sums its reductions in another order)."""
+    pr = _chp_problem(hipdev)
+    try:
+        r0 = pr.solve("qmr", eps=1e-10, host_scalars=True)
+        c0 = pr.cext()
+        r = pr.solve("qmr", eps=1e-10, chp_type=hipdev.CHP_REGULAR, chp_time=1e-9, chp_dir=str(tmp_path))
+        assert r["rc"] == 0 and r["niter"] == r0["niter"]
+        assert pr.cext() == c0
+        h, _, _ = hipdev.chp_read(str(tmp_path), "qmr", 3 * pr.nd)
+        assert h["niter"] == r["niter"] + 1 and h["counter"] == 0
+    finally:
+        pr.close()

@@ -161,8 +161,10 @@ int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state, int niter,
 int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, void *x, void *r, void *p, void *Av);
 
 /* Matvec configuration per precision: radix schedules along x, y, z and tile sizes, as a string
- * "x=16.16 y=16.16 z=16.16 LY=4 TC=16 LP=2" (x radices: P1/P5; y: P3; z: P2/P4; LY y-rows (all three components) per P1/P5 block, TC columns
- * per P2/P4 block, LP x-pairs per P3 block). set_config validates it against the grid. */
+ * "x=16.16 y=16.16 z=16.16 LY=4 TC=16 LP=2" (x radices: P1/P5; y: P3; z: P2/P4; LY y-rows (all three components) per
+ * P1/P5 block, TC columns per P2/P4 block, LP column units per P3 block), optionally followed by the P3 options
+ * "R3=1" (D-matrix product in registers), "XP=0" (single columns instead of column pairs), "TG=1" (twiddles read
+ * from global memory) and "PT=1" (one thread per D-matrix element). set_config validates it against the grid. */
 int adda_hip_get_config(adda_hip_op *op, int prec, char *buf, size_t len);
 int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg);
 

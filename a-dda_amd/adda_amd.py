@@ -189,6 +189,15 @@ class Operator:
         check(hip().adda_hip_matvec(self.h, prec, a, r, ctypes.byref(ip) if inprod else None, int(her)))
         return ip.value if inprod else None
 
+    def matvec_host(self, x, her=False):
+        """The drop-in MatVec on host vectors (adda_hip_matvec_host: copies in and out over PCIe each call); returns
+        (y, ||y||^2)."""
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.empty_like(x)
+        ip = ctypes.c_double()
+        check(hip().adda_hip_matvec_host(self.h, _p(x), _p(y), ctypes.byref(ip), int(her)))
+        return y, ip.value
+
     def matvec(self, x, her=False, prec=F64):
         """Host convenience: y = A x (or A^H x) with the requested device precision; returns (y, ||y||^2)."""
         a, r = self.alloc(prec), self.alloc(prec)
@@ -323,7 +332,12 @@ class SolverOpts(ctypes.Structure):
                 ("inner_eps", ctypes.c_double), ("max_outer", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("host_scalars", ctypes.c_int), ("max_inner", ctypes.c_int), ("graph", ctypes.c_int),
                 ("chp_type", ctypes.c_int), ("chp_time", ctypes.c_double), ("chp_dir", ctypes.c_char_p),
-                ("chp_load", ctypes.c_int), ("run_dir", ctypes.c_char_p)]
+                ("chp_load", ctypes.c_int), ("run_dir", ctypes.c_char_p), ("inner_method", ctypes.c_int)]
+
+
+def _inner(method):
+    """SolverOpts.inner_method of a method name (None: automatic choice)."""
+    return 0 if method is None else ITER_NAMES.get(method, method) + 1
 
 
 class ChpHead(ctypes.Structure):
@@ -367,7 +381,7 @@ def chp_write(chp_dir, method, head, scal, vecs, run_dir=None):
 class SolveStats(ctypes.Structure):
     _fields_ = [("rc", ctypes.c_int), ("niter", ctypes.c_int), ("nmatvec", ctypes.c_int),
                 ("nmatvec_fp64", ctypes.c_int), ("nouter", ctypes.c_int), ("resid", ctypes.c_double),
-                ("true_resid", ctypes.c_double)]
+                ("true_resid", ctypes.c_double), ("chp_saved", ctypes.c_int)]
 
 
 _host = None
@@ -501,13 +515,15 @@ class Problem:
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
               max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0,
-              graph=False, chp_type=CHP_NONE, chp_time=0.0, chp_dir=None, chp_load=False):
-        """IterativeSolver; chp_*: ADDA's iteration checkpoints (FP64, host-form recurrences), see include/adda.h."""
+              graph=False, chp_type=CHP_NONE, chp_time=0.0, chp_dir=None, chp_load=False, inner_method=None):
+        """IterativeSolver; chp_*: ADDA's iteration checkpoints (FP64, host-form recurrences); inner_method: the
+        FP32 inner solver of the mixed forms (None: automatic), see include/adda.h."""
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
                        inner_eps, max_outer, int(verbose), int(host_scalars), int(max_inner), int(graph),
-                       chp_type, chp_time, chp_dir.encode() if chp_dir else None, int(chp_load), None)
+                       chp_type, chp_time, chp_dir.encode() if chp_dir else None, int(chp_load), None,
+                       _inner(inner_method))
         st = SolveStats()
         hist = np.zeros(hist_len)
         rc = host().adda_iterative_solver(self.op.h, ctypes.byref(o), self.einc, self.x, self.pvec, _p(hist),
@@ -518,14 +534,14 @@ class Problem:
                     resid=st.resid, true_resid=st.true_resid, hist=hist[:n].copy())
 
     def adjoint_gradient(self, method="qmr", eps=1e-10, maxiter=None, precision=PREC_FP64, inner_eps=0.0,
-                         max_outer=0, graph=True, max_inner=0):
+                         max_outer=0, graph=True, max_inner=0, inner_method=None):
         """d(Cext, Cabs)/d(eps_j) for every dipole j of the current solution (call solve() first): two adjoint
         solves batched together, then per-dipole assembly on the GPU. Returns gext, gabs [nd][2] (d/dRe eps_j,
         d/dIm eps_j) and the stats of the two adjoint solves."""
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, IF_AUTO, 0, precision, inner_eps, max_outer, 0, 0,
-                       int(max_inner), int(graph))
+                       int(max_inner), int(graph), inner_method=_inner(inner_method))
         ge, ga = np.empty(2 * self.nd), np.empty(2 * self.nd)
         st = (SolveStats * 2)()
         m = np.ascontiguousarray(self.m)
@@ -615,11 +631,11 @@ class BatchProblem:
             self.op = None
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
-              max_outer=0, recalc_resid=False, hist_len=0, graph=True, max_inner=0):
+              max_outer=0, recalc_resid=False, hist_len=0, graph=True, max_inner=0, inner_method=None):
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
-                       inner_eps, max_outer, 0, 0, int(max_inner), int(graph))
+                       inner_eps, max_outer, 0, 0, int(max_inner), int(graph), inner_method=_inner(inner_method))
         st = (SolveStats * self.K)()
         hist = np.zeros((self.K, max(hist_len, 1)))
         rc = host().adda_iterative_solver_batch(self.op.h, ctypes.byref(o), self.K, self.einc, self.x, self.pvec,

@@ -157,6 +157,7 @@ struct adda_hip_op {
 	int mv32 = 0;  // FP64 matvecs run their transforms in FP32 (adda_hip_set_matvec_mode)
 	// geometry
 	int *cellmap = nullptr;  // [bZ][bY][bX] -> dipole index or -1
+	RowDesc *rows = nullptr; // [bZ][bY] box rows (dense rows: dipole indices without the cell map)
 	int *mat = nullptr;      // [nd]
 	unsigned short *pos = nullptr;  // [3*nd]
 	double *wbuf = nullptr;         // [nmat*3] weights / direction scratch
@@ -184,6 +185,7 @@ struct adda_hip_op {
 	double *partials2 = nullptr; // [kcap][4*kRedBlocks] second region for device-resident solver dots
 	double *red = nullptr;       // [4*kcap] device result
 	double *red_host = nullptr;  // [4*kcap] pinned
+	void *hv_arg = nullptr, *hv_res = nullptr;  // device copies of adda_hip_matvec_host's vectors (made at first use)
 };
 
 static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
@@ -660,7 +662,7 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 	using Fn = std::conditional_t<std::is_same_v<T, TV>, P1Fn<T>, P1WFn>;
 	auto k = (Fn)c.kx.k[std::is_same_v<T, TV> ? MV_P1 : MV_P1W][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(op->bZ * nyt * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
-		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->mat, ccs, c.qx,
+		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->rows, op->mat, ccs, c.qx,
 		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
 }
 
@@ -715,7 +717,7 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
 	auto k = (Fn)c.kx.k[dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
-		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
+		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
 	return nblk5;
 }
@@ -1336,8 +1338,20 @@ extern "C" int adda_hip_device_count(void)
 	return n;
 }
 
+// whether n > 0 has no prime factor other than 2, 3, 5
+static bool smooth235(int n)
+{
+	if (n <= 0) return false;
+	while (n % 2 == 0) n /= 2;
+	while (n % 3 == 0) n /= 3;
+	while (n % 5 == 0) n /= 5;
+	return n == 1;
+}
+
 extern "C" int adda_hip_fftfit(int x, int divis)
 {
+	// a divisor with another prime factor has no 2/3/5-smooth multiple: no size exists
+	if (!smooth235(divis) || x < 1) return ERR("no 2/3/5-smooth grid size divisible by %d", divis);
 	for (;; x++) {
 		if (x % 2 != 0 || x % divis != 0) continue;
 		int y = x;
@@ -1351,6 +1365,8 @@ extern "C" int adda_hip_fftfit(int x, int divis)
 extern "C" int adda_hip_fastgrid(int x, int divis)
 {
 	const int n = adda_hip_fftfit(x, divis);
+	if (n < 0) return n;
+	if (divis & (divis - 1)) return n;  // no power of two is divisible by a divisor with an odd factor
 	int p = 2;
 	while (p < x || p % divis != 0) p *= 2;
 	return 4 * (long)p <= 5 * (long)n ? p : n;  // the power of two when at most 25 % larger
@@ -1436,6 +1452,24 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	}
 	HIPCHK(hipMalloc(&op->cellmap, nbox * sizeof(int)));
 	HIPCHK(hipMemcpy(op->cellmap, cm.data(), nbox * sizeof(int), hipMemcpyHostToDevice));
+	// row descriptors: a row is dense when its cells x0 .. x0+n-1 hold the consecutive dipoles s .. s+n-1 (the z, y, x
+	// enumeration of ADDA's particles gives that for every row without holes); empty rows are dense with n = 0
+	std::vector<RowDesc> rd((size_t)bZ * bY);
+	for (size_t r = 0; r < rd.size(); r++) {
+		const int *c = cm.data() + r * bX;
+		int x0 = 0, x1 = -1, n = 0;
+		for (int x = 0; x < bX; x++)
+			if (c[x] >= 0) {
+				if (!n) x0 = x;
+				x1 = x;
+				n++;
+			}
+		bool dense = n == 0 || x1 - x0 + 1 == n;
+		for (int x = x0; dense && n && x <= x1; x++) dense = c[x] == c[x0] + (x - x0);
+		rd[r] = RowDesc{n ? c[x0] : 0, x0, n, dense ? 1 : 0};
+	}
+	HIPCHK(hipMalloc(&op->rows, rd.size() * sizeof(RowDesc)));
+	HIPCHK(hipMemcpy(op->rows, rd.data(), rd.size() * sizeof(RowDesc), hipMemcpyHostToDevice));
 	HIPCHK(hipMalloc(&op->mat, (op->nd ? op->nd : 1) * sizeof(int)));
 	HIPCHK(hipMemcpy(op->mat, material, op->nd * sizeof(int), hipMemcpyHostToDevice));
 	HIPCHK(hipMalloc(&op->pos, 3 * op->nd * sizeof(unsigned short)));
@@ -1527,7 +1561,7 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (!op) return;
 	(void)hipSetDevice(op->dev);
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
-	void *ptrs[] = {op->partials2, op->cellmap, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
+	void *ptrs[] = {op->hv_arg, op->hv_res, op->partials2, op->cellmap, op->rows, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
 		op->D32, op->Xa, op->Xb, op->partials, op->red, op->coef, op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d,
 		op->twz.f, op->twys.d, op->twys.f};
 	for (void *p : ptrs)
@@ -1710,13 +1744,12 @@ extern "C" int adda_hip_matvec_host(adda_hip_op *op, const adda_cplx *argvec, ad
 	int her)
 {
 	if (!op || !argvec || !resultvec) return ERR("null argument");
-	void *a = adda_hip_vec_alloc(op, ADDA_F64), *r = adda_hip_vec_alloc(op, ADDA_F64);
-	int rc = (a && r) ? 0 : -1;
-	if (!rc) rc = adda_hip_vec_upload(op, ADDA_F64, a, argvec);
-	if (!rc) rc = adda_hip_matvec(op, ADDA_F64, a, r, inprod, her);
-	if (!rc) rc = adda_hip_vec_download(op, ADDA_F64, resultvec, r);
-	if (a) adda_hip_vec_free(op, a);
-	if (r) adda_hip_vec_free(op, r);
+	// the device vectors are kept for the next call (a solve calls this once or twice per iteration)
+	if (!op->hv_arg && !(op->hv_arg = adda_hip_vec_alloc(op, ADDA_F64))) return -1;
+	if (!op->hv_res && !(op->hv_res = adda_hip_vec_alloc(op, ADDA_F64))) return -1;
+	int rc = adda_hip_vec_upload(op, ADDA_F64, op->hv_arg, argvec);
+	if (!rc) rc = adda_hip_matvec(op, ADDA_F64, op->hv_arg, op->hv_res, inprod, her);
+	if (!rc) rc = adda_hip_vec_download(op, ADDA_F64, resultvec, op->hv_res);
 	return rc;
 }
 
@@ -2294,11 +2327,8 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 {
 	const int pi = sizeof(T) == 8 ? ADDA_F64 : ADDA_F32;
 	const size_t es = sizeof(cx<T>);
-	const cx<T> *ccs = cc_of<T>(op);
-	const cx<T> *D;
-	if constexpr (sizeof(T) == 8) D = op->D64;
-	else D = op->D32;
-	if (!D) return ERR("autotune of FP32 needs an operator created with the FP32 flag");
+	if (!(sizeof(T) == 8 ? (const void *)op->D64 : (const void *)op->D32))
+		return ERR("autotune of FP32 needs an operator created with the FP32 flag");
 	if (reserve(op, K)) return -1;
 	const long vs = K > 1 ? (long)op->vs : 0;
 	const size_t nv = vec_elems(op, K);
@@ -2306,19 +2336,30 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 	HIPCHK(hipMalloc(&a, nv * es));
 	HIPCHK(hipMalloc(&r, nv * es));
 	HIPCHK(hipMemsetAsync(a, 0, nv * es, op->stream));
-	hipEvent_t e0, e1;
-	HIPCHK(hipEventCreate(&e0));
-	HIPCHK(hipEventCreate(&e1));
 	MvCfg best = op->cfg[pi];
-	auto timeit = [&](auto &&fn) -> float {
-		for (int w = 0; w < 2; w++) fn();
-		(void)hipEventRecord(e0, op->stream);
-		for (int i = 0; i < reps; i++) fn();
-		(void)hipEventRecord(e1, op->stream);
-		(void)hipEventSynchronize(e1);
-		float ms = 0;
-		(void)hipEventElapsedTime(&ms, e0, e1);
-		return ms / reps;
+	hipEvent_t ev[6];
+	for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+	// A candidate is timed inside the whole matvec (its own passes, HIP events between the passes): alone, a pass
+	// would find its operands in the 256 MB Infinity Cache from its previous repetition (the FP64 D-matrix octant at
+	// 256^3 is 206 MB), which the other passes evict in a real matvec -- measured, that favoured configurations
+	// that read the D-matrix twice.
+	auto timeit = [&](const MvCfg &c, unsigned mask) -> float {
+		const MvCfg keep = op->cfg[pi];
+		op->cfg[pi] = c;
+		float acc = 0;
+		for (int i = 0; i < 2 + reps; i++) {
+			(void)matvec_io<T, T>(op, K, vs, nullptr, a, r, nullptr, 0, ev, false, nullptr, nullptr, 0);
+			if (i < 2) continue;
+			(void)hipEventSynchronize(ev[5]);
+			for (int q = 0; q < 5; q++)
+				if ((mask >> q) & 1) {
+					float ms = 0;
+					(void)hipEventElapsedTime(&ms, ev[q], ev[q + 1]);
+					acc += ms;
+				}
+		}
+		op->cfg[pi] = keep;
+		return acc / reps;
 	};
 	std::vector<FftPlan2> px, py, pz;
 	enum_plans(op->gX, px);
@@ -2335,10 +2376,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
-			float t = timeit([&] {
-				launch_p1<T>(op, c, K, vs, nullptr, a, ccs, 0);
-				launch_p5<T>(op, c, K, vs, nullptr, a, r, ccs, 0, nullptr);
-			});
+			float t = timeit(c, 1u | 16u);
 			if (t < tx) { tx = t; best.qx = q; best.LY = LY; }
 		}
 	// z passes (P2 + P4): radix schedule of z and TC
@@ -2350,7 +2388,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
-			float t = timeit([&] { launch_zfft<T, false>(op, c, K, nullptr); launch_zfft<T, true>(op, c, K, nullptr); });
+			float t = timeit(c, 2u | 8u);
 			if (t < tz) { tz = t; best.qz = q; best.TC = TC; }
 		}
 	// y pass (P3): radix schedule of y, LP, column pairs and the register-product variant
@@ -2376,11 +2414,12 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
-			float t = timeit([&] { launch_p3<T>(op, c, K, nullptr, D, 0); });
-			if (t < ty) { ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; best.p3t = pt; }
+			float t = timeit(c, 4u);
+			if (t < ty) {
+				ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; best.p3t = pt;
+			}
 		}
-	(void)hipEventDestroy(e0);
-	(void)hipEventDestroy(e1);
+	for (auto &e : ev) (void)hipEventDestroy(e);
 	HIPCHK(hipStreamSynchronize(op->stream));
 	HIPCHK(hipFree(a));
 	HIPCHK(hipFree(r));

@@ -305,8 +305,13 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 		const int j = t + u * TPL;
 		if (active && (NB % TPL == 0 || j < NB)) {
 #pragma unroll
-			for (int q = 0; q < (HI ? R / 2 : R); q++)
-				v[u][q] = FIRST ? ld(line, j + q * NB) : buf[LAY::pos(line, j + q * NB, KS, LS)];
+			for (int q = 0; q < (HI ? R / 2 : R); q++) {
+				// a Load functor that takes a third argument also gets the input's slot u*R + q (the order of
+				// Fix::for_first), e.g. to return a value it gathered before the transform
+				if constexpr (!FIRST) v[u][q] = buf[LAY::pos(line, j + q * NB, KS, LS)];
+				else if constexpr (std::is_invocable_v<Ld &, int, int, int>) v[u][q] = ld(line, j + q * NB, u * R + q);
+				else v[u][q] = ld(line, j + q * NB);
+			}
 			if (!FIRST) {
 				const int k = j % NS;
 #pragma unroll
@@ -327,7 +332,12 @@ __device__ __forceinline__ void sstage(cx<T> *buf, const cx<T> *tws, int KS, int
 			const int j = t + u * TPL;
 			if (active && (NB % TPL == 0 || j < NB)) {
 #pragma unroll
-				for (int q = 0; q < (HO ? R / 2 : R); q++) st(line, j + q * NS, v[u][q]);
+				for (int q = 0; q < (HO ? R / 2 : R); q++) {
+					// a Store functor that takes a fourth argument also gets the output's slot u*R + q (the order of
+					// Fix::for_last), e.g. to find data it prefetched for that output
+					if constexpr (std::is_invocable_v<St &, int, int, cx<T>, int>) st(line, j + q * NS, v[u][q], u * R + q);
+					else st(line, j + q * NS, v[u][q]);
+				}
 			}
 		}
 	}
@@ -499,6 +509,37 @@ template <int... Rs> struct Fix {
 	}
 	__device__ static __forceinline__ constexpr int n(const FftPlan2 &) { return N; }
 	__device__ static __forceinline__ constexpr int tpl(const FftPlan2 &) { return TPL; }
+	// The outputs of the last stage that thread t of a line stores: f(slot, k) for element k, in the order and with
+	// the slot numbers the last stage passes to a four-argument Store functor (HOUT: only k < N/2 with an even last
+	// radix). NSLOT bounds the slot numbers.
+	static constexpr int RL = RAD[NST - 1], NBL = N / RL, MUL = (RMAX + RL - 1) / RL, NSLOT = MUL * RL;
+	// The same for the inputs of the first stage that thread t of a line loads (HIN: only k < N/2 with an even first
+	// radix), with the slot numbers the first stage passes to a three-argument Load functor.
+	static constexpr int RF = RAD[0], NBF = N / RF, MUF = (RMAX + RF - 1) / RF, NSLOTF = MUF * RF;
+	template <bool HIN, class F> __device__ static __forceinline__ void for_first(int t, F &&f)
+	{
+		constexpr bool HI = HIN && RF % 2 == 0;
+#pragma unroll
+		for (int u = 0; u < MUF; u++) {
+			const int j = t + u * TPL;
+			if (NBF % TPL == 0 || j < NBF) {
+#pragma unroll
+				for (int q = 0; q < (HI ? RF / 2 : RF); q++) f(u * RF + q, j + q * NBF);
+			}
+		}
+	}
+	template <bool HOUT, class F> __device__ static __forceinline__ void for_last(int t, F &&f)
+	{
+		constexpr bool HO = HOUT && RL % 2 == 0;
+#pragma unroll
+		for (int u = 0; u < MUL; u++) {
+			const int j = t + u * TPL;
+			if (NBL % TPL == 0 || j < NBL) {
+#pragma unroll
+				for (int q = 0; q < (HO ? RL / 2 : RL); q++) f(u * RL + q, j + q * NBL);
+			}
+		}
+	}
 	template <typename T, bool INV, bool HIN, bool HOUT, bool GIN, bool GOUT, class LAY, int S, int NS, class Ld,
 		class St>
 	__device__ static __forceinline__ void run(cx<T> *buf, const cx<T> *tws, int KS, int LS, int lineA, int tA,

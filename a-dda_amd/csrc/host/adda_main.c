@@ -6,9 +6,10 @@
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
  *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
  *            [-store_dip_pol] [-store_int_field] [-chpoint <time>] [-chp_type normal|regular|always]
- *            [-chp_dir <d>] [-chp_load]
+ *            [-chp_dir <d>] [-chp_load] [-sym auto|no|enf] [-iter_inner <method>]
  *
- * Checkpoints (ADDA's chp.0 format, interchangeable with the reference's) cover the first polarization's solve.
+ * Checkpoints (ADDA's chp.0 format, interchangeable with the reference's) cover the first polarization's solve, and,
+ * as in the reference, need a run whose symmetry computes the internal field once (rotational symmetry, or -sym enf).
  *
  * It runs the Calculator flow for one orientation (src/calculator.c:293-344): InitCC, GenerateB, IterativeSolver,
  * Cext/Cabs, the yz-plane amplitudes, for the Y then the X incident polarization, and writes CrossSec-Y, CrossSec-X,
@@ -112,12 +113,13 @@ int main(int argc, char **argv)
 	const char *shape = "sphere", *geom = NULL, *dir = "run_adda_hip";
 	double asp_y = 1, asp_z = 1, size = -1, dpl = -1, lambda = 2 * PI, eps_k = 5, eps_inner_k = 0;
 	int grid = 16, maxiter = -1, ntheta = -1, gpu = 0, reduced = 1, have_orient = 0;
+	int sym = 0; /* -sym: 0 auto, 1 no, 2 enf (src/param.c:1529-1540, FinalizeSymmetry 2160-2164) */
 	int store_dip_pol = 0, store_int_field = 0;
 	double orient[3] = {0, 0, 0};
 	adda_cplx m[16];
 	int nm = 0;
 	adda_solver_opts opt = {ADDA_IT_QMR_CS, 1e-5, 0, ADDA_IF_AUTO, 0, ADDA_PREC_FP64, 0, 0, 1, 0, 0, 0,
-		ADDA_CHP_NONE, 0, "chpoint", 0, NULL};
+		ADDA_CHP_NONE, 0, "chpoint", 0, NULL, 0};
 	int chp_time_set = 0;
 	int polrel = ADDA_POL_LDR;
 	for (int i = 1; i < argc; i++) {
@@ -142,17 +144,16 @@ int main(int argc, char **argv)
 				nm++;
 			}
 		}
-		else if (!strcmp(a, "-iter")) {
+		else if (!strcmp(a, "-iter") || !strcmp(a, "-iter_inner")) {
 			NEED(1);
 			const char *s = argv[++i];
-			if (!strcmp(s, "qmr")) opt.method = ADDA_IT_QMR_CS;
-			else if (!strcmp(s, "bicgstab")) opt.method = ADDA_IT_BICGSTAB;
-			else if (!strcmp(s, "cgnr")) opt.method = ADDA_IT_CGNR;
-			else if (!strcmp(s, "bicg")) opt.method = ADDA_IT_BICG_CS;
-			else if (!strcmp(s, "csym")) opt.method = ADDA_IT_CSYM;
-			else if (!strcmp(s, "bcgs2")) opt.method = ADDA_IT_BCGS2;
-			else if (!strcmp(s, "qmr2")) opt.method = ADDA_IT_QMR_CS_2;
-			else { fprintf(stderr, "ERROR: unsupported -iter %s\n", s); return EXIT_FAILURE; }
+			static const char *names[7] = {"qmr", "bicgstab", "cgnr", "bicg", "csym", "bcgs2", "qmr2"}; /* enum order */
+			int m = -1;
+			for (int j = 0; j < 7; j++)
+				if (!strcmp(s, names[j])) m = j;
+			if (m < 0) { fprintf(stderr, "ERROR: unsupported %s %s\n", a, s); return EXIT_FAILURE; }
+			if (!strcmp(a, "-iter")) opt.method = m;
+			else opt.inner_method = m + 1; /* the inner solver of the mixed forms (default: automatic) */
 		}
 		else if (!strcmp(a, "-eps")) { NEED(1); eps_k = atof(argv[++i]); }
 		else if (!strcmp(a, "-eps_inner")) { NEED(1); eps_inner_k = atof(argv[++i]); }
@@ -175,6 +176,14 @@ int main(int argc, char **argv)
 			have_orient = 1;
 		}
 		else if (!strcmp(a, "-no_reduced_fft")) reduced = 0;
+		else if (!strcmp(a, "-sym")) {
+			NEED(1);
+			const char *v = argv[++i];
+			if (!strcmp(v, "auto")) sym = 0;
+			else if (!strcmp(v, "no")) sym = 1;
+			else if (!strcmp(v, "enf")) sym = 2;
+			else { fprintf(stderr, "ERROR: Unknown argument '%s' to '-sym' option\n", v); return EXIT_FAILURE; }
+		}
 		else if (!strcmp(a, "-gpu")) { NEED(1); gpu = atoi(argv[++i]); }
 		else if (!strcmp(a, "-dir")) { NEED(1); dir = argv[++i]; }
 		else if (!strcmp(a, "-host_scalars")) opt.host_scalars = 1;
@@ -280,9 +289,16 @@ int main(int argc, char **argv)
 	 * src/CalculateE.c:396-418, calculator.c:674) */
 	int symXY = !geom && !have_orient, symR = symXY && (!strcmp(shape, "sphere") || asp_y == 1),
 		symZ = !geom && !have_orient;
+	if (sym) symXY = symR = symZ = sym == 2; /* -sym no / enf */
 	int sym_incY = ((symXY || fabs(incY[0]) < 1e-10) && (symXY || fabs(incY[1]) < 1e-10)) || symR;
 	sym_incY = sym_incY && (symZ || fabs(incY[2]) < 1e-10);
 	const int npoints = sym_incY ? ntheta : 2 * (ntheta - 1);
+	/* the reference takes checkpoints only on runs that compute the internal field once, for a single polarization
+	 * (FinalizeSymmetry, src/param.c:2171-2174); this front end solves the second polarization of such a run afresh
+	 * (equal to the reference's rotated first one), without checkpoints */
+	if ((opt.chp_type != ADDA_CHP_NONE || opt.chp_load) && !symR)
+		die("Currently checkpoints can be used when internal fields are calculated only once, i.e. for a single "
+			"incident polarization");
 	adda_cplx *eplX = malloc(2 * npoints * sizeof(adda_cplx)), *eplY = malloc(2 * npoints * sizeof(adda_cplx));
 	adda_cplx *cc = malloc((size_t)p.nmat * 3 * sizeof(adda_cplx)), *ccs = malloc((size_t)p.nmat * 3 * sizeof(adda_cplx));
 	int total_mv = 0, total_iter = 0;
@@ -314,7 +330,7 @@ int main(int argc, char **argv)
 		t_solve += ts;
 		if (rc < 0) die("IterativeSolver");
 		if (rc == 1) fprintf(stderr, "WARNING: iterations haven't converged in %d iterations\n", maxiter);
-		if (o.chp_type != ADDA_CHP_NONE) fprintf(log, "Checkpoint (iteration) saved\n");
+		if (st.chp_saved) fprintf(log, "Checkpoint (iteration) saved\n");
 		if (rc == 2) { /* stopped after saving a checkpoint (CHP_EXIT, src/ADDAmain.c) */
 			printf("Checkpoint (iteration) saved after %d iterations; the run stops here\n", st.niter);
 			fclose(log);

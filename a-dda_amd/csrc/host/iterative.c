@@ -28,6 +28,7 @@ typedef struct {
 	adda_hip_op *op;
 	int prec;
 	int nmv;
+	int chp_saved; /* an iteration checkpoint was written */
 } kctx;
 
 #define CK(x) do { if ((x) != 0) return host_err_hip(__FILE__, __LINE__); } while (0)
@@ -82,6 +83,12 @@ static double wall_s(void)
 	clock_gettime(CLOCK_MONOTONIC, &ts);
 	return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
+
+/* Start of the checkpoint clock: the time the library was loaded, i.e. program start for a program linked against
+ * it. The reference starts last_chp_wt at program start (src/timing.c:102) and never resets it per solve, so set-up
+ * and earlier solves count towards -chpoint. */
+static double g_chp_clock0;
+__attribute__((constructor)) static void chp_clock_init(void) { g_chp_clock0 = wall_s(); }
 
 /* One Krylov solve of A.x = b in precision k->prec (IterativeSolver body, src/iterative.c:1398-1530).
  * b: right-hand side (already S.Einc); x: in/out (initial field per init_field); on return *niter_out iterations.
@@ -197,8 +204,7 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 			else { t = pc_old; pc_old = pc_new; pc_new = t; }
 		}
 	}
-	const double t_start = wall_s();
-	double t_last = t_start;
+	double t_last = g_chp_clock0;
 	/* SaveIterChpoint, src/iterative.c:165-222 */
 #define CHP_SAVE() do { \
 		for (int i_ = 0; i_ < nsd; i_++) sc[i_] = *sd[i_]; \
@@ -206,6 +212,7 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		ch.niter = niter; ch.counter = counter; ch.inprodR = inprodR; ch.prev_err = prev_err; \
 		ch.resid_scale = resid_scale; \
 		CKG(chp_move(k, chp, method, 1, &ch, sc, dv)); \
+		k->chp_saved = 1; \
 		if (verbose) printf("Checkpoint (iteration) saved\n"); \
 	} while (0)
 	/* PHASE_INIT */
@@ -241,7 +248,11 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		c_old = 0;
 		s_new = s_old = 0;
 	}
+	const int chp_always = chp && chp->chp_type == ADDA_CHP_ALWAYS;
 	while (inprodR > epsB && niter <= maxiter && counter <= mc && !chp_exit) {
+		/* complete: the iteration ran to its end; an iteration stopped by a convergence test in its middle saves no
+		 * timed checkpoint (src/iterative.c:81,309,703-707) */
+		int complete = 1;
 		if (method == ADDA_IT_QMR_CS) { /* src/iterative.c:962-1030 */
 			double dtmp1 = 1 / (omega_new * omega_new), dtmp2, zetaabs;
 			dcx theta, eta, zetatilda, zeta, temp1, temp2, temp4;
@@ -304,7 +315,12 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 			if (cabs(temp1) / cabs(ro_new) < 1e-10) { rc = host_err(__FILE__, __LINE__, "BiCGStab fails: |v.r~|/|r.r~| is too small"); break; }
 			alpha = ro_new / temp1;
 			CKG(lc(k, sv, vv, -alpha, r, 1, NULL, 0, 0, &inprodRp1));
-			if (inprodRp1 < epsB) CKG(lc(k, x, x, 1, p, alpha, NULL, 0, 0, NULL));
+			/* convergence at the half step: skipped when every run ends with a checkpoint (type always), which must
+			 * hold a consistent r and ro_old (src/iterative.c:702-707) */
+			if (inprodRp1 < epsB && !chp_always) {
+				CKG(lc(k, x, x, 1, p, alpha, NULL, 0, 0, NULL));
+				complete = 0;
+			}
 			else {
 				double denumOmega;
 				CKG(mv(k, sv, Av, &denumOmega, 0));
@@ -488,7 +504,7 @@ static int krylov(kctx *k, int method, double eps, int maxiter, int init_field, 
 		if (hist && niter < hist_len) hist[niter] = err;
 		niter++;
 		/* checkpoint on time (src/iterative.c:307-317): normal and always stop after it, regular goes on */
-		if (chp && chp->chp_type != ADDA_CHP_NONE && chp->chp_time > 0 && wall_s() - t_last > chp->chp_time) {
+		if (chp && chp->chp_type != ADDA_CHP_NONE && chp->chp_time > 0 && complete && wall_s() - t_last > chp->chp_time) {
 			CHP_SAVE();
 			t_last = wall_s();
 			if (chp->chp_type != ADDA_CHP_REGULAR) chp_exit = 1;
@@ -529,6 +545,7 @@ typedef struct {
 	int niter;   /* iterations */
 	int nmv;     /* matvecs */
 	double rel;  /* final relative residual of the recurrence */
+	int chp_saved;
 } kres;
 
 static const char *method_name(int m)
@@ -775,7 +792,7 @@ static int krylov_run(adda_hip_op *op, const adda_solver_opts *opt, int prec, in
 	if (device_method(opt))
 		return krylov_dev_batch(op, prec, K, opt->method, eps, maxiter, init_field, verbose, active, b, x, hist,
 			hist_len, opt->graph, res);
-	kctx k = {op, prec, 0};
+	kctx k = {op, prec, 0, 0};
 	int ni = 0;
 	double rel = 0;
 	const int chp = opt->chp_type != ADDA_CHP_NONE || opt->chp_load;
@@ -785,6 +802,7 @@ static int krylov_run(adda_hip_op *op, const adda_solver_opts *opt, int prec, in
 	res[0].niter = ni;
 	res[0].nmv = k.nmv;
 	res[0].rel = rel;
+	res[0].chp_saved = k.chp_saved;
 	return rc;
 }
 
@@ -827,6 +845,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 			st[k].niter = kr[k].niter;
 			st[k].nmatvec = kr[k].nmv;
 			st[k].resid = kr[k].rel;
+			st[k].chp_saved = kr[k].chp_saved;
 		}
 		if (rc < 0) goto out;
 		if (opt->recalc_resid) { /* ResidualNorm2, src/iterative.c:1512-1520 */
@@ -858,6 +877,16 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		 * residual on the 1.1 M-dipole sphere), where its recurrence residual still tracks the true one. Over four
 		 * test problems this took 2.5 % more iterations than FP64 on average, 1e-5 took 5 %. */
 		const double inner = opt->inner_eps > 0 ? opt->inner_eps : 1e-4;
+		/* inner Krylov method (adda_solver_opts.inner_method): the complex-symmetric Lanczos methods assume A^T = A,
+		 * which the FP32 operator holds to about 1e-7; their FP32 inner solves lose the symmetric Lanczos basis
+		 * (measured on the 1.1 M-dipole sphere: the true residual stalls near 1e-3 of the step's right-hand side while
+		 * the recurrence residual keeps falling), so by default they run BiCGStab, which assumes no symmetry. */
+		adda_solver_opts iopt = *opt;
+		if (opt->inner_method > 0 && opt->inner_method <= 7) iopt.method = opt->inner_method - 1;
+		else if (opt->method != ADDA_IT_BICGSTAB && opt->method != ADDA_IT_CGNR && opt->method != ADDA_IT_BCGS2)
+			iopt.method = ADDA_IT_BICGSTAB;
+		if (K > 1 && !device_method(&iopt))
+			{ rc = host_err(__FILE__, __LINE__, "batched mixed solves need a device-resident inner method"); goto out; }
 		const int max_outer = opt->max_outer > 0 ? opt->max_outer : 30;
 		const int cap = opt->max_inner > 0 && opt->max_inner < opt->maxiter ? opt->max_inner : opt->maxiter;
 		const int ihl = (hist_len < cap ? hist_len : cap) + 2; /* inner entries beyond hist_len are not kept */
@@ -924,13 +953,13 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 			 * decides what happens next */
 			if (v64) { /* FP64 vectors and recurrences, FP32 transforms */
 				CKB(adda_hip_set_matvec_mode(op, 1));
-				krylov_run(op, opt, ADDA_F64, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, Ax, e64, ih,
+				krylov_run(op, &iopt, ADDA_F64, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, Ax, e64, ih,
 					ih ? ihl : 0, kr);
 				CKB(adda_hip_set_matvec_mode(op, 0));
 			}
 			else {
 				CKB(adda_hip_vec_convert_batch(op, K, r32, ADDA_F32, Ax, ADDA_F64));
-				krylov_run(op, opt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, r32, e32, ih,
+				krylov_run(op, &iopt, ADDA_F32, K, eps, cap, ADDA_IF_ZERO, opt->verbose & 2, active, r32, e32, ih,
 					ih ? ihl : 0, kr);
 			}
 			for (int k = 0; k < K; k++) {

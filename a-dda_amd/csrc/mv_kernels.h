@@ -85,6 +85,13 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 	}
 }
 
+// Box row (z, y) of the particle: with dense = 1 its dipoles are the consecutive indices s .. s+n-1 at the consecutive
+// cells x0 .. x0+n-1 (a row without holes of a particle whose dipoles are enumerated in z, y, x order, as ADDA's are);
+// an empty row is dense with n = 0. Other rows (holes, or another dipole order) are read through the cell map.
+struct alignas(16) RowDesc {
+	int s, x0, n, dense;
+};
+
 // ==================================================================================================================
 // kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][y<boxY][c], Xb[z<gridZ][x<gridX][y<boxY][c],
 // D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
@@ -103,8 +110,8 @@ __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ tw
 // vectors around an FP32 transform: TV = double, T = float; otherwise TV = T).
 template <typename T, class PL, typename TV = T>
 __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T> *__restrict__ Xa,
-	const int *__restrict__ cellmap, const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p,
-	const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
+	const int *__restrict__ cellmap, const RowDesc *__restrict__, const int *__restrict__ mat,
+	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
@@ -163,6 +170,70 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 	const size_t xs = (size_t)3 * bY;
 	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
 	PL::template fft<T, false, true, false, false, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
+		st);
+}
+
+// P1 for compile-time plans, with the gather fused into the first FFT stage: that stage loads its inputs -- element x of
+// line (yl, c) is cc_sqrt[c]*arg[3d + c] of the dipole d at cell x of row y0 + yl, zero for an empty cell -- straight
+// from the dipole vector, so the gathered lines never make an LDS round trip of their own. Thread t of a line loads the
+// cells t + 16q (radix 16): lanes of the three component lines of a row read the 48 (24) contiguous bytes of each
+// dipole together. Dense rows (RowDesc) give the dipole index arithmetically, so the argument loads depend on one
+// descriptor load per thread and never on a cell-map read. Same arithmetic as k_p1, so the results are the same.
+template <typename T, class PL, typename TV = T>
+__global__ void __launch_bounds__(512) k_p1s(const cx<TV> *__restrict__ arg, cx<T> *__restrict__ Xa,
+	const int *__restrict__ cellmap, const RowDesc *__restrict__ rows, const int *__restrict__ mat,
+	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
+{
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	arg += kq * bq.vs;
+	Xa += kq * bq.xs2;
+	ccs += (size_t)kq * bq.cs;
+	extern __shared__ char smem_raw[];
+	using LAY = std::conditional_t<PL::SWZ, LaySwz<false>, LayPad>;  // lines are also walked line-fastest
+	constexpr int N = PL::N, TPL = PL::TPL;
+	const int LS = PL::SWZ ? N + 1 : rowstride_odd(N), NL = 3 * LY;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	PL::template load_tw<T>(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
+	const int ny = bY - y0 < LY ? bY - y0 : LY;
+	const int tid = threadIdx.x;
+	const int lineA = tid / TPL, tA = tid % TPL;  // x fastest (the gather of the first stage)
+	const int lineB = tid % NL, tB = tid / NL;    // (y, c) fastest (writes Xa)
+	const bool actA = lineA < 3 * ny, actB = tB < TPL && lineB < 3 * ny;
+	const int yl = lineA / 3, c = lineA - 3 * yl;
+	const size_t row = (size_t)z * bY + y0 + yl;
+	RowDesc rd = {0, 0, 0, 1};
+	if (actA) rd = rows[row];
+	const int *cm = cellmap + row * bX;
+	const bool onemat = bq.cs == 3;  // one material: no per-dipole material lookup
+	// The gather runs in phases over all inputs of the thread's first stage -- dipole indices, then material indices,
+	// then arguments and coupling constants -- so each phase's loads are in flight together (one element at a time
+	// would wait on each load in turn). The loads of empty cells read dipole 0 and are discarded: a load under a
+	// condition gets a basic block of its own, and the compiler waits for it before the next.
+	constexpr int NSF = PL::NSLOTF;
+	int dd[NSF], mm[NSF];
+	cx<TV> av[NSF];
+	cx<T> gv[NSF];
+	if (rd.dense) PL::template for_first<true>(tA, [&](int sl, int x) {
+		const unsigned o = (unsigned)(x - rd.x0);
+		dd[sl] = actA && x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
+	});
+	else PL::template for_first<true>(tA, [&](int sl, int x) { dd[sl] = actA && x < bX ? cm[x] : -1; });
+	if (!onemat) PL::template for_first<true>(tA, [&](int sl, int) { mm[sl] = 3 * mat[dd[sl] > 0 ? dd[sl] : 0]; });
+	else PL::template for_first<true>(tA, [&](int sl, int) { mm[sl] = 0; });
+	PL::template for_first<true>(tA, [&](int sl, int) { gv[sl] = ccs[mm[sl] + c]; });
+	PL::template for_first<true>(tA, [&](int sl, int) { av[sl] = arg[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c]; });
+	PL::template for_first<true>(tA, [&](int sl, int) {
+		const cx<T> a = cvt<T>(her ? conjc(av[sl]) : av[sl]);
+		gv[sl] = dd[sl] >= 0 ? mul(gv[sl], a) : mk<T>(0, 0);
+	});
+	auto ld = [&](int, int, int sl) -> cx<T> { return gv[sl]; };
+	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineB;
+	const size_t xs = (size_t)3 * bY;
+	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
+	PL::template fft<T, false, true, false, true, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
 		st);
 }
 
@@ -517,7 +588,7 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 // TV = double, T = float the correction S.X is rounded to FP32 and added to the FP64 argument in FP64.
 template <typename T, class PL, bool DOT, typename TV = T>
 __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
-	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const int *__restrict__ mat,
+	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const RowDesc *__restrict__, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
 	double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
 {
@@ -646,6 +717,112 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	}
 }
 
+// P5 for compile-time plans, with the gather fused into the last stage of the inverse FFT-x: each output x < boxX of
+// line (yl, c) goes straight to result[3d + c] = arg[3d + c] + cc_sqrt*X for the dipole d at that cell (her: conj),
+// without passing through LDS. The lanes of the three component lines of a row write the 48 (24) contiguous bytes of
+// each dipole in one store instruction, so every written cache line is complete. Dense rows (RowDesc) give the dipole
+// index arithmetically. The partial sums are those of k_p5 (rows 0..2 at partials[(k*rows + row)*nb + tile]),
+// accumulated in another order. The last stage leaves its outputs in registers; then the dipole indices, materials,
+// arguments, coupling constants (and dot weights) of all of them are found in phases whose loads are in flight together
+// -- one round trip per thread for dense rows of one material -- and nothing is held in registers through the
+// transform (which would cost a resident block per CU).
+template <typename T, class PL, bool DOT, typename TV = T>
+__global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
+	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const RowDesc *__restrict__ rows,
+	const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX,
+	int bY, int LY, int her, double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
+{
+	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq)) return;
+	Xa += kq * bq.xs;
+	arg += kq * bq.vs;
+	res += kq * bq.vs;
+	if (dotw) dotw += kq * bq.vs;
+	ccs += (size_t)kq * bq.cs;
+	extern __shared__ char smem_raw[];
+	using LAY = std::conditional_t<PL::SWZ, LaySwz<false>, LayPad>;  // lines are also walked line-fastest
+	constexpr int N = PL::N, TPL = PL::TPL;
+	const int LS = PL::SWZ ? N + 1 : rowstride_odd(N), NL = 3 * LY;
+	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
+	double *red_lds = reinterpret_cast<double *>(buf);  // the lines' space, reused after the transform (>= 48*LY doubles)
+	PL::template load_tw<T>(tws, twg, N);
+	const int nyt = (bY + LY - 1) / LY;
+	const int z = tile / nyt, y0 = (tile % nyt) * LY;
+	const int ny = bY - y0 < LY ? bY - y0 : LY;
+	const int tid = threadIdx.x;
+	const int lineA = tid % NL, tA = tid / NL;    // (y, c) fastest (reads Xa)
+	const int lineB = tid / TPL, tB = tid % TPL;  // x fastest (the gather of the last stage)
+	const bool actA = tA < TPL && lineA < 3 * ny, actB = lineB < 3 * ny;
+	const int yl = lineB / 3, c = lineB - 3 * yl;
+	const size_t row = (size_t)z * bY + y0 + yl;
+	RowDesc rd = {0, 0, 0, 1};
+	if (actB) rd = rows[row];
+	const int *cm = cellmap + row * bX;
+	const bool onemat = bq.cs == 3;
+	constexpr int NS = PL::NSLOT;
+	double s = 0, dr = 0, di = 0;
+	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
+	const size_t xs = (size_t)3 * bY;
+	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
+	cx<T> ov[NS];
+	auto st = [&](int, int, cx<T> v, int sl) { ov[sl] = v; };
+	// only x < boxX <= N/2 is gathered: the inverse transform computes the lower half of each line
+	PL::template fft<T, true, false, true, true, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
+		st);
+	if (actB) {
+		// phases whose loads are in flight together (loaded one output at a time, each would wait on the last); empty
+		// cells load dipole 0 and discard it (a conditional load would get a basic block and a wait of its own)
+		int dd[NS], mm[NS];
+		cx<TV> av[NS], wv[NS];
+		cx<T> cv[NS];
+		if (rd.dense) PL::template for_last<true>(tB, [&](int sl, int x) {
+			const unsigned o = (unsigned)(x - rd.x0);
+			dd[sl] = x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
+		});
+		else PL::template for_last<true>(tB, [&](int sl, int x) { dd[sl] = x < bX ? cm[x] : -1; });
+		if (!onemat) PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 3 * mat[dd[sl] > 0 ? dd[sl] : 0]; });
+		else PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 0; });
+		PL::template for_last<true>(tB, [&](int sl, int) {
+			const size_t i = 3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c;
+			av[sl] = arg[i];
+			cv[sl] = ccs[mm[sl] + c];
+			// weights through one global pointer (dmode 2 re-reads the argument from cache): a select between a
+			// register copy and a load would go through scratch memory
+			if (DOT) wv[sl] = (dmode == 1 ? dotw : arg)[i];
+		});
+		PL::template for_last<true>(tB, [&](int sl, int) {
+			const int d = dd[sl];
+			if (d < 0) return;
+			const cx<TV> a = av[sl];
+			cx<TV> rv = add(her ? conjc(a) : a, cvt<TV>(mul(cv[sl], ov[sl])));
+			if (her) rv = conjc(rv);
+			res[3 * (size_t)d + c] = rv;
+			s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+			if (DOT) {
+				const cx<TV> w = wv[sl];
+				dr += (double)rv.x * w.x + (double)rv.y * w.y;
+				di += (double)rv.y * w.x - (double)rv.x * w.y;
+			}
+		});
+	}
+	if (partials) {  // block_sum starts with a barrier: the transform's last LDS reads are done before red_lds is reused
+		const int nr = DOT ? 3 : 1;
+		const size_t nb = gridDim.x / bq.K, o = (size_t)kq * nr * nb + tile;
+		if (DOT) {
+			block_sum3<T>(s, dr, di, red_lds);
+			if (threadIdx.x == 0) {
+				partials[o] = s;
+				partials[o + nb] = dr;
+				partials[o + 2 * nb] = di;
+			}
+		}
+		else {
+			const double tt = block_sum<T>(s, red_lds);
+			if (threadIdx.x == 0) partials[o] = tt;
+		}
+	}
+}
+
 // The staged twiddle table of a compile-time plan (Fix::load_tw's layout) written to global memory, for k_p3<..., true>.
 // One block.
 template <typename T, class PL>
@@ -681,13 +858,21 @@ struct MvKernels {
 template <typename T, class PL> void mv_fill_t(MvKernels &m)
 {
 	const int f = sizeof(T) == 4;
-	m.k[MV_P1][f] = (const void *)&k_p1<T, PL>;
+	// compile-time plans: P1/P5 with the gather fused into the first / last FFT stage
+	if constexpr (is_fix<PL>::value) {
+		m.k[MV_P1][f] = (const void *)&k_p1s<T, PL>;
+		m.k[MV_P5][f] = (const void *)&k_p5s<T, PL, false>;
+		m.k[MV_P5D][f] = (const void *)&k_p5s<T, PL, true>;
+	}
+	else {
+		m.k[MV_P1][f] = (const void *)&k_p1<T, PL>;
+		m.k[MV_P5][f] = (const void *)&k_p5<T, PL, false>;
+		m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
+	}
 	m.k[MV_ZFWD][f] = (const void *)&k_zfft<T, false, PL>;
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
 	m.k[MV_P31][f] = (const void *)&k_p3<T, PL, 1>;
-	m.k[MV_P5][f] = (const void *)&k_p5<T, PL, false>;
-	m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
 	m.k[MV_P3G][f] = m.k[MV_TWST][f] = m.k[MV_P3G1][f] = nullptr;
@@ -698,9 +883,16 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	}
 	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = nullptr;
 	if constexpr (sizeof(T) == 4) {
-		m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
-		m.k[MV_P5W][f] = (const void *)&k_p5<T, PL, false, double>;
-		m.k[MV_P5DW][f] = (const void *)&k_p5<T, PL, true, double>;
+		if constexpr (is_fix<PL>::value) {
+			m.k[MV_P1W][f] = (const void *)&k_p1s<T, PL, double>;
+			m.k[MV_P5W][f] = (const void *)&k_p5s<T, PL, false, double>;
+			m.k[MV_P5DW][f] = (const void *)&k_p5s<T, PL, true, double>;
+		}
+		else {
+			m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
+			m.k[MV_P5W][f] = (const void *)&k_p5<T, PL, false, double>;
+			m.k[MV_P5DW][f] = (const void *)&k_p5<T, PL, true, double>;
+		}
 	}
 }
 template <class PL> void mv_fill(MvKernels &m)
@@ -716,35 +908,28 @@ constexpr int kMvLdsMax = 96 * 1024;
 template <typename T, class PL> hipError_t mv_lds_t()
 {
 	hipError_t e;
-	if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
-		return e;
-	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax)))
-		return e;
-	if ((e = hipFuncSetAttribute((const void *)k_p3<T, PL, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax))) return e;
-	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax))) return e;
-	if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax))) return e;
-	if ((e = hipFuncSetAttribute((const void *)k_zfft<T, false, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax))) return e;
-	if constexpr (p3r_ok<T, PL>())
-		if ((e = hipFuncSetAttribute((const void *)k_p3r<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			kMvLdsMax))) return e;
-	if constexpr (is_fix<PL>::value)
-		if ((e = hipFuncSetAttribute((const void *)k_p3g<T, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			kMvLdsMax)) || (e = hipFuncSetAttribute((const void *)k_p3g<T, PL, 1>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
-	if constexpr (sizeof(T) == 4) {
-		if ((e = hipFuncSetAttribute((const void *)k_p1<T, PL, double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-			kMvLdsMax))) return e;
-		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, false, double>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
-		if ((e = hipFuncSetAttribute((const void *)k_p5<T, PL, true, double>,
-			hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax))) return e;
+	auto lim = [&](const void *k) { return e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax); };
+	// P1/P5: the stage-fused gathers for compile-time plans, the cell-map gathers for run-time plans
+	if constexpr (is_fix<PL>::value) {
+		if (lim((const void *)k_p1s<T, PL>) || lim((const void *)k_p5s<T, PL, false>) ||
+			lim((const void *)k_p5s<T, PL, true>)) return e;
+		if constexpr (sizeof(T) == 4)
+			if (lim((const void *)k_p1s<T, PL, double>) || lim((const void *)k_p5s<T, PL, false, double>) ||
+				lim((const void *)k_p5s<T, PL, true, double>)) return e;
+		if (lim((const void *)k_p3g<T, PL>) || lim((const void *)k_p3g<T, PL, 1>)) return e;
 	}
-	return hipFuncSetAttribute((const void *)k_zfft<T, true, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		kMvLdsMax);
+	else {
+		if (lim((const void *)k_p1<T, PL>) || lim((const void *)k_p5<T, PL, false>) ||
+			lim((const void *)k_p5<T, PL, true>)) return e;
+		if constexpr (sizeof(T) == 4)
+			if (lim((const void *)k_p1<T, PL, double>) || lim((const void *)k_p5<T, PL, false, double>) ||
+				lim((const void *)k_p5<T, PL, true, double>)) return e;
+	}
+	if (lim((const void *)k_p3<T, PL>) || lim((const void *)k_p3<T, PL, 1>) || lim((const void *)k_zfft<T, false, PL>) ||
+		lim((const void *)k_zfft<T, true, PL>)) return e;
+	if constexpr (p3r_ok<T, PL>())
+		if (lim((const void *)k_p3r<T, PL>)) return e;
+	return hipSuccess;
 }
 
 template <class... Ps> struct PlanList {};

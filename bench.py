@@ -52,9 +52,12 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-10, help="tolerance of --solve-only (default 1e-10)")
     ap.add_argument("--m", type=float, nargs=2, default=[1.5, 0.01], metavar=("RE", "IM"),
                     help="refractive index of the sphere (default 1.5 0.01; the 640^3 workload uses 1.33 0.001)")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="timed plain FP64 BiCGStab solves (after one untimed), averaged for tts_fp64_s")
     ap.add_argument("--qmr-max-inner", type=int, default=2000,
-                    help="inner-iteration cap of QMR_CS with FP32 vectors (its inner solves stall near a 1e-3 true "
-                         "residual while the quasi-residual keeps falling, so an uncapped step runs on without gain)")
+                    help="inner-iteration cap of the mixed QMR_CS form forced to FP32 QMR_CS inner solves (they stall "
+                         "near a 1e-3 true residual while the quasi-residual keeps falling, so an uncapped step runs on "
+                         "without gain)")
     ap.add_argument("--qmr-max-inner-v64", type=int, default=3000,
                     help="the same cap for QMR_CS with FP64 vectors around FP32 transforms")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-RHS and adjoint-gradient workloads")
@@ -142,9 +145,26 @@ def main():
         pr.close()
         return
 
-    # ---- plain FP64 BiCGStab (reference solve) ----------------------------------------------------------------------
-    t64, cext64, r64 = run("bicgstab", A.PREC_FP64)
-    log(f"FP64 BiCGStab: {t64:.3f} s, {r64['niter']} iterations, {r64['nmatvec']} matvecs, Cext {cext64:.10g}")
+    # ---- the drop-in boundary on host vectors (adda_hip_matvec_host: ADDA's MatVec signature, PCIe both ways) ------
+    xh = op.download(pr.einc)
+    op.matvec_host(xh)
+    op.sync()
+    t = time.perf_counter()
+    nh = 10
+    for _ in range(nh):
+        op.matvec_host(xh)
+    t_host = (time.perf_counter() - t) / nh
+    log(f"matvec on host vectors (PCIe in and out): {t_host * 1e3:.2f} ms, device-resident FP64 {mv['fp64']['ms']:.3f} ms")
+
+    # ---- plain FP64 BiCGStab (reference solve): one untimed, then the mean of --fp64-steps -------------------------
+    run("bicgstab", A.PREC_FP64)
+    t64s = []
+    for _ in range(max(1, args.fp64_steps)):
+        t64, cext64, r64 = run("bicgstab", A.PREC_FP64)
+        t64s.append(t64)
+    t64 = float(np.mean(t64s))
+    log(f"FP64 BiCGStab: {t64:.3f} s (mean of {len(t64s)}: " + ", ".join(f"{t:.3f}" for t in t64s) +
+        f"), {r64['niter']} iterations, {r64['nmatvec']} matvecs, Cext {cext64:.10g}")
 
     # ---- timed steps: FP32-mixed BiCGStab time-to-solution ---------------------------------------------------------
     for _ in range(args.warmup):
@@ -166,17 +186,29 @@ def main():
     extra = {}
     if not args.no_qmr:
         tq64, cq64, rq64 = run("qmr", A.PREC_FP64)
-        tqm, cqm, rqm = run("qmr", A.PREC_MIXED, max_inner=args.qmr_max_inner)
+        # the mixed form with its automatic inner method: FP32 BiCGStab inner solves (FP32 QMR_CS loses its symmetric
+        # Lanczos basis on this operator, below), FP64 defect correction; a rejected correction finishes in FP64 QMR_CS
+        tqm, cqm, rqm = run("qmr", A.PREC_MIXED)
         extra["qmr"] = dict(tts_fp64_s=round(tq64, 4), tts_mixed_s=round(tqm, 4), speedup=round(tq64 / tqm, 3),
+                            inner_method="bicgstab (automatic for QMR_CS)",
                             converged_fp64=rq64["rc"] == 0, converged_mixed=rqm["rc"] == 0,
                             fp64_matvecs_mixed=rqm["nmatvec_fp64"],
                             iterations_fp64=rq64["niter"], iterations_mixed=rqm["niter"], outer_mixed=rqm["nouter"],
-                            true_resid_mixed=rqm["true_resid"], max_inner=args.qmr_max_inner,
+                            true_resid_mixed=rqm["true_resid"],
                             cext_rel_err=abs(cqm - cq64) / abs(cq64), cext_vs_bicgstab_fp64=abs(cq64 - cext64) / cext64)
         log(f"QMR_CS: FP64 {tq64:.3f} s ({rq64['niter']} it), mixed {tqm:.3f} s ({rqm['niter']} it, "
             f"{rqm['nouter']} outer, true residual {rqm['true_resid']:.2e}), Cext rel err {extra['qmr']['cext_rel_err']:.2e}")
-        # FP64 Krylov vectors and recurrences around FP32 transforms (the matvec's FFTs and D-matrix product in FP32)
-        tqv, cqv, rqv = run("qmr", A.PREC_MIXED_V64, max_inner=args.qmr_max_inner_v64)
+        # the same with the inner solves forced to FP32 QMR_CS, capped: the measured limit of that form
+        tqq, cqq, rqq = run("qmr", A.PREC_MIXED, max_inner=args.qmr_max_inner, inner_method="qmr")
+        extra["qmr"]["inner_qmr_cs"] = dict(
+            tts_s=round(tqq, 4), speedup=round(tq64 / tqq, 3), converged=rqq["rc"] == 0, iterations=rqq["niter"],
+            outer=rqq["nouter"], fp64_matvecs=rqq["nmatvec_fp64"], true_resid=rqq["true_resid"],
+            max_inner=args.qmr_max_inner, cext_rel_err=abs(cqq - cq64) / abs(cq64))
+        log(f"QMR_CS mixed with FP32 QMR_CS inner solves (cap {args.qmr_max_inner}): {tqq:.3f} s ({rqq['niter']} it, "
+            f"{rqq['nouter']} outer), speedup {tq64 / tqq:.2f}x")
+        # FP64 Krylov vectors and recurrences around FP32 transforms (the matvec's FFTs and D-matrix product in FP32),
+        # QMR_CS inner solves
+        tqv, cqv, rqv = run("qmr", A.PREC_MIXED_V64, max_inner=args.qmr_max_inner_v64, inner_method="qmr")
         extra["qmr"]["fp64_vectors_fp32_transforms"] = dict(
             tts_s=round(tqv, 4), speedup=round(tq64 / tqv, 3), converged=rqv["rc"] == 0, iterations=rqv["niter"],
             outer=rqv["nouter"], true_resid=rqv["true_resid"], max_inner=args.qmr_max_inner_v64,
@@ -190,6 +222,10 @@ def main():
         log(f"BiCGStab with FP64 vectors around FP32 transforms: {tbv:.3f} s ({rbv['niter']} it), speedup "
             f"{t64 / tbv:.2f}x")
 
+    extra["matvec_host_pcie"] = {
+        "what": "adda_hip_matvec_host (ADDA's MatVec on host vectors: 52.8 MB copied in and out over PCIe per call), "
+                "FP64, wall clock, mean of 10 calls", "ms": round(t_host * 1e3, 3),
+        "device_resident_ms": round(mv["fp64"]["ms"], 4), "ratio": round(t_host * 1e3 / mv["fp64"]["ms"], 2)}
     if not args.no_extra:
         extra["batched_rhs"] = bench_batched(A, args)
         extra["adjoint_gradient"] = bench_adjoint(A)
@@ -217,8 +253,12 @@ def main():
         oc.matvec_fft(dmat, pr.cc_sqrt, x)
         tc = time.perf_counter() - t
         cpu = {"value": round(1.0 / tc, 5), "unit": "matvecs/s", "cores": 1, "kind": "port",
-               "sample": f"one FP64 FFT matvec of this workload (grid {pr.grid[0]}^3) by the oracle (textbook FFT, "
-                         f"1 thread, {tc:.1f} s); the FP64 BiCGStab solve ({r64['nmatvec']} matvecs) extrapolates to "
+               "what": "the oracle's FP64 FFT matvec: a textbook recursive mixed-radix FFT over the full padded grid (no zero-plane "
+                       "skipping), not FFTW -- ADDA's FFTW build cannot be built without FFTW3",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+               "host_cpus_usable": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+               "sample": f"one FP64 FFT matvec of this workload (grid {pr.grid[0]}^3) by the oracle (1 thread, "
+                         f"{tc:.1f} s); the FP64 BiCGStab solve ({r64['nmatvec']} matvecs) extrapolates to "
                          f"{tc * r64['nmatvec']:.0f} s"}
         log(f"CPU baseline: {tc:.2f} s per matvec")
     pr.close()
@@ -231,7 +271,8 @@ def main():
                                f"m=1.5+0.01i, size parameter ~40, LDR, BiCGStab to 1e-10 (FP32-mixed); plain FP64 timed "
                                f"beside; QMR_CS in both forms under extra",
                    "dipoles": nd, "grid": list(pr.grid), "eps": 1e-10, "solver": "bicgstab"},
-        "tts_fp64_s": round(t64, 4), "tts_mixed_s": round(tmix, 4), "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
+        "tts_fp64_s": round(t64, 4), "tts_fp64_runs_s": [round(t, 4) for t in t64s], "tts_mixed_s": round(tmix, 4),
+        "tts_mixed_runs_s": [round(m[0], 4) for m in mixed], "speedup_mixed_vs_fp64": round(t64 / tmix, 3),
         "cext_fp64": cext64, "cext_mixed": cextmix, "cext_rel_err": err,
         "iterations": {"fp64": r64["niter"], "mixed_fp32": rmix["niter"], "mixed_outer": rmix["nouter"]},
         "matvecs_per_s": {"fp64": round(1e3 / mv["fp64"]["ms"], 1), "fp32": round(1e3 / mv["fp32"]["ms"], 1)},
@@ -240,6 +281,18 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(pass_index, cfg):

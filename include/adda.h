@@ -77,11 +77,19 @@ typedef struct {
 	/* iteration checkpoints (-chpoint, -chp_type, -chp_dir, -chp_load; src/param.c:412-417, src/iterative.c:165-318);
 	 * FP64 single-RHS solves, which then run the host form of the recurrences */
 	int chp_type;        /* enum adda_chp: when to save <chp_dir>/chp.0 */
-	double chp_time;     /* -chpoint: seconds after the start of the solve (> 0) at which to save (normal: then stop,
-	                      * regular: every chp_time seconds); <= 0 with ADDA_CHP_ALWAYS: save at the end only */
+	double chp_time;     /* -chpoint: seconds after program start (> 0) -- the load of this library -- at which to save
+	                      * (normal: then stop, regular: every chp_time seconds after the last save); <= 0 with
+	                      * ADDA_CHP_ALWAYS: save at the end only. Set-up and earlier solves count, as the reference's
+	                      * clock starts at program start (src/timing.c:102) */
 	const char *chp_dir; /* checkpoint directory (NULL: "chpoint", the reference's default) */
 	int chp_load;        /* 1: resume from <chp_dir>/chp.0 instead of starting from the initial field */
 	const char *run_dir; /* output directory named in chp.log (NULL: ".") */
+	int inner_method;    /* mixed forms: Krylov method of the FP32 inner solves, enum adda_iter + 1. 0 (auto): BiCGStab
+	                      * for the complex-symmetric Lanczos methods (QMR_CS, BiCG_CS, CSYM, QMR_CS_2) -- they rely
+	                      * on A^T = A, which the FP32 operator meets only to rounding (about 1e-7), and on a large
+	                      * particle their inner solves stall while the recurrence residual keeps falling --, the
+	                      * requested method otherwise. The outer FP64 residual decides every correction, and the FP64
+	                      * finishing solve (a rejected correction) runs the requested method. */
 } adda_solver_opts;
 
 /* checkpoint types, ADDA's enum chpoint (src/const.h:313-318) */
@@ -96,6 +104,7 @@ typedef struct {
 	int nouter;          /* outer correction steps (mixed mode) */
 	double resid;        /* final relative residual of the recurrence (FP64 mode) or of the true residual (mixed) */
 	double true_resid;   /* true ||b-Ax||/||b|| when recalculated (recalc_resid or mixed), else -1 */
+	int chp_saved;       /* 1 when the solve wrote an iteration checkpoint */
 } adda_solve_stats;
 
 /* IterativeSolver (src/iterative.c:1398-1530) on device vectors: solves (I+S.D.S).x = S.Einc, returns x and the

@@ -43,12 +43,14 @@ const char *adda_hip_last_error(void);
 int adda_hip_device_count(void);
 
 /* fftFit (src/fft.c:528-562) under the 2/3/5 rule that ADDA applies for clAmdFft (CLFFT_AMD, src/fft.c:542-545):
- * the first x' >= x that is even, divisible by divis and has no prime factor other than 2, 3, 5. */
+ * the first x' >= x that is even, divisible by divis and has no prime factor other than 2, 3, 5. Negative (error set)
+ * when divis has another prime factor, so that no such size exists. */
 int adda_hip_fftfit(int x, int divis);
 /* FFT grid size chosen for speed (no reference counterpart; ADDA itself always takes fftFit): the smallest power of two
  * >= x divisible by divis when it is at most 25 % larger than adda_hip_fftfit(x, divis), else that size. The
  * convolution is exact for any grid >= 2*box, and power-of-two lines run on the radix-16/8 plans: a 108-point y-grid
- * (2^2*3^3, four radix-3/4 stages) costs twice as much per point as 128. Results differ only by rounding. */
+ * (2^2*3^3, four radix-3/4 stages) costs twice as much per point as 128. Results differ only by rounding. A divis
+ * with an odd factor gives adda_hip_fftfit's size; negative where adda_hip_fftfit is. */
 int adda_hip_fastgrid(int x, int divis);
 
 /* Creates the operator of one particle on `device` and builds its D-matrix on the GPU in FP64.

@@ -55,6 +55,12 @@ def test_fastgrid_rule():
         assert g >= x and g % 2 == 0
         assert g == n or (g & (g - 1) == 0 and 4 * g <= 5 * n), (x, n, g)
     assert adda_amd.fast_grid((64, 52, 32)) == (128, 128, 64)
+    # divisors (ADDA passes process counts): an odd factor rules out the power of two, a factor other than 2, 3, 5
+    # any size (an error, not an endless search)
+    assert L.adda_hip_fastgrid(104, 3) == L.adda_hip_fftfit(104, 3) == 108
+    assert L.adda_hip_fastgrid(100, 5) == L.adda_hip_fftfit(100, 5) == 100
+    assert L.adda_hip_fastgrid(120, 4) == 128 and L.adda_hip_fftfit(120, 4) == 120
+    assert L.adda_hip_fftfit(100, 7) < 0 and L.adda_hip_fastgrid(100, 7) < 0
 
 
 @pytest.mark.parametrize("shape,grid,ay,az,nd", [("sphere", 16, 1, 1, 2176), ("ellipsoid", 64, 0.8, 0.5, None),

@@ -96,3 +96,27 @@ def test_batched_workload_matvec_vs_oracle(hipdev):
         y_or, _ = c.matvec_fft(d, ccs, xs[k])
         assert _rel(yb[hipdev.F64][k], y_or) < 1e-12, k
         assert _rel(yb[hipdev.F32][k], y_or) < 2e-5, k
+
+
+def test_grid256_converged_solves(hipdev):
+    """The bench workload solved to ||b - Ax||/||b|| <= 1e-10 (true FP64 residual) by the plain FP64 BiCGStab and
+    QMR_CS and by the FP32-mixed forms (FP32 inner iterations, FP64 defect correction): BiCGStab, and QMR_CS with its
+    automatic inner method. Every mixed solve ends below 1e-10 on the true residual, and its Cext is within 1e-6 of
+    the FP64 BiCGStab solve's (the 1e-10 solves agree far closer, about 1e-10)."""
+    pos, mat, box = hipdev.make_particle("sphere", 128)
+    nd = len(pos)
+    dpl = 2 * np.pi * (nd / (np.pi / 6)) ** (1 / 3) / 80.0
+    pr = hipdev.Problem(pos, box, 1.5 + 0.01j, dpl=dpl, fp32=True)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        r64 = pr.solve("bicgstab", eps=1e-10, precision=hipdev.PREC_FP64, recalc_resid=True)
+        c64 = pr.cext()
+        assert r64["rc"] == 0 and r64["true_resid"] <= 1e-10 * 1.05
+        q64 = pr.solve("qmr", eps=1e-10, precision=hipdev.PREC_FP64, recalc_resid=True)
+        assert q64["rc"] == 0 and abs(pr.cext() - c64) / c64 < 1e-8
+        for method in ("bicgstab", "qmr"):
+            r = pr.solve(method, eps=1e-10, precision=hipdev.PREC_MIXED)
+            assert r["rc"] == 0 and r["true_resid"] <= 1e-10, (method, r)
+            assert abs(pr.cext() - c64) / c64 < 1e-6, method
+    finally:
+        pr.close()

@@ -123,6 +123,58 @@ def test_matvec_fp64_vectors_fp32_transform(hipdev, shape, box, m, ay, az, kw, h
         op.close()
 
 
+@pytest.mark.parametrize("her", [False, True])
+def test_matvec_any_dipole_order(hipdev, her):
+    """The gather of P1/P5 takes rows whose dipoles are consecutive at consecutive cells (ADDA's z, y, x order)
+    arithmetically and every other row through the cell map. A spherical shell (rows through the hollow core have a
+    gap) with three materials, its dipoles in ADDA's order and in a random order: both match the oracle (FP64 1e-12,
+    FP32 2e-5), and the random order gives the permuted product of the ordered one bit for bit."""
+    pos, box = make_shape("sphere", 20)
+    ctr = (np.array(box) - 1) / 2
+    pos = np.ascontiguousarray(pos[((pos - ctr) ** 2).sum(1) >= 5.0 ** 2])
+    nd = len(pos)
+    mat = (pos[:, 0] + pos[:, 1] // 3 + pos[:, 2] // 5) % 3
+    m = [1.5 + 0.01j, 1.2 + 0.1j, 1.33]
+    perm = np.random.default_rng(11).permutation(nd)
+    x = np.random.default_rng(12).standard_normal(3 * nd) * (1 - 0.4j)
+    xp = x.reshape(nd, 3)[perm].reshape(-1)
+    ys = {}
+    for name, p, mt, xv in (("ordered", pos, mat, x), ("shuffled", pos[perm], mat[perm], xp)):
+        c = Case(np.ascontiguousarray(p), box, m, mat=np.ascontiguousarray(mt))
+        cc, ccs = c.cc(PROP_Z, POL_Y)
+        op = _op(hipdev, c, fp32=True)
+        try:
+            op.set_cc_sqrt(ccs)
+            y64, _ = op.matvec(xv, her=her)
+            y32, _ = op.matvec(xv, her=her, prec=hipdev.F32)
+            ref = c.matvec_fft(c.dmatrix(), ccs, xv, her=her)[0]
+            assert _rel(y64, ref) < 1e-12, name
+            assert _rel(y32, ref) < 2e-5, name
+            ys[name] = (y64, y32)
+        finally:
+            op.close()
+    for i in range(2):
+        assert np.array_equal(ys["shuffled"][i].reshape(nd, 3), ys["ordered"][i].reshape(nd, 3)[perm])
+
+
+def test_matvec_host_vectors(hipdev):
+    """adda_hip_matvec_host (ADDA's MatVec signature on host vectors; the operator keeps its device copies between
+    calls) equals the device-resident matvec bit for bit, ||y||^2 included, over repeated calls and both her."""
+    c = _case("ellipsoid", 16, 1.33 + 0.001j, 0.8, 0.5)
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    op = _op(hipdev, c)
+    try:
+        op.set_cc_sqrt(ccs)
+        rng = np.random.default_rng(13)
+        for her in (False, True, False):
+            x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+            yh, ih = op.matvec_host(x, her=her)
+            yd, idv = op.matvec(x, her=her)
+            assert np.array_equal(yh, yd) and ih == idv
+    finally:
+        op.close()
+
+
 def test_vector_kernels(hipdev):
     c = _case("sphere", 8, 1.5)
     op = _op(hipdev, c)

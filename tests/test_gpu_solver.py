@@ -152,18 +152,21 @@ def test_cm_and_multimaterial(hipdev):
         pr.close()
 
 
+@pytest.mark.parametrize("inner", ["auto", "same"])
 @pytest.mark.parametrize("v64", [False, True])
 @pytest.mark.parametrize("max_inner", [0, 40])
 @pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr", "bcgs2", "qmr2"])
-def test_mixed_precision_c1(hipdev, method, max_inner, v64):
+def test_mixed_precision_c1(hipdev, method, max_inner, v64, inner):
     """FP32 inner Krylov (or FP64 inner Krylov around FP32 transforms, v64) + FP64 defect correction reaches the FP64
     answer: true residual <= 1e-10 and Cext within 1e-6 (the reference value 135.0448603 is itself converged to
-    1e-10)."""
+    1e-10). inner: the automatic inner method (BiCGStab for the complex-symmetric Lanczos methods) or the requested
+    method itself."""
     pr = _problem(hipdev, "sphere16", 1.5, 15, fp32=True)
     try:
         pr.set_polarization(PROP_Z, POL_Y)
         prec = hipdev.PREC_MIXED_V64 if v64 else hipdev.PREC_MIXED
-        r = pr.solve(method, eps=1e-10, precision=prec, max_inner=max_inner)
+        r = pr.solve(method, eps=1e-10, precision=prec, max_inner=max_inner,
+                     inner_method=None if inner == "auto" else method)
         assert r["true_resid"] <= 1e-10, r
         assert abs(pr.cext() - 135.0448603) / 135.0448603 < 1e-6
         assert r["nmatvec_fp64"] >= 1 and r["nouter"] >= 2
@@ -378,8 +381,8 @@ def test_cli_checkpoint_resume(hipdev, tmp_path):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "a-dda_amd", "adda_hip")
     shutil.copytree(os.path.dirname(G.path("chp_qmr", "chp.0")), tmp_path / "chp")
-    base = [exe, "-shape", "read", G.path("geom", "box4.geom"), "-dpl", "10", "-m", "1.5", "0.1", "-iter", "qmr",
-            "-eps", "10"]
+    base = [exe, "-shape", "read", G.path("geom", "box4.geom"), "-sym", "enf", "-dpl", "10", "-m", "1.5", "0.1",
+            "-iter", "qmr", "-eps", "10"]
     subprocess.run(base + ["-chp_load", "-chp_dir", str(tmp_path / "chp"), "-dir", str(tmp_path / "cont")],
                    check=True, capture_output=True, timeout=120)
     g = G.crosssec("chp_qmr", "Y")
@@ -395,6 +398,12 @@ def test_cli_checkpoint_resume(hipdev, tmp_path):
     with open(tmp_path / "cont2" / "CrossSec-Y") as f:
         b = {l.split("=")[0].strip(): float(l.split("=")[1]) for l in f}
     assert G.numeric_close(b["Cext"], g["Cext"], 1e-14, 1e-8)[0]
+    with open(tmp_path / "part" / "log") as f:
+        assert "Checkpoint (iteration) saved" in f.read()
+    # without the symmetry that computes the internal field once, checkpoints are refused (src/param.c:2171-2174)
+    p = subprocess.run(base[:4] + base[6:] + ["-chpoint", "0s", "-dir", str(tmp_path / "nosym")], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode != 0 and "single incident polarization" in p.stderr
 
 
 def test_chp_regular_type_continues(hipdev, tmp_path):
@@ -411,5 +420,28 @@ def test_chp_regular_type_continues(hipdev, tmp_path):
         assert pr.cext() == c0
         h, _, _ = hipdev.chp_read(str(tmp_path), "qmr", 3 * pr.nd)
         assert h["niter"] == r["niter"] + 1 and h["counter"] == 0
+    finally:
+        pr.close()
+
+
+def test_chp_always_bicgstab_resume_tighter(hipdev, tmp_path):
+    """BiCGStab with checkpoint type always run to convergence at several tolerances, then resumed to 1e-10: each
+    resumed solve reaches the uninterrupted 1e-10 answer. Under type always the half-step convergence shortcut is
+    skipped (src/iterative.c:702-707), so the final checkpoint holds a consistent r and ro_old; with the shortcut it
+    would hold x advanced by alpha*p beside the previous r."""
+    pr = _chp_problem(hipdev)
+    try:
+        r0 = pr.solve("bicgstab", eps=1e-10, host_scalars=True)
+        c0 = pr.cext()
+        assert r0["rc"] == 0
+        for k in (3, 4, 5, 6, 7, 8):
+            d = str(tmp_path / f"e{k}")
+            r = pr.solve("bicgstab", eps=10.0 ** -k, chp_type=hipdev.CHP_ALWAYS, chp_dir=d)
+            assert r["rc"] == 0 and r["resid"] < 10.0 ** -k, (k, r)
+            h, _, _ = hipdev.chp_read(d, "bicgstab", 3 * pr.nd)
+            assert h["niter"] == r["niter"] + 1, k
+            r = pr.solve("bicgstab", eps=1e-10, chp_load=True, chp_dir=d)
+            assert r["rc"] == 0 and r["resid"] < 1e-10, (k, r)
+            assert abs(pr.cext() - c0) <= 1e-8 * abs(c0), (k, pr.cext(), c0)
     finally:
         pr.close()

@@ -641,9 +641,12 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 // Per-RHS strides of the work buffers (complex elements)
 static long xa_stride(const adda_hip_op *op) { return (long)3 * op->bZ * op->bY * op->gX; }
 static long xb_stride(const adda_hip_op *op) { return (long)3 * op->gZ * op->bY * op->gX; }
-static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, const double *gate)
+static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, const double *gate, int pair = 0,
+	int ntile = 0)
 {
 	Bq b;
+	b.pair = pair;
+	b.ntile = ntile;
 	b.K = K;
 	b.vs = vs;
 	b.xs = xs;
@@ -712,13 +715,15 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	// many waves per CU (the tile height is free here, it does not change the Xa layout)
 	MvCfg cd = c;
 	if (dmode && cd.LY > 2 && (3 * (cd.LY / 2) * (op->gX / c.qx.rmax)) % 64 == 0) cd.LY /= 2;  // whole waves only
-	const int nyt = (op->bY + cd.LY - 1) / cd.LY, nblk5 = op->bZ * nyt;
+	const int nyt = (op->bY + cd.LY - 1) / cd.LY;
+	// tiles 2p and 2p + 1 (which share the cache lines where their Xa runs meet) go 8 blocks apart, on one XCD
+	const int pair = nyt % 2 == 0, nblk5 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
 	constexpr bool same = std::is_same_v<T, TV>;
 	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
 	auto k = (Fn)c.kx.k[dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
-		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate));
+		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate, pair, op->bZ * nyt));
 	return nblk5;
 }
 
@@ -1404,7 +1409,7 @@ static int reserve(adda_hip_op *op, int K)
 	HIPCHK(hipMalloc(&op->Xa, (size_t)K * xa_stride(op) * sizeof(cx<double>)));
 	HIPCHK(hipMalloc(&op->Xb, (size_t)K * xb_stride(op) * sizeof(cx<double>)));
 	// partials: field sums, K rows of gather blocks (LY >= 1), K rows of up to 3*kRedBlocks vector-kernel blocks
-	size_t per = (size_t)3 * op->bZ * op->bY;
+	size_t per = (size_t)3 * (op->bZ * op->bY + 16);  // P5 tiles, padded to whole groups of 16 for the paired launch
 	if (per < (size_t)3 * kRedBlocks) per = (size_t)3 * kRedBlocks;
 	op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);
 	if ((size_t)K * per > op->npartials) op->npartials = (size_t)K * per;
@@ -2335,7 +2340,21 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 	cx<T> *a = nullptr, *r = nullptr;
 	HIPCHK(hipMalloc(&a, nv * es));
 	HIPCHK(hipMalloc(&r, nv * es));
-	HIPCHK(hipMemsetAsync(a, 0, nv * es, op->stream));
+	{
+		// pseudo-random arguments in [-1, 1): timed on zeros, the passes ran faster (FP64 P3 at 256^3: 254 against
+		// 280 us on a plane wave), so zeros would mislead the comparison of candidates
+		std::vector<cx<T>> h(nv);
+		unsigned long long st = 0x9E3779B97F4A7C15ull;
+		auto next = [&st] {
+			st = st * 6364136223846793005ull + 1442695040888963407ull;
+			return (T)((double)(st >> 11) * (1.0 / 4503599627370496.0) - 1.0);
+		};
+		for (auto &v : h) {
+			v.x = next();
+			v.y = next();
+		}
+		HIPCHK(hipMemcpy(a, h.data(), nv * es, hipMemcpyHostToDevice));
+	}
 	MvCfg best = op->cfg[pi];
 	hipEvent_t ev[6];
 	for (auto &e : ev) HIPCHK(hipEventCreate(&e));

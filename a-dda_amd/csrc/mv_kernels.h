@@ -18,7 +18,16 @@ struct Bq {
 	long xs, xs2;        // strides of the work buffers the pass reads / writes
 	int cs;              // stride of the cc_sqrt tables (complex elements)
 	const double *gate;  // RHS k is skipped when gate[k*DK_NSLOTS + DK_STATUS] != 0 (null: never)
+	int pair = 0;        // P5: launch-order remap of the tiles (pair_tile), the launch padded to whole groups of 16
+	int ntile = 0;       // P5 with pair: the number of real tiles (the padding blocks write zero partial sums)
 };
+// Tile of the b-th block of a launch whose tiles 2p and 2p + 1 sit 8 blocks apart (groups of 16 blocks: the even
+// member of 8 pairs, then the odd ones). Blocks are dealt to the 8 XCDs in turn, so a pair shares one XCD and its L2.
+// A bijection of [0, 16*ceil(n/16)); tiles >= n are padding.
+__device__ __forceinline__ int pair_tile(int b)
+{
+	return 2 * ((b >> 4) * 8 + (b & 7)) + ((b >> 3) & 1);
+}
 static __device__ __forceinline__ bool skip_rhs(const Bq &b, int k)
 {
 	return b.gate && b.gate[(size_t)k * DK_NSLOTS + DK_STATUS] != 0;
@@ -94,7 +103,7 @@ struct alignas(16) RowDesc {
 
 // ==================================================================================================================
 // kernels: matvec pipeline, layout Xa[z<boxZ][x<gridX][y<boxY][c], Xb[z<gridZ][x<gridX][y<boxY][c],
-// D[zt<DsizeZ][x<gridX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
+// D[zt<DsizeZ][x<DsizeX][yt<DsizeY][6]. Every kernel loads its twiddle table into LDS first.
 
 template <typename T>
 __device__ __forceinline__ void load_tw(cx<T> *tws, const cx<T> *__restrict__ twg, int N)
@@ -592,8 +601,16 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
 	double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
+	if (bq.pair && tile >= bq.ntile) {  // padding of the paired launch: zero partial sums, nothing else
+		if (partials && threadIdx.x == 0) {
+			const int nr = DOT ? 3 : 1;
+			const size_t nb = gridDim.x / bq.K;
+			for (int r = 0; r < nr; r++) partials[(size_t)kq * nr * nb + r * nb + tile] = 0;
+		}
+		return;
+	}
 	Xa += kq * bq.xs;
 	arg += kq * bq.vs;
 	res += kq * bq.vs;
@@ -725,15 +742,24 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 // accumulated in another order. The last stage leaves its outputs in registers; then the dipole indices, materials,
 // arguments, coupling constants (and dot weights) of all of them are found in phases whose loads are in flight together
 // -- one round trip per thread for dense rows of one material -- and nothing is held in registers through the
-// transform (which would cost a resident block per CU).
+// transform. (Measured: loading the arguments before the transform, beside its own loads, took FP32 P5 from 44 to
+// 47 us -- 99 registers, or a spill under a budget of five waves per SIMD.)
 template <typename T, class PL, bool DOT, typename TV = T>
 __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
 	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const RowDesc *__restrict__ rows,
 	const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX,
 	int bY, int LY, int her, double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
+	if (bq.pair && tile >= bq.ntile) {  // padding of the paired launch: zero partial sums, nothing else
+		if (partials && threadIdx.x == 0) {
+			const int nr = DOT ? 3 : 1;
+			const size_t nb = gridDim.x / bq.K;
+			for (int r = 0; r < nr; r++) partials[(size_t)kq * nr * nb + r * nb + tile] = 0;
+		}
+		return;
+	}
 	Xa += kq * bq.xs;
 	arg += kq * bq.vs;
 	res += kq * bq.vs;
@@ -760,6 +786,20 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 	const int *cm = cellmap + row * bX;
 	const bool onemat = bq.cs == 3;
 	constexpr int NS = PL::NSLOT;
+	// phases whose loads are in flight together (loaded one output at a time, each would wait on the last); empty cells
+	// load dipole 0 and discard it (a conditional load would get a basic block and a wait of its own)
+	int dd[NS];
+	cx<TV> av[NS];
+	auto find_dipoles = [&] {
+		if (rd.dense) PL::template for_last<true>(tB, [&](int sl, int x) {
+			const unsigned o = (unsigned)(x - rd.x0);
+			dd[sl] = x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
+		});
+		else PL::template for_last<true>(tB, [&](int sl, int x) { dd[sl] = x < bX ? cm[x] : -1; });
+	};
+	auto load_args = [&] {
+		PL::template for_last<true>(tB, [&](int sl, int) { av[sl] = arg[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c]; });
+	};
 	double s = 0, dr = 0, di = 0;
 	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
 	const size_t xs = (size_t)3 * bY;
@@ -770,21 +810,15 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 	PL::template fft<T, true, false, true, true, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
 		st);
 	if (actB) {
-		// phases whose loads are in flight together (loaded one output at a time, each would wait on the last); empty
-		// cells load dipole 0 and discard it (a conditional load would get a basic block and a wait of its own)
-		int dd[NS], mm[NS];
-		cx<TV> av[NS], wv[NS];
+		int mm[NS];
+		cx<TV> wv[NS];
 		cx<T> cv[NS];
-		if (rd.dense) PL::template for_last<true>(tB, [&](int sl, int x) {
-			const unsigned o = (unsigned)(x - rd.x0);
-			dd[sl] = x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
-		});
-		else PL::template for_last<true>(tB, [&](int sl, int x) { dd[sl] = x < bX ? cm[x] : -1; });
+		find_dipoles();
 		if (!onemat) PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 3 * mat[dd[sl] > 0 ? dd[sl] : 0]; });
 		else PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 0; });
+		load_args();
 		PL::template for_last<true>(tB, [&](int sl, int) {
 			const size_t i = 3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c;
-			av[sl] = arg[i];
 			cv[sl] = ccs[mm[sl] + c];
 			// weights through one global pointer (dmode 2 re-reads the argument from cache): a select between a
 			// register copy and a load would go through scratch memory

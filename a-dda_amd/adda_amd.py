@@ -102,6 +102,7 @@ def hip():
     L.adda_hip_matvec_batch.argtypes = [P, I, I, P, P, P, P, I]
     L.adda_hip_lincomb_batch.argtypes = [P, I, I, P, P, P, P, P, P, P, P, I, P]
     L.adda_hip_dot_batch.argtypes = [P, I, I, P, P, P, I, P, P]
+    L.adda_hip_dot_noalloc.argtypes = [P, I, I, P, P, I, P]
     L.adda_hip_mult_mat_batch.argtypes = [P, I, I, P, P]
     L.adda_hip_profile_matvec_batch.argtypes = [P, I, I, P, P, I, P, P]
     L.adda_hip_matvec_bytes_batch.argtypes = [P, I, I, I]

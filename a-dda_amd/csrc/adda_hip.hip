@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include <functional>
 
 using namespace adda;
@@ -401,6 +402,27 @@ __global__ void __launch_bounds__(256) k_dot(const cx<T> *a, const cx<T> *b, int
 		partials[blockIdx.x] = s0;
 		partials[nblk + blockIdx.x] = s1;
 		partials[2 * nblk + blockIdx.x] = s2;
+	}
+}
+
+// adda_hip_dot_noalloc's finish: block k reduces the three partial rows of RHS k exactly as k_reduce_final does (so
+// the results equal adda_hip_dot_batch's bit for bit) and writes {res.re, res.im, norm2} to out[3k..3k+2]: the
+// product and 0 for ADDA_DOT_CONJ / ADDA_DOT_BILIN, sum a*a and ||a||^2 for the self modes (src/linalg.c:127-144).
+__global__ void __launch_bounds__(256) k_dot_final(const double *partials, int nblk, int mode, double *out)
+{
+	__shared__ double red_lds[512];
+	const int k = blockIdx.x;
+	double w[3];
+	for (int r = 0; r < 3; r++) {
+		double s = 0;
+		for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partials[((size_t)k * 3 + r) * nblk + i];
+		w[r] = block_sum<double>(s, red_lds);  // total on thread 0; block_sum begins with a barrier
+	}
+	if (threadIdx.x == 0) {
+		const bool two = mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN;
+		out[3 * k] = two ? w[0] : w[0] - w[1];
+		out[3 * k + 1] = two ? w[1] : 2 * w[2];
+		out[3 * k + 2] = two ? 0.0 : w[0] + w[1];
 	}
 }
 
@@ -1422,11 +1444,13 @@ static int reserve(adda_hip_op *op, int K)
 	return 0;
 }
 
+static int scratch_setup(int dev);
+
 static int op_build(adda_hip_op *op, const unsigned short *position, const int *material)
 {
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
-	if (set_lds_limits()) return -1;
+	if (set_lds_limits() || scratch_setup(op->dev)) return -1;
 	const int bX = op->bX, bY = op->bY, bZ = op->bZ, gX = op->gX, gY = op->gY, gZ = op->gZ;
 	if (!make_plan(gX, op->px) || !make_plan(gY, op->py) || !make_plan(gZ, op->pz))
 		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
@@ -1844,6 +1868,57 @@ extern "C" int adda_hip_dot_batch(adda_hip_op *op, int prec, int K, const void *
 			if (norm2) norm2[k] = w[0] + w[1];
 		}
 	}
+	return 0;
+}
+
+// ---- the dot product that allocates nothing ------------------------------------------------------------------------
+// Its partial sums live in `scratch`, a file-scope device array (each GPU holds its own copy for the life of the
+// process). In host code the array's name is not a device address: every use goes through the address that
+// hipGetSymbolAddress returned for the device (scratch_addr, set when an operator is created on that device).
+static constexpr size_t kScratchDoubles = (size_t)3 * kRedBlocks * 16;  // 768 KiB: 16 RHS at the most blocks
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static constexpr int kMaxDevices = 64;
+static double *scratch_addr[kMaxDevices] = {};
+static std::mutex scratch_mu;
+
+// Called with `dev` current (operator creation): finds this device's copy of `scratch` once.
+static int scratch_setup(int dev)
+{
+	if (dev < 0 || dev >= kMaxDevices) return ERR("device %d beyond the %d the scratch table covers", dev, kMaxDevices);
+	std::lock_guard<std::mutex> lock(scratch_mu);
+	if (scratch_addr[dev]) return 0;
+	void *p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || !p) return ERR("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+	scratch_addr[dev] = (double *)p;
+	return 0;
+}
+
+extern "C" int adda_hip_dot_noalloc(adda_hip_op *op, int prec, int K, const void *a, const void *b, int mode,
+	double *d_out)
+{
+	if (!op || !a || !d_out) return ERR("null argument");
+	if ((mode == ADDA_DOT_CONJ || mode == ADDA_DOT_BILIN) && !b) return ERR("dot mode %d needs two vectors", mode);
+	if (mode < ADDA_DOT_CONJ || mode > ADDA_DOT_NORM2) return ERR("unknown dot mode %d", mode);
+	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
+	const size_t n = 3 * op->nd;
+	const int nb = nblocks_red(n);
+	if ((size_t)3 * nb * K > kScratchDoubles)
+		return ERR("adda_hip_dot_noalloc: %d right-hand sides of %d blocks need %zu doubles of scratch; it holds %zu",
+			K, nb, (size_t)3 * nb * K, kScratchDoubles);
+	double *scr = op->dev >= 0 && op->dev < kMaxDevices ? scratch_addr[op->dev] : nullptr;
+	if (!scr) return ERR("no scratch address for device %d", op->dev);
+	HIPCHK(hipSetDevice(op->dev));
+	const long vs = K > 1 ? (long)op->vs : 0;
+	// every block of k_dot writes its own partial slots, so nothing in scratch needs clearing first
+	if (prec == ADDA_F64)
+		hipLaunchKernelGGL(k_dot<double>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<double> *)a,
+			(const cx<double> *)b, mode, n, vs, scr, nb, nullptr);
+	else
+		hipLaunchKernelGGL(k_dot<float>, dim3(nb, K), dim3(256), 0, op->stream, (const cx<float> *)a,
+			(const cx<float> *)b, mode, n, vs, scr, nb, nullptr);
+	hipLaunchKernelGGL(k_dot_final, dim3(K), dim3(256), 0, op->stream, (const double *)scr, nb, mode, d_out);
+	HIPCHK(hipGetLastError());
 	return 0;
 }
 

@@ -203,6 +203,15 @@ int adda_hip_lincomb_batch(adda_hip_op *op, int prec, int K, const void *gate_st
 	double *norm2); /* coefficients and norm2: [K] */
 int adda_hip_dot_batch(adda_hip_op *op, int prec, int K, const void *gate_state, const void *a, const void *b,
 	int mode, adda_cplx *res, double *norm2); /* res, norm2: [K] */
+/* adda_hip_dot_batch without a gate, for callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc. No counterpart in the reference. It allocates nothing, copies nothing to the
+ * host and does not synchronise: the results go to the caller's device buffer d_out, [K][3] doubles {re, im, norm2}
+ * (the product and 0 for ADDA_DOT_CONJ / ADDA_DOT_BILIN; sum a*a and ||a||^2 for the self modes), equal bit for bit to
+ * adda_hip_dot_batch's, on the operator's stream. The partial sums use a static device array of 3*2048*16 doubles
+ * (768 KiB per GPU, held until the process ends): 3*nb*K doubles must fit, nb = min(2048, ceil(3*nd/256)) -- 16 RHS
+ * at the most blocks; larger requests fail with an error before anything is launched. Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other streams or threads) race. */
+int adda_hip_dot_noalloc(adda_hip_op *op, int prec, int K, const void *a, const void *b, int mode, double *d_out);
 int adda_hip_mult_mat_batch(adda_hip_op *op, int prec, int K, void *out, const void *in);
 int adda_hip_profile_matvec_batch(adda_hip_op *op, int prec, int K, const void *argvec, void *resultvec, int nrep,
 	double *pass_ms, double *total_ms);

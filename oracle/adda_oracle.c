@@ -13,6 +13,9 @@
  *   or_couple_constant   - sample log KAT (CoupleConstant 0.005259037197+1.843854148e-05i); CM closed form
  *   or_iterative_solver  - converged Cext/Cabs/Mueller/internal fields vs the reference SPARSE build (golden files)
  *   or_ext/abs_cross, or_eplane_yz, or_mueller_yz - sample golden CrossSec-Y/mueller + SPARSE goldens
+ *   or_refl_img          - closed form of the image dipole's field at hand-computed offsets; SPARSE -surf goldens
+ *   or_init_rmatrix, surface MatVec - FFT MatVec == direct sum with the reflected term; A symmetric; SPARSE -surf
+ *   or_generate_b_surf, or_ext_cross_gen - SPARSE -surf goldens (from above and below, finite and perfect reflector)
  */
 #include "adda_oracle.h"
 
@@ -171,12 +174,77 @@ void or_init_dmatrix(const or_grid *g, or_cplx *dmat)
 }
 
 /*====================================================================================================================*/
+/* ReflTerm_img via vCopyIntRealShift, ReflParams and ReflTerm_core: src/interaction.c:1081-1153. The image of the
+ * source lies at the mirror height, so the separation is (i, j, ksum + zsum_shift); the result is InterTerm_core's
+ * tensor times rcn with the xz, yz and zz components negated. */
+void or_refl_img(int i, int j, int ksum, double zsum_shift, or_cplx rcn, double wavenum, double gridspace,
+	or_cplx res[6])
+{
+	double q[3] = {i, j, ksum + zsum_shift};
+	double rn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+	for (int c = 0; c < 3; c++) q[c] /= rn;
+	double rr = rn * gridspace;
+	double invr3 = 1 / (rr * rr * rr);
+	double kr = wavenum * rr, kr2 = kr * kr;
+	double qmunu[6] = {q[0] * q[0], q[0] * q[1], q[0] * q[2], q[1] * q[1], q[1] * q[2], q[2] * q[2]};
+	double t1 = 3 - kr2, t2 = -3 * kr, t3 = kr2 - 1;
+	dcx scale = C(rcn) * invr3 * (cos(kr) + I * sin(kr));
+	for (int c = 0; c < 6; c++) {
+		dcx r;
+		if (c == 0 || c == 3 || c == 5) r = ((t1 * qmunu[c] + t3) + I * (kr + t2 * qmunu[c])) * scale;
+		else r = (t1 + I * t2) * qmunu[c] * scale;
+		if (c == 2 || c == 4 || c == 5) r = -r;
+		res[c] = O(r);
+	}
+}
+
+static size_t rsize_y(const or_grid *g) { return g->reduced_fft ? (size_t)g->gridY / 2 + 1 : (size_t)g->gridY; }
+
+size_t or_rmatrix_size(const or_grid *g) { return 6 * (size_t)g->gridX * g->gridZ * rsize_y(g); }
+
+/* InitRmatrix: src/fft.c:819-891. GR is placed on the periodic grid for |i| < boxX, |j| < boxY (the y mirror of
+ * 864-873 is GR's parity in y, which direct evaluation reproduces) and height sums 0 <= k <= 2*boxZ-2 (not periodic:
+ * gridZ >= 2*boxZ), forward 3D FFT, stored as -GR~/Ngrid with IndexRmatrix_mv's layout. */
+void or_init_rmatrix(const or_grid *g, or_cplx *rmat)
+{
+	const or_surface *sf = g->surf;
+	size_t gx = g->gridX, gy = g->gridY, gz = g->gridZ, n = gx * gy * gz, ry = rsize_y(g);
+	double invN = 1.0 / (double)n;
+	or_cplx *gt = xmalloc(6 * n * sizeof(or_cplx));
+	for (int k = 0; k <= 2 * g->boxZ - 2; k++)
+		for (int j = 1 - g->boxY; j < g->boxY; j++)
+			for (int i = 1 - g->boxX; i < g->boxX; i++) {
+				or_cplx r[6];
+				or_refl_img(i, j, k, sf->zsum_shift, sf->rcn, g->wavenum, g->gridspace, r);
+				size_t x = (size_t)((i + (int)gx) % (int)gx), y = (size_t)((j + (int)gy) % (int)gy), z = (size_t)k;
+				for (int c = 0; c < 6; c++) gt[c * n + (z * gy + y) * gx + x] = r[c];
+			}
+	for (int c = 0; c < 6; c++) {
+		or_cplx *a = gt + c * n;
+		fft3d(a, gx, gy, gz, -1);
+		for (size_t x = 0; x < gx; x++)
+			for (size_t z = 0; z < gz; z++)
+				for (size_t y = 0; y < ry; y++)
+					rmat[6 * ((x * gz + z) * ry + y) + c] = O(-invN * C(a[(z * gy + y) * gx + x]));
+	}
+	free(gt);
+}
+
+/*====================================================================================================================*/
 /* cSymMatrVec: src/cmplx.h:369-376 */
 static inline void sym_mv(const dcx m[6], const dcx x[3], dcx y[3])
 {
 	y[0] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
 	y[1] = m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
 	y[2] = m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
+}
+
+/* cReflMatrVec: src/cmplx.h:391-401 (M21 = M12, M31 = -M13, M32 = -M23) */
+static inline void refl_mv(const dcx m[6], const dcx x[3], dcx y[3])
+{
+	y[0] = m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+	y[1] = m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
+	y[2] = -m[2] * x[0] - m[4] * x[1] + m[5] * x[2];
 }
 
 /* MatVec: src/matvec.c:128-425 (FFT mode, one process, no surface) */
@@ -196,6 +264,19 @@ void or_matvec_fft(const or_grid *g, const or_cplx *dmat, const or_cplx *cc_sqrt
 			dcx a = C(arg[3 * i + c]);
 			if (her) a = conj(a);
 			X[c * n + idx] = O(C(cc_sqrt[3 * g->mat[i] + c]) * a);
+		}
+	}
+	/* the reflected term transforms a copy with the z direction backward: src/matvec.c:249, src/fft.c:383-415 */
+	or_cplx *XR = NULL;
+	size_t ry = rsize_y(g);
+	if (g->surf) {
+		XR = xmalloc(3 * n * sizeof(or_cplx));
+		memcpy(XR, X, 3 * n * sizeof(or_cplx));
+		for (int c = 0; c < 3; c++) {
+			or_cplx *a = XR + c * n;
+			or_fft1d(a, gx, 1, gy * gz, gx, -1);
+			for (size_t z = 0; z < gz; z++) or_fft1d(a + z * gx * gy, gy, gx, gx, 1, -1);
+			or_fft1d(a, gz, gx * gy, gx * gy, 1, +1);
 		}
 	}
 	for (int c = 0; c < 3; c++) fft3d(X + c * n, gx, gy, gz, -1);
@@ -230,6 +311,22 @@ void or_matvec_fft(const or_grid *g, const or_cplx *dmat, const or_cplx *cc_sqrt
 				size_t idx = (z * gy + y) * gx + x;
 				for (int c = 0; c < 3; c++) xv[c] = C(X[c * n + idx]);
 				sym_mv(f, xv, yv);
+				if (XR) { /* src/matvec.c:288-302, IndexRmatrix_mv 107-121 */
+					size_t xr = x, yr = y;
+					if (transposed) {
+						if (xr > 0) xr = gx - xr;
+						if (yr > 0) yr = gy - yr;
+					}
+					else if (yr >= ry) yr = gy - yr;
+					const or_cplx *rm = g->surf->rmat + 6 * ((xr * gz + z) * ry + yr);
+					dcx r[6], xr3[3], yr3[3];
+					for (int c = 0; c < 6; c++) r[c] = C(rm[c]);
+					if (g->reduced_fft && y >= ry) { r[1] = -r[1]; r[4] = -r[4]; }
+					if (transposed) { r[2] = -r[2]; r[4] = -r[4]; }
+					for (int c = 0; c < 3; c++) xr3[c] = C(XR[c * n + idx]);
+					refl_mv(r, xr3, yr3);
+					for (int c = 0; c < 3; c++) yv[c] += yr3[c];
+				}
 				for (int c = 0; c < 3; c++) X[c * n + idx] = O(yv[c]);
 			}
 	for (int c = 0; c < 3; c++) fft3d(X + c * n, gx, gy, gz, +1);
@@ -249,6 +346,7 @@ void or_matvec_fft(const or_grid *g, const or_cplx *dmat, const or_cplx *cc_sqrt
 	}
 	if (inprod) *inprod = s;
 	free(X);
+	free(XR);
 }
 
 /* SPARSE MatVec: src/matvec.c:434-465 with AijProd/DiagProd/CcMul of src/sparse_ops.h (non-SSE branch) */
@@ -268,14 +366,22 @@ void or_matvec_direct(const or_grid *g, const or_cplx *cc_sqrt, const or_cplx *a
 		dcx acc[3] = {0, 0, 0};
 		const unsigned short *pi = g->pos + 3 * i;
 		for (size_t j = 0; j < nd; j++) {
-			if (j == i) continue;
 			const unsigned short *pj = g->pos + 3 * j;
 			or_cplx t[6];
-			or_green_poi((int)pi[0] - pj[0], (int)pi[1] - pj[1], (int)pi[2] - pj[2], g->wavenum, g->gridspace, t);
 			dcx f[6], yv[3];
-			for (int c = 0; c < 6; c++) f[c] = C(t[c]);
-			sym_mv(f, sx + 3 * j, yv);
-			for (int c = 0; c < 3; c++) acc[c] += yv[c];
+			if (j != i) {
+				or_green_poi((int)pi[0] - pj[0], (int)pi[1] - pj[1], (int)pi[2] - pj[2], g->wavenum, g->gridspace, t);
+				for (int c = 0; c < 6; c++) f[c] = C(t[c]);
+				sym_mv(f, sx + 3 * j, yv);
+				for (int c = 0; c < 3; c++) acc[c] += yv[c];
+			}
+			if (g->surf) { /* src/sparse_ops.h:160-165: always, the dipole's own image included */
+				or_refl_img((int)pi[0] - pj[0], (int)pi[1] - pj[1], (int)pi[2] + pj[2], g->surf->zsum_shift,
+					g->surf->rcn, g->wavenum, g->gridspace, t);
+				for (int c = 0; c < 6; c++) f[c] = C(t[c]);
+				refl_mv(f, sx + 3 * j, yv);
+				for (int c = 0; c < 3; c++) acc[c] += yv[c];
+			}
 		}
 		for (int c = 0; c < 3; c++) {
 			dcx a = C(arg[3 * i + c]);
@@ -321,6 +427,112 @@ void or_generate_b(const or_grid *g, const double prop[3], const double incpol[3
 		dcx e = cos(ph) + I * sin(ph);
 		for (int c = 0; c < 3; c++) einc[3 * i + c] = O(e * incpol[c]);
 	}
+}
+
+/* cSqrtCut: src/cmplx.h:58-82 (branch cut not depending on the sign of a zero imaginary part) */
+static dcx csqrt_cut(dcx a)
+{
+	if (cimag(a) == 0) return creal(a) >= 0 ? sqrt(creal(a)) : I * sqrt(-creal(a));
+	return csqrt(a);
+}
+
+/* Fresnel coefficients: src/cmplx.h:672-709 */
+static dcx fresnel_rs(dcx ki, dcx kt) { return (ki - kt) / (ki + kt); }
+static dcx fresnel_ts(dcx ki, dcx kt) { return 2 * ki / (ki + kt); }
+static dcx fresnel_rp(dcx ki, dcx kt, dcx mr) { return (mr * mr * ki - kt) / (mr * mr * ki + kt); }
+static dcx fresnel_tp(dcx ki, dcx kt, dcx mr) { return 2 * mr * ki / (mr * mr * ki + kt); }
+
+/* InitBeam (B_PLANE with surface, src/GenerateB.c:96-137) and GenerateB (src/GenerateB.c:258-322) */
+double or_generate_b_surf(const or_grid *g, or_cplx msub_o, int msub_inf, double hsub, const double prop[3],
+	const double incpol_x[3], const double incpol_y[3], int which_x, or_cplx *einc)
+{
+	const dcx msub = C(msub_o);
+	const double k = g->wavenum;
+	double inc_scale = 1;
+	dcx ki, kt = 0, ktv[3] = {0, 0, 0};
+	const double pp = prop[0] * prop[0] + prop[1] * prop[1];
+	if (prop[2] > 0) { /* from the substrate */
+		inc_scale = 1 / creal(msub);
+		ki = msub * prop[2];
+		if (cabs(msub - 1) < 1e-15 && cabs(ki) < 3e-8) kt = ki;
+		else kt = csqrt_cut(1 - msub * msub * pp);
+		ktv[0] = msub * prop[0];
+		ktv[1] = msub * prop[1];
+		ktv[2] = kt;
+	}
+	else {
+		ki = -prop[2];
+		if (!msub_inf) {
+			if (cabs(msub - 1) < 1e-15 && cabs(ki) < 3e-8) kt = ki;
+			else kt = csqrt_cut(msub * msub - pp);
+			ktv[0] = prop[0];
+			ktv[1] = prop[1];
+			ktv[2] = -kt;
+		}
+	}
+	const double prefl[3] = {prop[0], prop[1], -prop[2]}; /* vRefl */
+	/* ex = incPolY, ey = -incPolX for the Y polarization; ex = incPolX, ey = incPolY for X (src/GenerateB.c:247-254) */
+	double ex[3], ey[3];
+	for (int c = 0; c < 3; c++) {
+		ex[c] = which_x ? incpol_x[c] : incpol_y[c];
+		ey[c] = which_x ? incpol_y[c] : -incpol_x[c];
+	}
+	dcx erefl[3], etran[3], rc, tc = 0;
+	if (prop[2] > 0) {
+		if (!which_x) { /* s-polarized */
+			for (int c = 0; c < 3; c++) erefl[c] = etran[c] = ex[c];
+			rc = fresnel_rs(ki, kt);
+			tc = fresnel_ts(ki, kt);
+		}
+		else {
+			erefl[0] = -ex[0]; erefl[1] = -ex[1]; erefl[2] = ex[2];          /* vInvRefl_cr */
+			etran[0] = ey[1] * ktv[2] - ey[2] * ktv[1];                       /* crCrossProd(ey, ktVec) */
+			etran[1] = ey[2] * ktv[0] - ey[0] * ktv[2];
+			etran[2] = ey[0] * ktv[1] - ey[1] * ktv[0];
+			rc = fresnel_rp(ki, kt, 1 / msub);
+			tc = fresnel_tp(ki, kt, 1 / msub);
+		}
+		dcx st = tc * cexp(I * k * (kt - ki) * hsub);
+		for (int c = 0; c < 3; c++) etran[c] *= st;
+		for (size_t i = 0; i < g->nd; i++) {
+			const unsigned short *p = g->pos + 3 * i;
+			double r[3] = {(p[0] - (g->boxX - 1) / 2.0) * g->gridspace, (p[1] - (g->boxY - 1) / 2.0) * g->gridspace,
+				(p[2] - (g->boxZ - 1) / 2.0) * g->gridspace};
+			dcx e = cexp(I * k * (ktv[0] * r[0] + ktv[1] * r[1] + ktv[2] * r[2]));
+			for (int c = 0; c < 3; c++) einc[3 * i + c] = O(e * etran[c]);
+		}
+	}
+	else {
+		if (!which_x) {
+			for (int c = 0; c < 3; c++) erefl[c] = ex[c];
+			rc = msub_inf ? -1 : fresnel_rs(ki, kt);
+		}
+		else {
+			erefl[0] = -ex[0]; erefl[1] = -ex[1]; erefl[2] = ex[2];
+			rc = msub_inf ? 1 : fresnel_rp(ki, kt, msub);
+		}
+		double ph = 2 * k * creal(ki) * hsub;
+		dcx sr = rc * (cos(ph) + I * sin(ph));
+		for (int c = 0; c < 3; c++) erefl[c] *= sr;
+		for (size_t i = 0; i < g->nd; i++) {
+			const unsigned short *p = g->pos + 3 * i;
+			double r[3] = {(p[0] - (g->boxX - 1) / 2.0) * g->gridspace, (p[1] - (g->boxY - 1) / 2.0) * g->gridspace,
+				(p[2] - (g->boxZ - 1) / 2.0) * g->gridspace};
+			double p1 = k * (r[0] * prop[0] + r[1] * prop[1] + r[2] * prop[2]);
+			double p2 = k * (r[0] * prefl[0] + r[1] * prefl[1] + r[2] * prefl[2]);
+			dcx e1 = cos(p1) + I * sin(p1), e2 = cos(p2) + I * sin(p2);
+			for (int c = 0; c < 3; c++) einc[3 * i + c] = O(e1 * ex[c] + e2 * erefl[c]);
+		}
+	}
+	return inc_scale;
+}
+
+/* ExtCross, general branch: src/crosssec.c:824-832 (cDotProd_Im = Im(P.conj(Einc))) */
+double or_ext_cross_gen(const or_grid *g, const or_cplx *pvec, const or_cplx *einc)
+{
+	double s = 0;
+	for (size_t i = 0; i < 3 * g->nd; i++) s += cimag(C(pvec[i]) * conj(C(einc[i])));
+	return FOUR_PI * g->wavenum * s;
 }
 
 /*====================================================================================================================*/

@@ -18,6 +18,13 @@ extern "C" {
 /* complex numbers cross the boundary as interleaved (re,im) doubles, layout-compatible with C99 double complex */
 typedef struct { double re, im; } or_cplx;
 
+/* Particle above a plane substrate (-surf h {mre mim | inf} with -int_surf img): the image-dipole reflected term */
+typedef struct {
+	double zsum_shift;           /* ZsumShift = 2*(hsub/d - (boxZ-1)/2), src/make_particle.c:2312 */
+	or_cplx rcn;                 /* surfRCn: -1 for a perfect reflector, else (1-msub^2)/(1+msub^2), src/interaction.c:1411 */
+	const or_cplx *rmat;         /* R-matrix of or_init_rmatrix, used by the FFT MatVec (the direct sum ignores it) */
+} or_surface;
+
 typedef struct {
 	int boxX, boxY, boxZ;        /* particle box (src/vars.h boxX..) */
 	int gridX, gridY, gridZ;     /* padded FFT grid (src/comm.c:489-494) */
@@ -27,6 +34,7 @@ typedef struct {
 	double wavenum;              /* WaveNum = 2*pi/lambda */
 	double gridspace;            /* d = lambda/dpl */
 	int reduced_fft;             /* reduced_FFT (src/param.c:531) */
+	const or_surface *surf;      /* NULL: free space */
 } or_grid;
 
 /* fftFit (src/fft.c:528-562): first x' >= x, even, divisible by divis, with prime factors {2,3,5} (rule 1, the
@@ -49,13 +57,33 @@ size_t or_dmatrix_size(const or_grid *g);
 void or_init_dmatrix(const or_grid *g, or_cplx *dmat);
 
 /* MatVec (src/matvec.c:128-425) restated on the full (not zero-skipped) grid: y = x + S.F^-1[D.F(S.x)];
- * her => conj(A conj x); inprod (nullable) => ||y||^2. cc_sqrt is [nmat][3]. */
+ * her => conj(A conj x); inprod (nullable) => ||y||^2. cc_sqrt is [nmat][3]. With g->surf the reflected term
+ * R.F'(S.x) is added to the product (src/matvec.c:249,288-302): F' is the 3D transform with the z direction backward
+ * (src/fft.c:383-415), cReflMatrVec (src/cmplx.h:391-401) applies R. */
 void or_matvec_fft(const or_grid *g, const or_cplx *dmat, const or_cplx *cc_sqrt, const or_cplx *arg, or_cplx *res,
 	double *inprod, int her);
 
-/* SPARSE MatVec (src/matvec.c:434-465, src/sparse_ops.h AijProd/DiagProd): y_i = x_i - S_i sum_{j!=i} G_ij S_j x_j. */
+/* SPARSE MatVec (src/matvec.c:434-465, src/sparse_ops.h AijProd/DiagProd): y_i = x_i - S_i sum_{j!=i} G_ij S_j x_j;
+ * with g->surf also - S_i sum_j GR(x_i-x_j, y_i-y_j, z_i+z_j) S_j x_j over every j, i included
+ * (src/sparse_ops.h:160-165). */
 void or_matvec_direct(const or_grid *g, const or_cplx *cc_sqrt, const or_cplx *arg, or_cplx *res, double *inprod,
 	int her);
+
+/* ReflTerm_img through its integer wrapper (src/interaction.c:1081-1153, vCopyIntRealShift): the image-dipole
+ * reflected Green's tensor for grid offset (i, j) and height sum ksum (units of d; qvec = (i, j, ksum + zsum_shift)),
+ * components as InterTerm with xz, yz, zz negated and scaled by rcn. Pinned by the closed form at hand-computed
+ * offsets (an independent evaluation of the image dipole's field) and by SPARSE -surf runs. */
+void or_refl_img(int i, int j, int ksum, double zsum_shift, or_cplx rcn, double wavenum, double gridspace,
+	or_cplx res[6]);
+
+/* Size of the R-matrix in complex elements: 6*gridX*gridZ*RsizeY, RsizeY = gridY/2+1 reduced, else gridY
+ * (src/fft.c:982-1000). */
+size_t or_rmatrix_size(const or_grid *g);
+
+/* InitRmatrix (src/fft.c:819-891): R = -FFT3(GR)/Ngrid with GR[i][j][k] = ReflTerm for |i|<boxX, |j|<boxY,
+ * 0 <= k <= 2*boxZ-2, stored as IndexRmatrix_mv (src/matvec.c:107-121): 6*((x*gridZ + z)*RsizeY + y). Needs
+ * g->surf. Pinned by FFT MatVec == direct sum with the reflected term (brute force). */
+void or_init_rmatrix(const or_grid *g, or_cplx *rmat);
 
 /* CoupleConstant for scalar m (src/calculator.c:100-133,170-262). polrel: 0 = LDR, 1 = CM, 2 = RRC.
  * prop/incpol: unit vectors (LDR S-term = sum_i (prop_i*incpol_i)^2, src/calculator.c:214-221).
@@ -66,6 +94,17 @@ or_cplx or_couple_constant(or_cplx m, int polrel, double wavenum, double gridspa
 /* GenerateB plane wave (src/GenerateB.c:324-330): Einc_i = exp(i k r_i.prop) e_pol, with DipoleCoord
  * r = (pos - (box-1)/2)*d (src/make_particle.c:2266-2275). */
 void or_generate_b(const or_grid *g, const double prop[3], const double incpol[3], or_cplx *einc);
+
+/* GenerateB plane wave above a substrate (src/GenerateB.c:258-322) with InitBeam's ki, kt, ktVec, prIncRefl
+ * (src/GenerateB.c:96-137): from above (prop[2] < 0) incident plus reflected wave, from below (prop[2] > 0) the
+ * transmitted wave; which_x = 0 takes ex = incPolY (s-polarized), 1 takes ex = incPolX (p-polarized). msub_inf: a
+ * perfectly reflecting substrate (msub ignored). Returns inc_scale (1/Re(msub) from below, else 1). Pinned by SPARSE
+ * -surf runs (incidence from above and below, finite and perfectly reflecting substrate). */
+double or_generate_b_surf(const or_grid *g, or_cplx msub, int msub_inf, double hsub, const double prop[3],
+	const double incpol_x[3], const double incpol_y[3], int which_x, or_cplx *einc);
+
+/* ExtCross, general form (src/crosssec.c:824-846): 4*pi*k*sum Im(P.conj(Einc)) (times inc_scale near a substrate). */
+double or_ext_cross_gen(const or_grid *g, const or_cplx *pvec, const or_cplx *einc);
 
 /* IterativeSolver (src/iterative.c:1398-1530) with QMR_CS (898-1032), BiCGStab (640-724), CGNR (730-770),
  * BiCG_CS (495-634), CSYM (774-894), BCGS2 (342-488) and QMR_CS_2 (1039-1143) restated; matvec is the FFT one if

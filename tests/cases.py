@@ -19,11 +19,16 @@ class OrCplx(ctypes.Structure):
     _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
 
 
+class OrSurface(ctypes.Structure):
+    _fields_ = [("zsum_shift", ctypes.c_double), ("rcn", OrCplx), ("rmat", ctypes.c_void_p)]
+
+
 class OrGrid(ctypes.Structure):
     _fields_ = [("boxX", ctypes.c_int), ("boxY", ctypes.c_int), ("boxZ", ctypes.c_int),
                 ("gridX", ctypes.c_int), ("gridY", ctypes.c_int), ("gridZ", ctypes.c_int),
                 ("nd", ctypes.c_size_t), ("pos", ctypes.c_void_p), ("mat", ctypes.c_void_p),
-                ("wavenum", ctypes.c_double), ("gridspace", ctypes.c_double), ("reduced_fft", ctypes.c_int)]
+                ("wavenum", ctypes.c_double), ("gridspace", ctypes.c_double), ("reduced_fft", ctypes.c_int),
+                ("surf", ctypes.POINTER(OrSurface))]
 
 
 def fit_box(box):
@@ -106,6 +111,14 @@ def oracle():
         lib.or_abs_cross.restype = ctypes.c_double
         lib.or_eplane_yz.argtypes = [P, P, P, P, P, ctypes.c_int, ctypes.c_int, P]
         lib.or_mueller_yz.argtypes = [P, P, ctypes.c_int, P]
+        lib.or_refl_img.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, OrCplx, ctypes.c_double, ctypes.c_double, P]
+        lib.or_rmatrix_size.argtypes = [P]
+        lib.or_rmatrix_size.restype = ctypes.c_size_t
+        lib.or_init_rmatrix.argtypes = [P, P]
+        lib.or_generate_b_surf.argtypes = [P, OrCplx, ctypes.c_int, ctypes.c_double, P, P, P, ctypes.c_int, P]
+        lib.or_generate_b_surf.restype = ctypes.c_double
+        lib.or_ext_cross_gen.argtypes = [P, P, P]
+        lib.or_ext_cross_gen.restype = ctypes.c_double
         _lib = lib
     return _lib
 
@@ -117,7 +130,8 @@ def ptr(a):
 class Case:
     """One DDA problem: geometry, grid, wavelength, materials; matches ADDA's variables for the same command line."""
 
-    def __init__(self, pos, box, m, dpl=None, lam=TWO_PI, mat=None, rule=1, reduced_fft=True, grid=None):
+    def __init__(self, pos, box, m, dpl=None, lam=TWO_PI, mat=None, rule=1, reduced_fft=True, grid=None,
+                 surface=None):
         self.pos = np.ascontiguousarray(pos, dtype=np.uint16)
         self.box = tuple(int(b) for b in box)
         self.nd = len(self.pos)
@@ -136,6 +150,16 @@ class Case:
         self.reduced_fft = reduced_fft
         self._g = OrGrid(*self.box, *self.grid, self.nd, self.pos.ctypes.data, self.mat.ctypes.data,
                          self.wavenum, self.gridspace, int(reduced_fft))
+        # surface = (hsub, msub) with msub None for a perfect reflector ('-surf h inf'): ZsumShift
+        # (src/make_particle.c:2312) and surfRCn (src/interaction.c:1411)
+        self.surface = surface
+        self._rmat = None
+        if surface is not None:
+            hsub, msub = surface
+            rcn = -1.0 if msub is None else (1 - msub * msub) / (1 + msub * msub)
+            self.zsum_shift = 2 * (hsub / self.gridspace - (self.box[2] - 1) / 2.0)
+            self._surf = OrSurface(self.zsum_shift, OrCplx(complex(rcn).real, complex(rcn).imag), None)
+            self._g.surf = ctypes.pointer(self._surf)
         self.gp = ctypes.byref(self._g)
 
     def cc(self, prop, incpol, polrel=0):
@@ -165,7 +189,7 @@ class Case:
     def with_m(self, m):
         """The same geometry and grid with another material table."""
         return Case(self.pos, self.box, m, dpl=self.dpl, lam=self.lam, mat=self.mat, reduced_fft=self.reduced_fft,
-                    grid=self.grid)
+                    grid=self.grid, surface=self.surface)
 
     def adjoint_grad(self, dmat, prop, incpol, polrel=0, method=0, eps=1e-12):
         """Oracle adjoint gradient of (Cext, Cabs) w.r.t. each dipole's permittivity: forward solve, adjoint solves
@@ -219,10 +243,35 @@ class Case:
         return out
 
     def dmatrix(self):
+        """D-matrix; near a substrate the R-matrix is built alongside and attached to the grid (the FFT MatVec and
+        the solver then add the reflected term)."""
         lib = oracle()
         d = np.empty(lib.or_dmatrix_size(self.gp), np.complex128)
         lib.or_init_dmatrix(self.gp, ptr(d))
+        if self.surface is not None and self._rmat is None:
+            self._rmat = self.rmatrix()
+            self._surf.rmat = self._rmat.ctypes.data
         return d
+
+    def rmatrix(self):
+        lib = oracle()
+        r = np.empty(lib.or_rmatrix_size(self.gp), np.complex128)
+        lib.or_init_rmatrix(self.gp, ptr(r))
+        return r
+
+    def einc_surf(self, prop, incpol_x, incpol_y, which_x):
+        """Incident field near the substrate and inc_scale (or_generate_b_surf)."""
+        hsub, msub = self.surface
+        e = np.empty(3 * self.nd, np.complex128)
+        ms = 1.0 if msub is None else complex(msub)
+        sc = oracle().or_generate_b_surf(self.gp, OrCplx(ms.real, ms.imag), int(msub is None), hsub,
+                                         ptr(np.ascontiguousarray(prop, np.float64)),
+                                         ptr(np.ascontiguousarray(incpol_x, np.float64)),
+                                         ptr(np.ascontiguousarray(incpol_y, np.float64)), int(which_x), ptr(e))
+        return e, sc
+
+    def cext_gen(self, p, einc):
+        return oracle().or_ext_cross_gen(self.gp, ptr(p), ptr(np.ascontiguousarray(einc)))
 
     def matvec_fft(self, dmat, cc_sqrt, x, her=False):
         lib = oracle()

@@ -72,6 +72,19 @@ RUNS = {
     "box8_multimat": ("box8mm", "-dpl 14 -m 1.2 0.01 1.5 0.1 1.8 0 -iter bicgstab -eps 10 -ntheta 19 -store_dip_pol",
                       ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
 }
+# particle above a substrate, image-dipole reflection (-int_surf img): incidence from above and from below (the default
+# direction), a perfect reflector, oblique incidence from above, and from below beyond the critical angle (evanescent
+# transmitted wave)
+SURF = "-dpl 15 -m 1.5 0.01 -int_surf img -iter qmr -eps 10 -store_dip_pol"
+RUNS.update({
+    "surf_img_above": ("sphere8", SURF + " -surf 2.0 1.5 0.1 -prop 0 0 -1", ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
+    "surf_img_below": ("sphere8", SURF + " -surf 2.0 1.5 0.1", ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
+    "surf_inf_above": ("sphere8", SURF + " -surf 2.0 inf -prop 0 0 -1", ["CrossSec-Y", "CrossSec-X", "DipPol-Y"]),
+    "surf_img_oblique": ("box8mm", "-dpl 14 -m 1.2 0.01 1.5 0.1 1.8 0 -int_surf img -iter qmr -eps 10 -store_dip_pol "
+                         "-surf 1.7 2.0 0.5 -prop 0.5 0.3 -1", ["CrossSec-Y", "CrossSec-X", "DipPol-Y", "DipPol-X"]),
+    "surf_img_tir": ("sphere8", SURF + " -surf 1.9 1.5 0 -prop 0.8 0 0.6",
+                     ["CrossSec-Y", "CrossSec-X", "DipPol-Y", "DipPol-X"]),
+})
 # orientations of the batched-RHS workload (ellipsoid 1:0.8:0.5, m = 1.33+0.001i, default dpl 10|m|), first four
 for _i, (_a, _b, _g) in enumerate(orientations(4)):
     RUNS["ell12_c3o%d" % _i] = ("ell12", "-dpl 13.3 -m 1.33 0.001 -orient %.6f %.6f %.6f -iter qmr -eps 10" % (_a, _b, _g),

@@ -1,6 +1,7 @@
 """Pins the CPU oracle (oracle/adda_oracle.c) against the reference's golden outputs, known answers and the
 mathematics. Runs on the CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -350,3 +351,106 @@ def test_oracle_adjoint_vs_reference_differences(pol):
             tol = G.adj8_fd_resolution(base["Cext" if row == 0 else "Cabs"])
             assert abs(dre - ref[0]) < 1e-4 * abs(ref[0]) + tol and abs(dim - ref[1]) < 1e-4 * abs(ref[1]) + tol, \
                 (j, row, dre, dim, ref, tol)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# particle above a substrate (-surf, image-dipole reflection)
+
+def _surf_frame(prop):
+    """prop normalized as -prop does (src/param.c:1372-1382) and the two incident polarizations of
+    src/param.c:2116-2131."""
+    prop = np.asarray(prop, float) / np.linalg.norm(prop)
+    if abs(prop[2]) == 1:
+        return prop, np.array([math.copysign(1.0, prop[2]), 0, 0]), np.array([0, 1.0, 0])
+    t = math.sqrt(1 - prop[2] ** 2)
+    return prop, np.array([prop[0] * prop[2] / t, prop[1] * prop[2] / t, -t]), np.array([-prop[1] / t, prop[0] / t, 0])
+
+
+# name: geometry, materials, dpl, (hsub, msub or None = perfect reflector), prop, multi-material
+SURF_RUNS = {
+    "surf_img_above": ("sphere8", 1.5 + 0.01j, 15, (2.0, 1.5 + 0.1j), (0, 0, -1), False),
+    "surf_img_below": ("sphere8", 1.5 + 0.01j, 15, (2.0, 1.5 + 0.1j), (0, 0, 1), False),
+    "surf_inf_above": ("sphere8", 1.5 + 0.01j, 15, (2.0, None), (0, 0, -1), False),
+    "surf_img_oblique": ("box8mm", [1.2 + 0.01j, 1.5 + 0.1j, 1.8], 14, (1.7, 2.0 + 0.5j), (0.5, 0.3, -1), True),
+    "surf_img_tir": ("sphere8", 1.5 + 0.01j, 15, (1.9, 1.5 + 0j), (0.8, 0, 0.6), False),
+}
+
+
+def surf_case(name):
+    geo, m, dpl, surf, prop, mm = SURF_RUNS[name]
+    pos, mat = G.geometry(geo)
+    p, box = read_shape(pos)
+    return Case(p, box, m, mat=mat if mm else None, dpl=dpl, rule=0, surface=surf), _surf_frame(prop)
+
+
+def test_refl_img_is_the_perfect_conductor_image():
+    """With rcn = -1 the reflected term applied as cReflMatrVec is the free-space field of the image dipole
+    p' = (-px, -py, pz) placed at the mirror height: GR.v = G(i, j, ksum + shift).(-v0, -v1, v2)."""
+    lib = oracle()
+    from cases import OrCplx
+    rng = np.random.default_rng(5)
+    for (i, j, k, sh) in ((0, 0, 0, 6.0), (2, -1, 3, 4.0), (-3, 2, 1, 7.0), (1, 1, 5, 2.0)):
+        r, g = (OrCplx * 6)(), (OrCplx * 6)()
+        lib.or_refl_img(i, j, k, sh, OrCplx(-1.0, 0.0), 1.0, 0.4, r)
+        lib.or_green_poi(i, j, int(k + sh), 1.0, 0.4, g)
+        R = np.array([complex(t.re, t.im) for t in r])
+        Gm = np.array([complex(t.re, t.im) for t in g])
+        Gfull = np.array([[Gm[0], Gm[1], Gm[2]], [Gm[1], Gm[3], Gm[4]], [Gm[2], Gm[4], Gm[5]]])
+        v = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+        refl = np.array([R[0] * v[0] + R[1] * v[1] + R[2] * v[2], R[1] * v[0] + R[3] * v[1] + R[4] * v[2],
+                         -R[2] * v[0] - R[4] * v[1] + R[5] * v[2]])
+        img = Gfull @ np.array([-v[0], -v[1], v[2]])
+        assert np.abs(refl - img).max() < 1e-13 * np.abs(img).max()
+    # a finite substrate scales the same tensor by (1 - m^2)/(1 + m^2)
+    m = 1.5 + 0.1j
+    rcn = (1 - m * m) / (1 + m * m)
+    r1, r2 = (OrCplx * 6)(), (OrCplx * 6)()
+    lib.or_refl_img(1, 2, 3, 3.5, OrCplx(rcn.real, rcn.imag), 1.0, 0.4, r1)
+    lib.or_refl_img(1, 2, 3, 3.5, OrCplx(1.0, 0.0), 1.0, 0.4, r2)
+    a = np.array([complex(t.re, t.im) for t in r1])
+    b = np.array([complex(t.re, t.im) for t in r2])
+    assert np.abs(a - rcn * b).max() < 1e-15 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("reduced", [True, False])
+def test_surface_fft_matvec_equals_direct_sum(reduced):
+    """The R-matrix correlation (z transformed backward) is exactly the direct sum over source images, the dipole's
+    own image included; the operator stays complex symmetric (R_ji = R_ij^T)."""
+    pos, b = make_shape("ellipsoid", 8, 0.8, 0.6)
+    c = Case(pos, b, 1.5 + 0.05j, reduced_fft=reduced, surface=(1.2, 1.5 + 0.1j))
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    d = c.dmatrix()
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+    y = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+    for her in (False, True):
+        y1, n1 = c.matvec_fft(d, ccs, x, her)
+        y2, n2 = c.matvec_direct(ccs, x, her)
+        assert np.abs(y1 - y2).max() < 1e-12 * np.abs(y2).max()
+        assert abs(n1 - n2) < 1e-12 * n2
+    Ay, _ = c.matvec_fft(d, ccs, y)
+    Ax, _ = c.matvec_fft(d, ccs, x)
+    assert abs(np.sum(x * Ay) - np.sum(y * Ax)) < 1e-12 * abs(np.sum(x * Ay))
+    # the reflected term is really there: without the substrate the result differs
+    c0 = Case(pos, b, 1.5 + 0.05j, reduced_fft=reduced)
+    y0, _ = c0.matvec_fft(c0.dmatrix(), ccs, x)
+    assert np.abs(y0 - y1).max() > 1e-3 * np.abs(y1).max()
+
+
+@pytest.mark.parametrize("name", list(SURF_RUNS))
+def test_surface_vs_sparse_reference(name):
+    """Converged Cext, Cabs (general ExtCross with inc_scale) and dipole polarizations against the reference's SPARSE
+    build with -surf ... -int_surf img (tests/golden/surf_*/cmd.txt)."""
+    c, (prop, px, py) = surf_case(name)
+    d = c.dmatrix()
+    for pol, inc, wx in (("Y", py, 0), ("X", px, 1)):
+        cc, ccs = c.cc(prop, inc)
+        e, sc = c.einc_surf(prop, px, py, wx)
+        r = c.solve(d, ccs, e, method=0, eps=1e-10)
+        assert r["rc"] == 0
+        g = G.crosssec(name, pol)
+        assert G.numeric_close([c.cext_gen(r["p"], e) * sc, c.cabs(r["p"], cc) * sc], [g["Cext"], g["Cabs"]],
+                               1e-14, 1e-8)[0]
+        if os.path.exists(G.path(name, "DipPol-" + pol)):
+            gp = G.dippol(name, pol)
+            assert G.numeric_close(np.c_[r["p"].real, r["p"].imag], np.c_[gp.real, gp.imag], 1e-12, 1e-6)[0]

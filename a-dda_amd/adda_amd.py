@@ -118,6 +118,11 @@ def hip():
     L.adda_hip_weighted_conj.argtypes = [P, P, P, P]
     L.adda_hip_adjoint_grad.argtypes = [P, P, P, P, P, D, D, P, P]
     L.adda_hip_vec_download_part.argtypes = [P, I, P, P, S, S]
+    L.adda_hip_set_surface.argtypes = [P, D, Cplx]
+    L.adda_hip_get_rmatrix.argtypes = [P, P]
+    L.adda_hip_rmatrix_size.argtypes = [P]
+    L.adda_hip_rmatrix_size.restype = S
+    L.adda_hip_plane_waves.argtypes = [P, D, D, I, P, P, P]
     _hip = L
     return L
 
@@ -165,6 +170,16 @@ class Operator:
         d = np.empty(L.adda_hip_dmatrix_size(self.h), np.complex128)
         check(L.adda_hip_get_dmatrix(self.h, _p(d)))
         return d
+
+    def set_surface(self, zsum_shift, rcn):
+        """Image-dipole substrate term (adda_hip_set_surface): every later matvec adds it."""
+        check(hip().adda_hip_set_surface(self.h, zsum_shift, Cplx.of(rcn)))
+
+    def rmatrix(self):
+        L = hip()
+        r = np.empty(L.adda_hip_rmatrix_size(self.h), np.complex128)
+        check(L.adda_hip_get_rmatrix(self.h, _p(r)))
+        return r
 
     # device vectors
     def alloc(self, prec=F64):
@@ -388,6 +403,22 @@ class SolveStats(ctypes.Structure):
 _host = None
 
 
+class Surface(ctypes.Structure):
+    _fields_ = [("hsub", ctypes.c_double), ("msub", Cplx), ("inf", ctypes.c_int)]
+
+
+def incident_frame(prop):
+    """prop normalized (-prop, src/param.c:1372-1382) and the incident polarizations incPolX, incPolY of
+    src/param.c:2116-2131."""
+    prop = np.asarray(prop, np.float64)
+    prop = prop / np.linalg.norm(prop)
+    if prop[0] == 0 and prop[1] == 0:
+        return prop, np.array([np.copysign(1.0, prop[2]), 0, 0]), np.array([0.0, 1.0, 0])
+    t = np.sqrt(1 - prop[2] ** 2)
+    return (prop, np.array([prop[0] * prop[2] / t, prop[1] * prop[2] / t, -t]),
+            np.array([-prop[1] / t, prop[0] / t, 0.0]))
+
+
 def host():
     """Load libadda.so (C host layer); raise if missing."""
     global _host
@@ -426,6 +457,9 @@ def host():
     L.adda_eplane_yz_batch.argtypes = [P, P, D, D, P, I, P, P, P, I, I, P]
     L.adda_mueller_yz.argtypes = [P, P, I, P]
     L.adda_mueller_yz.restype = None
+    L.adda_set_surface.argtypes = [P, P, D, ctypes.POINTER(Surface)]
+    L.adda_generate_b_surf.argtypes = [P, P, ctypes.POINTER(Surface), D, D, P, P, P, I, P, P]
+    L.adda_ext_cross_gen.argtypes = [P, D, D, P, P, P]
     _host = L
     return L
 
@@ -465,7 +499,9 @@ class Problem:
     flow of src/calculator.c:293-344) on the GPU."""
 
     def __init__(self, pos, box, m, mat=None, dpl=None, lam=2 * np.pi, reduced_fft=True, fp32=False, device=0,
-                 polrel=POL_LDR, grid=None):
+                 polrel=POL_LDR, grid=None, surface=None):
+        """surface: None (free space) or (hsub, msub) with msub None for a perfectly reflecting substrate
+        ('-surf h inf'): the particle's centre hsub above a substrate, image-dipole reflection."""
         self.pos = np.ascontiguousarray(pos, np.uint16)
         self.box = tuple(int(b) for b in box)
         self.nd = len(self.pos)
@@ -487,6 +523,12 @@ class Problem:
         self.einc = self.op.alloc()
         self.x = self.op.alloc()
         self.pvec = self.op.alloc()
+        self.surface = surface
+        self.inc_scale = 1.0
+        if surface is not None:
+            hsub, msub = surface
+            self._surf = Surface(hsub, Cplx.of(0 if msub is None else msub), int(msub is None))
+            hcheck(host().adda_set_surface(self.op.h, ctypes.byref(self._p), self.gridspace, ctypes.byref(self._surf)))
 
     def close(self):
         if self.op is not None:
@@ -511,8 +553,23 @@ class Problem:
         self.cc = np.repeat(cc[:, None], 3, axis=1)
         self.cc_sqrt = np.sqrt(self.cc)
         self.op.set_cc_sqrt(self.cc_sqrt)
-        hcheck(host().adda_generate_b(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, _p(self.prop),
-                                      _p(self.incpol), self.einc))
+        if self.surface is None:
+            hcheck(host().adda_generate_b(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace,
+                                          _p(self.prop), _p(self.incpol), self.einc))
+            return
+        # near the substrate the beam is one of the two polarizations of prop's incident frame
+        _, px, py = incident_frame(self.prop)
+        if np.allclose(self.incpol, py, atol=1e-12):
+            which = 0
+        elif np.allclose(self.incpol, px, atol=1e-12):
+            which = 1
+        else:
+            raise ValueError("near a substrate incpol must be incPolY or incPolX of incident_frame(prop)")
+        sc = ctypes.c_double()
+        hcheck(host().adda_generate_b_surf(self.op.h, ctypes.byref(self._p), ctypes.byref(self._surf), self.wavenum,
+                                           self.gridspace, _p(self.prop), _p(np.ascontiguousarray(px)),
+                                           _p(np.ascontiguousarray(py)), which, self.einc, ctypes.byref(sc)))
+        self.inc_scale = sc.value
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
               max_outer=0, recalc_resid=False, hist_len=20000, verbose=False, host_scalars=False, max_inner=0,
@@ -539,6 +596,8 @@ class Problem:
         """d(Cext, Cabs)/d(eps_j) for every dipole j of the current solution (call solve() first): two adjoint
         solves batched together, then per-dipole assembly on the GPU. Returns gext, gabs [nd][2] (d/dRe eps_j,
         d/dIm eps_j) and the stats of the two adjoint solves."""
+        if self.surface is not None:
+            raise NotImplementedError("adjoint gradients near a substrate are not implemented")
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, IF_AUTO, 0, precision, inner_eps, max_outer, 0, 0,
@@ -554,6 +613,10 @@ class Problem:
 
     def cext(self):
         out = ctypes.c_double()
+        if self.surface is not None:
+            hcheck(host().adda_ext_cross_gen(self.op.h, self.wavenum, self.inc_scale, self.pvec, self.einc,
+                                             ctypes.byref(out)))
+            return out.value
         hcheck(host().adda_ext_cross(self.op.h, ctypes.byref(self._p), self.wavenum, self.gridspace, self.pvec,
                                      _p(self.prop), _p(self.incpol), ctypes.byref(out)))
         return out.value
@@ -563,9 +626,11 @@ class Problem:
         cc = np.ascontiguousarray(self.cc)
         hcheck(host().adda_abs_cross(self.op.h, ctypes.byref(self._p), self.wavenum, _p(cc), self.pvec,
                                      ctypes.byref(out)))
-        return out.value
+        return out.value * self.inc_scale
 
     def eplane(self, par, per, ntheta, npoints=None):
+        if self.surface is not None:
+            raise NotImplementedError("the scattered field near a substrate (CalcFieldSurf) is not implemented")
         npoints = ntheta if npoints is None else npoints
         out = np.empty(2 * npoints, np.complex128)
         par = np.ascontiguousarray(par, np.float64)

@@ -169,6 +169,10 @@ struct adda_hip_op {
 	// D-matrix, layout [zt<DsZ][xt<DsX][yt<DsY][6]
 	cx<double> *D64 = nullptr;
 	cx<float> *D32 = nullptr;
+	// particle above a substrate: R-matrix of the image-dipole term, layout [z<gZ][xt<DsX][yt<DsY][6]
+	int surf = 0;
+	cx<double> *R64 = nullptr;
+	cx<float> *R32 = nullptr;
 	// work buffers, kcap right-hand sides (FP64-sized; FP32 uses the first half)
 	void *Xa = nullptr;  // [kcap][bZ][gX][3][bY]
 	void *Xb = nullptr;  // [kcap][gZ][gX][3][bY]
@@ -304,6 +308,70 @@ __global__ void k_extract_d(const cx<double> *G, cx<double> *D, int gX, int gY, 
 		for (int c = 0; c < 6; c++) {
 			cx<double> v = G[c * n + src];
 			D[idx * 6 + c] = mk<double>(-invN * v.x, -invN * v.y);
+		}
+	}
+}
+
+// Image-dipole reflected term at lattice offset (i, j) and height sum ksum: ReflTerm_img through its integer wrapper
+// (src/interaction.c:1081-1153): the separation from the source's image is (i, j, ksum + zsum_shift); InterTerm_core's
+// tensor scaled by rcn (surfRCn, src/interaction.c:1411) with the xz, yz and zz components negated.
+__device__ void refl_img(int i, int j, int ksum, double zsum_shift, cx<double> rcn, double wavenum, double gridspace,
+	cx<double> r[6])
+{
+	double qx = i, qy = j, qz = ksum + zsum_shift;
+	const double rn = sqrt(qx * qx + qy * qy + qz * qz);
+	qx /= rn;
+	qy /= rn;
+	qz /= rn;
+	const double rr = rn * gridspace;
+	const double invr3 = 1 / (rr * rr * rr);
+	const double kr = wavenum * rr, kr2 = kr * kr;
+	const double t1 = 3 - kr2, t2 = -3 * kr, t3 = kr2 - 1;
+	double sn, cs;
+	sincos(kr, &sn, &cs);
+	const cx<double> sc = mul(rcn, mk<double>(invr3 * cs, invr3 * sn));
+	const double q[6] = {qx * qx, qx * qy, qx * qz, qy * qy, qy * qz, qz * qz};
+#pragma unroll
+	for (int c = 0; c < 6; c++) {
+		cx<double> t;
+		if (c == 0 || c == 3 || c == 5) t = mk<double>(t1 * q[c] + t3, kr + t2 * q[c]);
+		else t = mk<double>(t1 * q[c], t2 * q[c]);
+		t = mul(t, sc);
+		r[c] = (c == 2 || c == 4 || c == 5) ? mk<double>(-t.x, -t.y) : t;
+	}
+}
+
+// GR on the full grid, 6 component planes: offsets |i| < boxX, |j| < boxY (periodic), height sums 0 <= k <= 2*boxZ-2
+// (not periodic, gridZ >= 2*boxZ), zero elsewhere (InitRmatrix, src/fft.c:819-850)
+__global__ void k_green_refl(cx<double> *G, int bX, int bY, int bZ, int gX, int gY, int gZ, double wavenum,
+	double gridspace, double zsum_shift, cx<double> rcn)
+{
+	const size_t n = (size_t)gX * gY * gZ;
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+		const int x = (int)(idx % gX), y = (int)((idx / gX) % gY), z = (int)(idx / ((size_t)gX * gY));
+		const int i = x < bX ? x : (x > gX - bX ? x - gX : INT32_MAX);
+		const int j = y < bY ? y : (y > gY - bY ? y - gY : INT32_MAX);
+		cx<double> r[6];
+		if (i == INT32_MAX || j == INT32_MAX || z > 2 * bZ - 2) {
+			for (int c = 0; c < 6; c++) r[c] = mk<double>(0, 0);
+		}
+		else refl_img(i, j, z, zsum_shift, rcn, wavenum, gridspace, r);
+		for (int c = 0; c < 6; c++) G[c * n + idx] = r[c];
+	}
+}
+
+// R[z][xt][yt][c] = -GR~[c][z][yt][xt]/Ngrid for every z, xt < DsX, yt < DsY (src/fft.c:876-881). GR is even or odd
+// under x -> -x and y -> -y (odd: xy, xz in x; xy, yz in y) like G, so the x and y halves are stored once.
+__global__ void k_extract_r(const cx<double> *G, cx<double> *R, int gX, int gY, int gZ, int DsX, int DsY)
+{
+	const size_t n = (size_t)gX * gY * gZ, m = (size_t)gZ * DsY * DsX;
+	const double invN = 1.0 / (double)n;
+	for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < m; idx += (size_t)gridDim.x * blockDim.x) {
+		const int y = (int)(idx % DsY), x = (int)((idx / DsY) % DsX), z = (int)(idx / ((size_t)DsX * DsY));
+		const size_t src = ((size_t)z * gY + y) * gX + x;
+		for (int c = 0; c < 6; c++) {
+			cx<double> v = G[c * n + src];
+			R[idx * 6 + c] = mk<double>(-invN * v.x, -invN * v.y);
 		}
 	}
 }
@@ -464,6 +532,28 @@ __global__ void k_plane_wave(cx<double> *e, const unsigned short *pos, size_t nd
 		e[3 * i] = mk<double>(cs * ex, sn * ex);
 		e[3 * i + 1] = mk<double>(cs * ey, sn * ey);
 		e[3 * i + 2] = mk<double>(cs * ez, sn * ez);
+	}
+}
+
+// Sum of nw <= 2 plane waves with complex wave vectors (units of k) and complex amplitudes:
+// e_i = sum_w amp_w exp(i k kv_w.r_i), r_i = (pos_i - (box-1)/2) d (the substrate's incident, reflected and
+// transmitted waves, src/GenerateB.c:258-322; an evanescent wave has a complex kv)
+__global__ void k_plane_waves(cx<double> *e, const unsigned short *pos, size_t nd, double cx_, double cy_, double cz_,
+	double d, double k, int nw, const double *w)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += (size_t)gridDim.x * blockDim.x) {
+		const double rx = (pos[3 * i] - cx_) * d, ry = (pos[3 * i + 1] - cy_) * d, rz = (pos[3 * i + 2] - cz_) * d;
+		cx<double> o[3] = {mk<double>(0, 0), mk<double>(0, 0), mk<double>(0, 0)};
+		for (int q = 0; q < nw; q++) {
+			const double *kv = w + 12 * q, *am = kv + 6;  // kv[3] complex, amp[3] complex
+			const double ph = k * (kv[0] * rx + kv[2] * ry + kv[4] * rz), at = -k * (kv[1] * rx + kv[3] * ry + kv[5] * rz);
+			double sn, cs;
+			sincos(ph, &sn, &cs);
+			const double a = exp(at);
+			const cx<double> f = mk<double>(a * cs, a * sn);
+			for (int c = 0; c < 3; c++) o[c] = add(o[c], mul(f, mk<double>(am[2 * c], am[2 * c + 1])));
+		}
+		for (int c = 0; c < 3; c++) e[3 * i + c] = o[c];
 	}
 }
 
@@ -652,6 +742,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 		if (!k.k[MV_P3R][es == 8] || 2 * c.LP * (op->gY / c.qy.rmax) > 256) return false;
 	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
+	if (op->surf && (c.p3r || c.p3g)) return false;  // the substrate's P3 (k_p3sub) keeps its twiddles in LDS
 	if (c.p3g && (c.p3r || !mv_kernels(c.qy).k[MV_P3G][es == 8])) return false;  // compile-time plans only
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
 	if (c.p3t && (c.p3r || p3_threads(c, op->gY, op->reduced) > 512)) return false;
@@ -704,10 +795,23 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, make_bq(op, K, 0, xin, xout, gate));
 }
 
+template <typename T> using P3SFn = decltype(&k_p3sub<T, Dyn<8>>);
+
 template <typename T>
 static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate, const cx<T> *D, int transposed)
 {
 	const int nz3 = op->reduced ? op->gZ / 2 + 1 : op->gZ, f = sizeof(T) == 4;
+	if (op->surf) {  // with the substrate's reflected term (cfg_ok keeps p3r and p3g off; the reduced form is required)
+		const cx<T> *R;
+		if constexpr (sizeof(T) == 8) R = op->R64;
+		else R = op->R32;
+		const int ntile = (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
+		hipLaunchKernelGGL((P3SFn<T>)c.ky.k[MV_P3S][f], dim3(nz3 * ntile * K), dim3(p3_threads(c, op->gY, op->reduced)),
+			lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, R, c.qy, tw_of<T>(op->twy),
+			op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
+			make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
+		return;
+	}
 	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? (c.LP == 1 ? MV_P3G1 : MV_P3G) : (c.LP == 1 ? MV_P31 : MV_P3)][f];
 	const cx<T> *tw = tw_of<T>(op->twy);
 	if (c.p3g) {  // the staged table of this plan, written once on the operator's stream
@@ -1446,6 +1550,35 @@ static int reserve(adda_hip_op *op, int K)
 
 static int scratch_setup(int dev);
 
+// Forward 3D FFT of the 6 component planes of a tensor on the full grid (x rows, y and z columns), FP64 (the D- and
+// R-matrix constructions)
+static int fft3_components(adda_hip_op *op, cx<double> *G)
+{
+	const int gX = op->gX, gY = op->gY, gZ = op->gZ;
+	const size_t ng = (size_t)gX * gY * gZ;
+	int LLr = 8;
+	while (LLr > 1 && ((size_t)LLr * gX * 16 > 64 * 1024 || !plan_fits(op->px, LLr, 256))) LLr /= 2;
+	int TXc = 16;
+	while (TXc > 1 && ((size_t)gY * TXc * 16 > 64 * 1024 || (size_t)gZ * TXc * 16 > 64 * 1024 ||
+		!plan_fits(op->py, TXc, 256) || !plan_fits(op->pz, TXc, 256))) TXc /= 2;
+	for (int c = 0; c < 6; c++) {
+		cx<double> *g = G + c * ng;
+		const long nl = (long)gY * gZ;
+		hipLaunchKernelGGL((k_rowfft<double, false>), dim3((unsigned)((nl + LLr - 1) / LLr)), dim3(256),
+			(size_t)LLr * gX * 16, op->stream, g, op->px, op->twx.d, nl, LLr);
+		// y columns: batch z (offset z*gX*gY)
+		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gZ), dim3(256),
+			(size_t)gY * TXc * 16, op->stream, g, g, op->py, op->twy.d, gX, TXc, gY, gY, (long)gX, (long)gX, 1,
+			(long)gX * gY, 0L, (long)gX * gY, 0L);
+		// z columns: batch y (offset y*gX)
+		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gY), dim3(256),
+			(size_t)gZ * TXc * 16, op->stream, g, g, op->pz, op->twz.d, gX, TXc, gZ, gZ, (long)gX * gY, (long)gX * gY,
+			1, (long)gX, 0L, (long)gX, 0L);
+		HIPCHK(hipGetLastError());
+	}
+	return 0;
+}
+
 static int op_build(adda_hip_op *op, const unsigned short *position, const int *material)
 {
 	HIPCHK(hipSetDevice(op->dev));
@@ -1518,26 +1651,7 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	hipLaunchKernelGGL(k_green, dim3(4096), dim3(256), 0, op->stream, G, bX, bY, bZ, gX, gY, gZ, op->wavenum,
 		op->gridspace);
 	HIPCHK(hipGetLastError());
-	int LLr = 8;
-	while (LLr > 1 && ((size_t)LLr * gX * 16 > 64 * 1024 || !plan_fits(op->px, LLr, 256))) LLr /= 2;
-	int TXc = 16;
-	while (TXc > 1 && ((size_t)gY * TXc * 16 > 64 * 1024 || (size_t)gZ * TXc * 16 > 64 * 1024 ||
-		!plan_fits(op->py, TXc, 256) || !plan_fits(op->pz, TXc, 256))) TXc /= 2;
-	for (int c = 0; c < 6; c++) {
-		cx<double> *g = G + c * ng;
-		const long nl = (long)gY * gZ;
-		hipLaunchKernelGGL((k_rowfft<double, false>), dim3((unsigned)((nl + LLr - 1) / LLr)), dim3(256),
-			(size_t)LLr * gX * 16, op->stream, g, op->px, op->twx.d, nl, LLr);
-		// y columns: batch z (offset z*gX*gY)
-		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gZ), dim3(256),
-			(size_t)gY * TXc * 16, op->stream, g, g, op->py, op->twy.d, gX, TXc, gY, gY, (long)gX, (long)gX, 1,
-			(long)gX * gY, 0L, (long)gX * gY, 0L);
-		// z columns: batch y (offset y*gX)
-		hipLaunchKernelGGL((k_colfft<double, false>), dim3((gX + TXc - 1) / TXc, gY), dim3(256),
-			(size_t)gZ * TXc * 16, op->stream, g, g, op->pz, op->twz.d, gX, TXc, gZ, gZ, (long)gX * gY, (long)gX * gY,
-			1, (long)gX, 0L, (long)gX, 0L);
-		HIPCHK(hipGetLastError());
-	}
+	if (fft3_components(op, G)) return -1;
 	HIPCHK(hipMalloc(&op->D64, nD * sizeof(cx<double>)));
 	hipLaunchKernelGGL(k_extract_d, dim3(4096), dim3(256), 0, op->stream, G, op->D64, gX, gY, gZ, op->DsX, op->DsY,
 		op->DsZ);
@@ -1590,7 +1704,7 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (!op) return;
 	(void)hipSetDevice(op->dev);
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
-	void *ptrs[] = {op->hv_arg, op->hv_res, op->partials2, op->cellmap, op->rows, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
+	void *ptrs[] = {op->R64, op->R32, op->hv_arg, op->hv_res, op->partials2, op->cellmap, op->rows, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
 		op->D32, op->Xa, op->Xb, op->partials, op->red, op->coef, op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d,
 		op->twz.f, op->twys.d, op->twys.f};
 	for (void *p : ptrs)
@@ -1653,6 +1767,99 @@ extern "C" int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst)
 					dst[6 * (((size_t)x * op->DsZ + z) * op->DsY + y) + c] = v;
 				}
 		}
+	return 0;
+}
+
+// InitRmatrix (src/fft.c:819-891) for a particle above a substrate with the image-dipole reflection: GR on the full
+// grid, 3D FFT, extraction of the stored x and y halves; from here on every matvec adds the reflected term in P3.
+extern "C" int adda_hip_set_surface(adda_hip_op *op, double zsum_shift, adda_cplx rcn)
+{
+	if (!op) return ERR("null operator");
+	if (!op->reduced) return ERR("the substrate's reflected term needs the reduced D-matrix (reduced_fft)");
+	if (op->gZ < 2 * op->bZ) return ERR("gridZ %d < 2*boxZ: the height sums would wrap", op->gZ);
+	if (zsum_shift <= 0) return ERR("ZsumShift %g: the particle must be above the substrate", zsum_shift);
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t ng = (size_t)op->gX * op->gY * op->gZ, nR = (size_t)op->gZ * op->DsX * op->DsY * 6;
+	cx<double> *G = nullptr;
+	HIPCHK(hipMalloc(&G, 6 * ng * sizeof(cx<double>)));
+	hipLaunchKernelGGL(k_green_refl, dim3(4096), dim3(256), 0, op->stream, G, op->bX, op->bY, op->bZ, op->gX, op->gY,
+		op->gZ, op->wavenum, op->gridspace, zsum_shift, mk<double>(rcn.re, rcn.im));
+	int rc = fft3_components(op, G);
+	if (!rc && !op->R64 && hipMalloc(&op->R64, nR * sizeof(cx<double>)) != hipSuccess) rc = ERR("out of device memory");
+	if (!rc) {
+		hipLaunchKernelGGL(k_extract_r, dim3(4096), dim3(256), 0, op->stream, G, op->R64, op->gX, op->gY, op->gZ,
+			op->DsX, op->DsY);
+		if (op->flags & 1) {
+			if (!op->R32 && hipMalloc(&op->R32, nR * sizeof(cx<float>)) != hipSuccess) rc = ERR("out of device memory");
+			else hipLaunchKernelGGL((k_convert<float, double>), dim3(4096), dim3(256), 0, op->stream, op->R32, op->R64,
+				nR);
+		}
+	}
+	if (!rc && hipGetLastError() != hipSuccess) rc = ERR("R-matrix kernels failed");
+	(void)hipStreamSynchronize(op->stream);
+	(void)hipFree(G);
+	if (rc) return rc;
+	op->surf = 1;
+	// tuned configurations with the register or global-twiddle P3 do not apply: back to the defaults
+	if (!cfg_ok(op, op->cfg[ADDA_F64], 16)) default_cfg(op, ADDA_F64, 16);
+	if (!cfg_ok(op, op->cfg[ADDA_F32], 8)) default_cfg(op, ADDA_F32, 8);
+	return 0;
+}
+
+extern "C" size_t adda_hip_rmatrix_size(const adda_hip_op *op)
+{
+	return op ? (size_t)6 * op->gX * op->gZ * (op->gY / 2 + 1) : 0;
+}
+
+extern "C" int adda_hip_get_rmatrix(adda_hip_op *op, adda_cplx *dst)
+{
+	if (!op || !dst) return ERR("null argument");
+	if (!op->surf) return ERR("no substrate set (adda_hip_set_surface)");
+	HIPCHK(hipSetDevice(op->dev));
+	const size_t n = (size_t)6 * op->gZ * op->DsX * op->DsY;
+	std::vector<adda_cplx> tmp(n);
+	HIPCHK(hipMemcpyAsync(tmp.data(), op->R64, n * sizeof(adda_cplx), hipMemcpyDeviceToHost, op->stream));
+	HIPCHK(hipStreamSynchronize(op->stream));
+	// device [z][xt][yt][6] -> ADDA [x][z][yt][6] (IndexRmatrix_mv, RsizeY = gridY/2+1); columns x >= DsX are the
+	// stored column gX - x with the xy and xz elements negated
+	for (int z = 0; z < op->gZ; z++)
+		for (int x = 0; x < op->gX; x++) {
+			const bool m = x >= op->DsX;
+			const int xt = m ? op->gX - x : x;
+			for (int y = 0; y < op->DsY; y++)
+				for (int c = 0; c < 6; c++) {
+					adda_cplx v = tmp[(((size_t)z * op->DsX + xt) * op->DsY + y) * 6 + c];
+					if (m && (c == 1 || c == 2)) {
+						v.re = -v.re;
+						v.im = -v.im;
+					}
+					dst[6 * (((size_t)x * op->gZ + z) * op->DsY + y) + c] = v;
+				}
+		}
+	return 0;
+}
+
+extern "C" int adda_hip_plane_waves(adda_hip_op *op, double wavenum, double gridspace, int nw, const adda_cplx *kvec,
+	const adda_cplx *amp, void *einc)
+{
+	if (!op || !kvec || !amp || !einc) return ERR("null argument");
+	if (nw < 1 || nw > 2) return ERR("%d plane waves (1 or 2 supported)", nw);
+	HIPCHK(hipSetDevice(op->dev));
+	double w[24];
+	for (int q = 0; q < nw; q++)
+		for (int c = 0; c < 3; c++) {
+			w[12 * q + 2 * c] = kvec[3 * q + c].re;
+			w[12 * q + 2 * c + 1] = kvec[3 * q + c].im;
+			w[12 * q + 6 + 2 * c] = amp[3 * q + c].re;
+			w[12 * q + 6 + 2 * c + 1] = amp[3 * q + c].im;
+		}
+	// the directions buffer holds 3*kMaxDir >= 24 doubles; the copy is ordered before the kernel on the stream
+	HIPCHK(hipMemcpyAsync(op->dirbuf, w, 12 * nw * sizeof(double), hipMemcpyHostToDevice, op->stream));
+	hipLaunchKernelGGL(k_plane_waves, dim3(nblocks_red(op->nd)), dim3(256), 0, op->stream, (cx<double> *)einc, op->pos,
+		op->nd, (op->bX - 1) / 2.0, (op->bY - 1) / 2.0, (op->bZ - 1) / 2.0, gridspace, wavenum, nw,
+		(const double *)op->dirbuf);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(op->stream));
 	return 0;
 }
 
@@ -2544,8 +2751,9 @@ extern "C" int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int rep
 	HIPCHK(hipSetDevice(op->dev));
 	if (reps < 1) reps = 3;
 	char key[200];
-	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d dmatrix %s prec %s", op->gX, op->gY, op->gZ,
-		op->bX, op->bY, op->bZ, op->reduced ? "octant" : "full", prec == ADDA_F32 ? "fp32" : "fp64");
+	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d dmatrix %s prec %s%s", op->gX, op->gY, op->gZ,
+		op->bX, op->bY, op->bZ, op->reduced ? "octant" : "full", prec == ADDA_F32 ? "fp32" : "fp64",
+		op->surf ? " surface" : "");
 	if (K > 1) snprintf(key + kl, sizeof key - kl, " batch %d", K);
 	if (cache_path) {
 		FILE *f = fopen(cache_path, "r");

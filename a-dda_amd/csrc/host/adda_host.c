@@ -189,6 +189,148 @@ int adda_generate_b(adda_hip_op *op, const adda_particle *p, double wavenum, dou
 	return 0;
 }
 
+/* ---- particle above a substrate ------------------------------------------------------------------------------ */
+int adda_set_surface(adda_hip_op *op, const adda_particle *p, double gridspace, const adda_surface *s)
+{
+	if (!op || !p || !s) return host_err(__FILE__, __LINE__, "null argument");
+	if (!(s->hsub > 0)) return host_err(__FILE__, __LINE__, "height above surface %g is not positive", s->hsub);
+	/* lowest dipole coordinate relative to the centre, src/make_particle.c:2268-2284 */
+	const double cz = (p->boxZ - 1) / 2.0;
+	int zmin = p->boxZ;
+	for (size_t i = 0; i < p->nd; i++)
+		if (p->position[3 * i + 2] < zmin) zmin = p->position[3 * i + 2];
+	const double minZco = (zmin - cz) * gridspace;
+	if (s->hsub <= -minZco)
+		return host_err(__FILE__, __LINE__, "The particle must be entirely above the substrate. There exist a dipole "
+			"with z=%g (relative to the center), making specified height of the center (%g) too small", minZco, s->hsub);
+	const double zsum_shift = 2 * (s->hsub / gridspace - cz);
+	dcx rcn = -1;
+	if (!s->inf) {
+		const dcx m = s->msub.re + I * s->msub.im;
+		rcn = (1 - m * m) / (1 + m * m);
+	}
+	const adda_cplx r = {creal(rcn), cimag(rcn)};
+	if (adda_hip_set_surface(op, zsum_shift, r)) return host_err_hip(__FILE__, __LINE__);
+	return 0;
+}
+
+/* cSqrtCut (src/cmplx.h:58-82): the branch cut does not depend on the sign of a zero imaginary part */
+static dcx csqrt_cut(dcx a)
+{
+	if (cimag(a) == 0) return creal(a) >= 0 ? sqrt(creal(a)) : I * sqrt(-creal(a));
+	return csqrt(a);
+}
+
+/* Fresnel coefficients, src/cmplx.h:672-709 (normal wave-vector components ki, kt; mr = mt/mi) */
+static dcx fr_rs(dcx ki, dcx kt) { return (ki - kt) / (ki + kt); }
+static dcx fr_ts(dcx ki, dcx kt) { return 2 * ki / (ki + kt); }
+static dcx fr_rp(dcx ki, dcx kt, dcx mr) { return (mr * mr * ki - kt) / (mr * mr * ki + kt); }
+static dcx fr_tp(dcx ki, dcx kt, dcx mr) { return 2 * mr * ki / (mr * mr * ki + kt); }
+
+int adda_generate_b_surf(adda_hip_op *op, const adda_particle *p, const adda_surface *s, double wavenum,
+	double gridspace, const double prop[3], const double incpol_x[3], const double incpol_y[3], int which_x,
+	void *d_einc, double *inc_scale)
+{
+	(void)p;
+	if (!op || !s || !prop || !incpol_x || !incpol_y || !d_einc) return host_err(__FILE__, __LINE__, "null argument");
+	if (prop[2] == 0)
+		return host_err(__FILE__, __LINE__, "Ambiguous setting of beam propagating along the surface. Please specify "
+			"the incident direction to have (arbitrary) small positive or negative z-component");
+	if (s->inf && prop[2] > 0)
+		return host_err(__FILE__, __LINE__, "Perfectly reflecting surface ('-surf ... inf') is incompatible with "
+			"incident direction from below (including the default one)");
+	const dcx msub = s->msub.re + I * s->msub.im;
+	const double pp = prop[0] * prop[0] + prop[1] * prop[1];
+	/* InitBeam: ki, kt, ktVec (ROUND_ERR 1e-15, SQRT_RND_ERR 3e-8: src/const.h:337-338) */
+	dcx ki, kt = 0, ktv[3] = {0, 0, 0};
+	double scale = 1;
+	if (prop[2] > 0) {
+		scale = 1 / creal(msub);
+		ki = msub * prop[2];
+		kt = (cabs(msub - 1) < 1e-15 && cabs(ki) < 3e-8) ? ki : csqrt_cut(1 - msub * msub * pp);
+		ktv[0] = msub * prop[0];
+		ktv[1] = msub * prop[1];
+		ktv[2] = kt;
+	}
+	else {
+		ki = -prop[2];
+		if (!s->inf) {
+			kt = (cabs(msub - 1) < 1e-15 && cabs(ki) < 3e-8) ? ki : csqrt_cut(msub * msub - pp);
+			ktv[0] = prop[0];
+			ktv[1] = prop[1];
+			ktv[2] = -kt;
+		}
+	}
+	/* the beam's frame: ex = incPolY, ey = -incPolX (Y); ex = incPolX, ey = incPolY (X), src/GenerateB.c:247-254 */
+	double ex[3], ey[3];
+	for (int c = 0; c < 3; c++) {
+		ex[c] = which_x ? incpol_x[c] : incpol_y[c];
+		ey[c] = which_x ? incpol_y[c] : -incpol_x[c];
+	}
+	adda_cplx kv[6], am[6];
+	int nw;
+	if (prop[2] > 0) { /* transmitted wave only */
+		dcx et[3], tc;
+		if (!which_x) {
+			for (int c = 0; c < 3; c++) et[c] = ex[c];
+			tc = fr_ts(ki, kt);
+		}
+		else { /* crCrossProd(ey, ktVec) */
+			et[0] = ey[1] * ktv[2] - ey[2] * ktv[1];
+			et[1] = ey[2] * ktv[0] - ey[0] * ktv[2];
+			et[2] = ey[0] * ktv[1] - ey[1] * ktv[0];
+			tc = fr_tp(ki, kt, 1 / msub);
+		}
+		const dcx f = tc * cexp(I * wavenum * (kt - ki) * s->hsub);
+		for (int c = 0; c < 3; c++) {
+			kv[c].re = creal(ktv[c]);
+			kv[c].im = cimag(ktv[c]);
+			am[c].re = creal(f * et[c]);
+			am[c].im = cimag(f * et[c]);
+		}
+		nw = 1;
+	}
+	else { /* incident plus reflected wave (prIncRefl = prop with z inverted) */
+		dcx er[3], rc;
+		if (!which_x) {
+			for (int c = 0; c < 3; c++) er[c] = ex[c];
+			rc = s->inf ? -1 : fr_rs(ki, kt);
+		}
+		else { /* vInvRefl_cr */
+			er[0] = -ex[0];
+			er[1] = -ex[1];
+			er[2] = ex[2];
+			rc = s->inf ? 1 : fr_rp(ki, kt, msub);
+		}
+		const double ph = 2 * wavenum * creal(ki) * s->hsub;
+		const dcx f = rc * (cos(ph) + I * sin(ph));
+		for (int c = 0; c < 3; c++) {
+			kv[c].re = prop[c];
+			kv[c].im = 0;
+			am[c].re = ex[c];
+			am[c].im = 0;
+			kv[3 + c].re = c == 2 ? -prop[c] : prop[c];
+			kv[3 + c].im = 0;
+			am[3 + c].re = creal(f * er[c]);
+			am[3 + c].im = cimag(f * er[c]);
+		}
+		nw = 2;
+	}
+	if (adda_hip_plane_waves(op, wavenum, gridspace, nw, kv, am, d_einc)) return host_err_hip(__FILE__, __LINE__);
+	if (inc_scale) *inc_scale = scale;
+	return 0;
+}
+
+int adda_ext_cross_gen(adda_hip_op *op, double wavenum, double inc_scale, const void *d_pvec, const void *d_einc,
+	double *cext)
+{
+	adda_cplx d;
+	/* sum P.conj(Einc): the conjugating inner product (nDotProd, src/linalg.c:75-88) */
+	if (adda_hip_dot(op, ADDA_F64, d_pvec, d_einc, ADDA_DOT_CONJ, &d, NULL)) return host_err_hip(__FILE__, __LINE__);
+	*cext = FOUR_PI * wavenum * d.im * inc_scale;
+	return 0;
+}
+
 /* ---- scattering quantities -------------------------------------------------------------------------------------- */
 /* CalcFieldFree, src/crosssec.c:506-595 (DRAINE scattering relation, free space), for K solutions with ndir
  * directions each (K == 1: a single device vector; K > 1: a batched one) */

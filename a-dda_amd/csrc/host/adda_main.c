@@ -6,7 +6,12 @@
  *            [-ntheta <n>] [-orient <alpha> <beta> <gamma>] [-no_reduced_fft] [-gpu <i>] [-dir <d>]
  *            [-prec fp64|mixed|mixed64] [-eps_inner <k>] [-max_inner <n>] [-host_scalars] [-graph]
  *            [-store_dip_pol] [-store_int_field] [-chpoint <time>] [-chp_type normal|regular|always]
- *            [-chp_dir <d>] [-chp_load] [-sym auto|no|enf] [-iter_inner <method>]
+ *            [-chp_dir <d>] [-chp_load] [-sym auto|no|enf] [-iter_inner <method>] [-prop <x> <y> <z>]
+ *            [-surf <h> <mre> <mim> | -surf <h> inf] [-int_surf img]
+ *
+ * With -surf the particle sits above a substrate (image-dipole reflection). Cext and Cabs are those of ExtCross's
+ * general form and AbsCross with inc_scale (src/crosssec.c:824-846,915); the scattered field near the substrate
+ * (CalcFieldSurf) is not computed, so no mueller file is written.
  *
  * Checkpoints (ADDA's chp.0 format, interchangeable with the reference's) cover the first polarization's solve, and,
  * as in the reference, need a run whose symmetry computes the internal field once (rotational symmetry, or -sym enf).
@@ -122,6 +127,9 @@ int main(int argc, char **argv)
 		ADDA_CHP_NONE, 0, "chpoint", 0, NULL, 0};
 	int chp_time_set = 0;
 	int polrel = ADDA_POL_LDR;
+	double prop_in[3] = {0, 0, 1};
+	int have_prop = 0, have_surf = 0;
+	adda_surface surf = {0, {0, 0}, 0};
 	for (int i = 1; i < argc; i++) {
 		const char *a = argv[i];
 #define NEED(n) do { if (i + (n) >= argc) usage(); } while (0)
@@ -175,6 +183,32 @@ int main(int argc, char **argv)
 			for (int c = 0; c < 3; c++) orient[c] = atof(argv[++i]);
 			have_orient = 1;
 		}
+		else if (!strcmp(a, "-prop")) { /* src/param.c:1372-1382 */
+			NEED(3);
+			for (int c = 0; c < 3; c++) prop_in[c] = atof(argv[++i]);
+			const double n2 = prop_in[0] * prop_in[0] + prop_in[1] * prop_in[1] + prop_in[2] * prop_in[2];
+			if (n2 == 0) { fprintf(stderr, "ERROR: Given propagation vector is null\n"); return EXIT_FAILURE; }
+			for (int c = 0; c < 3; c++) prop_in[c] /= sqrt(n2);
+			have_prop = 1;
+		}
+		else if (!strcmp(a, "-surf")) { /* src/param.c:1510-1528 */
+			NEED(2);
+			surf.hsub = atof(argv[++i]);
+			if (!strcmp(argv[i + 1], "inf")) { surf.inf = 1; i++; }
+			else {
+				NEED(2);
+				surf.msub.re = atof(argv[++i]);
+				surf.msub.im = atof(argv[++i]);
+			}
+			have_surf = 1;
+		}
+		else if (!strcmp(a, "-int_surf")) {
+			NEED(1);
+			if (strcmp(argv[++i], "img")) {
+				fprintf(stderr, "ERROR: -int_surf %s is not implemented (image-dipole reflection only: img)\n", argv[i]);
+				return EXIT_FAILURE;
+			}
+		}
 		else if (!strcmp(a, "-no_reduced_fft")) reduced = 0;
 		else if (!strcmp(a, "-sym")) {
 			NEED(1);
@@ -215,6 +249,13 @@ int main(int argc, char **argv)
 		else { fprintf(stderr, "ERROR: unknown option %s\n", a); usage(); }
 	}
 	if (nm == 0) { m[0].re = 1.5; m[0].im = 0; nm = 1; } /* default refractive index, src/param.c:1852 */
+	if (have_surf) { /* src/param.c:2070-2073 */
+		if (have_orient) { fprintf(stderr, "ERROR: Currently '-orient' and '-surf' can not be used together\n"); return 1; }
+		if (!reduced) {
+			fprintf(stderr, "ERROR: '-surf' needs the reduced D-matrix here (drop -no_reduced_fft)\n");
+			return EXIT_FAILURE;
+		}
+	}
 	/* -chpoint without -chp_type: normal for a positive time, always for zero (src/param.c:1011-1016) */
 	for (int i = 1; i < argc; i++)
 		if (!strcmp(argv[i], "-chpoint") && opt.chp_type == ADDA_CHP_NONE)
@@ -250,8 +291,19 @@ int main(int argc, char **argv)
 		ntheta = nbox < 1000 ? 91 : nbox < 10000 ? 181 : nbox < 100000 ? 361 : 721;
 	}
 	const int gx = adda_hip_fftfit(2 * p.boxX, 1), gy = adda_hip_fftfit(2 * p.boxY, 1), gz = adda_hip_fftfit(2 * p.boxZ, 1);
-	/* incident frame: prop along z, polarizations Y (par) and X (per), src/param.c:2116-2119; rotated by -orient */
-	double prop0[3] = {0, 0, 1}, py0[3] = {0, 1, 0}, px0[3] = {1, 0, 0}, prop[3], incY[3], incX[3];
+	/* incident frame: polarizations Y (par) and X (per) of prop (src/param.c:2116-2131); rotated by -orient */
+	double prop0[3] = {prop_in[0], prop_in[1], prop_in[2]}, py0[3] = {0, 1, 0}, px0[3] = {1, 0, 0}, prop[3], incY[3],
+		incX[3];
+	if (fabs(prop0[0]) < 1e-300 && fabs(prop0[1]) < 1e-300) px0[0] = prop0[2] > 0 ? 1 : -1;
+	else {
+		const double t = sqrt(1 - prop0[2] * prop0[2]);
+		px0[0] = prop0[0] * prop0[2] / t;
+		px0[1] = prop0[1] * prop0[2] / t;
+		px0[2] = -t;
+		py0[0] = -prop0[1] / t;
+		py0[1] = prop0[0] / t;
+		py0[2] = 0;
+	}
 	if (have_orient) {
 		rotate(orient[0], orient[1], orient[2], prop0, prop);
 		rotate(orient[0], orient[1], orient[2], py0, incY);
@@ -280,15 +332,22 @@ int main(int argc, char **argv)
 	adda_hip_op *op = adda_hip_op_create(gpu, p.boxX, p.boxY, p.boxZ, gx, gy, gz, p.nd, p.position, p.material, p.nmat,
 		wavenum, gridspace, reduced, opt.precision != ADDA_PREC_FP64);
 	if (!op) die("creating the operator");
+	if (have_surf && adda_set_surface(op, &p, gridspace, &surf)) die("setting the substrate");
 	tD = now() - tD;
+	if (have_surf) {
+		if (surf.inf) fprintf(log, "Particle is placed near the perfectly reflecting substrate\n");
+		else fprintf(log, "Particle is placed near the substrate with refractive index %.10g%+.10gi\n", surf.msub.re,
+			surf.msub.im);
+		fprintf(log, "  height of the particle center: %.10g\n", surf.hsub);
+	}
 	void *einc = adda_hip_vec_alloc(op, ADDA_F64), *x = adda_hip_vec_alloc(op, ADDA_F64),
 		*pv = adda_hip_vec_alloc(op, ADDA_F64);
 	if (!einc || !x || !pv) die("allocating vectors");
 	/* symmetries of the predefined shapes (src/make_particle.c:1478,1633; param.c:1903,2138) and the 360-degree
 	 * extension of the yz-plane range when incPolY and -incPolY are not equivalent (TestExtendThetaRange,
 	 * src/CalculateE.c:396-418, calculator.c:674) */
-	int symXY = !geom && !have_orient, symR = symXY && (!strcmp(shape, "sphere") || asp_y == 1),
-		symZ = !geom && !have_orient;
+	int symXY = !geom && !have_orient && !have_prop, symR = symXY && (!strcmp(shape, "sphere") || asp_y == 1),
+		symZ = !geom && !have_orient && !have_prop && !have_surf;
 	if (sym) symXY = symR = symZ = sym == 2; /* -sym no / enf */
 	int sym_incY = ((symXY || fabs(incY[0]) < 1e-10) && (symXY || fabs(incY[1]) < 1e-10)) || symR;
 	sym_incY = sym_incY && (symZ || fabs(incY[2]) < 1e-10);
@@ -317,7 +376,12 @@ int main(int argc, char **argv)
 		}
 		fprintf(log, "\nhere we go, calc %s\n\nCoupleConstant:%.10g%+.10gi\n", nm_, cc[0].re, cc[0].im);
 		if (adda_hip_set_cc_sqrt(op, ccs)) die("uploading cc_sqrt");
-		if (adda_generate_b(op, &p, wavenum, gridspace, prop, inc, einc)) die("GenerateB");
+		double inc_scale = 1;
+		if (have_surf) {
+			if (adda_generate_b_surf(op, &p, &surf, wavenum, gridspace, prop, incX, incY, pol, einc, &inc_scale))
+				die("GenerateB");
+		}
+		else if (adda_generate_b(op, &p, wavenum, gridspace, prop, inc, einc)) die("GenerateB");
 		double hist[4096];
 		adda_solve_stats st;
 		double ts = now();
@@ -340,9 +404,14 @@ int main(int argc, char **argv)
 		total_mv += st.nmatvec + st.nmatvec_fp64;
 		total_iter += st.niter;
 		double cext, cabs;
-		if (adda_ext_cross(op, &p, wavenum, gridspace, pv, prop, inc, &cext)) die("ExtCross");
+		if (have_surf) {
+			if (adda_ext_cross_gen(op, wavenum, inc_scale, pv, einc, &cext)) die("ExtCross");
+		}
+		else if (adda_ext_cross(op, &p, wavenum, gridspace, pv, prop, inc, &cext)) die("ExtCross");
 		if (adda_abs_cross(op, &p, wavenum, cc, pv, &cabs)) die("AbsCross");
-		if (adda_eplane_yz(op, &p, wavenum, gridspace, pv, prop, incY, incX, ntheta, npoints, pol == 0 ? eplY : eplX))
+		cabs *= inc_scale;
+		if (!have_surf &&
+			adda_eplane_yz(op, &p, wavenum, gridspace, pv, prop, incY, incX, ntheta, npoints, pol == 0 ? eplY : eplX))
 			die("CalcEplaneYZ");
 		if (store_dip_pol || store_int_field) { /* CalculateE, src/CalculateE.c:875-877 */
 			adda_cplx *P = malloc(3 * p.nd * sizeof(adda_cplx)), *chi_inv = malloc(p.nmat * sizeof(adda_cplx));
@@ -370,17 +439,19 @@ int main(int argc, char **argv)
 			st.niter, st.nmatvec, st.nmatvec_fp64, st.resid, cext, cabs, ts);
 	}
 	double *mu = malloc((size_t)npoints * 16 * sizeof(double));
-	adda_mueller_yz(eplX, eplY, npoints, mu);
-	snprintf(fn, sizeof fn, "%s/mueller", dir);
-	FILE *f = fopen(fn, "w");
-	if (!f) die("writing mueller");
-	fprintf(f, "theta s11 s12 s13 s14 s21 s22 s23 s24 s31 s32 s33 s34 s41 s42 s43 s44\n");
-	for (int i = 0; i < npoints; i++) {
-		fprintf(f, "%.2f", i * 180.0 / (ntheta - 1));
-		for (int j = 0; j < 16; j++) fprintf(f, " %.10E", mu[16 * i + j]);
-		fprintf(f, "\n");
+	if (!have_surf) {
+		adda_mueller_yz(eplX, eplY, npoints, mu);
+		snprintf(fn, sizeof fn, "%s/mueller", dir);
+		FILE *f = fopen(fn, "w");
+		if (!f) die("writing mueller");
+		fprintf(f, "theta s11 s12 s13 s14 s21 s22 s23 s24 s31 s32 s33 s34 s41 s42 s43 s44\n");
+		for (int i = 0; i < npoints; i++) {
+			fprintf(f, "%.2f", i * 180.0 / (ntheta - 1));
+			for (int j = 0; j < 16; j++) fprintf(f, " %.10E", mu[16 * i + j]);
+			fprintf(f, "\n");
+		}
+		fclose(f);
 	}
-	fclose(f);
 	fprintf(log, "\nTotal number of iterations: %d\nTotal number of matrix-vector products: %d\n", total_iter, total_mv);
 	fprintf(log, "Total wall time: %.4f\n  init Dmatrix: %.4f\n  solves: %.4f\n", now() - t0, tD, t_solve);
 	fclose(log);

@@ -332,10 +332,63 @@ __device__ __forceinline__ void sym3_lds(cx<T> *q0, cx<T> *q1, cx<T> *q2, cx<T> 
 // forward transform skips the zero upper half and the inverse computes only the lower half; FULL (boxY == gridY/2,
 // a plan with both half options) drops the y < boxY checks, which then hold for every index the transforms touch.
 // Without the reduced D-matrix every group is one column at one plane (s = 0).
-template <typename T, class PL, bool FULL>
+// Reflected product (cReflMatrVec, src/cmplx.h:391-401: M21 = M12, M31 = -M13, M32 = -M23) accumulated into o0..o2,
+// with the compile-time sign SX of a mirrored column (flips xy and xz, as for D).
+template <typename T, int SX>
+__device__ __forceinline__ void refl3_acc(cx<T> &o0, cx<T> &o1, cx<T> &o2, cx<T> r0, cx<T> r1, cx<T> r2, cx<T> r3,
+	cx<T> r4, cx<T> r5, cx<T> b0, cx<T> b1, cx<T> b2)
+{
+	o0 = cmac<T, SX>(cmac<T, SX>(cmac<T, 1>(o0, r0, b0), r1, b1), r2, b2);
+	o1 = cmac<T, 1>(cmac<T, 1>(cmac<T, SX>(o1, r1, b0), r3, b1), r4, b2);
+	o2 = cmac<T, 1>(cmac<T, -1>(cmac<T, -SX>(o2, r2, b0), r4, b1), r5, b2);
+}
+
+// Symmetric product of sym3_lds returning its result (the inputs stay unchanged)
+template <typename T, int SX, int SZ>
+__device__ __forceinline__ void sym3_val(cx<T> &o0, cx<T> &o1, cx<T> &o2, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3,
+	cx<T> f4, cx<T> d5, cx<T> a0, cx<T> a1, cx<T> a2)
+{
+	o0 = cmac<T, SX * SZ>(cmac<T, SX>(mul(d0, a0), f1, a1), d2, a2);
+	o1 = cmac<T, SZ>(cmac<T, SX>(mul(d3, a1), f1, a0), f4, a2);
+	o2 = cmac<T, SX * SZ>(cmac<T, SZ>(mul(d5, a2), f4, a1), d2, a0);
+}
+
+// D-matrix and reflected products of one column side at frequency y (SURF): lines A (plane z) and, when the block
+// holds it (hz), B (plane gZ - z). The reflected correlation along z is a product with the z-backward transform of the
+// source (src/matvec.c:249,288-302); that transform at plane kz is the forward one at plane -kz, so plane z takes
+// R(z).X(gZ - z) and plane gZ - z takes R(gZ - z).X(z) -- both lines are in this block. Self-mirrored planes (z = 0,
+// 2z = gZ) take R(z).X(z).
+template <typename T, int SX>
+__device__ __forceinline__ void surf_side(cx<T> *qa0, cx<T> *qa1, cx<T> *qa2, cx<T> *qb0, cx<T> *qb1, cx<T> *qb2,
+	bool hz, cx<T> d0, cx<T> f1, cx<T> d2, cx<T> d3, cx<T> f4, cx<T> d5, const cx<T> *ra, const cx<T> *rb,
+	unsigned my, unsigned mx)
+{
+	const cx<T> a0 = *qa0, a1 = *qa1, a2 = *qa2;
+	const cx<T> b0 = hz ? *qb0 : a0, b1 = hz ? *qb1 : a1, b2 = hz ? *qb2 : a2;
+	cx<T> o0, o1, o2;
+	sym3_val<T, SX, 1>(o0, o1, o2, d0, f1, d2, d3, f4, d5, a0, a1, a2);
+	refl3_acc<T, SX>(o0, o1, o2, ra[0], flip<T>(ra[1], my ^ mx), flip<T>(ra[2], mx), ra[3], flip<T>(ra[4], my), ra[5],
+		b0, b1, b2);
+	*qa0 = o0;
+	*qa1 = o1;
+	*qa2 = o2;
+	if (hz) {
+		sym3_val<T, SX, -1>(o0, o1, o2, d0, f1, d2, d3, f4, d5, b0, b1, b2);
+		refl3_acc<T, SX>(o0, o1, o2, rb[0], flip<T>(rb[1], my ^ mx), flip<T>(rb[2], mx), rb[3], flip<T>(rb[4], my),
+			rb[5], a0, a1, a2);
+		*qb0 = o0;
+		*qb1 = o1;
+		*qb2 = o2;
+	}
+}
+
+// SURF: the particle is above a substrate and R (the transformed image-dipole term, [z < gridZ][xt < DsizeX][yt <
+// DsizeY][6], even or odd in x and y like D, full in z) is added to the product (requires the reduced D-matrix, whose
+// plane pairs put every line's reflected partner into its block).
+template <typename T, class PL, bool FULL, bool SURF = false>
 __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp,
 	const cx<T> *__restrict__ D, int u0, int z, int gX, int bY, int gZ, int DsX, int DsY, int LP, int NL1,
-	unsigned pm, int reduced, int transposed, int line, int t, bool act)
+	unsigned pm, int reduced, int transposed, int line, int t, bool act, const cx<T> *__restrict__ R = nullptr)
 {
 	// padded rows: their offsets stay immediates (a swizzle would cost P3, which is close to VALU-bound, an address
 	// computation per access)
@@ -365,6 +418,20 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 		const cx<T> f1 = flip<T>(d[1], my ^ mx), f4 = flip<T>(d[4], my), d2 = flip<T>(d[2], mx);
 		auto q = [&](int l) { return buf + LAY::pos(l, y, 1, LS); };
 		const int l0 = 3 * pr;
+		if constexpr (SURF) {
+			// the R rows of planes z and gZ - z at the stored (column, y) of D's row (drow without the transposed
+			// case: SURF runs with the reduced D-matrix only); R has the same row extents (DsX, DsY) per plane
+			const int xs = u0 + pr < DsX ? u0 + pr : gX - (u0 + pr), ys = y < DsY ? y : N - y;
+			const bool hz = (pm >> (LP + pr)) & 1;
+			const cx<T> *ra = R + (((size_t)z * DsX + xs) * DsY + ys) * 6;
+			const cx<T> *rb = hz ? R + (((size_t)(gZ - z) * DsX + xs) * DsY + ys) * 6 : ra;
+			surf_side<T, 1>(q(l0), q(l0 + 1), q(l0 + 2), q(NL1 + l0), q(NL1 + l0 + 1), q(NL1 + l0 + 2), hz, d[0], f1,
+				d2, d[3], f4, d[5], ra, rb, my, mx);
+			if ((pm >> (2 * LP + pr)) & 1)
+				surf_side<T, -1>(q(2 * NL1 + l0), q(2 * NL1 + l0 + 1), q(2 * NL1 + l0 + 2), q(3 * NL1 + l0),
+					q(3 * NL1 + l0 + 1), q(3 * NL1 + l0 + 2), hz, d[0], f1, d2, d[3], f4, d[5], ra, rb, my, mx);
+			return;
+		}
 		sym3_lds<T, 1, 1>(q(l0), q(l0 + 1), q(l0 + 2), d[0], f1, d2, d[3], f4, d[5]);
 		if ((pm >> (LP + pr)) & 1)
 			sym3_lds<T, 1, -1>(q(NL1 + l0), q(NL1 + l0 + 1), q(NL1 + l0 + 2), d[0], f1, d2, d[3], f4, d[5]);
@@ -377,7 +444,7 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	// In FP32 the D rows of this thread's first KP elements are loaded before the forward transform, which hides their
 	// latency (KP covers all of them with the reduced D-matrix: blockDim = 12*LP*N/RMAX threads for LP*N elements). In
 	// FP64 the extra registers would cost more occupancy than the prefetch gains.
-	constexpr int KP = sizeof(T) == 4 ? (PL::RMAXC + 11) / 12 : 0;
+	constexpr int KP = sizeof(T) == 4 && !SURF ? (PL::RMAXC + 11) / 12 : 0;
 	cx<T> dpre[KP > 0 ? KP : 1][6];
 	unsigned mpre[KP > 0 ? KP : 1];
 #pragma unroll
@@ -428,10 +495,10 @@ __host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { retu
 // lets one more block share a CU.
 // LPC > 0: the number of column units per block fixed at compile time (LP = 1 is what the autotuner picks at the
 // sizes measured), which turns the group offsets and the group tests of the D-matrix product into constants.
-template <typename T, class PL, bool TWG, int LPC>
+template <typename T, class PL, bool TWG, int LPC, bool SURF = false>
 __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LPr, int reduced, int xpair,
-	int transposed, Bq bq)
+	int transposed, Bq bq, const cx<T> *__restrict__ R = nullptr)
 {
 	const int LP = LPC > 0 ? LPC : LPr;
 	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
@@ -468,11 +535,11 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 	const int zz = (s & 1) ? gZ - z : z, xx = (s & 2) ? gX - (u0 + pr) : u0 + pr;
 	cx<T> *lp = act ? Xb + ((size_t)zz * gX + xx) * 3 * bY + line % 3 : Xb;
 	if (PL::HALF_IN && PL::HALF_OUT && 2 * bY == N)
-		p3_body<T, PL, true>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed, line,
-			t, act);
+		p3_body<T, PL, true, SURF>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
+			line, t, act, R);
 	else
-		p3_body<T, PL, false>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
-			line, t, act);
+		p3_body<T, PL, false, SURF>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
+			transposed, line, t, act, R);
 }
 
 template <typename T, class PL, int LPC = 0>
@@ -481,6 +548,14 @@ __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__
 	int transposed, Bq bq)
 {
 	p3_kernel<T, PL, false, LPC>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
+}
+// P3 for a particle above a substrate: the D-matrix product plus the reflected term R (twiddles in LDS, any LP)
+template <typename T, class PL>
+__global__ void __launch_bounds__(512) k_p3sub(cx<T> *Xb, const cx<T> *__restrict__ D, const cx<T> *__restrict__ R,
+	FftPlan2 p, const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced,
+	int xpair, int transposed, Bq bq)
+{
+	p3_kernel<T, PL, false, 0, true>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq, R);
 }
 // Register budget of five waves per SIMD: the twiddle loads from global memory would otherwise be hoisted into
 // registers, costing the occupancy that the LDS saving buys.
@@ -880,9 +955,10 @@ template <typename T, class PL> constexpr bool p3r_ok()
 // MV_P3G: P3 with its twiddles read from global memory; MV_TWST: the kernel that stages that table (compile-time
 // plans only).
 // MV_P31 / MV_P3G1: the same two P3 kernels compiled for one column unit per block (LP = 1).
+// MV_P3S: P3 with the substrate's reflected term (k_p3sub).
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
-	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_KINDS = 14
+	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_KINDS = 15
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -907,6 +983,7 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
 	m.k[MV_P31][f] = (const void *)&k_p3<T, PL, 1>;
+	m.k[MV_P3S][f] = (const void *)&k_p3sub<T, PL>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
 	m.k[MV_P3G][f] = m.k[MV_TWST][f] = m.k[MV_P3G1][f] = nullptr;
@@ -959,7 +1036,8 @@ template <typename T, class PL> hipError_t mv_lds_t()
 			if (lim((const void *)k_p1<T, PL, double>) || lim((const void *)k_p5<T, PL, false, double>) ||
 				lim((const void *)k_p5<T, PL, true, double>)) return e;
 	}
-	if (lim((const void *)k_p3<T, PL>) || lim((const void *)k_p3<T, PL, 1>) || lim((const void *)k_zfft<T, false, PL>) ||
+	if (lim((const void *)k_p3<T, PL>) || lim((const void *)k_p3<T, PL, 1>) || lim((const void *)k_p3sub<T, PL>) ||
+		lim((const void *)k_zfft<T, false, PL>) ||
 		lim((const void *)k_zfft<T, true, PL>)) return e;
 	if constexpr (p3r_ok<T, PL>())
 		if (lim((const void *)k_p3r<T, PL>)) return e;

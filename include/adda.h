@@ -56,6 +56,30 @@ adda_cplx adda_couple_constant(adda_cplx m, int polrel, double wavenum, double g
 int adda_generate_b(adda_hip_op *op, const adda_particle *p, double wavenum, double gridspace, const double prop[3],
 	const double incpol[3], void *d_einc /* FP64 device vector */);
 
+/* Particle above a plane substrate parallel to the xy plane (-surf <h> <mre> <mim> | -surf <h> inf, src/param.c:1510-1528)
+ * with the image-dipole reflection (-int_surf img, ReflTerm_img, src/interaction.c:1136-1153); the Sommerfeld-integral
+ * reflection (-int_surf som) is not implemented. */
+typedef struct {
+	double hsub;      /* height of the particle centre above the surface (um) */
+	adda_cplx msub;   /* refractive index of the substrate (ignored with inf) */
+	int inf;          /* perfectly reflecting substrate (msubInf) */
+} adda_surface;
+
+/* Puts the operator's particle above the substrate: ZsumShift = 2*(hsub/d - (boxZ-1)/2) (src/make_particle.c:2312),
+ * surfRCn = -1 (inf) or (1-msub^2)/(1+msub^2) (src/interaction.c:1411), then adda_hip_set_surface. Fails, as
+ * src/make_particle.c:2282-2284 does, unless every dipole is above the surface. */
+int adda_set_surface(adda_hip_op *op, const adda_particle *p, double gridspace, const adda_surface *s);
+/* The incident plane wave near the substrate (InitBeam, src/GenerateB.c:96-137; GenerateB, 258-322): from above
+ * (prop[2] < 0) the incident plus the reflected wave, from below (prop[2] > 0) the transmitted wave (evanescent
+ * beyond the critical angle). which_x = 0: polarization ex = incPolY (s-polarized), 1: ex = incPolX (p-polarized);
+ * incpol_x, incpol_y: the incident frame of src/param.c:2116-2131. *inc_scale = 1/Re(msub) from below, else 1. */
+int adda_generate_b_surf(adda_hip_op *op, const adda_particle *p, const adda_surface *s, double wavenum,
+	double gridspace, const double prop[3], const double incpol_x[3], const double incpol_y[3], int which_x,
+	void *d_einc, double *inc_scale);
+/* ExtCross, general form (src/crosssec.c:824-846), used near a substrate: 4 pi k inc_scale sum Im(P.conj(Einc)). */
+int adda_ext_cross_gen(adda_hip_op *op, double wavenum, double inc_scale, const void *d_pvec, const void *d_einc,
+	double *cext);
+
 /* ---- solver ---------------------------------------------------------------------------------------------------- */
 typedef struct {
 	int method;          /* enum adda_iter */

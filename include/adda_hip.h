@@ -76,6 +76,28 @@ int adda_hip_set_cc_sqrt(adda_hip_op *op, const adda_cplx *cc_sqrt);
 int adda_hip_get_dmatrix(adda_hip_op *op, adda_cplx *dst);
 size_t adda_hip_dmatrix_size(const adda_hip_op *op);
 
+/* Particle above a plane substrate, image-dipole reflection (-surf with -int_surf img): builds the R-matrix on the GPU
+ * (InitRmatrix, src/fft.c:819-891, with ReflTerm_img, src/interaction.c:1081-1153) and from then on every matvec adds
+ * the reflected term (src/matvec.c:249,288-302). zsum_shift: ZsumShift = 2*(hsub/gridspace - (boxZ-1)/2)
+ * (src/make_particle.c:2312), > 0; rcn: surfRCn = -1 for a perfect reflector, else (1-msub^2)/(1+msub^2)
+ * (src/interaction.c:1411). Needs reduced_fft and gridZ >= 2*boxZ. The z-backward transform of the reflected source
+ * is taken from the forward one at plane -kz, which P3 already holds (no extra transforms). Tuned matvec
+ * configurations whose P3 variant has no reflected term are replaced by the defaults. */
+int adda_hip_set_surface(adda_hip_op *op, double zsum_shift, adda_cplx rcn);
+
+/* Copies the R-matrix to the host in ADDA's layout Rmatrix[6*((x*gridZ+z)*RsizeY+y)+comp], RsizeY = gridY/2+1
+ * (IndexRmatrix_mv, src/matvec.c:107-121); the device stores x <= gridX/2 (columns beyond are filled from gridX - x
+ * with xy and xz negated). For checks against a CPU build. */
+int adda_hip_get_rmatrix(adda_hip_op *op, adda_cplx *dst);
+size_t adda_hip_rmatrix_size(const adda_hip_op *op);
+
+/* Incident field as a sum of nw (1 or 2) plane waves with complex wave vectors kvec[nw][3] (units of wavenum; an
+ * evanescent wave has a complex one) and complex amplitudes amp[nw][3]: Einc_i = sum_w amp_w exp(i k kvec_w.r_i),
+ * r_i as in adda_hip_plane_wave. The device half of GenerateB's substrate branch (src/GenerateB.c:258-322): incident
+ * plus reflected wave from above, transmitted wave from below; FP64 device vector; synchronises. */
+int adda_hip_plane_waves(adda_hip_op *op, double wavenum, double gridspace, int nw, const adda_cplx *kvec,
+	const adda_cplx *amp, void *einc);
+
 /* Device vectors: allocation, host <-> device copies (FP64 host data), precision conversion. */
 void *adda_hip_vec_alloc(adda_hip_op *op, int prec);
 int adda_hip_vec_free(adda_hip_op *op, void *v);

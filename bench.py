@@ -48,7 +48,10 @@ def main():
     ap.add_argument("--solve-only", choices=["fp64", "mixed"],
                     help="profiling mode: autotune, then one solve (--method, to --eps) in the given form; prints its "
                          "time")
-    ap.add_argument("--method", default="bicgstab", help="solver of --solve-only (ADDA's -iter names)")
+    ap.add_argument("--batched-solve", choices=["fp64", "mixed"],
+                    help="only one batched solve of the 64-RHS workload (after an untimed one), in this form: the run "
+                         "a kernel trace of a batched solve is taken from")
+    ap.add_argument("--method", default="bicgstab", help="solver of --solve-only / --batched-solve (ADDA's -iter names)")
     ap.add_argument("--eps", type=float, default=1e-10, help="tolerance of --solve-only (default 1e-10)")
     ap.add_argument("--m", type=float, nargs=2, default=[1.5, 0.01], metavar=("RE", "IM"),
                     help="refractive index of the sphere (default 1.5 0.01; the 640^3 workload uses 1.33 0.001)")
@@ -75,6 +78,9 @@ def main():
     import adda_amd as A
     if A.device_count() < 1:
         raise RuntimeError("no HIP device visible")
+    if args.batched_solve:
+        batched_solve_only(A, args)
+        return
 
     lam = 2 * np.pi
     pos, mat, box = A.make_particle("sphere", args.box)
@@ -242,6 +248,9 @@ def main():
                           "bytes_per_matvec": v["total_bytes"], "us_per_matvec": round(v["ms"] * 1e3, 2),
                           "pass_us": [round(p * 1e3, 2) for p in v["passes"]]} for name, v in mv.items()}
 
+    # ---- what a plain device copy reaches on this GPU (context for the fractions of the 8 TB/s peak above) ----------
+    extra["hbm_copy"] = hbm_copy_rate()
+
     # ---- CPU baseline: the oracle's FP64 FFT matvec on one host core, bounded sample of this workload ---------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -283,6 +292,29 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def hbm_copy_rate(nbytes=1 << 30, nrep=20):
+    """Read + write GB/s of a device-to-device copy of nbytes (torch's copy kernel, HIP events), and of a
+    read-only sum and a write-only fill of the same buffer: the achievable ceilings next to the 8 TB/s peak."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    out = {"bytes": nbytes}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("fill", lambda: b.fill_(1.0), nbytes),
+                            ("read", lambda: torch.sum(a), nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(nrep):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_GBps"] = round(moved * nrep / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (/proc/cpuinfo), or None."""
     try:
@@ -312,22 +344,53 @@ def measured_traffic(pass_index, cfg):
     return None
 
 
-def bench_batched(A, args):
-    """Batched multi-RHS workload: ellipsoid on a 64^3 grid (semi-axes 1:0.8:0.5, box 64x52x32), m = 1.33+0.001i,
-    default dpl 10|m|, 32 seeded orientations x 2 polarizations = 64 right-hand sides solved together to 1e-10 (FP64
-    and FP32-mixed, HIP-graph replay), against the same 64 solved one after another (ADDA's orientation loop). The FFT
-    grid is adda_hip_fastgrid's (128 along y, where fftFit gives 108), for the batched and the sequential solves alike."""
+def batched_workload(A):
+    """The batched multi-RHS workload: ellipsoid box 64x52x32 (semi-axes 1:0.8:0.5), 32 seeded orientations x 2
+    polarizations; returns (pos, box, props, pols, pars, pers)."""
     from adda_rot import rotate_zyz
     from cases import orientations, PROP_Z, POL_X, POL_Y
     pos, _, box = A.make_particle("ellipsoid", 64, 0.8, 0.5)
-    props, pols = [], []
-    pars, pers = [], []  # scattering plane of each orientation's beam frame, for both polarizations
+    props, pols, pars, pers = [], [], [], []
     for a, b, g in orientations(32):
         prop, pY, pX = (rotate_zyz(a, b, g, v) for v in (PROP_Z, POL_Y, POL_X))
         props += [prop, prop]
         pols += [pY, pX]
         pars += [pY, pY]
         pers += [pX, pX]
+    return pos, box, props, pols, pars, pers
+
+
+def batched_solve_only(A, args):
+    """One batched solve of the 64-RHS workload in one form (--batched-solve), after an untimed one."""
+    pos, box, props, pols, _, _ = batched_workload(A)
+    K = len(props)
+    bp = A.BatchProblem(pos, box, 1.33 + 0.001j, np.array(props), np.array(pols), fp32=True, grid=A.fast_grid(box))
+    try:
+        for prec in (A.F64, A.F32):
+            bp.op.autotune_batch(K, prec, reps=3, cache=args.tune_cache)
+        kw = {"precision": A.PREC_MIXED} if args.batched_solve == "mixed" else {}
+        bp.solve(args.method, eps=1e-10, **kw)
+        bp.op.sync()
+        t = time.perf_counter()
+        st = bp.solve(args.method, eps=1e-10, **kw)
+        bp.op.sync()
+        dt = time.perf_counter() - t
+        print(json.dumps({"mode": "batched-solve", "form": args.batched_solve, "method": args.method, "rhs": K,
+                          "s": round(dt, 4), "iterations_max": max(s["niter"] for s in st),
+                          "matvecs": sum(s["nmatvec"] for s in st), "fp64_matvecs": sum(s["nmatvec_fp64"] for s in st),
+                          "outer_max": max(s["nouter"] for s in st),
+                          "max_true_resid": max(s["true_resid"] for s in st)}), flush=True)
+    finally:
+        bp.close()
+
+
+def bench_batched(A, args):
+    """Batched multi-RHS workload: ellipsoid on a 64^3 grid (semi-axes 1:0.8:0.5, box 64x52x32), m = 1.33+0.001i,
+    default dpl 10|m|, 32 seeded orientations x 2 polarizations = 64 right-hand sides solved together to 1e-10 (FP64
+    and FP32-mixed, HIP-graph replay), against the same 64 solved one after another (ADDA's orientation loop). The FFT
+    grid is adda_hip_fastgrid's (128 along y, where fftFit gives 108), for the batched and the sequential solves alike."""
+    # pars, pers: the scattering plane of each orientation's beam frame, for both polarizations
+    pos, box, props, pols, pars, pers = batched_workload(A)
     K = len(props)
     m = 1.33 + 0.001j
     grid = A.fast_grid(box)

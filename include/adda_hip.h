@@ -153,8 +153,9 @@ int adda_hip_weighted_norm2_batch(adda_hip_op *op, int prec, const void *v, int 
 
 /* Measurement: runs nrep matvecs and times each of the 5 passes with HIP events on the operator's stream;
  * pass_ms[5] receives the average duration of P1 (scatter + FFT-x), P2 (FFT-z), P3 (FFT-y, D-matrix product,
- * inverse FFT-y), P4 (inverse FFT-z), P5 (inverse FFT-x + gather); total_ms the average per matvec.
- * (The role of ADDA's PRECISE_TIMING breakdown, src/matvec.c:147-422.) */
+ * inverse FFT-y), P4 (inverse FFT-z), P5 (inverse FFT-x + gather); total_ms the average per matvec of another nrep
+ * matvecs issued back to back with events at the two ends only (as a solver issues them; the events between the passes
+ * add gaps of their own). (The role of ADDA's PRECISE_TIMING breakdown, src/matvec.c:147-422.) */
 int adda_hip_profile_matvec(adda_hip_op *op, int prec, const void *argvec, void *resultvec, int nrep, double *pass_ms,
 	double *total_ms);
 

@@ -199,7 +199,8 @@ def test_vector_kernels(hipdev):
 
 @pytest.mark.parametrize("cfg", ["x=8.4 y=8.4 z=8.4 LY=1 TC=4 LP=1", "x=4.8 y=4.4.2 z=2.4.4 LY=8 TC=32 LP=2",
                                  "x=2.2.2.4 y=16.2 z=8.2.2 LY=2 TC=8 LP=4", "x=8.4 y=8.4 z=16.2 LY=3 TC=8 LP=1",
-                                 "x=16.2 y=8.4 z=8.4 LY=5 TC=16 LP=1"])
+                                 "x=16.2 y=8.4 z=8.4 LY=5 TC=16 LP=1", "x=8.4 y=16.2 z=8.4 LY=2 TC=8 LP=1 XP=0",
+                                 "x=8.4 y=8.4 z=8.4 LY=4 TC=16 LP=1 XP=0 PT=1"])
 def test_matvec_configs_agree(hipdev, cfg):
     """Every radix schedule / tile configuration the autotuner may pick gives the same product (grid 32^3)."""
     c = _case("sphere", 16, 1.5)

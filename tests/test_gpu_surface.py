@@ -113,7 +113,8 @@ def test_surface_matvec_fp32(hipdev, i):
 
 @pytest.mark.parametrize("cfg", ["x=8.4 y=8.4 z=8.4 LY=1 TC=4 LP=1", "x=4.8 y=4.4.2 z=2.4.4 LY=8 TC=32 LP=2",
                                  "x=2.2.2.4 y=16.2 z=8.2.2 LY=2 TC=8 LP=4", "x=8.4 y=8.4 z=16.2 LY=3 TC=8 LP=1 XP=0",
-                                 "x=16.2 y=8.4 z=8.4 LY=5 TC=16 LP=3 PT=1"])
+                                 "x=16.2 y=8.4 z=8.4 LY=5 TC=16 LP=3 PT=1",
+                                 "x=8.4 y=8.4 z=8.4 LY=4 TC=16 LP=1 XP=0 PT=1"])
 def test_surface_configs_agree(hipdev, cfg):
     """Every P3 tile the substrate's kernel accepts (LP, column pairs on and off, thread counts) gives the same product
     (grid 32^3); the register and global-twiddle P3 variants, which lack the reflected term, are refused."""

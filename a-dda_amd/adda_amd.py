@@ -123,6 +123,7 @@ def hip():
     L.adda_hip_rmatrix_size.argtypes = [P]
     L.adda_hip_rmatrix_size.restype = S
     L.adda_hip_plane_waves.argtypes = [P, D, D, I, P, P, P]
+    L.adda_hip_hbm_rates.argtypes = [I, S, I, P]
     _hip = L
     return L
 
@@ -134,6 +135,13 @@ def check(rc):
 
 def device_count():
     return hip().adda_hip_device_count()
+
+
+def hbm_rates(device=0, nbytes=1 << 30, nrep=20):
+    """GB/s of a plain device copy (read + write bytes), a read-only sum and a fill of nbytes (HIP events)."""
+    out = np.zeros(3)
+    check(hip().adda_hip_hbm_rates(device, nbytes, nrep, _p(out)))
+    return {"bytes": nbytes, "copy_GBps": round(out[0], 1), "read_GBps": round(out[1], 1), "fill_GBps": round(out[2], 1)}
 
 
 class Operator:

@@ -130,6 +130,7 @@ struct MvCfg {
 	int p3g = 0;          // P3 reads the staged twiddles of its y plan from global memory (k_p3 TWG; compile-time plans)
 	int p3t = 0;          // P3 blocks get one thread per D-matrix element (LP*gridY, whole waves) when that is more
 	                      // than the transforms use; the extra threads take part in the product only
+	int p3s = 0;          // P3 lines in unpadded swizzled LDS rows (k_p3 SWZ; LP = 1, compile-time plans, gridY % 16 == 0)
 };
 
 // threads of a P3 block (k_p3: 3 lines per (plane, column) group, LP column units; k_p3r: LP columns, 2*LP groups)
@@ -730,7 +731,7 @@ static size_t lds_p3(const MvCfg &c, int gY, int reduced, size_t es)
 {
 	// twiddles + lines (+ the block's D rows for the register variant, which always reserves both planes' lines)
 	if (c.p3r) return (size_t)(gY + 6 * c.LP * rowstride_lm(gY) + (size_t)c.LP * (reduced ? gY / 2 + 1 : gY) * 6) * es;
-	return (size_t)((c.p3g ? 0 : gY) + 3 * p3_groups(reduced, c.p3x) * c.LP * rowstride_lm(gY)) * es;
+	return (size_t)((c.p3g ? 0 : gY) + 3 * p3_groups(reduced, c.p3x) * c.LP * (c.p3s ? gY : rowstride_lm(gY))) * es;
 }
 static constexpr size_t kLdsMax = 96 * 1024;
 
@@ -742,8 +743,9 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 		if (!k.k[MV_P3R][es == 8] || 2 * c.LP * (op->gY / c.qy.rmax) > 256) return false;
 	}
 	const int tplx = op->gX / c.qx.rmax, tply = op->gY / c.qy.rmax, tplz = op->gZ / c.qz.rmax;
-	if (op->surf && (c.p3r || c.p3g)) return false;  // the substrate's P3 (k_p3sub) keeps its twiddles in LDS
+	if (op->surf && (c.p3r || c.p3g || c.p3s)) return false;  // the substrate's P3 (k_p3sub): twiddles in LDS, padded rows
 	if (c.p3g && (c.p3r || !mv_kernels(c.qy).k[MV_P3G][es == 8])) return false;  // compile-time plans only
+	if (c.p3s && (c.p3r || c.p3g || c.LP != 1 || !mv_kernels(c.qy).k[MV_P3W1][es == 8])) return false;
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
 	if (c.p3t && (c.p3r || p3_threads(c, op->gY, op->reduced) > 512)) return false;
 	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
@@ -812,7 +814,8 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 			make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
 		return;
 	}
-	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? (c.LP == 1 ? MV_P3G1 : MV_P3G) : (c.LP == 1 ? MV_P31 : MV_P3)][f];
+	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? (c.LP == 1 ? MV_P3G1 : MV_P3G) : c.p3s ? MV_P3W1
+		: (c.LP == 1 ? MV_P31 : MV_P3)][f];
 	const cx<T> *tw = tw_of<T>(op->twy);
 	if (c.p3g) {  // the staged table of this plan, written once on the operator's stream
 		const FftPlan2 &h = op->twys_plan[f];
@@ -826,10 +829,14 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 	}
 	const int nthr = p3_threads(c, op->gY, op->reduced);
 	const int ntile = c.p3r ? op->gX / c.LP : (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
-	hipLaunchKernelGGL(k, dim3(nz3 * ntile * K), dim3(nthr),
+	// single columns of the reduced D-matrix: mirror columns paired on one XCD (p3_kernel), the launch padded to whole
+	// groups of 8 pairs
+	const bool mpair = !c.p3r && !c.p3x && op->reduced && c.LP == 1 && op->gX % 2 == 0;
+	const int npairs = nz3 * (op->gX / 2), nblk = mpair ? (npairs + 7) / 8 * 16 : nz3 * ntile;
+	hipLaunchKernelGGL(k, dim3(nblk * K), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw,
 		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
-		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
+		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate, mpair, npairs));
 }
 
 // returns the number of P5 blocks per RHS (the row length of its partial sums)
@@ -2515,8 +2522,8 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
-		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "");
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "", c.p3s ? " SW=1" : "");
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2529,11 +2536,13 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.LY = atoi(ly + 3);
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
-	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG="), *pt = strstr(s, "PT=");
+	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG="), *pt = strstr(s, "PT="),
+		*sw = strstr(s, "SW=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	c.p3x = xp ? atoi(xp + 3) : 1;
 	c.p3g = tg ? atoi(tg + 3) : 0;
 	c.p3t = pt ? atoi(pt + 3) : 0;
+	c.p3s = sw ? atoi(sw + 3) : 0;
 	return true;
 }
 
@@ -2711,7 +2720,8 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 		for (int xp : {1, 0})
 		for (int r3 : {0, 1})
 		for (int tg : {0, 1})
-		for (int pt : {0, 1}) {
+		for (int pt : {0, 1})
+		for (int sw : {0, 1}) {
 			if (!xp && !op->reduced) continue;  // without the reduced form every block holds single columns
 			MvCfg c = best;
 			c.qy = q;
@@ -2720,6 +2730,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			c.p3x = xp;
 			c.p3g = tg;
 			c.p3t = pt;
+			c.p3s = sw;
 			if (pt) {  // without extra threads the launch is the same as PT=0
 				MvCfg c0 = c;
 				c0.p3t = 0;
@@ -2731,6 +2742,7 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			float t = timeit(c, 4u);
 			if (t < ty) {
 				ty = t; best.qy = q; best.LP = LP; best.p3r = r3; best.p3x = xp; best.p3g = tg; best.p3t = pt;
+				best.p3s = sw;
 			}
 		}
 	for (auto &e : ev) (void)hipEventDestroy(e);
@@ -2870,4 +2882,71 @@ extern "C" int adda_hip_sync(adda_hip_op *op)
 	HIPCHK(hipSetDevice(op->dev));
 	HIPCHK(hipStreamSynchronize(op->stream));
 	return 0;
+}
+
+// ==================================================================================================================
+// measurement: achievable HBM rates of plain streaming kernels (16-byte accesses, grid-stride), the ceilings next to
+// the 8 TB/s peak that the matvec passes are compared with
+
+__global__ void __launch_bounds__(256) k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) k_stream_fill(double2 *__restrict__ dst, size_t n, double v)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = make_double2(v, v);
+}
+// read-only: one partial sum per thread, written only when it equals a value the data never holds (keeps the loads)
+__global__ void __launch_bounds__(256) k_stream_read(const double2 *__restrict__ src, size_t n, double *sink)
+{
+	double s = 0;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const double2 v = src[i];
+		s += v.x + v.y;
+	}
+	if (s == -1.0) sink[0] = s;
+}
+
+extern "C" int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *gbps)
+{
+	if (nbytes < (1 << 20) || nrep < 1 || !gbps) return ERR("bad argument");
+	HIPCHK(hipSetDevice(device));
+	const size_t n = nbytes / sizeof(double2);
+	double2 *a = nullptr, *b = nullptr;
+	double *sink = nullptr;
+	hipStream_t s = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = 0;
+	if (hipMalloc(&a, n * sizeof(double2)) != hipSuccess || hipMalloc(&b, n * sizeof(double2)) != hipSuccess ||
+		hipMalloc(&sink, sizeof(double)) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
+		hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+		rc = ERR("allocation of the probe buffers failed");
+	int ncu = 256;
+	(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+	const int nb = ncu * 16;  // 16 blocks of 4 waves per CU: enough loads in flight, grid-stride over the buffer
+	for (int kind = 0; kind < 3 && !rc; kind++) {
+		auto launch = [&]() {
+			if (kind == 0) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, s, b, a, n);
+			else if (kind == 1) hipLaunchKernelGGL(k_stream_read, dim3(nb), dim3(256), 0, s, a, n, sink);
+			else hipLaunchKernelGGL(k_stream_fill, dim3(nb), dim3(256), 0, s, b, n, 1.0);
+		};
+		if (kind == 0) hipLaunchKernelGGL(k_stream_fill, dim3(nb), dim3(256), 0, s, a, n, 0.5);
+		launch();
+		float ms = 0;
+		if (hipEventRecord(e0, s) != hipSuccess) { rc = ERR("event record failed"); break; }
+		for (int r = 0; r < nrep; r++) launch();
+		if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+			hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = ERR("probe kernel failed"); break; }
+		const double moved = (kind == 0 ? 2.0 : 1.0) * n * sizeof(double2) * nrep;
+		gbps[kind] = moved / (ms * 1e-3) / 1e9;
+	}
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	if (s) (void)hipStreamDestroy(s);
+	(void)hipFree(a);
+	(void)hipFree(b);
+	(void)hipFree(sink);
+	return rc;
 }

@@ -385,15 +385,17 @@ __device__ __forceinline__ void surf_side(cx<T> *qa0, cx<T> *qa1, cx<T> *qa2, cx
 // SURF: the particle is above a substrate and R (the transformed image-dipole term, [z < gridZ][xt < DsizeX][yt <
 // DsizeY][6], even or odd in x and y like D, full in z) is added to the product (requires the reduced D-matrix, whose
 // plane pairs put every line's reflected partner into its block).
-template <typename T, class PL, bool FULL, bool SURF = false>
+// SWZ: the lines in unpadded swizzled rows (LaySwz<true>, line stride N; compile-time plans with N % 16 == 0) instead
+// of padded ones: N/16 fewer elements per line, at the cost of an address computation per LDS access.
+template <typename T, class PL, bool FULL, bool SURF = false, bool SWZ = false>
 __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp,
 	const cx<T> *__restrict__ D, int u0, int z, int gX, int bY, int gZ, int DsX, int DsY, int LP, int NL1,
 	unsigned pm, int reduced, int transposed, int line, int t, bool act, const cx<T> *__restrict__ R = nullptr)
 {
-	// padded rows: their offsets stay immediates (a swizzle would cost P3, which is close to VALU-bound, an address
-	// computation per access)
-	using LAY = LayPad;
-	const int N = PL::n(p), LS = rowstride_lm(N);
+	// padded rows by default: their offsets stay immediates (a swizzle costs an address computation per access, which
+	// matters where P3 is close to VALU-bound, in FP32)
+	using LAY = std::conditional_t<SWZ, LaySwz<true>, LayPad>;
+	const int N = PL::n(p), LS = SWZ ? N : rowstride_lm(N);
 	// D-matrix row of element idx = (pr, y) of the block: the stored row index (mirrored for the reduced form and for
 	// the transposed product, src/matvec.c:181 and 270-305) and the sign masks of the y mirror (xy and yz elements,
 	// low half of m) and of a single column beyond DsizeX (xy and xz, high half); the block's own plane never mirrors
@@ -441,10 +443,10 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 			sym3_lds<T, -1, -1>(q(3 * NL1 + l0), q(3 * NL1 + l0 + 1), q(3 * NL1 + l0 + 2), d[0], f1, d2, d[3], f4,
 				d[5]);
 	};
-	// In FP32 the D rows of this thread's first KP elements are loaded before the forward transform, which hides their
-	// latency (KP covers all of them with the reduced D-matrix: blockDim = 12*LP*N/RMAX threads for LP*N elements). In
-	// FP64 the extra registers would cost more occupancy than the prefetch gains.
-	constexpr int KP = sizeof(T) == 4 && !SURF ? (PL::RMAXC + 11) / 12 : 0;
+	// The D rows of this thread's first KP elements are loaded before the forward transform, which hides their latency
+	// (in FP32 KP covers all of them with the reduced D-matrix: blockDim = 12*LP*N/RMAX threads for LP*N elements; in
+	// FP64 one element, all of them with one thread per element, PT=1).
+	constexpr int KP = SURF ? 0 : sizeof(T) == 4 ? (PL::RMAXC + 11) / 12 : 1;
 	cx<T> dpre[KP > 0 ? KP : 1][6];
 	unsigned mpre[KP > 0 ? KP : 1];
 #pragma unroll
@@ -495,7 +497,7 @@ __host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { retu
 // lets one more block share a CU.
 // LPC > 0: the number of column units per block fixed at compile time (LP = 1 is what the autotuner picks at the
 // sizes measured), which turns the group offsets and the group tests of the D-matrix product into constants.
-template <typename T, class PL, bool TWG, int LPC, bool SURF = false>
+template <typename T, class PL, bool TWG, int LPC, bool SURF = false, bool SWZ = false>
 __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LPr, int reduced, int xpair,
 	int transposed, Bq bq, const cx<T> *__restrict__ R = nullptr)
@@ -519,7 +521,18 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 		buf = tl + N;
 	}
 	const int nu = p3_units(gX, DsX, reduced, xpair), nut = (nu + LP - 1) / LP;
-	const int z = tile / nut, u0 = (tile % nut) * LP;
+	int z = tile / nut, u0 = (tile % nut) * LP;
+	if (bq.pair) {
+		// single columns of the reduced D-matrix (LP = 1, even gridX): column x and its mirror gridX - x read the same
+		// stored D row, so they go 8 blocks apart -- on one XCD at about the same time -- and the second read is served
+		// by that XCD's L2. Pair p of plane z is (p, gridX - p) for 0 < p < gridX/2 and (0, gridX/2) for p = 0; the
+		// launch is padded to whole groups of 16 blocks (8 pairs).
+		const int hp = gX / 2, pi = (tile >> 4) * 8 + (tile & 7), mem = (tile >> 3) & 1;
+		if (pi >= bq.ntile) return;
+		z = pi / hp;
+		const int pp = pi - z * hp;
+		u0 = mem ? (pp ? gX - pp : hp) : pp;
+	}
 	// blocks run over z <= gridZ/2 (< DsizeZ) with the reduced D-matrix; mirror planes and columns are handled alongside
 	const bool hz = reduced && z != 0 && 2 * z != gZ;
 	unsigned pm = 0;
@@ -535,19 +548,20 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 	const int zz = (s & 1) ? gZ - z : z, xx = (s & 2) ? gX - (u0 + pr) : u0 + pr;
 	cx<T> *lp = act ? Xb + ((size_t)zz * gX + xx) * 3 * bY + line % 3 : Xb;
 	if (PL::HALF_IN && PL::HALF_OUT && 2 * bY == N)
-		p3_body<T, PL, true, SURF>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
+		p3_body<T, PL, true, SURF, SWZ>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
 			line, t, act, R);
 	else
-		p3_body<T, PL, false, SURF>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
+		p3_body<T, PL, false, SURF, SWZ>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
 			transposed, line, t, act, R);
 }
 
-template <typename T, class PL, int LPC = 0>
+template <typename T, class PL, int LPC = 0, bool SWZ = false>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
 	int transposed, Bq bq)
 {
-	p3_kernel<T, PL, false, LPC>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed, bq);
+	p3_kernel<T, PL, false, LPC, false, SWZ>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed,
+		bq);
 }
 // P3 for a particle above a substrate: the D-matrix product plus the reflected term R (twiddles in LDS, any LP)
 template <typename T, class PL>
@@ -956,9 +970,10 @@ template <typename T, class PL> constexpr bool p3r_ok()
 // plans only).
 // MV_P31 / MV_P3G1: the same two P3 kernels compiled for one column unit per block (LP = 1).
 // MV_P3S: P3 with the substrate's reflected term (k_p3sub).
+// MV_P3W1: P3 with one column unit per block and the lines in unpadded swizzled rows (compile-time plans, N % 16 == 0).
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
-	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_KINDS = 15
+	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_P3W1 = 15, MV_KINDS = 16
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -986,8 +1001,9 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_P3S][f] = (const void *)&k_p3sub<T, PL>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
-	m.k[MV_P3G][f] = m.k[MV_TWST][f] = m.k[MV_P3G1][f] = nullptr;
+	m.k[MV_P3G][f] = m.k[MV_TWST][f] = m.k[MV_P3G1][f] = m.k[MV_P3W1][f] = nullptr;
 	if constexpr (is_fix<PL>::value) {
+		if constexpr (PL::SWZ) m.k[MV_P3W1][f] = (const void *)&k_p3<T, PL, 1, true>;
 		m.k[MV_P3G][f] = (const void *)&k_p3g<T, PL>;
 		m.k[MV_P3G1][f] = (const void *)&k_p3g<T, PL, 1>;
 		m.k[MV_TWST][f] = (const void *)&k_stage_tw<T, PL>;
@@ -1028,6 +1044,8 @@ template <typename T, class PL> hipError_t mv_lds_t()
 			if (lim((const void *)k_p1s<T, PL, double>) || lim((const void *)k_p5s<T, PL, false, double>) ||
 				lim((const void *)k_p5s<T, PL, true, double>)) return e;
 		if (lim((const void *)k_p3g<T, PL>) || lim((const void *)k_p3g<T, PL, 1>)) return e;
+		if constexpr (PL::SWZ)
+			if (lim((const void *)k_p3<T, PL, 1, true>)) return e;
 	}
 	else {
 		if (lim((const void *)k_p1<T, PL>) || lim((const void *)k_p5<T, PL, false>) ||

@@ -144,7 +144,7 @@ def main():
         a, b = op.alloc(A.F64), op.alloc(A.F64)
         A.check(A.hip().adda_hip_vec_convert(op.h, b, A.F64, a, A.F64))  # plain copy of 3*nd*16 bytes
         op.sync()
-        print(json.dumps({"mode": "matvec-only", "copy_bytes": 3 * nd * 16, "matvec": {
+        print(json.dumps({"mode": "matvec-only", "copy_bytes": 3 * nd * 16, "hbm": A.hbm_rates(), "matvec": {
             k: {"us": round(v["ms"] * 1e3, 2), "pass_us": [round(x * 1e3, 2) for x in v["passes"]],
                 "bytes": v["total_bytes"], "pass_bytes": v["bytes"]} for k, v in mv.items()}, "config": tune}),
               flush=True)
@@ -249,7 +249,7 @@ def main():
                           "pass_us": [round(p * 1e3, 2) for p in v["passes"]]} for name, v in mv.items()}
 
     # ---- what a plain device copy reaches on this GPU (context for the fractions of the 8 TB/s peak above) ----------
-    extra["hbm_copy"] = hbm_copy_rate()
+    extra["hbm_copy"] = A.hbm_rates()
 
     # ---- CPU baseline: the oracle's FP64 FFT matvec on one host core, bounded sample of this workload ---------------
     cpu = None
@@ -290,29 +290,6 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)
-
-
-def hbm_copy_rate(nbytes=1 << 30, nrep=20):
-    """Read + write GB/s of a device-to-device copy of nbytes (torch's copy kernel, HIP events), and of a
-    read-only sum and a write-only fill of the same buffer: the achievable ceilings next to the 8 TB/s peak."""
-    import torch
-    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").uniform_()
-    b = torch.empty_like(a)
-    out = {"bytes": nbytes}
-    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("fill", lambda: b.fill_(1.0), nbytes),
-                            ("read", lambda: torch.sum(a), nbytes)):
-        fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(nrep):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        out[name + "_GBps"] = round(moved * nrep / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
-    del a, b
-    torch.cuda.empty_cache()
-    return out
 
 
 def cpu_model():

@@ -270,6 +270,11 @@ int adda_hip_vec_download_part(adda_hip_op *op, int prec, adda_cplx *dst, const 
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 
+/* Measurement, no counterpart in the reference: achievable HBM rates on `device` of a plain copy, a read-only sum and a
+ * fill of nbytes (16-byte accesses, grid-stride), timed with HIP events over nrep launches each; gbps[0..2] = copy
+ * (read + write bytes), read, fill in GB/s. The ceilings the matvec passes are compared with beside the 8 TB/s peak. */
+int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

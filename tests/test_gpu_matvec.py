@@ -308,3 +308,11 @@ def test_matvec_compiled_plans(hipdev, axis, n):
                     assert op.get_config(prec) == " ".join(vp)
                     yp, _ = op.matvec(x, prec=prec)
                     assert np.array_equal(yp, y), (prec, vp)
+                    # the lines in unpadded swizzled LDS rows: the same arithmetic at other LDS addresses, so the same
+                    # bits (one column unit per block, lengths divisible by 16)
+                    if n % 16 == 0:
+                        vs = vp + ["SW=1"]
+                        op.set_config(" ".join(vs), prec)
+                        assert op.get_config(prec) == " ".join(vs)
+                        ys, _ = op.matvec(x, prec=prec)
+                        assert np.array_equal(ys, y), (prec, vs)

@@ -131,6 +131,7 @@ struct MvCfg {
 	int p3t = 0;          // P3 blocks get one thread per D-matrix element (LP*gridY, whole waves) when that is more
 	                      // than the transforms use; the extra threads take part in the product only
 	int p3s = 0;          // P3 lines in unpadded swizzled LDS rows (k_p3 SWZ; LP = 1, compile-time plans, gridY % 16 == 0)
+	int p3k = 1;          // right-hand sides per P3 block in batched launches (they share its D rows and twiddles)
 };
 
 // threads of a P3 block (k_p3: 3 lines per (plane, column) group, LP column units; k_p3r: LP columns, 2*LP groups)
@@ -192,6 +193,11 @@ struct adda_hip_op {
 	double *red = nullptr;       // [4*kcap] device result
 	double *red_host = nullptr;  // [4*kcap] pinned
 	void *hv_arg = nullptr, *hv_res = nullptr;  // device copies of adda_hip_matvec_host's vectors (made at first use)
+	// vectors released by adda_hip_vec_free, kept for the next allocation of the same size (a solver allocates its work
+	// vectors at every start, and the mixed forms start one inner solve per correction step: hipMalloc and hipFree of
+	// hundreds of MB each time, and hipFree's synchronisation, would sit between the steps); freed at destroy
+	std::vector<std::pair<size_t, void *>> vcache;
+	std::vector<std::pair<void *, size_t>> vlive;  // vectors handed out by adda_hip_vec_alloc[_batch], with their sizes
 };
 
 static int nblocks_red(size_t n) { size_t b = (n + 255) / 256; return (int)(b > kRedBlocks ? kRedBlocks : (b ? b : 1)); }
@@ -746,6 +752,8 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 	if (op->surf && (c.p3r || c.p3g || c.p3s)) return false;  // the substrate's P3 (k_p3sub): twiddles in LDS, padded rows
 	if (c.p3g && (c.p3r || !mv_kernels(c.qy).k[MV_P3G][es == 8])) return false;  // compile-time plans only
 	if (c.p3s && (c.p3r || c.p3g || c.LP != 1 || !mv_kernels(c.qy).k[MV_P3W1][es == 8])) return false;
+	// several RHS per P3 block: the generic kernel of padded rows with its twiddles in LDS
+	if (c.p3k < 1 || c.p3k > 16 || (c.p3k > 1 && (c.p3r || c.p3g || c.p3s))) return false;
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
 	if (c.p3t && (c.p3r || p3_threads(c, op->gY, op->reduced) > 512)) return false;
 	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
@@ -814,8 +822,9 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 			make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
 		return;
 	}
+	const int kl = c.p3r ? 1 : std::min(c.p3k, K);  // groups of kl consecutive RHS per block
 	auto k = (P3Fn<T>)c.ky.k[c.p3r ? MV_P3R : c.p3g ? (c.LP == 1 ? MV_P3G1 : MV_P3G) : c.p3s ? MV_P3W1
-		: (c.LP == 1 ? MV_P31 : MV_P3)][f];
+		: kl > 1 ? MV_P3M : (c.LP == 1 ? MV_P31 : MV_P3)][f];
 	const cx<T> *tw = tw_of<T>(op->twy);
 	if (c.p3g) {  // the staged table of this plan, written once on the operator's stream
 		const FftPlan2 &h = op->twys_plan[f];
@@ -833,10 +842,12 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 	// groups of 8 pairs
 	const bool mpair = !c.p3r && !c.p3x && op->reduced && c.LP == 1 && op->gX % 2 == 0;
 	const int npairs = nz3 * (op->gX / 2), nblk = mpair ? (npairs + 7) / 8 * 16 : nz3 * ntile;
-	hipLaunchKernelGGL(k, dim3(nblk * K), dim3(nthr),
+	const int nkg = (K + kl - 1) / kl;
+	Bq bq = make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate, mpair, npairs);
+	bq.kl = kl;
+	hipLaunchKernelGGL(k, dim3(nblk * nkg), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw,
-		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
-		make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate, mpair, npairs));
+		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed, bq);
 }
 
 // returns the number of P5 blocks per RHS (the row length of its partial sums)
@@ -1711,11 +1722,13 @@ extern "C" void adda_hip_op_destroy(adda_hip_op *op)
 	if (!op) return;
 	(void)hipSetDevice(op->dev);
 	if (op->stream) (void)hipStreamSynchronize(op->stream);
-	void *ptrs[] = {op->R64, op->R32, op->hv_arg, op->hv_res, op->partials2, op->cellmap, op->rows, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
+	void *ptrs[] = {op->R64, op->R32, op->partials2, op->cellmap, op->rows, op->mat, op->pos, op->wbuf, op->dirbuf, op->cc64, op->cc32, op->D64,
 		op->D32, op->Xa, op->Xb, op->partials, op->red, op->coef, op->twx.d, op->twx.f, op->twy.d, op->twy.f, op->twz.d,
 		op->twz.f, op->twys.d, op->twys.f};
 	for (void *p : ptrs)
 		if (p) (void)hipFree(p);
+	for (auto &e : op->vcache) (void)hipFree(e.second);
+	for (auto &e : op->vlive) (void)hipFree(e.first);  // the vectors still handed out (hv_arg and hv_res among them)
 	if (op->red_host) (void)hipHostFree(op->red_host);
 	if (op->stream) (void)hipStreamDestroy(op->stream);
 	delete op;
@@ -1884,19 +1897,40 @@ extern "C" void *adda_hip_vec_alloc_batch(adda_hip_op *op, int prec, int K)
 	if (hipSetDevice(op->dev) != hipSuccess) { ERR("hipSetDevice failed"); return nullptr; }
 	void *p = nullptr;
 	const size_t bytes = vec_elems(op, K) * elsize(prec);
-	hipError_t e = hipMalloc(&p, bytes);
-	if (e != hipSuccess) { ERR("hipMalloc of a vector failed: %s", hipGetErrorString(e)); return nullptr; }
+	for (size_t i = 0; i < op->vcache.size(); i++)
+		if (op->vcache[i].first == bytes) {  // a released vector of this size: reused in stream order
+			p = op->vcache[i].second;
+			op->vcache.erase(op->vcache.begin() + i);
+			break;
+		}
+	if (!p) {
+		hipError_t e = hipMalloc(&p, bytes);
+		if (e != hipSuccess && !op->vcache.empty()) {  // release the kept vectors and try once more
+			(void)hipStreamSynchronize(op->stream);
+			for (auto &c : op->vcache) (void)hipFree(c.second);
+			op->vcache.clear();
+			e = hipMalloc(&p, bytes);
+		}
+		if (e != hipSuccess) { ERR("hipMalloc of a vector failed: %s", hipGetErrorString(e)); return nullptr; }
+	}
+	op->vlive.emplace_back(p, bytes);
 	if (hipMemsetAsync(p, 0, bytes, op->stream) != hipSuccess) { ERR("memset failed"); return nullptr; }
 	return p;
 }
 
+// The vector is kept for a later allocation of the same size (work queued on the operator's stream may still use it;
+// a reuse is ordered after that work on the same stream), and freed at adda_hip_op_destroy.
 extern "C" int adda_hip_vec_free(adda_hip_op *op, void *v)
 {
 	if (!op) return ERR("null operator");
-	HIPCHK(hipSetDevice(op->dev));
-	HIPCHK(hipStreamSynchronize(op->stream));
-	if (v) HIPCHK(hipFree(v));
-	return 0;
+	if (!v) return 0;
+	for (size_t i = 0; i < op->vlive.size(); i++)
+		if (op->vlive[i].first == v) {
+			op->vcache.emplace_back(op->vlive[i].second, v);
+			op->vlive.erase(op->vlive.begin() + i);
+			return 0;
+		}
+	return ERR("adda_hip_vec_free: %p is not a vector of this operator", v);
 }
 
 extern "C" int adda_hip_vec_upload(adda_hip_op *op, int prec, void *dst, const adda_cplx *src)
@@ -2522,8 +2556,10 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
-		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "", c.p3s ? " SW=1" : "");
+	char kl[16] = "";
+	if (c.p3k > 1) snprintf(kl, sizeof kl, " KL=%d", c.p3k);
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "", c.p3s ? " SW=1" : "", kl);
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2537,12 +2573,13 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
 	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG="), *pt = strstr(s, "PT="),
-		*sw = strstr(s, "SW=");
+		*sw = strstr(s, "SW="), *kl = strstr(s, "KL=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	c.p3x = xp ? atoi(xp + 3) : 1;
 	c.p3g = tg ? atoi(tg + 3) : 0;
 	c.p3t = pt ? atoi(pt + 3) : 0;
 	c.p3s = sw ? atoi(sw + 3) : 0;
+	c.p3k = kl ? atoi(kl + 3) : 1;
 	return true;
 }
 
@@ -2745,6 +2782,18 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 				best.p3s = sw;
 			}
 		}
+	// batched launches: right-hand sides per P3 block, with the rest of the y configuration fixed
+	for (int kl : {2, 4, 8}) {
+		if (kl > K) break;
+		MvCfg c = best;
+		c.p3k = kl;
+		c.p3s = c.p3g = 0;  // the kernel of several RHS per block has padded rows and LDS twiddles
+		if (!cfg_ok(op, c, es)) continue;
+		resolve(c);
+		ncand++;
+		float t = timeit(c, 4u);
+		if (t < ty) { ty = t; best.p3k = kl; best.p3s = best.p3g = 0; }
+	}
 	for (auto &e : ev) (void)hipEventDestroy(e);
 	HIPCHK(hipStreamSynchronize(op->stream));
 	HIPCHK(hipFree(a));
@@ -2888,23 +2937,40 @@ extern "C" int adda_hip_sync(adda_hip_op *op)
 // measurement: achievable HBM rates of plain streaming kernels (16-byte accesses, grid-stride), the ceilings next to
 // the 8 TB/s peak that the matvec passes are compared with
 
+// Each thread moves U 16-byte elements per round (loads first, then stores), so U accesses per thread are in flight.
+constexpr int kStreamU = 4;
 __global__ void __launch_bounds__(256) k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n)
 {
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-		dst[i] = src[i];
+	const size_t step = (size_t)gridDim.x * blockDim.x;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step) {
+		double2 v[kStreamU];
+#pragma unroll
+		for (int u = 0; u < kStreamU; u++)
+			if (i + u * step < n) v[u] = src[i + u * step];
+#pragma unroll
+		for (int u = 0; u < kStreamU; u++)
+			if (i + u * step < n) dst[i + u * step] = v[u];
+	}
 }
 __global__ void __launch_bounds__(256) k_stream_fill(double2 *__restrict__ dst, size_t n, double v)
 {
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-		dst[i] = make_double2(v, v);
+	const size_t step = (size_t)gridDim.x * blockDim.x;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step)
+#pragma unroll
+		for (int u = 0; u < kStreamU; u++)
+			if (i + u * step < n) dst[i + u * step] = make_double2(v, v);
 }
 // read-only: one partial sum per thread, written only when it equals a value the data never holds (keeps the loads)
 __global__ void __launch_bounds__(256) k_stream_read(const double2 *__restrict__ src, size_t n, double *sink)
 {
+	const size_t step = (size_t)gridDim.x * blockDim.x;
 	double s = 0;
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-		const double2 v = src[i];
-		s += v.x + v.y;
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step) {
+		double2 v[kStreamU];
+#pragma unroll
+		for (int u = 0; u < kStreamU; u++) v[u] = i + u * step < n ? src[i + u * step] : make_double2(0, 0);
+#pragma unroll
+		for (int u = 0; u < kStreamU; u++) s += v[u].x + v[u].y;
 	}
 	if (s == -1.0) sink[0] = s;
 }
@@ -2925,7 +2991,7 @@ extern "C" int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *g
 		rc = ERR("allocation of the probe buffers failed");
 	int ncu = 256;
 	(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-	const int nb = ncu * 16;  // 16 blocks of 4 waves per CU: enough loads in flight, grid-stride over the buffer
+	const int nb = ncu * 8;  // 8 blocks of 4 waves per CU, grid-stride over the buffer
 	for (int kind = 0; kind < 3 && !rc; kind++) {
 		auto launch = [&]() {
 			if (kind == 0) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, s, b, a, n);

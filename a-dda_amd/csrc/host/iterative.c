@@ -531,8 +531,10 @@ done:
 
 /* ---- device-resident recurrences, K right-hand sides at once ---------------------------------------------------
  * CalcInitField and PHASE_INIT as in krylov() (host, synchronising, once), then whole iterations enqueued with
- * adda_hip_dk_*_iter -- whose FP64 scalar recurrences run on the GPU -- in chunks of DK_CHUNK, polling the device
- * status slots after each chunk. Iterations enqueued after convergence are no-ops on the device. With K > 1 every
+ * adda_hip_dk_*_iter -- whose FP64 scalar recurrences run on the GPU -- in chunks of up to DK_CHUNK, polling the
+ * device status slots after each chunk. Iterations enqueued after convergence are no-ops on the device, but they
+ * still cost their launches: near convergence the chunks shrink to the number of iterations the residuals' decay
+ * over the last chunk predicts (the slowest running RHS; at least one). With K > 1 every
  * launch advances all K solves (vectors batched at adda_hip_batch_stride); each RHS has its own scalars, tolerance,
  * status and history, so the K results equal K separate solves. Results are identical in exact arithmetic to
  * krylov() (same recurrences, same order of operations per vector element; reductions differ only in summation
@@ -554,10 +556,10 @@ static const char *method_name(int m)
 	return m >= 0 && m < 7 ? n[m] : "?";
 }
 
-static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int *it, int matvec_ready, void **v,
+static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int n, int *it, int matvec_ready, void **v,
 	void **vtilda, void *Av, void **p_old, void **p_new, void *x, void *r, void *p, void *v1, void *v2, void *v3)
 {
-	for (int c = 0; c < DK_CHUNK; c++, (*it)++) {
+	for (int c = 0; c < n; c++, (*it)++) {
 		if (method == ADDA_IT_QMR_CS) {
 			if (adda_hip_dk_qmr_iter(op, prec, st, *it, matvec_ready, *v, *vtilda, Av, *p_old, *p_new, x, r)) return -1;
 			if (*it >= 2) { void *t = *p_old; *p_old = *p_new; *p_new = t; }
@@ -687,31 +689,50 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 #undef SETC
 	{
 		void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;
-		int it = 1, running = 0;
-		for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+		int it = 1, running = 0, chunk = DK_CHUNK, tail = 0;
+		for (int k = 0; k < K; k++) {
+			running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+			inprodR[k] = S[(size_t)k * NS];  /* the residual before the chunk, for the decay rate */
+		}
 		struct timespec tp0, tp1;
 		clock_gettime(CLOCK_MONOTONIC, &tp0);
 		while (running) {
-			if (use_graph && it > 3) {
-				if (!graph) {
-					/* record the chunk (nothing runs while capturing); the pointer rotation of a chunk is the
-					 * identity (DK_CHUNK is even), so the recorded chunk is valid for every later one */
-					int it0 = it;
-					CKG(adda_hip_capture_begin(op));
-					int erc = enqueue_chunk(op, prec, method, st, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x,
-						r, p, v1, v2, v3);
-					graph = adda_hip_capture_end(op);
-					if (erc || !graph) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
-					it = it0;
-				}
+			/* full chunks replay the recorded graph; the shorter chunks near convergence (tail) are enqueued directly,
+			 * and the graph -- recorded for one pointer rotation of QMR_CS -- is not replayed after them */
+			if (graph && !tail && chunk == DK_CHUNK) {
 				CKG(adda_hip_graph_launch(op, graph));
 				it += DK_CHUNK;
 			}
-			else CKG(enqueue_chunk(op, prec, method, st, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x, r, p,
-				v1, v2, v3));
+			else CKG(enqueue_chunk(op, prec, method, st, chunk, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x,
+				r, p, v1, v2, v3));
+			if (use_graph && !graph && !tail && it > 3) {
+				/* record the next full chunk while the GPU runs the one just enqueued (a capture runs nothing); the
+				 * pointer rotation of a chunk is the identity (DK_CHUNK is even), so the recorded chunk is valid for
+				 * every later one */
+				int it0 = it;
+				CKG(adda_hip_capture_begin(op));
+				int erc = enqueue_chunk(op, prec, method, st, DK_CHUNK, &it, matvec_ready, &v, &vtilda, Av, &p_old,
+					&p_new, x, r, p, v1, v2, v3);
+				graph = adda_hip_capture_end(op);
+				if (erc || !graph) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+				it = it0;
+			}
+			const int ran = chunk;
 			CKG(adda_hip_dk_get(op, st, 0, S, K * NS));
 			running = 0;
-			for (int k = 0; k < K; k++) running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
+			/* iterations the slowest running RHS still needs at the decay rate of its best residual over this chunk */
+			double need = 0;
+			for (int k = 0; k < K; k++) {
+				const double *sk = S + (size_t)k * NS;
+				if (sk[ADDA_DK_STATUS] != 0) continue;
+				running = 1;
+				const double r0 = inprodR[k], r1 = sk[0], eb = sk[2];
+				const double n = r1 < r0 && r1 > 0 && eb > 0 ? ran * log(r1 / eb) / log(r0 / r1) : DK_CHUNK;
+				if (n > need) need = n;
+				inprodR[k] = r1;
+			}
+			chunk = need >= DK_CHUNK ? DK_CHUNK : need < 1 ? 1 : (int)ceil(need);
+			if (chunk < DK_CHUNK) tail = 1;
 			if (verbose & 2) { /* progress of a long solve: the largest relative residual of the running RHS */
 				clock_gettime(CLOCK_MONOTONIC, &tp1);
 				if (tp1.tv_sec - tp0.tv_sec + 1e-9 * (tp1.tv_nsec - tp0.tv_nsec) >= 10) {

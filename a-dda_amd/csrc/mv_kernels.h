@@ -20,6 +20,7 @@ struct Bq {
 	const double *gate;  // RHS k is skipped when gate[k*DK_NSLOTS + DK_STATUS] != 0 (null: never)
 	int pair = 0;        // P5: launch-order remap of the tiles (pair_tile), the launch padded to whole groups of 16
 	int ntile = 0;       // P5 with pair: the number of real tiles (the padding blocks write zero partial sums)
+	int kl = 1;          // P3: right-hand sides per block (consecutive RHS sharing the block's D rows and twiddles)
 };
 // Tile of the b-th block of a launch whose tiles 2p and 2p + 1 sit 8 blocks apart (groups of 16 blocks: the even
 // member of 8 pairs, then the odd ones). Blocks are dealt to the 8 XCDs in turn, so a pair shares one XCD and its L2.
@@ -387,10 +388,13 @@ __device__ __forceinline__ void surf_side(cx<T> *qa0, cx<T> *qa1, cx<T> *qa2, cx
 // plane pairs put every line's reflected partner into its block).
 // SWZ: the lines in unpadded swizzled rows (LaySwz<true>, line stride N; compile-time plans with N % 16 == 0) instead
 // of padded ones: N/16 fewer elements per line, at the cost of an address computation per LDS access.
-template <typename T, class PL, bool FULL, bool SURF = false, bool SWZ = false>
-__device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp,
+// MULTI: the block runs bq.kl right-hand sides in turn (batched launches); otherwise one (a loop the compiler cannot
+// see through keeps loop-invariant addresses live across it, which costs registers and occupancy).
+template <typename T, class PL, bool FULL, bool SURF = false, bool SWZ = false, bool MULTI = false>
+__device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftPlan2 &p, cx<T> *lp0,
 	const cx<T> *__restrict__ D, int u0, int z, int gX, int bY, int gZ, int DsX, int DsY, int LP, int NL1,
-	unsigned pm, int reduced, int transposed, int line, int t, bool act, const cx<T> *__restrict__ R = nullptr)
+	unsigned pm, int reduced, int transposed, int line, int t, bool act, const Bq &bq, int k0,
+	const cx<T> *__restrict__ R = nullptr)
 {
 	// padded rows by default: their offsets stay immediates (a swizzle costs an address computation per access, which
 	// matters where P3 is close to VALU-bound, in FP32)
@@ -447,41 +451,50 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	// (in FP32 KP covers all of them with the reduced D-matrix: blockDim = 12*LP*N/RMAX threads for LP*N elements; in
 	// FP64 one element, all of them with one thread per element, PT=1).
 	constexpr int KP = SURF ? 0 : sizeof(T) == 4 ? (PL::RMAXC + 11) / 12 : 1;
-	cx<T> dpre[KP > 0 ? KP : 1][6];
-	unsigned mpre[KP > 0 ? KP : 1];
+	// the block's RHS k0 .. k0 + kl - 1 (those that exist and are not gated off) in turn; the skip test is uniform over
+	// the block, so every thread meets the same barriers. The D rows are loaded again for each RHS (from the cache after
+	// the first): held across the loop they would stay live through the inverse transform, at the cost of occupancy.
+	for (int kk = 0; kk < (MULTI ? bq.kl : 1); kk++) {
+		const int kq = k0 + kk;
+		if (kq >= bq.K || skip_rhs(bq, kq)) continue;
+		if (kk > 0) __syncthreads();  // the previous RHS's last LDS reads precede this forward transform's LDS writes
+		cx<T> dpre[KP > 0 ? KP : 1][6];
+		unsigned mpre[KP > 0 ? KP : 1];
 #pragma unroll
-	for (int k = 0; k < KP; k++) {
-		const int idx = threadIdx.x + k * blockDim.x;
-		if (idx < LP * N && ((pm >> (idx / N)) & 1)) {
-			const cx<T> *dm = drow(idx, mpre[k]);
+		for (int k = 0; k < KP; k++) {
+			const int idx = threadIdx.x + k * blockDim.x;
+			if (idx < LP * N && ((pm >> (idx / N)) & 1)) {
+				const cx<T> *dm = drow(idx, mpre[k]);
 #pragma unroll
-			for (int e = 0; e < 6; e++) dpre[k][e] = dm[e];
+				for (int e = 0; e < 6; e++) dpre[k][e] = dm[e];
+			}
 		}
-	}
-	auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
-	auto stl = [&](int l, int y, cx<T> v) { buf[LAY::pos(l, y, 1, LS)] = v; };
-	PL::template fft<T, false, true, false, true, false, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
-	__syncthreads();
+		cx<T> *lp = lp0 + (size_t)kq * bq.xs;
+		auto ldg = [&](int, int y) -> cx<T> { return (FULL || y < bY) ? lp[3 * y] : mk<T>(0, 0); };
+		auto stl = [&](int l, int y, cx<T> v) { buf[LAY::pos(l, y, 1, LS)] = v; };
+		PL::template fft<T, false, true, false, true, false, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldg, stl);
+		__syncthreads();
 #pragma unroll
-	for (int k = 0; k < KP; k++) {
-		const int idx = threadIdx.x + k * blockDim.x;
-		if (idx < LP * N) dprod(idx, dpre[k], mpre[k]);
-	}
-	for (int idx = threadIdx.x + KP * blockDim.x; idx < LP * N; idx += blockDim.x) {
-		if (!((pm >> (idx / N)) & 1)) continue;
-		unsigned m;
-		const cx<T> *dm = drow(idx, m);
-		cx<T> d[6];
+		for (int k = 0; k < KP; k++) {
+			const int idx = threadIdx.x + k * blockDim.x;
+			if (idx < LP * N) dprod(idx, dpre[k], mpre[k]);
+		}
+		for (int idx = threadIdx.x + KP * blockDim.x; idx < LP * N; idx += blockDim.x) {
+			if (!((pm >> (idx / N)) & 1)) continue;
+			unsigned m;
+			const cx<T> *dm = drow(idx, m);
+			cx<T> d[6];
 #pragma unroll
-		for (int e = 0; e < 6; e++) d[e] = dm[e];
-		dprod(idx, d, m);
+			for (int e = 0; e < 6; e++) d[e] = dm[e];
+			dprod(idx, d, m);
+		}
+		__syncthreads();
+		auto ldl = [&](int l, int y) -> cx<T> { return buf[LAY::pos(l, y, 1, LS)]; };
+		auto stg = [&](int, int y, cx<T> v) {
+			if (FULL || y < bY) lp[3 * y] = v;
+		};
+		PL::template fft<T, true, false, true, false, true, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 	}
-	__syncthreads();
-	auto ldl = [&](int l, int y) -> cx<T> { return buf[LAY::pos(l, y, 1, LS)]; };
-	auto stg = [&](int, int y, cx<T> v) {
-		if (FULL || y < bY) lp[3 * y] = v;
-	};
-	PL::template fft<T, true, false, true, false, true, LAY>(buf, tws, p, 1, LS, line, t, act, line, t, act, ldl, stg);
 }
 
 // Column units of P3: x < DsizeX with column pairs (each with its mirror), else every x; and the (plane, column)
@@ -497,15 +510,19 @@ __host__ __device__ __forceinline__ int p3_groups(int reduced, int xpair) { retu
 // lets one more block share a CU.
 // LPC > 0: the number of column units per block fixed at compile time (LP = 1 is what the autotuner picks at the
 // sizes measured), which turns the group offsets and the group tests of the D-matrix product into constants.
-template <typename T, class PL, bool TWG, int LPC, bool SURF = false, bool SWZ = false>
+template <typename T, class PL, bool TWG, int LPC, bool SURF = false, bool SWZ = false, bool MULTI = false>
 __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LPr, int reduced, int xpair,
 	int transposed, Bq bq, const cx<T> *__restrict__ R = nullptr)
 {
 	const int LP = LPC > 0 ? LPC : LPr;
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
-	if (skip_rhs(bq, kq)) return;
-	Xb += kq * bq.xs;
+	// blocks of kl consecutive RHS (the group index fastest, so the groups that share a D tile run together)
+	const int kl = MULTI ? bq.kl : 1, nkg = (bq.K + kl - 1) / kl, k0 = (blockIdx.x % nkg) * kl, tile = blockIdx.x / nkg;
+	{
+		bool any = false;
+		for (int kk = 0; kk < kl && k0 + kk < bq.K; kk++) any |= !skip_rhs(bq, k0 + kk);
+		if (!any) return;
+	}
 	extern __shared__ char smem_raw[];
 	const int N = PL::n(p), TPL = PL::tpl(p), NL1 = 3 * LP;
 	const cx<T> *tws;
@@ -544,24 +561,25 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 	const int tid = threadIdx.x, line = tid / TPL, t = tid % TPL;
 	const int g = line / 3, s = g / LP, pr = g - s * LP;
 	const bool act = (pm >> g) & 1;
-	// line (s, pr, c): element y of it at lp[3*y], lp = &Xb[plane][column][0][c] (inactive lines never access it)
+	// line (s, pr, c): element y of it at lp[3*y], lp = &Xb[plane][column][0][c] of RHS 0 (plus k*xs for RHS k;
+	// inactive lines never access it)
 	const int zz = (s & 1) ? gZ - z : z, xx = (s & 2) ? gX - (u0 + pr) : u0 + pr;
 	cx<T> *lp = act ? Xb + ((size_t)zz * gX + xx) * 3 * bY + line % 3 : Xb;
 	if (PL::HALF_IN && PL::HALF_OUT && 2 * bY == N)
-		p3_body<T, PL, true, SURF, SWZ>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced, transposed,
-			line, t, act, R);
+		p3_body<T, PL, true, SURF, SWZ, MULTI>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
+			transposed, line, t, act, bq, k0, R);
 	else
-		p3_body<T, PL, false, SURF, SWZ>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
-			transposed, line, t, act, R);
+		p3_body<T, PL, false, SURF, SWZ, MULTI>(buf, tws, p, lp, D, u0, z, gX, bY, gZ, DsX, DsY, LP, NL1, pm, reduced,
+			transposed, line, t, act, bq, k0, R);
 }
 
-template <typename T, class PL, int LPC = 0, bool SWZ = false>
+template <typename T, class PL, int LPC = 0, bool SWZ = false, bool MULTI = false>
 __global__ void __launch_bounds__(512) k_p3(cx<T> *Xb, const cx<T> *__restrict__ D, FftPlan2 p,
 	const cx<T> *__restrict__ twg, int gX, int bY, int gZ, int DsX, int DsY, int DsZ, int LP, int reduced, int xpair,
 	int transposed, Bq bq)
 {
-	p3_kernel<T, PL, false, LPC, false, SWZ>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair, transposed,
-		bq);
+	p3_kernel<T, PL, false, LPC, false, SWZ, MULTI>(Xb, D, p, twg, gX, bY, gZ, DsX, DsY, DsZ, LP, reduced, xpair,
+		transposed, bq);
 }
 // P3 for a particle above a substrate: the D-matrix product plus the reflected term R (twiddles in LDS, any LP)
 template <typename T, class PL>
@@ -971,9 +989,11 @@ template <typename T, class PL> constexpr bool p3r_ok()
 // MV_P31 / MV_P3G1: the same two P3 kernels compiled for one column unit per block (LP = 1).
 // MV_P3S: P3 with the substrate's reflected term (k_p3sub).
 // MV_P3W1: P3 with one column unit per block and the lines in unpadded swizzled rows (compile-time plans, N % 16 == 0).
+// MV_P3M: P3 whose blocks run several right-hand sides in turn (batched launches; padded rows, twiddles in LDS).
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
-	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_P3W1 = 15, MV_KINDS = 16
+	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_P3W1 = 15, MV_P3M = 16,
+	MV_KINDS = 17
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -998,6 +1018,7 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
 	m.k[MV_P3][f] = (const void *)&k_p3<T, PL>;
 	m.k[MV_P31][f] = (const void *)&k_p3<T, PL, 1>;
+	m.k[MV_P3M][f] = (const void *)&k_p3<T, PL, 0, false, true>;
 	m.k[MV_P3S][f] = (const void *)&k_p3sub<T, PL>;
 	m.k[MV_P3R][f] = nullptr;
 	if constexpr (p3r_ok<T, PL>()) m.k[MV_P3R][f] = (const void *)&k_p3r<T, PL>;
@@ -1055,6 +1076,7 @@ template <typename T, class PL> hipError_t mv_lds_t()
 				lim((const void *)k_p5<T, PL, true, double>)) return e;
 	}
 	if (lim((const void *)k_p3<T, PL>) || lim((const void *)k_p3<T, PL, 1>) || lim((const void *)k_p3sub<T, PL>) ||
+		lim((const void *)k_p3<T, PL, 0, false, true>) ||
 		lim((const void *)k_zfft<T, false, PL>) ||
 		lim((const void *)k_zfft<T, true, PL>)) return e;
 	if constexpr (p3r_ok<T, PL>())

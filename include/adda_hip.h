@@ -98,7 +98,10 @@ size_t adda_hip_rmatrix_size(const adda_hip_op *op);
 int adda_hip_plane_waves(adda_hip_op *op, double wavenum, double gridspace, int nw, const adda_cplx *kvec,
 	const adda_cplx *amp, void *einc);
 
-/* Device vectors: allocation, host <-> device copies (FP64 host data), precision conversion. */
+/* Device vectors: allocation (zero-filled), host <-> device copies (FP64 host data), precision conversion. A freed
+ * vector is kept by the operator for the next allocation of the same size (reuse is ordered on the operator's stream,
+ * so no synchronisation is needed) and released at adda_hip_op_destroy; freeing a pointer the operator did not hand
+ * out is an error. */
 void *adda_hip_vec_alloc(adda_hip_op *op, int prec);
 int adda_hip_vec_free(adda_hip_op *op, void *v);
 int adda_hip_vec_upload(adda_hip_op *op, int prec, void *dst, const adda_cplx *src);

@@ -48,15 +48,20 @@ def test_batched_kernels_equal_single(hipdev, prec):
         for k in range(K):
             op.upload(op.rhs(a, k, prec), xs[k], prec)
             op.upload(op.rhs(b2, k, prec), xs[(k + 1) % K], prec)
-        for her in (False, True):
-            ip = op.matvec_batch(a, r, K, prec, her=her, inprod=True)
-            yb = [op.download(op.rhs(r, k, prec), prec) for k in range(K)]
-            for k in range(K):
-                op.set_cc_sqrt(bp.cc_sqrt[k])
-                y1, ip1 = op.matvec(xs[k], her=her, prec=prec)
-                assert np.array_equal(y1, yb[k]), (her, k, np.abs(y1 - yb[k]).max())
-                assert ip1 == ip[k]
-            op.set_cc_sqrt_batch(bp.cc_sqrt)
+        base = op.get_config(prec)
+        # P3 with 1, 2 or 4 consecutive RHS per block (KL; K = 5 leaves a ragged last group)
+        for cfg in (base, base + " KL=2", base + " KL=4"):
+            op.set_config(cfg, prec)
+            for her in (False, True):
+                ip = op.matvec_batch(a, r, K, prec, her=her, inprod=True)
+                yb = [op.download(op.rhs(r, k, prec), prec) for k in range(K)]
+                for k in range(K):
+                    op.set_cc_sqrt(bp.cc_sqrt[k])
+                    y1, ip1 = op.matvec(xs[k], her=her, prec=prec)
+                    assert np.array_equal(y1, yb[k]), (cfg, her, k, np.abs(y1 - yb[k]).max())
+                    assert ip1 == ip[k]
+                op.set_cc_sqrt_batch(bp.cc_sqrt)
+        op.set_config(base, prec)
         ca = rng.standard_normal(K) + 1j * rng.standard_normal(K)
         n2 = op.lincomb_batch(K, r, a, ca, b2, 0.5 - 0.25j, None, 0, prec=prec, norm=True)
         d, dn = op.dot_batch(K, a, b2, hipdev.DOT_BILIN, prec=prec)
@@ -168,10 +173,15 @@ def test_batch_of_one_and_gating(hipdev):
         op.lincomb_batch(K, r, None, 0, None, 0, None, 0)  # zero
         sentinel = np.full(3 * bp.nd, 7 - 3j)
         op.upload(op.rhs(r, 1), sentinel)
-        op.matvec_batch(a, r, K, gate=st)
-        assert np.array_equal(op.download(op.rhs(r, 1)), sentinel)
         y1, _ = op.matvec(x)
-        assert np.array_equal(op.download(op.rhs(r, 0)), y1)
+        base = op.get_config(hipdev.F64)
+        # with KL=2 the gated RHS shares its P3 blocks with the running one
+        for cfg in (base, base + " KL=2"):
+            op.set_config(cfg, hipdev.F64)
+            op.upload(op.rhs(r, 1), sentinel)
+            op.matvec_batch(a, r, K, gate=st)
+            assert np.array_equal(op.download(op.rhs(r, 1)), sentinel), cfg
+            assert np.array_equal(op.download(op.rhs(r, 0)), y1), cfg
         hipdev.check(hipdev.hip().adda_hip_dk_free(op.h, st))
         op.free(a)
         op.free(r)

@@ -864,7 +864,8 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	const int pair = nyt % 2 == 0, nblk5 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
 	constexpr bool same = std::is_same_v<T, TV>;
 	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
-	auto k = (Fn)c.kx.k[dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
+	auto k = (Fn)c.kx.k[dmode == 2 ? (same ? MV_P5D2 : MV_P5DW2) : dmode ? (same ? MV_P5D : MV_P5DW)
+		: (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate, pair, op->bZ * nyt));
@@ -965,6 +966,10 @@ static void default_cfg(adda_hip_op *op, int pi, size_t es)
 
 // Fixed-order reduction of nrows (<= 3) rows of nblk partials by one block; row sums land in out[] (thread 0). Each
 // thread adds its strided elements in index order; the loads of four strides of every row are issued together.
+// (Sixteen at a time was measured slower: these one-block kernels are bound by launch and latency, not by the loads.
+// Nor does running the step in the last block of the kernel that writes the partial sums pay off: with thousands of
+// blocks counting in on one device-scope atomic per right-hand side, the atomics serialise -- measured, the FP32 P5
+// with its inner product went from 50 to 140 us.)
 __device__ void dk_reduce(const double *partials, int nblk, int nrows, double *out, double *red_lds)
 {
 	const int B = blockDim.x;

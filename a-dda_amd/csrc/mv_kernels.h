@@ -851,12 +851,13 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 // -- one round trip per thread for dense rows of one material -- and nothing is held in registers through the
 // transform. (Measured: loading the arguments before the transform, beside its own loads, took FP32 P5 from 44 to
 // 47 us -- 99 registers, or a spill under a budget of five waves per SIMD.)
-template <typename T, class PL, bool DOT, typename TV = T>
+template <typename T, class PL, int DM, typename TV = T>
 __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
 	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const RowDesc *__restrict__ rows,
 	const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX,
-	int bY, int LY, int her, double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
+	int bY, int LY, int her, double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int, Bq bq)
 {
+	constexpr bool DOT = DM != 0;
 	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
 	if (skip_rhs(bq, kq)) return;
 	if (bq.pair && tile >= bq.ntile) {  // padding of the paired launch: zero partial sums, nothing else
@@ -925,11 +926,10 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 		else PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 0; });
 		load_args();
 		PL::template for_last<true>(tB, [&](int sl, int) {
-			const size_t i = 3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c;
 			cv[sl] = ccs[mm[sl] + c];
-			// weights through one global pointer (dmode 2 re-reads the argument from cache): a select between a
-			// register copy and a load would go through scratch memory
-			if (DOT) wv[sl] = (dmode == 1 ? dotw : arg)[i];
+			// the weights: another vector (DM 1), or the argument itself (DM 2), already in registers
+			if constexpr (DM == 1) wv[sl] = dotw[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c];
+			else if constexpr (DM == 2) wv[sl] = av[sl];
 		});
 		PL::template for_last<true>(tB, [&](int sl, int) {
 			const int d = dd[sl];
@@ -984,6 +984,8 @@ template <typename T, class PL> constexpr bool p3r_ok()
 
 // The kernels of one plan: k[kind][0] FP64, k[kind][1] FP32 (entry addresses of the instantiations).
 // MV_P1W / MV_P5W / MV_P5DW: FP64 dipole vectors around the FP32 transform (FP32 slot only).
+// MV_P5D / MV_P5DW: P5 with the inner product against dotw (dmode 1); MV_P5D2 / MV_P5DW2 against the argument
+// (dmode 2) -- the same kernel for run-time plans, which take dmode as an argument.
 // MV_P3G: P3 with its twiddles read from global memory; MV_TWST: the kernel that stages that table (compile-time
 // plans only).
 // MV_P31 / MV_P3G1: the same two P3 kernels compiled for one column unit per block (LP = 1).
@@ -993,7 +995,7 @@ template <typename T, class PL> constexpr bool p3r_ok()
 enum {
 	MV_P1 = 0, MV_ZFWD = 1, MV_ZINV = 2, MV_P3 = 3, MV_P5 = 4, MV_P3R = 5, MV_P5D = 6, MV_P1W = 7, MV_P5W = 8,
 	MV_P5DW = 9, MV_P3G = 10, MV_TWST = 11, MV_P31 = 12, MV_P3G1 = 13, MV_P3S = 14, MV_P3W1 = 15, MV_P3M = 16,
-	MV_KINDS = 17
+	MV_P5D2 = 17, MV_P5DW2 = 18, MV_KINDS = 19
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
@@ -1006,13 +1008,14 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 	// compile-time plans: P1/P5 with the gather fused into the first / last FFT stage
 	if constexpr (is_fix<PL>::value) {
 		m.k[MV_P1][f] = (const void *)&k_p1s<T, PL>;
-		m.k[MV_P5][f] = (const void *)&k_p5s<T, PL, false>;
-		m.k[MV_P5D][f] = (const void *)&k_p5s<T, PL, true>;
+		m.k[MV_P5][f] = (const void *)&k_p5s<T, PL, 0>;
+		m.k[MV_P5D][f] = (const void *)&k_p5s<T, PL, 1>;
+		m.k[MV_P5D2][f] = (const void *)&k_p5s<T, PL, 2>;
 	}
 	else {
 		m.k[MV_P1][f] = (const void *)&k_p1<T, PL>;
 		m.k[MV_P5][f] = (const void *)&k_p5<T, PL, false>;
-		m.k[MV_P5D][f] = (const void *)&k_p5<T, PL, true>;
+		m.k[MV_P5D][f] = m.k[MV_P5D2][f] = (const void *)&k_p5<T, PL, true>;
 	}
 	m.k[MV_ZFWD][f] = (const void *)&k_zfft<T, false, PL>;
 	m.k[MV_ZINV][f] = (const void *)&k_zfft<T, true, PL>;
@@ -1029,17 +1032,18 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 		m.k[MV_P3G1][f] = (const void *)&k_p3g<T, PL, 1>;
 		m.k[MV_TWST][f] = (const void *)&k_stage_tw<T, PL>;
 	}
-	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = nullptr;
+	m.k[MV_P1W][f] = m.k[MV_P5W][f] = m.k[MV_P5DW][f] = m.k[MV_P5DW2][f] = nullptr;
 	if constexpr (sizeof(T) == 4) {
 		if constexpr (is_fix<PL>::value) {
 			m.k[MV_P1W][f] = (const void *)&k_p1s<T, PL, double>;
-			m.k[MV_P5W][f] = (const void *)&k_p5s<T, PL, false, double>;
-			m.k[MV_P5DW][f] = (const void *)&k_p5s<T, PL, true, double>;
+			m.k[MV_P5W][f] = (const void *)&k_p5s<T, PL, 0, double>;
+			m.k[MV_P5DW][f] = (const void *)&k_p5s<T, PL, 1, double>;
+			m.k[MV_P5DW2][f] = (const void *)&k_p5s<T, PL, 2, double>;
 		}
 		else {
 			m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
 			m.k[MV_P5W][f] = (const void *)&k_p5<T, PL, false, double>;
-			m.k[MV_P5DW][f] = (const void *)&k_p5<T, PL, true, double>;
+			m.k[MV_P5DW][f] = m.k[MV_P5DW2][f] = (const void *)&k_p5<T, PL, true, double>;
 		}
 	}
 }
@@ -1059,11 +1063,11 @@ template <typename T, class PL> hipError_t mv_lds_t()
 	auto lim = [&](const void *k) { return e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, kMvLdsMax); };
 	// P1/P5: the stage-fused gathers for compile-time plans, the cell-map gathers for run-time plans
 	if constexpr (is_fix<PL>::value) {
-		if (lim((const void *)k_p1s<T, PL>) || lim((const void *)k_p5s<T, PL, false>) ||
-			lim((const void *)k_p5s<T, PL, true>)) return e;
+		if (lim((const void *)k_p1s<T, PL>) || lim((const void *)k_p5s<T, PL, 0>) ||
+			lim((const void *)k_p5s<T, PL, 1>) || lim((const void *)k_p5s<T, PL, 2>)) return e;
 		if constexpr (sizeof(T) == 4)
-			if (lim((const void *)k_p1s<T, PL, double>) || lim((const void *)k_p5s<T, PL, false, double>) ||
-				lim((const void *)k_p5s<T, PL, true, double>)) return e;
+			if (lim((const void *)k_p1s<T, PL, double>) || lim((const void *)k_p5s<T, PL, 0, double>) ||
+				lim((const void *)k_p5s<T, PL, 1, double>) || lim((const void *)k_p5s<T, PL, 2, double>)) return e;
 		if (lim((const void *)k_p3g<T, PL>) || lim((const void *)k_p3g<T, PL, 1>)) return e;
 		if constexpr (PL::SWZ)
 			if (lim((const void *)k_p3<T, PL, 1, true>)) return e;

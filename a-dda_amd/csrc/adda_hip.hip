@@ -787,9 +787,11 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
 	using Fn = std::conditional_t<std::is_same_v<T, TV>, P1Fn<T>, P1WFn>;
 	auto k = (Fn)c.kx.k[std::is_same_v<T, TV> ? MV_P1 : MV_P1W][sizeof(T) == 4];
-	hipLaunchKernelGGL(k, dim3(op->bZ * nyt * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
+	// tiles 2p and 2p + 1 (whose Xa runs share cache lines where they meet) go 8 blocks apart, on one XCD, as in P5
+	const int pair = nyt % 2 == 0, nblk1 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
+	hipLaunchKernelGGL(k, dim3(nblk1 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
 		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->rows, op->mat, ccs, c.qx,
-		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate));
+		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate, pair, op->bZ * nyt));
 }
 
 template <typename T, bool INV>
@@ -800,9 +802,13 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 	cx<T> *out = (cx<T> *)(INV ? op->Xa : op->Xb);
 	auto k = (ZFn<T>)c.kz.k[INV ? MV_ZINV : MV_ZFWD][sizeof(T) == 4];
 	const long xin = INV ? xb_stride(op) : xa_stride(op), xout = INV ? xa_stride(op) : xb_stride(op);
-	hipLaunchKernelGGL(k, dim3((unsigned)((Q + c.TC - 1) / c.TC * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
+	// tiles of less than a 128-byte cache line per row share lines with their neighbour: pairs go to one XCD
+	const long ntile = (Q + c.TC - 1) / c.TC;
+	const int pair = c.TC * (int)sizeof(cx<T>) < 128 && ntile % 2 == 0;
+	const long nblk = pair ? (ntile + 15) / 16 * 16 : ntile;
+	hipLaunchKernelGGL(k, dim3((unsigned)(nblk * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
 		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
-		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, make_bq(op, K, 0, xin, xout, gate));
+		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, make_bq(op, K, 0, xin, xout, gate, pair, (int)ntile));
 }
 
 template <typename T> using P3SFn = decltype(&k_p3sub<T, Dyn<8>>);

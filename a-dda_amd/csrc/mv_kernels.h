@@ -123,8 +123,9 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 	const int *__restrict__ cellmap, const RowDesc *__restrict__, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
-	if (skip_rhs(bq, kq)) return;
+	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq) || (bq.pair && tile >= bq.ntile)) return;  // pair: as in P5 (the Xa runs of tiles 2p, 2p + 1
+	                                                                  // share cache lines)
 	arg += kq * bq.vs;
 	Xa += kq * bq.xs2;
 	ccs += (size_t)kq * bq.cs;
@@ -194,8 +195,9 @@ __global__ void __launch_bounds__(512) k_p1s(const cx<TV> *__restrict__ arg, cx<
 	const int *__restrict__ cellmap, const RowDesc *__restrict__ rows, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
-	if (skip_rhs(bq, kq)) return;
+	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	if (skip_rhs(bq, kq) || (bq.pair && tile >= bq.ntile)) return;  // pair: as in P5 (the Xa runs of tiles 2p, 2p + 1
+	                                                                  // share cache lines)
 	arg += kq * bq.vs;
 	Xa += kq * bq.xs2;
 	ccs += (size_t)kq * bq.cs;
@@ -254,6 +256,10 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 {
 	const int kq = blockIdx.x % bq.K;
 	if (skip_rhs(bq, kq)) return;
+	// pair: tiles narrower than a cache line -- tiles 2p and 2p + 1 share lines -- go 8 blocks apart, on one XCD
+	// (pair_tile; the launch is padded to whole groups of 16 tiles)
+	const int tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	if (bq.pair && tile >= bq.ntile) return;
 	in += kq * bq.xs;
 	out += kq * bq.xs2;
 	extern __shared__ char smem_raw[];
@@ -261,7 +267,7 @@ __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T
 	cx<T> *tws = reinterpret_cast<cx<T> *>(smem_raw), *buf = tws + N;
 	PL::template load_tw<T>(tws, twg, N);
 	__builtin_assume(TC >= 2);  // column tiles (element k of column l at buf[k*TC + l]); the launcher guarantees TC >= 2
-	const long q0 = (long)(blockIdx.x / bq.K) * TC;
+	const long q0 = (long)tile * TC;
 	const int line = threadIdx.x % TC, t = threadIdx.x / TC;
 	const bool act = t < TPL && q0 + line < Q;
 	const cx<T> *ip = in + q0 + line;

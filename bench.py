@@ -53,6 +53,10 @@ def main():
                          "a kernel trace of a batched solve is taken from")
     ap.add_argument("--method", default="bicgstab", help="solver of --solve-only / --batched-solve (ADDA's -iter names)")
     ap.add_argument("--eps", type=float, default=1e-10, help="tolerance of --solve-only (default 1e-10)")
+    ap.add_argument("--inner-method", default=None,
+                    help="Krylov method of the inner solves of --solve-only mixed (default: automatic, BiCGStab for QMR_CS)")
+    ap.add_argument("--inner-eps", type=float, default=0.0,
+                    help="relative tolerance of the FP32 inner solves of the mixed forms (default: the solver's, 1e-4)")
     ap.add_argument("--m", type=float, nargs=2, default=[1.5, 0.01], metavar=("RE", "IM"),
                     help="refractive index of the sphere (default 1.5 0.01; the 640^3 workload uses 1.33 0.001)")
     ap.add_argument("--fp64-steps", type=int, default=3,
@@ -106,7 +110,8 @@ def main():
         op.sync()
         t = time.perf_counter()
         # verbose bit 1: a progress line about every 10 s, so that a long solve (the 640^3 sphere) shows it is alive
-        res = pr.solve(args.method, eps=args.eps, precision=prec, verbose=2, recalc_resid=True)
+        res = pr.solve(args.method, eps=args.eps, precision=prec, verbose=2, recalc_resid=True, inner_eps=args.inner_eps,
+                       inner_method=args.inner_method)
         cext = pr.cext()
         dt = time.perf_counter() - t
         print(json.dumps({"mode": "solve-only", "form": args.solve_only, "method": args.method, "eps": args.eps,

@@ -138,10 +138,12 @@ def device_count():
 
 
 def hbm_rates(device=0, nbytes=1 << 30, nrep=20):
-    """GB/s of a plain device copy (read + write bytes), a read-only sum and a fill of nbytes (HIP events)."""
-    out = np.zeros(3)
+    """GB/s of a plain device copy (read + write bytes), a read-only pass, a fill and a nontemporal copy of nbytes (one
+    16-byte element per thread, HIP events)."""
+    out = np.zeros(4)
     check(hip().adda_hip_hbm_rates(device, nbytes, nrep, _p(out)))
-    return {"bytes": nbytes, "copy_GBps": round(out[0], 1), "read_GBps": round(out[1], 1), "fill_GBps": round(out[2], 1)}
+    return {"bytes": nbytes, "copy_GBps": round(out[0], 1), "read_GBps": round(out[1], 1), "fill_GBps": round(out[2], 1),
+            "copy_nt_GBps": round(out[3], 1)}
 
 
 class Operator:

@@ -132,6 +132,8 @@ struct MvCfg {
 	                      // than the transforms use; the extra threads take part in the product only
 	int p3s = 0;          // P3 lines in unpadded swizzled LDS rows (k_p3 SWZ; LP = 1, compile-time plans, gridY % 16 == 0)
 	int p3k = 1;          // right-hand sides per P3 block in batched launches (they share its D rows and twiddles)
+	int xn = 0;           // nontemporal hint on the Xa stores of P1 (bit 0) and the Xa loads of P5 (bit 1); whether it
+	                      // pays depends on the tile (it loses where neighbouring tiles share cache lines), so it is tuned
 };
 
 // threads of a P3 block (k_p3: 3 lines per (plane, column) group, LP column units; k_p3r: LP columns, 2*LP groups)
@@ -754,6 +756,7 @@ static bool cfg_ok(const adda_hip_op *op, const MvCfg &c, size_t es)
 	if (c.p3s && (c.p3r || c.p3g || c.LP != 1 || !mv_kernels(c.qy).k[MV_P3W1][es == 8])) return false;
 	// several RHS per P3 block: the generic kernel of padded rows with its twiddles in LDS
 	if (c.p3k < 1 || c.p3k > 16 || (c.p3k > 1 && (c.p3r || c.p3g || c.p3s))) return false;
+	if (c.xn < 0 || c.xn > 3 || (c.xn && !mv_kernels(c.qx).nt[MV_P1][es == 8])) return false;  // compile-time plans
 	const int ml = c.p3r ? 2 : 3 * p3_groups(op->reduced, c.p3x);
 	if (c.p3t && (c.p3r || p3_threads(c, op->gY, op->reduced) > 512)) return false;
 	return c.LY >= 1 && c.TC >= 2 && c.LP >= 1 && c.LP <= 8 && (!c.p3r || (op->gX % c.LP == 0 && c.p3x)) &&
@@ -786,7 +789,8 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 {
 	const int nyt = (op->bY + c.LY - 1) / c.LY;
 	using Fn = std::conditional_t<std::is_same_v<T, TV>, P1Fn<T>, P1WFn>;
-	auto k = (Fn)c.kx.k[std::is_same_v<T, TV> ? MV_P1 : MV_P1W][sizeof(T) == 4];
+	const int kind = std::is_same_v<T, TV> ? MV_P1 : MV_P1W, f = sizeof(T) == 4;
+	auto k = (Fn)((c.xn & 1) ? c.kx.nt[kind][f] : c.kx.k[kind][f]);
 	// tiles 2p and 2p + 1 (whose Xa runs share cache lines where they meet) go 8 blocks apart, on one XCD, as in P5
 	const int pair = nyt % 2 == 0, nblk1 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
 	hipLaunchKernelGGL(k, dim3(nblk1 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
@@ -870,8 +874,9 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	const int pair = nyt % 2 == 0, nblk5 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
 	constexpr bool same = std::is_same_v<T, TV>;
 	using Fn = std::conditional_t<same, P5Fn<T>, P5WFn>;
-	auto k = (Fn)c.kx.k[dmode == 2 ? (same ? MV_P5D2 : MV_P5DW2) : dmode ? (same ? MV_P5D : MV_P5DW)
-		: (same ? MV_P5 : MV_P5W)][sizeof(T) == 4];
+	const int kind = dmode == 2 ? (same ? MV_P5D2 : MV_P5DW2) : dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W);
+	const int f = sizeof(T) == 4;
+	auto k = (Fn)((c.xn & 2) ? c.kx.nt[kind][f] : c.kx.k[kind][f]);
 	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
 		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
 		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate, pair, op->bZ * nyt));
@@ -2567,10 +2572,11 @@ static void cfg_str(const MvCfg &c, char *b, size_t n)
 	plan_str(c.qx, x, sizeof x);
 	plan_str(c.qy, y, sizeof y);
 	plan_str(c.qz, z, sizeof z);
-	char kl[16] = "";
+	char kl[16] = "", xn[16] = "";
 	if (c.p3k > 1) snprintf(kl, sizeof kl, " KL=%d", c.p3k);
-	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
-		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "", c.p3s ? " SW=1" : "", kl);
+	if (c.xn) snprintf(xn, sizeof xn, " XN=%d", c.xn);
+	snprintf(b, n, "x=%s y=%s z=%s LY=%d TC=%d LP=%d%s%s%s%s%s%s%s", x, y, z, c.LY, c.TC, c.LP, c.p3r ? " R3=1" : "",
+		c.p3x ? "" : " XP=0", c.p3g ? " TG=1" : "", c.p3t ? " PT=1" : "", c.p3s ? " SW=1" : "", kl, xn);
 }
 
 static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
@@ -2584,13 +2590,14 @@ static bool cfg_parse(const adda_hip_op *op, const char *s, MvCfg &c)
 	c.TC = atoi(tc + 3);
 	c.LP = atoi(lp + 3);
 	const char *r3 = strstr(s, "R3="), *xp = strstr(s, "XP="), *tg = strstr(s, "TG="), *pt = strstr(s, "PT="),
-		*sw = strstr(s, "SW="), *kl = strstr(s, "KL=");
+		*sw = strstr(s, "SW="), *kl = strstr(s, "KL="), *xn = strstr(s, "XN=");
 	c.p3r = r3 ? atoi(r3 + 3) : 0;
 	c.p3x = xp ? atoi(xp + 3) : 1;
 	c.p3g = tg ? atoi(tg + 3) : 0;
 	c.p3t = pt ? atoi(pt + 3) : 0;
 	c.p3s = sw ? atoi(sw + 3) : 0;
 	c.p3k = kl ? atoi(kl + 3) : 1;
+	c.xn = xn ? atoi(xn + 3) : 0;
 	return true;
 }
 
@@ -2750,6 +2757,16 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 			float t = timeit(c, 1u | 16u);
 			if (t < tx) { tx = t; best.qx = q; best.LY = LY; }
 		}
+	// the nontemporal hint on the Xa accesses of P1 and P5, with the x configuration fixed
+	for (int xn = 1; xn < 4; xn++) {
+		MvCfg c = best;
+		c.xn = xn;
+		if (!cfg_ok(op, c, es)) continue;
+		resolve(c);
+		ncand++;
+		float t = timeit(c, 1u | 16u);
+		if (t < tx) { tx = t; best.xn = xn; }
+	}
 	// z passes (P2 + P4): radix schedule of z and TC
 	for (auto &q : pz)
 		for (int TC : {4, 8, 16, 32, 64}) {
@@ -2945,69 +2962,55 @@ extern "C" int adda_hip_sync(adda_hip_op *op)
 }
 
 // ==================================================================================================================
-// measurement: achievable HBM rates of plain streaming kernels (16-byte accesses, grid-stride), the ceilings next to
-// the 8 TB/s peak that the matvec passes are compared with
-
-// Each thread moves U 16-byte elements per round (loads first, then stores), so U accesses per thread are in flight.
-constexpr int kStreamU = 4;
-__global__ void __launch_bounds__(256) k_stream_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n)
+// measurement: achievable HBM rates of plain streaming kernels, the ceilings next to the 8 TB/s peak that the matvec
+// passes are compared with. One 16-byte element per thread and one block per 256 consecutive elements: the blocks
+// resident at any time cover a contiguous window of the buffer. (Measured on MI355X, 1 GiB: a copy in this form
+// reaches 6.2 TB/s, 6.7 TB/s with nontemporal loads and stores; a grid-stride copy with 8-64 blocks per CU and four
+// elements per thread in flight reaches 4.2-4.7 TB/s.)
+typedef double stream_t __attribute__((ext_vector_type(2)));
+template <bool NT>
+__global__ void __launch_bounds__(256) k_stream_copy(stream_t *__restrict__ dst, const stream_t *__restrict__ src, size_t n)
 {
-	const size_t step = (size_t)gridDim.x * blockDim.x;
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step) {
-		double2 v[kStreamU];
-#pragma unroll
-		for (int u = 0; u < kStreamU; u++)
-			if (i + u * step < n) v[u] = src[i + u * step];
-#pragma unroll
-		for (int u = 0; u < kStreamU; u++)
-			if (i + u * step < n) dst[i + u * step] = v[u];
-	}
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	if (NT) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+	else dst[i] = src[i];
 }
-__global__ void __launch_bounds__(256) k_stream_fill(double2 *__restrict__ dst, size_t n, double v)
+__global__ void __launch_bounds__(256) k_stream_fill(stream_t *__restrict__ dst, size_t n, double v)
 {
-	const size_t step = (size_t)gridDim.x * blockDim.x;
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step)
-#pragma unroll
-		for (int u = 0; u < kStreamU; u++)
-			if (i + u * step < n) dst[i + u * step] = make_double2(v, v);
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) dst[i] = stream_t{v, v};
 }
-// read-only: one partial sum per thread, written only when it equals a value the data never holds (keeps the loads)
-__global__ void __launch_bounds__(256) k_stream_read(const double2 *__restrict__ src, size_t n, double *sink)
+// read-only: each element is compared with a value the data never holds, so the loads stay and nothing is written
+__global__ void __launch_bounds__(256) k_stream_read(const stream_t *__restrict__ src, size_t n, double *sink)
 {
-	const size_t step = (size_t)gridDim.x * blockDim.x;
-	double s = 0;
-	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kStreamU * step) {
-		double2 v[kStreamU];
-#pragma unroll
-		for (int u = 0; u < kStreamU; u++) v[u] = i + u * step < n ? src[i + u * step] : make_double2(0, 0);
-#pragma unroll
-		for (int u = 0; u < kStreamU; u++) s += v[u].x + v[u].y;
-	}
-	if (s == -1.0) sink[0] = s;
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const stream_t v = src[i];
+	if (v.x + v.y == -1.0) sink[0] = v.x;
 }
 
 extern "C" int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *gbps)
 {
 	if (nbytes < (1 << 20) || nrep < 1 || !gbps) return ERR("bad argument");
 	HIPCHK(hipSetDevice(device));
-	const size_t n = nbytes / sizeof(double2);
-	double2 *a = nullptr, *b = nullptr;
+	const size_t n = nbytes / sizeof(stream_t);
+	stream_t *a = nullptr, *b = nullptr;
 	double *sink = nullptr;
 	hipStream_t s = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	int rc = 0;
-	if (hipMalloc(&a, n * sizeof(double2)) != hipSuccess || hipMalloc(&b, n * sizeof(double2)) != hipSuccess ||
+	if (hipMalloc(&a, n * sizeof(stream_t)) != hipSuccess || hipMalloc(&b, n * sizeof(stream_t)) != hipSuccess ||
 		hipMalloc(&sink, sizeof(double)) != hipSuccess || hipStreamCreate(&s) != hipSuccess ||
 		hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
 		rc = ERR("allocation of the probe buffers failed");
-	int ncu = 256;
-	(void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-	const int nb = ncu * 8;  // 8 blocks of 4 waves per CU, grid-stride over the buffer
-	for (int kind = 0; kind < 3 && !rc; kind++) {
+	const unsigned nb = (unsigned)((n + 255) / 256);  // one element per thread
+	for (int kind = 0; kind < 4 && !rc; kind++) {
 		auto launch = [&]() {
-			if (kind == 0) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, s, b, a, n);
+			if (kind == 0) hipLaunchKernelGGL(k_stream_copy<false>, dim3(nb), dim3(256), 0, s, b, a, n);
 			else if (kind == 1) hipLaunchKernelGGL(k_stream_read, dim3(nb), dim3(256), 0, s, a, n, sink);
-			else hipLaunchKernelGGL(k_stream_fill, dim3(nb), dim3(256), 0, s, b, n, 1.0);
+			else if (kind == 2) hipLaunchKernelGGL(k_stream_fill, dim3(nb), dim3(256), 0, s, b, n, 1.0);
+			else hipLaunchKernelGGL(k_stream_copy<true>, dim3(nb), dim3(256), 0, s, b, a, n);
 		};
 		if (kind == 0) hipLaunchKernelGGL(k_stream_fill, dim3(nb), dim3(256), 0, s, a, n, 0.5);
 		launch();
@@ -3016,7 +3019,7 @@ extern "C" int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *g
 		for (int r = 0; r < nrep; r++) launch();
 		if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
 			hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = ERR("probe kernel failed"); break; }
-		const double moved = (kind == 0 ? 2.0 : 1.0) * n * sizeof(double2) * nrep;
+		const double moved = (kind == 0 || kind == 3 ? 2.0 : 1.0) * n * sizeof(stream_t) * nrep;
 		gbps[kind] = moved / (ms * 1e-3) / 1e9;
 	}
 	if (e0) (void)hipEventDestroy(e0);

@@ -21,6 +21,29 @@ template <typename T> __device__ __forceinline__ cx<T> mul(cx<T> a, cx<T> b)
 	return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T> __device__ __forceinline__ cx<T> conjc(cx<T> a) { return mk<T>(a.x, -a.y); }
+// Loads and stores with the nontemporal hint (the line is not kept in the caches for reuse).
+template <typename T> __device__ __forceinline__ cx<T> ld_nt(const cx<T> *p)
+{
+	typedef T v2 __attribute__((ext_vector_type(2)));
+	const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+	return mk<T>(v.x, v.y);
+}
+template <typename T> __device__ __forceinline__ void st_nt(cx<T> *p, cx<T> a)
+{
+	typedef T v2 __attribute__((ext_vector_type(2)));
+	__builtin_nontemporal_store(v2{a.x, a.y}, reinterpret_cast<v2 *>(p));
+}
+// The element at a 32-bit byte offset from a base pointer. With a base that is the same for the whole block the
+// access takes the scalar-base form of the global load/store: one 32-bit add per access instead of a 64-bit product
+// and add (an index scaled in 64 bits costs multiply-adds that issue at a quarter of the VALU rate).
+template <typename P> __device__ __forceinline__ P &at_b(P *base, unsigned byte_off)
+{
+	return *reinterpret_cast<P *>(reinterpret_cast<char *>(base) + byte_off);
+}
+template <typename P> __device__ __forceinline__ const P &at_b(const P *base, unsigned byte_off)
+{
+	return *reinterpret_cast<const P *>(reinterpret_cast<const char *>(base) + byte_off);
+}
 // precision conversion (identity when the types agree)
 template <typename T, typename U> __device__ __forceinline__ cx<T> cvt(cx<U> a) { return mk<T>((T)a.x, (T)a.y); }
 template <typename T> __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }

@@ -190,7 +190,9 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 // cells t + 16q (radix 16): lanes of the three component lines of a row read the 48 (24) contiguous bytes of each
 // dipole together. Dense rows (RowDesc) give the dipole index arithmetically, so the argument loads depend on one
 // descriptor load per thread and never on a cell-map read. Same arithmetic as k_p1, so the results are the same.
-template <typename T, class PL, typename TV = T>
+// NT: the Xa stores with the nontemporal hint (a kernel of its own: the same instructions otherwise, so the same
+// bits; the launcher picks it for configurations with XN bit 0).
+template <typename T, class PL, typename TV = T, bool NT = false>
 __global__ void __launch_bounds__(512) k_p1s(const cx<TV> *__restrict__ arg, cx<T> *__restrict__ Xa,
 	const int *__restrict__ cellmap, const RowDesc *__restrict__ rows, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
@@ -225,26 +227,41 @@ __global__ void __launch_bounds__(512) k_p1s(const cx<TV> *__restrict__ arg, cx<
 	// would wait on each load in turn). The loads of empty cells read dipole 0 and are discarded: a load under a
 	// condition gets a basic block of its own, and the compiler waits for it before the next.
 	constexpr int NSF = PL::NSLOTF;
+	// Indices are 32-bit (a dipole vector holds fewer than 2^32 elements). ai: element index 3d + c of the argument
+	// (dipole 0 for an empty cell); on a dense row it is a0 + 3x, linear in the cell, with no product per element.
 	int dd[NSF], mm[NSF];
+	unsigned ai[NSF];
 	cx<TV> av[NSF];
 	cx<T> gv[NSF];
-	if (rd.dense) PL::template for_first<true>(tA, [&](int sl, int x) {
-		const unsigned o = (unsigned)(x - rd.x0);
-		dd[sl] = actA && x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
-	});
-	else PL::template for_first<true>(tA, [&](int sl, int x) { dd[sl] = actA && x < bX ? cm[x] : -1; });
-	if (!onemat) PL::template for_first<true>(tA, [&](int sl, int) { mm[sl] = 3 * mat[dd[sl] > 0 ? dd[sl] : 0]; });
+	if (rd.dense) {
+		const unsigned a0 = 3u * (unsigned)(rd.s - rd.x0) + (unsigned)c;  // modulo 2^32, exact where used
+		PL::template for_first<true>(tA, [&](int sl, int x) {
+			const unsigned o = (unsigned)(x - rd.x0);
+			const bool in = actA && x < bX && o < (unsigned)rd.n;
+			dd[sl] = in ? rd.s + (int)o : -1;
+			ai[sl] = in ? a0 + 3u * (unsigned)x : (unsigned)c;
+		});
+	}
+	else {
+		PL::template for_first<true>(tA, [&](int sl, int x) { dd[sl] = actA && x < bX ? cm[x] : -1; });
+		PL::template for_first<true>(tA, [&](int sl, int) { ai[sl] = 3u * (unsigned)max(dd[sl], 0) + c; });
+	}
+	if (!onemat) PL::template for_first<true>(tA, [&](int sl, int) { mm[sl] = 3 * mat[(unsigned)max(dd[sl], 0)]; });
 	else PL::template for_first<true>(tA, [&](int sl, int) { mm[sl] = 0; });
-	PL::template for_first<true>(tA, [&](int sl, int) { gv[sl] = ccs[mm[sl] + c]; });
-	PL::template for_first<true>(tA, [&](int sl, int) { av[sl] = arg[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c]; });
+	PL::template for_first<true>(tA, [&](int sl, int) { gv[sl] = ccs[(unsigned)(mm[sl] + c)]; });
+	PL::template for_first<true>(tA, [&](int sl, int) { av[sl] = arg[ai[sl]]; });
 	PL::template for_first<true>(tA, [&](int sl, int) {
 		const cx<T> a = cvt<T>(her ? conjc(av[sl]) : av[sl]);
 		gv[sl] = dd[sl] >= 0 ? mul(gv[sl], a) : mk<T>(0, 0);
 	});
 	auto ld = [&](int, int, int sl) -> cx<T> { return gv[sl]; };
-	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineB;
-	const size_t xs = (size_t)3 * bY;
-	auto st = [&](int, int x, cx<T> v) { xo[x * xs] = v; };
+	// the tile's base is the block's; a thread's elements lie within one z-plane of Xa (byte offsets far below 2^32)
+	cx<T> *xo = Xa + (size_t)z * N * 3 * bY + 3 * y0;
+	const unsigned xsb = 3u * bY * (unsigned)sizeof(cx<T>), lb = (unsigned)lineB * (unsigned)sizeof(cx<T>);
+	auto st = [&](int, int x, cx<T> v) {
+		if constexpr (NT) st_nt(&at_b(xo, lb + (unsigned)x * xsb), v);
+		else at_b(xo, lb + (unsigned)x * xsb) = v;
+	};
 	PL::template fft<T, false, true, false, true, true, LAY>(buf, tws, p, 1, LS, lineA, tA, actA, lineB, tB, actB, ld,
 		st);
 }
@@ -857,7 +874,8 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 // -- one round trip per thread for dense rows of one material -- and nothing is held in registers through the
 // transform. (Measured: loading the arguments before the transform, beside its own loads, took FP32 P5 from 44 to
 // 47 us -- 99 registers, or a spill under a budget of five waves per SIMD.)
-template <typename T, class PL, int DM, typename TV = T>
+// NT: the Xa loads with the nontemporal hint (a kernel of its own, as in P1; XN bit 1).
+template <typename T, class PL, int DM, typename TV = T, bool NT = false>
 __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const cx<TV> *__restrict__ arg,
 	cx<TV> *__restrict__ res, const int *__restrict__ cellmap, const RowDesc *__restrict__ rows,
 	const int *__restrict__ mat, const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX,
@@ -902,22 +920,35 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 	constexpr int NS = PL::NSLOT;
 	// phases whose loads are in flight together (loaded one output at a time, each would wait on the last); empty cells
 	// load dipole 0 and discard it (a conditional load would get a basic block and a wait of its own)
+	// Indices are 32-bit (a dipole vector holds fewer than 2^32 elements). ai: element index 3d + c of the argument and
+	// result (dipole 0 for an empty cell); on a dense row it is a0 + 3x, linear in the cell, with no product per element.
 	int dd[NS];
+	unsigned ai[NS];
 	cx<TV> av[NS];
 	auto find_dipoles = [&] {
-		if (rd.dense) PL::template for_last<true>(tB, [&](int sl, int x) {
-			const unsigned o = (unsigned)(x - rd.x0);
-			dd[sl] = x < bX && o < (unsigned)rd.n ? rd.s + (int)o : -1;
-		});
-		else PL::template for_last<true>(tB, [&](int sl, int x) { dd[sl] = x < bX ? cm[x] : -1; });
+		if (rd.dense) {
+			const unsigned a0 = 3u * (unsigned)(rd.s - rd.x0) + (unsigned)c;  // modulo 2^32, exact where used
+			PL::template for_last<true>(tB, [&](int sl, int x) {
+				const unsigned o = (unsigned)(x - rd.x0);
+				const bool in = x < bX && o < (unsigned)rd.n;
+				dd[sl] = in ? rd.s + (int)o : -1;
+				ai[sl] = in ? a0 + 3u * (unsigned)x : (unsigned)c;
+			});
+		}
+		else {
+			PL::template for_last<true>(tB, [&](int sl, int x) { dd[sl] = x < bX ? cm[x] : -1; });
+			PL::template for_last<true>(tB, [&](int sl, int) { ai[sl] = 3u * (unsigned)max(dd[sl], 0) + c; });
+		}
 	};
-	auto load_args = [&] {
-		PL::template for_last<true>(tB, [&](int sl, int) { av[sl] = arg[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c]; });
-	};
+	auto load_args = [&] { PL::template for_last<true>(tB, [&](int sl, int) { av[sl] = arg[ai[sl]]; }); };
 	double s = 0, dr = 0, di = 0;
-	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0 + lineA;
-	const size_t xs = (size_t)3 * bY;
-	auto ld = [&](int, int x) -> cx<T> { return xi[x * xs]; };
+	// the tile's base is the block's; a thread's elements lie within one z-plane of Xa (byte offsets far below 2^32)
+	const cx<T> *xi = Xa + (size_t)z * N * 3 * bY + 3 * y0;
+	const unsigned xsb = 3u * bY * (unsigned)sizeof(cx<T>), la = (unsigned)lineA * (unsigned)sizeof(cx<T>);
+	auto ld = [&](int, int x) -> cx<T> {
+		if constexpr (NT) return ld_nt(&at_b(xi, la + (unsigned)x * xsb));
+		else return at_b(xi, la + (unsigned)x * xsb);
+	};
 	cx<T> ov[NS];
 	auto st = [&](int, int, cx<T> v, int sl) { ov[sl] = v; };
 	// only x < boxX <= N/2 is gathered: the inverse transform computes the lower half of each line
@@ -928,13 +959,13 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 		cx<TV> wv[NS];
 		cx<T> cv[NS];
 		find_dipoles();
-		if (!onemat) PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 3 * mat[dd[sl] > 0 ? dd[sl] : 0]; });
+		if (!onemat) PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 3 * mat[(unsigned)max(dd[sl], 0)]; });
 		else PL::template for_last<true>(tB, [&](int sl, int) { mm[sl] = 0; });
 		load_args();
 		PL::template for_last<true>(tB, [&](int sl, int) {
-			cv[sl] = ccs[mm[sl] + c];
+			cv[sl] = ccs[(unsigned)(mm[sl] + c)];
 			// the weights: another vector (DM 1), or the argument itself (DM 2), already in registers
-			if constexpr (DM == 1) wv[sl] = dotw[3 * (size_t)(dd[sl] > 0 ? dd[sl] : 0) + c];
+			if constexpr (DM == 1) wv[sl] = dotw[ai[sl]];
 			else if constexpr (DM == 2) wv[sl] = av[sl];
 		});
 		PL::template for_last<true>(tB, [&](int sl, int) {
@@ -943,7 +974,7 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 			const cx<TV> a = av[sl];
 			cx<TV> rv = add(her ? conjc(a) : a, cvt<TV>(mul(cv[sl], ov[sl])));
 			if (her) rv = conjc(rv);
-			res[3 * (size_t)d + c] = rv;
+			res[ai[sl]] = rv;
 			s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
 			if (DOT) {
 				const cx<TV> w = wv[sl];
@@ -1005,6 +1036,8 @@ enum {
 };
 struct MvKernels {
 	const void *k[MV_KINDS][2];
+	// the P1 / P5 kinds with the nontemporal hint on Xa (compile-time plans; null otherwise): nt[kind][fp32]
+	const void *nt[MV_KINDS][2] = {};
 	bool swz = false;  // the plan's P1/P5 kernels use the swizzled LDS rows (line stride N + 1)
 };
 
@@ -1017,6 +1050,10 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 		m.k[MV_P5][f] = (const void *)&k_p5s<T, PL, 0>;
 		m.k[MV_P5D][f] = (const void *)&k_p5s<T, PL, 1>;
 		m.k[MV_P5D2][f] = (const void *)&k_p5s<T, PL, 2>;
+		m.nt[MV_P1][f] = (const void *)&k_p1s<T, PL, T, true>;
+		m.nt[MV_P5][f] = (const void *)&k_p5s<T, PL, 0, T, true>;
+		m.nt[MV_P5D][f] = (const void *)&k_p5s<T, PL, 1, T, true>;
+		m.nt[MV_P5D2][f] = (const void *)&k_p5s<T, PL, 2, T, true>;
 	}
 	else {
 		m.k[MV_P1][f] = (const void *)&k_p1<T, PL>;
@@ -1045,6 +1082,10 @@ template <typename T, class PL> void mv_fill_t(MvKernels &m)
 			m.k[MV_P5W][f] = (const void *)&k_p5s<T, PL, 0, double>;
 			m.k[MV_P5DW][f] = (const void *)&k_p5s<T, PL, 1, double>;
 			m.k[MV_P5DW2][f] = (const void *)&k_p5s<T, PL, 2, double>;
+			m.nt[MV_P1W][f] = (const void *)&k_p1s<T, PL, double, true>;
+			m.nt[MV_P5W][f] = (const void *)&k_p5s<T, PL, 0, double, true>;
+			m.nt[MV_P5DW][f] = (const void *)&k_p5s<T, PL, 1, double, true>;
+			m.nt[MV_P5DW2][f] = (const void *)&k_p5s<T, PL, 2, double, true>;
 		}
 		else {
 			m.k[MV_P1W][f] = (const void *)&k_p1<T, PL, double>;
@@ -1071,9 +1112,15 @@ template <typename T, class PL> hipError_t mv_lds_t()
 	if constexpr (is_fix<PL>::value) {
 		if (lim((const void *)k_p1s<T, PL>) || lim((const void *)k_p5s<T, PL, 0>) ||
 			lim((const void *)k_p5s<T, PL, 1>) || lim((const void *)k_p5s<T, PL, 2>)) return e;
-		if constexpr (sizeof(T) == 4)
+		if (lim((const void *)k_p1s<T, PL, T, true>) || lim((const void *)k_p5s<T, PL, 0, T, true>) ||
+			lim((const void *)k_p5s<T, PL, 1, T, true>) || lim((const void *)k_p5s<T, PL, 2, T, true>)) return e;
+		if constexpr (sizeof(T) == 4) {
 			if (lim((const void *)k_p1s<T, PL, double>) || lim((const void *)k_p5s<T, PL, 0, double>) ||
 				lim((const void *)k_p5s<T, PL, 1, double>) || lim((const void *)k_p5s<T, PL, 2, double>)) return e;
+			if (lim((const void *)k_p1s<T, PL, double, true>) || lim((const void *)k_p5s<T, PL, 0, double, true>) ||
+				lim((const void *)k_p5s<T, PL, 1, double, true>) || lim((const void *)k_p5s<T, PL, 2, double, true>))
+				return e;
+		}
 		if (lim((const void *)k_p3g<T, PL>) || lim((const void *)k_p3g<T, PL, 1>)) return e;
 		if constexpr (PL::SWZ)
 			if (lim((const void *)k_p3<T, PL, 1, true>)) return e;

@@ -4,6 +4,7 @@
     python a-dda_amd/rocprof_summary.py pmc DIR COUNTER  # from `rocprofv3 --pmc COUNTER --output-format csv`
     python a-dda_amd/rocprof_summary.py traffic FETCHDIR WRITEDIR BENCH_JSON COPY_BYTES  # HBM bytes per pass kernel
     python a-dda_amd/rocprof_summary.py sq SQDIR TRACEDIR  # per-wave instruction mix and VALU issue time
+    python a-dda_amd/rocprof_summary.py groups DIR NITER   # a solve's kernel time by group, per iteration
 
 `stats` prints, per kernel (template arguments kept, parameter list dropped): calls, average / min / max duration
 in microseconds and total milliseconds, from the *_kernel_stats.csv (or *_kernel_trace.csv) found under DIR.
@@ -147,6 +148,44 @@ def traffic(fetch_dir, write_dir, bench_json, copy_bytes):
         print(f"P{i + 1:<4}{k[:51]:<52}{f:>12.1f}{w:>12.1f}{hbm:>13.4e}{alg:>13.4e}{hbm / alg:>7.2f}")
 
 
+_VECTOR = ("k_dk_lc", "k_dk_xr", "k_dk_qmr_upd", "k_dot", "k_lincomb", "k_convert")
+_BUILD = ("k_green", "k_rowfft", "k_colfft", "k_extract_d", "k_green_refl", "k_extract_r")
+
+
+def _group_of(kernel):
+    base = kernel.split("<")[0].split("::")[-1]
+    pt = _pass_of(kernel)
+    if pt is not None:
+        return f"P{pt[0] + 1} {pt[1]}"
+    if base in _VECTOR:
+        return "vector kernels " + ("double" if "<double" in kernel else "float")
+    if base.startswith("k_dk_") or base == "k_reduce_final":
+        return "scalar kernels"
+    if base in _BUILD:
+        return "D-matrix build"
+    return "other"
+
+
+def groups(d, niter):
+    """Kernel time of a solve grouped into matvec passes (per precision), Krylov vector kernels, the one-block scalar
+    kernels and the rest, in total and per iteration (niter iterations)."""
+    agg = {}
+    for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            g = _group_of(kname(_col(r, "Kernel_Name")))
+            t = (int(_col(r, "End_Timestamp")) - int(_col(r, "Start_Timestamp"))) / 1e3
+            a = agg.setdefault(g, [0, 0.0])
+            a[0] += 1
+            a[1] += t
+    print(f"{'group':<24}{'calls':>8}{'total_ms':>11}{'us/iter':>10}")
+    tot = 0.0
+    for g in sorted(agg):
+        n, t = agg[g]
+        tot += t if g != "D-matrix build" else 0.0
+        print(f"{g:<24}{n:>8}{t / 1e3:>11.3f}{t / niter:>10.1f}")
+    print(f"{'all but D-matrix build':<24}{'':>8}{tot / 1e3:>11.3f}{tot / niter:>10.1f}")
+
+
 SQ = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR")
 
 
@@ -177,6 +216,8 @@ if __name__ == "__main__":
         pmc(sys.argv[2], sys.argv[3])
     elif len(sys.argv) >= 6 and sys.argv[1] == "traffic":
         traffic(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]))
+    elif len(sys.argv) >= 4 and sys.argv[1] == "groups":
+        groups(sys.argv[2], int(sys.argv[3]))
     elif len(sys.argv) >= 4 and sys.argv[1] == "sq":
         sq(sys.argv[2], sys.argv[3])
     else:

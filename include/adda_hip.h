@@ -273,9 +273,10 @@ int adda_hip_vec_download_part(adda_hip_op *op, int prec, adda_cplx *dst, const 
 /* Waits for all work queued on the operator's stream. */
 int adda_hip_sync(adda_hip_op *op);
 
-/* Measurement, no counterpart in the reference: achievable HBM rates on `device` of a plain copy, a read-only sum and a
- * fill of nbytes (16-byte accesses, grid-stride), timed with HIP events over nrep launches each; gbps[0..2] = copy
- * (read + write bytes), read, fill in GB/s. The ceilings the matvec passes are compared with beside the 8 TB/s peak. */
+/* Measurement, no counterpart in the reference: achievable HBM rates on `device` of a plain copy, a read-only pass, a
+ * fill and a copy with nontemporal loads and stores, over nbytes (one 16-byte element per thread), timed with HIP
+ * events over nrep launches each; gbps[0..3] = copy (read + write bytes), read, fill, nontemporal copy in GB/s. The
+ * ceilings the matvec passes are compared with beside the 8 TB/s peak. */
 int adda_hip_hbm_rates(int device, size_t nbytes, int nrep, double *gbps);
 
 #ifdef __cplusplus

@@ -2745,28 +2745,21 @@ static int tune_t(adda_hip_op *op, int K, int reps, char *report, size_t rlen)
 	enum_plans(op->gZ, pz);
 	float tx = 1e30f, ty = 1e30f, tz = 1e30f;
 	int ncand = 0;
-	// x passes (P1 + P5): radix schedule of x and LY
+	// x passes (P1 + P5): radix schedule of x, LY and the nontemporal hint on their Xa accesses (whether it pays
+	// depends on the tile, so it is searched together with the tile rather than after it)
 	for (auto &q : px)
-		for (int LY : {2, 4, 8, 16, 32}) {
+		for (int LY : {2, 4, 8, 16, 32})
+		for (int xn = 0; xn < 4; xn++) {
 			MvCfg c = best;
 			c.qx = q;
 			c.LY = LY;
+			c.xn = xn;
 			if (!cfg_ok(op, c, es)) continue;
 			resolve(c);
 			ncand++;
 			float t = timeit(c, 1u | 16u);
-			if (t < tx) { tx = t; best.qx = q; best.LY = LY; }
+			if (t < tx) { tx = t; best.qx = q; best.LY = LY; best.xn = xn; }
 		}
-	// the nontemporal hint on the Xa accesses of P1 and P5, with the x configuration fixed
-	for (int xn = 1; xn < 4; xn++) {
-		MvCfg c = best;
-		c.xn = xn;
-		if (!cfg_ok(op, c, es)) continue;
-		resolve(c);
-		ncand++;
-		float t = timeit(c, 1u | 16u);
-		if (t < tx) { tx = t; best.xn = xn; }
-	}
 	// z passes (P2 + P4): radix schedule of z and TC
 	for (auto &q : pz)
 		for (int TC : {4, 8, 16, 32, 64}) {

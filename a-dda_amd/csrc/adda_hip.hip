@@ -68,12 +68,15 @@ static constexpr int kDirBlocks = 32;    // dipole chunks per direction
 
 static bool make_plan(int n, FftPlan &p)
 {
-	// radices 5 and 3 first, then the power of two as radix-8 stages with at most two radix-4 stages and no lone
-	// radix-2 stage unless the power is 2 itself (so a stage never has fewer than 4 elements per butterfly where
-	// avoidable)
+	// the odd primes first (13, 11, 7 of ADDA's FFTW rule, 5, 3), then the power of two as radix-8 stages with at most
+	// two radix-4 stages and no lone radix-2 stage unless the power is 2 itself (so a stage never has fewer than 4
+	// elements per butterfly where avoidable)
 	p.n = n;
 	p.nst = 0;
 	int m = n, k2 = 0;
+	while (m % 13 == 0) { p.rad[p.nst++] = 13; m /= 13; if (p.nst >= kMaxStages) return false; }
+	while (m % 11 == 0) { p.rad[p.nst++] = 11; m /= 11; if (p.nst >= kMaxStages) return false; }
+	while (m % 7 == 0) { p.rad[p.nst++] = 7; m /= 7; if (p.nst >= kMaxStages) return false; }
 	while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; if (p.nst >= kMaxStages) return false; }
 	while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; if (p.nst >= kMaxStages) return false; }
 	while (m % 2 == 0) { k2++; m /= 2; }
@@ -1619,14 +1622,27 @@ static int op_build(adda_hip_op *op, const unsigned short *position, const int *
 	HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
 	if (set_lds_limits() || scratch_setup(op->dev)) return -1;
 	const int bX = op->bX, bY = op->bY, bZ = op->bZ, gX = op->gX, gY = op->gY, gZ = op->gZ;
-	if (!make_plan(gX, op->px) || !make_plan(gY, op->py) || !make_plan(gZ, op->pz))
-		return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
+	// grids of ADDA's fftFit rules (src/fft.c:528-562): 2/3/5-smooth (Temperton FFT, clAmdFft), or with factors of 7
+	// and one 11 or 13 per dimension (FFTW3)
+	auto fftw_rule = [](int n) {
+		while (n % 2 == 0) n /= 2;
+		while (n % 3 == 0) n /= 3;
+		while (n % 5 == 0) n /= 5;
+		while (n % 7 == 0) n /= 7;
+		if (n % 11 == 0) n /= 11;
+		else if (n % 13 == 0) n /= 13;
+		return n == 1;
+	};
+	if (!fftw_rule(gX) || !fftw_rule(gY) || !fftw_rule(gZ) || !make_plan(gX, op->px) || !make_plan(gY, op->py) ||
+		!make_plan(gZ, op->pz))
+		return ERR("FFT grid %dx%dx%d is not of ADDA's fftFit rules (prime factors 2, 3, 5, 7 and one 11 or 13 per "
+			"dimension)", gX, gY, gZ);
 	if (gX < 2 * bX || gY < 2 * bY || gZ < 2 * bZ || gY % 2 || gZ % 2)
 		return ERR("FFT grid %dx%dx%d does not embed box %dx%dx%d (need even grid >= 2*box)", gX, gY, gZ, bX, bY, bZ);
 	{
 		FftPlan2 t;
 		if (!make_plan2(gX, t) || !make_plan2(gY, t) || !make_plan2(gZ, t))
-			return ERR("FFT grid %dx%dx%d has prime factors other than 2, 3, 5", gX, gY, gZ);
+			return ERR("FFT grid %dx%dx%d needs more than %d radix stages", gX, gY, gZ, kMaxStages);
 	}
 	default_cfg(op, ADDA_F64, 16);
 	default_cfg(op, ADDA_F32, 8);
@@ -2556,7 +2572,8 @@ static bool plan_parse(const char *s, int n, FftPlan2 &p)
 	while (*s && *s != ' ' && p.nst < kMaxStages) {
 		char *e;
 		long r = strtol(s, &e, 10);
-		if (e == s || !(r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16)) return false;
+		if (e == s || !(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13 || r == 16))
+			return false;
 		p.rad[p.nst++] = (int)r;
 		if (r > p.rmax) p.rmax = (int)r;
 		prod *= r;
@@ -2629,11 +2646,14 @@ extern "C" int adda_hip_set_config(adda_hip_op *op, int prec, const char *cfg)
 }
 
 // Candidate radix schedules for length n: the power of two split into radix-16/8/4/2 stages (at most 4 stages,
-// non-increasing and non-decreasing orders), the radix-5 and radix-3 stages after them, before them, or after the first
-// of them (an even first and last radix lets the transforms skip the zero-padded half, see fft_core.h).
+// non-increasing and non-decreasing orders), the odd-prime stages (13, 11, 7, 5, 3) after them, before them, or after
+// the first of them (an even first and last radix lets the transforms skip the zero-padded half, see fft_core.h).
 static void enum_plans(int n, std::vector<FftPlan2> &out)
 {
-	int m = n, k = 0, r3 = 0, r5 = 0;
+	int m = n, k = 0, r3 = 0, r5 = 0, r7 = 0, r11 = 0, r13 = 0;
+	while (m % 13 == 0) { r13++; m /= 13; }
+	while (m % 11 == 0) { r11++; m /= 11; }
+	while (m % 7 == 0) { r7++; m /= 7; }
 	while (m % 5 == 0) { r5++; m /= 5; }
 	while (m % 3 == 0) { r3++; m /= 3; }
 	while (m % 2 == 0) { k++; m /= 2; }
@@ -2662,9 +2682,12 @@ static void enum_plans(int n, std::vector<FftPlan2> &out)
 			}
 			for (int e : v2) rad.push_back(1 << e);
 			for (int place = 0; place < 3; place++) {
-				if (place && r3 + r5 == 0) continue;
+				if (place && r3 + r5 + r7 + r11 + r13 == 0) continue;
 				if (place == 2 && rad.size() < 2) continue;
 				std::vector<int> full, odd;
+				for (int i = 0; i < r13; i++) odd.push_back(13);
+				for (int i = 0; i < r11; i++) odd.push_back(11);
+				for (int i = 0; i < r7; i++) odd.push_back(7);
 				for (int i = 0; i < r5; i++) odd.push_back(5);
 				for (int i = 0; i < r3; i++) odd.push_back(3);
 				if (place == 1) full = odd;

@@ -250,6 +250,17 @@ __device__ __forceinline__ void stage_dispatch(int R, cx<T> *buf, const cx<T> *t
 		case 4: stage2<T, INV, 4, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
 		case 2: stage2<T, INV, 2, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
 		case 3: stage2<T, INV, 3, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
+		// the radices of FFTW-sized grids are compiled into the RM = 16 instances only (mv_find_dyn sends every plan
+		// with one of them there; radices above 8 need them anyway: a line then has N/RMAX >= N/16 threads)
+		case 7:
+			if constexpr (RM >= 16) stage2<T, INV, 7, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st);
+			break;
+		case 11:
+			if constexpr (RM >= 16) stage2<T, INV, 11, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st);
+			break;
+		case 13:
+			if constexpr (RM >= 16) stage2<T, INV, 13, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st);
+			break;
 		default: stage2<T, INV, 5, FIRST, LAST, RM>(buf, tws, N, Ns, KS, LS, line, t, TPL, active, ld, st); break;
 	}
 }
@@ -590,13 +601,16 @@ template <int... Rs> struct Fix {
 	}
 };
 
-// Host-side plan: radices 5 and 3, then the power of two as radix-16 stages with one radix-8/4/2 remainder
-// (2^(4a+1) uses 8*4 instead of 16*2). Stages run largest radix first.
+// Host-side plan: the power of two as radix-16 stages with one radix-8/4/2 remainder (2^(4a+1) uses 8*4 instead of
+// 16*2), then the odd primes 13, 11, 7, 5, 3 (7, 11 and 13 only in grids of ADDA's FFTW rule).
 inline bool make_plan2(int n, FftPlan2 &p)
 {
 	p.n = n;
 	p.nst = 0;
-	int m = n, k2 = 0, r5 = 0, r3 = 0;
+	int m = n, k2 = 0, r5 = 0, r3 = 0, r7 = 0, r11 = 0, r13 = 0;
+	while (m % 13 == 0) { r13++; m /= 13; }
+	while (m % 11 == 0) { r11++; m /= 11; }
+	while (m % 7 == 0) { r7++; m /= 7; }
 	while (m % 5 == 0) { r5++; m /= 5; }
 	while (m % 3 == 0) { r3++; m /= 3; }
 	while (m % 2 == 0) { k2++; m /= 2; }
@@ -610,9 +624,12 @@ inline bool make_plan2(int n, FftPlan2 &p)
 	}
 	else if (r == 2) extra[ne++] = 4;
 	else if (r == 3) extra[ne++] = 8;
-	if (a + ne + r3 + r5 > kMaxStages) return false;
+	if (a + ne + r3 + r5 + r7 + r11 + r13 > kMaxStages) return false;
 	for (int i = 0; i < a; i++) rad[ns++] = 16;
 	for (int i = 0; i < ne; i++) rad[ns++] = extra[i];
+	for (int i = 0; i < r13; i++) rad[ns++] = 13;
+	for (int i = 0; i < r11; i++) rad[ns++] = 11;
+	for (int i = 0; i < r7; i++) rad[ns++] = 7;
 	for (int i = 0; i < r5; i++) rad[ns++] = 5;
 	for (int i = 0; i < r3; i++) rad[ns++] = 3;
 	if (ns == 0) return false;  // n == 1 is not a valid grid

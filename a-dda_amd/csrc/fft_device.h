@@ -53,7 +53,8 @@ template <typename T, bool INV> __device__ __forceinline__ cx<T> mulmi(cx<T> a)
 	return INV ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
 }
 
-// Maximum number of Stockham stages of a plan and radices supported by the kernels.
+// Maximum number of Stockham stages of a plan and radices supported by the kernels (2, 3, 4, 5, 8 and 16; 7, 11 and 13
+// for the grids of ADDA's FFTW rule).
 constexpr int kMaxStages = 12;
 
 struct FftPlan {
@@ -131,12 +132,71 @@ template <typename T, bool INV> __device__ __forceinline__ void dft5(cx<T> *v)
 	v[3] = sub(a2, b2);
 }
 
+// cos(2 pi m/R) and sin(2 pi m/R), m = 1..(R-1)/2, for the odd prime radices of FFTW-sized grids (40 digits)
+template <int R> struct PrimeTw;
+template <> struct PrimeTw<7> {
+	static constexpr double C[3] = {0.6234898018587335305250048840042398106, -0.2225209339563144042889025644967947594,
+		-0.9009688679024191262361023195074450511};
+	static constexpr double S[3] = {0.7818314824680298087084445266740577502, 0.9749279121818236070181316829939312172,
+		0.4338837391175581204757683328483587546};
+};
+template <> struct PrimeTw<11> {
+	static constexpr double C[5] = {0.8412535328311811688618116489193677175, 0.4154150130018864255292741492296232035,
+		-0.1423148382732851404437926686163696687, -0.6548607339452850640569250724662935531,
+		-0.9594929736144973898903680570663276990};
+	static constexpr double S[5] = {0.5406408174555975821076359543186916954, 0.9096319953545183714117153830790284601,
+		0.9898214418809327323760920377767187874, 0.7557495743542582837740358439723444202,
+		0.2817325568414296977114179153466168990};
+};
+template <> struct PrimeTw<13> {
+	static constexpr double C[6] = {0.8854560256532098959003755220150988786, 0.5680647467311558025118075591275166245,
+		0.1205366802553230533490676874525435823, -0.3546048870425356259696378926000184743,
+		-0.7485107481711010986346305997013513838, -0.9709418174260520271569822762937892272};
+	static constexpr double S[6] = {0.4647231720437685456560153351331047776, 0.8229838658936563945796174234393819907,
+		0.9927088740980539928007516494925201793, 0.9350162426854148234397845998378307291,
+		0.6631226582407952023767854926667662795, 0.2393156642875577671487537262602118952};
+};
+
+// DFT of odd prime size R (7, 11, 13) on the symmetric pairs s_j = v_j + v_{R-j}, d_j = v_j - v_{R-j}:
+// X_k = v_0 + sum_j s_j cos(2 pi jk/R) -/+ i sum_j d_j sin(2 pi jk/R) (forward/inverse), X_{R-k} the conjugate
+// combination -- the form of dft5 for any odd prime.
+template <typename T, bool INV, int R> __device__ __forceinline__ void dftP(cx<T> *v)
+{
+	constexpr int H = (R - 1) / 2;
+	cx<T> sp[H], dm[H];
+	cx<T> s0 = v[0];
+#pragma unroll
+	for (int j = 1; j <= H; j++) {
+		sp[j - 1] = add(v[j], v[R - j]);
+		dm[j - 1] = sub(v[j], v[R - j]);
+		s0 = add(s0, sp[j - 1]);
+	}
+	const cx<T> x0 = v[0];
+#pragma unroll
+	for (int k = 1; k <= H; k++) {
+		cx<T> a = x0, b = mk<T>(0, 0);
+#pragma unroll
+		for (int j = 1; j <= H; j++) {
+			const int m = (j * k) % R;  // cos is even and sin odd about R/2: fold m into 1..H
+			const T c = (T)PrimeTw<R>::C[(m <= H ? m : R - m) - 1];
+			const T sn = (T)(m <= H ? PrimeTw<R>::S[m - 1] : -PrimeTw<R>::S[R - m - 1]);
+			a = mk<T>(a.x + c * sp[j - 1].x, a.y + c * sp[j - 1].y);
+			b = mk<T>(b.x + sn * dm[j - 1].x, b.y + sn * dm[j - 1].y);
+		}
+		const cx<T> bi = mulmi<T, INV>(b);  // -i b (forward) or +i b (inverse)
+		v[k] = add(a, bi);
+		v[R - k] = sub(a, bi);
+	}
+	v[0] = s0;
+}
+
 template <typename T, bool INV, int R> __device__ __forceinline__ void dftR(cx<T> *v)
 {
 	if constexpr (R == 2) dft2<T, INV>(v);
 	else if constexpr (R == 3) dft3<T, INV>(v);
 	else if constexpr (R == 4) dft4<T, INV>(v);
 	else if constexpr (R == 5) dft5<T, INV>(v);
+	else if constexpr (R == 7 || R == 11 || R == 13) dftP<T, INV, R>(v);
 	else dft8<T, INV>(v);
 }
 
@@ -199,6 +259,9 @@ __device__ void lds_fft(cx<T> *buf, const FftPlan &plan, int nlines, int KS, int
 			case 4: stockham_stage<T, INV, COLMAJ, MAXIT, 4>(buf, N, nlines, KS, LS, Ns, tw); break;
 			case 2: stockham_stage<T, INV, COLMAJ, MAXIT, 2>(buf, N, nlines, KS, LS, Ns, tw); break;
 			case 3: stockham_stage<T, INV, COLMAJ, MAXIT, 3>(buf, N, nlines, KS, LS, Ns, tw); break;
+			case 7: stockham_stage<T, INV, COLMAJ, MAXIT, 7>(buf, N, nlines, KS, LS, Ns, tw); break;
+			case 11: stockham_stage<T, INV, COLMAJ, MAXIT, 11>(buf, N, nlines, KS, LS, Ns, tw); break;
+			case 13: stockham_stage<T, INV, COLMAJ, MAXIT, 13>(buf, N, nlines, KS, LS, Ns, tw); break;
 			default: stockham_stage<T, INV, COLMAJ, MAXIT, 5>(buf, N, nlines, KS, LS, Ns, tw); break;
 		}
 		Ns *= plan.rad[s];

@@ -57,7 +57,8 @@ int adda_hip_fastgrid(int x, int divis);
  * Replaces InitDmatrix (src/fft.c:918-1416, with InterTerm_poi, src/interaction.c:290-331) together with oclinit
  * (src/oclcore.c:327-437). position: ushort[3*nd] grid coordinates (src/vars.h:94); material: int[nd] indices into the
  * cc_sqrt table of nmat rows (src/vars.h:65; nmat may equal nd for per-dipole materials). grid*: padded FFT grid
- * (ParSetup, src/comm.c:489-494), each >= 2*box and a 2/3/5-smooth even number. flags: bit0 = also keep an FP32
+ * (ParSetup, src/comm.c:489-494), each >= 2*box and even, of one of ADDA's fftFit rules (src/fft.c:528-562): 2/3/5-smooth
+ * (Temperton FFT, clAmdFft), or also with factors of 7 and one 11 or 13 (FFTW3). flags: bit0 = also keep an FP32
  * copy of the D-matrix and FP32 work buffers (mixed precision).  Returns NULL on failure. */
 adda_hip_op *adda_hip_op_create(int device, int boxX, int boxY, int boxZ, int gridX, int gridY, int gridZ, size_t nd,
 	const unsigned short *position, const int *material, int nmat, double wavenum, double gridspace, int reduced_fft,

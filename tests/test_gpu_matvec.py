@@ -328,3 +328,39 @@ def test_matvec_compiled_plans(hipdev, axis, n):
                         assert op.get_config(prec) == " ".join(vs)
                         ys, _ = op.matvec(x, prec=prec)
                         assert np.array_equal(ys, y), (prec, vs)
+
+
+@pytest.mark.parametrize("box,grid", [((14, 22, 26), (28, 44, 52)), ((52, 28, 44), (104, 56, 88))])
+def test_matvec_fftw_grids(hipdev, box, grid):
+    """Grids of ADDA's FFTW fftFit rule (src/fft.c:528-562: factors of 7 and one 11 or 13 per dimension), which an
+    FFTW-built ADDA hands to MatVec (104 = 8*13 is a grid of the batched workload's class): radix-7/11/13 stages in
+    the run-time plans and the D-matrix build, against the oracle's FFT matvec (its transform recurses over any
+    prime factor), FP64 1e-12 and FP32 2e-5, both her; the small grid also against the direct sum."""
+    from cases import read_shape
+    g = np.indices(box).reshape(3, -1).T
+    ctr = (np.array(box) - 1) / 2
+    keep = (((g - ctr) / (np.array(box) / 2)) ** 2).sum(1) <= 1.0  # an ellipsoid filling the box
+    pos, b = read_shape(g[keep])
+    assert tuple(b) == tuple(box)
+    c = Case(pos, b, 1.33 + 0.05j, dpl=15, grid=grid)
+    assert tuple(hipdev.hip().adda_hip_fftfit(2 * n, 1) for n in box) != grid  # not grids of the 2/3/5 rule
+    cc, ccs = c.cc(PROP_Z, POL_Y)
+    op = _op(hipdev, c, fp32=True)
+    try:
+        op.set_cc_sqrt(ccs)
+        rng = np.random.default_rng(17)
+        x = rng.standard_normal(3 * c.nd) + 1j * rng.standard_normal(3 * c.nd)
+        d = c.dmatrix()
+        assert _rel(op.dmatrix(), d) < 1e-12
+        for her in (False, True):
+            y_or, ip_or = c.matvec_fft(d, ccs, x, her=her)
+            y64, ip64 = op.matvec(x, her=her)
+            assert _rel(y64, y_or) < 1e-12, her
+            assert abs(ip64 - ip_or) / ip_or < 1e-12
+            y32, _ = op.matvec(x, her=her, prec=hipdev.F32)
+            assert _rel(y32, y_or) < 2e-5, her
+        if c.nd < 5000:
+            y_d, _ = c.matvec_direct(ccs, x)
+            assert _rel(op.matvec(x)[0], y_d) < 1e-12
+    finally:
+        op.close()

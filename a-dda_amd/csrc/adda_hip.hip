@@ -1373,6 +1373,60 @@ __global__ void __launch_bounds__(kDkThreads) k_dk_cg_s2(double *st, const doubl
 	st[DK_SC + 1] = st[DK_SC + 0];  // ro_old = ro_new
 }
 
+// BiCG_CS scalar steps (src/iterative.c:549-630). Slots: 0 ro_new (c), 2 ro_old (c), 4 alpha (c), 20 p-coef beta
+// (c), 22 one (c), 24 alpha (c, x update), 26 -alpha (c, r update). ro = r^T r comes from self-bilinear rows re^2,
+// im^2, re*im (a dot pass in the first iteration, the r update's own partial sums afterwards): r^T r = (re^2 - im^2)
+// + 2i re*im and ||r||^2 = re^2 + im^2 (nDotProdSelf_conj_Norm2, src/linalg.c:127-144).
+__device__ void dk_bcs_ro(double *st, const double *v, int niter)
+{
+	const dcx ro_new = dmk(v[0] - v[1], 2 * v[2]);
+	if (dabs(ro_new) / st[DK_INPRODR] < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 1; return; }
+	if (niter == 1) st_c(st, DK_SC + 20, dmk(0, 0));
+	else st_c(st, DK_SC + 20, ddiv(ro_new, ld_c(st, DK_SC + 2)));
+	st_c(st, DK_SC + 0, ro_new);
+}
+__global__ void __launch_bounds__(kDkThreads) k_dk_bcs_s0(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 3, v, red_lds);
+	if (threadIdx.x == 0) dk_bcs_ro(st, v, 1);
+}
+// mu = p^T A p (rows 0, 1 of a bilinear dot pass) -> alpha
+__global__ void __launch_bounds__(kDkThreads) k_dk_bcs_s1(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx mu = dmk(v[0], v[1]), ro_new = ld_c(st, DK_SC + 0);
+	if (dabs(mu) / dabs(ro_new) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 2; return; }
+	const dcx alpha = ddiv(ro_new, mu);
+	st_c(st, DK_SC + 4, alpha);
+	st_c(st, DK_SC + 24, alpha);
+	st_c(st, DK_SC + 26, dneg(alpha));
+	st_c(st, DK_SC + 2, ro_new);  // ro_old = ro_new
+}
+// End of an iteration: the progress test on ||r||^2 and, if the solve goes on, the next iteration's ro and beta from
+// the same rows of the r update
+__global__ void __launch_bounds__(kDkThreads) k_dk_bcs_prog(double *st, double *hist, const double *partials, int nblk,
+	int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	if (hist) hist += (size_t)blockIdx.x * (size_t)st[DK_HISTLEN];
+	dk_reduce(partials, nblk, 3, v, red_lds);
+	if (threadIdx.x != 0) return;
+	dk_progress(st, hist, v[0] + v[1]);
+	if (st[DK_STATUS] == 0) dk_bcs_ro(st, v, 2);
+}
+
 struct dk_state {
 	double *st;    // [K][DK_NSLOTS]
 	double *hist;  // [K][histlen]
@@ -1472,6 +1526,30 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
 		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)rt);
 	hipLaunchKernelGGL(k_dk_bs_prog_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// BiCG_CS (src/iterative.c:549-630): p = beta p + r; Av = A.p; mu = p^T Av; x += alpha p; r -= alpha Av with the
+// self-bilinear rows that give ||r||^2 and the next r^T r
+template <typename T>
+static int bicgcs_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *x, void *r, void *p, void *Av)
+{
+	const DkGeo g = dk_geo(op, k);
+	double *st = k->st;
+	if (niter == 1) {
+		dot_dev<T>(op, g, st, r, nullptr, ADDA_DOT_SELF_BILIN);
+		hipLaunchKernelGGL(k_dk_bcs_s0, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	}
+	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, r, nullptr, st, DK_SC + 20, DK_SC + 22, DK_SC + 22, 0);
+	if (!(niter == 1 && matvec_ready))
+		if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0)) return -1;
+	dot_dev<T>(op, g, st, p, Av, ADDA_DOT_BILIN);
+	hipLaunchKernelGGL(k_dk_bcs_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	lc_dev<T>(op, g, x, x, p, nullptr, st, DK_SC + 22, DK_SC + 24, DK_SC + 22, 0);
+	lc_dev<T>(op, g, r, r, Av, nullptr, st, DK_SC + 22, DK_SC + 26, DK_SC + 22, 2);
+	hipLaunchKernelGGL(k_dk_bcs_prog, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb,
+		g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -2498,6 +2576,16 @@ extern "C" int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state,
 	dk_state *k = (dk_state *)state;
 	if (prec == ADDA_F64) return bicgstab_iter_t<double>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
 	return bicgstab_iter_t<float>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
+}
+
+extern "C" int adda_hip_dk_bicgcs_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,
+	void *r, void *p, void *Av)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) return bicgcs_iter_t<double>(op, k, niter, matvec_ready, x, r, p, Av);
+	return bicgcs_iter_t<float>(op, k, niter, matvec_ready, x, r, p, Av);
 }
 
 extern "C" int adda_hip_dk_cgnr_iter(adda_hip_op *op, int prec, void *state, int niter, void *x, void *r, void *p,

@@ -569,6 +569,9 @@ static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int n,
 			/* v = vec1, s = vec2, rtilda = vec3, t = Avecbuffer (src/iterative.c:660-662) */
 			if (adda_hip_dk_bicgstab_iter(op, prec, st, *it, matvec_ready, x, r, p, v1, v2, v3, Av)) return -1;
 		}
+		else if (method == ADDA_IT_BICG_CS) {
+			if (adda_hip_dk_bicgcs_iter(op, prec, st, *it, matvec_ready, x, r, p, Av)) return -1;
+		}
 		else if (adda_hip_dk_cgnr_iter(op, prec, st, *it, x, r, p, Av)) return -1;
 	}
 	return 0;
@@ -683,6 +686,8 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 		CKG(adda_hip_lincomb_batch(op, prec, K, st, v3, r, ca, NULL, cb, NULL, cb, 0, NULL)); /* r~ = r_0 */
 		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 24] = 1;
 	}
+	else if (method == ADDA_IT_BICG_CS)
+		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 22] = 1;
 	else
 		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 6] = 1;
 	CKG(adda_hip_dk_set(op, st, 0, S, K * NS));
@@ -765,7 +770,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			res[k].niter = done_it;
 			res[k].rel = sqrt(s[3] * s[0]);
 			/* matvecs of the iterations the device ran */
-			if (method == ADDA_IT_QMR_CS) res[k].nmv = done_it - (done_it > 0 && matvec_ready);
+			if (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICG_CS) res[k].nmv = done_it - (done_it > 0 && matvec_ready);
 			else if (method == ADDA_IT_BICGSTAB) res[k].nmv = 2 * done_it - (done_it > 0 && matvec_ready);
 			else res[k].nmv = 2 * done_it;
 			res[k].nmv += init_mv;
@@ -803,7 +808,8 @@ done:
 static int device_method(const adda_solver_opts *opt)
 {
 	return !opt->host_scalars && opt->chp_type == ADDA_CHP_NONE && !opt->chp_load &&
-		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR);
+		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR ||
+		opt->method == ADDA_IT_BICG_CS);
 }
 
 /* one Krylov solve of K right-hand sides: device-resident recurrences, or (K = 1) the host form */

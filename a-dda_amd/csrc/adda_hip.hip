@@ -1026,11 +1026,12 @@ __device__ void dk_reduce(const double *partials, int nblk, int nrows, double *o
 	st += (size_t)blockIdx.x * DK_NSLOTS;                                                                        \
 	partials += (size_t)blockIdx.x * (ps)
 
-// out = st[ia]*a + st[ib]*b + st[ic]*c (complex coefficient slots; null vectors skipped); red: 0 none, 1 ||out||^2
-// (row 0), 2 self-bilinear sums re^2, im^2, re*im (rows 0-2).
+// out = st[ia]*a' + st[ib]*b' + st[ic]*c (complex coefficient slots; null vectors skipped; a' = conj(a) if
+// ADDA_CONJ_A is set in conj, b' likewise); red: 0 none, 1 ||out||^2 (row 0), 2 self-bilinear sums re^2, im^2, re*im
+// (rows 0-2).
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const cx<T> *b, const cx<T> *c,
-	const double *st, int ia, int ib, int ic, size_t n, long vs, int red, double *partials, int ps)
+	const double *st, int ia, int ib, int ic, size_t n, long vs, int red, double *partials, int ps, int conj)
 {
 	__shared__ double red_lds[512];
 	DK_RHS_VEC(st, partials, ps);
@@ -1043,8 +1044,18 @@ __global__ void __launch_bounds__(256) k_dk_lc(cx<T> *out, const cx<T> *a, const
 	double s0 = 0, s1 = 0, s2 = 0;
 	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
 		double rx = 0, ry = 0;
-		if (a) { const cx<T> v = a[i]; rx += ca.x * v.x - ca.y * v.y; ry += ca.x * v.y + ca.y * v.x; }
-		if (b) { const cx<T> v = b[i]; rx += cb.x * v.x - cb.y * v.y; ry += cb.x * v.y + cb.y * v.x; }
+		if (a) {
+			const cx<T> v = a[i];
+			const double vx = v.x, vy = (conj & ADDA_CONJ_A) ? -(double)v.y : (double)v.y;
+			rx += ca.x * vx - ca.y * vy;
+			ry += ca.x * vy + ca.y * vx;
+		}
+		if (b) {
+			const cx<T> v = b[i];
+			const double vx = v.x, vy = (conj & ADDA_CONJ_B) ? -(double)v.y : (double)v.y;
+			rx += cb.x * vx - cb.y * vy;
+			ry += cb.x * vy + cb.y * vx;
+		}
 		if (c) { const cx<T> v = c[i]; rx += cc.x * v.x - cc.y * v.y; ry += cc.x * v.y + cc.y * v.x; }
 		const cx<T> r = mk<T>((T)rx, (T)ry);
 		out[i] = r;
@@ -1119,11 +1130,14 @@ __global__ void __launch_bounds__(256) k_dk_xr(cx<T> *x, const cx<T> *p, const c
 	}
 }
 
-// QMR_CS tail (src/iterative.c:1007-1029): pn = q0*pa + q1*pb + q2*v -> pdst; x += tau*pn; vt = vtilda/beta;
-// vtilda = vt; r = |s|^2 r + cr*vt; partial ||r||^2.
+// Tail of the three-term-recurrence methods: pn = q0*pa + q1*pb + q2*v -> pdst; x += tau*pn; then by mode:
+// QU_VT_SCALE vt = ib*vtilda' -> vtilda (vtilda' = conj(vtilda) with QU_VT_CONJ), else vt = vtilda as it is;
+// QU_R r = cs*r + cr*vt with partial ||r||^2. QMR_CS (src/iterative.c:1007-1029): SCALE|R; QMR_CS_2 (1122-1131):
+// R; CSYM (866-882): SCALE|CONJ.
+enum { QU_VT_SCALE = 1, QU_VT_CONJ = 2, QU_R = 4 };
 template <typename T>
 __global__ void __launch_bounds__(256) k_dk_qmr_upd(cx<T> *pdst, const cx<T> *pa, const cx<T> *pb, const cx<T> *v,
-	cx<T> *x, cx<T> *vtilda, cx<T> *r, const double *st, size_t n, long vs, double *partials, int ps)
+	cx<T> *x, cx<T> *vtilda, cx<T> *r, const double *st, size_t n, long vs, double *partials, int ps, int mode)
 {
 	__shared__ double red_lds[512];
 	DK_RHS_VEC(st, partials, ps);
@@ -1134,7 +1148,7 @@ __global__ void __launch_bounds__(256) k_dk_qmr_upd(cx<T> *pdst, const cx<T> *pa
 	v += voff;
 	x += voff;
 	vtilda += voff;
-	r += voff;
+	if (r) r += voff;
 	const dcx q0 = ld_c(st, DK_SC + 30), q1 = ld_c(st, DK_SC + 32), q2 = ld_c(st, DK_SC + 34);
 	const dcx tau = ld_c(st, DK_SC + 36), ib = ld_c(st, DK_SC + 38), cr = ld_c(st, DK_SC + 40);
 	const double cs = st[DK_SC + 42];
@@ -1148,16 +1162,24 @@ __global__ void __launch_bounds__(256) k_dk_qmr_upd(cx<T> *pdst, const cx<T> *pa
 		pdst[i] = pn;
 		const cx<T> xv = x[i];
 		x[i] = mk<T>((T)((double)xv.x + tau.x * pn.x - tau.y * pn.y), (T)((double)xv.y + tau.x * pn.y + tau.y * pn.x));
-		const cx<T> w = vtilda[i];
-		const cx<T> vt = mk<T>((T)(ib.x * w.x - ib.y * w.y), (T)(ib.x * w.y + ib.y * w.x));
-		vtilda[i] = vt;
-		const cx<T> rv0 = r[i];
-		const cx<T> rv = mk<T>((T)(cs * rv0.x + cr.x * vt.x - cr.y * vt.y), (T)(cs * rv0.y + cr.x * vt.y + cr.y * vt.x));
-		r[i] = rv;
-		s0 += (double)rv.x * rv.x + (double)rv.y * rv.y;
+		cx<T> vt = vtilda[i];
+		if (mode & QU_VT_SCALE) {
+			const double wx = vt.x, wy = (mode & QU_VT_CONJ) ? -(double)vt.y : (double)vt.y;
+			vt = mk<T>((T)(ib.x * wx - ib.y * wy), (T)(ib.x * wy + ib.y * wx));
+			vtilda[i] = vt;
+		}
+		if (mode & QU_R) {
+			const cx<T> rv0 = r[i];
+			const cx<T> rv = mk<T>((T)(cs * rv0.x + cr.x * vt.x - cr.y * vt.y),
+				(T)(cs * rv0.y + cr.x * vt.y + cr.y * vt.x));
+			r[i] = rv;
+			s0 += (double)rv.x * rv.x + (double)rv.y * rv.y;
+		}
 	}
-	const double t = block_sum<T>(s0, red_lds);
-	if (threadIdx.x == 0) partials[blockIdx.x] = t;
+	if (mode & QU_R) {
+		const double t = block_sum<T>(s0, red_lds);
+		if (threadIdx.x == 0) partials[blockIdx.x] = t;
+	}
 }
 
 // ProgressReport (src/iterative.c:282-318) + the loop condition of IterativeSolver (1467) + the final tests
@@ -1427,6 +1449,133 @@ __global__ void __launch_bounds__(kDkThreads) k_dk_bcs_prog(double *st, double *
 	if (st[DK_STATUS] == 0) dk_bcs_ro(st, v, 2);
 }
 
+// QMR_CS_2 scalar steps (src/iterative.c:1082-1136). Slots: 0 ro, 1 c, 2 theta, 4 eps (c), 6 eta (c), 8 delta (c),
+// 10 1/ro (c), 12 p coefficient -ro delta/eps (c), 14 one (c), 16 -beta (c), 18 beta (c); the tail kernel's 30 f^2 (c),
+// 34 eta (c), 36 one (c), 40 -eta (c), 42 |s|^2.
+// delta = v^T v from the self-bilinear rows of the scaling v = v/ro
+__global__ void __launch_bounds__(kDkThreads) k_dk_q2_s1(double *st, const double *partials, int nblk, int niter, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 3, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx delta = dmk(v[0] - v[1], 2 * v[2]);
+	if (dabs(delta) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 1; return; }
+	st_c(st, DK_SC + 8, delta);
+	if (niter > 1) st_c(st, DK_SC + 12, ddiv(dscl(delta, -st[DK_SC + 0]), ld_c(st, DK_SC + 4)));
+}
+// eps = p^T A p (rows 0, 1 of a bilinear dot pass), beta = eps/delta
+__global__ void __launch_bounds__(kDkThreads) k_dk_q2_s2(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx eps = dmk(v[0], v[1]), bt = ddiv(eps, ld_c(st, DK_SC + 8));
+	if (dabs(eps) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 2; return; }
+	st_c(st, DK_SC + 4, eps);
+	st_c(st, DK_SC + 16, dneg(bt));
+	st_c(st, DK_SC + 18, bt);
+}
+// ||v||^2 of v = A p - beta v (row 0) -> the rotation and the coefficients of the tail
+__global__ void __launch_bounds__(kDkThreads) k_dk_q2_s3(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 1, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const double ro = st[DK_SC + 0], c = st[DK_SC + 1], theta = st[DK_SC + 2];
+	const dcx bt = ld_c(st, DK_SC + 18);
+	const double ro_new = sqrt(v[0]);
+	const double th = ro_new / (c * dabs(bt));
+	const double c_nw = 1 / sqrt(1 + th * th), sabs2 = th * th / (1 + th * th), cr = c_nw / c;
+	const dcx eta = ddiv(dscl(ld_c(st, DK_SC + 6), -ro * cr * cr), bt);
+	const double f = theta * c_nw;
+	st_c(st, DK_SC + 6, eta);
+	st_c(st, DK_SC + 30, dmk(f * f, 0));
+	st_c(st, DK_SC + 34, eta);
+	st_c(st, DK_SC + 40, dneg(eta));
+	st[DK_SC + 42] = sabs2;
+	st[DK_SC + 0] = ro_new;
+	st[DK_SC + 1] = c_nw;
+	st[DK_SC + 2] = th;
+	st_c(st, DK_SC + 10, dmk(1 / ro_new, 0));
+}
+
+// CSYM scalar steps (src/iterative.c:822-888). Slots: 0 beta, 1 c_old, 2 c_new, 4 tau (c), 6 s_old (c), 8 s_new (c),
+// 10 eta (c), 12 gamma (c), 14 theta (c), 20 -beta (c), 22 -alpha (c), 24 one (c); the tail kernel's 30
+// -theta/ksi (c), 32 -eta/ksi (c), 34 1/ksi (c), 36 c_new tau (c), 38 1/beta (c).
+// alpha = q^T A q (rows 0, 1 of a bilinear dot pass)
+__global__ void __launch_bounds__(kDkThreads) k_dk_cs_s1(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx alpha = dmk(v[0], v[1]);
+	const double beta = st[DK_SC + 0], c_old = st[DK_SC + 1], c_new = st[DK_SC + 2];
+	const dcx s_old = ld_c(st, DK_SC + 6), s_new = ld_c(st, DK_SC + 8);
+	st_c(st, DK_SC + 10, dadd(dmk(c_old * c_new * beta, 0), dmul(alpha, dconj(s_new))));
+	st_c(st, DK_SC + 12, dsub(dscl(alpha, c_new), dscl(dscl(s_new, c_old), beta)));
+	st_c(st, DK_SC + 14, dscl(dconj(s_old), beta));
+	st_c(st, DK_SC + 20, dmk(-beta, 0));
+	st_c(st, DK_SC + 22, dneg(alpha));
+}
+// beta = ||q~|| (row 0 of the q~ update) -> the Givens rotation and the coefficients of the tail
+__global__ void __launch_bounds__(kDkThreads) k_dk_cs_s2(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[1];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 1, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const double beta = sqrt(v[0]);
+	const dcx gamma = ld_c(st, DK_SC + 12);
+	st[DK_SC + 1] = st[DK_SC + 2];
+	st_c(st, DK_SC + 6, ld_c(st, DK_SC + 8));
+	double c_new, dtmp = dabs(gamma);
+	dcx s_new, invksi;
+	if (dtmp == 0) {
+		if (beta == 0) { st[DK_STATUS] = -1; st[DK_CODE] = 1; return; }
+		c_new = 0;
+		s_new = dmk(1, 0);
+		invksi = dmk(1 / beta, 0);
+	}
+	else {
+		if (dtmp < beta) { dtmp = dtmp / beta; c_new = dtmp / sqrt(1 + dtmp * dtmp); }
+		else { dtmp = beta / dtmp; c_new = 1 / sqrt(1 + dtmp * dtmp); }
+		invksi = ddiv(dmk(c_new, 0), gamma);
+		s_new = dscl(invksi, beta);
+	}
+	const dcx tau = ld_c(st, DK_SC + 4);
+	st[DK_SC + 0] = beta;
+	st[DK_SC + 2] = c_new;
+	st_c(st, DK_SC + 8, s_new);
+	st_c(st, DK_SC + 30, dmul(dneg(ld_c(st, DK_SC + 14)), invksi));
+	st_c(st, DK_SC + 32, dmul(dneg(ld_c(st, DK_SC + 10)), invksi));
+	st_c(st, DK_SC + 34, invksi);
+	st_c(st, DK_SC + 36, dscl(tau, c_new));
+	st_c(st, DK_SC + 38, dmk(1 / beta, 0));
+	st_c(st, DK_SC + 4, dmul(tau, dneg(s_new)));
+}
+// CSYM's residual is the recurrence's |tau|^2 (src/iterative.c:884-885)
+__global__ void __launch_bounds__(kDkThreads) k_dk_cs_prog(double *st, double *hist)
+{
+	st += (size_t)blockIdx.x * DK_NSLOTS;
+	DK_ACTIVE(st);
+	if (hist) hist += (size_t)blockIdx.x * (size_t)st[DK_HISTLEN];
+	if (threadIdx.x == 0) dk_progress(st, hist, dabs2(ld_c(st, DK_SC + 4)));
+}
+
 struct dk_state {
 	double *st;    // [K][DK_NSLOTS]
 	double *hist;  // [K][histlen]
@@ -1462,10 +1611,10 @@ static void dot_dev(adda_hip_op *op, const DkGeo &g, const double *gate, const v
 
 template <typename T>
 static void lc_dev(adda_hip_op *op, const DkGeo &g, void *out, const void *a, const void *b, const void *c,
-	const double *st, int ia, int ib, int ic, int red)
+	const double *st, int ia, int ib, int ic, int red, int conj = 0)
 {
 	hipLaunchKernelGGL(k_dk_lc<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)out, (const cx<T> *)a,
-		(const cx<T> *)b, (const cx<T> *)c, st, ia, ib, ic, g.n, g.vs, red, op->partials2, g.ps2);
+		(const cx<T> *)b, (const cx<T> *)c, st, ia, ib, ic, g.n, g.vs, red, op->partials2, g.ps2, conj);
 }
 
 template <typename T>
@@ -1484,7 +1633,7 @@ static int qmr_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready,
 	const void *pa = niter >= 3 ? p_old : nullptr, *pb = niter >= 2 ? p_new : nullptr;
 	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
 		(const cx<T> *)pb, (const cx<T> *)v, (cx<T> *)x, (cx<T> *)vtilda, (cx<T> *)r, st, g.n, g.vs, op->partials2,
-		g.ps2);
+		g.ps2, QU_VT_SCALE | QU_R);
 	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1550,6 +1699,55 @@ static int bicgcs_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_rea
 	lc_dev<T>(op, g, r, r, Av, nullptr, st, DK_SC + 22, DK_SC + 26, DK_SC + 22, 2);
 	hipLaunchKernelGGL(k_dk_bcs_prog, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb,
 		g.ps2);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// QMR_CS_2 (src/iterative.c:1082-1136): v = v/ro with v^T v; p = v - ro delta/eps p; Av = A.p; eps = p^T Av;
+// v = Av - beta v with ||v||^2; d, x and r in one tail pass
+template <typename T>
+static int qmr2_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *v, void *d, void *p, void *Av,
+	void *x, void *r)
+{
+	const DkGeo g = dk_geo(op, k);
+	double *st = k->st;
+	lc_dev<T>(op, g, v, v, nullptr, nullptr, st, DK_SC + 10, DK_SC + 10, DK_SC + 10, 2);
+	hipLaunchKernelGGL(k_dk_q2_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, niter, g.ps2);
+	lc_dev<T>(op, g, p, niter == 1 ? nullptr : p, v, nullptr, st, DK_SC + 12, DK_SC + 14, DK_SC + 14, 0);
+	if (niter == 1 && matvec_ready) lc_dev<T>(op, g, Av, Av, nullptr, nullptr, st, DK_SC + 10, DK_SC + 10, DK_SC + 10, 0);
+	else if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)p, (cx<T> *)Av, nullptr, 0)) return -1;
+	dot_dev<T>(op, g, st, p, Av, ADDA_DOT_BILIN);
+	hipLaunchKernelGGL(k_dk_q2_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	lc_dev<T>(op, g, v, v, Av, nullptr, st, DK_SC + 16, DK_SC + 14, DK_SC + 14, 1);
+	hipLaunchKernelGGL(k_dk_q2_s3, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)d,
+		(const cx<T> *)(niter == 1 ? nullptr : d), (const cx<T> *)nullptr, (const cx<T> *)p, (cx<T> *)x, (cx<T> *)v,
+		(cx<T> *)r, st, g.n, g.vs, op->partials2, g.ps2, (int)QU_R);
+	hipLaunchKernelGGL(k_dk_progress, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, k->hist, op->partials2, g.nb, g.ps2);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// CSYM (src/iterative.c:822-888): Aq = A.q; alpha = q^T Aq; q~ = Aq - alpha q* - beta q_old* with ||q~||; the Givens
+// rotation; p, x and q_old = q~*/beta in one tail pass; the residual from |tau|^2
+template <typename T>
+static int csym_iter_t(adda_hip_op *op, dk_state *k, int niter, void *q_new, void *q_old, void *p_new, void *p_old,
+	void *Av, void *x)
+{
+	const DkGeo g = dk_geo(op, k);
+	double *st = k->st;
+	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)q_new, (cx<T> *)Av, nullptr, 0)) return -1;
+	dot_dev<T>(op, g, st, q_new, Av, ADDA_DOT_BILIN);
+	hipLaunchKernelGGL(k_dk_cs_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	lc_dev<T>(op, g, q_old, niter == 1 ? nullptr : q_old, q_new, Av, st, DK_SC + 20, DK_SC + 22, DK_SC + 24, 1,
+		ADDA_CONJ_A | ADDA_CONJ_B);
+	hipLaunchKernelGGL(k_dk_cs_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+	void *pdst = niter == 1 ? p_new : p_old;
+	const void *pa = niter >= 3 ? p_old : nullptr, *pb = niter >= 2 ? p_new : nullptr;
+	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
+		(const cx<T> *)pb, (const cx<T> *)q_new, (cx<T> *)x, (cx<T> *)q_old, (cx<T> *)nullptr, st, g.n, g.vs,
+		op->partials2, g.ps2, QU_VT_SCALE | QU_VT_CONJ);
+	hipLaunchKernelGGL(k_dk_cs_prog, dim3(g.K), dim3(64), 0, op->stream, st, k->hist);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -2576,6 +2774,26 @@ extern "C" int adda_hip_dk_bicgstab_iter(adda_hip_op *op, int prec, void *state,
 	dk_state *k = (dk_state *)state;
 	if (prec == ADDA_F64) return bicgstab_iter_t<double>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
 	return bicgstab_iter_t<float>(op, k, niter, matvec_ready, x, r, p, v, s, rtilda, t);
+}
+
+extern "C" int adda_hip_dk_qmr2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *v,
+	void *d, void *p, void *Av, void *x, void *r)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) return qmr2_iter_t<double>(op, k, niter, matvec_ready, v, d, p, Av, x, r);
+	return qmr2_iter_t<float>(op, k, niter, matvec_ready, v, d, p, Av, x, r);
+}
+
+extern "C" int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int niter, void *q_new, void *q_old,
+	void *p_new, void *p_old, void *Av, void *x)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) return csym_iter_t<double>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
+	return csym_iter_t<float>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
 }
 
 extern "C" int adda_hip_dk_bicgcs_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,

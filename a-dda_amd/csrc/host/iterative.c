@@ -572,6 +572,16 @@ static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int n,
 		else if (method == ADDA_IT_BICG_CS) {
 			if (adda_hip_dk_bicgcs_iter(op, prec, st, *it, matvec_ready, x, r, p, Av)) return -1;
 		}
+		else if (method == ADDA_IT_QMR_CS_2) {
+			/* v = vec1, d = vec2, p_k = pvec (src/iterative.c:1050-1053) */
+			if (adda_hip_dk_qmr2_iter(op, prec, st, *it, matvec_ready, v1, v2, p, Av, x, r)) return -1;
+		}
+		else if (method == ADDA_IT_CSYM) {
+			/* q_new, q_old, p_new, p_old renamed every iteration as in src/iterative.c:880-882 */
+			if (adda_hip_dk_csym_iter(op, prec, st, *it, *v, *vtilda, *p_new, *p_old, Av, x)) return -1;
+			if (*it >= 2) { void *t = *p_old; *p_old = *p_new; *p_new = t; }
+			{ void *t = *v; *v = *vtilda; *vtilda = t; }
+		}
 		else if (adda_hip_dk_cgnr_iter(op, prec, st, *it, x, r, p, Av)) return -1;
 	}
 	return 0;
@@ -584,7 +594,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	int init_field, int verbose, const int *active, const void *b, void *x, double *hist, int hist_len, int use_graph,
 	kres *res)
 {
-	const int mc_tab[5] = {50000, 30000, 10, 50000, 10}; /* params[].mc, src/iterative.c:118-132 */
+	const int mc_tab[7] = {50000, 30000, 10, 50000, 10, 15000, 50000}; /* params[].mc, src/iterative.c:118-132 */
 	const int NS = ADDA_DK_NSLOTS, hl = hist_len > 1 ? hist_len : 2;
 	int rc = 0, matvec_ready = 0, init_mv = 0;
 	void *graph = NULL;
@@ -688,12 +698,40 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	}
 	else if (method == ADDA_IT_BICG_CS)
 		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 22] = 1;
+	else if (method == ADDA_IT_QMR_CS_2) { /* src/iterative.c:1070-1080 */
+		for (int k = 0; k < K; k++) {
+			double *sc = S + (size_t)k * NS + ADDA_DK_SC;
+			if (active && !active[k]) continue;
+			sc[0] = sqrt(inprodR[k]); /* ro */
+			sc[1] = 1;                /* c */
+			sc[4] = 1;                /* eps */
+			sc[6] = -1;               /* eta */
+			sc[10] = 1 / sc[0];
+			sc[14] = sc[36] = sc[38] = 1;
+		}
+		SETC(ca, 1); SETC(cb, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, v1, r, ca, NULL, cb, NULL, cb, 0, NULL));
+	}
+	else if (method == ADDA_IT_CSYM) { /* src/iterative.c:808-818: tau = ||r_0||, q_1 = r_0* / tau (in rvec) */
+		for (int k = 0; k < K; k++) {
+			double *sc = S + (size_t)k * NS + ADDA_DK_SC;
+			ca[k].re = ca[k].im = 0;
+			if (active && !active[k]) continue;
+			sc[4] = sqrt(inprodR[k]); /* tau */
+			sc[2] = 1;                /* c_new; c_old, s_old, s_new, beta = 0 */
+			sc[24] = 1;
+			ca[k].re = 1 / sc[4];
+		}
+		SETC(cb, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, r, r, ca, NULL, cb, NULL, cb, ADDA_CONJ_A, NULL));
+	}
 	else
 		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 6] = 1;
 	CKG(adda_hip_dk_set(op, st, 0, S, K * NS));
 #undef SETC
 	{
 		void *v = v1, *vtilda = v2, *p_new = p, *p_old = v3;
+		if (method == ADDA_IT_CSYM) { v = r; vtilda = v1; p_old = v2; } /* q_new, q_old, p_new, p_old */
 		int it = 1, running = 0, chunk = DK_CHUNK, tail = 0;
 		for (int k = 0; k < K; k++) {
 			running |= S[(size_t)k * NS + ADDA_DK_STATUS] == 0;
@@ -770,7 +808,9 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			res[k].niter = done_it;
 			res[k].rel = sqrt(s[3] * s[0]);
 			/* matvecs of the iterations the device ran */
-			if (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICG_CS) res[k].nmv = done_it - (done_it > 0 && matvec_ready);
+			if (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICG_CS || method == ADDA_IT_QMR_CS_2)
+				res[k].nmv = done_it - (done_it > 0 && matvec_ready);
+			else if (method == ADDA_IT_CSYM) res[k].nmv = done_it;
 			else if (method == ADDA_IT_BICGSTAB) res[k].nmv = 2 * done_it - (done_it > 0 && matvec_ready);
 			else res[k].nmv = 2 * done_it;
 			res[k].nmv += init_mv;
@@ -809,7 +849,7 @@ static int device_method(const adda_solver_opts *opt)
 {
 	return !opt->host_scalars && opt->chp_type == ADDA_CHP_NONE && !opt->chp_load &&
 		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR ||
-		opt->method == ADDA_IT_BICG_CS);
+		opt->method == ADDA_IT_BICG_CS || opt->method == ADDA_IT_QMR_CS_2 || opt->method == ADDA_IT_CSYM);
 }
 
 /* one Krylov solve of K right-hand sides: device-resident recurrences, or (K = 1) the host form */

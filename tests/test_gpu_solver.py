@@ -33,8 +33,7 @@ def _problem(adda, geo, m, dpl, mat=None, polrel=0, fp32=False):
 @pytest.mark.parametrize("method,gold,mi", METHODS)
 def test_c1_solvers(hipdev, method, gold, mi, host_scalars):
     """Both forms of the recurrences: on the host (the reference's control flow, one synchronising reduction per
-    inner product) and device-resident (FP64 scalar kernels on the GPU; CSYM, BCGS2 and QMR_CS_2 fall back to the host
-    form)."""
+    inner product) and device-resident (FP64 scalar kernels on the GPU; BCGS2 falls back to the host form)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
     try:
         for pol, vec in (("Y", POL_Y), ("X", POL_X)):
@@ -77,26 +76,30 @@ def test_c1_bcgs2_qmr2_history_vs_reference_log(hipdev, method, gold, nmv_per_it
         pr.close()
 
 
-def test_c1_bicg_device_form_vs_host_form(hipdev):
-    """BiCG_CS: the device-resident recurrences (r^T r taken from the r update's own partial sums) against the host
-    form (a separate reduction per inner product): iteration and matvec counts, the residual histories to 1e-6 over
-    their first half (later the two summation orders drift apart, as in ADDA's own runs across processor counts), the
-    converged fields to 1e-8, and a breakdown-free run from a non-zero initial field (matvec_ready off)."""
+@pytest.mark.parametrize("method,mv_ready", [("bicg", True), ("qmr2", True), ("csym", False)])
+def test_c1_device_form_vs_host_form(hipdev, method, mv_ready):
+    """BiCG_CS, QMR_CS_2 and CSYM: the device-resident recurrences (inner products taken from the partial sums of the
+    vector updates that produce their operands where the method allows) against the host form (a separate reduction
+    per inner product): iteration and matvec counts, the residual histories to 1e-6 over their first half (later the
+    two summation orders drift apart, as ADDA's own runs do across processor counts), the converged fields to 1e-8,
+    and a run from the incident field as the initial one (no reuse of its matvec)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
     try:
         for vec in (POL_Y, POL_X):
             pr.set_polarization(PROP_Z, vec)
-            rh = pr.solve("bicg", eps=1e-10, host_scalars=True, hist_len=400)
+            rh = pr.solve(method, eps=1e-10, host_scalars=True, hist_len=400)
             xh = pr.field_x()
-            rd = pr.solve("bicg", eps=1e-10, host_scalars=False, hist_len=400)
+            rd = pr.solve(method, eps=1e-10, host_scalars=False, hist_len=400)
             xd = pr.field_x()
             assert rh["rc"] == 0 and rd["rc"] == 0
             assert abs(rd["niter"] - rh["niter"]) <= 1 and rd["hist"][-1] < 1e-10
-            assert rd["nmatvec"] == rd["niter"] and rh["nmatvec"] == rh["niter"]  # x_0 = 0 serves iteration 1
+            # x_0 = 0: the initial field's matvec serves iteration 1 where the method reuses it
+            extra = 0 if mv_ready else 1
+            assert rd["nmatvec"] == rd["niter"] + extra and rh["nmatvec"] == rh["niter"] + extra
             k = rd["niter"] // 2
             assert np.allclose(rd["hist"][:k], rh["hist"][:k], rtol=1e-6)
             assert np.abs(xd - xh).max() / np.abs(xh).max() < 1e-8
-            ri = pr.solve("bicg", eps=1e-10, host_scalars=False, init_field=hipdev.IF_INC)
+            ri = pr.solve(method, eps=1e-10, host_scalars=False, init_field=hipdev.IF_INC)
             assert ri["rc"] == 0 and ri["nmatvec"] == ri["niter"] + 1
             assert np.abs(pr.field_x() - xh).max() / np.abs(xh).max() < 1e-8
     finally:

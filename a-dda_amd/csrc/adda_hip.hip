@@ -194,7 +194,7 @@ struct adda_hip_op {
 	// reductions
 	double *partials = nullptr;  // max(field-sum partials, kcap * gather blocks)
 	size_t npartials = 0;
-	double *partials2 = nullptr; // [kcap][4*kRedBlocks] second region for device-resident solver dots
+	double *partials2 = nullptr; // [kcap][9*kRedBlocks] second region for device-resident solver dots
 	double *red = nullptr;       // [4*kcap] device result
 	double *red_host = nullptr;  // [4*kcap] pinned
 	void *hv_arg = nullptr, *hv_res = nullptr;  // device copies of adda_hip_matvec_host's vectors (made at first use)
@@ -1565,15 +1565,130 @@ __global__ void __launch_bounds__(kDkThreads) k_dk_cs_s2(double *st, const doubl
 	st_c(st, DK_SC + 34, invksi);
 	st_c(st, DK_SC + 36, dscl(tau, c_new));
 	st_c(st, DK_SC + 38, dmk(1 / beta, 0));
-	st_c(st, DK_SC + 4, dmul(tau, dneg(s_new)));
+	const dcx tau_new = dmul(tau, dneg(s_new));
+	st_c(st, DK_SC + 4, tau_new);
+	st[DK_SC + 16] = dabs2(tau_new);  // CSYM's residual: |tau|^2 (src/iterative.c:884-885)
 }
-// CSYM's residual is the recurrence's |tau|^2 (src/iterative.c:884-885)
-__global__ void __launch_bounds__(kDkThreads) k_dk_cs_prog(double *st, double *hist)
+// The progress test on a residual norm^2 that a scalar step left in a slot (CSYM, BCGS2)
+__global__ void __launch_bounds__(kDkThreads) k_dk_prog_slot(double *st, double *hist, int slot)
 {
 	st += (size_t)blockIdx.x * DK_NSLOTS;
 	DK_ACTIVE(st);
 	if (hist) hist += (size_t)blockIdx.x * (size_t)st[DK_HISTLEN];
-	if (threadIdx.x == 0) dk_progress(st, hist, dabs2(ld_c(st, DK_SC + 4)));
+	if (threadIdx.x == 0) dk_progress(st, hist, st[slot]);
+}
+
+// BCGS2 (l = 2) scalar steps (src/iterative.c:391-483). Slots: 0 rho0 (c), 2 rho1 (c), 4 alpha (c), 6 omega (c),
+// 8 -beta (c), 10 one (c), 14 -alpha (c), 16 ||r||^2 after the polynomial step, 20 -y1, 22 -y2, 24 y1, 26 y2 (c).
+// rho1 = r_j . r~* (rows 0, 1 of a conjugate dot pass) and beta = alpha rho1/rho0 (none in the very first step)
+__global__ void __launch_bounds__(kDkThreads) k_dk_b2_s1(double *st, const double *partials, int nblk, int first, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx rho1 = dmk(v[0], v[1]);
+	if (!first) {
+		const dcx rho0 = ld_c(st, DK_SC + 0), alpha = ld_c(st, DK_SC + 4);
+		if (dabs(rho0) / (dabs(rho1) * dabs(alpha)) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 1; return; }
+		st_c(st, DK_SC + 8, dneg(ddiv(dmul(alpha, rho1), rho0)));
+	}
+	st_c(st, DK_SC + 0, rho1);
+	st_c(st, DK_SC + 2, rho1);
+}
+// sigma = u_j+1 . r~* -> alpha = rho1/sigma
+__global__ void __launch_bounds__(kDkThreads) k_dk_b2_s2(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[3];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	dk_reduce(partials, nblk, 2, v, red_lds);
+	if (threadIdx.x != 0) return;
+	const dcx sigma = dmk(v[0], v[1]), rho1 = ld_c(st, DK_SC + 2);
+	if (dabs(sigma) / dabs(rho1) < 1e-10) { st[DK_STATUS] = -1; st[DK_CODE] = 2; return; }
+	const dcx alpha = ddiv(rho1, sigma);
+	st_c(st, DK_SC + 4, alpha);
+	st_c(st, DK_SC + 14, dneg(alpha));
+}
+// Gram matrix of r_0, r_1, r_2 in one pass: rows |r0|^2, |r1|^2, |r2|^2, then Re/Im of r0.r1*, r0.r2*, r1.r2*
+template <typename T>
+__global__ void __launch_bounds__(256) k_dk_gram3(const cx<T> *r0, const cx<T> *r1, const cx<T> *r2, size_t n, long vs,
+	const double *st, double *partials, int ps)
+{
+	__shared__ double red_lds[512];
+	DK_RHS_VEC(st, partials, ps);
+	DK_ACTIVE(st);
+	r0 += voff;
+	r1 += voff;
+	r2 += voff;
+	double z[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+		const cx<T> a = r0[i], b = r1[i], c = r2[i];
+		const double ax = a.x, ay = a.y, bx = b.x, by = b.y, cx_ = c.x, cy = c.y;
+		z[0] += ax * ax + ay * ay;
+		z[1] += bx * bx + by * by;
+		z[2] += cx_ * cx_ + cy * cy;
+		z[3] += ax * bx + ay * by;  // r0 . conj(r1)
+		z[4] += ay * bx - ax * by;
+		z[5] += ax * cx_ + ay * cy; // r0 . conj(r2)
+		z[6] += ay * cx_ - ax * cy;
+		z[7] += bx * cx_ + by * cy; // r1 . conj(r2)
+		z[8] += by * cx_ - bx * cy;
+	}
+	const int nb = gridDim.x;
+	for (int q = 0; q < 3; q++) {
+		block_sum3<T>(z[3 * q], z[3 * q + 1], z[3 * q + 2], red_lds);
+		if (threadIdx.x == 0)
+			for (int j = 0; j < 3; j++) partials[(size_t)(3 * q + j) * nb + blockIdx.x] = z[3 * q + j];
+	}
+}
+// The minimal-residual polynomial of degree 2 over Z = R^H R (src/iterative.c:455-482): y0, omega, ||r||^2
+__global__ void __launch_bounds__(kDkThreads) k_dk_b2_poly(double *st, const double *partials, int nblk, int ps)
+{
+	__shared__ double red_lds[512];
+	__shared__ double v[9];
+	DK_RHS_SCALAR(st, partials, ps);
+	DK_ACTIVE(st);
+	for (int q = 0; q < 3; q++) dk_reduce(partials + (size_t)3 * q * nblk, nblk, 3, v + 3 * q, red_lds);
+	if (threadIdx.x != 0) return;
+	dcx Z[3][3];
+	Z[0][0] = dmk(v[0], 0); Z[1][1] = dmk(v[1], 0); Z[2][2] = dmk(v[2], 0);
+	Z[1][0] = dmk(v[3], v[4]); Z[2][0] = dmk(v[5], v[6]); Z[2][1] = dmk(v[7], v[8]);
+	Z[0][1] = dconj(Z[1][0]); Z[0][2] = dconj(Z[2][0]); Z[1][2] = dconj(Z[2][1]);
+	dcx y0[3] = {dmk(-1, 0), ddiv(Z[1][0], Z[1][1]), dmk(0, 0)};
+	const dcx yl[3] = {dmk(0, 0), ddiv(Z[1][2], Z[1][1]), dmk(-1, 0)};
+	dcx zy0[3], zyl[3], vr = dmk(0, 0);
+	double k0 = 0, kl = 0;
+	for (int i = 0; i < 3; i++) {
+		zy0[i] = dadd(dadd(dmul(Z[i][0], y0[0]), dmul(Z[i][1], y0[1])), dmul(Z[i][2], y0[2]));
+		zyl[i] = dadd(dadd(dmul(Z[i][0], yl[0]), dmul(Z[i][1], yl[1])), dmul(Z[i][2], yl[2]));
+	}
+	for (int i = 0; i < 3; i++) {
+		k0 += dmul(zy0[i], dconj(y0[i])).x;
+		kl += dmul(zyl[i], dconj(yl[i])).x;
+		vr = dadd(vr, dmul(zy0[i], dconj(yl[i])));
+	}
+	k0 = sqrt(k0);
+	kl = sqrt(kl);
+	vr = dscl(vr, 1 / (k0 * kl));
+	const double avr = dabs(vr);
+	const dcx hg = dscl(vr, (avr > 0.7 ? avr : 0.7) / avr);
+	const dcx f = dscl(dneg(hg), k0 / kl);
+	for (int i = 0; i < 3; i++) y0[i] = dadd(y0[i], dmul(f, yl[i]));
+	double nr = 0;
+	for (int i = 0; i < 3; i++)
+		nr += dmul(dadd(dadd(dmul(Z[i][0], y0[0]), dmul(Z[i][1], y0[1])), dmul(Z[i][2], y0[2])), dconj(y0[i])).x;
+	const dcx omega = y0[2];
+	st_c(st, DK_SC + 6, omega);
+	st_c(st, DK_SC + 20, dneg(y0[1]));
+	st_c(st, DK_SC + 22, dneg(y0[2]));
+	st_c(st, DK_SC + 24, y0[1]);
+	st_c(st, DK_SC + 26, y0[2]);
+	st[DK_SC + 16] = nr;
+	st_c(st, DK_SC + 0, dmul(ld_c(st, DK_SC + 0), dneg(omega)));  // rho0 *= -omega
 }
 
 struct dk_state {
@@ -1747,7 +1862,45 @@ static int csym_iter_t(adda_hip_op *op, dk_state *k, int niter, void *q_new, voi
 	hipLaunchKernelGGL(k_dk_qmr_upd<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)pdst, (const cx<T> *)pa,
 		(const cx<T> *)pb, (const cx<T> *)q_new, (cx<T> *)x, (cx<T> *)q_old, (cx<T> *)nullptr, st, g.n, g.vs,
 		op->partials2, g.ps2, QU_VT_SCALE | QU_VT_CONJ);
-	hipLaunchKernelGGL(k_dk_cs_prog, dim3(g.K), dim3(64), 0, op->stream, st, k->hist);
+	hipLaunchKernelGGL(k_dk_prog_slot, dim3(g.K), dim3(64), 0, op->stream, st, k->hist, DK_SC + 16);
+	HIPCHK(hipGetLastError());
+	return 0;
+}
+
+// BCGS2 (src/iterative.c:391-483): two BiCG steps (four matvecs) and the minimal-residual polynomial of degree 2
+// over the Gram matrix of r_0, r_1, r_2. rt is the shadow residual r~_0.
+template <typename T>
+static int bcgs2_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_ready, void *x, void *rt, void *R0,
+	void *R1, void *R2, void *U0, void *U1, void *U2)
+{
+	const DkGeo g = dk_geo(op, k);
+	double *st = k->st;
+	void *R[3] = {R0, R1, R2}, *U[3] = {U0, U1, U2};
+	for (int j = 0; j < 2; j++) {
+		const bool first = niter == 1 && j == 0;
+		dot_dev<T>(op, g, st, R[j], rt, ADDA_DOT_CONJ);
+		hipLaunchKernelGGL(k_dk_b2_s1, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, (int)first,
+			g.ps2);
+		for (int i = 0; i <= j; i++)
+			lc_dev<T>(op, g, U[i], first ? nullptr : U[i], R[i], nullptr, st, DK_SC + 8, DK_SC + 10, DK_SC + 10, 0);
+		if (!(first && matvec_ready))
+			if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)U[j], (cx<T> *)U[j + 1], nullptr, 0)) return -1;
+		dot_dev<T>(op, g, st, U[j + 1], rt, ADDA_DOT_CONJ);
+		hipLaunchKernelGGL(k_dk_b2_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, g.ps2);
+		// x += alpha u_0 and r_0 -= alpha u_1 in one pass; r_1 -= alpha u_2
+		hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)U0,
+			(const cx<T> *)nullptr, (cx<T> *)R0, (const cx<T> *)R0, (const cx<T> *)U1, st, DK_SC + 4, DK_SC + 4,
+			DK_SC + 10, DK_SC + 14, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)nullptr);
+		if (j == 1) lc_dev<T>(op, g, R1, R1, U2, nullptr, st, DK_SC + 10, DK_SC + 14, DK_SC + 10, 0);
+		if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)R[j], (cx<T> *)R[j + 1], nullptr, 0)) return -1;
+	}
+	hipLaunchKernelGGL(k_dk_gram3<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (const cx<T> *)R0, (const cx<T> *)R1,
+		(const cx<T> *)R2, g.n, g.vs, st, op->partials2, 9 * g.nb);
+	hipLaunchKernelGGL(k_dk_b2_poly, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, 9 * g.nb);
+	lc_dev<T>(op, g, U0, U0, U1, U2, st, DK_SC + 10, DK_SC + 20, DK_SC + 22, 0);
+	lc_dev<T>(op, g, x, x, R0, R1, st, DK_SC + 10, DK_SC + 24, DK_SC + 26, 0);
+	lc_dev<T>(op, g, R0, R0, R1, R2, st, DK_SC + 10, DK_SC + 20, DK_SC + 22, 0);
+	hipLaunchKernelGGL(k_dk_prog_slot, dim3(g.K), dim3(64), 0, op->stream, st, k->hist, DK_SC + 16);
 	HIPCHK(hipGetLastError());
 	return 0;
 }
@@ -1853,7 +2006,7 @@ static int reserve(adda_hip_op *op, int K)
 	op->npartials = (size_t)kMaxDir * 6 * (kDirBlocks + 1);
 	if ((size_t)K * per > op->npartials) op->npartials = (size_t)K * per;
 	HIPCHK(hipMalloc(&op->partials, op->npartials * sizeof(double)));
-	HIPCHK(hipMalloc(&op->partials2, (size_t)K * 4 * kRedBlocks * sizeof(double)));
+	HIPCHK(hipMalloc(&op->partials2, (size_t)K * 9 * kRedBlocks * sizeof(double)));
 	HIPCHK(hipMalloc(&op->red, (size_t)4 * K * sizeof(double)));
 	HIPCHK(hipHostMalloc((void **)&op->red_host, (size_t)4 * K * sizeof(double), hipHostMallocDefault));
 	HIPCHK(hipMalloc(&op->coef, (size_t)6 * K * sizeof(double)));
@@ -2794,6 +2947,16 @@ extern "C" int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int
 	dk_state *k = (dk_state *)state;
 	if (prec == ADDA_F64) return csym_iter_t<double>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
 	return csym_iter_t<float>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
+}
+
+extern "C" int adda_hip_dk_bcgs2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,
+	void *rtilda, void *r0, void *r1, void *r2, void *u0, void *u1, void *u2)
+{
+	if (!op || !state) return ERR("null argument");
+	HIPCHK(hipSetDevice(op->dev));
+	dk_state *k = (dk_state *)state;
+	if (prec == ADDA_F64) return bcgs2_iter_t<double>(op, k, niter, matvec_ready, x, rtilda, r0, r1, r2, u0, u1, u2);
+	return bcgs2_iter_t<float>(op, k, niter, matvec_ready, x, rtilda, r0, r1, r2, u0, u1, u2);
 }
 
 extern "C" int adda_hip_dk_bicgcs_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,

@@ -557,7 +557,8 @@ static const char *method_name(int m)
 }
 
 static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int n, int *it, int matvec_ready, void **v,
-	void **vtilda, void *Av, void **p_old, void **p_new, void *x, void *r, void *p, void *v1, void *v2, void *v3)
+	void **vtilda, void *Av, void **p_old, void **p_new, void *x, void *r, void *p, void *v1, void *v2, void *v3,
+	void *v4)
 {
 	for (int c = 0; c < n; c++, (*it)++) {
 		if (method == ADDA_IT_QMR_CS) {
@@ -571,6 +572,10 @@ static int enqueue_chunk(adda_hip_op *op, int prec, int method, void *st, int n,
 		}
 		else if (method == ADDA_IT_BICG_CS) {
 			if (adda_hip_dk_bicgcs_iter(op, prec, st, *it, matvec_ready, x, r, p, Av)) return -1;
+		}
+		else if (method == ADDA_IT_BCGS2) {
+			/* residuals r, vec1, vec3; directions vec2, Avecbuffer, vec4; r~_0 in pvec (as in krylov()) */
+			if (adda_hip_dk_bcgs2_iter(op, prec, st, *it, matvec_ready, x, p, r, v1, v3, v2, Av, v4)) return -1;
 		}
 		else if (method == ADDA_IT_QMR_CS_2) {
 			/* v = vec1, d = vec2, p_k = pvec (src/iterative.c:1050-1053) */
@@ -601,6 +606,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	void *r = adda_hip_vec_alloc_batch(op, prec, K), *Av = adda_hip_vec_alloc_batch(op, prec, K);
 	void *p = adda_hip_vec_alloc_batch(op, prec, K), *v1 = adda_hip_vec_alloc_batch(op, prec, K);
 	void *v2 = adda_hip_vec_alloc_batch(op, prec, K), *v3 = adda_hip_vec_alloc_batch(op, prec, K);
+	void *v4 = method == ADDA_IT_BCGS2 ? adda_hip_vec_alloc_batch(op, prec, K) : NULL;
 	void *st = adda_hip_dk_create_batch(op, K, hl);
 	double *S = calloc((size_t)K * NS, sizeof(double)), *temp = calloc(K, sizeof(double));
 	double *inprodR = calloc(K, sizeof(double));
@@ -609,7 +615,10 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	adda_cplx *t = calloc(K, sizeof(adda_cplx));
 	double *hbuf = NULL;
 #define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
-	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
+	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st || (method == ADDA_IT_BCGS2 && !v4)) {
+		rc = host_err_hip(__FILE__, __LINE__);
+		goto done;
+	}
 	if (!S || !temp || !inprodR || !ready || !ca || !cb || !cc || !t) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
 	for (int k = 0; k < K; k++) {
 		res[k].rc = res[k].niter = res[k].nmv = 0;
@@ -698,6 +707,15 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	}
 	else if (method == ADDA_IT_BICG_CS)
 		for (int k = 0; k < K; k++) S[(size_t)k * NS + ADDA_DK_SC + 22] = 1;
+	else if (method == ADDA_IT_BCGS2) { /* src/iterative.c:384-387: r~_0 = r_0, rho_0 = -1 */
+		SETC(ca, 1); SETC(cb, 0);
+		CKG(adda_hip_lincomb_batch(op, prec, K, st, p, r, ca, NULL, cb, NULL, cb, 0, NULL));
+		for (int k = 0; k < K; k++) {
+			double *sc = S + (size_t)k * NS + ADDA_DK_SC;
+			sc[0] = -1;
+			sc[10] = 1;
+		}
+	}
 	else if (method == ADDA_IT_QMR_CS_2) { /* src/iterative.c:1070-1080 */
 		for (int k = 0; k < K; k++) {
 			double *sc = S + (size_t)k * NS + ADDA_DK_SC;
@@ -747,7 +765,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 				it += DK_CHUNK;
 			}
 			else CKG(enqueue_chunk(op, prec, method, st, chunk, &it, matvec_ready, &v, &vtilda, Av, &p_old, &p_new, x,
-				r, p, v1, v2, v3));
+				r, p, v1, v2, v3, v4));
 			if (use_graph && !graph && !tail && it > 3) {
 				/* record the next full chunk while the GPU runs the one just enqueued (a capture runs nothing); the
 				 * pointer rotation of a chunk is the identity (DK_CHUNK is even), so the recorded chunk is valid for
@@ -755,7 +773,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 				int it0 = it;
 				CKG(adda_hip_capture_begin(op));
 				int erc = enqueue_chunk(op, prec, method, st, DK_CHUNK, &it, matvec_ready, &v, &vtilda, Av, &p_old,
-					&p_new, x, r, p, v1, v2, v3);
+					&p_new, x, r, p, v1, v2, v3, v4);
 				graph = adda_hip_capture_end(op);
 				if (erc || !graph) { rc = host_err_hip(__FILE__, __LINE__); goto done; }
 				it = it0;
@@ -811,6 +829,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			if (method == ADDA_IT_QMR_CS || method == ADDA_IT_BICG_CS || method == ADDA_IT_QMR_CS_2)
 				res[k].nmv = done_it - (done_it > 0 && matvec_ready);
 			else if (method == ADDA_IT_CSYM) res[k].nmv = done_it;
+			else if (method == ADDA_IT_BCGS2) res[k].nmv = 4 * done_it - (done_it > 0 && matvec_ready);
 			else if (method == ADDA_IT_BICGSTAB) res[k].nmv = 2 * done_it - (done_it > 0 && matvec_ready);
 			else res[k].nmv = 2 * done_it;
 			res[k].nmv += init_mv;
@@ -834,8 +853,8 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	}
 done:
 	{
-		void *vs[] = {r, Av, p, v1, v2, v3};
-		for (int i = 0; i < 6; i++)
+		void *vs[] = {r, Av, p, v1, v2, v3, v4};
+		for (int i = 0; i < 7; i++)
 			if (vs[i]) adda_hip_vec_free(op, vs[i]);
 		if (graph) adda_hip_graph_free(op, graph);
 		if (st) adda_hip_dk_free(op, st);
@@ -849,7 +868,8 @@ static int device_method(const adda_solver_opts *opt)
 {
 	return !opt->host_scalars && opt->chp_type == ADDA_CHP_NONE && !opt->chp_load &&
 		(opt->method == ADDA_IT_QMR_CS || opt->method == ADDA_IT_BICGSTAB || opt->method == ADDA_IT_CGNR ||
-		opt->method == ADDA_IT_BICG_CS || opt->method == ADDA_IT_QMR_CS_2 || opt->method == ADDA_IT_CSYM);
+		opt->method == ADDA_IT_BICG_CS || opt->method == ADDA_IT_QMR_CS_2 || opt->method == ADDA_IT_CSYM ||
+		opt->method == ADDA_IT_BCGS2);
 }
 
 /* one Krylov solve of K right-hand sides: device-resident recurrences, or (K = 1) the host form */

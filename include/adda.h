@@ -94,7 +94,7 @@ typedef struct {
 	                        10 s of a device-resident solve, inner solves of the mixed forms included */
 	int host_scalars;    /* 1: scalar recurrences on the host with a synchronising reduction per inner product (the
 	                      * reference's control flow, src/iterative.c); 0: device-resident recurrences (QMR_CS,
-	                      * BiCGStab, CGNR, BiCG_CS, CSYM, QMR_CS_2; BCGS2 always uses the host form) */
+	                      * every method; checkpoints always use the host form) */
 	int max_inner;       /* mixed: iteration cap of one FP32 inner solve before the FP64 correction (0 = no cap) */
 	int graph;           /* device-resident recurrences: record the steady-state chunks of iterations once as a HIP
 	                      * graph and replay it (1), or enqueue every iteration (0) */
@@ -142,7 +142,7 @@ int adda_iterative_solver(adda_hip_op *op, const adda_solver_opts *opt, const vo
  * (adda_hip_vec_alloc_batch, stride adda_hip_batch_stride), the cc_sqrt table of RHS k set by
  * adda_hip_set_cc_sqrt_batch. Every RHS converges on its own (own scalars, tolerance test, status); in mixed
  * precision each runs its own defect correction, and the inner FP32 solves of the RHS still correcting share their
- * launches. Needs the device-resident recurrences (every method but BCGS2, host_scalars 0) when K > 1.
+ * launches. Needs the device-resident recurrences (host_scalars 0, no checkpoints) when K > 1.
  * resid_hist: [K][hist_len] or NULL; st: [K]. Returns the most severe per-RHS code. */
 int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, int K, const void *d_einc, void *d_x,
 	void *d_pvec, double *resid_hist, int hist_len, adda_solve_stats *st);

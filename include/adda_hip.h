@@ -169,15 +169,16 @@ double adda_hip_matvec_bytes(const adda_hip_op *op, int prec, int pass);
 
 /* ---- device-resident Krylov iterations ------------------------------------------------------------------------
  * The FP64 scalar recurrences of QMR_CS (src/iterative.c:898-1032), BiCGStab (640-724), CGNR (730-770), BiCG_CS
- * (549-630), CSYM (775-888) and QMR_CS_2 (1037-1136) run on
- * the GPU: one call enqueues one whole iteration (matvecs, fused vector updates with their reductions, one-block
- * scalar kernels, ProgressReport 282-318) on the operator's stream without synchronising. The state is an array of
+ * (549-630), CSYM (775-888), QMR_CS_2 (1037-1136) and BCGS2 (342-483) run on the GPU: one call enqueues one whole
+ * iteration (matvecs, fused vector updates with their reductions, one-block scalar kernels, ProgressReport 282-318)
+ * on the operator's stream without synchronising. The state is an array of
  * ADDA_DK_NSLOTS doubles in device memory plus a residual history; slot 6 holds the status (0 running, 1 converged,
  * 2 maxiter, -1 breakdown, -2 stagnation), and once it is non-zero every later enqueued kernel returns at once.
  * Slots: 0 inprodR, 1 inprodRp1, 2 epsB, 3 1/||b||^2, 4 niter, 5 counter, 6 status, 7 maxiter, 8 mc, 9 history
  * length, 10 breakdown code, 16.. method scalars (set by the caller at PHASE_INIT, see host/iterative.c). niter is
  * the iteration number the caller is enqueueing (it selects the niter==1 and niter==2 forms); matvec_ready: the
- * first iteration reuses A.b held in Av (QMR_CS, BiCG_CS, QMR_CS_2; BiCGStab: in t; CSYM never does). CSYM and
+ * first iteration reuses A.b held in Av (QMR_CS, BiCG_CS, QMR_CS_2; BiCGStab: in t; BCGS2: in u1; CSYM never
+ * does). CSYM and
  * QMR_CS rename their vectors after every iteration (the caller swaps q_new/q_old, v/vtilda every time and
  * p_new/p_old from the second on). Vectors are in precision prec. */
 enum { ADDA_DK_NSLOTS = 96, ADDA_DK_STATUS = 6, ADDA_DK_NITER = 4, ADDA_DK_SC = 16 };
@@ -197,6 +198,8 @@ int adda_hip_dk_qmr2_iter(adda_hip_op *op, int prec, void *state, int niter, int
 	void *p, void *Av, void *x, void *r);
 int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int niter, void *q_new, void *q_old, void *p_new,
 	void *p_old, void *Av, void *x);
+int adda_hip_dk_bcgs2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x, void *rtilda,
+	void *r0, void *r1, void *r2, void *u0, void *u1, void *u2);
 
 /* Matvec configuration per precision: radix schedules along x, y, z and tile sizes, as a string
  * "x=16.16 y=16.16 z=16.16 LY=4 TC=16 LP=2" (x radices: P1/P5; y: P3; z: P2/P4; LY y-rows (all three components) per

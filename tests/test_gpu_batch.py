@@ -77,7 +77,7 @@ def test_batched_kernels_equal_single(hipdev, prec):
         bp.close()
 
 
-@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr", "bicg", "qmr2", "csym"])
+@pytest.mark.parametrize("method", ["qmr", "bicgstab", "cgnr", "bicg", "qmr2", "csym", "bcgs2"])
 def test_batched_fp64_equals_single_and_reference(hipdev, method):
     """8 RHS (4 orientations x 2 polarizations): x of every RHS equals the single-RHS solve bit for bit, graph replay
     equals eager enqueueing, and Cext/Cabs match the reference's outputs."""

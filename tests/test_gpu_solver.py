@@ -33,7 +33,7 @@ def _problem(adda, geo, m, dpl, mat=None, polrel=0, fp32=False):
 @pytest.mark.parametrize("method,gold,mi", METHODS)
 def test_c1_solvers(hipdev, method, gold, mi, host_scalars):
     """Both forms of the recurrences: on the host (the reference's control flow, one synchronising reduction per
-    inner product) and device-resident (FP64 scalar kernels on the GPU; BCGS2 falls back to the host form)."""
+    inner product) and device-resident (FP64 scalar kernels on the GPU)."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
     try:
         for pol, vec in (("Y", POL_Y), ("X", POL_X)):
@@ -49,8 +49,8 @@ def test_c1_solvers(hipdev, method, gold, mi, host_scalars):
 
 @pytest.mark.parametrize("method,gold,nmv_per_it", [("bcgs2", "c1_bcgs2", 4), ("qmr2", "c1_qmr2", 1)])
 def test_c1_bcgs2_qmr2_history_vs_reference_log(hipdev, method, gold, nmv_per_it):
-    """BCGS2 and QMR_CS_2 (host form): iteration and matvec counts of the reference's log, the first half of each
-    residual history to 1e-6. Later iterations drift apart with the summation order of the reductions and the
+    """BCGS2 and QMR_CS_2 (device-resident form): iteration and matvec counts of the reference's log, the first half
+    of each residual history to 1e-6. Later iterations drift apart with the summation order of the reductions and the
     transforms (BCGS2: about 100x per iteration below 1e-6, to 10x at 1e-9), so only convergence is checked there;
     the converged Cext is checked by test_c1_solvers."""
     pr = _problem(hipdev, "sphere16", 1.5, 15)
@@ -76,9 +76,10 @@ def test_c1_bcgs2_qmr2_history_vs_reference_log(hipdev, method, gold, nmv_per_it
         pr.close()
 
 
-@pytest.mark.parametrize("method,mv_ready", [("bicg", True), ("qmr2", True), ("csym", False)])
-def test_c1_device_form_vs_host_form(hipdev, method, mv_ready):
-    """BiCG_CS, QMR_CS_2 and CSYM: the device-resident recurrences (inner products taken from the partial sums of the
+@pytest.mark.parametrize("method,mv_ready,nmv_per_it", [("bicg", True, 1), ("qmr2", True, 1), ("csym", False, 1),
+                                                        ("bcgs2", True, 4)])
+def test_c1_device_form_vs_host_form(hipdev, method, mv_ready, nmv_per_it):
+    """BiCG_CS, QMR_CS_2, CSYM and BCGS2: the device-resident recurrences (inner products taken from the partial sums of the
     vector updates that produce their operands where the method allows) against the host form (a separate reduction
     per inner product): iteration and matvec counts, the residual histories to 1e-6 over their first half (later the
     two summation orders drift apart, as ADDA's own runs do across processor counts), the converged fields to 1e-8,
@@ -95,12 +96,13 @@ def test_c1_device_form_vs_host_form(hipdev, method, mv_ready):
             assert abs(rd["niter"] - rh["niter"]) <= 1 and rd["hist"][-1] < 1e-10
             # x_0 = 0: the initial field's matvec serves iteration 1 where the method reuses it
             extra = 0 if mv_ready else 1
-            assert rd["nmatvec"] == rd["niter"] + extra and rh["nmatvec"] == rh["niter"] + extra
+            assert rd["nmatvec"] == nmv_per_it * rd["niter"] + extra
+            assert rh["nmatvec"] == nmv_per_it * rh["niter"] + extra
             k = rd["niter"] // 2
             assert np.allclose(rd["hist"][:k], rh["hist"][:k], rtol=1e-6)
             assert np.abs(xd - xh).max() / np.abs(xh).max() < 1e-8
             ri = pr.solve(method, eps=1e-10, host_scalars=False, init_field=hipdev.IF_INC)
-            assert ri["rc"] == 0 and ri["nmatvec"] == ri["niter"] + 1
+            assert ri["rc"] == 0 and ri["nmatvec"] == nmv_per_it * ri["niter"] + 1
             assert np.abs(pr.field_x() - xh).max() / np.abs(xh).max() < 1e-8
     finally:
         pr.close()

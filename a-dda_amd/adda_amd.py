@@ -81,6 +81,7 @@ def hip():
     L.adda_hip_get_config.argtypes = [P, I, ctypes.c_char_p, S]
     L.adda_hip_set_config.argtypes = [P, I, ctypes.c_char_p]
     L.adda_hip_set_matvec_mode.argtypes = [P, I]
+    L.adda_hip_set_launch_list.argtypes = [P, I, P]
     L.adda_hip_autotune.argtypes = [P, I, I, ctypes.c_char_p, ctypes.c_char_p, S]
     L.adda_hip_dk_create.argtypes = [P, I]
     L.adda_hip_dk_create.restype = P
@@ -270,6 +271,14 @@ class Operator:
     def set_matvec_mode(self, mode):
         """1: FP64 matvecs run their transforms in FP32 (FP64 vectors in and out); 0: in FP64."""
         check(hip().adda_hip_set_matvec_mode(self.h, mode))
+
+    def set_launch_list(self, K, run=None):
+        """Batched matvecs of K RHS launch only the RHS k with run[k] true (None clears the list)."""
+        if run is None:
+            check(hip().adda_hip_set_launch_list(self.h, K, None))
+            return
+        r = np.ascontiguousarray(run, dtype=np.int32)
+        check(hip().adda_hip_set_launch_list(self.h, K, r.ctypes.data_as(ctypes.c_void_p)))
 
     def autotune(self, prec=F64, reps=3, cache=None):
         """Pick the fastest radix schedules and tiles per pass (HIP-event timing); returns the report string."""

@@ -185,6 +185,9 @@ struct adda_hip_op {
 	void *Xb = nullptr;  // [kcap][gZ][gX][3][bY]
 	int kcap = 0;        // right-hand sides the buffers and tables hold
 	size_t vs = 0;       // stride between the right-hand sides of a batched device vector (complex elements)
+	// launch list of the batched matvecs of rl_K right-hand sides (adda_hip_set_launch_list): bit k set = RHS k runs
+	unsigned long long rl_mask = 0;
+	int rl_K = 0;
 	double *coef = nullptr;       // [kcap][6] per-RHS coefficients of batched vector updates
 	FftPlan px, py, pz;      // plans of the D-matrix construction kernels
 	MvCfg cfg[2];            // matvec plans and tiles per precision (index ADDA_F64 / ADDA_F32)
@@ -782,8 +785,14 @@ static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, cons
 	b.xs2 = xs2;
 	b.cs = 3 * op->nmat;
 	b.gate = gate;
+	if (op->rl_mask && K == op->rl_K) {
+		b.mask = op->rl_mask;
+		b.nl = __builtin_popcountll(op->rl_mask);
+	}
 	return b;
 }
+// launch slots per tile: the listed RHS, or all K
+static int bq_slots(const Bq &b) { return b.mask ? b.nl : b.K; }
 
 // TV: the dipole-vector type (double with an FP32 transform: the MV_P1W / MV_P5W kernels)
 template <typename T, typename TV>
@@ -796,9 +805,10 @@ static void launch_p1(adda_hip_op *op, const MvCfg &c, int K, long vs, const dou
 	auto k = (Fn)((c.xn & 1) ? c.kx.nt[kind][f] : c.kx.k[kind][f]);
 	// tiles 2p and 2p + 1 (whose Xa runs share cache lines where they meet) go 8 blocks apart, on one XCD, as in P5
 	const int pair = nyt % 2 == 0, nblk1 = pair ? (op->bZ * nyt + 15) / 16 * 16 : op->bZ * nyt;
-	hipLaunchKernelGGL(k, dim3(nblk1 * K), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
+	const Bq bq = make_bq(op, K, vs, 0, xa_stride(op), gate, pair, op->bZ * nyt);
+	hipLaunchKernelGGL(k, dim3(nblk1 * bq_slots(bq)), dim3(3 * c.LY * (op->gX / c.qx.rmax)),
 		lds_p15(c, op->gX, sizeof(cx<T>)), op->stream, arg, (cx<T> *)op->Xa, op->cellmap, op->rows, op->mat, ccs, c.qx,
-		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, make_bq(op, K, vs, 0, xa_stride(op), gate, pair, op->bZ * nyt));
+		tw_of<T>(op->twx), op->bX, op->bY, c.LY, her, bq);
 }
 
 template <typename T, bool INV>
@@ -813,9 +823,10 @@ static void launch_zfft(adda_hip_op *op, const MvCfg &c, int K, const double *ga
 	const long ntile = (Q + c.TC - 1) / c.TC;
 	const int pair = c.TC * (int)sizeof(cx<T>) < 128 && ntile % 2 == 0;
 	const long nblk = pair ? (ntile + 15) / 16 * 16 : ntile;
-	hipLaunchKernelGGL(k, dim3((unsigned)(nblk * K)), dim3(c.TC * (op->gZ / c.qz.rmax)),
+	const Bq bq = make_bq(op, K, 0, xin, xout, gate, pair, (int)ntile);
+	hipLaunchKernelGGL(k, dim3((unsigned)(nblk * bq_slots(bq))), dim3(c.TC * (op->gZ / c.qz.rmax)),
 		lds_p24(c, op->gZ, sizeof(cx<T>)), op->stream, in, out, c.qz, tw_of<T>(op->twz), Q, c.TC,
-		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, make_bq(op, K, 0, xin, xout, gate, pair, (int)ntile));
+		INV ? op->gZ : op->bZ, INV ? op->bZ : op->gZ, bq);
 }
 
 template <typename T> using P3SFn = decltype(&k_p3sub<T, Dyn<8>>);
@@ -829,10 +840,11 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 		if constexpr (sizeof(T) == 8) R = op->R64;
 		else R = op->R32;
 		const int ntile = (p3_units(op->gX, op->DsX, op->reduced, c.p3x) + c.LP - 1) / c.LP;
-		hipLaunchKernelGGL((P3SFn<T>)c.ky.k[MV_P3S][f], dim3(nz3 * ntile * K), dim3(p3_threads(c, op->gY, op->reduced)),
-			lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, R, c.qy, tw_of<T>(op->twy),
-			op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed,
-			make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate));
+		const Bq bq = make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate);
+		hipLaunchKernelGGL((P3SFn<T>)c.ky.k[MV_P3S][f], dim3(nz3 * ntile * bq_slots(bq)),
+			dim3(p3_threads(c, op->gY, op->reduced)), lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream,
+			(cx<T> *)op->Xb, D, R, c.qy, tw_of<T>(op->twy), op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP,
+			op->reduced, c.p3x, transposed, bq);
 		return;
 	}
 	const int kl = c.p3r ? 1 : std::min(c.p3k, K);  // groups of kl consecutive RHS per block
@@ -855,9 +867,9 @@ static void launch_p3(adda_hip_op *op, const MvCfg &c, int K, const double *gate
 	// groups of 8 pairs
 	const bool mpair = !c.p3r && !c.p3x && op->reduced && c.LP == 1 && op->gX % 2 == 0;
 	const int npairs = nz3 * (op->gX / 2), nblk = mpair ? (npairs + 7) / 8 * 16 : nz3 * ntile;
-	const int nkg = (K + kl - 1) / kl;
 	Bq bq = make_bq(op, K, 0, xb_stride(op), xb_stride(op), gate, mpair, npairs);
 	bq.kl = kl;
+	const int nkg = (bq_slots(bq) + kl - 1) / kl;
 	hipLaunchKernelGGL(k, dim3(nblk * nkg), dim3(nthr),
 		lds_p3(c, op->gY, op->reduced, sizeof(cx<T>)), op->stream, (cx<T> *)op->Xb, D, c.qy, tw,
 		op->gX, op->bY, op->gZ, op->DsX, op->DsY, op->DsZ, c.LP, op->reduced, c.p3x, transposed, bq);
@@ -880,9 +892,10 @@ static int launch_p5(adda_hip_op *op, const MvCfg &c, int K, long vs, const doub
 	const int kind = dmode == 2 ? (same ? MV_P5D2 : MV_P5DW2) : dmode ? (same ? MV_P5D : MV_P5DW) : (same ? MV_P5 : MV_P5W);
 	const int f = sizeof(T) == 4;
 	auto k = (Fn)((c.xn & 2) ? c.kx.nt[kind][f] : c.kx.k[kind][f]);
-	hipLaunchKernelGGL(k, dim3(nblk5 * K), dim3(3 * cd.LY * (op->gX / c.qx.rmax)), lds_p15(cd, op->gX, sizeof(cx<T>)),
-		op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat, ccs, c.qx, tw_of<T>(op->twx), op->bX,
-		op->bY, cd.LY, her, partials, dotw, dmode, make_bq(op, K, vs, xa_stride(op), 0, gate, pair, op->bZ * nyt));
+	const Bq bq = make_bq(op, K, vs, xa_stride(op), 0, gate, pair, op->bZ * nyt);
+	hipLaunchKernelGGL(k, dim3(nblk5 * bq_slots(bq)), dim3(3 * cd.LY * (op->gX / c.qx.rmax)),
+		lds_p15(cd, op->gX, sizeof(cx<T>)), op->stream, (const cx<T> *)op->Xa, arg, res, op->cellmap, op->rows, op->mat,
+		ccs, c.qx, tw_of<T>(op->twx), op->bX, op->bY, cd.LY, her, partials, dotw, dmode, bq);
 	return nblk5;
 }
 
@@ -2947,6 +2960,21 @@ extern "C" int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int
 	dk_state *k = (dk_state *)state;
 	if (prec == ADDA_F64) return csym_iter_t<double>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
 	return csym_iter_t<float>(op, k, niter, q_new, q_old, p_new, p_old, Av, x);
+}
+
+extern "C" int adda_hip_set_launch_list(adda_hip_op *op, int K, const int *run)
+{
+	if (!op) return ERR("null argument");
+	op->rl_mask = 0;
+	op->rl_K = 0;
+	if (!run) return 0;
+	if (K < 2 || K > 64) return ERR("launch lists take 2 to 64 right-hand sides, not %d", K);
+	unsigned long long m = 0;
+	for (int k = 0; k < K; k++)
+		if (run[k]) m |= 1ull << k;
+	op->rl_mask = m;
+	op->rl_K = m ? K : 0;
+	return 0;
 }
 
 extern "C" int adda_hip_dk_bcgs2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x,

@@ -614,12 +614,19 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 	adda_cplx *ca = calloc(K, sizeof(adda_cplx)), *cb = calloc(K, sizeof(adda_cplx)), *cc = calloc(K, sizeof(adda_cplx));
 	adda_cplx *t = calloc(K, sizeof(adda_cplx));
 	double *hbuf = NULL;
+	/* launch list of the matvecs (K <= 64): the RHS still running at the last poll */
+	const int use_list = K > 1 && K <= 64;
+	int *run = calloc(K, sizeof(int)), list_set = 0;
+	unsigned long long list_mask = 0;
 #define CKG(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto done; } } while (0)
 	if (!r || !Av || !p || !v1 || !v2 || !v3 || !st || (method == ADDA_IT_BCGS2 && !v4)) {
 		rc = host_err_hip(__FILE__, __LINE__);
 		goto done;
 	}
-	if (!S || !temp || !inprodR || !ready || !ca || !cb || !cc || !t) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
+	if (!S || !temp || !inprodR || !ready || !ca || !cb || !cc || !t || !run) {
+		rc = host_err(__FILE__, __LINE__, "out of host memory");
+		goto done;
+	}
 	for (int k = 0; k < K; k++) {
 		res[k].rc = res[k].niter = res[k].nmv = 0;
 		res[k].rel = 0;
@@ -757,6 +764,23 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 		}
 		struct timespec tp0, tp1;
 		clock_gettime(CLOCK_MONOTONIC, &tp0);
+		/* the list follows the statuses of the last poll (RHS that stop inside a chunk stay gated until its end); a
+		 * recorded graph holds the list it was recorded with, so a new list drops it (it is recorded again) */
+#define UPDATE_LIST() do { \
+		if (use_list) { \
+			unsigned long long m_ = 0; \
+			for (int k_ = 0; k_ < K; k_++) { \
+				run[k_] = S[(size_t)k_ * NS + ADDA_DK_STATUS] == 0; \
+				if (run[k_]) m_ |= 1ull << k_; \
+			} \
+			if (m_ && m_ != list_mask) { \
+				CKG(adda_hip_set_launch_list(op, K, run)); \
+				list_set = 1; \
+				list_mask = m_; \
+				if (graph) { adda_hip_graph_free(op, graph); graph = NULL; } \
+			} \
+		} } while (0)
+		UPDATE_LIST();
 		while (running) {
 			/* full chunks replay the recorded graph; the shorter chunks near convergence (tail) are enqueued directly,
 			 * and the graph -- recorded for one pointer rotation of QMR_CS -- is not replayed after them */
@@ -780,6 +804,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 			}
 			const int ran = chunk;
 			CKG(adda_hip_dk_get(op, st, 0, S, K * NS));
+			UPDATE_LIST();
 			running = 0;
 			/* iterations the slowest running RHS still needs at the decay rate of its best residual over this chunk */
 			double need = 0;
@@ -812,6 +837,7 @@ static int krylov_dev_batch(adda_hip_op *op, int prec, int K, int method, const 
 				}
 			}
 		}
+#undef UPDATE_LIST
 		if (hist && hist_len > 1) {
 			hbuf = malloc((size_t)K * hl * sizeof(double));
 			if (!hbuf) { rc = host_err(__FILE__, __LINE__, "out of host memory"); goto done; }
@@ -857,8 +883,9 @@ done:
 		for (int i = 0; i < 7; i++)
 			if (vs[i]) adda_hip_vec_free(op, vs[i]);
 		if (graph) adda_hip_graph_free(op, graph);
+		if (list_set) adda_hip_set_launch_list(op, K, NULL);
 		if (st) adda_hip_dk_free(op, st);
-		free(S); free(temp); free(inprodR); free(ready); free(ca); free(cb); free(cc); free(t); free(hbuf);
+		free(S); free(temp); free(inprodR); free(ready); free(ca); free(cb); free(cc); free(t); free(hbuf); free(run);
 	}
 	return rc;
 #undef CKG

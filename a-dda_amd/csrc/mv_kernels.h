@@ -21,7 +21,24 @@ struct Bq {
 	int pair = 0;        // P5: launch-order remap of the tiles (pair_tile), the launch padded to whole groups of 16
 	int ntile = 0;       // P5 with pair: the number of real tiles (the padding blocks write zero partial sums)
 	int kl = 1;          // P3: right-hand sides per block (consecutive RHS sharing the block's D rows and twiddles)
+	// launch list (K <= 64): only the RHS whose bit is set get blocks, nl = popcount(mask) launch slots per tile; 0 =
+	// every RHS has its slot (gated ones exit at once, which still costs their dispatch)
+	unsigned long long mask = 0;
+	int nl = 0;
 };
+// The launch slots per tile and the RHS of slot s: the s-th set bit of the mask (a uniform binary search on popcounts)
+__device__ __forceinline__ int bq_nk(const Bq &b) { return b.mask ? b.nl : b.K; }
+__device__ __forceinline__ int bq_rhs(const Bq &b, int s)
+{
+	if (!b.mask) return s;
+	unsigned long long m = b.mask;
+	int pos = 0;
+	for (int w = 32; w > 0; w >>= 1) {
+		const int c = __popcll(m & ((1ull << w) - 1));
+		if (s >= c) { s -= c; m >>= w; pos += w; }
+	}
+	return pos;
+}
 // Tile of the b-th block of a launch whose tiles 2p and 2p + 1 sit 8 blocks apart (groups of 16 blocks: the even
 // member of 8 pairs, then the odd ones). Blocks are dealt to the 8 XCDs in turn, so a pair shares one XCD and its L2.
 // A bijection of [0, 16*ceil(n/16)); tiles >= n are padding.
@@ -123,7 +140,8 @@ __global__ void __launch_bounds__(512) k_p1(const cx<TV> *__restrict__ arg, cx<T
 	const int *__restrict__ cellmap, const RowDesc *__restrict__, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk);
+	const int tile = bq.pair ? pair_tile(blockIdx.x / nk) : blockIdx.x / nk;
 	if (skip_rhs(bq, kq) || (bq.pair && tile >= bq.ntile)) return;  // pair: as in P5 (the Xa runs of tiles 2p, 2p + 1
 	                                                                  // share cache lines)
 	arg += kq * bq.vs;
@@ -197,7 +215,8 @@ __global__ void __launch_bounds__(512) k_p1s(const cx<TV> *__restrict__ arg, cx<
 	const int *__restrict__ cellmap, const RowDesc *__restrict__ rows, const int *__restrict__ mat,
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk);
+	const int tile = bq.pair ? pair_tile(blockIdx.x / nk) : blockIdx.x / nk;
 	if (skip_rhs(bq, kq) || (bq.pair && tile >= bq.ntile)) return;  // pair: as in P5 (the Xa runs of tiles 2p, 2p + 1
 	                                                                  // share cache lines)
 	arg += kq * bq.vs;
@@ -271,11 +290,11 @@ template <typename T, bool INV, class PL>
 __global__ void __launch_bounds__(512) k_zfft(const cx<T> *__restrict__ in, cx<T> *__restrict__ out, FftPlan2 p,
 	const cx<T> *__restrict__ twg, long Q, int TC, int n_in, int n_out, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk);
 	if (skip_rhs(bq, kq)) return;
 	// pair: tiles narrower than a cache line -- tiles 2p and 2p + 1 share lines -- go 8 blocks apart, on one XCD
 	// (pair_tile; the launch is padded to whole groups of 16 tiles)
-	const int tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	const int tile = bq.pair ? pair_tile(blockIdx.x / nk) : blockIdx.x / nk;
 	if (bq.pair && tile >= bq.ntile) return;
 	in += kq * bq.xs;
 	out += kq * bq.xs2;
@@ -478,8 +497,9 @@ __device__ __forceinline__ void p3_body(cx<T> *buf, const cx<T> *tws, const FftP
 	// the block, so every thread meets the same barriers. The D rows are loaded again for each RHS (from the cache after
 	// the first): held across the loop they would stay live through the inverse transform, at the cost of occupancy.
 	for (int kk = 0; kk < (MULTI ? bq.kl : 1); kk++) {
-		const int kq = k0 + kk;
-		if (kq >= bq.K || skip_rhs(bq, kq)) continue;
+		if (k0 + kk >= bq_nk(bq)) continue;
+		const int kq = bq_rhs(bq, k0 + kk);
+		if (skip_rhs(bq, kq)) continue;
 		if (kk > 0) __syncthreads();  // the previous RHS's last LDS reads precede this forward transform's LDS writes
 		cx<T> dpre[KP > 0 ? KP : 1][6];
 		unsigned mpre[KP > 0 ? KP : 1];
@@ -540,10 +560,12 @@ __device__ __forceinline__ void p3_kernel(cx<T> *Xb, const cx<T> *__restrict__ D
 {
 	const int LP = LPC > 0 ? LPC : LPr;
 	// blocks of kl consecutive RHS (the group index fastest, so the groups that share a D tile run together)
-	const int kl = MULTI ? bq.kl : 1, nkg = (bq.K + kl - 1) / kl, k0 = (blockIdx.x % nkg) * kl, tile = blockIdx.x / nkg;
+	// (k0: the first launch slot of the block's group; its RHS through the launch list)
+	const int nk = bq_nk(bq), kl = MULTI ? bq.kl : 1, nkg = (nk + kl - 1) / kl, k0 = (blockIdx.x % nkg) * kl;
+	const int tile = blockIdx.x / nkg;
 	{
 		bool any = false;
-		for (int kk = 0; kk < kl && k0 + kk < bq.K; kk++) any |= !skip_rhs(bq, k0 + kk);
+		for (int kk = 0; kk < kl && k0 + kk < nk; kk++) any |= !skip_rhs(bq, bq_rhs(bq, k0 + kk));
 		if (!any) return;
 	}
 	extern __shared__ char smem_raw[];
@@ -669,7 +691,7 @@ __global__ void __launch_bounds__(256) k_p3r(cx<T> *Xb, const cx<T> *__restrict_
 	int transposed, Bq bq)
 {
 	constexpr int N = PL::N, TPL = PL::TPL, RMAX = PL::RMAX, RL = PL::RAD[PL::NST - 1], NSL = PL::NS_LAST;
-	const int kq = blockIdx.x % bq.K, tile = blockIdx.x / bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk), tile = blockIdx.x / nk;
 	if (skip_rhs(bq, kq)) return;
 	Xb += kq * bq.xs;
 	extern __shared__ char smem_raw[];
@@ -731,12 +753,13 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	const cx<T> *__restrict__ ccs, FftPlan2 p, const cx<T> *__restrict__ twg, int bX, int bY, int LY, int her,
 	double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int dmode, Bq bq)
 {
-	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk);
+	const int tile = bq.pair ? pair_tile(blockIdx.x / nk) : blockIdx.x / nk;
 	if (skip_rhs(bq, kq)) return;
 	if (bq.pair && tile >= bq.ntile) {  // padding of the paired launch: zero partial sums, nothing else
 		if (partials && threadIdx.x == 0) {
 			const int nr = DOT ? 3 : 1;
-			const size_t nb = gridDim.x / bq.K;
+			const size_t nb = gridDim.x / nk;
 			for (int r = 0; r < nr; r++) partials[(size_t)kq * nr * nb + r * nb + tile] = 0;
 		}
 		return;
@@ -848,7 +871,7 @@ __global__ void __launch_bounds__(512) k_p5(const cx<T> *__restrict__ Xa, const 
 	}
 	if (partials) {
 		const int rows = DOT ? 3 : 1;
-		const size_t nb = gridDim.x / bq.K, o = (size_t)kq * rows * nb + tile;
+		const size_t nb = gridDim.x / nk, o = (size_t)kq * rows * nb + tile;
 		if (DOT) {
 			block_sum3<T>(s, dr, di, red_lds);
 			if (threadIdx.x == 0) {
@@ -882,12 +905,13 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 	int bY, int LY, int her, double *__restrict__ partials, const cx<TV> *__restrict__ dotw, int, Bq bq)
 {
 	constexpr bool DOT = DM != 0;
-	const int kq = blockIdx.x % bq.K, tile = bq.pair ? pair_tile(blockIdx.x / bq.K) : blockIdx.x / bq.K;
+	const int nk = bq_nk(bq), kq = bq_rhs(bq, blockIdx.x % nk);
+	const int tile = bq.pair ? pair_tile(blockIdx.x / nk) : blockIdx.x / nk;
 	if (skip_rhs(bq, kq)) return;
 	if (bq.pair && tile >= bq.ntile) {  // padding of the paired launch: zero partial sums, nothing else
 		if (partials && threadIdx.x == 0) {
 			const int nr = DOT ? 3 : 1;
-			const size_t nb = gridDim.x / bq.K;
+			const size_t nb = gridDim.x / nk;
 			for (int r = 0; r < nr; r++) partials[(size_t)kq * nr * nb + r * nb + tile] = 0;
 		}
 		return;
@@ -985,7 +1009,7 @@ __global__ void __launch_bounds__(512) k_p5s(const cx<T> *__restrict__ Xa, const
 	}
 	if (partials) {  // block_sum starts with a barrier: the transform's last LDS reads are done before red_lds is reused
 		const int nr = DOT ? 3 : 1;
-		const size_t nb = gridDim.x / bq.K, o = (size_t)kq * nr * nb + tile;
+		const size_t nb = gridDim.x / nk, o = (size_t)kq * nr * nb + tile;
 		if (DOT) {
 			block_sum3<T>(s, dr, di, red_lds);
 			if (threadIdx.x == 0) {

@@ -198,6 +198,12 @@ int adda_hip_dk_qmr2_iter(adda_hip_op *op, int prec, void *state, int niter, int
 	void *p, void *Av, void *x, void *r);
 int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int niter, void *q_new, void *q_old, void *p_new,
 	void *p_old, void *Av, void *x);
+/* Launch list of the batched matvecs of K right-hand sides (2 <= K <= 64), no counterpart in the reference (its
+ * solves are one RHS at a time): from now on those matvecs launch blocks only for the RHS k with run[k] != 0 and
+ * neither read nor write the others, as if their gate status were non-zero (a gate still applies to the listed ones).
+ * A gated RHS costs the dispatch of its blocks even though they exit at once; a solver that knows which RHS are still
+ * running (krylov_dev_batch after each chunk) saves that. run == NULL, or no entry set, clears the list. */
+int adda_hip_set_launch_list(adda_hip_op *op, int K, const int *run);
 int adda_hip_dk_bcgs2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x, void *rtilda,
 	void *r0, void *r1, void *r2, void *u0, void *u1, void *u2);
 

@@ -189,6 +189,54 @@ def test_batch_of_one_and_gating(hipdev):
         bp.close()
 
 
+def test_launch_list(hipdev):
+    """Batched matvecs under a launch list compute the listed RHS exactly as the full launch does (every P3 form the
+    tuner picks between: one RHS per block and KL=2), leave the others untouched, and a gate still applies to the
+    listed ones; clearing the list restores the full launch."""
+    import ctypes
+    bp, _ = _batch(hipdev, 3)
+    op = bp.op
+    K = bp.K
+    try:
+        a, r = op.alloc_batch(K), op.alloc_batch(K)
+        rng = np.random.default_rng(5)
+        for k in range(K):
+            op.upload(op.rhs(a, k), rng.standard_normal(3 * bp.nd) + 1j * rng.standard_normal(3 * bp.nd))
+        op.matvec_batch(a, r, K)
+        full = [op.download(op.rhs(r, k)) for k in range(K)]
+        sentinel = np.full(3 * bp.nd, 7 - 3j)
+        run = np.zeros(K, dtype=np.int32)
+        run[[0, 3, 4]] = 1
+        st = hipdev.hip().adda_hip_dk_create_batch(op.h, K, 2)
+        assert st
+        vals = np.array([1.0])
+        hipdev.check(hipdev.hip().adda_hip_dk_set(op.h, st, 3 * 96 + 6, vals.ctypes.data_as(ctypes.c_void_p), 1))
+        base = op.get_config(hipdev.F64)
+        try:
+            for cfg in (base, base + " KL=2"):
+                op.set_config(cfg, hipdev.F64)
+                for gate in (None, st):
+                    for k in range(K):
+                        op.upload(op.rhs(r, k), sentinel)
+                    op.set_launch_list(K, run)
+                    op.matvec_batch(a, r, K, gate=gate)
+                    op.set_launch_list(K, None)
+                    for k in range(K):
+                        got = op.download(op.rhs(r, k))
+                        listed = run[k] and not (gate is not None and k == 3)
+                        assert np.array_equal(got, full[k] if listed else sentinel), (cfg, gate is not None, k)
+            op.set_config(base, hipdev.F64)
+            op.matvec_batch(a, r, K)
+            for k in range(K):
+                assert np.array_equal(op.download(op.rhs(r, k)), full[k]), k
+        finally:
+            hipdev.check(hipdev.hip().adda_hip_dk_free(op.h, st))
+            op.free(a)
+            op.free(r)
+    finally:
+        bp.close()
+
+
 def test_batched_postprocessing(hipdev):
     """Cext, Cabs and the scattering-plane fields of every RHS from one batched evaluation equal the single-RHS
     functions bit for bit; for the orientation the reference ran (30, 40, 50; m = 1.33+0.05i, dpl 12) the batched

@@ -940,11 +940,11 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	double *r2new = malloc(K * sizeof(double)), *ih = NULL, *og_h = NULL;
 	kres *kr = malloc(K * sizeof(kres));
 	adda_cplx *ca = malloc(K * sizeof(adda_cplx)), *cb = malloc(K * sizeof(adda_cplx)), *cz = calloc(K, sizeof(adda_cplx));
-	int *active = malloc(K * sizeof(int)), *hist_pos = calloc(K, sizeof(int));
+	int *active = malloc(K * sizeof(int)), *hist_pos = calloc(K, sizeof(int)), *sel_run = calloc(K, sizeof(int));
 	void *r64 = NULL, *Ax = NULL, *r32 = NULL, *e32 = NULL, *e64 = NULL, *xt = NULL, *og = NULL;
 	int *fin = NULL, *acc = NULL;
 #define CKB(x) do { if ((x) != 0) { rc = host_err_hip(__FILE__, __LINE__); goto out; } } while (0)
-	if (!eps || !b2 || !r2 || !r2new || !kr || !ca || !cb || !cz || !active || !hist_pos) {
+	if (!eps || !b2 || !r2 || !r2new || !kr || !ca || !cb || !cz || !active || !hist_pos || !sel_run) {
 		rc = host_err(__FILE__, __LINE__, "out of host memory");
 		goto out;
 	}
@@ -1042,7 +1042,12 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		GATE(sel); \
 		for (int k_ = 0; k_ < K; k_++) { ca[k_].re = 1; cb[k_].re = sqrt(r2[k_]); } \
 		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, xt, d_x, ca, e64, cb, NULL, cz, 0, NULL)); \
+		if (K > 1 && K <= 64) { /* the FP64 matvec launches only the selected RHS */ \
+			for (int k_ = 0; k_ < K; k_++) sel_run[k_] = (sel) ? 1 : 0; \
+			CKB(adda_hip_set_launch_list(op, K, sel_run)); \
+		} \
 		CKB(adda_hip_matvec_batch(op, ADDA_F64, K, og, xt, Ax, NULL, 0)); \
+		if (K > 1 && K <= 64) CKB(adda_hip_set_launch_list(op, K, NULL)); \
 		for (int k_ = 0; k_ < K; k_++) cb[k_].re = -1; \
 		CKB(adda_hip_lincomb_batch(op, ADDA_F64, K, og, e64, d_pvec, ca, Ax, cb, NULL, cz, 0, r2new)); } while (0)
 		/* x = x', r = r' for the RHS where acc */
@@ -1170,6 +1175,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 	CKB(adda_hip_sync(op));
 out:
 	if (opt->precision == ADDA_PREC_MIXED_V64) adda_hip_set_matvec_mode(op, 0); /* also after a failed inner solve */
+	if (K > 1 && K <= 64) adda_hip_set_launch_list(op, K, NULL);                  /* also after a failed step */
 	{
 		void *vs[] = {r64, Ax, r32, e32, e64, xt};
 		for (int i = 0; i < 6; i++)
@@ -1177,7 +1183,7 @@ out:
 		if (og) adda_hip_dk_free(op, og);
 	}
 	free(eps); free(b2); free(r2); free(r2new); free(ih); free(og_h); free(kr); free(ca); free(cb); free(cz);
-	free(active); free(hist_pos); free(fin); free(acc);
+	free(active); free(hist_pos); free(fin); free(acc); free(sel_run);
 	return rc;
 #undef CKB
 }

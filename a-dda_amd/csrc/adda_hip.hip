@@ -793,6 +793,14 @@ static Bq make_bq(const adda_hip_op *op, int K, long vs, long xs, long xs2, cons
 }
 // launch slots per tile: the listed RHS, or all K
 static int bq_slots(const Bq &b) { return b.mask ? b.nl : b.K; }
+// Suspends the operator's launch list for a scope (the measurements time full launches whatever a caller listed)
+struct ListOff {
+	adda_hip_op *op;
+	unsigned long long m;
+	int k;
+	explicit ListOff(adda_hip_op *o) : op(o), m(o->rl_mask), k(o->rl_K) { o->rl_mask = 0; o->rl_K = 0; }
+	~ListOff() { op->rl_mask = m; op->rl_K = k; }
+};
 
 // TV: the dipole-vector type (double with an FP32 transform: the MV_P1W / MV_P5W kernels)
 template <typename T, typename TV>
@@ -2785,6 +2793,7 @@ extern "C" int adda_hip_profile_matvec_batch(adda_hip_op *op, int prec, int K, c
 	if (!op || !argvec || !resultvec || nrep <= 0 || !pass_ms || !total_ms) return ERR("bad argument");
 	if (K < 1 || K > op->kcap) return ERR("batch of %d right-hand sides exceeds the reserved %d", K, op->kcap);
 	HIPCHK(hipSetDevice(op->dev));
+	ListOff lo(op);
 	const long vs = K > 1 ? (long)op->vs : 0;
 	hipEvent_t ev[6];
 	for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
@@ -3364,6 +3373,7 @@ extern "C" int adda_hip_autotune_batch(adda_hip_op *op, int prec, int K, int rep
 	if (!op) return ERR("null operator");
 	if (K < 1) return ERR("batch of %d right-hand sides", K);
 	HIPCHK(hipSetDevice(op->dev));
+	ListOff lo(op);
 	if (reps < 1) reps = 3;
 	char key[200];
 	int kl = snprintf(key, sizeof key, "grid %dx%dx%d box %dx%dx%d dmatrix %s prec %s%s", op->gX, op->gY, op->gZ,

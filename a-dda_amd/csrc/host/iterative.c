@@ -936,6 +936,7 @@ int adda_iterative_solver_batch(adda_hip_op *op, const adda_solver_opts *opt, in
 		return host_err(__FILE__, __LINE__, "batched solves need the device-resident recurrences (QMR_CS, BiCGStab, CGNR)");
 	int rc = 0;
 	const int NS = ADDA_DK_NSLOTS;
+	if (K > 1 && K <= 64 && adda_hip_set_launch_list(op, K, NULL)) return host_err_hip(__FILE__, __LINE__);
 	double *eps = malloc(K * sizeof(double)), *b2 = malloc(K * sizeof(double)), *r2 = malloc(K * sizeof(double));
 	double *r2new = malloc(K * sizeof(double)), *ih = NULL, *og_h = NULL;
 	kres *kr = malloc(K * sizeof(kres));

@@ -202,7 +202,9 @@ int adda_hip_dk_csym_iter(adda_hip_op *op, int prec, void *state, int niter, voi
  * solves are one RHS at a time): from now on those matvecs launch blocks only for the RHS k with run[k] != 0 and
  * neither read nor write the others, as if their gate status were non-zero (a gate still applies to the listed ones).
  * A gated RHS costs the dispatch of its blocks even though they exit at once; a solver that knows which RHS are still
- * running (krylov_dev_batch after each chunk) saves that. run == NULL, or no entry set, clears the list. */
+ * running (krylov_dev_batch after each chunk) saves that. run == NULL, or no entry set, clears the list. The batched
+ * solver (adda_iterative_solver_batch) replaces any list with its own and clears it on return; autotuning and
+ * adda_hip_profile_matvec_batch time full launches whatever is listed. */
 int adda_hip_set_launch_list(adda_hip_op *op, int K, const int *run);
 int adda_hip_dk_bcgs2_iter(adda_hip_op *op, int prec, void *state, int niter, int matvec_ready, void *x, void *rtilda,
 	void *r0, void *r1, void *r2, void *u0, void *u1, void *u2);

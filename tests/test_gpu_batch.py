@@ -192,7 +192,8 @@ def test_batch_of_one_and_gating(hipdev):
 def test_launch_list(hipdev):
     """Batched matvecs under a launch list compute the listed RHS exactly as the full launch does (every P3 form the
     tuner picks between: one RHS per block and KL=2), leave the others untouched, and a gate still applies to the
-    listed ones; clearing the list restores the full launch."""
+    listed ones; clearing the list restores the full launch, and a list a caller left set does not reach the batched
+    solver."""
     import ctypes
     bp, _ = _batch(hipdev, 3)
     op = bp.op
@@ -226,6 +227,17 @@ def test_launch_list(hipdev):
                         listed = run[k] and not (gate is not None and k == 3)
                         assert np.array_equal(got, full[k] if listed else sentinel), (cfg, gate is not None, k)
             op.set_config(base, hipdev.F64)
+            op.matvec_batch(a, r, K)
+            for k in range(K):
+                assert np.array_equal(op.download(op.rhs(r, k)), full[k]), k
+            # a list left set by a caller does not reach the solver (it sets its own and clears it)
+            ref = bp.solve("bicgstab", eps=1e-10)
+            x_ref = [bp.field_x(k) for k in range(K)]
+            op.set_launch_list(K, run)
+            res = bp.solve("bicgstab", eps=1e-10)
+            for k in range(K):
+                assert res[k]["rc"] == 0 and res[k]["niter"] == ref[k]["niter"], k
+                assert np.array_equal(bp.field_x(k), x_ref[k]), k
             op.matvec_batch(a, r, K)
             for k in range(K):
                 assert np.array_equal(op.download(op.rhs(r, k)), full[k]), k

@@ -1357,31 +1357,26 @@ __global__ void __launch_bounds__(kDkThreads) k_dk_bs_s2(double *st, const doubl
 	st_c(st, DK_SC + 28, alpha);
 }
 
-// ||s||^2 -> half-step convergence test (src/iterative.c:700-707): if converged, omega := 0 so the tail computes
-// x += alpha p and r := s (whose norm is the one reported).
-__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s3(double *st, const double *partials, int nblk, double *hist, int ps)
+// s3 + s4 + s5 in one launch: ||s||^2 (row 0 of the s update's partial sums, ps at stride pss) -> the half-step
+// convergence test (src/iterative.c:700-707): if converged, omega := 0 so the tail computes x += alpha p and r := s
+// (whose norm is the one reported); then from the fused rows of the t = A.s matvec: row 0 ||t||^2, rows 1, 2
+// sum t.conj(s) = conj(s.t~). The t matvec runs in either case (the half step only decides omega).
+__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s345(double *st, const double *partials, int nblk, int ps,
+	const double *ps_s, int nblk_s, int pss)
 {
 	__shared__ double red_lds[512];
-	__shared__ double v[1];
+	__shared__ double v[3], vs[1];
 	DK_RHS_SCALAR(st, partials, ps);
 	DK_ACTIVE(st);
-	dk_reduce(partials, nblk, 1, v, red_lds);
-	if (threadIdx.x != 0) return;
-	st[DK_SC + 36] = v[0] < st[DK_EPSB] ? 1 : 0;  // half-step flag
-}
-
-// s4 + s5 from the fused rows of the t = A.s matvec: row 0 ||t||^2, rows 1, 2 sum t.conj(s) = conj(s.t~).
-__global__ void __launch_bounds__(kDkThreads) k_dk_bs_s45(double *st, const double *partials, int nblk, int ps)
-{
-	__shared__ double red_lds[512];
-	__shared__ double v[3];
-	DK_RHS_SCALAR(st, partials, ps);
-	DK_ACTIVE(st);
+	ps_s += (size_t)blockIdx.x * pss;
+	dk_reduce(ps_s, nblk_s, 1, vs, red_lds);
 	dk_reduce(partials, nblk, 3, v, red_lds);
 	if (threadIdx.x != 0) return;
+	const bool half = vs[0] < st[DK_EPSB];
+	st[DK_SC + 36] = half ? 1 : 0;  // half-step flag
 	st[DK_SC + 34] = v[0];
 	dcx omega = dscl(dmk(v[1], -v[2]), 1 / v[0]);
-	if (st[DK_SC + 36] != 0) omega = dmk(0, 0);
+	if (half) omega = dmk(0, 0);
 	else { st_c(st, DK_SC + 6, omega); st_c(st, DK_SC + 2, ld_c(st, DK_SC + 0)); }  // ro_old = ro_new
 	st_c(st, DK_SC + 30, omega);
 	st_c(st, DK_SC + 32, dneg(omega));
@@ -1802,11 +1797,12 @@ static int bicgstab_iter_t(adda_hip_op *op, dk_state *k, int niter, int matvec_r
 		hipLaunchKernelGGL(k_dk_bs_s2, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials + nb5, nb5, 3 * nb5);
 	}
 	lc_dev<T>(op, g, s, v, r, nullptr, st, DK_SC + 26, DK_SC + 24, DK_SC + 24, 1);
-	hipLaunchKernelGGL(k_dk_bs_s3, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials2, g.nb, k->hist, g.ps2);
-	// t = A.s with ||t||^2 and t.s~ fused
+	// t = A.s with ||t||^2 and t.s~ fused; the half-step test on ||s||^2 (the s update's partial sums, which the matvec
+	// leaves alone) runs in the same scalar launch as omega
 	if (matvec_t<T>(op, g.K, g.vs, st, (const cx<T> *)s, (cx<T> *)t, nullptr, 0, nullptr, true, &nb5, nullptr, 2))
 		return -1;
-	hipLaunchKernelGGL(k_dk_bs_s45, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials, nb5, 3 * nb5);
+	hipLaunchKernelGGL(k_dk_bs_s345, dim3(g.K), dim3(kDkThreads), 0, op->stream, st, op->partials, nb5, 3 * nb5,
+		op->partials2, g.nb, g.ps2);
 	hipLaunchKernelGGL(k_dk_xr<T>, dim3(g.nb, g.K), dim3(256), 0, op->stream, (cx<T> *)x, (const cx<T> *)p,
 		(const cx<T> *)s, (cx<T> *)r, (const cx<T> *)s, (const cx<T> *)t, st, DK_SC + 28, DK_SC + 30, DK_SC + 24,
 		DK_SC + 32, g.n, g.vs, op->partials2, g.ps2, (const cx<T> *)rt);

@@ -133,7 +133,7 @@ def test_batched_history_and_oracle_field(hipdev):
 
 
 @pytest.mark.parametrize("v64", [False, True])
-@pytest.mark.parametrize("method", ["bicgstab", "qmr"])
+@pytest.mark.parametrize("method", ["bicgstab", "qmr", "cgnr", "bcgs2", "csym"])
 def test_batched_mixed_precision(hipdev, method, v64):
     """Inner solves of all RHS in shared launches (FP32, or FP64 around FP32 transforms) + per-RHS FP64 defect
     correction."""

@@ -109,6 +109,30 @@ def test_c1_device_form_vs_host_form(hipdev, method, mv_ready, nmv_per_it):
 
 
 @pytest.mark.parametrize("host_scalars", [False, True])
+@pytest.mark.parametrize("method,name", [("bicg", "BiCG_CS"), ("qmr", "QMR_CS"), ("qmr2", "QMR_CS_2")])
+def test_complex_symmetric_breakdown(hipdev, method, name, host_scalars):
+    """The complex-symmetric Lanczos methods break down when r^T r = 0 with r != 0 (src/iterative.c:555-560,
+    962-965, 1089-1092). An incident field (1, i, 0) at every dipole with one material gives b = cc_sqrt*(1, i, 0) per
+    dipole, whose self-bilinear sums re^2 and im^2 add the same two squares and whose re*im terms cancel, so
+    r^T r = 0 exactly at iteration 1 from x = 0. Both forms must stop with an error that names the method, not
+    iterate on NaNs."""
+    pr = _problem(hipdev, "sphere16", 1.5, 15)
+    try:
+        pr.set_polarization(PROP_Z, POL_Y)
+        e = np.zeros((pr.nd, 3), dtype=np.complex128)
+        e[:, 0] = 1
+        e[:, 1] = 1j
+        pr.op.upload(pr.einc, e.ravel())
+        with pytest.raises(RuntimeError, match=name):
+            pr.solve(method, eps=1e-10, init_field=hipdev.IF_ZERO, host_scalars=host_scalars)
+        # the operator stays usable: a plane wave solves normally afterwards
+        pr.set_polarization(PROP_Z, POL_Y)
+        assert pr.solve(method, eps=1e-10, host_scalars=host_scalars)["rc"] == 0
+    finally:
+        pr.close()
+
+
+@pytest.mark.parametrize("host_scalars", [False, True])
 def test_c1_history_and_fields_vs_oracle(hipdev, host_scalars):
     """QMR at eps 1e-5: the residual history follows the sample's RE_ lines; at 1e-10 the field x matches the oracle's
     FP64 solve to 1e-8 and the Mueller matrix matches the reference's 360-degree yz-plane output."""

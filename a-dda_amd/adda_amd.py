@@ -716,7 +716,10 @@ class BatchProblem:
             self.op = None
 
     def solve(self, method="qmr", eps=1e-5, maxiter=None, init_field=IF_AUTO, precision=PREC_FP64, inner_eps=0.0,
-              max_outer=0, recalc_resid=False, hist_len=0, graph=True, max_inner=0, inner_method=None):
+              max_outer=0, recalc_resid=False, hist_len=0, graph=True, max_inner=0, inner_method=None,
+              raise_on_error=True):
+        """adda_iterative_solver_batch; per-RHS results. raise_on_error=False returns them even when some RHS failed
+        (their rc < 0), instead of raising the error of the first failing one."""
         if maxiter is None:
             maxiter = 3 * self.box[0] * self.box[1] * self.box[2]
         o = SolverOpts(ITER_NAMES.get(method, method), eps, maxiter, init_field, int(recalc_resid), precision,
@@ -725,7 +728,8 @@ class BatchProblem:
         hist = np.zeros((self.K, max(hist_len, 1)))
         rc = host().adda_iterative_solver_batch(self.op.h, ctypes.byref(o), self.K, self.einc, self.x, self.pvec,
                                                 _p(hist) if hist_len else None, hist_len, st)
-        hcheck(rc)
+        if raise_on_error:
+            hcheck(rc)
         out = [dict(rc=s.rc, niter=s.niter, nmatvec=s.nmatvec, nmatvec_fp64=s.nmatvec_fp64, nouter=s.nouter,
                     resid=s.resid, true_resid=s.true_resid) for s in st]
         if hist_len:

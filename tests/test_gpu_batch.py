@@ -249,6 +249,28 @@ def test_launch_list(hipdev):
         bp.close()
 
 
+@pytest.mark.parametrize("method", ["bicg", "qmr"])
+def test_batched_breakdown_is_per_rhs(hipdev, method):
+    """One RHS of a batch breaks down (incident field (1, i, 0) at every dipole: r^T r = 0 at iteration 1 of the
+    complex-symmetric methods); it alone reports the error, and the other RHS converge to the same bits as in a batch
+    without the breakdown."""
+    bp, _ = _batch(hipdev, 1)
+    try:
+        assert bp.K == 2
+        ref = bp.solve(method, eps=1e-10, init_field=hipdev.IF_ZERO)
+        x1 = bp.field_x(1)
+        e = np.zeros((bp.nd, 3), dtype=np.complex128)
+        e[:, 0] = 1
+        e[:, 1] = 1j
+        bp.op.upload(bp.op.rhs(bp.einc, 0), e.ravel())
+        res = bp.solve(method, eps=1e-10, init_field=hipdev.IF_ZERO, raise_on_error=False)
+        assert res[0]["rc"] < 0
+        assert res[1]["rc"] == 0 and res[1]["niter"] == ref[1]["niter"]
+        assert np.array_equal(bp.field_x(1), x1)
+    finally:
+        bp.close()
+
+
 def test_batched_postprocessing(hipdev):
     """Cext, Cabs and the scattering-plane fields of every RHS from one batched evaluation equal the single-RHS
     functions bit for bit; for the orientation the reference ran (30, 40, 50; m = 1.33+0.05i, dpl 12) the batched

@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--batched-solve", choices=["fp64", "mixed"],
                     help="only one batched solve of the 64-RHS workload (after an untimed one), in this form: the run "
                          "a kernel trace of a batched solve is taken from")
+    ap.add_argument("--methods", action="store_true",
+                    help="per-iteration cost of every Krylov method on this workload in FP64: 200 iterations each, "
+                         "device-resident recurrences against the host form (one host sync per inner product)")
     ap.add_argument("--method", default="bicgstab", help="solver of --solve-only / --batched-solve (ADDA's -iter names)")
     ap.add_argument("--eps", type=float, default=1e-10, help="tolerance of --solve-only (default 1e-10)")
     ap.add_argument("--inner-method", default=None,
@@ -104,6 +107,20 @@ def main():
     tune = {name: op.autotune(prec, reps=3, cache=args.tune_cache) for prec, name in ((A.F64, "fp64"), (A.F32, "fp32"))}
     t_tune = time.perf_counter() - t0
     log(f"autotune {t_tune:.2f} s: " + "; ".join(f"{k}: {v}" for k, v in tune.items()))
+
+    if args.methods:
+        for meth in ("bicgstab", "qmr", "cgnr", "bicg", "csym", "qmr2", "bcgs2"):
+            row = {"mode": "methods", "method": meth, "dipoles": nd, "grid": list(pr.grid), "iterations": 200}
+            for hs, key in ((False, "device_us_per_iter"), (True, "host_us_per_iter")):
+                op.sync()
+                t = time.perf_counter()
+                res = pr.solve(meth, eps=1e-14, maxiter=200, host_scalars=hs, hist_len=4)
+                op.sync()
+                row[key] = round((time.perf_counter() - t) / max(res["niter"], 1) * 1e6, 1)
+                row["matvecs"] = res["nmatvec"]
+            print(json.dumps(row), flush=True)
+        pr.close()
+        return
 
     if args.solve_only:
         prec = A.PREC_FP64 if args.solve_only == "fp64" else A.PREC_MIXED
